@@ -1,0 +1,778 @@
+// The C ABI of include/hdns.h: context lifetime, the boundary conversions between the
+// reference's full half-spectrum layout and the compact retained-mode layout, and the stage
+// schedule of step! + deposit! on one HIP stream.
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "context.hpp"
+
+using namespace hdns;
+
+struct hdns_ctx {
+  Ctx c;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+hdns_status fail(hdns_ctx *ctx, hdns_status code, const std::string &msg) {
+  if (ctx) ctx->c.err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                      \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      return fail(ctx, HDNS_ERR_DEVICE, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+hdns_status launched(hdns_ctx *ctx) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ctx, HDNS_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+  return HDNS_OK;
+}
+
+bool power_of_two(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// The element-type value of a rational p/q: rounded once for FP64; for FP32 the nearest float
+// to the extended-precision quotient.
+double round_t(long double p, long double q, bool fp32) {
+  const long double v = p / q;
+  return fp32 ? (double)(float)v : (double)v;
+}
+
+struct Coeffs {
+  int stages;
+  double a[5], b[5], d[5];
+};
+
+// src/Propagator.jl:264-274 (Williamson RK3) and :304-320 (Carpenter-Kennedy RK4): exact
+// rationals rounded to the element type; d are differences of the rounded abscissae formed in T.
+Coeffs scheme_coefficients(int scheme, bool fp32) {
+  Coeffs k{};
+  if (scheme == HDNS_IF_LOW_STORAGE_RK3) {
+    k.stages = 3;
+    const long double an[3][2] = {{0, 1}, {-5, 9}, {-153, 128}};
+    const long double bn[3][2] = {{1, 3}, {15, 16}, {8, 15}};
+    const long double dn[3][2] = {{1, 3}, {5, 12}, {1, 4}};
+    for (int i = 0; i < 3; ++i) {
+      k.a[i] = round_t(an[i][0], an[i][1], fp32);
+      k.b[i] = round_t(bn[i][0], bn[i][1], fp32);
+      k.d[i] = round_t(dn[i][0], dn[i][1], fp32);
+    }
+    return k;
+  }
+  k.stages = 5;
+  const long double an[5][2] = {{0, 1},
+                                {-567301805773.0L, 1357537059087.0L},
+                                {-2404267990393.0L, 2016746695238.0L},
+                                {-3550918686646.0L, 2091501179385.0L},
+                                {-1275806237668.0L, 842570457699.0L}};
+  const long double bn[5][2] = {{1432997174477.0L, 9575080441755.0L},
+                                {5161836677717.0L, 13612068292357.0L},
+                                {1720146321549.0L, 2090206949498.0L},
+                                {3134564353537.0L, 4481467310338.0L},
+                                {2277821191437.0L, 14882151754819.0L}};
+  for (int i = 0; i < 5; ++i) {
+    k.a[i] = round_t(an[i][0], an[i][1], fp32);
+    k.b[i] = round_t(bn[i][0], bn[i][1], fp32);
+  }
+  const double c2 = round_t(1432997174477.0L, 9575080441755.0L, fp32);
+  const double c3 = round_t(2526269341429.0L, 6820363962896.0L, fp32);
+  const double c4 = round_t(2006345519317.0L, 3224310063776.0L, fp32);
+  const double c5 = round_t(2802321613138.0L, 2924317926251.0L, fp32);
+  if (fp32) {
+    k.d[0] = (float)c2;
+    k.d[1] = (float)((float)c3 - (float)c2);
+    k.d[2] = (float)((float)c4 - (float)c3);
+    k.d[3] = (float)((float)c5 - (float)c4);
+    k.d[4] = (float)(1.0f - (float)c5);
+  } else {
+    k.d[0] = c2;
+    k.d[1] = c3 - c2;
+    k.d[2] = c4 - c3;
+    k.d[3] = c5 - c4;
+    k.d[4] = 1.0 - c5;
+  }
+  return k;
+}
+
+// src/ParticlePropagator.jl:261-280: the integral over [0, h] of each Lagrange basis polynomial
+// on the sample times, by 4-point Gauss-Legendre.
+void adams_weights(const double *nodes, int n, double h, double *out) {
+  static const double xi[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563, 0.8611363115940526};
+  static const double wq[4] = {0.3478548451374538, 0.6521451548625461, 0.6521451548625461, 0.3478548451374538};
+  for (int j = 0; j < n; ++j) {
+    double weight = 0.0;
+    for (int g = 0; g < 4; ++g) {
+      const double t = h * (xi[g] + 1) / 2;
+      double basis = 1.0;
+      for (int m = 0; m < n; ++m) {
+        if (m == j) continue;
+        basis *= (t - nodes[m]) / (nodes[j] - nodes[m]);
+      }
+      weight += wq[g] * h / 2 * basis;
+    }
+    out[j] = weight;
+  }
+}
+
+int signed_index(int idx, int m, int k) { return idx <= m ? idx : idx - k; }
+int bin_index(int idx, const Geo &g) { return idx <= g.m ? idx : g.n - g.k + idx; }
+
+hdns_status alloc(hdns_ctx *ctx, void **p, size_t bytes) {
+  if (bytes == 0) {
+    *p = nullptr;
+    return HDNS_OK;
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(ctx, HDNS_ERR_DEVICE, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+  }
+  ctx->c.bytes += bytes;
+  return HDNS_OK;
+}
+
+void release(Ctx &c) {
+  void *ptrs[] = {c.uh, c.hu, c.ph, c.hp, c.bufa, c.bufb, c.phys, c.physw, c.tw, c.partials, c.scalars,
+                  c.maxbits, c.spec_bins, c.pos, c.vel, c.phis, c.order, c.tile_counts, c.tile_cursor};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c.stream) (void)hipStreamDestroy(c.stream);
+  c.stream = nullptr;
+}
+
+// Zero-fill a device array with a kernel (no memset nodes in a captured step).
+void zero_f64(Ctx &c, void *p, size_t bytes) {
+  const size_t n = bytes / sizeof(double);
+  hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, c.stream, (double *)p, 0.0, n);
+}
+
+// The inverse half of the pipeline for the current state: the field deposit!
+// (src/Propagator.jl:808-842): u, w, phi in physical space, their products for the next
+// stage 1, max|u|^2, and the reductions of the state.
+hdns_status run_deposit(hdns_ctx *ctx) {
+  Ctx &c = ctx->c;
+  c.pipe->zi(c);
+  const FinalizeArgs fa{(int)((c.geo.kxp / c.pipe->tile_width()) * c.geo.k), 1.0 / pow((double)c.geo.n, 6.0),
+                        c.cfg.velocity_forcing == HDNS_CONSTANT_POWER, c.cfg.injection_rate,
+                        c.fp64 ? 2.220446049250313e-16 : 1.1920928955078125e-07};
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c.stream, (const double *)c.partials, c.scalars, fa);
+  c.pipe->yinv(c);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c.stream, c.maxbits, 0ull, 1);
+  XArgs xa{};
+  xa.store_u = (c.cfg.store_physical & 1) != 0;
+  xa.store_phi = (c.cfg.store_physical & 2) != 0 && c.cfg.has_scalar;
+  xa.store_w = (c.cfg.store_physical & 4) != 0;
+  xa.want_max = 1;
+  xa.phys = c.phys;
+  xa.physw = c.physw;
+  xa.max_bits = c.maxbits;
+  c.pipe->xpass(c, xa);
+  c.deposited = true;
+  return launched(ctx);
+}
+
+}  // namespace
+
+extern "C" {
+
+hdns_status hdns_ctx_create(const hdns_config *config, hdns_ctx **out) {
+  if (!config || !out) return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "config and out must not be NULL");
+  *out = nullptr;
+  const hdns_config &k = *config;
+  if (!power_of_two(k.grid_points) || k.grid_points < 16 || k.grid_points > 1024)
+    return fail(nullptr, HDNS_ERR_UNSUPPORTED,
+                "grid_points must be a power of two in 16..1024, got " + std::to_string(k.grid_points));
+  if (!(k.side_length > 0) || !isfinite(k.side_length))
+    return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "side_length must be positive and finite");
+  if (k.precision != HDNS_FP32_BULK && k.precision != HDNS_FP64)
+    return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "precision must be HDNS_FP32_BULK or HDNS_FP64");
+  if (k.scheme != HDNS_IF_LOW_STORAGE_RK3 && k.scheme != HDNS_IF_LOW_STORAGE_RK4)
+    return fail(nullptr, HDNS_ERR_UNSUPPORTED, "scheme must be IFLowStorageRK3 or IFLowStorageRK4");
+  if (k.dealiasing != HDNS_TWO_THIRDS) return fail(nullptr, HDNS_ERR_UNSUPPORTED, "only TwoThirds dealiasing is built");
+  // src/Generator.jl:107-112
+  if (!(k.viscosity > 0))
+    return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                "`viscosity` must be positive: it sets the Reynolds number through Re = 1/nu, and a non-positive "
+                "value has no dissipation scale");
+  if (!(k.schmidt > 0))
+    return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                "`schmidt` must be positive: the scalar diffusivity is nu/Sc, so a non-positive Schmidt number makes "
+                "it infinite or negative");
+  if (k.velocity_forcing != HDNS_NO_FORCING && k.velocity_forcing != HDNS_CONSTANT_POWER)
+    return fail(nullptr, HDNS_ERR_UNSUPPORTED, "velocity_forcing must be NoForcing or ConstantPower");
+  if (k.velocity_forcing == HDNS_CONSTANT_POWER) {  // src/Forcing.jl:388-409
+    if (k.lowest_shell < 1)
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                  "`lowest_shell` must be at least 1: shell 0 is the mean (kappa = 0) mode, which a band forcing "
+                  "cannot legitimately drive");
+    if (k.highest_shell != -1 && k.highest_shell < k.lowest_shell)
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                  "`highest_shell` must be at least `lowest_shell`: the band would be empty and the forcing would "
+                  "silently do nothing");
+    if (!isfinite(k.injection_rate))
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "injection_rate must be finite");
+  }
+  if (!isfinite(k.mean_gradient)) return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "mean_gradient must be finite");
+  if (k.tracer_capacity < 0) return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "tracer_capacity must be >= 0");
+  if (k.tracer_capacity > 0 && (k.interpolation != HDNS_CATMULL_ROM || k.particle_scheme != HDNS_HEUN))
+    return fail(nullptr, HDNS_ERR_UNSUPPORTED, "tracers support CatmullRom interpolation and the Heun scheme");
+  if (k.tracer_capacity > (int64_t)INT32_MAX)
+    return fail(nullptr, HDNS_ERR_UNSUPPORTED, "tracer_capacity above 2^31-1 is not supported");
+
+  hdns_ctx *ctx = new (std::nothrow) hdns_ctx();
+  if (!ctx) return fail(nullptr, HDNS_ERR_DEVICE, "out of host memory");
+  Ctx &c = ctx->c;
+  c.cfg = k;
+  if (c.cfg.tracer_capacity > 0) c.cfg.store_physical |= 1 | (c.cfg.has_scalar ? 2 : 0);
+  c.fp64 = k.precision == HDNS_FP64;
+  const int n = k.grid_points;
+
+  hipError_t e = hipSetDevice(k.device);
+  if (e != hipSuccess) {
+    delete ctx;
+    return fail(nullptr, HDNS_ERR_DEVICE, std::string("hipSetDevice failed: ") + hipGetErrorString(e));
+  }
+  c.pipe = c.fp64 ? make_pipeline_f64(n) : make_pipeline_f32(n);
+  if (!c.pipe) {
+    delete ctx;
+    return fail(nullptr, HDNS_ERR_UNSUPPORTED, "no pipeline for this grid size");
+  }
+  const int tx = c.pipe->tile_width();
+  c.geo.n = n;
+  c.geo.m = n / 3;
+  c.geo.k = 2 * c.geo.m + 1;
+  c.geo.kxp = ((c.geo.m + 1 + tx - 1) / tx) * tx;
+  c.geo.k0 = 2.0 * M_PI / k.side_length;
+  c.elem = c.fp64 ? 16 : 8;
+  const size_t realb = c.fp64 ? 8 : 4;
+  c.comp = (size_t)c.geo.k * c.geo.k * c.geo.kxp;
+  c.stages = k.scheme == HDNS_IF_LOW_STORAGE_RK4 ? 5 : 3;
+  c.band.on = k.velocity_forcing == HDNS_CONSTANT_POWER;
+  c.band.lo4 = (2 * k.lowest_shell - 1) * (2 * k.lowest_shell - 1);
+  c.band.hi4 = k.highest_shell < 0 ? -1 : (2 * k.highest_shell + 1) * (2 * k.highest_shell + 1);
+  const size_t ncf = k.has_scalar ? 6 : 3, nci = k.has_scalar ? 7 : 4;
+  const size_t nn = (size_t)n * n, nk = (size_t)n * c.geo.k;
+  c.buf_a = std::max(ncf * nn, nci * nk) * c.geo.kxp;
+  c.buf_b = std::max(ncf * nk, nci * nn) * c.geo.kxp;
+  const size_t vol = nn * n;
+
+  hdns_status s = HDNS_OK;
+  e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+  if (e != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
+  const int nzb = (c.geo.kxp / tx) * c.geo.k;
+  c.nbins = (int)floor(sqrt(3.0) * n / 2) + 2;
+  c.ntiles = (n / 8) * (n / 8) * (n / 8);
+  c.m = 0;
+  if (s == HDNS_OK) s = alloc(ctx, &c.uh, 3 * c.comp * c.elem);
+  if (s == HDNS_OK) s = alloc(ctx, &c.hu, 3 * c.comp * c.elem);
+  if (s == HDNS_OK && k.has_scalar) s = alloc(ctx, &c.ph, c.comp * c.elem);
+  if (s == HDNS_OK && k.has_scalar) s = alloc(ctx, &c.hp, c.comp * c.elem);
+  if (s == HDNS_OK) s = alloc(ctx, &c.bufa, c.buf_a * c.elem);
+  if (s == HDNS_OK) s = alloc(ctx, &c.bufb, c.buf_b * c.elem);
+  if (s == HDNS_OK && (c.cfg.store_physical & 3)) s = alloc(ctx, &c.phys, (k.has_scalar ? 4 : 3) * vol * realb);
+  if (s == HDNS_OK && (c.cfg.store_physical & 4)) s = alloc(ctx, &c.physw, 3 * vol * realb);
+  if (s == HDNS_OK) s = alloc(ctx, (void **)&c.tw, n * sizeof(cx<double>));
+  if (s == HDNS_OK) s = alloc(ctx, (void **)&c.partials, (size_t)nzb * NRED * sizeof(double));
+  if (s == HDNS_OK) s = alloc(ctx, (void **)&c.scalars, 8 * sizeof(double));
+  if (s == HDNS_OK) s = alloc(ctx, (void **)&c.maxbits, sizeof(unsigned long long));
+  if (s == HDNS_OK) s = alloc(ctx, (void **)&c.spec_bins, c.nbins * sizeof(double));
+  if (s == HDNS_OK && k.tracer_capacity > 0) {
+    const size_t cap = (size_t)k.tracer_capacity;
+    s = alloc(ctx, &c.pos, 3 * cap * realb);
+    if (s == HDNS_OK) s = alloc(ctx, &c.vel, 3 * cap * realb);
+    if (s == HDNS_OK && k.has_scalar) s = alloc(ctx, &c.phis, cap * realb);
+    if (s == HDNS_OK) s = alloc(ctx, (void **)&c.order, cap * sizeof(int));
+    if (s == HDNS_OK) s = alloc(ctx, (void **)&c.tile_counts, (size_t)c.ntiles * sizeof(int));
+    if (s == HDNS_OK) s = alloc(ctx, (void **)&c.tile_cursor, (size_t)c.ntiles * sizeof(int));
+  }
+  if (s == HDNS_OK) {
+    // W_N^m = exp(-2 pi i m / N), each entry the FP64 rounding of an extended-precision value
+    std::vector<cx<double>> tw(n);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int m = 0; m < n; ++m) {
+      const long double ang = 2.0L * pi * (long double)m / (long double)n;
+      tw[m] = {(double)cosl(ang), (double)(-sinl(ang))};
+    }
+    e = hipMemcpy(c.tw, tw.data(), n * sizeof(cx<double>), hipMemcpyHostToDevice);
+    if (e != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, std::string("twiddle upload failed: ") + hipGetErrorString(e));
+  }
+  if (s == HDNS_OK) {
+    // every array the step reads starts at zero (padding columns included)
+    zero_f64(c, c.uh, 3 * c.comp * c.elem);
+    zero_f64(c, c.hu, 3 * c.comp * c.elem);
+    if (c.ph) zero_f64(c, c.ph, c.comp * c.elem);
+    if (c.hp) zero_f64(c, c.hp, c.comp * c.elem);
+    zero_f64(c, c.bufa, c.buf_a * c.elem);
+    zero_f64(c, c.bufb, c.buf_b * c.elem);
+    zero_f64(c, c.scalars, 8 * sizeof(double));
+    if (c.phys) zero_f64(c, c.phys, (k.has_scalar ? 4 : 3) * vol * realb);
+    if (c.physw) zero_f64(c, c.physw, 3 * vol * realb);
+    s = launched(ctx);
+    if (s == HDNS_OK) {
+      e = hipStreamSynchronize(c.stream);
+      if (e != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, std::string("init failed: ") + hipGetErrorString(e));
+    }
+  }
+  if (s != HDNS_OK) {
+    g_create_error = c.err;
+    release(c);
+    delete ctx;
+    return s;
+  }
+  *out = ctx;
+  return HDNS_OK;
+}
+
+hdns_status hdns_ctx_destroy(hdns_ctx *ctx) {
+  if (!ctx) return HDNS_OK;
+  (void)hipSetDevice(ctx->c.cfg.device);
+  if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
+  release(ctx->c);
+  delete ctx;
+  return HDNS_OK;
+}
+
+const char *hdns_last_error(const hdns_ctx *ctx) { return ctx ? ctx->c.err.c_str() : g_create_error.c_str(); }
+
+hdns_status hdns_memory_bytes(const hdns_ctx *ctx, int64_t *bytes) {
+  if (!ctx || !bytes) return HDNS_ERR_INVALID_ARGUMENT;
+  *bytes = (int64_t)ctx->c.bytes;
+  return HDNS_OK;
+}
+
+hdns_status hdns_scheme_coefficients(int32_t scheme, int32_t precision, double *a, double *b, double *d,
+                                     int32_t *stages) {
+  if (scheme != HDNS_IF_LOW_STORAGE_RK3 && scheme != HDNS_IF_LOW_STORAGE_RK4) return HDNS_ERR_INVALID_ARGUMENT;
+  if (!a || !b || !d || !stages) return HDNS_ERR_INVALID_ARGUMENT;
+  const Coeffs k = scheme_coefficients(scheme, precision == HDNS_FP32_BULK);
+  *stages = k.stages;
+  for (int i = 0; i < k.stages; ++i) {
+    a[i] = k.a[i];
+    b[i] = k.b[i];
+    d[i] = k.d[i];
+  }
+  return HDNS_OK;
+}
+
+hdns_status hdns_heun_weights(double h, double *out3) {
+  if (!out3 || !(h > 0) || !isfinite(h)) return HDNS_ERR_INVALID_ARGUMENT;
+  const double n1[1] = {0.0};
+  double b[1];
+  adams_weights(n1, 1, h, b);
+  const double n2[2] = {h, 0.0};
+  double beta[2];
+  adams_weights(n2, 2, h, beta);
+  out3[0] = b[0];
+  out3[1] = beta[0];
+  out3[2] = beta[1] - b[0];
+  return HDNS_OK;
+}
+
+hdns_status hdns_state_upload(hdns_ctx *ctx, const double *u_hat, const double *phi_hat) {
+  if (!ctx || !u_hat) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "u_hat must not be NULL");
+  Ctx &c = ctx->c;
+  if (c.cfg.has_scalar && !phi_hat)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "this context evolves a scalar: phi_hat must not be NULL");
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  const Geo &g = c.geo;
+  const int n = g.n, hx = n / 2 + 1;
+  const size_t full = (size_t)n * n * hx;
+  const int ncomp = c.cfg.has_scalar ? 4 : 3;
+  // content outside the 2/3 ball cannot be carried by the retained-mode store
+  for (int q = 0; q < ncomp; ++q) {
+    const double *src = q < 3 ? u_hat + 2 * full * q : phi_hat;
+    for (int iz = 0; iz < n; ++iz) {
+      const int sz = iz <= n / 2 ? iz : iz - n;
+      for (int iy = 0; iy < n; ++iy) {
+        const int sy = iy <= n / 2 ? iy : iy - n;
+        const double *row = src + 2 * (((size_t)iz * n + iy) * hx);
+        for (int ix = 0; ix < hx; ++ix) {
+          if (in_two_thirds(ix * ix + sy * sy + sz * sz, n)) continue;
+          if (row[2 * ix] != 0.0 || row[2 * ix + 1] != 0.0) {
+            char msg[400];
+            snprintf(msg, sizeof msg,
+                     "the %s spectrum has content outside the 2/3 dealiasing ball (mode %d,%d,%d); the step keeps "
+                     "only the retained modes, so truncate the state to the band first (truncate_to_band!)",
+                     q < 3 ? "velocity" : "scalar", ix, sy, sz);
+            return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, msg);
+          }
+        }
+      }
+    }
+  }
+  std::vector<unsigned char> host(c.comp * c.elem);
+  for (int q = 0; q < ncomp; ++q) {
+    const double *src = q < 3 ? u_hat + 2 * full * q : phi_hat;
+    std::fill(host.begin(), host.end(), 0);
+    for (int kz = 0; kz < g.k; ++kz) {
+      const int iz = bin_index(kz, g);
+      for (int ky = 0; ky < g.k; ++ky) {
+        const int iy = bin_index(ky, g);
+        const double *row = src + 2 * (((size_t)iz * n + iy) * hx);
+        const size_t at = ((size_t)kz * g.k + ky) * g.kxp;
+        for (int ix = 0; ix <= g.m; ++ix) {
+          if (c.fp64) {
+            double *dst = (double *)host.data() + 2 * (at + ix);
+            dst[0] = row[2 * ix];
+            dst[1] = row[2 * ix + 1];
+          } else {
+            float *dst = (float *)host.data() + 2 * (at + ix);
+            dst[0] = (float)row[2 * ix];
+            dst[1] = (float)row[2 * ix + 1];
+          }
+        }
+      }
+    }
+    void *dev = q < 3 ? (char *)c.uh + q * c.comp * c.elem : c.ph;
+    HIP_TRY(ctx, hipMemcpyAsync(dev, host.data(), host.size(), hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  }
+  hdns_status s = run_deposit(ctx);
+  if (s != HDNS_OK) return s;
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  return HDNS_OK;
+}
+
+hdns_status hdns_state_download(hdns_ctx *ctx, double *u_hat, double *phi_hat) {
+  if (!ctx || !u_hat) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "u_hat must not be NULL");
+  Ctx &c = ctx->c;
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  const Geo &g = c.geo;
+  const int n = g.n, hx = n / 2 + 1;
+  const size_t full = (size_t)n * n * hx;
+  const int ncomp = c.cfg.has_scalar && phi_hat ? 4 : 3;
+  std::vector<unsigned char> host(c.comp * c.elem);
+  for (int q = 0; q < ncomp; ++q) {
+    double *dst = q < 3 ? u_hat + 2 * full * q : phi_hat;
+    const void *dev = q < 3 ? (char *)c.uh + q * c.comp * c.elem : c.ph;
+    HIP_TRY(ctx, hipMemcpyAsync(host.data(), dev, host.size(), hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+    memset(dst, 0, 2 * full * sizeof(double));
+    for (int kz = 0; kz < g.k; ++kz) {
+      const int iz = bin_index(kz, g);
+      for (int ky = 0; ky < g.k; ++ky) {
+        const int iy = bin_index(ky, g);
+        double *row = dst + 2 * (((size_t)iz * n + iy) * hx);
+        const size_t at = ((size_t)kz * g.k + ky) * g.kxp;
+        for (int ix = 0; ix <= g.m; ++ix) {
+          if (c.fp64) {
+            const double *v = (const double *)host.data() + 2 * (at + ix);
+            row[2 * ix] = v[0];
+            row[2 * ix + 1] = v[1];
+          } else {
+            const float *v = (const float *)host.data() + 2 * (at + ix);
+            row[2 * ix] = v[0];
+            row[2 * ix + 1] = v[1];
+          }
+        }
+      }
+    }
+  }
+  return HDNS_OK;
+}
+
+hdns_status hdns_step(hdns_ctx *ctx, double dt) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  Ctx &c = ctx->c;
+  if (!(isfinite(dt) && dt > 0))
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "the timestep must be positive and finite");
+  if (!c.deposited)
+    return fail(ctx, HDNS_ERR_STATE, "no deposit of the current state: upload a state (or deposit) before stepping");
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  const bool fp32 = !c.fp64;
+  const Coeffs k = scheme_coefficients(c.cfg.scheme, fp32);
+  const double dt_t = fp32 ? (double)(float)dt : dt;
+  const double kappa = c.cfg.viscosity / c.cfg.schmidt;
+  const FinalizeArgs fa{(int)((c.geo.kxp / c.pipe->tile_width()) * c.geo.k), 1.0 / pow((double)c.geo.n, 6.0),
+                        c.cfg.velocity_forcing == HDNS_CONSTANT_POWER, c.cfg.injection_rate,
+                        c.fp64 ? 2.220446049250313e-16 : 1.1920928955078125e-07};
+  XArgs mid{};
+  XArgs fin{};
+  fin.store_u = (c.cfg.store_physical & 1) != 0;
+  fin.store_phi = (c.cfg.store_physical & 2) != 0 && c.cfg.has_scalar;
+  fin.store_w = (c.cfg.store_physical & 4) != 0;
+  fin.want_max = 1;
+  fin.phys = c.phys;
+  fin.physw = c.physw;
+  fin.max_bits = c.maxbits;
+  for (int i = 0; i < k.stages; ++i) {
+    const bool last = i == k.stages - 1;
+    StageArgs sa{};
+    sa.stage = i;
+    sa.last = last;
+    sa.a = k.a[i];
+    sa.b = k.b[i];
+    sa.dt = dt_t;
+    const double sub = k.d[i] * dt;  // d_i in T, promoted: the exponent's substep (src/Propagator.jl:375)
+    sa.coef_u = -c.cfg.viscosity * sub;
+    sa.coef_p = -kappa * sub;
+    sa.mean_gradient = c.cfg.mean_gradient;
+    sa.band = c.band;
+    sa.scalars = c.scalars;
+    c.pipe->yfwd(c);
+    c.pipe->zf(c, sa);
+    if (last) c.pipe->reality(c);
+    c.pipe->zi(c);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c.stream, (const double *)c.partials, c.scalars, fa);
+    c.pipe->yinv(c);
+    if (last) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c.stream, c.maxbits, 0ull, 1);
+    c.pipe->xpass(c, last ? fin : mid);
+  }
+  c.deposited = true;
+  return launched(ctx);
+}
+
+hdns_status hdns_deposit(hdns_ctx *ctx) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  HIP_TRY(ctx, hipSetDevice(ctx->c.cfg.device));
+  return run_deposit(ctx);
+}
+
+hdns_status hdns_max_speed(hdns_ctx *ctx, double *out) {
+  if (!ctx || !out) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "out must not be NULL");
+  Ctx &c = ctx->c;
+  if (!c.deposited) return fail(ctx, HDNS_ERR_STATE, "max_speed has not been deposited");
+  unsigned long long bits = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&bits, c.maxbits, sizeof bits, hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  double v;
+  memcpy(&v, &bits, sizeof v);
+  *out = sqrt(v);
+  return HDNS_OK;
+}
+
+hdns_status hdns_reductions(hdns_ctx *ctx, double *out) {
+  if (!ctx || !out) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "out must not be NULL");
+  Ctx &c = ctx->c;
+  if (!c.deposited) return fail(ctx, HDNS_ERR_STATE, "no deposit of the current state");
+  double s[8];
+  HIP_TRY(ctx, hipMemcpyAsync(s, c.scalars, sizeof s, hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  const double kappa = c.cfg.viscosity / c.cfg.schmidt;
+  out[0] = 0.5 * s[1];
+  out[1] = c.cfg.viscosity * s[3];
+  out[2] = c.cfg.has_scalar ? s[4] : 0.0;
+  out[3] = c.cfg.has_scalar ? 2.0 * kappa * s[5] : 0.0;
+  out[4] = 0.5 * s[2];
+  return HDNS_OK;
+}
+
+hdns_status hdns_spectrum(hdns_ctx *ctx, int32_t which, double *out, int32_t nshells) {
+  if (!ctx || !out) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "out must not be NULL");
+  Ctx &c = ctx->c;
+  if (which != HDNS_FIELD_VELOCITY && which != HDNS_FIELD_SCALAR)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "which must be the velocity or the scalar");
+  if (which == HDNS_FIELD_SCALAR && !c.cfg.has_scalar)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "this context carries no scalar");
+  if (nshells < c.nbins)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "nshells must be at least " + std::to_string(c.nbins));
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  zero_f64(c, c.spec_bins, c.nbins * sizeof(double));
+  c.pipe->spectrum(c, which, c.spec_bins);
+  hdns_status s = launched(ctx);
+  if (s != HDNS_OK) return s;
+  std::vector<double> bins(c.nbins);
+  HIP_TRY(ctx, hipMemcpyAsync(bins.data(), c.spec_bins, c.nbins * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  const double norm = pow((double)c.geo.n, 6.0);
+  const double width = which == HDNS_FIELD_VELOCITY ? 2.0 * c.geo.k0 : c.geo.k0;
+  for (int i = 0; i < nshells; ++i) out[i] = i < c.nbins ? bins[i] / norm / width : 0.0;
+  return HDNS_OK;
+}
+
+hdns_status hdns_field_download(hdns_ctx *ctx, int32_t which, double *out) {
+  if (!ctx || !out) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "out must not be NULL");
+  Ctx &c = ctx->c;
+  if (!c.deposited) return fail(ctx, HDNS_ERR_STATE, "no deposit of the current state");
+  const size_t vol = (size_t)c.geo.n * c.geo.n * c.geo.n;
+  const void *src = nullptr;
+  int ncomp = 0;
+  if (which == HDNS_FIELD_VELOCITY && (c.cfg.store_physical & 1)) {
+    src = c.phys;
+    ncomp = 3;
+  } else if (which == HDNS_FIELD_SCALAR && (c.cfg.store_physical & 2) && c.cfg.has_scalar) {
+    src = (const char *)c.phys + 3 * vol * (c.fp64 ? 8 : 4);
+    ncomp = 1;
+  } else if (which == HDNS_FIELD_VORTICITY && (c.cfg.store_physical & 4)) {
+    src = c.physw;
+    ncomp = 3;
+  } else {
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "this context does not keep that physical field (store_physical)");
+  }
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  if (c.fp64) {
+    HIP_TRY(ctx, hipMemcpyAsync(out, src, ncomp * vol * 8, hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  } else {
+    std::vector<float> tmp(ncomp * vol);
+    HIP_TRY(ctx, hipMemcpyAsync(tmp.data(), src, ncomp * vol * 4, hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+    for (size_t i = 0; i < tmp.size(); ++i) out[i] = tmp[i];
+  }
+  return HDNS_OK;
+}
+
+hdns_status hdns_tracers_upload(hdns_ctx *ctx, int64_t m, const double *positions) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  Ctx &c = ctx->c;
+  if (m < 0 || m > c.cfg.tracer_capacity)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT,
+                "tracer count must be within the context's tracer_capacity " + std::to_string(c.cfg.tracer_capacity));
+  if (m > 0 && !positions) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "positions must not be NULL");
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  c.m = m;
+  c.order_valid = false;
+  if (m == 0) return HDNS_OK;
+  if (c.fp64) {
+    HIP_TRY(ctx, hipMemcpyAsync(c.pos, positions, 3 * m * 8, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  } else {
+    std::vector<float> tmp(3 * m);
+    for (size_t i = 0; i < tmp.size(); ++i) tmp[i] = (float)positions[i];
+    HIP_TRY(ctx, hipMemcpyAsync(c.pos, tmp.data(), 3 * m * 4, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  }
+  return HDNS_OK;
+}
+
+static hdns_status copy_real_out(hdns_ctx *ctx, const void *src, size_t count, double *out) {
+  Ctx &c = ctx->c;
+  if (count == 0) return HDNS_OK;
+  if (c.fp64) {
+    HIP_TRY(ctx, hipMemcpyAsync(out, src, count * 8, hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  } else {
+    std::vector<float> tmp(count);
+    HIP_TRY(ctx, hipMemcpyAsync(tmp.data(), src, count * 4, hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+    for (size_t i = 0; i < count; ++i) out[i] = tmp[i];
+  }
+  return HDNS_OK;
+}
+
+hdns_status hdns_tracers_download(hdns_ctx *ctx, double *positions) {
+  if (!ctx || !positions) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "positions must not be NULL");
+  HIP_TRY(ctx, hipSetDevice(ctx->c.cfg.device));
+  return copy_real_out(ctx, ctx->c.pos, 3 * (size_t)ctx->c.m, positions);
+}
+
+hdns_status hdns_rebuild_traversal(hdns_ctx *ctx) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  Ctx &c = ctx->c;
+  if (c.cfg.tracer_capacity == 0) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "this context carries no tracers");
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  c.pipe->tile_sort(c);
+  c.order_valid = c.m > 0;
+  return launched(ctx);
+}
+
+hdns_status hdns_advect(hdns_ctx *ctx, double dt) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  Ctx &c = ctx->c;
+  if (!(isfinite(dt) && dt > 0))
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT,
+                "`dt` must be positive and finite: the samples a particle scheme integrates over must lie at "
+                "increasing times");
+  if (c.cfg.tracer_capacity == 0) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "this context carries no tracers");
+  if (!c.deposited) return fail(ctx, HDNS_ERR_STATE, "no deposited velocity to advect the tracers with");
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  double w[3];
+  hdns_heun_weights(dt, w);
+  c.pipe->advect(c, w[0], w[1], w[2]);
+  return launched(ctx);
+}
+
+hdns_status hdns_deposit_tracers(hdns_ctx *ctx) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  Ctx &c = ctx->c;
+  if (c.cfg.tracer_capacity == 0) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "this context carries no tracers");
+  if (!c.deposited) return fail(ctx, HDNS_ERR_STATE, "no deposited velocity to sample at the tracers");
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  c.pipe->sample(c);
+  return launched(ctx);
+}
+
+hdns_status hdns_tracer_samples(hdns_ctx *ctx, double *velocity, double *scalar) {
+  if (!ctx || !velocity) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "velocity must not be NULL");
+  Ctx &c = ctx->c;
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  hdns_status s = copy_real_out(ctx, c.vel, 3 * (size_t)c.m, velocity);
+  if (s != HDNS_OK || !scalar || !c.cfg.has_scalar) return s;
+  return copy_real_out(ctx, c.phis, (size_t)c.m, scalar);
+}
+
+hdns_status hdns_tracer_cells(hdns_ctx *ctx, int64_t *keys) {
+  if (!ctx || !keys) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "keys must not be NULL");
+  Ctx &c = ctx->c;
+  if (c.m == 0) return HDNS_OK;
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  int64_t *dev = nullptr;
+  HIP_TRY(ctx, hipMalloc((void **)&dev, c.m * sizeof(int64_t)));
+  c.pipe->cell_keys(c, dev);
+  hipError_t e = hipMemcpyAsync(keys, dev, c.m * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  (void)hipFree(dev);
+  if (e != hipSuccess) return fail(ctx, HDNS_ERR_DEVICE, std::string("cell keys: ") + hipGetErrorString(e));
+  return HDNS_OK;
+}
+
+hdns_status hdns_traversal_order(hdns_ctx *ctx, int32_t *order) {
+  if (!ctx || !order) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "order must not be NULL");
+  Ctx &c = ctx->c;
+  if (!c.order_valid) return fail(ctx, HDNS_ERR_STATE, "no traversal order has been built for these positions");
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  HIP_TRY(ctx, hipMemcpyAsync(order, c.order, c.m * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  return HDNS_OK;
+}
+
+hdns_status hdns_frame_tables(hdns_ctx *ctx, int32_t *mx, int32_t *my, int32_t *mz, uint8_t *mask) {
+  if (!ctx || !mx || !my || !mz || !mask) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "outputs must not be NULL");
+  Ctx &c = ctx->c;
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  const int n = c.geo.n, hx = n / 2 + 1;
+  const size_t total = (size_t)n * n * hx;
+  int *d = nullptr;
+  unsigned char *dm = nullptr;
+  HIP_TRY(ctx, hipMalloc((void **)&d, (hx + 2 * n) * sizeof(int)));
+  hipError_t e = hipMalloc((void **)&dm, total);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_frame_tables, dim3(1024), dim3(256), 0, c.stream, n, d, d + hx, d + hx + n, dm);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(mx, d, hx * sizeof(int), hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(my, d + hx, n * sizeof(int), hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(mz, d + hx + n, n * sizeof(int), hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(mask, dm, total, hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  (void)hipFree(d);
+  if (dm) (void)hipFree(dm);
+  if (e != hipSuccess) return fail(ctx, HDNS_ERR_DEVICE, std::string("frame tables: ") + hipGetErrorString(e));
+  return HDNS_OK;
+}
+
+hdns_status hdns_synchronize(hdns_ctx *ctx) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->c.stream));
+  return HDNS_OK;
+}
+
+hdns_status hdns_stream(hdns_ctx *ctx, void **stream) {
+  if (!ctx || !stream) return HDNS_ERR_INVALID_ARGUMENT;
+  *stream = (void *)ctx->c.stream;
+  return HDNS_OK;
+}
+
+}  // extern "C"

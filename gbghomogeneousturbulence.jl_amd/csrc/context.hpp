@@ -1,0 +1,66 @@
+// Host-side state of one run and the launch interface of a (precision, N) pipeline.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+
+#include "../../include/hdns.h"
+#include "spectral.hpp"
+
+namespace hdns {
+
+struct Ctx;
+
+// The kernels of one (element type, grid size) pair, launched on the context's stream.
+struct Pipeline {
+  virtual ~Pipeline() = default;
+  virtual void yfwd(Ctx &c) = 0;
+  virtual void zf(Ctx &c, const StageArgs &sa) = 0;
+  virtual void reality(Ctx &c) = 0;
+  virtual void zi(Ctx &c) = 0;
+  virtual void yinv(Ctx &c) = 0;
+  virtual void xpass(Ctx &c, const XArgs &xa) = 0;
+  virtual void spectrum(Ctx &c, int which, double *bins) = 0;
+  virtual void advect(Ctx &c, double b1, double beta1, double c2) = 0;
+  virtual void sample(Ctx &c) = 0;
+  virtual void cell_keys(Ctx &c, int64_t *keys) = 0;
+  virtual void tile_sort(Ctx &c) = 0;
+  virtual int tile_width() const = 0;  // kx columns per y / z block
+};
+
+std::unique_ptr<Pipeline> make_pipeline_f32(int n);
+std::unique_ptr<Pipeline> make_pipeline_f64(int n);
+
+struct Ctx {
+  hdns_config cfg{};
+  Geo geo{};
+  int fp64 = 0;
+  size_t elem = 0;          // bytes of one complex element
+  size_t comp = 0;          // complex elements of one compact component
+  size_t buf_a = 0, buf_b = 0;
+  int stages = 0;
+  BandSpec band{};
+  hipStream_t stream = nullptr;
+  void *uh = nullptr, *hu = nullptr, *ph = nullptr, *hp = nullptr;
+  void *bufa = nullptr, *bufb = nullptr;
+  void *phys = nullptr, *physw = nullptr;
+  cx<double> *tw = nullptr;
+  double *partials = nullptr;
+  double *scalars = nullptr;
+  unsigned long long *maxbits = nullptr;
+  double *spec_bins = nullptr;
+  int nbins = 0;
+  // tracers
+  int64_t m = 0;
+  void *pos = nullptr, *vel = nullptr, *phis = nullptr;
+  int *order = nullptr, *tile_counts = nullptr, *tile_cursor = nullptr;
+  int ntiles = 0;
+  bool order_valid = false;
+  bool deposited = false;
+  size_t bytes = 0;
+  std::unique_ptr<Pipeline> pipe;
+  std::string err;
+};
+
+}  // namespace hdns

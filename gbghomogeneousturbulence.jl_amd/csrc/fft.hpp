@@ -1,0 +1,205 @@
+// Device building blocks of the pencil transforms: a complex type, the small DFT butterflies,
+// and a block-cooperative Stockham (autosort) FFT over a batch of columns held in LDS.
+//
+// The batch layout in LDS is either
+//   interleaved  elem(n, col) = n*TX + col   (the column tile exactly as it sits in HBM, so the
+//                                            load and the store are straight copies), or
+//   contiguous   elem(n, col) = col*N + n    (one column after another; used for x rows).
+// Each pass with radix R over sub-transform length NS maps butterfly j (0 <= j < N/R) to
+//   inputs  j + q*N/R,  twiddle W_{NS*R}^{q*(j mod NS)},  outputs (j/NS)*NS*R + (j mod NS) + q*NS,
+// the Stockham self-sorting form, so the result lands in natural order with no bit reversal.
+// Twiddles come from an FP64 table of W_N^m = exp(-2 pi i m/N), rounded to the working type
+// at each use.  SIGN = -1 is the forward (unnormalised) transform, +1 the inverse.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace hdns {
+
+template <typename T>
+struct cx {
+  T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator*(cx<T> a, cx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> scale(cx<T> a, T s) { return {a.re * s, a.im * s}; }
+template <typename T>
+__device__ __forceinline__ cx<T> conj(cx<T> a) { return {a.re, -a.im}; }
+
+// a * (SIGN * i)
+template <int SIGN, typename T>
+__device__ __forceinline__ cx<T> mul_si(cx<T> a) {
+  if constexpr (SIGN > 0) return {-a.im, a.re};
+  else return {a.im, -a.re};
+}
+
+// a * (c + SIGN*i*s)
+template <int SIGN, typename T>
+__device__ __forceinline__ cx<T> mul_cs(cx<T> a, T c, T s) {
+  if constexpr (SIGN > 0) return {a.re * c - a.im * s, a.im * c + a.re * s};
+  else return {a.re * c + a.im * s, a.im * c - a.re * s};
+}
+
+template <typename T> struct fftc {
+  static constexpr T r2 = T(0.70710678118654752440084436210484903928);   // sqrt(2)/2
+  static constexpr T c8 = T(0.92387953251128675612818318939678828682);   // cos(pi/8)
+  static constexpr T s8 = T(0.38268343236508977172845998403039886676);   // sin(pi/8)
+  static constexpr T h3 = T(0.86602540378443864676372317075293618347);   // sqrt(3)/2
+};
+
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft2(cx<T> *v) {
+  cx<T> a = v[0];
+  v[0] = a + v[1];
+  v[1] = a - v[1];
+}
+
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft3(cx<T> *v) {
+  cx<T> t = v[1] + v[2];
+  cx<T> d = v[1] - v[2];
+  cx<T> m = {v[0].re - T(0.5) * t.re, v[0].im - T(0.5) * t.im};
+  cx<T> r = mul_si<SIGN>(scale(d, fftc<T>::h3));
+  v[0] = v[0] + t;
+  v[1] = m + r;
+  v[2] = m - r;
+}
+
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft4(cx<T> *v) {
+  cx<T> t0 = v[0] + v[2], t1 = v[0] - v[2];
+  cx<T> t2 = v[1] + v[3], t3 = mul_si<SIGN>(v[1] - v[3]);
+  v[0] = t0 + t2;
+  v[2] = t0 - t2;
+  v[1] = t1 + t3;
+  v[3] = t1 - t3;
+}
+
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft8(cx<T> *v) {
+  cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+  cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<SIGN>(e);
+  dft4<SIGN>(o);
+  const T r = fftc<T>::r2;
+  o[1] = mul_cs<SIGN>(o[1], r, r);
+  o[2] = mul_si<SIGN>(o[2]);
+  o[3] = mul_cs<SIGN>(o[3], -r, r);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = e[k] + o[k];
+    v[k + 4] = e[k] - o[k];
+  }
+}
+
+template <int SIGN, typename T>
+__device__ __forceinline__ void dft16(cx<T> *v) {
+  cx<T> e[8], o[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    e[k] = v[2 * k];
+    o[k] = v[2 * k + 1];
+  }
+  dft8<SIGN>(e);
+  dft8<SIGN>(o);
+  const T r = fftc<T>::r2, c = fftc<T>::c8, s = fftc<T>::s8;
+  o[1] = mul_cs<SIGN>(o[1], c, s);
+  o[2] = mul_cs<SIGN>(o[2], r, r);
+  o[3] = mul_cs<SIGN>(o[3], s, c);
+  o[4] = mul_si<SIGN>(o[4]);
+  o[5] = mul_cs<SIGN>(o[5], -s, c);
+  o[6] = mul_cs<SIGN>(o[6], -r, r);
+  o[7] = mul_cs<SIGN>(o[7], -c, s);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    v[k] = e[k] + o[k];
+    v[k + 8] = e[k] - o[k];
+  }
+}
+
+template <int R, int SIGN, typename T>
+__device__ __forceinline__ void dft(cx<T> *v) {
+  if constexpr (R == 2) dft2<SIGN>(v);
+  else if constexpr (R == 3) dft3<SIGN>(v);
+  else if constexpr (R == 4) dft4<SIGN>(v);
+  else if constexpr (R == 8) dft8<SIGN>(v);
+  else if constexpr (R == 16) dft16<SIGN>(v);
+  else static_assert(R == 2, "unsupported radix");
+}
+
+template <int N, int TX, bool IL>
+__device__ __forceinline__ int lds_index(int n, int col) {
+  if constexpr (IL) return n * TX + col;
+  else return col * N + n;
+}
+
+// One Stockham pass over TX columns of length N held in LDS, by NT cooperating threads.
+template <typename T, int N, int R, int NS, int TX, int NT, int SIGN, bool IL>
+__device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<double> *__restrict__ tw) {
+  constexpr int NB = N / R;
+  constexpr int TOT = NB * TX;
+  constexpr int PER = (TOT + NT - 1) / NT;
+  const int tid = threadIdx.x;
+  cx<T> v[PER][R];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int b = tid + p * NT;
+    if (TOT % NT == 0 || b < TOT) {
+      const int col = IL ? b % TX : b / NB;
+      const int j = IL ? b / TX : b % NB;
+#pragma unroll
+      for (int q = 0; q < R; ++q) v[p][q] = buf[lds_index<N, TX, IL>(j + q * NB, col)];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int b = tid + p * NT;
+    if (TOT % NT == 0 || b < TOT) {
+      const int col = IL ? b % TX : b / NB;
+      const int j = IL ? b / TX : b % NB;
+      const int k = j % NS;
+      if constexpr (NS > 1) {
+#pragma unroll
+        for (int q = 1; q < R; ++q) {
+          const cx<double> w = tw[q * k * (N / (NS * R))];
+          const cx<T> wt = {T(w.re), SIGN < 0 ? T(w.im) : T(-w.im)};
+          v[p][q] = v[p][q] * wt;
+        }
+      }
+      dft<R, SIGN>(v[p]);
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) buf[lds_index<N, TX, IL>(base + q * NS, col)] = v[p][q];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int TX, int NT, int SIGN, bool IL, int NS, int R, int... REST>
+__device__ __forceinline__ void run_passes(cx<T> *buf, const cx<double> *__restrict__ tw) {
+  stockham_pass<T, N, R, NS, TX, NT, SIGN, IL>(buf, tw);
+  if constexpr (sizeof...(REST) > 0) run_passes<T, N, TX, NT, SIGN, IL, NS * R, REST...>(buf, tw);
+}
+
+// The radix plan of each supported length.
+template <typename T, int N, int TX, int NT, int SIGN, bool IL>
+__device__ __forceinline__ void fft_columns(cx<T> *buf, const cx<double> *__restrict__ tw) {
+  if constexpr (N == 16) run_passes<T, N, TX, NT, SIGN, IL, 1, 4, 4>(buf, tw);
+  else if constexpr (N == 32) run_passes<T, N, TX, NT, SIGN, IL, 1, 8, 4>(buf, tw);
+  else if constexpr (N == 64) run_passes<T, N, TX, NT, SIGN, IL, 1, 8, 8>(buf, tw);
+  else if constexpr (N == 128) run_passes<T, N, TX, NT, SIGN, IL, 1, 8, 4, 4>(buf, tw);
+  else if constexpr (N == 256) run_passes<T, N, TX, NT, SIGN, IL, 1, 8, 8, 4>(buf, tw);
+  else if constexpr (N == 512) run_passes<T, N, TX, NT, SIGN, IL, 1, 8, 8, 8>(buf, tw);
+  else if constexpr (N == 1024) run_passes<T, N, TX, NT, SIGN, IL, 1, 8, 8, 4, 4>(buf, tw);
+  else static_assert(N == 16, "unsupported transform length");
+}
+
+}  // namespace hdns

@@ -1,0 +1,236 @@
+// The pencil kernels of the DNS stage: the two y passes, the x pass that forms the physical
+// products, and the two z passes that carry every spectral pointwise operation of the stage.
+//
+// Retained modes only.  Under the 2/3 rule every field the step keeps lies inside the cube
+// |mx|, |my|, |mz| <= M = floor(N/3), so spectral data is stored compactly as
+//   [c][kz_idx][ky_idx][kx]   kx in [0, KXP) (padded past M with zeros),
+//                             ky_idx, kz_idx in [0, K), K = 2M + 1, m = idx <= M ? idx : idx - K,
+// and the transforms are pruned: a z pass only ever touches the K retained ky columns, a y pass
+// only the kx <= M columns, and the x pass zero-fills kx > M.  The half-transformed pencils are
+//   PX [c][z][y][kx]   after the forward x pass      (6 product components)
+//   PF [c][z][ky][kx]  after the forward y pass
+//   Q  [c][z][ky][kx]  after the inverse z pass      (7 components: u, w, phi)
+//   R  [c][z][y][kx]   after the inverse y pass
+// PX/Q share one buffer and PF/R another: each is dead before its twin is written.
+#pragma once
+#include "fft.hpp"
+
+namespace hdns {
+
+struct Geo {
+  int n;      // grid points per axis
+  int m;      // largest retained |m| per axis, floor(n/3)
+  int kxp;    // padded kx extent of every compact array
+  int k;      // 2m + 1 retained signed wavenumbers on y and z
+  double k0;  // 2 pi / L
+};
+
+// FFT bin of a compact signed-wavenumber index.
+__device__ __forceinline__ int bin_of(int idx, const Geo &g) { return idx <= g.m ? idx : g.n - g.k + idx; }
+// Signed wavenumber index of a compact index.
+__device__ __forceinline__ int signed_of(int idx, const Geo &g) { return idx <= g.m ? idx : idx - g.k; }
+// Compact index of an FFT bin, or -1 when the bin is not retained.
+__device__ __forceinline__ int idx_of_bin(int b, const Geo &g) {
+  if (b <= g.m) return b;
+  if (b >= g.n - g.m) return b - g.n + g.k;
+  return -1;
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> czero() { return {T(0), T(0)}; }
+
+// ---------------------------------------------------------------------------------------------
+// y passes: a block transforms TX adjacent kx columns of one z plane of one component.
+
+template <typename T, int N, int TX, int NT>
+__global__ __launch_bounds__(NT) void k_yfwd(const cx<T> *__restrict__ px, cx<T> *__restrict__ pf, Geo g,
+                                             const cx<double> *__restrict__ tw) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+  const int kx0 = blockIdx.x * TX, z = blockIdx.y, c = blockIdx.z;
+  const cx<T> *src = px + ((size_t)c * N + z) * (size_t)N * g.kxp + kx0;
+  for (int e = threadIdx.x; e < N * TX; e += NT) buf[e] = src[(size_t)(e / TX) * g.kxp + (e % TX)];
+  __syncthreads();
+  fft_columns<T, N, TX, NT, -1, true>(buf, tw);
+  cx<T> *dst = pf + ((size_t)c * N + z) * (size_t)g.k * g.kxp + kx0;
+  for (int e = threadIdx.x; e < g.k * TX; e += NT) {
+    const int idx = e / TX, col = e % TX;
+    dst[(size_t)idx * g.kxp + col] = buf[bin_of(idx, g) * TX + col];
+  }
+}
+
+template <typename T, int N, int TX, int NT>
+__global__ __launch_bounds__(NT) void k_yinv(const cx<T> *__restrict__ q, cx<T> *__restrict__ r, Geo g,
+                                             const cx<double> *__restrict__ tw) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+  const int kx0 = blockIdx.x * TX, z = blockIdx.y, c = blockIdx.z;
+  const cx<T> *src = q + ((size_t)c * N + z) * (size_t)g.k * g.kxp + kx0;
+  for (int e = threadIdx.x; e < N * TX; e += NT) {
+    const int idx = idx_of_bin(e / TX, g);
+    buf[e] = idx >= 0 ? src[(size_t)idx * g.kxp + (e % TX)] : czero<T>();
+  }
+  __syncthreads();
+  fft_columns<T, N, TX, NT, +1, true>(buf, tw);
+  cx<T> *dst = r + ((size_t)c * N + z) * (size_t)N * g.kxp + kx0;
+  for (int e = threadIdx.x; e < N * TX; e += NT) dst[(size_t)(e / TX) * g.kxp + (e % TX)] = buf[e];
+}
+
+// ---------------------------------------------------------------------------------------------
+// x pass: a block takes XR = 4 pairs of adjacent y rows of one z plane.  Two real rows travel as
+// one complex transform (row a in the real part, row b in the imaginary part), so each component
+// costs one complex FFT per row pair.  It inverts the 7 (or 4) components to physical space,
+// forms u x w and u phi there, transforms the 6 (or 3) products forward and keeps kx <= M.
+// On request it also stores the physical u and phi (the deposit) and the max of |u|^2.
+
+struct XArgs {
+  int store_u, store_phi, store_w;
+  int want_max;
+  void *phys;             // [4][N][N][N] real: u_x, u_y, u_z, phi
+  void *physw;            // [3][N][N][N] real: w
+  unsigned long long *max_bits;  // max |u|^2 as the bits of a non-negative double
+};
+
+template <typename T, int N, int NT, int NCIN>
+__global__ __launch_bounds__(NT) void k_xpass(const cx<T> *__restrict__ rin, cx<T> *__restrict__ pxo, Geo g,
+                                              const cx<double> *__restrict__ tw, XArgs xa) {
+  constexpr int XR = 4;              // row pairs per block
+  constexpr int TPC = NT / XR;       // threads per row pair
+  constexpr int EPT = N / TPC;       // elements per thread per row pair
+  static_assert(NT % XR == 0 && N % TPC == 0, "x pass shape");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+  __shared__ double red[NT / 64];
+
+  const int rows_per_plane = N / (2 * XR);
+  const int z = blockIdx.x / rows_per_plane;
+  const int y0 = (blockIdx.x % rows_per_plane) * (2 * XR);
+  const int tid = threadIdx.x;
+  const int pr = tid / TPC, lane = tid % TPC;
+  const int half = N / 2;
+  const size_t plane = (size_t)N * g.kxp;            // one z plane of one component in R / PX
+  const size_t vol = (size_t)N * N * N;
+
+  static_assert(NCIN == 7 || NCIN == 4, "u, w and optionally phi");
+  constexpr int NCOUT = NCIN == 7 ? 6 : 3;
+  cx<T> f[NCIN][EPT];
+#pragma unroll
+  for (int c = 0; c < NCIN; ++c) {
+    const cx<T> *src = rin + (size_t)c * N * plane + (size_t)z * plane;
+    for (int e = tid; e < XR * (half + 1); e += NT) {
+      const int p = e / (half + 1), kx = e % (half + 1);
+      const int ya = y0 + 2 * p;
+      cx<T> a = czero<T>(), b = czero<T>();
+      if (kx <= g.m) {
+        a = src[(size_t)ya * g.kxp + kx];
+        b = src[(size_t)(ya + 1) * g.kxp + kx];
+      }
+      if (kx == 0 || kx == half) {  // the self-conjugate coefficients: only the real part is seen
+        a.im = T(0);
+        b.im = T(0);
+      }
+      buf[p * N + kx] = {a.re - b.im, a.im + b.re};
+      if (kx > 0 && kx < half) buf[p * N + (N - kx)] = {a.re + b.im, b.re - a.im};
+    }
+    __syncthreads();
+    fft_columns<T, N, XR, NT, +1, false>(buf, tw);
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) f[c][j] = buf[pr * N + lane + TPC * j];
+    __syncthreads();
+  }
+
+  // physical space: (re, im) = (row a, row b) of every component
+  double mx = 0.0;
+  if (xa.want_max) {
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const T sa = f[0][j].re * f[0][j].re + f[1][j].re * f[1][j].re + f[2][j].re * f[2][j].re;
+      const T sb = f[0][j].im * f[0][j].im + f[1][j].im * f[1][j].im + f[2][j].im * f[2][j].im;
+      const double da = (double)sa, db = (double)sb;
+      if (da > mx || da != da) mx = da;
+      if (db > mx || db != db) mx = db;
+    }
+  }
+  if (xa.store_u || xa.store_phi || xa.store_w) {
+    const int ya = y0 + 2 * pr;
+    const size_t rowa = ((size_t)z * N + ya) * N, rowb = rowa + N;
+    T *ph = reinterpret_cast<T *>(xa.phys);
+    T *pw = reinterpret_cast<T *>(xa.physw);
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      const int x = lane + TPC * j;
+      if (xa.store_u) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          ph[c * vol + rowa + x] = f[c][j].re;
+          ph[c * vol + rowb + x] = f[c][j].im;
+        }
+      }
+      if constexpr (NCIN == 7) {
+        if (xa.store_phi) {
+          ph[3 * vol + rowa + x] = f[NCIN - 1][j].re;
+          ph[3 * vol + rowb + x] = f[NCIN - 1][j].im;
+        }
+      }
+      if (xa.store_w) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          pw[c * vol + rowa + x] = f[3 + c][j].re;
+          pw[c * vol + rowb + x] = f[3 + c][j].im;
+        }
+      }
+    }
+  }
+
+#pragma unroll
+  for (int d = 0; d < NCOUT; ++d) {
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      cx<T> v;
+      if (d < 3) {  // the negated Lamb vector u x w (cross!(w, u, w), src/Generator.jl:529)
+        const int c1 = (d + 1) % 3, c2 = (d + 2) % 3;
+        v.re = f[c1][j].re * f[3 + c2][j].re - f[c2][j].re * f[3 + c1][j].re;
+        v.im = f[c1][j].im * f[3 + c2][j].im - f[c2][j].im * f[3 + c1][j].im;
+      } else {      // the flux u phi (multiply!(u, phi, u), src/Generator.jl:727)
+        v.re = f[d - 3][j].re * f[NCIN - 1][j].re;
+        v.im = f[d - 3][j].im * f[NCIN - 1][j].im;
+      }
+      buf[pr * N + lane + TPC * j] = v;
+    }
+    __syncthreads();
+    fft_columns<T, N, XR, NT, -1, false>(buf, tw);
+    cx<T> *dst = pxo + (size_t)d * N * plane + (size_t)z * plane;
+    for (int e = tid; e < XR * g.kxp; e += NT) {
+      const int p = e / g.kxp, kx = e % g.kxp;
+      const int ya = y0 + 2 * p;
+      cx<T> A = czero<T>(), B = czero<T>();
+      if (kx <= g.m) {
+        const cx<T> C = buf[p * N + kx];
+        const cx<T> Cm = buf[p * N + (kx == 0 ? 0 : N - kx)];
+        A = {T(0.5) * (C.re + Cm.re), T(0.5) * (C.im - Cm.im)};
+        B = {T(0.5) * (C.im + Cm.im), T(0.5) * (Cm.re - C.re)};
+      }
+      dst[(size_t)ya * g.kxp + kx] = A;
+      dst[(size_t)(ya + 1) * g.kxp + kx] = B;
+    }
+    __syncthreads();
+  }
+
+  if (xa.want_max) {
+    // wave then block max, NaN-propagating, then one atomic per block
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_down(mx, off, 64);
+      if (o > mx || o != o) mx = o;
+    }
+    if ((tid & 63) == 0) red[tid >> 6] = mx;
+    __syncthreads();
+    if (tid == 0) {
+      double b = red[0];
+      for (int w = 1; w < NT / 64; ++w)
+        if (red[w] > b || red[w] != red[w]) b = red[w];
+      atomicMax(xa.max_bits, (unsigned long long)__double_as_longlong(b));
+    }
+  }
+}
+
+}  // namespace hdns

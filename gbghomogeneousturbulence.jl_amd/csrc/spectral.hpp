@@ -1,0 +1,410 @@
+// The z passes, which carry every spectral pointwise operation of a stage, and the small
+// kernels around them (reality projection, the FP64 reductions, spectra, frame tables).
+//
+// A z-pass block owns the TX adjacent kx columns of one retained ky, over all z.  Its threads
+// own retained modes (kz_idx, col) in a fixed round-robin; the column transform runs in LDS in
+// the interleaved layout [z][TX], which is exactly the tile as it sits in HBM.
+#pragma once
+#include "pencil.hpp"
+
+namespace hdns {
+
+// The canonical 2/3 mask: the closed ball 9|m|^2 <= N^2 in exact integers
+// (src/SimulationDomain.jl:342-376 builds the ball of radius (2/3)(N/2)(2 pi/L)).
+__host__ __device__ __forceinline__ bool in_two_thirds(int m2, int n) { return 9 * m2 <= n * n; }
+
+struct BandSpec {
+  int on;      // ConstantPower present
+  int lo4;     // 4|m|^2 >= lo4 = (2*lowest - 1)^2
+  int hi4;     // 4|m|^2 <  hi4 = (2*highest + 1)^2, or -1 for no upper edge
+};
+
+__device__ __forceinline__ bool in_band(int m2, const BandSpec &b) {
+  const int f = 4 * m2;
+  return b.on && f >= b.lo4 && (b.hi4 < 0 || f < b.hi4);
+}
+
+struct StageArgs {
+  int stage;          // 0-based stage index
+  int last;           // 1 at the final stage
+  double a, b;        // low-storage coefficients, already rounded to the element type
+  double dt;          // the step, rounded to the element type for the accumulator
+  double coef_u;      // -nu * d_i * dt: the velocity exponent per |k|^2
+  double coef_p;      // -(nu/Sc) * d_i * dt
+  double mean_gradient;
+  BandSpec band;
+  const double *scalars;  // [0] = the ConstantPower amplitude at this stage's state
+};
+
+// Every spectral operation of one stage (src/Propagator.jl:338-392), fused:
+//   N_u  = P mask FFT_z(PF_u), mean mode zeroed, + A 1_band u        (src/Generator.jl:484-534,
+//                                                                      src/Forcing.jl:1104-1117)
+//   N_phi = -mask i k . FFT_z(PF_flux) - G u_z                        (src/Generator.jl:648-730)
+//   h <- a h + dt N;  q <- E (q + b h);  h <- E h  (not at the last)  (src/Propagator.jl:366-391)
+//   and at the last stage the state Leray projection                  (src/Propagator.jl:401).
+// E = exp(coef |k|^2) is evaluated in FP64 per mode and rounded to T.
+template <typename T, int N, int TX, int NT, bool SCALAR>
+__global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *__restrict__ uh,
+                                           cx<T> *__restrict__ hu, cx<T> *__restrict__ ph,
+                                           cx<T> *__restrict__ hp, Geo g, const cx<double> *__restrict__ tw,
+                                           StageArgs sa) {
+  constexpr int M = N / 3, K = 2 * M + 1;
+  constexpr int MODES = K * TX;
+  constexpr int MPT = (MODES + NT - 1) / NT;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+  const int tid = threadIdx.x;
+  const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y;
+  const int my = signed_of(ky_idx, g);
+  const T ky = T(g.k0 * my);
+  const size_t comp = (size_t)K * K * g.kxp;           // one compact component
+  const size_t pencil = (size_t)N * K * g.kxp;         // one PF component
+
+  // velocity products, forward z transform
+  cx<T> F[3][MPT];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const cx<T> *src = pf + (size_t)c * pencil + (size_t)ky_idx * g.kxp + kx0;
+    for (int e = tid; e < N * TX; e += NT) buf[e] = src[(size_t)(e / TX) * K * g.kxp + (e % TX)];
+    __syncthreads();
+    fft_columns<T, N, TX, NT, -1, true>(buf, tw);
+#pragma unroll
+    for (int r = 0; r < MPT; ++r) {
+      const int e = tid + NT * r;
+      if (MODES % NT == 0 || e < MODES) F[c][r] = buf[bin_of(e / TX, g) * TX + (e % TX)];
+    }
+    __syncthreads();
+  }
+
+  const T A = sa.band.on ? T(sa.scalars[0]) : T(0);
+  const T a = T(sa.a), b = T(sa.b), dt = T(sa.dt);
+  cx<T> uz_old[MPT];
+#pragma unroll
+  for (int r = 0; r < MPT; ++r) {
+    const int e = tid + NT * r;
+    if (!(MODES % NT == 0 || e < MODES)) continue;
+    const int col = e % TX, kz_idx = e / TX;
+    const int mx = kx0 + col, mz = signed_of(kz_idx, g);
+    const int m2 = mx * mx + my * my + mz * mz;
+    const T kx = T(g.k0 * mx), kz = T(g.k0 * mz);
+    const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
+    const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
+    const bool keep = in_two_thirds(m2, N) && m2 > 0;
+    cx<T> nl[3];
+    if (keep) {
+      const T k2 = T(k2d);
+      const cx<T> kf = {kx * F[0][r].re + ky * F[1][r].re + kz * F[2][r].re,
+                        kx * F[0][r].im + ky * F[1][r].im + kz * F[2][r].im};
+      const cx<T> s = {kf.re / k2, kf.im / k2};
+      nl[0] = F[0][r] - scale(s, kx);
+      nl[1] = F[1][r] - scale(s, ky);
+      nl[2] = F[2][r] - scale(s, kz);
+    } else {
+      nl[0] = nl[1] = nl[2] = czero<T>();
+    }
+    const size_t at = (size_t)kz_idx * K * g.kxp + (size_t)ky_idx * g.kxp + mx;
+    cx<T> u[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u[c] = uh[c * comp + at];
+    uz_old[r] = u[2];
+    if (in_band(m2, sa.band)) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) nl[c] = nl[c] + scale(u[c], A);
+    }
+    const T E = T(exp(sa.coef_u * k2d));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      cx<T> h = scale(nl[c], dt);
+      if (sa.stage > 0) h = scale(hu[c * comp + at], a) + h;
+      u[c] = scale(u[c] + scale(h, b), E);
+      if (!sa.last) hu[c * comp + at] = scale(h, E);
+    }
+    if (sa.last && m2 > 0) {  // helmholtz_projection!(u, u): P without the mask
+      const T k2 = T(k2d);
+      const cx<T> kf = {kx * u[0].re + ky * u[1].re + kz * u[2].re, kx * u[0].im + ky * u[1].im + kz * u[2].im};
+      const cx<T> s = {kf.re / k2, kf.im / k2};
+      u[0] = u[0] - scale(s, kx);
+      u[1] = u[1] - scale(s, ky);
+      u[2] = u[2] - scale(s, kz);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) uh[c * comp + at] = u[c];
+  }
+
+  if constexpr (SCALAR) {
+    cx<T> G[MPT];
+#pragma unroll
+    for (int r = 0; r < MPT; ++r) G[r] = czero<T>();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const cx<T> *src = pf + (size_t)(3 + c) * pencil + (size_t)ky_idx * g.kxp + kx0;
+      for (int e = tid; e < N * TX; e += NT) buf[e] = src[(size_t)(e / TX) * K * g.kxp + (e % TX)];
+      __syncthreads();
+      fft_columns<T, N, TX, NT, -1, true>(buf, tw);
+#pragma unroll
+      for (int r = 0; r < MPT; ++r) {
+        const int e = tid + NT * r;
+        if (MODES % NT == 0 || e < MODES) {
+          const int mc = c == 0 ? kx0 + (e % TX) : (c == 1 ? my : signed_of(e / TX, g));
+          const T kc = T(g.k0 * mc);
+          const cx<T> v = buf[bin_of(e / TX, g) * TX + (e % TX)];
+          G[r] = {G[r].re + kc * v.re, G[r].im + kc * v.im};
+        }
+      }
+      __syncthreads();
+    }
+    const T Gm = T(sa.mean_gradient);
+#pragma unroll
+    for (int r = 0; r < MPT; ++r) {
+      const int e = tid + NT * r;
+      if (!(MODES % NT == 0 || e < MODES)) continue;
+      const int col = e % TX, kz_idx = e / TX;
+      const int mx = kx0 + col, mz = signed_of(kz_idx, g);
+      const int m2 = mx * mx + my * my + mz * mz;
+      const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
+      const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
+      cx<T> nl = in_two_thirds(m2, N) ? cx<T>{G[r].im, -G[r].re} : czero<T>();   // -i k.F
+      nl = nl - scale(uz_old[r], Gm);
+      const size_t at = (size_t)kz_idx * K * g.kxp + (size_t)ky_idx * g.kxp + mx;
+      const T E = T(exp(sa.coef_p * k2d));
+      cx<T> h = scale(nl, dt);
+      if (sa.stage > 0) h = scale(hp[at], a) + h;
+      const cx<T> q = scale(ph[at] + scale(h, b), E);
+      ph[at] = q;
+      if (!sa.last) hp[at] = scale(h, E);
+    }
+  }
+}
+
+// The inverse z pass of the state: u, w = i k x u and phi (with the 1/N^3 of the inverse
+// transform folded in) into Q, plus the per-block FP64 partial sums every later consumer needs:
+// [0] sum w|u|^2, [1] band sum w|u|^2, [2] sum w |k|^2 |u|^2, [3] sum w|phi|^2, [4] sum w |k|^2 |phi|^2
+// (w the Parseval weight of the half spectrum).  This is to_physical! + vorticity_physical!'s
+// spectral half (src/Propagator.jl:349-363, src/Observables.jl:329-333) and the state side of
+// weighted_inner (src/Forcing.jl:1109, src/Observables.jl:150-173).
+constexpr int NRED = 5;
+
+template <typename T, int N, int TX, int NT, bool SCALAR>
+__global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const cx<T> *__restrict__ ph,
+                                           cx<T> *__restrict__ q, Geo g, const cx<double> *__restrict__ tw,
+                                           BandSpec band, double *__restrict__ partials) {
+  constexpr int M = N / 3, K = 2 * M + 1;
+  constexpr int MODES = K * TX;
+  constexpr int MPT = (MODES + NT - 1) / NT;
+  constexpr int NCIN = SCALAR ? 7 : 4;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+  __shared__ double red[NT / 64][NRED];
+  const int tid = threadIdx.x;
+  const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y;
+  const int my = signed_of(ky_idx, g);
+  const size_t comp = (size_t)K * K * g.kxp;
+  const size_t pencil = (size_t)N * K * g.kxp;
+  const T inv = T(1.0 / ((double)N * N * N));
+
+  cx<T> u[3][MPT];
+  cx<T> p[MPT];
+  double acc[NRED] = {0, 0, 0, 0, 0};
+#pragma unroll
+  for (int r = 0; r < MPT; ++r) {
+    const int e = tid + NT * r;
+    if (!(MODES % NT == 0 || e < MODES)) continue;
+    const int col = e % TX, kz_idx = e / TX;
+    const int mx = kx0 + col, mz = signed_of(kz_idx, g);
+    const size_t at = (size_t)kz_idx * K * g.kxp + (size_t)ky_idx * g.kxp + mx;
+    const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
+    const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
+    const double w = mx == 0 ? 1.0 : 2.0;
+    double e2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      u[c][r] = uh[c * comp + at];
+      e2 += (double)u[c][r].re * (double)u[c][r].re + (double)u[c][r].im * (double)u[c][r].im;
+    }
+    acc[0] += w * e2;
+    if (in_band(mx * mx + my * my + mz * mz, band)) acc[1] += w * e2;
+    acc[2] += w * k2d * e2;
+    if constexpr (SCALAR) {
+      p[r] = ph[at];
+      const double pp = (double)p[r].re * (double)p[r].re + (double)p[r].im * (double)p[r].im;
+      acc[3] += w * pp;
+      acc[4] += w * k2d * pp;
+    }
+  }
+
+#pragma unroll
+  for (int c = 0; c < NCIN; ++c) {
+    // the gap between the retained bins is zero
+    for (int e = tid; e < (N - K) * TX; e += NT) buf[(M + 1 + e / TX) * TX + (e % TX)] = czero<T>();
+#pragma unroll
+    for (int r = 0; r < MPT; ++r) {
+      const int e = tid + NT * r;
+      if (!(MODES % NT == 0 || e < MODES)) continue;
+      const int col = e % TX, kz_idx = e / TX;
+      cx<T> v;
+      if (c < 3) {
+        v = u[c][r];
+      } else if (c < 6) {  // i k x u
+        const int mx = kx0 + col, mz = signed_of(kz_idx, g);
+        const T kx = T(g.k0 * mx), ky = T(g.k0 * my), kz = T(g.k0 * mz);
+        cx<T> t;
+        if (c == 3) t = scale(u[2][r], ky) - scale(u[1][r], kz);
+        else if (c == 4) t = scale(u[0][r], kz) - scale(u[2][r], kx);
+        else t = scale(u[1][r], kx) - scale(u[0][r], ky);
+        v = {-t.im, t.re};
+      } else {
+        v = p[r];
+      }
+      buf[bin_of(kz_idx, g) * TX + col] = scale(v, inv);
+    }
+    __syncthreads();
+    fft_columns<T, N, TX, NT, +1, true>(buf, tw);
+    cx<T> *dst = q + (size_t)c * pencil + (size_t)ky_idx * g.kxp + kx0;
+    for (int e = tid; e < N * TX; e += NT) dst[(size_t)(e / TX) * K * g.kxp + (e % TX)] = buf[e];
+    __syncthreads();
+  }
+
+  // deterministic block reduction: wave shuffles, then the waves in order
+#pragma unroll
+  for (int i = 0; i < NRED; ++i) {
+    double v = acc[i];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((tid & 63) == 0) red[tid >> 6][i] = v;
+  }
+  __syncthreads();
+  if (tid < NRED) {
+    double s = 0.0;
+    for (int w = 0; w < NT / 64; ++w) s += red[w][tid];
+    partials[(size_t)(blockIdx.y * gridDim.x + blockIdx.x) * NRED + tid] = s;
+  }
+}
+
+// reality_projection! on the kx = 0 plane of u_hat and phi_hat (src/Propagator.jl:411-412): every
+// conjugate pair becomes (f(k) + conj f(-k))/2; the self-paired mean mode keeps its real part.
+template <typename T>
+__global__ void k_reality(cx<T> *__restrict__ uh, cx<T> *__restrict__ ph, Geo g, int ncomp) {
+  const int K = g.k;
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= K * K) return;
+  const int kz_idx = l / K, ky_idx = l % K;
+  const int pz = (K - kz_idx) % K, py = (K - ky_idx) % K;
+  const int lp = pz * K + py;
+  if (lp < l) return;
+  const size_t comp = (size_t)K * K * g.kxp;
+  const size_t a0 = (size_t)kz_idx * K * g.kxp + (size_t)ky_idx * g.kxp;
+  const size_t a1 = (size_t)pz * K * g.kxp + (size_t)py * g.kxp;
+  for (int c = 0; c < ncomp; ++c) {
+    cx<T> *f = c < 3 ? uh + c * comp : ph;
+    const cx<T> x = f[a0], y = f[a1];
+    const cx<T> nx = {T(0.5) * (x.re + y.re), T(0.5) * (x.im - y.im)};
+    f[a0] = nx;
+    if (lp != l) f[a1] = conj(nx);
+  }
+}
+
+// Neumaier-compensated FP64 sum of the per-block partials, then the ConstantPower amplitude of
+// the state they measured (src/Forcing.jl:1091-1117, the empty-band clamp at :110-111).
+// scalars: [0] amplitude A, [1] <u,u>, [2] <u,1_band u>, [3] <u,|k|^2 u>, [4] <phi,phi>,
+//          [5] <phi,|k|^2 phi>   (normalised inner products)
+struct FinalizeArgs {
+  int nblocks;
+  double inv_norm;       // 1/N^6
+  int forcing;
+  double injection_rate;
+  double eps_t;          // eps of the run's element type
+};
+
+static __global__ void k_finalize(const double *__restrict__ partials, double *__restrict__ scalars, FinalizeArgs fa) {
+  __shared__ double ss[256][NRED], cc[256][NRED];
+  const int tid = threadIdx.x;
+  double s[NRED] = {0, 0, 0, 0, 0}, c[NRED] = {0, 0, 0, 0, 0};
+  for (int b = tid; b < fa.nblocks; b += 256) {
+#pragma unroll
+    for (int i = 0; i < NRED; ++i) {
+      const double x = partials[(size_t)b * NRED + i];
+      const double t = s[i] + x;
+      c[i] += fabs(s[i]) >= fabs(x) ? (s[i] - t) + x : (x - t) + s[i];
+      s[i] = t;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NRED; ++i) {
+    ss[tid][i] = s[i];
+    cc[tid][i] = c[i];
+  }
+  __syncthreads();
+  if (tid < NRED) {
+    double S = 0.0, C = 0.0;
+    for (int t = 0; t < 256; ++t) {
+      const double x = ss[t][tid];
+      const double u = S + x;
+      C += fabs(S) >= fabs(x) ? (S - u) + x : (x - u) + S;
+      S = u;
+      C += cc[t][tid];
+    }
+    scalars[1 + tid] = (S + C) * fa.inv_norm;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double A = 0.0;
+    if (fa.forcing) {
+      const double band = 0.5 * scalars[2], whole = 0.5 * scalars[1];
+      const bool empty = !(isfinite(band) && band > 10.0 * fa.eps_t * whole);
+      A = empty ? 0.0 : fa.injection_rate / (2.0 * band);
+    }
+    scalars[0] = A;
+  }
+}
+
+// Shell-binned spectrum of the compact state (src/Observables.jl:502-514): shell n of a mode is
+// the integer with (2n-1)^2 <= 4|m|^2 < (2n+1)^2.  Atomic FP64 accumulation: a diagnostic off the
+// step, reproducible to the last bit only up to summation order.
+template <typename T>
+__global__ void k_spectrum(const cx<T> *__restrict__ f, int ncomp, Geo g, double *__restrict__ bins) {
+  const int K = g.k;
+  const size_t total = (size_t)K * K * g.kxp;
+  const size_t comp = total;
+  for (size_t l = blockIdx.x * (size_t)blockDim.x + threadIdx.x; l < total; l += (size_t)gridDim.x * blockDim.x) {
+    const int mx = (int)(l % g.kxp);
+    if (mx > g.m) continue;
+    const int ky_idx = (int)((l / g.kxp) % K), kz_idx = (int)(l / ((size_t)g.kxp * K));
+    const int my = signed_of(ky_idx, g), mz = signed_of(kz_idx, g);
+    const int four = 4 * (mx * mx + my * my + mz * mz);
+    int n = (int)floor(sqrt((double)four) * 0.5 + 0.5);
+    while ((2 * n - 1) * (2 * n - 1) > four) --n;
+    while ((2 * n + 1) * (2 * n + 1) <= four) ++n;
+    double e2 = 0.0;
+    for (int c = 0; c < ncomp; ++c) {
+      const cx<T> v = f[c * comp + l];
+      e2 += (double)v.re * (double)v.re + (double)v.im * (double)v.im;
+    }
+    const double w = mx == 0 ? 1.0 : 2.0;
+    if (e2 != 0.0) atomicAdd(&bins[n], w * e2);
+  }
+}
+
+// The frame tables as the device computes them: signed indices of the three axes and the mask
+// over the reference's full half-spectrum layout [N][N][N/2+1].
+static __global__ void k_frame_tables(int n, int *mx, int *my, int *mz, unsigned char *mask) {
+  const size_t hx = n / 2 + 1;
+  const size_t total = (size_t)n * n * hx;
+  for (size_t l = blockIdx.x * (size_t)blockDim.x + threadIdx.x; l < total; l += (size_t)gridDim.x * blockDim.x) {
+    const int ix = (int)(l % hx), iy = (int)((l / hx) % n), iz = (int)(l / (hx * n));
+    const int sx = ix, sy = iy <= n / 2 ? iy : iy - n, sz = iz <= n / 2 ? iz : iz - n;
+    if (iy == 0 && iz == 0) mx[ix] = sx;
+    if (ix == 0 && iz == 0) my[iy] = sy;
+    if (ix == 0 && iy == 0) mz[iz] = sz;
+    mask[l] = in_two_thirds(sx * sx + sy * sy + sz * sz, n) ? 1 : 0;
+  }
+}
+
+static __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+static __global__ void k_fill_f64(double *p, double v, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+}  // namespace hdns

@@ -1,0 +1,194 @@
+// Tracer kernels: the Catmull-Rom gather, the Heun push with the periodic wrap, the closing
+// sample, the canonical cell keys, and the counting sort that builds the traversal order.
+//
+// Positions and samples are stored [3][M] (the reference's (M, 3) column-major PointSet matrix).
+// Grid point (i, j, k) sits at (i, j, k) * L/N; the field is [c][z][y][x].  The cell of x is
+// floor(x * (N/L)) with N/L rounded once to the element type and the product formed in it, the
+// contract the tests hold bit-exact.  N is a power of two, so periodic indices wrap with a mask.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hdns {
+
+template <typename T>
+__device__ __forceinline__ T wrap_box(T x, T L) {
+  // test/regressions/02-api-contracts-02.jl:1-6: fmod, lift negatives by L, and an equality
+  // test at L (NaN and Inf fall through both comparisons untouched)
+  T r = fmod(x, L);
+  if (r < T(0)) r += L;
+  if (r == L) r = T(0);
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void catmull_rom(T t, T *w) {
+  w[0] = t * ((T(2) - t) * t - T(1)) * T(0.5);
+  w[1] = (t * t * (T(3) * t - T(5)) + T(2)) * T(0.5);
+  w[2] = t * ((T(4) - T(3) * t) * t + T(1)) * T(0.5);
+  w[3] = t * t * (t - T(1)) * T(0.5);
+}
+
+struct Stencil {
+  int i, j, k;
+};
+
+template <typename T>
+__device__ __forceinline__ Stencil stencil_of(T x, T y, T z, T inv_dx, int n, T *wx, T *wy, T *wz) {
+  const T sx = x * inv_dx, sy = y * inv_dx, sz = z * inv_dx;
+  const T fx = floor(sx), fy = floor(sy), fz = floor(sz);
+  catmull_rom(sx - fx, wx);
+  catmull_rom(sy - fy, wy);
+  catmull_rom(sz - fz, wz);
+  return {((int)fx) & (n - 1), ((int)fy) & (n - 1), ((int)fz) & (n - 1)};
+}
+
+// Separable 4^3 gather of NF components (stride vol apart) at one point.
+template <typename T, int NF>
+__device__ __forceinline__ void gather(const T *__restrict__ f, size_t vol, int n, const Stencil &s, const T *wx,
+                                       const T *wy, const T *wz, T *out) {
+  const int msk = n - 1;
+  int ii[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) ii[a] = (s.i + a - 1) & msk;
+#pragma unroll
+  for (int q = 0; q < NF; ++q) out[q] = T(0);
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int kk = (s.k + c - 1) & msk;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int jj = (s.j + b - 1) & msk;
+      const size_t row = ((size_t)kk * n + jj) * n;
+      const T wzy = wz[c] * wy[b];
+#pragma unroll
+      for (int q = 0; q < NF; ++q) {
+        const T *fr = f + q * vol + row;
+        T r = T(0);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) r += wx[a] * fr[ii[a]];
+        out[q] += wzy * r;
+      }
+    }
+  }
+}
+
+struct TracerArgs {
+  int64_t m;
+  int n;
+  double side_length;
+};
+
+// advect!(Heun) (src/ParticlePropagator.jl:288-315): X* = wrap(X + b1 v), f* = u(X*),
+// X <- wrap(X* + beta1 f* + (beta2 - b1) v).  The coefficients arrive rounded to T.
+template <typename T>
+__global__ void k_advect(T *__restrict__ pos, const T *__restrict__ vel, const T *__restrict__ u,
+                         const int *__restrict__ order, TracerArgs ta, T b1, T beta1, T c2) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= ta.m) return;
+  const int64_t p = order ? order[t] : t;
+  const T L = T(ta.side_length), inv_dx = T(ta.n / ta.side_length);
+  const size_t vol = (size_t)ta.n * ta.n * ta.n;
+  T X[3], V[3], Xs[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    X[d] = pos[d * ta.m + p];
+    V[d] = vel[d * ta.m + p];
+    Xs[d] = wrap_box(X[d] + b1 * V[d], L);
+  }
+  T wx[4], wy[4], wz[4], fs[3];
+  const Stencil s = stencil_of(Xs[0], Xs[1], Xs[2], inv_dx, ta.n, wx, wy, wz);
+  gather<T, 3>(u, vol, ta.n, s, wx, wy, wz, fs);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) pos[d * ta.m + p] = wrap_box((Xs[d] + beta1 * fs[d]) + c2 * V[d], L);
+}
+
+// The particle deposit! (src/ParticlePropagator.jl:378-386): u (and phi) at the positions.
+template <typename T>
+__global__ void k_sample(const T *__restrict__ pos, T *__restrict__ vel, T *__restrict__ phis,
+                         const T *__restrict__ u, const T *__restrict__ phi, const int *__restrict__ order,
+                         TracerArgs ta) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= ta.m) return;
+  const int64_t p = order ? order[t] : t;
+  const T inv_dx = T(ta.n / ta.side_length);
+  const size_t vol = (size_t)ta.n * ta.n * ta.n;
+  T wx[4], wy[4], wz[4], v[3];
+  const Stencil s = stencil_of(pos[p], pos[ta.m + p], pos[2 * ta.m + p], inv_dx, ta.n, wx, wy, wz);
+  gather<T, 3>(u, vol, ta.n, s, wx, wy, wz, v);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) vel[d * ta.m + p] = v[d];
+  if (phis) {
+    T q[1];
+    gather<T, 1>(phi, vol, ta.n, s, wx, wy, wz, q);
+    phis[p] = q[0];
+  }
+}
+
+template <typename T>
+__global__ void k_cell_keys(const T *__restrict__ pos, int64_t *__restrict__ keys, TracerArgs ta) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= ta.m) return;
+  const T inv_dx = T(ta.n / ta.side_length);
+  const int msk = ta.n - 1;
+  const int i = ((int)floor(pos[p] * inv_dx)) & msk;
+  const int j = ((int)floor(pos[ta.m + p] * inv_dx)) & msk;
+  const int k = ((int)floor(pos[2 * ta.m + p] * inv_dx)) & msk;
+  keys[p] = ((int64_t)k * ta.n + j) * ta.n + i;
+}
+
+// Traversal order (src/Particles.jl:220-271): a counting sort of the tracers by 8^3-cell tile.
+// The order only reorders the visits, so every gather gives the same values with or without it.
+template <typename T>
+__device__ __forceinline__ int tile_of(const T *pos, int64_t m, int64_t p, int n, T inv_dx) {
+  const int msk = n - 1, nt = n >> 3;
+  const int i = ((int)floor(pos[p] * inv_dx)) & msk;
+  const int j = ((int)floor(pos[m + p] * inv_dx)) & msk;
+  const int k = ((int)floor(pos[2 * m + p] * inv_dx)) & msk;
+  return ((k >> 3) * nt + (j >> 3)) * nt + (i >> 3);
+}
+
+template <typename T>
+__global__ void k_tile_count(const T *__restrict__ pos, int *__restrict__ counts, TracerArgs ta) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= ta.m) return;
+  atomicAdd(&counts[tile_of(pos, ta.m, p, ta.n, T(ta.n / ta.side_length))], 1);
+}
+
+// Exclusive scan of the tile counts into the cursors, one block of 1024 threads.
+static __global__ void k_tile_scan(const int *__restrict__ counts, int *__restrict__ cursor, int ntiles) {
+  __shared__ int part[1024];
+  const int tid = threadIdx.x;
+  const int chunk = (ntiles + 1023) / 1024;
+  const int lo = tid * chunk, hi = min(ntiles, lo + chunk);
+  int s = 0;
+  for (int i = lo; i < hi; ++i) s += counts[i];
+  part[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = part[tid] - s;
+  for (int i = lo; i < hi; ++i) {
+    cursor[i] = run;
+    run += counts[i];
+  }
+}
+
+template <typename T>
+__global__ void k_tile_scatter(const T *__restrict__ pos, int *__restrict__ cursor, int *__restrict__ order,
+                               TracerArgs ta) {
+  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (p >= ta.m) return;
+  const int slot = atomicAdd(&cursor[tile_of(pos, ta.m, p, ta.n, T(ta.n / ta.side_length))], 1);
+  order[slot] = (int)p;
+}
+
+static __global__ void k_fill_i32(int *p, int v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+}  // namespace hdns

@@ -1,0 +1,150 @@
+/*
+ * hdns -- MI355X-native DNS time step of GBGHomogeneousTurbulence.jl behind a plain C ABI.
+ *
+ * One context owns all device memory of one run (one grid, one precision, one HIP stream).
+ * Host arrays cross the boundary only in the reference's own layouts:
+ *   spectral vector  Complex{T}[N/2+1, N, N, 3] column-major  == C order [3][N][N][N/2+1] complex,
+ *   spectral scalar  Complex{T}[N/2+1, N, N]                  == C order [N][N][N/2+1] complex,
+ *   physical vector  T[N, N, N, 3]                            == C order [3][N][N][N] real,
+ *   tracer positions T[M, 3] (a PointSet's matrix)            == C order [3][M] real,
+ * always as FP64 (complex interleaved re, im) with the unnormalised-forward convention of the
+ * reference (docs/src/validation.md:73-74).  A FP32-bulk context rounds to FP32 on upload.
+ *
+ * Every function returns an hdns_status; hdns_last_error(ctx) names the failure.
+ * Calls on one context are not re-entrant; all synchronisation happens inside calls that
+ * return host data.  Reference file:line citations are relative to the reference repository.
+ */
+#ifndef HDNS_H
+#define HDNS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  HDNS_OK = 0,
+  HDNS_ERR_INVALID_ARGUMENT = 1, /* the reference's ArgumentError class */
+  HDNS_ERR_STATE = 2,            /* the reference's ErrorException for schedule/epoch errors */
+  HDNS_ERR_NONFINITE = 3,        /* blow-up guard (src/SimulationRun.jl:1216-1224) */
+  HDNS_ERR_DEVICE = 4,           /* a HIP runtime failure */
+  HDNS_ERR_UNSUPPORTED = 5       /* a configuration this build does not implement */
+} hdns_status;
+
+enum { HDNS_FP32_BULK = 0, HDNS_FP64 = 1 };
+enum { HDNS_IF_LOW_STORAGE_RK3 = 3, HDNS_IF_LOW_STORAGE_RK4 = 4 };
+enum { HDNS_TWO_THIRDS = 0 };
+enum { HDNS_NO_FORCING = 0, HDNS_CONSTANT_POWER = 1 };
+enum { HDNS_CATMULL_ROM = 0 };
+enum { HDNS_HEUN = 0 };
+enum { HDNS_FIELD_VELOCITY = 0, HDNS_FIELD_SCALAR = 1, HDNS_FIELD_VORTICITY = 2 };
+
+/* Everything a run fixes at construction: FixedDomain (src/SimulationDomain.jl:295-328),
+ * FlowConditions (src/Generator.jl:81-147), ConstantPower (src/Forcing.jl:714-719,1069-1080),
+ * the scheme (src/Propagator.jl:264-320), and the tracer set (src/Particles.jl:135-176). */
+typedef struct {
+  int32_t device;            /* HIP device ordinal */
+  int32_t grid_points;       /* N (the box is N^3); a power of two, 16..1024 */
+  double side_length;        /* L */
+  int32_t precision;         /* HDNS_FP32_BULK or HDNS_FP64 */
+  int32_t scheme;            /* HDNS_IF_LOW_STORAGE_RK3 / _RK4 */
+  int32_t dealiasing;        /* HDNS_TWO_THIRDS */
+  int32_t has_scalar;        /* evolve the passive scalar */
+  double viscosity;          /* nu > 0 */
+  double schmidt;            /* Sc > 0 */
+  int32_t velocity_forcing;  /* HDNS_NO_FORCING or HDNS_CONSTANT_POWER */
+  double injection_rate;     /* ConstantPower epsilon */
+  int32_t lowest_shell;      /* >= 1 */
+  int32_t highest_shell;     /* >= lowest_shell; -1 means Inf (every mode but the mean) */
+  double mean_gradient;      /* G of an imposed mean scalar gradient along z (extension) */
+  int64_t tracer_capacity;   /* 0: no tracers */
+  int32_t interpolation;     /* HDNS_CATMULL_ROM */
+  int32_t particle_scheme;   /* HDNS_HEUN */
+  int32_t store_physical;    /* bit 0: keep physical u, bit 1: keep physical phi, bit 2: keep physical w */
+  int32_t reserved;
+} hdns_config;
+
+typedef struct hdns_ctx hdns_ctx;
+
+/* Build the frame, the registers and the device state.  Replaces the constructors
+ * FixedDomain (src/SimulationDomain.jl:295-328), allocate_buffers (src/Propagator.jl:709-739),
+ * ResolvedGenerator (src/Generator.jl:288-345) and ObservablesCache (src/Cache.jl:150-173)
+ * for the device-resident part of a Simulation (src/SimulationRun.jl:750-931). */
+hdns_status hdns_ctx_create(const hdns_config *config, hdns_ctx **out);
+hdns_status hdns_ctx_destroy(hdns_ctx *ctx);
+/* Message of the last failure on this context (or of the last failed create when ctx is NULL). */
+const char *hdns_last_error(const hdns_ctx *ctx);
+/* Device bytes the context holds. */
+hdns_status hdns_memory_bytes(const hdns_ctx *ctx, int64_t *bytes);
+
+/* Adopt a state (the door Simulation(domain; state = ResolvedFields(u_hat, phi_hat)),
+ * src/SimulationRun.jl:769-788).  The state must carry nothing outside the 2/3 ball
+ * (truncate_to_band!, src/SimulationDomain.jl:520-537); content there is an
+ * HDNS_ERR_INVALID_ARGUMENT.  phi_hat may be NULL for a context without a scalar.
+ * Ends with the field deposit of the adopted state (src/SimulationRun.jl:923). */
+hdns_status hdns_state_upload(hdns_ctx *ctx, const double *u_hat, const double *phi_hat);
+/* Read the state back in the reference layout (zeros outside the retained modes). */
+hdns_status hdns_state_download(hdns_ctx *ctx, double *u_hat, double *phi_hat);
+
+/* One step!(scheme, state, generator, domain, buffers, cache, dt) followed by the field
+ * deposit! (src/Propagator.jl:219,304-419,808-842): every stage, the state Leray and reality
+ * projections, and the recovery of u, w, phi and max|u| at the new state. */
+hdns_status hdns_step(hdns_ctx *ctx, double dt);
+/* The field deposit! of the current state on its own (src/Propagator.jl:808-842). */
+hdns_status hdns_deposit(hdns_ctx *ctx);
+/* max|u| of the last deposit (src/Observables.jl:304-305, src/Cache.jl:243). */
+hdns_status hdns_max_speed(hdns_ctx *ctx, double *out);
+/* out[0..4] = k, eps, <phi^2>, eps_phi, band energy of the last deposit's state, in FP64 from
+ * compensated device reductions (src/Observables.jl:84,155-158,345-400; src/Forcing.jl:1109). */
+hdns_status hdns_reductions(hdns_ctx *ctx, double *out);
+/* Shell spectrum E(kappa_n), n = 0..nshells-1 (src/Observables.jl:502-514); which = velocity or
+ * scalar.  nshells must be >= floor(sqrt(3)*N/2)+2. */
+hdns_status hdns_spectrum(hdns_ctx *ctx, int32_t which, double *out, int32_t nshells);
+/* A deposited physical field in the reference layout (velocity_physical / scalar_physical /
+ * vorticity_physical, src/Cache.jl:203-240); the context must keep it (store_physical). */
+hdns_status hdns_field_download(hdns_ctx *ctx, int32_t which, double *out);
+
+/* Tracers (src/Particles.jl:135-176): set M <= tracer_capacity positions, layout [3][M]. */
+hdns_status hdns_tracers_upload(hdns_ctx *ctx, int64_t m, const double *positions);
+hdns_status hdns_tracers_download(hdns_ctx *ctx, double *positions);
+/* rebuild_traversal! (src/Particles.jl:268-271): counting sort of the tracers by cell tile. */
+hdns_status hdns_rebuild_traversal(hdns_ctx *ctx);
+/* advect!(Heun, ...) (src/ParticlePropagator.jl:288-315,349-372): predictor, gather at X*,
+ * corrector, wrap.  Needs the deposited velocity one step ahead of the positions. */
+hdns_status hdns_advect(hdns_ctx *ctx, double dt);
+/* Particle deposit! (src/ParticlePropagator.jl:378-386): sample u (and phi) at the positions
+ * into the newest trail slot. */
+hdns_status hdns_deposit_tracers(hdns_ctx *ctx);
+/* The newest velocity sample [3][M] and, when the run carries a scalar, phi at the tracers [M]. */
+hdns_status hdns_tracer_samples(hdns_ctx *ctx, double *velocity, double *scalar);
+/* Canonical cell key (k*N + j)*N + i of every tracer, computed on the device. */
+hdns_status hdns_tracer_cells(hdns_ctx *ctx, int64_t *keys);
+/* The traversal permutation built by the last rebuild (0-based), for inspection. */
+hdns_status hdns_traversal_order(hdns_ctx *ctx, int32_t *order);
+
+/* The frame's integer tables as the device evaluates them: the signed wavenumber index of every
+ * bin of the three axes (lengths N/2+1, N, N) and the 2/3 mask over the reference layout
+ * [N][N][N/2+1] (src/SimulationDomain.jl:342-376). */
+hdns_status hdns_frame_tables(hdns_ctx *ctx, int32_t *mx, int32_t *my, int32_t *mz, uint8_t *mask);
+
+/* The low-storage coefficients a, b, d (d: the substep increments) of a scheme rounded to the
+ * precision's element type, exactly as a step uses them (src/Propagator.jl:264-274, :304-320).
+ * Arrays hold at least 5 entries; *stages receives 3 or 5.  Needs no device. */
+hdns_status hdns_scheme_coefficients(int32_t scheme, int32_t precision, double *a, double *b, double *d,
+                                     int32_t *stages);
+/* The Heun push weights for a step h: out[0] = b1 (predictor), out[1] = beta1 (on the predicted
+ * sample), out[2] = beta2 - b1 (correction on the previous sample), from the Adams weights by
+ * 4-point Gauss-Legendre (src/ParticlePropagator.jl:261-315).  Needs no device. */
+hdns_status hdns_heun_weights(double h, double *out3);
+
+/* Synchronise the context's stream (for timing from the host). */
+hdns_status hdns_synchronize(hdns_ctx *ctx);
+/* The HIP stream the context launches on (a hipStream_t), for event timing on that stream. */
+hdns_status hdns_stream(hdns_ctx *ctx, void **stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HDNS_H */
