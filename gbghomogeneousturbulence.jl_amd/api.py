@@ -1,0 +1,471 @@
+"""The reference's run API over the HIP step: FixedDomain, FlowConditions, the forcings, the
+schemes, ResolvedFields, Tracers, Heun and Simulation with advance/run.
+
+Names, argument meanings and error classes follow the reference (an ``ArgumentError`` there is a
+``ValueError`` here, an ``ErrorException`` a ``RuntimeError``).  Everything that moves a field runs
+in libhdns.so on the GPU; this module keeps the host-side bookkeeping the reference keeps on the
+host: the clock, the cache epochs, the CFL arithmetic and the blow-up guards.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from ctypes import c_double, c_int32, c_int64, c_uint8, c_void_p
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+
+# --- frame ---------------------------------------------------------------------------------
+
+class TwoThirds:
+    """The sharp 2/3 rule (src/SimulationDomain.jl:100-140): keep the closed ball |k| <= (2/3) k_Ny."""
+
+    def __repr__(self) -> str:
+        return "TwoThirds()"
+
+
+class FixedDomain:
+    """The periodic box a run lives on (src/SimulationDomain.jl:199-328).
+
+    Every argument is a keyword, as in the reference; ``element_type`` is ``np.float32``
+    (FP32-bulk: FP32 transforms and pointwise arithmetic, FP64 exponentials, twiddles and
+    reductions) or ``np.float64`` (plain FP64).  ``device`` is the HIP ordinal.
+    """
+
+    def __init__(self, *, grid_points_per_axis: int, side_length: float, dealiasing, element_type,
+                 device: int = 0):
+        n = int(grid_points_per_axis)
+        if n < 16 or n > 1024 or n & (n - 1):
+            raise NotImplementedError(f"grid_points_per_axis must be a power of two in 16..1024, got {n}")
+        if not (side_length > 0 and math.isfinite(side_length)):
+            raise ValueError("side_length must be positive and finite")
+        if not isinstance(dealiasing, TwoThirds):
+            raise NotImplementedError("only TwoThirds dealiasing is built")
+        if element_type not in (np.float32, np.float64):
+            raise ValueError("element_type must be np.float32 or np.float64")
+        self.grid_points_per_axis = n
+        self.side_length = float(side_length)
+        self.dealiasing = dealiasing
+        self.element_type = element_type
+        self.device = int(device)
+
+    @property
+    def precision(self) -> int:
+        return L.FP32_BULK if self.element_type is np.float32 else L.FP64
+
+    @property
+    def grid_spacing(self) -> float:
+        return self.side_length / self.grid_points_per_axis
+
+    def max_retained_wavenumber(self) -> float:
+        """(2/3)(N/2)(2 pi/L) (src/SimulationDomain.jl:181-195)."""
+        return (2.0 / 3.0) * (self.grid_points_per_axis / 2) * (2 * math.pi / self.side_length)
+
+    def __repr__(self) -> str:
+        n = self.grid_points_per_axis
+        return f"FixedDomain({n}x{n}x{n}, L = {self.side_length}, {self.dealiasing}, {self.element_type.__name__})"
+
+
+def truncate_to_band(f: np.ndarray, domain: FixedDomain) -> np.ndarray:
+    """Zero every coefficient outside the 2/3 ball, in place (src/SimulationDomain.jl:520-537)."""
+    n = domain.grid_points_per_axis
+    i = np.arange(n)
+    w = np.where(i <= n // 2, i, i - n)
+    m2 = np.arange(n // 2 + 1)[None, None, :] ** 2 + w[None, :, None] ** 2 + w[:, None, None] ** 2
+    f[..., 9 * m2 > n * n] = 0
+    return f
+
+
+# --- physics -------------------------------------------------------------------------------
+
+class NoForcing:
+    """The unforced case (src/Forcing.jl:456-464)."""
+
+    def __repr__(self) -> str:
+        return "NoForcing()"
+
+
+class ConstantPower:
+    """Hold the injection rate on shells lowest..highest (src/Forcing.jl:714-719,1069-1117).
+
+    A = eps / (2 k_band) is re-measured at every stage's own field on the device, with the
+    relative empty-band clamp (src/Forcing.jl:110-111).
+    """
+
+    def __init__(self, domain: FixedDomain, *, injection_rate: float, lowest_shell: int = 1, highest_shell=2):
+        # src/Forcing.jl:388-409
+        if lowest_shell < 1:
+            raise ValueError(f"`lowest_shell` must be at least 1, got {lowest_shell}: shell 0 is the mean "
+                             "(kappa = 0) mode, which a band forcing cannot legitimately drive")
+        if not (highest_shell == math.inf or (math.isfinite(highest_shell) and float(highest_shell).is_integer())):
+            raise ValueError(f"`highest_shell` must be an integer shell or `Inf`, got {highest_shell}")
+        if highest_shell < lowest_shell:
+            raise ValueError(f"`highest_shell` must be at least `lowest_shell`, got {highest_shell} < {lowest_shell}")
+        self.injection_rate = float(domain.element_type(injection_rate))
+        self.lowest_shell = int(lowest_shell)
+        self.highest_shell = math.inf if highest_shell == math.inf else int(highest_shell)
+
+    def __repr__(self) -> str:
+        return f"ConstantPower(injection_rate {self.injection_rate}, shells {self.lowest_shell}-{self.highest_shell})"
+
+
+class FlowConditions:
+    """nu, Sc and the two forcings (src/Generator.jl:81-147), with the reference's guards."""
+
+    def __init__(self, *, viscosity: float, schmidt: float, velocity_forcing=None, scalar_forcing=None):
+        if not viscosity > 0:
+            raise ValueError(f"`viscosity` must be positive, got {viscosity}: it sets the Reynolds number "
+                             "through Re = 1/nu, and a non-positive value has no dissipation scale")
+        if not schmidt > 0:
+            raise ValueError(f"`schmidt` must be positive, got {schmidt}: the scalar diffusivity is nu/Sc, "
+                             "so a non-positive Schmidt number makes it infinite or negative")
+        self.viscosity = float(viscosity)
+        self.schmidt = float(schmidt)
+        self.velocity_forcing = velocity_forcing if velocity_forcing is not None else NoForcing()
+        self.scalar_forcing = scalar_forcing if scalar_forcing is not None else NoForcing()
+        if not isinstance(self.scalar_forcing, NoForcing):
+            raise NotImplementedError("scalar forcings other than NoForcing are not built")
+        if not isinstance(self.velocity_forcing, (NoForcing, ConstantPower)):
+            raise NotImplementedError("velocity forcings other than NoForcing and ConstantPower are not built")
+
+    @property
+    def scalar_diffusivity(self) -> float:
+        return self.viscosity / self.schmidt
+
+
+def predicted_viscosity(domain: FixedDomain, dissipation: float, kmax_eta: float = 1.5) -> float:
+    """nu = eps^{1/3} eta^{4/3} with eta = kmax_eta / kappa_max (src/SimulationRun.jl:647-655)."""
+    eta = kmax_eta / domain.max_retained_wavenumber()
+    return dissipation ** (1.0 / 3.0) * eta ** (4.0 / 3.0)
+
+
+# --- schemes -------------------------------------------------------------------------------
+
+class IFLowStorageRK3:
+    """Williamson 2N third-order IF scheme (src/Propagator.jl:220-274)."""
+    code = L.RK3
+    base_cfl = 0.5
+
+
+class IFLowStorageRK4:
+    """Carpenter-Kennedy 5-stage 2N fourth-order IF scheme (src/Propagator.jl:277-320)."""
+    code = L.RK4
+    base_cfl = 0.96
+
+
+def recommended_cfl(scheme, domain: FixedDomain) -> float:
+    """The advective CFL ceiling (src/Propagator.jl:906-919); TwoThirds leaves the base unscaled."""
+    return scheme.base_cfl
+
+
+class Heun:
+    """ABM(1), the default tracer scheme (src/ParticlePropagator.jl:165)."""
+
+    def __repr__(self) -> str:
+        return "Heun()"
+
+
+# --- state and tracers ---------------------------------------------------------------------
+
+@dataclass
+class ResolvedFields:
+    """(u_hat, phi_hat) in the reference layout: complex [3, N, N, N/2+1] and [N, N, N/2+1] (src/State.jl:119-137)."""
+    velocity_spectral: np.ndarray
+    scalar_spectral: np.ndarray | None = None
+
+
+class Tracers:
+    """A tracer set (src/Particles.jl:135-176): positions [3, M] in [0, L)."""
+
+    def __init__(self, domain: FixedDomain, *, positions: np.ndarray | None = None,
+                 number_of_particles: int | None = None, seed: int = 2028):
+        if (positions is None) == (number_of_particles is None):
+            raise ValueError("give exactly one of `positions` (a [3, M] array) or `number_of_particles`")
+        if positions is None:
+            from .seeding import uniform_positions
+            positions = uniform_positions(int(number_of_particles), domain.side_length, seed, domain.element_type)
+        positions = np.ascontiguousarray(positions, dtype=np.float64)
+        if positions.ndim != 2 or positions.shape[0] != 3:
+            raise ValueError("positions must have shape [3, M]")
+        self.positions = positions
+        self.interpolation_scheme = "CatmullRom"
+
+    @property
+    def count(self) -> int:
+        return self.positions.shape[1]
+
+
+def _check_reality(f: np.ndarray, name: str, dtype) -> None:
+    """src/SimulationRun.jl:1236-1251: the kx = 0 plane must be Hermitian to sqrt(eps(T)) of scale."""
+    n = f.shape[-2]
+    neg = (-np.arange(n)) % n
+    s = f[..., 0]
+    res = float(np.max(np.abs(s - np.conj(s[..., neg, :][..., :, neg])))) if s.size else 0.0
+    scale = float(np.max(np.abs(f))) if f.size else 0.0
+    if res > math.sqrt(np.finfo(dtype).eps) * scale:
+        raise ValueError(f"Simulation: the given `state`'s {name} coefficients are not the spectrum of a real "
+                         f"field -- the conjugate-symmetry residual is {res} against a field scale of {scale}. "
+                         "Repair with `reality_projection!` or fix the conversion that produced them.")
+
+
+# --- the run -------------------------------------------------------------------------------
+
+class Simulation:
+    """A run: the device context (generator, registers, cache), the tracers and the clock.
+
+    ``Simulation(domain; state = ResolvedFields(u_hat, phi_hat))`` (src/SimulationRun.jl:750-931):
+    the state is adopted and deposited, and the tracer trail gets its first sample at t = 0.
+    ``mean_gradient`` is G of an imposed mean scalar gradient along z (an extension; the reference
+    defers it, docs/ARCHITECTURE.md:769-771).
+    """
+
+    def __init__(self, domain: FixedDomain, *, conditions: FlowConditions, state: ResolvedFields,
+                 field_integration_scheme=IFLowStorageRK4, tracers: Tracers | None = None,
+                 particle_integration_scheme=None, mean_gradient: float = 0.0, cfl: float | None = None,
+                 max_dt: float = math.inf, keep_vorticity: bool = False, tracer_capacity: int | None = None):
+        n = domain.grid_points_per_axis
+        carries_scalar = state.scalar_spectral is not None
+        u = np.ascontiguousarray(state.velocity_spectral, dtype=np.complex128)
+        if u.shape != (3, n, n, n // 2 + 1):
+            raise ValueError(f"velocity_spectral must have shape {(3, n, n, n // 2 + 1)}, got {u.shape}")
+        _check_reality(u, "velocity", domain.element_type)
+        p = None
+        if carries_scalar:
+            p = np.ascontiguousarray(state.scalar_spectral, dtype=np.complex128)
+            if p.shape != (n, n, n // 2 + 1):
+                raise ValueError(f"scalar_spectral must have shape {(n, n, n // 2 + 1)}, got {p.shape}")
+            _check_reality(p, "scalar", domain.element_type)
+        if mean_gradient and not carries_scalar:
+            raise ValueError("a mean scalar gradient needs a scalar in the state")
+        self.domain = domain
+        self.conditions = conditions
+        self.field_integration_scheme = field_integration_scheme
+        self.particle_integration_scheme = particle_integration_scheme or (Heun() if tracers is not None else None)
+        self.tracers = tracers
+        self.cfl = recommended_cfl(field_integration_scheme, domain) if cfl is None else float(cfl)
+        self.max_dt = float(max_dt)
+        self.time = 0.0
+        self.step = 0
+        self.carries_scalar = carries_scalar
+        cfg = L.HdnsConfig()
+        cfg.device = domain.device
+        cfg.grid_points = n
+        cfg.side_length = domain.side_length
+        cfg.precision = domain.precision
+        cfg.scheme = field_integration_scheme.code
+        cfg.dealiasing = 0
+        cfg.has_scalar = int(carries_scalar)
+        cfg.viscosity = conditions.viscosity
+        cfg.schmidt = conditions.schmidt
+        f = conditions.velocity_forcing
+        if isinstance(f, ConstantPower):
+            cfg.velocity_forcing = L.CONSTANT_POWER
+            cfg.injection_rate = f.injection_rate
+            cfg.lowest_shell = f.lowest_shell
+            cfg.highest_shell = -1 if f.highest_shell == math.inf else f.highest_shell
+        cfg.mean_gradient = float(mean_gradient)
+        cap = tracers.count if tracers is not None else 0
+        cfg.tracer_capacity = int(tracer_capacity if tracer_capacity is not None else cap)
+        cfg.store_physical = 1 | (2 if carries_scalar else 0) | (4 if keep_vorticity else 0)
+        h = c_void_p()
+        L.check(L.lib().hdns_ctx_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._ctx = h
+        self._cfg = cfg
+        self.fields_epoch = 0
+        self.particles_epoch = 0
+        self.trail_epoch = None
+        self.trail_time = None
+        try:
+            self._call("hdns_state_upload", L.dptr(u), L.dptr(p) if p is not None else None)
+            if tracers is not None:
+                self._call("hdns_tracers_upload", tracers.count, L.dptr(tracers.positions))
+                self._call("hdns_rebuild_traversal")
+                self._call("hdns_deposit_tracers")
+                self.trail_epoch = (self.fields_epoch, self.particles_epoch)
+                self.trail_time = 0.0
+        except Exception:
+            self.close()
+            raise
+
+    # plumbing
+    def _call(self, name: str, *args) -> None:
+        if self._ctx is None:
+            raise RuntimeError("this Simulation has been closed")
+        L.check(getattr(L.lib(), name)(self._ctx, *args), self._ctx)
+
+    def close(self) -> None:
+        if getattr(self, "_ctx", None) is not None:
+            L.lib().hdns_ctx_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):  # pragma: no cover - interpreter teardown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # observables of the last deposit
+    def max_speed(self) -> float:
+        """max|u| of the cached deposit (src/Cache.jl:243)."""
+        out = c_double()
+        self._call("hdns_max_speed", ctypes.byref(out))
+        return out.value
+
+    def reductions(self) -> dict:
+        """k, eps, <phi^2>, eps_phi and the forced-band energy (FP64, compensated on the device)."""
+        out = np.zeros(5)
+        self._call("hdns_reductions", L.dptr(out))
+        keys = ("kinetic_energy", "dissipation", "scalar_variance", "scalar_dissipation", "band_energy")
+        return dict(zip(keys, map(float, out)))
+
+    def energy_spectrum(self, which: int = L.FIELD_VELOCITY):
+        """(shell centres, E(kappa)) of the state (src/Observables.jl:502-514)."""
+        n = self.domain.grid_points_per_axis
+        nb = int(math.floor(math.sqrt(3.0) * n / 2)) + 2
+        out = np.zeros(nb)
+        self._call("hdns_spectrum", which, L.dptr(out), nb)
+        return np.arange(nb) * (2 * math.pi / self.domain.side_length), out
+
+    def state(self) -> ResolvedFields:
+        """The device state, copied back in the reference layout."""
+        n = self.domain.grid_points_per_axis
+        u = np.zeros((3, n, n, n // 2 + 1), dtype=np.complex128)
+        p = np.zeros((n, n, n // 2 + 1), dtype=np.complex128) if self.carries_scalar else None
+        self._call("hdns_state_download", L.dptr(u), L.dptr(p) if p is not None else None)
+        return ResolvedFields(u, p)
+
+    def physical(self, which: int = L.FIELD_VELOCITY) -> np.ndarray:
+        """A deposited physical field: velocity [3, N, N, N], scalar [N, N, N] or vorticity [3, N, N, N]."""
+        n = self.domain.grid_points_per_axis
+        shape = (n, n, n) if which == L.FIELD_SCALAR else (3, n, n, n)
+        out = np.zeros(shape)
+        self._call("hdns_field_download", which, L.dptr(out))
+        return out
+
+    def tracer_positions(self) -> np.ndarray:
+        out = np.zeros((3, self.tracers.count))
+        self._call("hdns_tracers_download", L.dptr(out))
+        return out
+
+    def tracer_samples(self):
+        """(velocity [3, M], scalar [M] or None) of the newest trail sample."""
+        m = self.tracers.count
+        v = np.zeros((3, m))
+        s = np.zeros(m) if self.carries_scalar else None
+        self._call("hdns_tracer_samples", L.dptr(v), L.dptr(s) if s is not None else None)
+        return v, s
+
+    def tracer_cells(self) -> np.ndarray:
+        keys = np.zeros(self.tracers.count, dtype=np.int64)
+        self._call("hdns_tracer_cells", keys.ctypes.data_as(ctypes.POINTER(c_int64)))
+        return keys
+
+    def traversal_order(self) -> np.ndarray:
+        order = np.zeros(self.tracers.count, dtype=np.int32)
+        self._call("hdns_traversal_order", order.ctypes.data_as(ctypes.POINTER(c_int32)))
+        return order
+
+    def frame_tables(self):
+        n = self.domain.grid_points_per_axis
+        mx = np.zeros(n // 2 + 1, dtype=np.int32)
+        my = np.zeros(n, dtype=np.int32)
+        mz = np.zeros(n, dtype=np.int32)
+        mask = np.zeros((n, n, n // 2 + 1), dtype=np.uint8)
+        P = ctypes.POINTER
+        self._call("hdns_frame_tables", mx.ctypes.data_as(P(c_int32)), my.ctypes.data_as(P(c_int32)),
+                   mz.ctypes.data_as(P(c_int32)), mask.ctypes.data_as(P(c_uint8)))
+        return mx, my, mz, mask
+
+    def memory_bytes(self) -> int:
+        b = c_int64()
+        self._call("hdns_memory_bytes", ctypes.byref(b))
+        return b.value
+
+    def synchronize(self) -> None:
+        self._call("hdns_synchronize")
+
+    # the hot path
+    def step_fields(self, dt: float) -> None:
+        """``step!`` then the field ``deposit!`` (src/Propagator.jl:304-419,808-842), asynchronously."""
+        self._call("hdns_step", float(dt))
+        self.fields_epoch += 1
+
+    def cfl_timestep(self) -> float:
+        """The advective/resolution/max_dt bound (src/SimulationRun.jl:1051-1096)."""
+        dx = self.domain.grid_spacing
+        speed = self.max_speed()
+        if not math.isfinite(speed):
+            raise RuntimeError(f"non-finite velocity at step {self.step}: the run appears to have blown up")
+        advective = self.cfl * dx / speed if speed > 0 else self.cfl * dx
+        c = self.conditions
+        diffusivity = max(c.viscosity, c.scalar_diffusivity) if self.carries_scalar else c.viscosity
+        kmin = 2 * math.pi / self.domain.side_length
+        resolution = 1 / (diffusivity * kmin ** 2)
+        return min(advective, resolution, self.max_dt)
+
+    def advect(self, dt: float) -> None:
+        """advect!(Heun, ...) (src/ParticlePropagator.jl:349-372) with its epoch check."""
+        if self.tracers is None:
+            raise ValueError("this run carries no tracers")
+        if self.trail_epoch != (self.fields_epoch - 1, self.particles_epoch):
+            raise RuntimeError("the fields must be exactly one step ahead of the tracer positions, and the "
+                               "trail's newest sample taken at those positions")
+        self._call("hdns_advect", float(dt))
+        self.particles_epoch += 1
+
+    def deposit_tracers(self, t: float) -> None:
+        """The particle deposit! (src/ParticlePropagator.jl:378-386)."""
+        if self.tracers is None:
+            raise ValueError("this run carries no tracers")
+        if not math.isfinite(t) or (self.trail_time is not None and t <= self.trail_time):
+            raise ValueError(f"the sample time must be finite and later than the previous sample's, got {t}")
+        self._call("hdns_deposit_tracers")
+        self.trail_epoch = (self.fields_epoch, self.particles_epoch)
+        self.trail_time = t
+
+    def advance(self, dt: float | None = None, check: bool = True) -> "Simulation":
+        """One iteration (src/SimulationRun.jl:1155-1229): dt, step + deposit, traversal rebuild,
+        tracer push and sample, then the blow-up guards and the clock."""
+        dt_ = self.cfl_timestep() if dt is None else float(dt)
+        if not (math.isfinite(dt_) and dt_ > 0):
+            raise ValueError(f"the timestep must be positive and finite, got {dt_}")
+        t_new = self.time + dt_
+        if not t_new > self.time:
+            raise ValueError(f"the timestep {dt_} is too small to advance the clock, which reads {self.time}")
+        self.step_fields(dt_)
+        if self.tracers is not None:
+            self._call("hdns_rebuild_traversal")
+            self.advect(dt_)
+            self.deposit_tracers(t_new)
+        if check:
+            red = None
+            if not math.isfinite(self.max_speed()):
+                raise L.BlowUpError(f"non-finite velocity at step {self.step + 1}: the run appears to have blown up")
+            if self.carries_scalar:
+                red = self.reductions()
+                if not math.isfinite(red["scalar_variance"]):
+                    raise L.BlowUpError(f"non-finite scalar at step {self.step + 1}: the run appears to have blown up")
+        self.time = t_new
+        self.step += 1
+        return self
+
+    def run(self, callback=None, *, steps: int, dt: float | None = None) -> "Simulation":
+        """run!(callback, sim; dt) for a fixed number of iterations (src/SimulationRun.jl:1395-1405)."""
+        for _ in range(int(steps)):
+            if callback is not None and callback(self) is False:
+                break
+            self.advance(dt)
+        return self
+
+    def __repr__(self) -> str:
+        n = self.domain.grid_points_per_axis
+        return f"Simulation({n}x{n}x{n} grid, time {self.time}, step {self.step})"

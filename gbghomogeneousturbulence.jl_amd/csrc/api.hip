@@ -144,6 +144,8 @@ hdns_status alloc(hdns_ctx *ctx, void **p, size_t bytes) {
 }
 
 void release(Ctx &c) {
+  for (hipEvent_t e : c.prof_ev) (void)hipEventDestroy(e);
+  c.prof_ev.clear();
   void *ptrs[] = {c.uh, c.hu, c.ph, c.hp, c.bufa, c.bufb, c.phys, c.physw, c.tw, c.partials, c.scalars,
                   c.maxbits, c.spec_bins, c.pos, c.vel, c.phis, c.order, c.tile_counts, c.tile_cursor};
   for (void *p : ptrs)
@@ -157,6 +159,44 @@ void zero_f64(Ctx &c, void *p, size_t bytes) {
   const size_t n = bytes / sizeof(double);
   hipLaunchKernelGGL(k_fill_f64, dim3(1024), dim3(256), 0, c.stream, (double *)p, 0.0, n);
 }
+
+// Per-kernel timing: an event before and after each launch on the context's stream, resolved
+// in batches (the pool synchronises only when it is full, or on a read).
+enum { K_YFWD, K_ZF, K_REALITY, K_ZI, K_FINALIZE, K_YINV, K_XPASS, K_OTHER };
+constexpr int PROF_POOL = 2048;
+
+void prof_resolve(Ctx &c) {
+  if (!c.prof || c.prof_used == 0) return;
+  (void)hipStreamSynchronize(c.stream);
+  for (int i = 0; i < c.prof_used; i += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c.prof_ev[i], c.prof_ev[i + 1]) == hipSuccess) {
+      c.prof_ms[c.prof_kind[i / 2]] += ms;
+      c.prof_cnt[c.prof_kind[i / 2]] += 1;
+    }
+  }
+  c.prof_used = 0;
+}
+
+void prof_begin(Ctx &c, int kind) {
+  if (!c.prof) return;
+  if (c.prof_used + 2 > (int)c.prof_ev.size()) prof_resolve(c);
+  c.prof_kind[c.prof_used / 2] = kind;
+  (void)hipEventRecord(c.prof_ev[c.prof_used], c.stream);
+}
+
+void prof_end(Ctx &c) {
+  if (!c.prof) return;
+  (void)hipEventRecord(c.prof_ev[c.prof_used + 1], c.stream);
+  c.prof_used += 2;
+}
+
+#define TIMED(c, kind, call) \
+  do {                       \
+    prof_begin(c, kind);     \
+    call;                    \
+    prof_end(c);             \
+  } while (0)
 
 // The inverse half of the pipeline for the current state: the field deposit!
 // (src/Propagator.jl:808-842): u, w, phi in physical space, their products for the next
@@ -521,14 +561,15 @@ hdns_status hdns_step(hdns_ctx *ctx, double dt) {
     sa.mean_gradient = c.cfg.mean_gradient;
     sa.band = c.band;
     sa.scalars = c.scalars;
-    c.pipe->yfwd(c);
-    c.pipe->zf(c, sa);
-    if (last) c.pipe->reality(c);
-    c.pipe->zi(c);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c.stream, (const double *)c.partials, c.scalars, fa);
-    c.pipe->yinv(c);
+    TIMED(c, K_YFWD, c.pipe->yfwd(c));
+    TIMED(c, K_ZF, c.pipe->zf(c, sa));
+    if (last) TIMED(c, K_REALITY, c.pipe->reality(c));
+    TIMED(c, K_ZI, c.pipe->zi(c));
+    TIMED(c, K_FINALIZE,
+          hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c.stream, (const double *)c.partials, c.scalars, fa));
+    TIMED(c, K_YINV, c.pipe->yinv(c));
     if (last) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c.stream, c.maxbits, 0ull, 1);
-    c.pipe->xpass(c, last ? fin : mid);
+    TIMED(c, K_XPASS, c.pipe->xpass(c, last ? fin : mid));
   }
   c.deposited = true;
   return launched(ctx);
@@ -760,6 +801,41 @@ hdns_status hdns_frame_tables(hdns_ctx *ctx, int32_t *mx, int32_t *my, int32_t *
   (void)hipFree(d);
   if (dm) (void)hipFree(dm);
   if (e != hipSuccess) return fail(ctx, HDNS_ERR_DEVICE, std::string("frame tables: ") + hipGetErrorString(e));
+  return HDNS_OK;
+}
+
+hdns_status hdns_profile_enable(hdns_ctx *ctx, int32_t on) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  Ctx &c = ctx->c;
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  if (on && !c.prof) {
+    c.prof_ev.resize(PROF_POOL);
+    c.prof_kind.assign(PROF_POOL / 2, K_OTHER);
+    for (auto &e : c.prof_ev) HIP_TRY(ctx, hipEventCreate(&e));
+    c.prof_used = 0;
+    c.prof = true;
+  } else if (!on && c.prof) {
+    prof_resolve(c);
+    for (hipEvent_t e : c.prof_ev) (void)hipEventDestroy(e);
+    c.prof_ev.clear();
+    c.prof = false;
+  }
+  return HDNS_OK;
+}
+
+hdns_status hdns_profile_read(hdns_ctx *ctx, double *ms, int64_t *launches, int32_t reset) {
+  if (!ctx || !ms || !launches) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "outputs must not be NULL");
+  Ctx &c = ctx->c;
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  prof_resolve(c);
+  for (int i = 0; i < Ctx::NKIND; ++i) {
+    ms[i] = c.prof_ms[i];
+    launches[i] = c.prof_cnt[i];
+    if (reset) {
+      c.prof_ms[i] = 0;
+      c.prof_cnt[i] = 0;
+    }
+  }
   return HDNS_OK;
 }
 

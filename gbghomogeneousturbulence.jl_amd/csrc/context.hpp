@@ -4,6 +4,7 @@
 
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/hdns.h"
 #include "spectral.hpp"
@@ -59,6 +60,14 @@ struct Ctx {
   bool order_valid = false;
   bool deposited = false;
   size_t bytes = 0;
+  // optional per-kernel timing with events on the context's stream
+  static constexpr int NKIND = 8;
+  bool prof = false;
+  std::vector<hipEvent_t> prof_ev;
+  std::vector<int> prof_kind;
+  int prof_used = 0;
+  double prof_ms[NKIND] = {0};
+  long long prof_cnt[NKIND] = {0};
   std::unique_ptr<Pipeline> pipe;
   std::string err;
 };

@@ -22,7 +22,7 @@ struct PipelineImpl final : Pipeline {
   static constexpr size_t LDS_X = (size_t)4 * N * sizeof(cx<T>);
 
   static void set_lds(const void *fn, size_t bytes) {
-    if (bytes > 65536) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   }
 
   PipelineImpl() {
@@ -32,6 +32,8 @@ struct PipelineImpl final : Pipeline {
     set_lds((const void *)k_zf<T, N, TX, NTZ, false>, LDS_COL);
     set_lds((const void *)k_zi<T, N, TX, NTZ, true>, LDS_COL);
     set_lds((const void *)k_zi<T, N, TX, NTZ, false>, LDS_COL);
+    set_lds((const void *)k_xpass<T, N, NTX, 7>, LDS_X);
+    set_lds((const void *)k_xpass<T, N, NTX, 4>, LDS_X);
   }
 
   int tiles(const Ctx &c) const { return c.geo.kxp / TX; }

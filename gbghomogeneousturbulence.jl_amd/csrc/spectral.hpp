@@ -356,8 +356,8 @@ static __global__ void k_finalize(const double *__restrict__ partials, double *_
   }
 }
 
-// Shell-binned spectrum of the compact state (src/Observables.jl:502-514): shell n of a mode is
-// the integer with (2n-1)^2 <= 4|m|^2 < (2n+1)^2.  Atomic FP64 accumulation: a diagnostic off the
+// Shell-binned spectrum of the compact state (src/Observables.jl:502-514): shell n >= 0 of a mode
+// is the integer with n - 1/2 <= |m| < n + 1/2.  Atomic FP64 accumulation: a diagnostic off the
 // step, reproducible to the last bit only up to summation order.
 template <typename T>
 __global__ void k_spectrum(const cx<T> *__restrict__ f, int ncomp, Geo g, double *__restrict__ bins) {
@@ -371,7 +371,7 @@ __global__ void k_spectrum(const cx<T> *__restrict__ f, int ncomp, Geo g, double
     const int my = signed_of(ky_idx, g), mz = signed_of(kz_idx, g);
     const int four = 4 * (mx * mx + my * my + mz * mz);
     int n = (int)floor(sqrt((double)four) * 0.5 + 0.5);
-    while ((2 * n - 1) * (2 * n - 1) > four) --n;
+    while (n > 0 && (2 * n - 1) * (2 * n - 1) > four) --n;
     while ((2 * n + 1) * (2 * n + 1) <= four) ++n;
     double e2 = 0.0;
     for (int c = 0; c < ncomp; ++c) {

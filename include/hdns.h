@@ -138,6 +138,13 @@ hdns_status hdns_scheme_coefficients(int32_t scheme, int32_t precision, double *
  * 4-point Gauss-Legendre (src/ParticlePropagator.jl:261-315).  Needs no device. */
 hdns_status hdns_heun_weights(double h, double *out3);
 
+/* Per-kernel timing with HIP events on the context's stream, for the roofline report.  Kinds:
+ * 0 y forward, 1 z forward + stage update, 2 reality projection, 3 z inverse + reductions,
+ * 4 reduction finalize, 5 y inverse, 6 x pass (products), 7 other.  read: ms[8] accumulated
+ * milliseconds and launches[8] counts since the last reset (synchronises the stream). */
+hdns_status hdns_profile_enable(hdns_ctx *ctx, int32_t on);
+hdns_status hdns_profile_read(hdns_ctx *ctx, double *ms, int64_t *launches, int32_t reset);
+
 /* Synchronise the context's stream (for timing from the host). */
 hdns_status hdns_synchronize(hdns_ctx *ctx);
 /* The HIP stream the context launches on (a hipStream_t), for event timing on that stream. */

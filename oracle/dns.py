@@ -380,11 +380,11 @@ def reductions(d: Domain, cond: Conditions, uh: np.ndarray, ph: np.ndarray | Non
 
 
 def shell_index(d: Domain) -> np.ndarray:
-    """Canonical shell of each mode: (2n-1)^2 <= 4|m|^2 < (2n+1)^2."""
+    """Canonical shell of each mode: n - 1/2 <= |m| < n + 1/2, in exact integers (n >= 0)."""
     four = 4 * d.m2
     n = np.floor(np.sqrt(four.astype(np.float64)) / 2 + 0.5).astype(np.int64)
     # exact integer correction of the float estimate
-    n = np.where((2 * n - 1) ** 2 > four, n - 1, n)
+    n = np.where((n > 0) & ((2 * n - 1) ** 2 > four), n - 1, n)
     n = np.where((2 * n + 1) ** 2 <= four, n + 1, n)
     return n
 

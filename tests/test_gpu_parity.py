@@ -1,0 +1,183 @@
+"""Parity of the HIP step with the FP64 oracle, through the C ABI, on an MI355X.
+
+Bars: integer tables and cell keys bit-exact; FP64 fields to 1e-12 relative; FP32-bulk fields to
+the FP32 transform floor (about 1e-6 relative per step), and the velocity within 1e-5 relative
+of the FP64 oracle after 10 IF-RK4 steps.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import dns
+from oracle import tracers as trc
+
+pytestmark = pytest.mark.gpu
+L = 2 * math.pi
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def make_sim(g, n, dtype, *, scheme=None, forcing=None, G=1.0, nu=0.02, seed=7, tracers=None,
+             keep_vorticity=False, scalar=True):
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
+    u = g.random_phase_velocity(n, seed=seed)
+    p = g.random_phase_scalar(n, seed=seed + 1) if scalar else None
+    vf = g.ConstantPower(dom, injection_rate=forcing, lowest_shell=1, highest_shell=2) if forcing else g.NoForcing()
+    cond = g.FlowConditions(viscosity=nu, schmidt=1.0, velocity_forcing=vf)
+    sim = g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=G if scalar else 0.0,
+                       field_integration_scheme=scheme or g.IFLowStorageRK4, tracers=tracers,
+                       keep_vorticity=keep_vorticity)
+    ocond = dns.Conditions(viscosity=nu, schmidt=1.0, forcing=(float(dtype(forcing)), 1, 2) if forcing else None,
+                           mean_gradient=G if scalar else 0.0, band_eps=float(np.finfo(dtype).eps))
+    return sim, u, p, ocond
+
+
+@pytest.mark.parametrize("n", [16, 64])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_frame_tables_bit_exact(g, n, dtype):
+    sim, *_ = make_sim(g, n, dtype)
+    with sim:
+        mx, my, mz, mask = sim.frame_tables()
+    d = dns.Domain(n)
+    assert np.array_equal(mx, d.mx) and np.array_equal(my, d.my) and np.array_equal(mz, d.mz)
+    assert np.array_equal(mask.astype(bool), d.mask)
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 3e-6)])
+def test_deposit_matches_the_oracle(g, dtype, tol):
+    n = 32
+    sim, u, p, oc = make_sim(g, n, dtype, keep_vorticity=True, forcing=0.1)
+    d = dns.Domain(n)
+    with sim:
+        dep = dns.deposit(d, u, p)
+        assert rel(sim.physical(0), dep.u) < tol
+        assert rel(sim.physical(2), dep.w) < tol
+        assert rel(sim.physical(1), dep.phi) < tol
+        assert sim.max_speed() == pytest.approx(dep.max_speed, rel=tol)
+        r = sim.reductions()
+        want = dns.reductions(d, oc, u, p)
+        for k in ("kinetic_energy", "dissipation", "scalar_variance", "scalar_dissipation"):
+            assert r[k] == pytest.approx(want[k], rel=10 * tol if dtype is np.float32 else 1e-13), k
+        band = 0.5 * dns.weighted_inner(d, u, d.shell_band(1, 2))
+        assert r["band_energy"] == pytest.approx(band, rel=10 * tol if dtype is np.float32 else 1e-13)
+        kap, e = sim.energy_spectrum()
+        _, eo = dns.energy_spectrum(d, u)
+        assert np.max(np.abs(e[: len(eo)] - eo)) <= 10 * tol * np.max(eo) + 1e-300
+
+
+@pytest.mark.parametrize("scheme", ["rk3", "rk4"])
+@pytest.mark.parametrize("forcing", [None, 0.1])
+def test_one_step_fp64_matches_the_oracle(g, scheme, forcing):
+    n = 32
+    sch = g.IFLowStorageRK4 if scheme == "rk4" else g.IFLowStorageRK3
+    sim, u, p, oc = make_sim(g, n, np.float64, scheme=sch, forcing=forcing)
+    d = dns.Domain(n)
+    with sim:
+        sim.advance(2e-3)
+        got = sim.state()
+        uo, po, dep = dns.step(d, oc, scheme, u, p, 2e-3)
+        assert rel(got.velocity_spectral, uo) < 1e-12
+        assert rel(got.scalar_spectral, po) < 1e-12
+        assert sim.max_speed() == pytest.approx(dep.max_speed, rel=1e-12)
+        assert rel(sim.physical(0), dep.u) < 1e-12
+
+
+def test_ten_steps_fp64_and_fp32_bulk_against_the_fp64_oracle(g):
+    """The north-star accuracy bar: FP32-bulk velocity within 1e-5 of the FP64 oracle after 10 steps."""
+    n, steps, dt = 64, 10, 2e-3
+    d = dns.Domain(n)
+    sim64, u, p, oc = make_sim(g, n, np.float64, forcing=0.1, nu=0.01)
+    sim32, *_ = make_sim(g, n, np.float32, forcing=0.1, nu=0.01)
+    uo, po, dep = u, p, None
+    errs = []
+    with sim64, sim32:
+        for s in range(steps):
+            uo, po, dep = dns.step(d, oc, "rk4", uo, po, dt, dep)
+            sim64.advance(dt)
+            sim32.advance(dt)
+            s32 = sim32.state()
+            errs.append((rel(s32.velocity_spectral, uo), rel(s32.scalar_spectral, po)))
+        s64 = sim64.state()
+        print("FP32-bulk rel. error per step (u, phi):", ["%.2e/%.2e" % e for e in errs], flush=True)
+        assert rel(s64.velocity_spectral, uo) < 1e-11
+        assert rel(s64.scalar_spectral, po) < 1e-11
+        assert errs[-1][0] <= 1e-5
+        assert errs[-1][1] <= 1e-5
+        _, e32 = sim32.energy_spectrum()
+        _, eo = dns.energy_spectrum(d, uo)
+        live = eo > 1e-12 * eo.max()
+        assert np.max(np.abs(e32[: len(eo)][live] - eo[live]) / eo[live]) < 1e-4
+
+
+def test_upload_rejects_content_outside_the_ball(g):
+    n = 16
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
+    u = g.random_phase_velocity(n, seed=1)
+    u[0, 0, 0, 7] = 1e-3                                  # mode (7, 0, 0): 9*49 > 256
+    with pytest.raises(ValueError, match="outside the 2/3"):
+        g.Simulation(dom, conditions=g.FlowConditions(viscosity=0.01, schmidt=1.0), state=g.ResolvedFields(u, None))
+
+
+def test_no_scalar_run_matches_the_oracle(g):
+    n = 32
+    sim, u, _, oc = make_sim(g, n, np.float64, scalar=False, forcing=0.1)
+    d = dns.Domain(n)
+    with sim:
+        sim.advance(2e-3)
+        uo, _, _ = dns.step(d, oc, "rk4", u, None, 2e-3)
+        assert rel(sim.state().velocity_spectral, uo) < 1e-12
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 5e-5)])
+def test_tracers_match_the_oracle(g, dtype, tol):
+    n, m, dt = 32, 500, 2e-3
+    pos = trc.uniform_positions(m, L, 5).T.copy()          # [3, M]
+    pos = pos.astype(dtype).astype(np.float64)
+    sim, u, p, oc = make_sim(g, n, dtype, tracers=g.Tracers(
+        g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype),
+        positions=pos))
+    d = dns.Domain(n)
+    with sim:
+        dep = dns.deposit(d, u, p)
+        v, s = sim.tracer_samples()
+        assert rel(v, trc.interpolate(dep.u, pos.T, L).T) < tol
+        assert rel(s, trc.interpolate(dep.phi, pos.T, L)) < tol
+        X = pos.T.copy()
+        vo = trc.interpolate(dep.u, X, L)
+        uo, po = u, p
+        for _ in range(3):
+            uo, po, dep = dns.step(d, oc, "rk4", uo, po, dt, dep)
+            X, vo = trc.heun_advect(X, vo, dep.u, dt, L)
+            sim.advance(dt)
+        got = sim.tracer_positions()
+        assert np.max(np.abs(got - X.T)) < 10 * tol
+        v, s = sim.tracer_samples()
+        assert rel(v, vo.T) < 10 * tol
+        # cell keys: the device's integer contract, bit-exact from the device's own positions
+        keys = sim.tracer_cells()
+        assert np.array_equal(keys, trc.cell_keys(got.T, n, L, dtype))
+        sim._call("hdns_rebuild_traversal")
+        order = sim.traversal_order()
+        assert np.array_equal(np.sort(order), np.arange(m))
+        tiles = (keys // (n * n) // 8 * (n // 8) + (keys // n) % n // 8) * (n // 8) + keys % n // 8
+        assert np.all(np.diff(tiles[order]) >= 0)
+
+
+def test_a_step_at_256_keeps_the_state_solenoidal_real_and_energy_consistent(g):
+    n = 256
+    sim, u, p, oc = make_sim(g, n, np.float32, forcing=0.1, nu=2.121e-3)
+    d = dns.Domain(n)
+    with sim:
+        e_spec = sim.reductions()["kinetic_energy"]
+        uphys = sim.physical(0)
+        e_phys = 0.5 * float(np.mean(np.sum(uphys.astype(np.float64) ** 2, axis=0)))
+        assert e_phys == pytest.approx(e_spec, rel=1e-5)
+        assert e_spec == pytest.approx(0.5, rel=1e-6)
+        sim.advance(1e-3)
+        st = sim.state().velocity_spectral
+        assert dns.divergence_residual(d, st) < 1e-6
+        assert np.array_equal(dns.reality_projection(d, st), st)
+        assert np.all(st[:, 0, 0, 0] == 0)

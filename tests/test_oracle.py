@@ -1,0 +1,296 @@
+"""Pins of the FP64 oracle against closed forms, invariants and brute force.
+
+The reference stores no golden arrays (every expected value in its test/ is computed in-test);
+these are its own test oracles re-expressed: observables on single-shell fields
+(test/observables_analytic.jl:1-87), exact diffusion (test/scalar.jl:42-60), Cauchy ratios
+(test/integrators.jl:49-78,165-188), the 2N zero-padded dealiasing oracle
+(test/dealiasing.jl:69-114, test/regressions/r6-inclusive-ball-exact.jl), the exact mean mode
+(r5-free-modules-01-mean-mode.jl), the reality projection (r7-reality-projection-forced-run.jl),
+<u.f> = eps for ConstantPower (test/forcing.jl:92-104), plus brute-force DFTs and the
+Carpenter-Kennedy stability polynomial (src/Propagator.jl:287-292).
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from oracle import dns
+
+TWO_PI = 2 * math.pi
+
+
+def _grid(n):
+    x = np.arange(n) * TWO_PI / n
+    z, y, xx = np.meshgrid(x, x, x, indexing="ij")
+    return xx, y, z
+
+
+# --- transforms and frame ------------------------------------------------------------------
+
+def test_transform_convention_matches_brute_force_dft():
+    n = 4
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((n, n, n))
+    got = dns.rfft3(a)
+    idx = np.arange(n)
+    want = np.zeros((n, n, n // 2 + 1), dtype=complex)
+    for kz in range(n):
+        for ky in range(n):
+            for kx in range(n // 2 + 1):
+                ph = np.exp(-2j * np.pi * (kx * idx[None, None, :] + ky * idx[None, :, None] + kz * idx[:, None, None]) / n)
+                want[kz, ky, kx] = np.sum(a * ph)
+    assert np.max(np.abs(got - want)) < 1e-12
+    assert np.max(np.abs(dns.irfft3(got, n) - a)) < 1e-14          # the 1/N^3 sits in the inverse
+
+
+@pytest.mark.parametrize("n", [12, 16, 48, 64])
+def test_mask_is_the_closed_two_thirds_ball_in_exact_arithmetic(n):
+    d = dns.Domain(n)
+    cut = Fraction(2, 3) * Fraction(n, 2)                        # (2/3)(N/2) in units of 2 pi/L
+    for (iz, iy, ix), keep in np.ndenumerate(d.mask):
+        m2 = int(d.mx[ix]) ** 2 + int(d.my[iy]) ** 2 + int(d.mz[iz]) ** 2
+        assert keep == (Fraction(m2) <= cut * cut)
+    if n % 3 == 0:                                               # the boundary shell is kept
+        assert d.mask[0, 0, n // 3]
+    assert d.my[n // 2] == n // 2 and d.my[n // 2 + 1] == -(n // 2 - 1)
+
+
+# --- observables (test/observables_analytic.jl) ----------------------------------------------
+
+A, B, C, D = 0.7, -0.4, 1.1, 0.9
+NU, KAP = 0.03, 0.05
+
+
+def _analytic(n=16):
+    x, y, z = _grid(n)
+    u = np.stack([A * np.sin(y), B * np.sin(z), C * np.sin(x)])
+    phi = D * np.sin(x)
+    return dns.Domain(n), dns.rfft3(u), dns.rfft3(phi), u
+
+
+def test_observables_closed_forms():
+    d, uh, ph, u = _analytic()
+    sq = A * A + B * B + C * C
+    r = dns.reductions(d, dns.Conditions(viscosity=NU, schmidt=NU / KAP), uh, ph)
+    assert r["kinetic_energy"] == pytest.approx(sq / 4, rel=1e-13)
+    assert r["dissipation"] == pytest.approx(NU * sq / 2, rel=1e-13)
+    assert r["scalar_variance"] == pytest.approx(D * D / 2, rel=1e-13)
+    assert r["scalar_dissipation"] == pytest.approx(KAP * D * D, rel=1e-13)
+    dep = dns.deposit(d, uh, ph)
+    assert dep.max_speed == pytest.approx(math.sqrt(sq), rel=1e-13)
+    kap, e = dns.energy_spectrum(d, uh)
+    assert e[1] * d.k0 == pytest.approx(sq / 4, rel=1e-13)       # all energy in shell 1
+    assert np.sum(e) * d.k0 == pytest.approx(sq / 4, rel=1e-13)  # Parseval over the shells
+    # the vorticity of the deposit is curl u = -(B cos z, C cos x, A cos y)
+    x, y, z = _grid(16)
+    w = -np.stack([B * np.cos(z), C * np.cos(x), A * np.cos(y)])
+    assert np.max(np.abs(dep.w - w)) < 1e-13
+
+
+# --- the nonlinear term ----------------------------------------------------------------------
+
+def _band_limited(n, seed, shell_frac=1.0):
+    """A random solenoidal, Hermitian field filling the 2/3 ball (boundary shell included)."""
+    d = dns.Domain(n)
+    rng = np.random.default_rng(seed)
+    f = (rng.standard_normal((3,) + d.m2.shape) + 1j * rng.standard_normal((3,) + d.m2.shape))
+    f = f * d.mask * (9 * d.m2 <= (shell_frac * n) ** 2)
+    f[:, 0, 0, 0] = 0
+    f = dns.leray(d, f)
+    return d, dns.reality_projection(d, f)
+
+
+@pytest.mark.parametrize("n", [12, 16])
+def test_dealiased_rhs_equals_the_zero_padded_2n_oracle(n):
+    d, uh = _band_limited(n, 3)
+    dep = dns.deposit(d, uh, None)
+    cond = dns.Conditions(viscosity=1.0)
+    got = dns.velocity_rhs(d, cond, uh, dep.u, dep.w)
+    # embed in a 2N grid (no aliasing there), form the raw product, project, restrict
+    n2 = 2 * n
+    d2 = dns.Domain(n2)
+    big = np.zeros((3, n2, n2, n + 1), dtype=complex)
+    idx = np.array([i if i <= n // 2 else i + n for i in range(n)])
+    big[:, idx[:, None], idx[None, :], : n // 2 + 1] = 8 * uh
+    dep2 = dns.deposit(d2, big, None)
+    raw = dns.leray(d2, dns.rfft3(np.cross(dep2.u, dep2.w, axis=0)))
+    want = raw[:, idx[:, None], idx[None, :], : n // 2 + 1] / 8
+    want = want * d.mask
+    want[:, 0, 0, 0] = 0
+    scale = np.max(np.abs(want))
+    assert np.max(np.abs(got - want)) / scale < 1e-12
+    # the comparison can see aliasing: a field reaching past the ball aliases into it
+    rng = np.random.default_rng(4)
+    wide = rng.standard_normal((3, n, n, n)) * 0.1
+    wh = dns.leray(d, dns.rfft3(wide))
+    wh[:, 0, 0, 0] = 0
+    dw = dns.deposit(d, wh, None)
+    gotw = dns.velocity_rhs(d, cond, wh, dw.u, dw.w)
+    bigw = np.zeros((3, n2, n2, n + 1), dtype=complex)
+    bigw[:, idx[:, None], idx[None, :], : n // 2 + 1] = 8 * wh
+    bigw[:, :, :, n // 2] = 0
+    bigw[:, n // 2 + n, :, :] = 0
+    bigw[:, :, n // 2 + n, :] = 0
+    dw2 = dns.deposit(d2, bigw, None)
+    truew = dns.leray(d2, dns.rfft3(np.cross(dw2.u, dw2.w, axis=0)))[:, idx[:, None], idx[None, :], : n // 2 + 1] / 8
+    truew = truew * d.mask
+    truew[:, 0, 0, 0] = 0
+    assert np.max(np.abs(gotw - truew)) / np.max(np.abs(truew)) > 1e-3
+
+
+def test_rhs_mean_mode_exactly_zero_and_solenoidal():
+    d, uh = _band_limited(16, 5)
+    dep = dns.deposit(d, uh, None)
+    nl = dns.velocity_rhs(d, dns.Conditions(viscosity=1.0), uh, dep.u, dep.w)
+    assert np.all(nl[:, 0, 0, 0] == 0)
+    assert dns.divergence_residual(d, nl) < 1e-14
+    # the scalar flux divergence has no mean by construction
+    phi = np.random.default_rng(2).standard_normal((16, 16, 16))
+    ns = dns.scalar_rhs(d, dns.Conditions(viscosity=1.0), uh, dep.u, phi)
+    assert ns[0, 0, 0] == 0
+
+
+def test_constant_power_injects_exactly_eps():
+    d, uh = _band_limited(16, 6)
+    eps = 0.1
+    cond = dns.Conditions(viscosity=0.01, forcing=(eps, 1, 2))
+    amp = dns.constant_power_amplitude(d, cond, uh)
+    f = amp * d.shell_band(1, 2) * uh
+    power = math.fsum(np.ravel((np.conj(uh) * f).real * d.weights)) / d.norm     # <u, f>
+    assert power == pytest.approx(eps, rel=1e-12)
+    # an empty band is clamped to zero (src/Forcing.jl:110-111)
+    empty = uh * (~d.shell_band(1, 2))
+    assert dns.constant_power_amplitude(d, cond, empty) == 0.0
+
+
+def test_shell_band_edges():
+    d = dns.Domain(16)
+    band = d.shell_band(1, 2)
+    m = np.sqrt(d.m2)
+    assert np.all(band == ((m >= 0.5) & (m < 2.5)))
+    assert np.all(d.shell_band(1, math.inf) == (d.m2 > 0))
+
+
+# --- the scheme --------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("scheme,order", [("rk3", 3), ("rk4", 4)])
+def test_stability_polynomial(scheme, order):
+    """The 2N loop applied to q' = z q (no integrating factor) has R(z) = sum_k<=p z^k/k! (+ z^5/200)."""
+    a, b, _ = dns.scheme_coefficients(scheme)
+    for z in (0.3 + 0.1j, -1.2 + 0.5j, 1.5j):
+        q, h = 1.0 + 0j, 0j
+        for i in range(len(a)):
+            h = (0 if i == 0 else a[i] * h) + z * q
+            q = q + b[i] * h
+        want = sum(z ** k / math.factorial(k) for k in range(order + 1))
+        if scheme == "rk4":
+            want += z ** 5 / 200
+        assert abs(q - want) < 1e-14
+
+
+def test_rk4_abscissae_match_the_b_partial_sums():
+    """d are differences of the nodes c, with c2 = b1 (Carpenter-Kennedy)."""
+    a, b, d = dns.scheme_coefficients("rk4")
+    c = np.cumsum((0.0,) + d)
+    assert c[1] == b[0]
+    assert c[-1] == pytest.approx(1.0, abs=1e-16)
+    assert c[2] == pytest.approx(0.3704, abs=1e-4) and c[4] == pytest.approx(0.9583, abs=1e-4)
+
+
+def test_exact_diffusion_at_zero_velocity():
+    n, sc, nu = 16, 0.7, 0.02
+    d = dns.Domain(n)
+    _, f = _band_limited(n, 9)
+    ph = f[0]
+    uh = np.zeros_like(f)
+    cond = dns.Conditions(viscosity=nu, schmidt=sc, mean_gradient=1.0)
+    dt, steps = 0.02, 25
+    u, p, dep = uh, ph, None
+    for _ in range(steps):
+        u, p, dep = dns.step(d, cond, "rk3", u, p, dt, dep)
+    want = ph * np.exp(-(nu / sc) * d.k2 * dt * steps)
+    assert np.linalg.norm(p - want) / np.linalg.norm(want) < 1e-10
+
+
+def _integrate(scheme, n, steps, T, uh, ph, cond):
+    d = dns.Domain(n)
+    u, p, dep = uh, ph, None
+    for _ in range(steps):
+        u, p, dep = dns.step(d, cond, scheme, u, p, T / steps, dep)
+    return u, p
+
+
+@pytest.mark.parametrize("scheme,lo,hi", [("rk3", 5.0, 12.0), ("rk4", 10.0, 24.0)])
+def test_temporal_self_convergence(scheme, lo, hi):
+    n = 16
+    d, uh = _band_limited(n, 1, shell_frac=0.5)
+    uh = uh * math.sqrt(0.5 / dns.reductions(d, dns.Conditions(viscosity=1), uh, None)["kinetic_energy"])
+    _, ph = _band_limited(n, 2, shell_frac=0.5)
+    ph = ph[0]
+    cond = dns.Conditions(viscosity=0.05, mean_gradient=0.5, forcing=(0.1, 1, 2))
+    T = 0.4
+    runs = [_integrate(scheme, n, k, T, uh, ph, cond) for k in (8, 16, 32)]
+    for f in (0, 1):
+        ratio = np.linalg.norm(runs[0][f] - runs[1][f]) / np.linalg.norm(runs[1][f] - runs[2][f])
+        assert lo < ratio < hi, (f, ratio)
+
+
+def test_mean_mode_and_reality_hold_over_steps():
+    n = 16
+    d, uh = _band_limited(n, 4, shell_frac=0.6)
+    ph = np.zeros((n, n, n // 2 + 1), dtype=complex)
+    cond = dns.Conditions(viscosity=0.02, forcing=(0.1, 1, 2), mean_gradient=1.0)
+    u, p, dep = uh, ph, None
+    for _ in range(5):
+        u, p, dep = dns.step(d, cond, "rk4", u, p, 0.01, dep)
+    assert np.all(u[:, 0, 0, 0] == 0)
+    assert np.array_equal(dns.reality_projection(d, u), u)
+    assert dns.divergence_residual(d, u) < 1e-15
+
+
+def test_reality_projection_removes_a_planted_antihermitian_pair():
+    d, uh = _band_limited(16, 8)
+    planted = uh.copy()
+    planted[0, 2, 1, 0] += 1j * 0.3          # mode (0, +1, +2)
+    planted[0, 14, 15, 0] += 1j * 0.3        # its partner (0, -1, -2): anti-Hermitian pair
+    fixed = dns.reality_projection(d, planted)
+    assert np.max(np.abs(fixed - uh)) < 1e-15
+    assert np.max(np.abs(dns.irfft3(planted, 16) - dns.irfft3(uh, 16))) < 1e-15   # invisible to c2r
+
+
+def test_mean_gradient_source_is_linear_in_g():
+    n = 16
+    d, uh = _band_limited(n, 11, shell_frac=0.6)
+    ph = np.zeros((n, n, n // 2 + 1), dtype=complex)
+    outs = []
+    for G in (0.0, 1.0, 2.5):
+        cond = dns.Conditions(viscosity=0.02, mean_gradient=G)
+        u, p, dep = uh, ph, None
+        for _ in range(3):
+            u, p, dep = dns.step(d, cond, "rk4", u, p, 0.01, dep)
+        outs.append(p)
+    assert np.all(outs[0] == 0)                                         # phi stays zero at G = 0
+    assert np.linalg.norm(outs[2] - 2.5 * outs[1]) / np.linalg.norm(outs[2]) < 1e-13
+
+
+def test_scalar_variance_budget():
+    """d<phi^2>/dt = -2 G <u_z phi> - eps_phi, checked at O(dt^4) across one RK4 step."""
+    n = 16
+    d, uh = _band_limited(n, 12, shell_frac=0.6)
+    _, pf = _band_limited(n, 13, shell_frac=0.6)
+    ph = pf[0]
+    G = 0.7
+    cond = dns.Conditions(viscosity=0.03, mean_gradient=G)
+
+    def rate(u, p):
+        dep = dns.deposit(d, u, p)
+        prod = -2 * G * float(np.mean(dep.u[2] * dep.phi))
+        return prod - dns.reductions(d, cond, u, p)["scalar_dissipation"]
+
+    dt = 1e-3
+    u1, p1, _ = dns.step(d, cond, "rk4", uh, ph, dt)
+    v0 = dns.reductions(d, cond, uh, ph)["scalar_variance"]
+    v1 = dns.reductions(d, cond, u1, p1)["scalar_variance"]
+    mid = 0.5 * (rate(uh, ph) + rate(u1, p1))
+    assert (v1 - v0) / dt == pytest.approx(mid, rel=1e-5)
