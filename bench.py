@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Headline benchmark: Mpoint*steps/s of the 512^3 IF-RK4 DNS step, FP32-bulk beside plain FP64.
+
+Workload (one GPU): 512^3 triple-periodic box of side 2 pi, 2/3 dealiasing, IF-RK4
+(Carpenter-Kennedy, 5 stages), velocity + passive scalar (Sc = 1) with a uniform mean gradient
+G = 1 along z, ConstantPower eps = 0.1 on shells 1-2, nu from k_max eta = 1.5 (8.419e-4), fixed
+dt = 1e-3.  Synthetic input: random-phase solenoidal u with E(k) ~ k^4 exp(-2 (k/4)^2), k = 1/2,
+seed 2026; phi = 0.  Tracers off.  A step is hdns_step: every stage plus the closing deposit,
+with the state resident in HBM before the timed region starts.
+
+Prints one JSON line (rank 0, one GPU).  Extra keys: fp64 (the plain-FP64 step on the same GPU),
+kernels (per-kernel event times), roofline (the dominant kernel against 8 TB/s), cpu_baseline (the
+FP64 oracle on the host cores, a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+KIND_NAMES = ["y_forward", "z_forward_stage_update", "reality_projection", "z_inverse_reductions",
+              "reduction_finalize", "y_inverse", "x_pass_products", "other"]
+KERNEL_SYMBOLS = ["k_yfwd", "k_zf", "k_reality", "k_zi", "k_finalize", "k_yinv", "k_xpass", ""]
+
+
+def hip():
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            h = ctypes.CDLL(name)
+            break
+        except OSError:
+            continue
+    else:
+        raise RuntimeError("libamdhip64.so not found")
+    h.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    h.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+    h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+    h.hipEventDestroy.argtypes = [ctypes.c_void_p]
+    return h
+
+
+def algorithmic_bytes(n: int, real_bytes: int, stages: int, scalar: bool = True) -> dict:
+    """Compulsory HBM bytes per launch of each kernel, counting retained modes only (kx <= M)."""
+    m = n // 3
+    k = 2 * m + 1
+    kx = m + 1
+    cb = 2 * real_bytes
+    nf, ni, ns = (6, 7, 4) if scalar else (3, 4, 3)
+    acc_frac = (stages - 1) / stages            # accumulators: not read at stage 1, not written at the last
+    comp = k * k * kx
+    return {
+        "y_forward": nf * n * kx * (n + k) * cb,
+        "z_forward_stage_update": (nf * n * k * kx + 2 * ns * comp + 2 * ns * comp * acc_frac) * cb,
+        "z_inverse_reductions": (ns * comp + ni * n * k * kx) * cb,
+        "y_inverse": ni * n * kx * (k + n) * cb,
+        "x_pass_products": (ni + nf) * n * n * kx * cb,
+        "reality_projection": 2 * ns * k * k * cb,
+        "reduction_finalize": 0,
+    }
+
+
+def time_gpu(g, n: int, dtype, steps: int, warmup: int, H, seed: int = 2026):
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=2 * math.pi, dealiasing=g.TwoThirds(), element_type=dtype)
+    nu = g.predicted_viscosity(dom, 0.1)
+    u = g.random_phase_velocity(n, seed=seed)
+    p = np.zeros((n, n, n // 2 + 1), dtype=np.complex128)
+    cond = g.FlowConditions(viscosity=nu, schmidt=1.0,
+                            velocity_forcing=g.ConstantPower(dom, injection_rate=0.1, lowest_shell=1, highest_shell=2))
+    dt = 1e-3
+    with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=1.0,
+                      field_integration_scheme=g.IFLowStorageRK4, keep_physical=False) as sim:
+        del u, p
+        for _ in range(warmup):
+            sim.step_fields(dt)
+        sim.synchronize()
+        stream = sim.stream()
+        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        H.hipEventCreate(ctypes.byref(e0))
+        H.hipEventCreate(ctypes.byref(e1))
+        sim.profile(True)
+        sim.profile_read(reset=True)
+        H.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        H.hipEventRecord(e0, ctypes.c_void_p(stream))
+        for _ in range(steps):
+            sim.step_fields(dt)
+        H.hipEventRecord(e1, ctypes.c_void_p(stream))
+        H.hipEventSynchronize(e1)
+        H.hipDeviceSynchronize()
+        wall = time.perf_counter() - t0
+        ms = ctypes.c_float()
+        H.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        kms, kcnt = sim.profile_read(reset=True)
+        sim.profile(False)
+        H.hipEventDestroy(e0)
+        H.hipEventDestroy(e1)
+        speed = sim.max_speed()
+        red = sim.reductions()
+        mem = sim.memory_bytes()
+    if not (math.isfinite(speed) and math.isfinite(red["scalar_variance"])):
+        raise RuntimeError("the benchmark run blew up")
+    return {"ms_total": float(ms.value), "wall_s": wall, "kernel_ms": kms, "kernel_launches": kcnt,
+            "max_speed": speed, "kinetic_energy": red["kinetic_energy"], "memory_bytes": mem, "nu": nu}
+
+
+def kernel_table(res: dict, n: int, real_bytes: int, stages: int) -> dict:
+    ab = algorithmic_bytes(n, real_bytes, stages)
+    out = {}
+    for i, name in enumerate(KIND_NAMES[:7]):
+        cnt = int(res["kernel_launches"][i])
+        if cnt == 0:
+            continue
+        avg_ms = float(res["kernel_ms"][i]) / cnt
+        b = ab.get(name, 0)
+        out[name] = {"symbol": KERNEL_SYMBOLS[i], "launches": cnt, "avg_ms": round(avg_ms, 5),
+                     "share": round(float(res["kernel_ms"][i]) / max(res["ms_total"], 1e-9), 4),
+                     "algorithmic_bytes": b, "achieved_GBs": round(b / (avg_ms * 1e-3) / 1e9, 1) if b else None}
+    return out
+
+
+def cpu_baseline(n_sample: int) -> dict:
+    """The FP64 oracle, one IF-RK4 step of the same physics at n_sample^3, on the host cores."""
+    from oracle import dns
+
+    import gbg_amd as g
+    cores = min(os.cpu_count() or 1, 16)
+    dns.set_workers(cores)
+    d = dns.Domain(n_sample)
+    u = g.random_phase_velocity(n_sample, seed=2026)
+    p = np.zeros((n_sample, n_sample, n_sample // 2 + 1), dtype=np.complex128)
+    dom = g.FixedDomain(grid_points_per_axis=n_sample, side_length=2 * math.pi, dealiasing=g.TwoThirds(),
+                        element_type=np.float64)
+    cond = dns.Conditions(viscosity=g.predicted_viscosity(dom, 0.1), schmidt=1.0, forcing=(0.1, 1, 2),
+                          mean_gradient=1.0)
+    dep = dns.deposit(d, u, p)
+    t0 = time.perf_counter()
+    dns.step(d, cond, "rk4", u, p, 1e-3, dep)
+    sec = time.perf_counter() - t0
+    return {"value": round(n_sample ** 3 / sec / 1e6, 3), "unit": "Mpoint*steps/s", "cores": cores,
+            "kind": "port",
+            "sample": f"one IF-RK4 step (+ closing deposit) of the same physics at {n_sample}^3 in the FP64 numpy "
+                      f"oracle: scipy.fft (pocketfft) on {cores} threads, numpy pointwise on one; "
+                      f"{sec:.2f} s/step; rate per grid point, so comparable with the 512^3 metric"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain-FP64 comparison run")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-n", type=int, default=256)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this DNS step runs on a single MI355X and --gpus {args.gpus} is not "
+              "supported (the slab-decomposed multi-GPU path is a design note, not built)", file=sys.stderr)
+        return 2
+    import __graft_entry__
+    g = __graft_entry__.load_package()
+    H = hip()
+    n, stages = args.n, 5
+    print(f"bench: {n}^3 IF-RK4 FP32-bulk, warmup {args.warmup}, steps {args.steps}", file=sys.stderr, flush=True)
+    r32 = time_gpu(g, n, np.float32, args.steps, args.warmup, H)
+    ms32 = r32["ms_total"] / args.steps
+    value = n ** 3 / (ms32 * 1e-3) / 1e6
+    k32 = kernel_table(r32, n, 4, stages)
+    fp64 = None
+    if not args.no_fp64:
+        print(f"bench: {n}^3 IF-RK4 plain FP64", file=sys.stderr, flush=True)
+        r64 = time_gpu(g, n, np.float64, max(3, args.steps // 2), 1, H)
+        ms64 = r64["ms_total"] / max(3, args.steps // 2)
+        fp64 = {"value": round(n ** 3 / (ms64 * 1e-3) / 1e6, 2), "ms_per_step": round(ms64, 3),
+                "fp32_bulk_speedup": round(ms64 / ms32, 3), "kernels": kernel_table(r64, n, 8, stages)}
+    dom_name = max(k32, key=lambda k: k32[k]["share"])
+    dk = k32[dom_name]
+    achieved = dk["achieved_GBs"]
+    step_bytes = sum(dk_["algorithmic_bytes"] * dk_["launches"] for dk_ in k32.values()) / args.steps
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dk["symbol"],
+                "kernel_role": dom_name, "algorithmic_bytes_per_launch": dk["algorithmic_bytes"],
+                "step_algorithmic_GBs": round(step_bytes / (ms32 * 1e-3) / 1e9, 1),
+                "survey_model_frac": round(value * 1e6 * 811.0 / (HBM_PEAK_GBS * 1e9), 4)}
+    traffic_file = ROOT / "profiles" / "traffic.json"
+    if traffic_file.exists():
+        try:
+            t = json.loads(traffic_file.read_text()).get(dk["symbol"])
+            if t and t.get("n") == n:
+                roofline["traffic"] = t["bytes_per_launch"]
+        except (ValueError, KeyError):
+            pass
+    cpu = None
+    if not args.no_cpu_baseline:
+        print(f"bench: cpu baseline at {args.cpu_sample_n}^3", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_sample_n)
+    line = {
+        "metric": "Mpoint*steps/s at 512^3 IF-RK4, FP32-bulk vs FP64 (+% HBM roofline)",
+        "value": round(value, 2), "unit": "Mpoint*steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms32, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 (FP64 integrating factors, twiddles, reductions)",
+        "data": "synthetic: random-phase k^4 exp(-2(k/4)^2) velocity, seed 2026, phi = 0",
+        "config": {"workload": f"{n}^3 forced isotropic turbulence + passive scalar with mean gradient, IF-RK4, "
+                               f"dt 1e-3, ConstantPower eps 0.1 shells 1-2, nu {r32['nu']:.4e}, tracers off",
+                   "grid": n, "scheme": "IFLowStorageRK4", "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu, "fp64": fp64, "kernels": k32,
+        "wall_ms_per_step": round(r32["wall_s"] * 1e3 / args.steps, 3),
+        "memory_GiB": round(r32["memory_bytes"] / 2 ** 30, 2),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

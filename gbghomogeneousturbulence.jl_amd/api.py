@@ -225,7 +225,8 @@ class Simulation:
     def __init__(self, domain: FixedDomain, *, conditions: FlowConditions, state: ResolvedFields,
                  field_integration_scheme=IFLowStorageRK4, tracers: Tracers | None = None,
                  particle_integration_scheme=None, mean_gradient: float = 0.0, cfl: float | None = None,
-                 max_dt: float = math.inf, keep_vorticity: bool = False, tracer_capacity: int | None = None):
+                 max_dt: float = math.inf, keep_vorticity: bool = False, keep_physical: bool = True,
+                 tracer_capacity: int | None = None):
         n = domain.grid_points_per_axis
         carries_scalar = state.scalar_spectral is not None
         u = np.ascontiguousarray(state.velocity_spectral, dtype=np.complex128)
@@ -269,7 +270,8 @@ class Simulation:
         cfg.mean_gradient = float(mean_gradient)
         cap = tracers.count if tracers is not None else 0
         cfg.tracer_capacity = int(tracer_capacity if tracer_capacity is not None else cap)
-        cfg.store_physical = 1 | (2 if carries_scalar else 0) | (4 if keep_vorticity else 0)
+        keep = keep_physical or tracers is not None
+        cfg.store_physical = (1 | (2 if carries_scalar else 0) if keep else 0) | (4 if keep_vorticity else 0)
         h = c_void_p()
         L.check(L.lib().hdns_ctx_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._ctx = h
@@ -392,6 +394,24 @@ class Simulation:
 
     def synchronize(self) -> None:
         self._call("hdns_synchronize")
+
+    def stream(self) -> int:
+        """The hipStream_t the context launches on (for event timing on that stream)."""
+        s = c_void_p()
+        self._call("hdns_stream", ctypes.byref(s))
+        return s.value or 0
+
+    def profile(self, on: bool) -> None:
+        """Time every kernel launch with HIP events on the context's stream."""
+        self._call("hdns_profile_enable", int(on))
+
+    def profile_read(self, reset: bool = True):
+        """(ms, launches) per kernel kind: y forward, z forward + update, reality, z inverse,
+        finalize, y inverse, x pass, other."""
+        ms = np.zeros(8)
+        cnt = np.zeros(8, dtype=np.int64)
+        self._call("hdns_profile_read", L.dptr(ms), cnt.ctypes.data_as(ctypes.POINTER(c_int64)), int(reset))
+        return ms, cnt
 
     # the hot path
     def step_fields(self, dt: float) -> None:
