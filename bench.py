@@ -57,7 +57,7 @@ def algorithmic_bytes(n: int, real_bytes: int, stages: int, scalar: bool = True)
     k = 2 * m + 1
     kx = m + 1
     cb = 2 * real_bytes
-    nf, ni, ns = (6, 7, 4) if scalar else (3, 4, 3)
+    nf, ni, ns = (6, 7, 4) if scalar else (3, 6, 3)
     acc_frac = (stages - 1) / stages            # accumulators: not read at stage 1, not written at the last
     comp = k * k * kx
     return {

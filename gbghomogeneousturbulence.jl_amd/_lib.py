@@ -54,6 +54,7 @@ _SIGS = {
     "hdns_state_upload": [c_void_p, _D, _D],
     "hdns_state_download": [c_void_p, _D, _D],
     "hdns_step": [c_void_p, c_double],
+    "hdns_step_graph": [c_void_p, c_double],
     "hdns_deposit": [c_void_p],
     "hdns_max_speed": [c_void_p, _D],
     "hdns_reductions": [c_void_p, _D],

@@ -414,9 +414,10 @@ class Simulation:
         return ms, cnt
 
     # the hot path
-    def step_fields(self, dt: float) -> None:
-        """``step!`` then the field ``deposit!`` (src/Propagator.jl:304-419,808-842), asynchronously."""
-        self._call("hdns_step", float(dt))
+    def step_fields(self, dt: float, graph: bool = False) -> None:
+        """``step!`` then the field ``deposit!`` (src/Propagator.jl:304-419,808-842), asynchronously;
+        ``graph`` replays the step as one captured HIP graph."""
+        self._call("hdns_step_graph" if graph else "hdns_step", float(dt))
         self.fields_epoch += 1
 
     def cfl_timestep(self) -> float:

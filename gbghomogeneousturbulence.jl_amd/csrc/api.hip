@@ -143,7 +143,10 @@ hdns_status alloc(hdns_ctx *ctx, void **p, size_t bytes) {
   return HDNS_OK;
 }
 
+void drop_graph(Ctx &c);
+
 void release(Ctx &c) {
+  drop_graph(c);
   for (hipEvent_t e : c.prof_ev) (void)hipEventDestroy(e);
   c.prof_ev.clear();
   void *ptrs[] = {c.uh, c.hu, c.ph, c.hp, c.bufa, c.bufb, c.phys, c.physw, c.tw, c.partials, c.scalars,
@@ -302,7 +305,7 @@ hdns_status hdns_ctx_create(const hdns_config *config, hdns_ctx **out) {
   c.band.on = k.velocity_forcing == HDNS_CONSTANT_POWER;
   c.band.lo4 = (2 * k.lowest_shell - 1) * (2 * k.lowest_shell - 1);
   c.band.hi4 = k.highest_shell < 0 ? -1 : (2 * k.highest_shell + 1) * (2 * k.highest_shell + 1);
-  const size_t ncf = k.has_scalar ? 6 : 3, nci = k.has_scalar ? 7 : 4;
+  const size_t ncf = k.has_scalar ? 6 : 3, nci = k.has_scalar ? 7 : 6;
   const size_t nn = (size_t)n * n, nk = (size_t)n * c.geo.k;
   c.buf_a = std::max(ncf * nn, nci * nk) * c.geo.kxp;
   c.buf_b = std::max(ncf * nk, nci * nn) * c.geo.kxp;
@@ -523,14 +526,12 @@ hdns_status hdns_state_download(hdns_ctx *ctx, double *u_hat, double *phi_hat) {
   return HDNS_OK;
 }
 
-hdns_status hdns_step(hdns_ctx *ctx, double dt) {
-  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
-  Ctx &c = ctx->c;
-  if (!(isfinite(dt) && dt > 0))
-    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "the timestep must be positive and finite");
-  if (!c.deposited)
-    return fail(ctx, HDNS_ERR_STATE, "no deposit of the current state: upload a state (or deposit) before stepping");
-  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+}  // extern "C"
+
+namespace {
+
+// Every launch of one step! + its closing deposit!, on the context's stream, no host sync.
+void launch_step(Ctx &c, double dt) {
   const bool fp32 = !c.fp64;
   const Coeffs k = scheme_coefficients(c.cfg.scheme, fp32);
   const double dt_t = fp32 ? (double)(float)dt : dt;
@@ -571,8 +572,66 @@ hdns_status hdns_step(hdns_ctx *ctx, double dt) {
     if (last) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c.stream, c.maxbits, 0ull, 1);
     TIMED(c, K_XPASS, c.pipe->xpass(c, last ? fin : mid));
   }
+}
+
+hdns_status step_checks(hdns_ctx *ctx, double dt) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  if (!(isfinite(dt) && dt > 0))
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "the timestep must be positive and finite");
+  if (!ctx->c.deposited)
+    return fail(ctx, HDNS_ERR_STATE, "no deposit of the current state: upload a state (or deposit) before stepping");
+  return HDNS_OK;
+}
+
+void drop_graph(Ctx &c) {
+  if (c.graph_exec) (void)hipGraphExecDestroy((hipGraphExec_t)c.graph_exec);
+  if (c.graph) (void)hipGraphDestroy((hipGraph_t)c.graph);
+  c.graph_exec = nullptr;
+  c.graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+hdns_status hdns_step(hdns_ctx *ctx, double dt) {
+  hdns_status s = step_checks(ctx, dt);
+  if (s != HDNS_OK) return s;
+  Ctx &c = ctx->c;
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  launch_step(c, dt);
   c.deposited = true;
   return launched(ctx);
+}
+
+hdns_status hdns_step_graph(hdns_ctx *ctx, double dt) {
+  hdns_status s = step_checks(ctx, dt);
+  if (s != HDNS_OK) return s;
+  Ctx &c = ctx->c;
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  if (!c.graph_exec || c.graph_dt != dt) {
+    drop_graph(c);
+    const bool prof = c.prof;
+    c.prof = false;  // no event records inside the captured step
+    HIP_TRY(ctx, hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
+    launch_step(c, dt);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(c.stream, &graph);
+    c.prof = prof;
+    if (e != hipSuccess) return fail(ctx, HDNS_ERR_DEVICE, std::string("step capture failed: ") + hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(ctx, HDNS_ERR_DEVICE, std::string("graph instantiation failed: ") + hipGetErrorString(e));
+    }
+    c.graph = graph;
+    c.graph_exec = exec;
+    c.graph_dt = dt;
+  }
+  HIP_TRY(ctx, hipGraphLaunch((hipGraphExec_t)c.graph_exec, c.stream));
+  c.deposited = true;
+  return HDNS_OK;
 }
 
 hdns_status hdns_deposit(hdns_ctx *ctx) {

@@ -68,6 +68,9 @@ struct Ctx {
   int prof_used = 0;
   double prof_ms[NKIND] = {0};
   long long prof_cnt[NKIND] = {0};
+  // the captured step (hdns_step_graph), valid for one dt
+  void *graph = nullptr, *graph_exec = nullptr;
+  double graph_dt = 0.0;
   std::unique_ptr<Pipeline> pipe;
   std::string err;
 };

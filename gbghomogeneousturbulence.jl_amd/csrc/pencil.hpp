@@ -79,7 +79,7 @@ __global__ __launch_bounds__(NT) void k_yinv(const cx<T> *__restrict__ q, cx<T> 
 // ---------------------------------------------------------------------------------------------
 // x pass: a block takes XR = 4 pairs of adjacent y rows of one z plane.  Two real rows travel as
 // one complex transform (row a in the real part, row b in the imaginary part), so each component
-// costs one complex FFT per row pair.  It inverts the 7 (or 4) components to physical space,
+// costs one complex FFT per row pair.  It inverts the 7 (or 6) components to physical space,
 // forms u x w and u phi there, transforms the 6 (or 3) products forward and keeps kx <= M.
 // On request it also stores the physical u and phi (the deposit) and the max of |u|^2.
 
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(NT) void k_xpass(const cx<T> *__restrict__ rin, cx<
   const size_t plane = (size_t)N * g.kxp;            // one z plane of one component in R / PX
   const size_t vol = (size_t)N * N * N;
 
-  static_assert(NCIN == 7 || NCIN == 4, "u, w and optionally phi");
+  static_assert(NCIN == 7 || NCIN == 6, "u, w and optionally phi");
   constexpr int NCOUT = NCIN == 7 ? 6 : 3;
   cx<T> f[NCIN][EPT];
 #pragma unroll

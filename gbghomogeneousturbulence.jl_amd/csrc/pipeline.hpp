@@ -33,7 +33,7 @@ struct PipelineImpl final : Pipeline {
     set_lds((const void *)k_zi<T, N, TX, NTZ, true>, LDS_COL);
     set_lds((const void *)k_zi<T, N, TX, NTZ, false>, LDS_COL);
     set_lds((const void *)k_xpass<T, N, NTX, 7>, LDS_X);
-    set_lds((const void *)k_xpass<T, N, NTX, 4>, LDS_X);
+    set_lds((const void *)k_xpass<T, N, NTX, 6>, LDS_X);
   }
 
   int tiles(const Ctx &c) const { return c.geo.kxp / TX; }
@@ -70,7 +70,7 @@ struct PipelineImpl final : Pipeline {
                          (const cx<T> *)c.ph, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, c.band, c.partials);
   }
   void yinv(Ctx &c) override {
-    const int nc = c.cfg.has_scalar ? 7 : 4;
+    const int nc = c.cfg.has_scalar ? 7 : 6;
     hipLaunchKernelGGL((k_yinv<T, N, TX, NTY>), dim3(tiles(c), N, nc), dim3(NTY), LDS_COL, c.stream,
                        (const cx<T> *)c.bufa, (cx<T> *)c.bufb, c.geo, (const cx<double> *)c.tw);
   }
@@ -80,7 +80,7 @@ struct PipelineImpl final : Pipeline {
       hipLaunchKernelGGL((k_xpass<T, N, NTX, 7>), dim3(blocks), dim3(NTX), LDS_X, c.stream, (const cx<T> *)c.bufb,
                          (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
     else
-      hipLaunchKernelGGL((k_xpass<T, N, NTX, 4>), dim3(blocks), dim3(NTX), LDS_X, c.stream, (const cx<T> *)c.bufb,
+      hipLaunchKernelGGL((k_xpass<T, N, NTX, 6>), dim3(blocks), dim3(NTX), LDS_X, c.stream, (const cx<T> *)c.bufb,
                          (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
   }
   void spectrum(Ctx &c, int which, double *bins) override {

@@ -191,7 +191,7 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
   constexpr int M = N / 3, K = 2 * M + 1;
   constexpr int MODES = K * TX;
   constexpr int MPT = (MODES + NT - 1) / NT;
-  constexpr int NCIN = SCALAR ? 7 : 4;
+  constexpr int NCIN = SCALAR ? 7 : 6;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
   __shared__ double red[NT / 64][NRED];

@@ -91,6 +91,10 @@ hdns_status hdns_state_download(hdns_ctx *ctx, double *u_hat, double *phi_hat);
  * deposit! (src/Propagator.jl:219,304-419,808-842): every stage, the state Leray and reality
  * projections, and the recovery of u, w, phi and max|u| at the new state. */
 hdns_status hdns_step(hdns_ctx *ctx, double dt);
+/* The same step as hdns_step, launched as one HIP graph captured from its kernel sequence on the
+ * first call (and again whenever dt changes), so a step costs one launch.  The step holds no host
+ * readback and no memset node, so the replay is bitwise the eager step. */
+hdns_status hdns_step_graph(hdns_ctx *ctx, double dt);
 /* The field deposit! of the current state on its own (src/Propagator.jl:808-842). */
 hdns_status hdns_deposit(hdns_ctx *ctx);
 /* max|u| of the last deposit (src/Observables.jl:304-305, src/Cache.jl:243). */
