@@ -8,8 +8,9 @@
 // Each pass with radix R over sub-transform length NS maps butterfly j (0 <= j < N/R) to
 //   inputs  j + q*N/R,  twiddle W_{NS*R}^{q*(j mod NS)},  outputs (j/NS)*NS*R + (j mod NS) + q*NS,
 // the Stockham self-sorting form, so the result lands in natural order with no bit reversal.
-// Twiddles come from an FP64 table of W_N^m = exp(-2 pi i m/N), rounded to the working type
-// at each use.  SIGN = -1 is the forward (unnormalised) transform, +1 the inverse.
+// Twiddles come from an FP64 table of W_N^m = exp(-2 pi i m/N) (each entry the rounding of an
+// extended-precision value), staged once per block into LDS rounded to the working type, so a
+// pass reads them at LDS latency.  SIGN = -1 is the forward (unnormalised) transform, +1 the inverse.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -142,7 +143,7 @@ __device__ __forceinline__ int lds_index(int n, int col) {
 
 // One Stockham pass over TX columns of length N held in LDS, by NT cooperating threads.
 template <typename T, int N, int R, int NS, int TX, int NT, int SIGN, bool IL>
-__device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<double> *__restrict__ tw) {
+__device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<T> *tw) {
   constexpr int NB = N / R;
   constexpr int TOT = NB * TX;
   constexpr int PER = (TOT + NT - 1) / NT;
@@ -169,8 +170,8 @@ __device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<double> *__re
       if constexpr (NS > 1) {
 #pragma unroll
         for (int q = 1; q < R; ++q) {
-          const cx<double> w = tw[q * k * (N / (NS * R))];
-          const cx<T> wt = {T(w.re), SIGN < 0 ? T(w.im) : T(-w.im)};
+          const cx<T> w = tw[q * k * (N / (NS * R))];
+          const cx<T> wt = {w.re, SIGN < 0 ? w.im : -w.im};
           v[p][q] = v[p][q] * wt;
         }
       }
@@ -184,14 +185,20 @@ __device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<double> *__re
 }
 
 template <typename T, int N, int TX, int NT, int SIGN, bool IL, int NS, int R, int... REST>
-__device__ __forceinline__ void run_passes(cx<T> *buf, const cx<double> *__restrict__ tw) {
+__device__ __forceinline__ void run_passes(cx<T> *buf, const cx<T> *tw) {
   stockham_pass<T, N, R, NS, TX, NT, SIGN, IL>(buf, tw);
   if constexpr (sizeof...(REST) > 0) run_passes<T, N, TX, NT, SIGN, IL, NS * R, REST...>(buf, tw);
 }
 
+// Stage the FP64 table into LDS in the working type (the caller synchronises before use).
+template <typename T, int N, int NT>
+__device__ __forceinline__ void stage_twiddles(cx<T> *twl, const cx<double> *__restrict__ tw) {
+  for (int i = threadIdx.x; i < N; i += NT) twl[i] = {T(tw[i].re), T(tw[i].im)};
+}
+
 // The radix plan of each supported length.
 template <typename T, int N, int TX, int NT, int SIGN, bool IL>
-__device__ __forceinline__ void fft_columns(cx<T> *buf, const cx<double> *__restrict__ tw) {
+__device__ __forceinline__ void fft_columns(cx<T> *buf, const cx<T> *tw) {
   if constexpr (N == 16) run_passes<T, N, TX, NT, SIGN, IL, 1, 4, 4>(buf, tw);
   else if constexpr (N == 32) run_passes<T, N, TX, NT, SIGN, IL, 1, 8, 4>(buf, tw);
   else if constexpr (N == 64) run_passes<T, N, TX, NT, SIGN, IL, 1, 8, 8>(buf, tw);

@@ -47,11 +47,14 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cx<T> *__restrict__ px, cx<T>
                                              const cx<double> *__restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+  constexpr int LDS_TILE = N * TX;
+  cx<T> *twl = buf + LDS_TILE;
+  stage_twiddles<T, N, NT>(twl, tw);
   const int kx0 = blockIdx.x * TX, z = blockIdx.y, c = blockIdx.z;
   const cx<T> *src = px + ((size_t)c * N + z) * (size_t)N * g.kxp + kx0;
   for (int e = threadIdx.x; e < N * TX; e += NT) buf[e] = src[(size_t)(e / TX) * g.kxp + (e % TX)];
   __syncthreads();
-  fft_columns<T, N, TX, NT, -1, true>(buf, tw);
+  fft_columns<T, N, TX, NT, -1, true>(buf, twl);
   cx<T> *dst = pf + ((size_t)c * N + z) * (size_t)g.k * g.kxp + kx0;
   for (int e = threadIdx.x; e < g.k * TX; e += NT) {
     const int idx = e / TX, col = e % TX;
@@ -64,6 +67,9 @@ __global__ __launch_bounds__(NT) void k_yinv(const cx<T> *__restrict__ q, cx<T> 
                                              const cx<double> *__restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+  constexpr int LDS_TILE = N * TX;
+  cx<T> *twl = buf + LDS_TILE;
+  stage_twiddles<T, N, NT>(twl, tw);
   const int kx0 = blockIdx.x * TX, z = blockIdx.y, c = blockIdx.z;
   const cx<T> *src = q + ((size_t)c * N + z) * (size_t)g.k * g.kxp + kx0;
   for (int e = threadIdx.x; e < N * TX; e += NT) {
@@ -71,7 +77,7 @@ __global__ __launch_bounds__(NT) void k_yinv(const cx<T> *__restrict__ q, cx<T> 
     buf[e] = idx >= 0 ? src[(size_t)idx * g.kxp + (e % TX)] : czero<T>();
   }
   __syncthreads();
-  fft_columns<T, N, TX, NT, +1, true>(buf, tw);
+  fft_columns<T, N, TX, NT, +1, true>(buf, twl);
   cx<T> *dst = r + ((size_t)c * N + z) * (size_t)N * g.kxp + kx0;
   for (int e = threadIdx.x; e < N * TX; e += NT) dst[(size_t)(e / TX) * g.kxp + (e % TX)] = buf[e];
 }
@@ -100,6 +106,9 @@ __global__ __launch_bounds__(NT) void k_xpass(const cx<T> *__restrict__ rin, cx<
   static_assert(NT % XR == 0 && N % TPC == 0, "x pass shape");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+  constexpr int LDS_TILE = XR * N;
+  cx<T> *twl = buf + LDS_TILE;
+  stage_twiddles<T, N, NT>(twl, tw);
   __shared__ double red[NT / 64];
 
   const int rows_per_plane = N / (2 * XR);
@@ -114,26 +123,46 @@ __global__ __launch_bounds__(NT) void k_xpass(const cx<T> *__restrict__ rin, cx<
   static_assert(NCIN == 7 || NCIN == 6, "u, w and optionally phi");
   constexpr int NCOUT = NCIN == 7 ? 6 : 3;
   cx<T> f[NCIN][EPT];
+  // the retained coefficients (kx <= M) of a row pair, prefetched one component ahead
+  constexpr int PF = (XR * (N / 3 + 1) + NT - 1) / NT;
+  cx<T> pa[PF], pb[PF];
+  auto fetch = [&](int c) {
+    const cx<T> *src = rin + (size_t)c * N * plane + (size_t)z * plane;
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int e = tid + NT * i;
+      const int p = e / (g.m + 1), kx = e % (g.m + 1);
+      if (e < XR * (g.m + 1)) {
+        pa[i] = src[(size_t)(y0 + 2 * p) * g.kxp + kx];
+        pb[i] = src[(size_t)(y0 + 2 * p + 1) * g.kxp + kx];
+      }
+    }
+  };
+  fetch(0);
 #pragma unroll
   for (int c = 0; c < NCIN; ++c) {
-    const cx<T> *src = rin + (size_t)c * N * plane + (size_t)z * plane;
-    for (int e = tid; e < XR * (half + 1); e += NT) {
-      const int p = e / (half + 1), kx = e % (half + 1);
-      const int ya = y0 + 2 * p;
-      cx<T> a = czero<T>(), b = czero<T>();
-      if (kx <= g.m) {
-        a = src[(size_t)ya * g.kxp + kx];
-        b = src[(size_t)(ya + 1) * g.kxp + kx];
-      }
-      if (kx == 0 || kx == half) {  // the self-conjugate coefficients: only the real part is seen
-        a.im = T(0);
-        b.im = T(0);
-      }
-      buf[p * N + kx] = {a.re - b.im, a.im + b.re};
-      if (kx > 0 && kx < half) buf[p * N + (N - kx)] = {a.re + b.im, b.re - a.im};
+    // the gap M < kx < N - M of both halves is zero
+    for (int e = tid; e < XR * (N - 2 * g.m - 1); e += NT) {
+      const int p = e / (N - 2 * g.m - 1);
+      buf[p * N + g.m + 1 + e % (N - 2 * g.m - 1)] = czero<T>();
     }
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int e = tid + NT * i;
+      if (e < XR * (g.m + 1)) {
+        const int p = e / (g.m + 1), kx = e % (g.m + 1);
+        cx<T> a = pa[i], b = pb[i];
+        if (kx == 0) {  // the self-conjugate coefficient: only its real part is seen
+          a.im = T(0);
+          b.im = T(0);
+        }
+        buf[p * N + kx] = {a.re - b.im, a.im + b.re};
+        if (kx > 0) buf[p * N + (N - kx)] = {a.re + b.im, b.re - a.im};
+      }
+    }
+    if (c + 1 < NCIN) fetch(c + 1);
     __syncthreads();
-    fft_columns<T, N, XR, NT, +1, false>(buf, tw);
+    fft_columns<T, N, XR, NT, +1, false>(buf, twl);
 #pragma unroll
     for (int j = 0; j < EPT; ++j) f[c][j] = buf[pr * N + lane + TPC * j];
     __syncthreads();
@@ -198,7 +227,7 @@ __global__ __launch_bounds__(NT) void k_xpass(const cx<T> *__restrict__ rin, cx<
       buf[pr * N + lane + TPC * j] = v;
     }
     __syncthreads();
-    fft_columns<T, N, XR, NT, -1, false>(buf, tw);
+    fft_columns<T, N, XR, NT, -1, false>(buf, twl);
     cx<T> *dst = pxo + (size_t)d * N * plane + (size_t)z * plane;
     for (int e = tid; e < XR * g.kxp; e += NT) {
       const int p = e / g.kxp, kx = e % g.kxp;

@@ -15,11 +15,11 @@ template <typename T, int N>
 struct PipelineImpl final : Pipeline {
   static constexpr int TX = tile_width_of<T>();
   static constexpr int M = N / 3, K = 2 * M + 1;
-  static constexpr int NTY = 256;
+  static constexpr int NTY = 512;
   static constexpr int NTZ = N >= 1024 ? 1024 : 512;
   static constexpr int NTX = N >= 1024 ? 1024 : (N >= 64 ? N : 64);
-  static constexpr size_t LDS_COL = (size_t)N * TX * sizeof(cx<T>);
-  static constexpr size_t LDS_X = (size_t)4 * N * sizeof(cx<T>);
+  static constexpr size_t LDS_COL = ((size_t)N * TX + N) * sizeof(cx<T>);   // tile + twiddles
+  static constexpr size_t LDS_X = ((size_t)4 * N + N) * sizeof(cx<T>);
 
   static void set_lds(const void *fn, size_t bytes) {
     (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);

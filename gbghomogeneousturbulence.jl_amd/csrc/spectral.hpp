@@ -53,6 +53,9 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
   constexpr int MPT = (MODES + NT - 1) / NT;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+  constexpr int LDS_TILE = N * TX;
+  cx<T> *twl = buf + LDS_TILE;
+  stage_twiddles<T, N, NT>(twl, tw);
   const int tid = threadIdx.x;
   const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y;
   const int my = signed_of(ky_idx, g);
@@ -67,7 +70,7 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
     const cx<T> *src = pf + (size_t)c * pencil + (size_t)ky_idx * g.kxp + kx0;
     for (int e = tid; e < N * TX; e += NT) buf[e] = src[(size_t)(e / TX) * K * g.kxp + (e % TX)];
     __syncthreads();
-    fft_columns<T, N, TX, NT, -1, true>(buf, tw);
+    fft_columns<T, N, TX, NT, -1, true>(buf, twl);
 #pragma unroll
     for (int r = 0; r < MPT; ++r) {
       const int e = tid + NT * r;
@@ -140,7 +143,7 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
       const cx<T> *src = pf + (size_t)(3 + c) * pencil + (size_t)ky_idx * g.kxp + kx0;
       for (int e = tid; e < N * TX; e += NT) buf[e] = src[(size_t)(e / TX) * K * g.kxp + (e % TX)];
       __syncthreads();
-      fft_columns<T, N, TX, NT, -1, true>(buf, tw);
+      fft_columns<T, N, TX, NT, -1, true>(buf, twl);
 #pragma unroll
       for (int r = 0; r < MPT; ++r) {
         const int e = tid + NT * r;
@@ -194,6 +197,9 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
   constexpr int NCIN = SCALAR ? 7 : 6;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+  constexpr int LDS_TILE = N * TX;
+  cx<T> *twl = buf + LDS_TILE;
+  stage_twiddles<T, N, NT>(twl, tw);
   __shared__ double red[NT / 64][NRED];
   const int tid = threadIdx.x;
   const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y;
@@ -258,7 +264,7 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
       buf[bin_of(kz_idx, g) * TX + col] = scale(v, inv);
     }
     __syncthreads();
-    fft_columns<T, N, TX, NT, +1, true>(buf, tw);
+    fft_columns<T, N, TX, NT, +1, true>(buf, twl);
     cx<T> *dst = q + (size_t)c * pencil + (size_t)ky_idx * g.kxp + kx0;
     for (int e = tid; e < N * TX; e += NT) dst[(size_t)(e / TX) * K * g.kxp + (e % TX)] = buf[e];
     __syncthreads();
