@@ -141,13 +141,30 @@ __device__ __forceinline__ int lds_index(int n, int col) {
   else return col * N + n;
 }
 
-// One Stockham pass over TX columns of length N held in LDS, by NT cooperating threads.
-template <typename T, int N, int R, int NS, int TX, int NT, int SIGN, bool IL>
+// The barrier between the read and write phases of a pass: the whole block, or (WAVE) only the
+// wavefront that owns a private LDS region -- LDS requests of one wave complete in order, so a
+// compiler fence around the wave barrier is all a wave-private exchange needs.
+template <bool WAVE>
+__device__ __forceinline__ void pass_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+// One Stockham pass over TX columns of length N held in LDS, by NT cooperating threads: the
+// threads t with t % NT equal to the same index form the group (NT = 64 lanes of one wavefront
+// when WAVE; otherwise every group of the block passes the same block-wide barriers).
+template <typename T, int N, int R, int NS, int TX, int NT, int SIGN, bool IL, bool WAVE>
 __device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<T> *tw) {
+  static_assert(!WAVE || NT == 64, "a wave-private transform runs on the 64 lanes of one wavefront");
   constexpr int NB = N / R;
   constexpr int TOT = NB * TX;
   constexpr int PER = (TOT + NT - 1) / NT;
-  const int tid = threadIdx.x;
+  const int tid = (int)(threadIdx.x % NT);   // NT threads of the block (or of a wave) own the transform
   cx<T> v[PER][R];
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
@@ -159,7 +176,7 @@ __device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<T> *tw) {
       for (int q = 0; q < R; ++q) v[p][q] = buf[lds_index<N, TX, IL>(j + q * NB, col)];
     }
   }
-  __syncthreads();
+  pass_sync<WAVE>();
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int b = tid + p * NT;
@@ -181,13 +198,13 @@ __device__ __forceinline__ void stockham_pass(cx<T> *buf, const cx<T> *tw) {
       for (int q = 0; q < R; ++q) buf[lds_index<N, TX, IL>(base + q * NS, col)] = v[p][q];
     }
   }
-  __syncthreads();
+  pass_sync<WAVE>();
 }
 
-template <typename T, int N, int TX, int NT, int SIGN, bool IL, int NS, int R, int... REST>
+template <typename T, int N, int TX, int NT, int SIGN, bool IL, bool WAVE, int NS, int R, int... REST>
 __device__ __forceinline__ void run_passes(cx<T> *buf, const cx<T> *tw) {
-  stockham_pass<T, N, R, NS, TX, NT, SIGN, IL>(buf, tw);
-  if constexpr (sizeof...(REST) > 0) run_passes<T, N, TX, NT, SIGN, IL, NS * R, REST...>(buf, tw);
+  stockham_pass<T, N, R, NS, TX, NT, SIGN, IL, WAVE>(buf, tw);
+  if constexpr (sizeof...(REST) > 0) run_passes<T, N, TX, NT, SIGN, IL, WAVE, NS * R, REST...>(buf, tw);
 }
 
 // Stage the FP64 table into LDS in the working type (the caller synchronises before use).
@@ -197,15 +214,15 @@ __device__ __forceinline__ void stage_twiddles(cx<T> *twl, const cx<double> *__r
 }
 
 // The radix plan of each supported length.
-template <typename T, int N, int TX, int NT, int SIGN, bool IL>
+template <typename T, int N, int TX, int NT, int SIGN, bool IL, bool WAVE = false>
 __device__ __forceinline__ void fft_columns(cx<T> *buf, const cx<T> *tw) {
-  if constexpr (N == 16) run_passes<T, N, TX, NT, SIGN, IL, 1, 4, 4>(buf, tw);
-  else if constexpr (N == 32) run_passes<T, N, TX, NT, SIGN, IL, 1, 8, 4>(buf, tw);
-  else if constexpr (N == 64) run_passes<T, N, TX, NT, SIGN, IL, 1, 8, 8>(buf, tw);
-  else if constexpr (N == 128) run_passes<T, N, TX, NT, SIGN, IL, 1, 8, 4, 4>(buf, tw);
-  else if constexpr (N == 256) run_passes<T, N, TX, NT, SIGN, IL, 1, 8, 8, 4>(buf, tw);
-  else if constexpr (N == 512) run_passes<T, N, TX, NT, SIGN, IL, 1, 8, 8, 8>(buf, tw);
-  else if constexpr (N == 1024) run_passes<T, N, TX, NT, SIGN, IL, 1, 8, 8, 4, 4>(buf, tw);
+  if constexpr (N == 16) run_passes<T, N, TX, NT, SIGN, IL, WAVE, 1, 4, 4>(buf, tw);
+  else if constexpr (N == 32) run_passes<T, N, TX, NT, SIGN, IL, WAVE, 1, 8, 4>(buf, tw);
+  else if constexpr (N == 64) run_passes<T, N, TX, NT, SIGN, IL, WAVE, 1, 8, 8>(buf, tw);
+  else if constexpr (N == 128) run_passes<T, N, TX, NT, SIGN, IL, WAVE, 1, 8, 4, 4>(buf, tw);
+  else if constexpr (N == 256) run_passes<T, N, TX, NT, SIGN, IL, WAVE, 1, 8, 8, 4>(buf, tw);
+  else if constexpr (N == 512) run_passes<T, N, TX, NT, SIGN, IL, WAVE, 1, 8, 8, 8>(buf, tw);
+  else if constexpr (N == 1024) run_passes<T, N, TX, NT, SIGN, IL, WAVE, 1, 8, 8, 4, 4>(buf, tw);
   else static_assert(N == 16, "unsupported transform length");
 }
 

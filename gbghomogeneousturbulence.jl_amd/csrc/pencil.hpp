@@ -83,11 +83,14 @@ __global__ __launch_bounds__(NT) void k_yinv(const cx<T> *__restrict__ q, cx<T> 
 }
 
 // ---------------------------------------------------------------------------------------------
-// x pass: a block takes XR = 4 pairs of adjacent y rows of one z plane.  Two real rows travel as
-// one complex transform (row a in the real part, row b in the imaginary part), so each component
-// costs one complex FFT per row pair.  It inverts the 7 (or 6) components to physical space,
-// forms u x w and u phi there, transforms the 6 (or 3) products forward and keeps kx <= M.
-// On request it also stores the physical u and phi (the deposit) and the max of |u|^2.
+// x pass: LPR lanes own one pair of adjacent y rows of one z plane (512 threads per block).  Two
+// real rows travel as one complex transform (row a in the real part, row b in the imaginary
+// part), so each component costs one complex FFT per row pair.  At FP32 and N <= 512 a single
+// wavefront owns a row pair and synchronises with wave barriers only; wider transforms spread over
+// 2 or 4 wavefronts so a lane holds at most 8 FP32 (4 FP64) values per component.  The kernel
+// inverts the 7 (or 6) components to physical space, forms u x w and u phi there, transforms the
+// 6 (or 3) products forward and keeps kx <= M.  On request it also stores the physical u and phi
+// (the deposit) and the max of |u|^2.
 
 struct XArgs {
   int store_u, store_phi, store_w;
@@ -97,115 +100,123 @@ struct XArgs {
   unsigned long long *max_bits;  // max |u|^2 as the bits of a non-negative double
 };
 
-template <typename T, int N, int NT, int NCIN>
-__global__ __launch_bounds__(NT) void k_xpass(const cx<T> *__restrict__ rin, cx<T> *__restrict__ pxo, Geo g,
-                                              const cx<double> *__restrict__ tw, XArgs xa) {
-  constexpr int XR = 4;              // row pairs per block
-  constexpr int TPC = NT / XR;       // threads per row pair
-  constexpr int EPT = N / TPC;       // elements per thread per row pair
-  static_assert(NT % XR == 0 && N % TPC == 0, "x pass shape");
+constexpr int XPASS_THREADS = 512;
+
+// lanes per row pair: a lane holds N/LPR values of each component
+template <typename T, int N>
+constexpr int xpass_lanes() {
+  constexpr int e = sizeof(T) == 4 ? 8 : 4;
+  constexpr int l = N / e;
+  return l < 64 ? 64 : (l > XPASS_THREADS ? XPASS_THREADS : l);
+}
+
+template <typename T, int N, int NCIN>
+__global__ __launch_bounds__(XPASS_THREADS) void k_xpass(const cx<T> *__restrict__ rin,
+                                                              cx<T> *__restrict__ pxo, Geo g,
+                                                              const cx<double> *__restrict__ tw, XArgs xa) {
+  constexpr int LPR = xpass_lanes<T, N>();
+  constexpr bool WAVE = LPR == 64;
+  constexpr int NT = XPASS_THREADS;
+  constexpr int XR = NT / LPR;                     // row pairs per block
+  constexpr int E = (N + LPR - 1) / LPR;           // physical elements per lane per row pair
+  constexpr int M = N / 3;
+  constexpr int PF = (M + 1 + LPR - 1) / LPR;      // retained coefficients per lane per row
+  static_assert(NCIN == 7 || NCIN == 6, "u, w and optionally phi");
+  constexpr int NCOUT = NCIN == 7 ? 6 : 3;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-  constexpr int LDS_TILE = XR * N;
-  cx<T> *twl = buf + LDS_TILE;
-  stage_twiddles<T, N, NT>(twl, tw);
+  cx<T> *twl = buf + XR * N;
   __shared__ double red[NT / 64];
+  stage_twiddles<T, N, NT>(twl, tw);
+  __syncthreads();
 
   const int rows_per_plane = N / (2 * XR);
   const int z = blockIdx.x / rows_per_plane;
-  const int y0 = (blockIdx.x % rows_per_plane) * (2 * XR);
-  const int tid = threadIdx.x;
-  const int pr = tid / TPC, lane = tid % TPC;
-  const int half = N / 2;
-  const size_t plane = (size_t)N * g.kxp;            // one z plane of one component in R / PX
+  const int w = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+  const int ya = (blockIdx.x % rows_per_plane) * (2 * XR) + 2 * w;
+  const size_t plane = (size_t)N * g.kxp;           // one z plane of one component in R / PX
   const size_t vol = (size_t)N * N * N;
+  cx<T> *wb = buf + w * N;                         // this row pair's transform buffer
 
-  static_assert(NCIN == 7 || NCIN == 6, "u, w and optionally phi");
-  constexpr int NCOUT = NCIN == 7 ? 6 : 3;
-  cx<T> f[NCIN][EPT];
-  // the retained coefficients (kx <= M) of a row pair, prefetched one component ahead
-  constexpr int PF = (XR * (N / 3 + 1) + NT - 1) / NT;
   cx<T> pa[PF], pb[PF];
   auto fetch = [&](int c) {
-    const cx<T> *src = rin + (size_t)c * N * plane + (size_t)z * plane;
+    const cx<T> *src = rin + (size_t)c * N * plane + (size_t)z * plane + (size_t)ya * g.kxp;
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
-      const int e = tid + NT * i;
-      const int p = e / (g.m + 1), kx = e % (g.m + 1);
-      if (e < XR * (g.m + 1)) {
-        pa[i] = src[(size_t)(y0 + 2 * p) * g.kxp + kx];
-        pb[i] = src[(size_t)(y0 + 2 * p + 1) * g.kxp + kx];
+      const int kx = lane + LPR * i;
+      if (kx <= M) {
+        pa[i] = src[kx];
+        pb[i] = src[g.kxp + kx];
       }
     }
   };
+
+  cx<T> f[NCIN][E];
   fetch(0);
 #pragma unroll
   for (int c = 0; c < NCIN; ++c) {
-    // the gap M < kx < N - M of both halves is zero
-    for (int e = tid; e < XR * (N - 2 * g.m - 1); e += NT) {
-      const int p = e / (N - 2 * g.m - 1);
-      buf[p * N + g.m + 1 + e % (N - 2 * g.m - 1)] = czero<T>();
-    }
+    for (int e = lane; e < N - 2 * M - 1; e += LPR) wb[M + 1 + e] = czero<T>();
 #pragma unroll
     for (int i = 0; i < PF; ++i) {
-      const int e = tid + NT * i;
-      if (e < XR * (g.m + 1)) {
-        const int p = e / (g.m + 1), kx = e % (g.m + 1);
+      const int kx = lane + LPR * i;
+      if (kx <= M) {
         cx<T> a = pa[i], b = pb[i];
         if (kx == 0) {  // the self-conjugate coefficient: only its real part is seen
           a.im = T(0);
           b.im = T(0);
         }
-        buf[p * N + kx] = {a.re - b.im, a.im + b.re};
-        if (kx > 0) buf[p * N + (N - kx)] = {a.re + b.im, b.re - a.im};
+        wb[kx] = {a.re - b.im, a.im + b.re};
+        if (kx > 0) wb[N - kx] = {a.re + b.im, b.re - a.im};
       }
     }
     if (c + 1 < NCIN) fetch(c + 1);
-    __syncthreads();
-    fft_columns<T, N, XR, NT, +1, false>(buf, twl);
+    pass_sync<WAVE>();
+    fft_columns<T, N, 1, LPR, +1, false, WAVE>(wb, twl);
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) f[c][j] = buf[pr * N + lane + TPC * j];
-    __syncthreads();
+    for (int q = 0; q < E; ++q)
+      if (lane + LPR * q < N) f[c][q] = wb[lane + LPR * q];
+    pass_sync<WAVE>();
   }
 
   // physical space: (re, im) = (row a, row b) of every component
   double mx = 0.0;
   if (xa.want_max) {
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) {
-      const T sa = f[0][j].re * f[0][j].re + f[1][j].re * f[1][j].re + f[2][j].re * f[2][j].re;
-      const T sb = f[0][j].im * f[0][j].im + f[1][j].im * f[1][j].im + f[2][j].im * f[2][j].im;
+    for (int q = 0; q < E; ++q) {
+      if (lane + LPR * q >= N) continue;
+      const T sa = f[0][q].re * f[0][q].re + f[1][q].re * f[1][q].re + f[2][q].re * f[2][q].re;
+      const T sb = f[0][q].im * f[0][q].im + f[1][q].im * f[1][q].im + f[2][q].im * f[2][q].im;
       const double da = (double)sa, db = (double)sb;
       if (da > mx || da != da) mx = da;
       if (db > mx || db != db) mx = db;
     }
   }
   if (xa.store_u || xa.store_phi || xa.store_w) {
-    const int ya = y0 + 2 * pr;
     const size_t rowa = ((size_t)z * N + ya) * N, rowb = rowa + N;
     T *ph = reinterpret_cast<T *>(xa.phys);
     T *pw = reinterpret_cast<T *>(xa.physw);
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) {
-      const int x = lane + TPC * j;
+    for (int q = 0; q < E; ++q) {
+      const int x = lane + LPR * q;
+      if (x >= N) continue;
       if (xa.store_u) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          ph[c * vol + rowa + x] = f[c][j].re;
-          ph[c * vol + rowb + x] = f[c][j].im;
+          ph[c * vol + rowa + x] = f[c][q].re;
+          ph[c * vol + rowb + x] = f[c][q].im;
         }
       }
       if constexpr (NCIN == 7) {
         if (xa.store_phi) {
-          ph[3 * vol + rowa + x] = f[NCIN - 1][j].re;
-          ph[3 * vol + rowb + x] = f[NCIN - 1][j].im;
+          ph[3 * vol + rowa + x] = f[NCIN - 1][q].re;
+          ph[3 * vol + rowb + x] = f[NCIN - 1][q].im;
         }
       }
       if (xa.store_w) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          pw[c * vol + rowa + x] = f[3 + c][j].re;
-          pw[c * vol + rowb + x] = f[3 + c][j].im;
+          pw[c * vol + rowa + x] = f[3 + c][q].re;
+          pw[c * vol + rowb + x] = f[3 + c][q].im;
         }
       }
     }
@@ -214,35 +225,34 @@ __global__ __launch_bounds__(NT) void k_xpass(const cx<T> *__restrict__ rin, cx<
 #pragma unroll
   for (int d = 0; d < NCOUT; ++d) {
 #pragma unroll
-    for (int j = 0; j < EPT; ++j) {
+    for (int q = 0; q < E; ++q) {
+      if (lane + LPR * q >= N) continue;
       cx<T> v;
       if (d < 3) {  // the negated Lamb vector u x w (cross!(w, u, w), src/Generator.jl:529)
         const int c1 = (d + 1) % 3, c2 = (d + 2) % 3;
-        v.re = f[c1][j].re * f[3 + c2][j].re - f[c2][j].re * f[3 + c1][j].re;
-        v.im = f[c1][j].im * f[3 + c2][j].im - f[c2][j].im * f[3 + c1][j].im;
+        v.re = f[c1][q].re * f[3 + c2][q].re - f[c2][q].re * f[3 + c1][q].re;
+        v.im = f[c1][q].im * f[3 + c2][q].im - f[c2][q].im * f[3 + c1][q].im;
       } else {      // the flux u phi (multiply!(u, phi, u), src/Generator.jl:727)
-        v.re = f[d - 3][j].re * f[NCIN - 1][j].re;
-        v.im = f[d - 3][j].im * f[NCIN - 1][j].im;
+        v.re = f[d - 3][q].re * f[NCIN - 1][q].re;
+        v.im = f[d - 3][q].im * f[NCIN - 1][q].im;
       }
-      buf[pr * N + lane + TPC * j] = v;
+      wb[lane + LPR * q] = v;
     }
-    __syncthreads();
-    fft_columns<T, N, XR, NT, -1, false>(buf, twl);
-    cx<T> *dst = pxo + (size_t)d * N * plane + (size_t)z * plane;
-    for (int e = tid; e < XR * g.kxp; e += NT) {
-      const int p = e / g.kxp, kx = e % g.kxp;
-      const int ya = y0 + 2 * p;
+    pass_sync<WAVE>();
+    fft_columns<T, N, 1, LPR, -1, false, WAVE>(wb, twl);
+    cx<T> *dst = pxo + (size_t)d * N * plane + (size_t)z * plane + (size_t)ya * g.kxp;
+    for (int kx = lane; kx < g.kxp; kx += LPR) {
       cx<T> A = czero<T>(), B = czero<T>();
-      if (kx <= g.m) {
-        const cx<T> C = buf[p * N + kx];
-        const cx<T> Cm = buf[p * N + (kx == 0 ? 0 : N - kx)];
+      if (kx <= M) {
+        const cx<T> C = wb[kx];
+        const cx<T> Cm = wb[(N - kx) & (N - 1)];
         A = {T(0.5) * (C.re + Cm.re), T(0.5) * (C.im - Cm.im)};
         B = {T(0.5) * (C.im + Cm.im), T(0.5) * (Cm.re - C.re)};
       }
-      dst[(size_t)ya * g.kxp + kx] = A;
-      dst[(size_t)(ya + 1) * g.kxp + kx] = B;
+      dst[kx] = A;
+      dst[g.kxp + kx] = B;
     }
-    __syncthreads();
+    pass_sync<WAVE>();
   }
 
   if (xa.want_max) {
@@ -251,12 +261,12 @@ __global__ __launch_bounds__(NT) void k_xpass(const cx<T> *__restrict__ rin, cx<
       const double o = __shfl_down(mx, off, 64);
       if (o > mx || o != o) mx = o;
     }
-    if ((tid & 63) == 0) red[tid >> 6] = mx;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
     __syncthreads();
-    if (tid == 0) {
+    if (threadIdx.x == 0) {
       double b = red[0];
-      for (int w = 1; w < NT / 64; ++w)
-        if (red[w] > b || red[w] != red[w]) b = red[w];
+      for (int i = 1; i < NT / 64; ++i)
+        if (red[i] > b || red[i] != red[i]) b = red[i];
       atomicMax(xa.max_bits, (unsigned long long)__double_as_longlong(b));
     }
   }

@@ -16,10 +16,10 @@ struct PipelineImpl final : Pipeline {
   static constexpr int TX = tile_width_of<T>();
   static constexpr int M = N / 3, K = 2 * M + 1;
   static constexpr int NTY = 512;
-  static constexpr int NTZ = N >= 1024 ? 1024 : 512;
-  static constexpr int NTX = N >= 1024 ? 1024 : (N >= 64 ? N : 64);
+  static constexpr int NTZ = N >= 512 ? 1024 : 512;
   static constexpr size_t LDS_COL = ((size_t)N * TX + N) * sizeof(cx<T>);   // tile + twiddles
-  static constexpr size_t LDS_X = ((size_t)4 * N + N) * sizeof(cx<T>);
+  static constexpr int XR = XPASS_THREADS / xpass_lanes<T, N>();
+  static constexpr size_t LDS_X = ((size_t)XR * N + N) * sizeof(cx<T>);
 
   static void set_lds(const void *fn, size_t bytes) {
     (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
@@ -32,8 +32,8 @@ struct PipelineImpl final : Pipeline {
     set_lds((const void *)k_zf<T, N, TX, NTZ, false>, LDS_COL);
     set_lds((const void *)k_zi<T, N, TX, NTZ, true>, LDS_COL);
     set_lds((const void *)k_zi<T, N, TX, NTZ, false>, LDS_COL);
-    set_lds((const void *)k_xpass<T, N, NTX, 7>, LDS_X);
-    set_lds((const void *)k_xpass<T, N, NTX, 6>, LDS_X);
+    set_lds((const void *)k_xpass<T, N, 7>, LDS_X);
+    set_lds((const void *)k_xpass<T, N, 6>, LDS_X);
   }
 
   int tiles(const Ctx &c) const { return c.geo.kxp / TX; }
@@ -75,12 +75,12 @@ struct PipelineImpl final : Pipeline {
                        (const cx<T> *)c.bufa, (cx<T> *)c.bufb, c.geo, (const cx<double> *)c.tw);
   }
   void xpass(Ctx &c, const XArgs &xa) override {
-    const int blocks = N * N / 8;
+    const int blocks = N * N / (2 * XR);
     if (c.cfg.has_scalar)
-      hipLaunchKernelGGL((k_xpass<T, N, NTX, 7>), dim3(blocks), dim3(NTX), LDS_X, c.stream, (const cx<T> *)c.bufb,
+      hipLaunchKernelGGL((k_xpass<T, N, 7>), dim3(blocks), dim3(XPASS_THREADS), LDS_X, c.stream, (const cx<T> *)c.bufb,
                          (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
     else
-      hipLaunchKernelGGL((k_xpass<T, N, NTX, 6>), dim3(blocks), dim3(NTX), LDS_X, c.stream, (const cx<T> *)c.bufb,
+      hipLaunchKernelGGL((k_xpass<T, N, 6>), dim3(blocks), dim3(XPASS_THREADS), LDS_X, c.stream, (const cx<T> *)c.bufb,
                          (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
   }
   void spectrum(Ctx &c, int which, double *bins) override {
