@@ -226,4 +226,120 @@ __device__ __forceinline__ void fft_columns(cx<T> *buf, const cx<T> *tw) {
   else static_assert(N == 16, "unsupported transform length");
 }
 
+// ---------------------------------------------------------------------------------------------
+// Transforms whose first pass reads its inputs through a loader and whose last pass hands its
+// outputs to a storer, so the data enter and leave through registers: HBM (or the caller's
+// registers) -> pass 1 -> LDS -> middle passes -> LDS -> last pass -> registers.  Butterfly b of a
+// pass is (col, j) as above; the loader is called as load(p, q, n, col) for input element n of the
+// p-th butterfly of this thread, the storer as store(p, q, n, col, value) for output element n.
+// The callbacks are inlined, so arrays they index with (p, q) stay in registers.
+
+template <int N, int R, int TX, int NT>
+struct PassShape {
+  static constexpr int NB = N / R;
+  static constexpr int TOT = NB * TX;
+  static constexpr int PER = (TOT + NT - 1) / NT;
+};
+
+// First pass (NS = 1: no twiddles).  SYNC_READ: the loader reads the buffer this pass writes.
+template <typename T, int N, int R, int TX, int NT, int SIGN, bool IL, bool WAVE, bool SYNC_READ, class Load>
+__device__ __forceinline__ void first_pass(cx<T> *buf, Load &&load) {
+  using S = PassShape<N, R, TX, NT>;
+  const int tid = (int)(threadIdx.x % NT);
+  cx<T> v[S::PER][R];
+#pragma unroll
+  for (int p = 0; p < S::PER; ++p) {
+    const int b = tid + p * NT;
+    if (S::TOT % NT == 0 || b < S::TOT) {
+      const int col = IL ? b % TX : b / S::NB;
+      const int j = IL ? b / TX : b % S::NB;
+#pragma unroll
+      for (int q = 0; q < R; ++q) v[p][q] = load(p, q, j + q * S::NB, col);
+    }
+  }
+  if constexpr (SYNC_READ) pass_sync<WAVE>();
+#pragma unroll
+  for (int p = 0; p < S::PER; ++p) {
+    const int b = tid + p * NT;
+    if (S::TOT % NT == 0 || b < S::TOT) {
+      const int col = IL ? b % TX : b / S::NB;
+      const int j = IL ? b / TX : b % S::NB;
+      dft<R, SIGN>(v[p]);
+#pragma unroll
+      for (int q = 0; q < R; ++q) buf[lds_index<N, TX, IL>(j * R + q, col)] = v[p][q];
+    }
+  }
+  pass_sync<WAVE>();
+}
+
+// Last pass: inputs from LDS, outputs to the storer; ends with a barrier so the buffer may be
+// rewritten at once.
+template <typename T, int N, int R, int NS, int TX, int NT, int SIGN, bool IL, bool WAVE, class Store>
+__device__ __forceinline__ void last_pass(const cx<T> *buf, const cx<T> *tw, Store &&store) {
+  using S = PassShape<N, R, TX, NT>;
+  const int tid = (int)(threadIdx.x % NT);
+#pragma unroll
+  for (int p = 0; p < S::PER; ++p) {
+    const int b = tid + p * NT;
+    if (S::TOT % NT == 0 || b < S::TOT) {
+      const int col = IL ? b % TX : b / S::NB;
+      const int j = IL ? b / TX : b % S::NB;
+      const int k = j % NS;
+      cx<T> v[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) v[q] = buf[lds_index<N, TX, IL>(j + q * S::NB, col)];
+#pragma unroll
+      for (int q = 1; q < R; ++q) {
+        const cx<T> w = tw[q * k * (N / (NS * R))];
+        v[q] = v[q] * cx<T>{w.re, SIGN < 0 ? w.im : -w.im};
+      }
+      dft<R, SIGN>(v);
+      const int base = (j / NS) * NS * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) store(p, q, base + q * NS, col, v[q]);
+    }
+  }
+  pass_sync<WAVE>();
+}
+
+template <int... RS>
+struct Radices {};
+
+template <typename T, int N, int TX, int NT, int SIGN, bool IL, bool WAVE, int NS, int R, int... REST, class Store>
+__device__ __forceinline__ void middle_and_last(Radices<R, REST...>, cx<T> *buf, const cx<T> *tw, Store &&store) {
+  if constexpr (sizeof...(REST) == 0) {
+    last_pass<T, N, R, NS, TX, NT, SIGN, IL, WAVE>(buf, tw, store);
+  } else {
+    stockham_pass<T, N, R, NS, TX, NT, SIGN, IL, WAVE>(buf, tw);
+    middle_and_last<T, N, TX, NT, SIGN, IL, WAVE, NS * R>(Radices<REST...>{}, buf, tw, store);
+  }
+}
+
+template <typename T, int N, int TX, int NT, int SIGN, bool IL, bool WAVE, bool SYNC_READ, int R1, int... REST,
+          class Load, class Store>
+__device__ __forceinline__ void fft_io_plan(Radices<R1, REST...>, cx<T> *buf, const cx<T> *tw, Load &&load,
+                                            Store &&store) {
+  first_pass<T, N, R1, TX, NT, SIGN, IL, WAVE, SYNC_READ>(buf, load);
+  middle_and_last<T, N, TX, NT, SIGN, IL, WAVE, R1>(Radices<REST...>{}, buf, tw, store);
+}
+
+// The radix plan of each length in the register-in/register-out form (first_radix / last_radix
+// in spectral.hpp name its first and last pass).
+template <typename T, int N, int TX, int NT, int SIGN, bool IL, bool WAVE, bool SYNC_READ = false, class Load,
+          class Store>
+__device__ __forceinline__ void fft_io(cx<T> *buf, const cx<T> *tw, Load &&load, Store &&store) {
+  if constexpr (N == 16) fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<4, 4>{}, buf, tw, load, store);
+  else if constexpr (N == 32) fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 4>{}, buf, tw, load, store);
+  else if constexpr (N == 64) fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 8>{}, buf, tw, load, store);
+  else if constexpr (N == 128)
+    fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 4, 4>{}, buf, tw, load, store);
+  else if constexpr (N == 256)
+    fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 8, 4>{}, buf, tw, load, store);
+  else if constexpr (N == 512)
+    fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 8, 8>{}, buf, tw, load, store);
+  else if constexpr (N == 1024)
+    fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 8, 4, 4>{}, buf, tw, load, store);
+  else static_assert(N == 16, "unsupported transform length");
+}
+
 }  // namespace hdns

@@ -47,19 +47,17 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cx<T> *__restrict__ px, cx<T>
                                              const cx<double> *__restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-  constexpr int LDS_TILE = N * TX;
-  cx<T> *twl = buf + LDS_TILE;
+  cx<T> *twl = buf + N * TX;
   stage_twiddles<T, N, NT>(twl, tw);
   const int kx0 = blockIdx.x * TX, z = blockIdx.y, c = blockIdx.z;
   const cx<T> *src = px + ((size_t)c * N + z) * (size_t)N * g.kxp + kx0;
-  for (int e = threadIdx.x; e < N * TX; e += NT) buf[e] = src[(size_t)(e / TX) * g.kxp + (e % TX)];
-  __syncthreads();
-  fft_columns<T, N, TX, NT, -1, true>(buf, twl);
   cx<T> *dst = pf + ((size_t)c * N + z) * (size_t)g.k * g.kxp + kx0;
-  for (int e = threadIdx.x; e < g.k * TX; e += NT) {
-    const int idx = e / TX, col = e % TX;
-    dst[(size_t)idx * g.kxp + col] = buf[bin_of(idx, g) * TX + col];
-  }
+  fft_io<T, N, TX, NT, -1, true, false>(
+      buf, twl, [&](int, int, int n, int col) { return src[(size_t)n * g.kxp + col]; },
+      [&](int, int, int n, int col, cx<T> v) {
+        const int idx = idx_of_bin(n, g);
+        if (idx >= 0) dst[(size_t)idx * g.kxp + col] = v;
+      });
 }
 
 template <typename T, int N, int TX, int NT>
@@ -67,19 +65,18 @@ __global__ __launch_bounds__(NT) void k_yinv(const cx<T> *__restrict__ q, cx<T> 
                                              const cx<double> *__restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-  constexpr int LDS_TILE = N * TX;
-  cx<T> *twl = buf + LDS_TILE;
+  cx<T> *twl = buf + N * TX;
   stage_twiddles<T, N, NT>(twl, tw);
   const int kx0 = blockIdx.x * TX, z = blockIdx.y, c = blockIdx.z;
   const cx<T> *src = q + ((size_t)c * N + z) * (size_t)g.k * g.kxp + kx0;
-  for (int e = threadIdx.x; e < N * TX; e += NT) {
-    const int idx = idx_of_bin(e / TX, g);
-    buf[e] = idx >= 0 ? src[(size_t)idx * g.kxp + (e % TX)] : czero<T>();
-  }
-  __syncthreads();
-  fft_columns<T, N, TX, NT, +1, true>(buf, twl);
   cx<T> *dst = r + ((size_t)c * N + z) * (size_t)N * g.kxp + kx0;
-  for (int e = threadIdx.x; e < N * TX; e += NT) dst[(size_t)(e / TX) * g.kxp + (e % TX)] = buf[e];
+  fft_io<T, N, TX, NT, +1, true, false>(
+      buf, twl,
+      [&](int, int, int n, int col) {
+        const int idx = idx_of_bin(n, g);
+        return idx >= 0 ? src[(size_t)idx * g.kxp + col] : czero<T>();
+      },
+      [&](int, int, int n, int col, cx<T> v) { dst[(size_t)n * g.kxp + col] = v; });
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -36,25 +36,33 @@ struct StageArgs {
   const double *scalars;  // [0] = the ConstantPower amplitude at this stage's state
 };
 
+// Radix of the first and the last pass of a length-N transform (fft_io's plans).
+template <int N>
+constexpr int first_radix() { return N == 16 ? 4 : 8; }
+template <int N>
+constexpr int last_radix() { return (N == 64 || N == 512) ? 8 : 4; }
+
 // Every spectral operation of one stage (src/Propagator.jl:338-392), fused:
 //   N_u  = P mask FFT_z(PF_u), mean mode zeroed, + A 1_band u        (src/Generator.jl:484-534,
 //                                                                      src/Forcing.jl:1104-1117)
 //   N_phi = -mask i k . FFT_z(PF_flux) - G u_z                        (src/Generator.jl:648-730)
 //   h <- a h + dt N;  q <- E (q + b h);  h <- E h  (not at the last)  (src/Propagator.jl:366-391)
 //   and at the last stage the state Leray projection                  (src/Propagator.jl:401).
-// E = exp(coef |k|^2) is evaluated in FP64 per mode and rounded to T.
+// E = exp(coef |k|^2) is evaluated in FP64 per mode and rounded to T.  A thread owns the modes
+// its last transform pass produces: kz bins n = j + q N/R of column col, for its butterflies
+// (col, j); the product tiles stream from HBM straight into the first pass.
 template <typename T, int N, int TX, int NT, bool SCALAR>
 __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *__restrict__ uh,
                                            cx<T> *__restrict__ hu, cx<T> *__restrict__ ph,
                                            cx<T> *__restrict__ hp, Geo g, const cx<double> *__restrict__ tw,
                                            StageArgs sa) {
   constexpr int M = N / 3, K = 2 * M + 1;
-  constexpr int MODES = K * TX;
-  constexpr int MPT = (MODES + NT - 1) / NT;
+  constexpr int RL = last_radix<N>();
+  using SL = PassShape<N, RL, TX, NT>;
+  constexpr int PER = SL::PER;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-  constexpr int LDS_TILE = N * TX;
-  cx<T> *twl = buf + LDS_TILE;
+  cx<T> *twl = buf + N * TX;
   stage_twiddles<T, N, NT>(twl, tw);
   const int tid = threadIdx.x;
   const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y;
@@ -62,119 +70,122 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
   const T ky = T(g.k0 * my);
   const size_t comp = (size_t)K * K * g.kxp;           // one compact component
   const size_t pencil = (size_t)N * K * g.kxp;         // one PF component
+  const size_t zstride = (size_t)K * g.kxp;
 
-  // velocity products, forward z transform
-  cx<T> F[3][MPT];
+  // velocity products, forward z transform, straight into the owned modes
+  cx<T> F[3][PER][RL];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const cx<T> *src = pf + (size_t)c * pencil + (size_t)ky_idx * g.kxp + kx0;
-    for (int e = tid; e < N * TX; e += NT) buf[e] = src[(size_t)(e / TX) * K * g.kxp + (e % TX)];
-    __syncthreads();
-    fft_columns<T, N, TX, NT, -1, true>(buf, twl);
-#pragma unroll
-    for (int r = 0; r < MPT; ++r) {
-      const int e = tid + NT * r;
-      if (MODES % NT == 0 || e < MODES) F[c][r] = buf[bin_of(e / TX, g) * TX + (e % TX)];
-    }
-    __syncthreads();
+    auto &Fc = F[c];
+    fft_io<T, N, TX, NT, -1, true, false>(
+        buf, twl, [&](int, int, int n, int col) { return src[(size_t)n * zstride + col]; },
+        [&](int p, int q, int, int, cx<T> v) { Fc[p][q] = v; });
   }
 
   const T A = sa.band.on ? T(sa.scalars[0]) : T(0);
   const T a = T(sa.a), b = T(sa.b), dt = T(sa.dt);
-  cx<T> uz_old[MPT];
+  cx<T> uz_old[PER][RL];
 #pragma unroll
-  for (int r = 0; r < MPT; ++r) {
-    const int e = tid + NT * r;
-    if (!(MODES % NT == 0 || e < MODES)) continue;
-    const int col = e % TX, kz_idx = e / TX;
-    const int mx = kx0 + col, mz = signed_of(kz_idx, g);
-    const int m2 = mx * mx + my * my + mz * mz;
-    const T kx = T(g.k0 * mx), kz = T(g.k0 * mz);
-    const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
-    const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
-    const bool keep = in_two_thirds(m2, N) && m2 > 0;
-    cx<T> nl[3];
-    if (keep) {
-      const T k2 = T(k2d);
-      const cx<T> kf = {kx * F[0][r].re + ky * F[1][r].re + kz * F[2][r].re,
-                        kx * F[0][r].im + ky * F[1][r].im + kz * F[2][r].im};
-      const cx<T> s = {kf.re / k2, kf.im / k2};
-      nl[0] = F[0][r] - scale(s, kx);
-      nl[1] = F[1][r] - scale(s, ky);
-      nl[2] = F[2][r] - scale(s, kz);
-    } else {
-      nl[0] = nl[1] = nl[2] = czero<T>();
+  for (int p = 0; p < PER; ++p) {
+    const int bb = tid + p * NT;
+    if (!(SL::TOT % NT == 0 || bb < SL::TOT)) continue;
+    const int col = bb % TX, j = bb / TX;
+    const int mx = kx0 + col;
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+      const int kz_idx = idx_of_bin(j + q * SL::NB, g);
+      if (kz_idx < 0) continue;
+      const int mz = signed_of(kz_idx, g);
+      const int m2 = mx * mx + my * my + mz * mz;
+      const T kx = T(g.k0 * mx), kz = T(g.k0 * mz);
+      const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
+      const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
+      const bool keep = in_two_thirds(m2, N) && m2 > 0;
+      cx<T> nl[3];
+      if (keep) {
+        const T k2 = T(k2d);
+        const cx<T> kf = {kx * F[0][p][q].re + ky * F[1][p][q].re + kz * F[2][p][q].re,
+                          kx * F[0][p][q].im + ky * F[1][p][q].im + kz * F[2][p][q].im};
+        const cx<T> s = {kf.re / k2, kf.im / k2};
+        nl[0] = F[0][p][q] - scale(s, kx);
+        nl[1] = F[1][p][q] - scale(s, ky);
+        nl[2] = F[2][p][q] - scale(s, kz);
+      } else {
+        nl[0] = nl[1] = nl[2] = czero<T>();
+      }
+      const size_t at = (size_t)kz_idx * zstride + (size_t)ky_idx * g.kxp + mx;
+      cx<T> u[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) u[c] = uh[c * comp + at];
+      uz_old[p][q] = u[2];
+      if (in_band(m2, sa.band)) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) nl[c] = nl[c] + scale(u[c], A);
+      }
+      const T E = T(exp(sa.coef_u * k2d));
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        cx<T> h = scale(nl[c], dt);
+        if (sa.stage > 0) h = scale(hu[c * comp + at], a) + h;
+        u[c] = scale(u[c] + scale(h, b), E);
+        if (!sa.last) hu[c * comp + at] = scale(h, E);
+      }
+      if (sa.last && m2 > 0) {  // helmholtz_projection!(u, u): P without the mask
+        const T k2 = T(k2d);
+        const cx<T> kf = {kx * u[0].re + ky * u[1].re + kz * u[2].re, kx * u[0].im + ky * u[1].im + kz * u[2].im};
+        const cx<T> s = {kf.re / k2, kf.im / k2};
+        u[0] = u[0] - scale(s, kx);
+        u[1] = u[1] - scale(s, ky);
+        u[2] = u[2] - scale(s, kz);
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) uh[c * comp + at] = u[c];
     }
-    const size_t at = (size_t)kz_idx * K * g.kxp + (size_t)ky_idx * g.kxp + mx;
-    cx<T> u[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) u[c] = uh[c * comp + at];
-    uz_old[r] = u[2];
-    if (in_band(m2, sa.band)) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) nl[c] = nl[c] + scale(u[c], A);
-    }
-    const T E = T(exp(sa.coef_u * k2d));
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      cx<T> h = scale(nl[c], dt);
-      if (sa.stage > 0) h = scale(hu[c * comp + at], a) + h;
-      u[c] = scale(u[c] + scale(h, b), E);
-      if (!sa.last) hu[c * comp + at] = scale(h, E);
-    }
-    if (sa.last && m2 > 0) {  // helmholtz_projection!(u, u): P without the mask
-      const T k2 = T(k2d);
-      const cx<T> kf = {kx * u[0].re + ky * u[1].re + kz * u[2].re, kx * u[0].im + ky * u[1].im + kz * u[2].im};
-      const cx<T> s = {kf.re / k2, kf.im / k2};
-      u[0] = u[0] - scale(s, kx);
-      u[1] = u[1] - scale(s, ky);
-      u[2] = u[2] - scale(s, kz);
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) uh[c * comp + at] = u[c];
   }
 
   if constexpr (SCALAR) {
-    cx<T> G[MPT];
+    cx<T> G[PER][RL];
 #pragma unroll
-    for (int r = 0; r < MPT; ++r) G[r] = czero<T>();
+    for (int p = 0; p < PER; ++p)
+#pragma unroll
+      for (int q = 0; q < RL; ++q) G[p][q] = czero<T>();
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const cx<T> *src = pf + (size_t)(3 + c) * pencil + (size_t)ky_idx * g.kxp + kx0;
-      for (int e = tid; e < N * TX; e += NT) buf[e] = src[(size_t)(e / TX) * K * g.kxp + (e % TX)];
-      __syncthreads();
-      fft_columns<T, N, TX, NT, -1, true>(buf, twl);
-#pragma unroll
-      for (int r = 0; r < MPT; ++r) {
-        const int e = tid + NT * r;
-        if (MODES % NT == 0 || e < MODES) {
-          const int mc = c == 0 ? kx0 + (e % TX) : (c == 1 ? my : signed_of(e / TX, g));
-          const T kc = T(g.k0 * mc);
-          const cx<T> v = buf[bin_of(e / TX, g) * TX + (e % TX)];
-          G[r] = {G[r].re + kc * v.re, G[r].im + kc * v.im};
-        }
-      }
-      __syncthreads();
+      fft_io<T, N, TX, NT, -1, true, false>(
+          buf, twl, [&](int, int, int n, int col) { return src[(size_t)n * zstride + col]; },
+          [&](int p, int q, int n, int col, cx<T> v) {
+            const int mc = c == 0 ? kx0 + col : (c == 1 ? my : signed_of(idx_of_bin(n, g), g));
+            const T kc = T(g.k0 * mc);
+            G[p][q] = {G[p][q].re + kc * v.re, G[p][q].im + kc * v.im};
+          });
     }
     const T Gm = T(sa.mean_gradient);
 #pragma unroll
-    for (int r = 0; r < MPT; ++r) {
-      const int e = tid + NT * r;
-      if (!(MODES % NT == 0 || e < MODES)) continue;
-      const int col = e % TX, kz_idx = e / TX;
-      const int mx = kx0 + col, mz = signed_of(kz_idx, g);
-      const int m2 = mx * mx + my * my + mz * mz;
-      const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
-      const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
-      cx<T> nl = in_two_thirds(m2, N) ? cx<T>{G[r].im, -G[r].re} : czero<T>();   // -i k.F
-      nl = nl - scale(uz_old[r], Gm);
-      const size_t at = (size_t)kz_idx * K * g.kxp + (size_t)ky_idx * g.kxp + mx;
-      const T E = T(exp(sa.coef_p * k2d));
-      cx<T> h = scale(nl, dt);
-      if (sa.stage > 0) h = scale(hp[at], a) + h;
-      const cx<T> q = scale(ph[at] + scale(h, b), E);
-      ph[at] = q;
-      if (!sa.last) hp[at] = scale(h, E);
+    for (int p = 0; p < PER; ++p) {
+      const int bb = tid + p * NT;
+      if (!(SL::TOT % NT == 0 || bb < SL::TOT)) continue;
+      const int col = bb % TX, j = bb / TX;
+      const int mx = kx0 + col;
+#pragma unroll
+      for (int q = 0; q < RL; ++q) {
+        const int kz_idx = idx_of_bin(j + q * SL::NB, g);
+        if (kz_idx < 0) continue;
+        const int mz = signed_of(kz_idx, g);
+        const int m2 = mx * mx + my * my + mz * mz;
+        const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
+        const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
+        cx<T> nl = in_two_thirds(m2, N) ? cx<T>{G[p][q].im, -G[p][q].re} : czero<T>();   // -i k.F
+        nl = nl - scale(uz_old[p][q], Gm);
+        const size_t at = (size_t)kz_idx * zstride + (size_t)ky_idx * g.kxp + mx;
+        const T E = T(exp(sa.coef_p * k2d));
+        cx<T> h = scale(nl, dt);
+        if (sa.stage > 0) h = scale(hp[at], a) + h;
+        const cx<T> qn = scale(ph[at] + scale(h, b), E);
+        ph[at] = qn;
+        if (!sa.last) hp[at] = scale(h, E);
+      }
     }
   }
 }
@@ -184,7 +195,9 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
 // [0] sum w|u|^2, [1] band sum w|u|^2, [2] sum w |k|^2 |u|^2, [3] sum w|phi|^2, [4] sum w |k|^2 |phi|^2
 // (w the Parseval weight of the half spectrum).  This is to_physical! + vorticity_physical!'s
 // spectral half (src/Propagator.jl:349-363, src/Observables.jl:329-333) and the state side of
-// weighted_inner (src/Forcing.jl:1109, src/Observables.jl:150-173).
+// weighted_inner (src/Forcing.jl:1109, src/Observables.jl:150-173).  A thread owns the modes its
+// first transform pass reads (kz bins n = j + q N/R of column col) and streams the transformed
+// columns from the last pass straight to HBM.
 constexpr int NRED = 5;
 
 template <typename T, int N, int TX, int NT, bool SCALAR>
@@ -192,13 +205,13 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
                                            cx<T> *__restrict__ q, Geo g, const cx<double> *__restrict__ tw,
                                            BandSpec band, double *__restrict__ partials) {
   constexpr int M = N / 3, K = 2 * M + 1;
-  constexpr int MODES = K * TX;
-  constexpr int MPT = (MODES + NT - 1) / NT;
+  constexpr int R1 = first_radix<N>();
+  using S1 = PassShape<N, R1, TX, NT>;
+  constexpr int PER = S1::PER;
   constexpr int NCIN = SCALAR ? 7 : 6;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-  constexpr int LDS_TILE = N * TX;
-  cx<T> *twl = buf + LDS_TILE;
+  cx<T> *twl = buf + N * TX;
   stage_twiddles<T, N, NT>(twl, tw);
   __shared__ double red[NT / 64][NRED];
   const int tid = threadIdx.x;
@@ -206,68 +219,75 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
   const int my = signed_of(ky_idx, g);
   const size_t comp = (size_t)K * K * g.kxp;
   const size_t pencil = (size_t)N * K * g.kxp;
+  const size_t zstride = (size_t)K * g.kxp;
   const T inv = T(1.0 / ((double)N * N * N));
 
-  cx<T> u[3][MPT];
-  cx<T> p[MPT];
+  cx<T> u[3][PER][R1];
+  cx<T> p[PER][R1];
   double acc[NRED] = {0, 0, 0, 0, 0};
 #pragma unroll
-  for (int r = 0; r < MPT; ++r) {
-    const int e = tid + NT * r;
-    if (!(MODES % NT == 0 || e < MODES)) continue;
-    const int col = e % TX, kz_idx = e / TX;
-    const int mx = kx0 + col, mz = signed_of(kz_idx, g);
-    const size_t at = (size_t)kz_idx * K * g.kxp + (size_t)ky_idx * g.kxp + mx;
-    const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
-    const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
-    const double w = mx == 0 ? 1.0 : 2.0;
-    double e2 = 0.0;
+  for (int pp = 0; pp < PER; ++pp) {
+    const int bb = tid + pp * NT;
+    const bool live = S1::TOT % NT == 0 || bb < S1::TOT;
+    const int col = bb % TX, j = bb / TX;
+    const int mx = kx0 + col;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      u[c][r] = uh[c * comp + at];
-      e2 += (double)u[c][r].re * (double)u[c][r].re + (double)u[c][r].im * (double)u[c][r].im;
-    }
-    acc[0] += w * e2;
-    if (in_band(mx * mx + my * my + mz * mz, band)) acc[1] += w * e2;
-    acc[2] += w * k2d * e2;
-    if constexpr (SCALAR) {
-      p[r] = ph[at];
-      const double pp = (double)p[r].re * (double)p[r].re + (double)p[r].im * (double)p[r].im;
-      acc[3] += w * pp;
-      acc[4] += w * k2d * pp;
+    for (int qq = 0; qq < R1; ++qq) {
+      const int kz_idx = live ? idx_of_bin(j + qq * S1::NB, g) : -1;
+      if (kz_idx < 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[c][pp][qq] = czero<T>();
+        p[pp][qq] = czero<T>();
+        continue;
+      }
+      const int mz = signed_of(kz_idx, g);
+      const size_t at = (size_t)kz_idx * zstride + (size_t)ky_idx * g.kxp + mx;
+      const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
+      const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
+      const double w = mx == 0 ? 1.0 : 2.0;
+      double e2 = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        u[c][pp][qq] = uh[c * comp + at];
+        e2 += (double)u[c][pp][qq].re * (double)u[c][pp][qq].re + (double)u[c][pp][qq].im * (double)u[c][pp][qq].im;
+      }
+      acc[0] += w * e2;
+      if (in_band(mx * mx + my * my + mz * mz, band)) acc[1] += w * e2;
+      acc[2] += w * k2d * e2;
+      if constexpr (SCALAR) {
+        p[pp][qq] = ph[at];
+        const double v2 = (double)p[pp][qq].re * (double)p[pp][qq].re + (double)p[pp][qq].im * (double)p[pp][qq].im;
+        acc[3] += w * v2;
+        acc[4] += w * k2d * v2;
+      } else {
+        p[pp][qq] = czero<T>();
+      }
     }
   }
 
 #pragma unroll
   for (int c = 0; c < NCIN; ++c) {
-    // the gap between the retained bins is zero
-    for (int e = tid; e < (N - K) * TX; e += NT) buf[(M + 1 + e / TX) * TX + (e % TX)] = czero<T>();
-#pragma unroll
-    for (int r = 0; r < MPT; ++r) {
-      const int e = tid + NT * r;
-      if (!(MODES % NT == 0 || e < MODES)) continue;
-      const int col = e % TX, kz_idx = e / TX;
-      cx<T> v;
-      if (c < 3) {
-        v = u[c][r];
-      } else if (c < 6) {  // i k x u
-        const int mx = kx0 + col, mz = signed_of(kz_idx, g);
-        const T kx = T(g.k0 * mx), ky = T(g.k0 * my), kz = T(g.k0 * mz);
-        cx<T> t;
-        if (c == 3) t = scale(u[2][r], ky) - scale(u[1][r], kz);
-        else if (c == 4) t = scale(u[0][r], kz) - scale(u[2][r], kx);
-        else t = scale(u[1][r], kx) - scale(u[0][r], ky);
-        v = {-t.im, t.re};
-      } else {
-        v = p[r];
-      }
-      buf[bin_of(kz_idx, g) * TX + col] = scale(v, inv);
-    }
-    __syncthreads();
-    fft_columns<T, N, TX, NT, +1, true>(buf, twl);
     cx<T> *dst = q + (size_t)c * pencil + (size_t)ky_idx * g.kxp + kx0;
-    for (int e = tid; e < N * TX; e += NT) dst[(size_t)(e / TX) * K * g.kxp + (e % TX)] = buf[e];
-    __syncthreads();
+    fft_io<T, N, TX, NT, +1, true, false>(
+        buf, twl,
+        [&](int pp, int qq, int n, int col) {
+          cx<T> v;
+          if (c < 3) {
+            v = u[c][pp][qq];
+          } else if (c < 6) {  // i k x u
+            const int mx = kx0 + col, mz = signed_of(idx_of_bin(n, g), g);
+            const T kx = T(g.k0 * mx), ky = T(g.k0 * my), kz = T(g.k0 * mz);
+            cx<T> t;
+            if (c == 3) t = scale(u[2][pp][qq], ky) - scale(u[1][pp][qq], kz);
+            else if (c == 4) t = scale(u[0][pp][qq], kz) - scale(u[2][pp][qq], kx);
+            else t = scale(u[1][pp][qq], kx) - scale(u[0][pp][qq], ky);
+            v = {-t.im, t.re};
+          } else {
+            v = p[pp][qq];
+          }
+          return scale(v, inv);
+        },
+        [&](int, int, int n, int col, cx<T> v) { dst[(size_t)n * zstride + col] = v; });
   }
 
   // deterministic block reduction: wave shuffles, then the waves in order
