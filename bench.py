@@ -82,6 +82,7 @@ def time_gpu(g, n: int, dtype, steps: int, warmup: int, H, seed: int = 2026):
     with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=1.0,
                       field_integration_scheme=g.IFLowStorageRK4, keep_physical=False) as sim:
         del u, p
+        tuned = sim.autotune(reps=2)
         for _ in range(warmup):
             sim.step_fields(dt)
         sim.synchronize()
@@ -112,7 +113,8 @@ def time_gpu(g, n: int, dtype, steps: int, warmup: int, H, seed: int = 2026):
     if not (math.isfinite(speed) and math.isfinite(red["scalar_variance"])):
         raise RuntimeError("the benchmark run blew up")
     return {"ms_total": float(ms.value), "wall_s": wall, "kernel_ms": kms, "kernel_launches": kcnt,
-            "max_speed": speed, "kinetic_energy": red["kinetic_energy"], "memory_bytes": mem, "nu": nu}
+            "max_speed": speed, "kinetic_energy": red["kinetic_energy"], "memory_bytes": mem, "nu": nu,
+            "schedule": tuned}
 
 
 def kernel_table(res: dict, n: int, real_bytes: int, stages: int) -> dict:
@@ -218,6 +220,7 @@ def main() -> int:
         "roofline": roofline, "cpu_baseline": cpu, "fp64": fp64, "kernels": k32,
         "wall_ms_per_step": round(r32["wall_s"] * 1e3 / args.steps, 3),
         "memory_GiB": round(r32["memory_bytes"] / 2 ** 30, 2),
+        "autotuned_schedule": r32["schedule"],
     }
     print(json.dumps(line), flush=True)
     return 0

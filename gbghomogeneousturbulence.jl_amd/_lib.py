@@ -71,6 +71,8 @@ _SIGS = {
     "hdns_frame_tables": [c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32), POINTER(c_uint8)],
     "hdns_scheme_coefficients": [c_int32, c_int32, _D, _D, _D, POINTER(c_int32)],
     "hdns_heun_weights": [c_double, _D],
+    "hdns_autotune": [c_void_p, c_int32, _D, POINTER(c_int32)],
+    "hdns_schedule": [c_void_p, POINTER(c_int32), POINTER(c_int32)],
     "hdns_profile_enable": [c_void_p, c_int32],
     "hdns_profile_read": [c_void_p, _D, POINTER(c_int64), c_int32],
     "hdns_synchronize": [c_void_p],

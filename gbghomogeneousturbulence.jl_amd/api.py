@@ -401,6 +401,23 @@ class Simulation:
         self._call("hdns_stream", ctypes.byref(s))
         return s.value or 0
 
+    def autotune(self, reps: int = 3) -> dict:
+        """Time both launch schedules of every stage kernel family and keep the fastest (the state is
+        left bitwise unchanged); returns {family: (ms_variant0, ms_variant1, chosen)}."""
+        ms = np.zeros(6)
+        chosen = np.zeros(3, dtype=np.int32)
+        self._call("hdns_autotune", int(reps), L.dptr(ms), chosen.ctypes.data_as(ctypes.POINTER(c_int32)))
+        return {name: (float(ms[2 * i]), float(ms[2 * i + 1]), int(chosen[i]))
+                for i, name in enumerate(("y_passes", "z_passes", "x_pass"))}
+
+    def schedule(self, set_to=None):
+        """Read (and optionally set) the launch schedule: three entries (y, z, x) of 0 or 1."""
+        get = np.zeros(3, dtype=np.int32)
+        P = ctypes.POINTER(c_int32)
+        s = None if set_to is None else np.ascontiguousarray(set_to, dtype=np.int32)
+        self._call("hdns_schedule", s.ctypes.data_as(P) if s is not None else None, get.ctypes.data_as(P))
+        return tuple(int(v) for v in get)
+
     def profile(self, on: bool) -> None:
         """Time every kernel launch with HIP events on the context's stream."""
         self._call("hdns_profile_enable", int(on))

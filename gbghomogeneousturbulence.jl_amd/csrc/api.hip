@@ -863,6 +863,87 @@ hdns_status hdns_frame_tables(hdns_ctx *ctx, int32_t *mx, int32_t *my, int32_t *
   return HDNS_OK;
 }
 
+hdns_status hdns_autotune(hdns_ctx *ctx, int32_t reps, double *ms, int32_t *chosen) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  Ctx &c = ctx->c;
+  if (!c.deposited) return fail(ctx, HDNS_ERR_STATE, "autotune needs an adopted state: upload one first");
+  if (reps < 1) reps = 3;
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  const bool prof = c.prof;
+  c.prof = false;
+  hipEvent_t e0, e1;
+  HIP_TRY(ctx, hipEventCreate(&e0));
+  HIP_TRY(ctx, hipEventCreate(&e1));
+  // A stage-shaped sequence that leaves the state bitwise unchanged: the z forward pass runs with
+  // dt = 0 and a unit exponential (q <- 1 * (q + b * 0)); every other kernel writes only
+  // registers the closing deposit rewrites.
+  StageArgs idle{};
+  idle.stage = 0;
+  idle.last = 0;
+  idle.b = 1.0;
+  idle.scalars = c.scalars;
+  XArgs xa{};
+  xa.phys = c.phys;
+  xa.physw = c.physw;
+  xa.max_bits = c.maxbits;
+  const int saved[3] = {c.sched[0], c.sched[1], c.sched[2]};
+  double best[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  for (int fam = 0; fam < 3; ++fam) {
+    for (int v = 0; v < 2; ++v) {
+      c.sched[0] = saved[0];
+      c.sched[1] = saved[1];
+      c.sched[2] = saved[2];
+      c.sched[fam] = v;
+      double total = 0.0;
+      for (int r = 0; r <= reps; ++r) {  // the first round warms up
+        float t[5];
+        for (int k = 0; k < 5; ++k) {
+          (void)hipEventRecord(e0, c.stream);
+          switch (k) {
+            case 0: c.pipe->zi(c); break;
+            case 1: c.pipe->yinv(c); break;
+            case 2: c.pipe->xpass(c, xa); break;
+            case 3: c.pipe->yfwd(c); break;
+            default: c.pipe->zf(c, idle); break;
+          }
+          (void)hipEventRecord(e1, c.stream);
+          HIP_TRY(ctx, hipEventSynchronize(e1));
+          (void)hipEventElapsedTime(&t[k], e0, e1);
+        }
+        if (r == 0) continue;
+        total += fam == 0 ? t[1] + t[3] : fam == 1 ? t[0] + t[4] : t[2];
+      }
+      best[fam][v] = total / reps;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (int fam = 0; fam < 3; ++fam) c.sched[fam] = best[fam][1] < best[fam][0] ? 1 : 0;
+  c.prof = prof;
+  if (ms)
+    for (int i = 0; i < 6; ++i) ms[i] = best[i / 2][i % 2];
+  if (chosen)
+    for (int i = 0; i < 3; ++i) chosen[i] = c.sched[i];
+  hdns_status s = launched(ctx);
+  if (s != HDNS_OK) return s;
+  drop_graph(c);  // a captured step holds the old schedule
+  return run_deposit(ctx);
+}
+
+hdns_status hdns_schedule(hdns_ctx *ctx, const int32_t *set, int32_t *get) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  Ctx &c = ctx->c;
+  if (set) {
+    for (int i = 0; i < 3; ++i)
+      if (set[i] != 0 && set[i] != 1) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "schedule entries are 0 or 1");
+    for (int i = 0; i < 3; ++i) c.sched[i] = set[i];
+    drop_graph(c);
+  }
+  if (get)
+    for (int i = 0; i < 3; ++i) get[i] = c.sched[i];
+  return HDNS_OK;
+}
+
 hdns_status hdns_profile_enable(hdns_ctx *ctx, int32_t on) {
   if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
   Ctx &c = ctx->c;

@@ -41,6 +41,9 @@ struct Ctx {
   size_t comp = 0;          // complex elements of one compact component
   size_t buf_a = 0, buf_b = 0;
   int stages = 0;
+  // launch schedule of the y, z and x passes (index into the two variants of each), set by
+  // hdns_autotune; the defaults are the fastest measured at 512^3 on MI355X
+  int sched[3] = {1, 1, 0};
   BandSpec band{};
   hipStream_t stream = nullptr;
   void *uh = nullptr, *hu = nullptr, *ph = nullptr, *hp = nullptr;

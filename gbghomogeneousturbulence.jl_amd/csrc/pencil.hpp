@@ -80,7 +80,7 @@ __global__ __launch_bounds__(NT) void k_yinv(const cx<T> *__restrict__ q, cx<T> 
 }
 
 // ---------------------------------------------------------------------------------------------
-// x pass: LPR lanes own one pair of adjacent y rows of one z plane (512 threads per block).  Two
+// x pass: LPR lanes own one pair of adjacent y rows of one z plane (XT threads per block).  Two
 // real rows travel as one complex transform (row a in the real part, row b in the imaginary
 // part), so each component costs one complex FFT per row pair.  At FP32 and N <= 512 a single
 // wavefront owns a row pair and synchronises with wave barriers only; wider transforms spread over
@@ -97,33 +97,25 @@ struct XArgs {
   unsigned long long *max_bits;  // max |u|^2 as the bits of a non-negative double
 };
 
-constexpr int XPASS_THREADS = 512;
-
 // lanes per row pair: a lane holds N/LPR values of each component
-template <typename T, int N>
+template <typename T, int N, int XT>
 constexpr int xpass_lanes() {
   constexpr int e = sizeof(T) == 4 ? 8 : 4;
   constexpr int l = N / e;
-  return l < 64 ? 64 : (l > XPASS_THREADS ? XPASS_THREADS : l);
+  return l < 64 ? 64 : (l > XT ? XT : l);
 }
 
-template <typename T, int N, int NCIN>
-__global__ __launch_bounds__(XPASS_THREADS) void k_xpass(const cx<T> *__restrict__ rin, cx<T> *__restrict__ pxo,
-                                                         Geo g, const cx<double> *__restrict__ tw, XArgs xa) {
-  constexpr int LPR = xpass_lanes<T, N>();
+template <typename T, int N, int NCIN, int XT>
+__global__ __launch_bounds__(XT) void k_xpass(const cx<T> *__restrict__ rin,
+                                                              cx<T> *__restrict__ pxo, Geo g,
+                                                              const cx<double> *__restrict__ tw, XArgs xa) {
+  constexpr int LPR = xpass_lanes<T, N, XT>();
   constexpr bool WAVE = LPR == 64;
-  constexpr int NT = XPASS_THREADS;
+  constexpr int NT = XT;
   constexpr int XR = NT / LPR;                     // row pairs per block
+  constexpr int E = (N + LPR - 1) / LPR;           // physical elements per lane per row pair
   constexpr int M = N / 3;
   constexpr int PF = (M + 1 + LPR - 1) / LPR;      // retained coefficients per lane per row
-  // the physical values a lane holds are the outputs of the inverse transform's last pass
-  constexpr int RL = N == 16 ? 4 : ((N == 64 || N == 512) ? 8 : 4);
-  constexpr int R1 = N == 16 ? 4 : 8;
-  using SL = PassShape<N, RL, 1, LPR>;
-  using S1 = PassShape<N, R1, 1, LPR>;
-  constexpr int P = SL::PER;
-  // the forward first pass reads exactly these registers when both passes share a shape
-  constexpr bool DIRECT = RL == R1;
   static_assert(NCIN == 7 || NCIN == 6, "u, w and optionally phi");
   constexpr int NCOUT = NCIN == 7 ? 6 : 3;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -135,11 +127,11 @@ __global__ __launch_bounds__(XPASS_THREADS) void k_xpass(const cx<T> *__restrict
 
   const int rows_per_plane = N / (2 * XR);
   const int z = blockIdx.x / rows_per_plane;
-  const int lane = threadIdx.x % LPR;
-  const int ya = (blockIdx.x % rows_per_plane) * (2 * XR) + 2 * (threadIdx.x / LPR);
+  const int w = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+  const int ya = (blockIdx.x % rows_per_plane) * (2 * XR) + 2 * w;
   const size_t plane = (size_t)N * g.kxp;           // one z plane of one component in R / PX
   const size_t vol = (size_t)N * N * N;
-  cx<T> *wb = buf + (threadIdx.x / LPR) * N;       // this row pair's transform buffer
+  cx<T> *wb = buf + w * N;                         // this row pair's transform buffer
 
   cx<T> pa[PF], pb[PF];
   auto fetch = [&](int c) {
@@ -153,13 +145,8 @@ __global__ __launch_bounds__(XPASS_THREADS) void k_xpass(const cx<T> *__restrict
       }
     }
   };
-  // x of the value a lane holds at (p, q), or -1
-  auto x_of = [&](int p, int q) {
-    const int b = lane + p * LPR;
-    return (SL::TOT % LPR == 0 || b < SL::TOT) ? b + q * SL::NB : -1;
-  };
 
-  cx<T> f[NCIN][P][RL];
+  cx<T> f[NCIN][E];
   fetch(0);
 #pragma unroll
   for (int c = 0; c < NCIN; ++c) {
@@ -179,86 +166,75 @@ __global__ __launch_bounds__(XPASS_THREADS) void k_xpass(const cx<T> *__restrict
     }
     if (c + 1 < NCIN) fetch(c + 1);
     pass_sync<WAVE>();
-    auto &fc = f[c];
-    fft_io<T, N, 1, LPR, +1, false, WAVE, true>(
-        wb, twl, [&](int, int, int n, int) { return wb[n]; },
-        [&](int p, int q, int, int, cx<T> v) { fc[p][q] = v; });
+    fft_columns<T, N, 1, LPR, +1, false, WAVE>(wb, twl);
+#pragma unroll
+    for (int q = 0; q < E; ++q)
+      if (lane + LPR * q < N) f[c][q] = wb[lane + LPR * q];
+    pass_sync<WAVE>();
   }
 
   // physical space: (re, im) = (row a, row b) of every component
   double mx = 0.0;
-  const size_t rowa = ((size_t)z * N + ya) * N, rowb = rowa + N;
-  T *phs = reinterpret_cast<T *>(xa.phys);
-  T *pws = reinterpret_cast<T *>(xa.physw);
+  if (xa.want_max) {
 #pragma unroll
-  for (int p = 0; p < P; ++p) {
+    for (int q = 0; q < E; ++q) {
+      if (lane + LPR * q >= N) continue;
+      const T sa = f[0][q].re * f[0][q].re + f[1][q].re * f[1][q].re + f[2][q].re * f[2][q].re;
+      const T sb = f[0][q].im * f[0][q].im + f[1][q].im * f[1][q].im + f[2][q].im * f[2][q].im;
+      const double da = (double)sa, db = (double)sb;
+      if (da > mx || da != da) mx = da;
+      if (db > mx || db != db) mx = db;
+    }
+  }
+  if (xa.store_u || xa.store_phi || xa.store_w) {
+    const size_t rowa = ((size_t)z * N + ya) * N, rowb = rowa + N;
+    T *ph = reinterpret_cast<T *>(xa.phys);
+    T *pw = reinterpret_cast<T *>(xa.physw);
 #pragma unroll
-    for (int q = 0; q < RL; ++q) {
-      const int x = x_of(p, q);
-      if (x < 0) continue;
-      if (xa.want_max) {
-        const T sa = f[0][p][q].re * f[0][p][q].re + f[1][p][q].re * f[1][p][q].re + f[2][p][q].re * f[2][p][q].re;
-        const T sb = f[0][p][q].im * f[0][p][q].im + f[1][p][q].im * f[1][p][q].im + f[2][p][q].im * f[2][p][q].im;
-        const double da = (double)sa, db = (double)sb;
-        if (da > mx || da != da) mx = da;
-        if (db > mx || db != db) mx = db;
-      }
+    for (int q = 0; q < E; ++q) {
+      const int x = lane + LPR * q;
+      if (x >= N) continue;
       if (xa.store_u) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          phs[c * vol + rowa + x] = f[c][p][q].re;
-          phs[c * vol + rowb + x] = f[c][p][q].im;
+          ph[c * vol + rowa + x] = f[c][q].re;
+          ph[c * vol + rowb + x] = f[c][q].im;
         }
       }
       if constexpr (NCIN == 7) {
         if (xa.store_phi) {
-          phs[3 * vol + rowa + x] = f[NCIN - 1][p][q].re;
-          phs[3 * vol + rowb + x] = f[NCIN - 1][p][q].im;
+          ph[3 * vol + rowa + x] = f[NCIN - 1][q].re;
+          ph[3 * vol + rowb + x] = f[NCIN - 1][q].im;
         }
       }
       if (xa.store_w) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          pws[c * vol + rowa + x] = f[3 + c][p][q].re;
-          pws[c * vol + rowb + x] = f[3 + c][p][q].im;
+          pw[c * vol + rowa + x] = f[3 + c][q].re;
+          pw[c * vol + rowb + x] = f[3 + c][q].im;
         }
       }
     }
   }
 
-  // product d at (p, q): the negated Lamb vector u x w (cross!(w, u, w), src/Generator.jl:529),
-  // then the flux u phi (multiply!(u, phi, u), src/Generator.jl:727)
-  auto product = [&](int d, int p, int q) {
-    cx<T> v;
-    if (d < 3) {
-      const int c1 = (d + 1) % 3, c2 = (d + 2) % 3;
-      v.re = f[c1][p][q].re * f[3 + c2][p][q].re - f[c2][p][q].re * f[3 + c1][p][q].re;
-      v.im = f[c1][p][q].im * f[3 + c2][p][q].im - f[c2][p][q].im * f[3 + c1][p][q].im;
-    } else {
-      v.re = f[d - 3][p][q].re * f[NCIN - 1][p][q].re;
-      v.im = f[d - 3][p][q].im * f[NCIN - 1][p][q].im;
-    }
-    return v;
-  };
-
 #pragma unroll
   for (int d = 0; d < NCOUT; ++d) {
-    auto store_lds = [&](int, int, int n, int, cx<T> v) { wb[n] = v; };
-    if constexpr (DIRECT) {
-      fft_io<T, N, 1, LPR, -1, false, WAVE, false>(
-          wb, twl, [&](int p, int q, int, int) { return product(d, p, q); }, store_lds);
-    } else {
 #pragma unroll
-      for (int p = 0; p < P; ++p)
-#pragma unroll
-        for (int q = 0; q < RL; ++q) {
-          const int x = x_of(p, q);
-          if (x >= 0) wb[x] = product(d, p, q);
-        }
-      pass_sync<WAVE>();
-      fft_io<T, N, 1, LPR, -1, false, WAVE, true>(
-          wb, twl, [&](int, int, int n, int) { return wb[n]; }, store_lds);
+    for (int q = 0; q < E; ++q) {
+      if (lane + LPR * q >= N) continue;
+      cx<T> v;
+      if (d < 3) {  // the negated Lamb vector u x w (cross!(w, u, w), src/Generator.jl:529)
+        const int c1 = (d + 1) % 3, c2 = (d + 2) % 3;
+        v.re = f[c1][q].re * f[3 + c2][q].re - f[c2][q].re * f[3 + c1][q].re;
+        v.im = f[c1][q].im * f[3 + c2][q].im - f[c2][q].im * f[3 + c1][q].im;
+      } else {      // the flux u phi (multiply!(u, phi, u), src/Generator.jl:727)
+        v.re = f[d - 3][q].re * f[NCIN - 1][q].re;
+        v.im = f[d - 3][q].im * f[NCIN - 1][q].im;
+      }
+      wb[lane + LPR * q] = v;
     }
+    pass_sync<WAVE>();
+    fft_columns<T, N, 1, LPR, -1, false, WAVE>(wb, twl);
     cx<T> *dst = pxo + (size_t)d * N * plane + (size_t)z * plane + (size_t)ya * g.kxp;
     for (int kx = lane; kx < g.kxp; kx += LPR) {
       cx<T> A = czero<T>(), B = czero<T>();
@@ -283,10 +259,10 @@ __global__ __launch_bounds__(XPASS_THREADS) void k_xpass(const cx<T> *__restrict
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
     __syncthreads();
     if (threadIdx.x == 0) {
-      double bmax = red[0];
+      double b = red[0];
       for (int i = 1; i < NT / 64; ++i)
-        if (red[i] > bmax || red[i] != red[i]) bmax = red[i];
-      atomicMax(xa.max_bits, (unsigned long long)__double_as_longlong(bmax));
+        if (red[i] > b || red[i] != red[i]) b = red[i];
+      atomicMax(xa.max_bits, (unsigned long long)__double_as_longlong(b));
     }
   }
 }

@@ -11,40 +11,56 @@ namespace hdns {
 template <typename T>
 constexpr int tile_width_of() { return sizeof(T) == 4 ? 16 : 8; }
 
+// Two launch schedules per stage kernel, chosen per context by hdns_autotune:
+//   y passes: 256 or 512 threads per block;  z passes: 512 or 1024;  x pass: 256 or 512.
+constexpr int kYThreads[2] = {256, 512};
+constexpr int kZThreads[2] = {512, 1024};
+constexpr int kXThreads[2] = {256, 512};
+
 template <typename T, int N>
 struct PipelineImpl final : Pipeline {
   static constexpr int TX = tile_width_of<T>();
   static constexpr int M = N / 3, K = 2 * M + 1;
-  static constexpr int NTY = 512;
-  static constexpr int NTZ = N >= 512 ? 1024 : 512;
   static constexpr size_t LDS_COL = ((size_t)N * TX + N) * sizeof(cx<T>);   // tile + twiddles
-  static constexpr int XR = XPASS_THREADS / xpass_lanes<T, N>();
-  static constexpr size_t LDS_X = ((size_t)XR * N + N) * sizeof(cx<T>);
+  template <int XT>
+  static constexpr int xr() { return XT / xpass_lanes<T, N, XT>(); }
+  template <int XT>
+  static constexpr size_t lds_x() { return ((size_t)xr<XT>() * N + N) * sizeof(cx<T>); }
 
   static void set_lds(const void *fn, size_t bytes) {
     (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   }
 
+  template <int V>
+  static void set_all() {
+    set_lds((const void *)k_yfwd<T, N, TX, kYThreads[V]>, LDS_COL);
+    set_lds((const void *)k_yinv<T, N, TX, kYThreads[V]>, LDS_COL);
+    set_lds((const void *)k_zf<T, N, TX, kZThreads[V], true>, LDS_COL);
+    set_lds((const void *)k_zf<T, N, TX, kZThreads[V], false>, LDS_COL);
+    set_lds((const void *)k_zi<T, N, TX, kZThreads[V], true>, LDS_COL);
+    set_lds((const void *)k_zi<T, N, TX, kZThreads[V], false>, LDS_COL);
+    set_lds((const void *)k_xpass<T, N, 7, kXThreads[V]>, lds_x<kXThreads[V]>());
+    set_lds((const void *)k_xpass<T, N, 6, kXThreads[V]>, lds_x<kXThreads[V]>());
+  }
+
   PipelineImpl() {
-    set_lds((const void *)k_yfwd<T, N, TX, NTY>, LDS_COL);
-    set_lds((const void *)k_yinv<T, N, TX, NTY>, LDS_COL);
-    set_lds((const void *)k_zf<T, N, TX, NTZ, true>, LDS_COL);
-    set_lds((const void *)k_zf<T, N, TX, NTZ, false>, LDS_COL);
-    set_lds((const void *)k_zi<T, N, TX, NTZ, true>, LDS_COL);
-    set_lds((const void *)k_zi<T, N, TX, NTZ, false>, LDS_COL);
-    set_lds((const void *)k_xpass<T, N, 7>, LDS_X);
-    set_lds((const void *)k_xpass<T, N, 6>, LDS_X);
+    set_all<0>();
+    set_all<1>();
   }
 
   int tiles(const Ctx &c) const { return c.geo.kxp / TX; }
   int tile_width() const override { return TX; }
 
-  void yfwd(Ctx &c) override {
+  template <int NTY>
+  void yfwd_v(Ctx &c) {
     const int nc = c.cfg.has_scalar ? 6 : 3;
     hipLaunchKernelGGL((k_yfwd<T, N, TX, NTY>), dim3(tiles(c), N, nc), dim3(NTY), LDS_COL, c.stream,
                        (const cx<T> *)c.bufa, (cx<T> *)c.bufb, c.geo, (const cx<double> *)c.tw);
   }
-  void zf(Ctx &c, const StageArgs &sa) override {
+  void yfwd(Ctx &c) override { c.sched[0] ? yfwd_v<kYThreads[1]>(c) : yfwd_v<kYThreads[0]>(c); }
+
+  template <int NTZ>
+  void zf_v(Ctx &c, const StageArgs &sa) {
     const dim3 grid(tiles(c), K);
     if (c.cfg.has_scalar)
       hipLaunchKernelGGL((k_zf<T, N, TX, NTZ, true>), grid, dim3(NTZ), LDS_COL, c.stream, (const cx<T> *)c.bufb,
@@ -55,12 +71,18 @@ struct PipelineImpl final : Pipeline {
                          (cx<T> *)c.uh, (cx<T> *)c.hu, (cx<T> *)c.ph, (cx<T> *)c.hp, c.geo,
                          (const cx<double> *)c.tw, sa);
   }
+  void zf(Ctx &c, const StageArgs &sa) override {
+    c.sched[1] ? zf_v<kZThreads[1]>(c, sa) : zf_v<kZThreads[0]>(c, sa);
+  }
+
   void reality(Ctx &c) override {
     const int n = K * K;
     hipLaunchKernelGGL((k_reality<T>), dim3((n + 255) / 256), dim3(256), 0, c.stream, (cx<T> *)c.uh, (cx<T> *)c.ph,
                        c.geo, c.cfg.has_scalar ? 4 : 3);
   }
-  void zi(Ctx &c) override {
+
+  template <int NTZ>
+  void zi_v(Ctx &c) {
     const dim3 grid(tiles(c), K);
     if (c.cfg.has_scalar)
       hipLaunchKernelGGL((k_zi<T, N, TX, NTZ, true>), grid, dim3(NTZ), LDS_COL, c.stream, (const cx<T> *)c.uh,
@@ -69,20 +91,30 @@ struct PipelineImpl final : Pipeline {
       hipLaunchKernelGGL((k_zi<T, N, TX, NTZ, false>), grid, dim3(NTZ), LDS_COL, c.stream, (const cx<T> *)c.uh,
                          (const cx<T> *)c.ph, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, c.band, c.partials);
   }
-  void yinv(Ctx &c) override {
+  void zi(Ctx &c) override { c.sched[1] ? zi_v<kZThreads[1]>(c) : zi_v<kZThreads[0]>(c); }
+
+  template <int NTY>
+  void yinv_v(Ctx &c) {
     const int nc = c.cfg.has_scalar ? 7 : 6;
     hipLaunchKernelGGL((k_yinv<T, N, TX, NTY>), dim3(tiles(c), N, nc), dim3(NTY), LDS_COL, c.stream,
                        (const cx<T> *)c.bufa, (cx<T> *)c.bufb, c.geo, (const cx<double> *)c.tw);
   }
-  void xpass(Ctx &c, const XArgs &xa) override {
-    const int blocks = N * N / (2 * XR);
+  void yinv(Ctx &c) override { c.sched[0] ? yinv_v<kYThreads[1]>(c) : yinv_v<kYThreads[0]>(c); }
+
+  template <int XT>
+  void xpass_v(Ctx &c, const XArgs &xa) {
+    const int blocks = N * N / (2 * xr<XT>());
     if (c.cfg.has_scalar)
-      hipLaunchKernelGGL((k_xpass<T, N, 7>), dim3(blocks), dim3(XPASS_THREADS), LDS_X, c.stream, (const cx<T> *)c.bufb,
-                         (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
+      hipLaunchKernelGGL((k_xpass<T, N, 7, XT>), dim3(blocks), dim3(XT), lds_x<XT>(), c.stream,
+                         (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
     else
-      hipLaunchKernelGGL((k_xpass<T, N, 6>), dim3(blocks), dim3(XPASS_THREADS), LDS_X, c.stream, (const cx<T> *)c.bufb,
-                         (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
+      hipLaunchKernelGGL((k_xpass<T, N, 6, XT>), dim3(blocks), dim3(XT), lds_x<XT>(), c.stream,
+                         (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
   }
+  void xpass(Ctx &c, const XArgs &xa) override {
+    c.sched[2] ? xpass_v<kXThreads[1]>(c, xa) : xpass_v<kXThreads[0]>(c, xa);
+  }
+
   void spectrum(Ctx &c, int which, double *bins) override {
     const cx<T> *f = which == HDNS_FIELD_SCALAR ? (const cx<T> *)c.ph : (const cx<T> *)c.uh;
     const int nc = which == HDNS_FIELD_SCALAR ? 1 : 3;

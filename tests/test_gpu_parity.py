@@ -201,3 +201,25 @@ def test_graph_captured_step_replays_the_eager_step_bitwise(g):
         eager.step_fields(dt / 2)
         graph.step_fields(dt / 2, graph=True)
         assert np.array_equal(eager.state().velocity_spectral, graph.state().velocity_spectral)
+
+
+def test_autotune_leaves_the_state_bitwise_unchanged_and_schedules_agree(g):
+    n, dt = 64, 2e-3
+    sim, *_ = make_sim(g, n, np.float32, forcing=0.1)
+    ref, *_ = make_sim(g, n, np.float32, forcing=0.1)
+    with sim, ref:
+        before = sim.state().velocity_spectral.copy()
+        report = sim.autotune(reps=2)
+        assert set(report) == {"y_passes", "z_passes", "x_pass"}
+        assert np.array_equal(sim.state().velocity_spectral, before)
+        assert sim.reductions() == ref.reductions()
+        for sched in [(0, 0, 0), (1, 1, 1), (0, 1, 0)]:
+            sim.schedule(sched)
+            assert sim.schedule() == sched
+        sim.schedule((0, 0, 0))
+        ref.schedule((1, 1, 1))
+        sim.step_fields(dt)
+        ref.step_fields(dt)
+        a, b = sim.state(), ref.state()
+        assert rel(a.velocity_spectral, b.velocity_spectral) < 1e-6
+        assert rel(a.scalar_spectral, b.scalar_spectral) < 1e-6
