@@ -207,7 +207,7 @@ void prof_end(Ctx &c) {
 hdns_status run_deposit(hdns_ctx *ctx) {
   Ctx &c = ctx->c;
   c.pipe->zi(c);
-  const FinalizeArgs fa{(int)((c.geo.kxp / c.pipe->tile_width()) * c.geo.k), 1.0 / pow((double)c.geo.n, 6.0),
+  const FinalizeArgs fa{c.pipe->z_blocks(c), 1.0 / pow((double)c.geo.n, 6.0),
                         c.cfg.velocity_forcing == HDNS_CONSTANT_POWER, c.cfg.injection_rate,
                         c.fp64 ? 2.220446049250313e-16 : 1.1920928955078125e-07};
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c.stream, (const double *)c.partials, c.scalars, fa);
@@ -314,7 +314,7 @@ hdns_status hdns_ctx_create(const hdns_config *config, hdns_ctx **out) {
   hdns_status s = HDNS_OK;
   e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
   if (e != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, std::string("hipStreamCreate failed: ") + hipGetErrorString(e));
-  const int nzb = (c.geo.kxp / tx) * c.geo.k;
+  const int nzb = (c.geo.kxp / (tx / 2)) * c.geo.k;  // the narrowest z tile of any schedule
   c.nbins = (int)floor(sqrt(3.0) * n / 2) + 2;
   c.ntiles = (n / 8) * (n / 8) * (n / 8);
   c.m = 0;
@@ -536,7 +536,7 @@ void launch_step(Ctx &c, double dt) {
   const Coeffs k = scheme_coefficients(c.cfg.scheme, fp32);
   const double dt_t = fp32 ? (double)(float)dt : dt;
   const double kappa = c.cfg.viscosity / c.cfg.schmidt;
-  const FinalizeArgs fa{(int)((c.geo.kxp / c.pipe->tile_width()) * c.geo.k), 1.0 / pow((double)c.geo.n, 6.0),
+  const FinalizeArgs fa{c.pipe->z_blocks(c), 1.0 / pow((double)c.geo.n, 6.0),
                         c.cfg.velocity_forcing == HDNS_CONSTANT_POWER, c.cfg.injection_rate,
                         c.fp64 ? 2.220446049250313e-16 : 1.1920928955078125e-07};
   XArgs mid{};

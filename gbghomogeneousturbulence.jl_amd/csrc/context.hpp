@@ -27,7 +27,8 @@ struct Pipeline {
   virtual void sample(Ctx &c) = 0;
   virtual void cell_keys(Ctx &c, int64_t *keys) = 0;
   virtual void tile_sort(Ctx &c) = 0;
-  virtual int tile_width() const = 0;  // kx columns per y / z block
+  virtual int tile_width() const = 0;  // kx columns per y block (the widest tile)
+  virtual int z_blocks(const Ctx &c) const = 0;  // blocks of a z pass under the current schedule
 };
 
 std::unique_ptr<Pipeline> make_pipeline_f32(int n);
@@ -43,7 +44,7 @@ struct Ctx {
   int stages = 0;
   // launch schedule of the y, z and x passes (index into the two variants of each), set by
   // hdns_autotune; the defaults are the fastest measured at 512^3 on MI355X
-  int sched[3] = {1, 1, 0};
+  int sched[3] = {1, 0, 0};
   BandSpec band{};
   hipStream_t stream = nullptr;
   void *uh = nullptr, *hu = nullptr, *ph = nullptr, *hp = nullptr;

@@ -12,10 +12,17 @@ template <typename T>
 constexpr int tile_width_of() { return sizeof(T) == 4 ? 16 : 8; }
 
 // Two launch schedules per stage kernel, chosen per context by hdns_autotune:
-//   y passes: 256 or 512 threads per block;  z passes: 512 or 1024;  x pass: 256 or 512.
+//   y passes: 256 or 512 threads per block;
+//   z passes: a TX-wide tile on 512 threads, or a TX/2-wide tile on 64*TX/2 threads (one radix-8
+//             butterfly per thread per pass at N = 512, half the registers, twice the blocks);
+//   x pass: 256 or 512 threads per block.
 constexpr int kYThreads[2] = {256, 512};
-constexpr int kZThreads[2] = {512, 1024};
 constexpr int kXThreads[2] = {256, 512};
+template <typename T>
+struct ZSched {
+  static constexpr int tx[2] = {tile_width_of<T>(), tile_width_of<T>() / 2};
+  static constexpr int nt[2] = {512, 64 * (tile_width_of<T>() / 2)};
+};
 
 template <typename T, int N>
 struct PipelineImpl final : Pipeline {
@@ -35,10 +42,12 @@ struct PipelineImpl final : Pipeline {
   static void set_all() {
     set_lds((const void *)k_yfwd<T, N, TX, kYThreads[V]>, LDS_COL);
     set_lds((const void *)k_yinv<T, N, TX, kYThreads[V]>, LDS_COL);
-    set_lds((const void *)k_zf<T, N, TX, kZThreads[V], true>, LDS_COL);
-    set_lds((const void *)k_zf<T, N, TX, kZThreads[V], false>, LDS_COL);
-    set_lds((const void *)k_zi<T, N, TX, kZThreads[V], true>, LDS_COL);
-    set_lds((const void *)k_zi<T, N, TX, kZThreads[V], false>, LDS_COL);
+    constexpr int ZT = ZSched<T>::tx[V], ZN = ZSched<T>::nt[V];
+    constexpr size_t ZL = ((size_t)N * ZT + N) * sizeof(cx<T>);
+    set_lds((const void *)k_zf<T, N, ZT, ZN, true>, ZL);
+    set_lds((const void *)k_zf<T, N, ZT, ZN, false>, ZL);
+    set_lds((const void *)k_zi<T, N, ZT, ZN, true>, ZL);
+    set_lds((const void *)k_zi<T, N, ZT, ZN, false>, ZL);
     set_lds((const void *)k_xpass<T, N, 7, kXThreads[V]>, lds_x<kXThreads[V]>());
     set_lds((const void *)k_xpass<T, N, 6, kXThreads[V]>, lds_x<kXThreads[V]>());
   }
@@ -50,6 +59,7 @@ struct PipelineImpl final : Pipeline {
 
   int tiles(const Ctx &c) const { return c.geo.kxp / TX; }
   int tile_width() const override { return TX; }
+  int z_blocks(const Ctx &c) const override { return c.geo.kxp / ZSched<T>::tx[c.sched[1]] * K; }
 
   template <int NTY>
   void yfwd_v(Ctx &c) {
@@ -59,21 +69,21 @@ struct PipelineImpl final : Pipeline {
   }
   void yfwd(Ctx &c) override { c.sched[0] ? yfwd_v<kYThreads[1]>(c) : yfwd_v<kYThreads[0]>(c); }
 
-  template <int NTZ>
+  template <int V>
   void zf_v(Ctx &c, const StageArgs &sa) {
-    const dim3 grid(tiles(c), K);
+    constexpr int ZT = ZSched<T>::tx[V], ZN = ZSched<T>::nt[V];
+    constexpr size_t ZL = ((size_t)N * ZT + N) * sizeof(cx<T>);
+    const dim3 grid(c.geo.kxp / ZT, K);
     if (c.cfg.has_scalar)
-      hipLaunchKernelGGL((k_zf<T, N, TX, NTZ, true>), grid, dim3(NTZ), LDS_COL, c.stream, (const cx<T> *)c.bufb,
+      hipLaunchKernelGGL((k_zf<T, N, ZT, ZN, true>), grid, dim3(ZN), ZL, c.stream, (const cx<T> *)c.bufb,
                          (cx<T> *)c.uh, (cx<T> *)c.hu, (cx<T> *)c.ph, (cx<T> *)c.hp, c.geo,
                          (const cx<double> *)c.tw, sa);
     else
-      hipLaunchKernelGGL((k_zf<T, N, TX, NTZ, false>), grid, dim3(NTZ), LDS_COL, c.stream, (const cx<T> *)c.bufb,
+      hipLaunchKernelGGL((k_zf<T, N, ZT, ZN, false>), grid, dim3(ZN), ZL, c.stream, (const cx<T> *)c.bufb,
                          (cx<T> *)c.uh, (cx<T> *)c.hu, (cx<T> *)c.ph, (cx<T> *)c.hp, c.geo,
                          (const cx<double> *)c.tw, sa);
   }
-  void zf(Ctx &c, const StageArgs &sa) override {
-    c.sched[1] ? zf_v<kZThreads[1]>(c, sa) : zf_v<kZThreads[0]>(c, sa);
-  }
+  void zf(Ctx &c, const StageArgs &sa) override { c.sched[1] ? zf_v<1>(c, sa) : zf_v<0>(c, sa); }
 
   void reality(Ctx &c) override {
     const int n = K * K;
@@ -81,17 +91,19 @@ struct PipelineImpl final : Pipeline {
                        c.geo, c.cfg.has_scalar ? 4 : 3);
   }
 
-  template <int NTZ>
+  template <int V>
   void zi_v(Ctx &c) {
-    const dim3 grid(tiles(c), K);
+    constexpr int ZT = ZSched<T>::tx[V], ZN = ZSched<T>::nt[V];
+    constexpr size_t ZL = ((size_t)N * ZT + N) * sizeof(cx<T>);
+    const dim3 grid(c.geo.kxp / ZT, K);
     if (c.cfg.has_scalar)
-      hipLaunchKernelGGL((k_zi<T, N, TX, NTZ, true>), grid, dim3(NTZ), LDS_COL, c.stream, (const cx<T> *)c.uh,
+      hipLaunchKernelGGL((k_zi<T, N, ZT, ZN, true>), grid, dim3(ZN), ZL, c.stream, (const cx<T> *)c.uh,
                          (const cx<T> *)c.ph, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, c.band, c.partials);
     else
-      hipLaunchKernelGGL((k_zi<T, N, TX, NTZ, false>), grid, dim3(NTZ), LDS_COL, c.stream, (const cx<T> *)c.uh,
+      hipLaunchKernelGGL((k_zi<T, N, ZT, ZN, false>), grid, dim3(ZN), ZL, c.stream, (const cx<T> *)c.uh,
                          (const cx<T> *)c.ph, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, c.band, c.partials);
   }
-  void zi(Ctx &c) override { c.sched[1] ? zi_v<kZThreads[1]>(c) : zi_v<kZThreads[0]>(c); }
+  void zi(Ctx &c) override { c.sched[1] ? zi_v<1>(c) : zi_v<0>(c); }
 
   template <int NTY>
   void yinv_v(Ctx &c) {

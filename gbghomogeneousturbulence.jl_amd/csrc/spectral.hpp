@@ -72,6 +72,55 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
   const size_t pencil = (size_t)N * K * g.kxp;         // one PF component
   const size_t zstride = (size_t)K * g.kxp;
 
+  // The scalar first: its mean-gradient source reads the stage velocity before the velocity
+  // update overwrites it.
+  if constexpr (SCALAR) {
+    cx<T> G[PER][RL];
+#pragma unroll
+    for (int p = 0; p < PER; ++p)
+#pragma unroll
+      for (int q = 0; q < RL; ++q) G[p][q] = czero<T>();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const cx<T> *src = pf + (size_t)(3 + c) * pencil + (size_t)ky_idx * g.kxp + kx0;
+      fft_io<T, N, TX, NT, -1, true, false>(
+          buf, twl, [&](int, int, int n, int col) { return src[(size_t)n * zstride + col]; },
+          [&](int p, int q, int n, int col, cx<T> v) {
+            const int mc = c == 0 ? kx0 + col : (c == 1 ? my : signed_of(idx_of_bin(n, g), g));
+            const T kc = T(g.k0 * mc);
+            G[p][q] = {G[p][q].re + kc * v.re, G[p][q].im + kc * v.im};
+          });
+    }
+    const T Gm = T(sa.mean_gradient);
+    const T dt = T(sa.dt), a = T(sa.a), b = T(sa.b);
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+      const int bb = tid + p * NT;
+      if (!(SL::TOT % NT == 0 || bb < SL::TOT)) continue;
+      const int col = bb % TX, j = bb / TX;
+      const int mx = kx0 + col;
+#pragma unroll
+      for (int q = 0; q < RL; ++q) {
+        const int kz_idx = idx_of_bin(j + q * SL::NB, g);
+        if (kz_idx < 0) continue;
+        const int mz = signed_of(kz_idx, g);
+        const int m2 = mx * mx + my * my + mz * mz;
+        const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
+        const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
+        cx<T> nl = in_two_thirds(m2, N) ? cx<T>{G[p][q].im, -G[p][q].re} : czero<T>();   // -i k.F
+        const size_t at = (size_t)kz_idx * zstride + (size_t)ky_idx * g.kxp + mx;
+        nl = nl - scale(uh[2 * comp + at], Gm);   // the stage velocity: the velocity update comes after
+        const T E = T(exp(sa.coef_p * k2d));
+        cx<T> h = scale(nl, dt);
+        if (sa.stage > 0) h = scale(hp[at], a) + h;
+        const cx<T> qn = scale(ph[at] + scale(h, b), E);
+        ph[at] = qn;
+        if (!sa.last) hp[at] = scale(h, E);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+
   // velocity products, forward z transform, straight into the owned modes
   cx<T> F[3][PER][RL];
 #pragma unroll
@@ -85,7 +134,6 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
 
   const T A = sa.band.on ? T(sa.scalars[0]) : T(0);
   const T a = T(sa.a), b = T(sa.b), dt = T(sa.dt);
-  cx<T> uz_old[PER][RL];
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int bb = tid + p * NT;
@@ -118,7 +166,6 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
       cx<T> u[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) u[c] = uh[c * comp + at];
-      uz_old[p][q] = u[2];
       if (in_band(m2, sa.band)) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) nl[c] = nl[c] + scale(u[c], A);
@@ -141,51 +188,8 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
       }
 #pragma unroll
       for (int c = 0; c < 3; ++c) uh[c * comp + at] = u[c];
-    }
-  }
-
-  if constexpr (SCALAR) {
-    cx<T> G[PER][RL];
-#pragma unroll
-    for (int p = 0; p < PER; ++p)
-#pragma unroll
-      for (int q = 0; q < RL; ++q) G[p][q] = czero<T>();
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const cx<T> *src = pf + (size_t)(3 + c) * pencil + (size_t)ky_idx * g.kxp + kx0;
-      fft_io<T, N, TX, NT, -1, true, false>(
-          buf, twl, [&](int, int, int n, int col) { return src[(size_t)n * zstride + col]; },
-          [&](int p, int q, int n, int col, cx<T> v) {
-            const int mc = c == 0 ? kx0 + col : (c == 1 ? my : signed_of(idx_of_bin(n, g), g));
-            const T kc = T(g.k0 * mc);
-            G[p][q] = {G[p][q].re + kc * v.re, G[p][q].im + kc * v.im};
-          });
-    }
-    const T Gm = T(sa.mean_gradient);
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      const int bb = tid + p * NT;
-      if (!(SL::TOT % NT == 0 || bb < SL::TOT)) continue;
-      const int col = bb % TX, j = bb / TX;
-      const int mx = kx0 + col;
-#pragma unroll
-      for (int q = 0; q < RL; ++q) {
-        const int kz_idx = idx_of_bin(j + q * SL::NB, g);
-        if (kz_idx < 0) continue;
-        const int mz = signed_of(kz_idx, g);
-        const int m2 = mx * mx + my * my + mz * mz;
-        const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
-        const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
-        cx<T> nl = in_two_thirds(m2, N) ? cx<T>{G[p][q].im, -G[p][q].re} : czero<T>();   // -i k.F
-        nl = nl - scale(uz_old[p][q], Gm);
-        const size_t at = (size_t)kz_idx * zstride + (size_t)ky_idx * g.kxp + mx;
-        const T E = T(exp(sa.coef_p * k2d));
-        cx<T> h = scale(nl, dt);
-        if (sa.stage > 0) h = scale(hp[at], a) + h;
-        const cx<T> qn = scale(ph[at] + scale(h, b), E);
-        ph[at] = qn;
-        if (!sa.last) hp[at] = scale(h, E);
-      }
+      // keep each mode's loads inside its own iteration: hoisting all of them costs occupancy
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
 }
