@@ -117,6 +117,57 @@ def time_gpu(g, n: int, dtype, steps: int, warmup: int, H, seed: int = 2026):
             "schedule": tuned}
 
 
+def time_tracers(g, n: int, m: int, dtype, steps: int, H) -> dict:
+    """Tracer throughput: traversal rebuild + Heun push + closing sample per step, M tracers on an
+    n^3 field (the field step itself is excluded from this figure)."""
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=2 * math.pi, dealiasing=g.TwoThirds(), element_type=dtype)
+    u = g.random_phase_velocity(n, seed=2026)
+    p = np.zeros((n, n, n // 2 + 1), dtype=np.complex128)
+    cond = g.FlowConditions(viscosity=g.predicted_viscosity(dom, 0.1), schmidt=1.0)
+    tr = g.Tracers(dom, number_of_particles=m, seed=2028)
+    dt = 1e-3
+    total = 0.0
+    with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=1.0, tracers=tr) as sim:
+        stream = ctypes.c_void_p(sim.stream())
+        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        H.hipEventCreate(ctypes.byref(e0))
+        H.hipEventCreate(ctypes.byref(e1))
+        t = 0.0
+        for s in range(steps + 1):
+            sim.step_fields(dt)
+            t += dt
+            H.hipEventRecord(e0, stream)
+            sim._call("hdns_rebuild_traversal")
+            sim.advect(dt)
+            sim.deposit_tracers(t)
+            H.hipEventRecord(e1, stream)
+            H.hipEventSynchronize(e1)
+            ms = ctypes.c_float()
+            H.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+            if s > 0:
+                total += ms.value
+        H.hipEventDestroy(e0)
+        H.hipEventDestroy(e1)
+    per = total / steps
+    return {"grid": n, "tracers": m, "dtype": np.dtype(dtype).name, "ms_per_step": round(per, 4),
+            "tracer_steps_per_s": round(m / (per * 1e-3), 1)}
+
+
+def copy_calibration(H, nbytes: int = 1 << 30) -> None:
+    """One device-to-device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    H.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    H.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    H.hipFree.argtypes = [ctypes.c_void_p]
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    H.hipMalloc(ctypes.byref(a), nbytes)
+    H.hipMalloc(ctypes.byref(b), nbytes)
+    for _ in range(2):
+        H.hipMemcpy(b, a, nbytes, 3)   # hipMemcpyDeviceToDevice
+    H.hipDeviceSynchronize()
+    H.hipFree(a)
+    H.hipFree(b)
+
+
 def kernel_table(res: dict, n: int, real_bytes: int, stages: int) -> dict:
     ab = algorithmic_bytes(n, real_bytes, stages)
     out = {}
@@ -166,6 +217,9 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain-FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=256)
+    ap.add_argument("--no-tracers", action="store_true", help="skip the tracer throughput figure")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 1 GiB device copy (the reference kernel for counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this DNS step runs on a single MI355X and --gpus {args.gpus} is not "
@@ -202,8 +256,15 @@ def main() -> int:
             t = json.loads(traffic_file.read_text()).get(dk["symbol"])
             if t and t.get("n") == n:
                 roofline["traffic"] = t["bytes_per_launch"]
+                roofline["traffic_source"] = "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated: " + t["kernel"]
         except (ValueError, KeyError):
             pass
+    tracers = None
+    if not args.no_tracers:
+        print("bench: tracers, 256^3 field, 2^20 tracers", file=sys.stderr, flush=True)
+        tracers = [time_tracers(g, 256, 1 << 20, np.float32, 5, H), time_tracers(g, 256, 1 << 20, np.float64, 5, H)]
+    if args.copy_calibration:
+        copy_calibration(H)
     cpu = None
     if not args.no_cpu_baseline:
         print(f"bench: cpu baseline at {args.cpu_sample_n}^3", file=sys.stderr, flush=True)
@@ -221,6 +282,7 @@ def main() -> int:
         "wall_ms_per_step": round(r32["wall_s"] * 1e3 / args.steps, 3),
         "memory_GiB": round(r32["memory_bytes"] / 2 ** 30, 2),
         "autotuned_schedule": r32["schedule"],
+        "tracers": tracers,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -6,7 +6,8 @@
 //   [c][kz_idx][ky_idx][kx]   kx in [0, KXP) (padded past M with zeros),
 //                             ky_idx, kz_idx in [0, K), K = 2M + 1, m = idx <= M ? idx : idx - K,
 // and the transforms are pruned: a z pass only ever touches the K retained ky columns, a y pass
-// only the kx <= M columns, and the x pass zero-fills kx > M.  The half-transformed pencils are
+// only the kx < KXP columns, and the x pass zero-fills M < kx < KXP.  The half-transformed
+// pencils are
 //   PX [c][z][y][kx]   after the forward x pass      (6 product components)
 //   PF [c][z][ky][kx]  after the forward y pass
 //   Q  [c][z][ky][kx]  after the inverse z pass      (7 components: u, w, phi)
@@ -39,6 +40,20 @@ __device__ __forceinline__ int idx_of_bin(int b, const Geo &g) {
 template <typename T>
 __device__ __forceinline__ cx<T> czero() { return {T(0), T(0)}; }
 
+// The (kx, ky) column of a z pencil meets the closed 2/3 ball 9|m|^2 <= N^2 only when
+// 9 (mx^2 + my^2) <= N^2.  Every retained state mode outside the ball is zero and stays zero
+// (the state is adopted truncated, and the mask, the Leray projection, the forcing and the
+// integrating factor all keep zeros zero), so the passes skip the (kx, ky) columns outside this
+// disk, at the granularity of whole 128-byte lines (partly written lines cost more than they
+// save): a ky row of PF or Q is written and read only in the lines whose first column lies in
+// the disk.  In a written Q line the columns outside the disk hold exact zeros (the transform of
+// zero modes), so the inverse y pass reads the whole line.
+__host__ __device__ __forceinline__ bool in_disk(int mx, int my, int n) { return 9 * (mx * mx + my * my) <= n * n; }
+
+// First column of the 128-byte line that holds column mx.
+template <typename T>
+__host__ __device__ __forceinline__ int line_start(int mx) { return mx & ~(int)(128 / sizeof(cx<T>) - 1); }
+
 // ---------------------------------------------------------------------------------------------
 // y passes: a block transforms TX adjacent kx columns of one z plane of one component.
 
@@ -52,11 +67,12 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cx<T> *__restrict__ px, cx<T>
   const int kx0 = blockIdx.x * TX, z = blockIdx.y, c = blockIdx.z;
   const cx<T> *src = px + ((size_t)c * N + z) * (size_t)N * g.kxp + kx0;
   cx<T> *dst = pf + ((size_t)c * N + z) * (size_t)g.k * g.kxp + kx0;
+  static_assert(TX * sizeof(cx<T>) == 128, "a y tile is one 128-byte line of a row");
   fft_io<T, N, TX, NT, -1, true, false>(
       buf, twl, [&](int, int, int n, int col) { return src[(size_t)n * g.kxp + col]; },
       [&](int, int, int n, int col, cx<T> v) {
         const int idx = idx_of_bin(n, g);
-        if (idx >= 0) dst[(size_t)idx * g.kxp + col] = v;
+        if (idx >= 0 && in_disk(kx0, signed_of(idx, g), N)) dst[(size_t)idx * g.kxp + col] = v;
       });
 }
 
@@ -74,7 +90,7 @@ __global__ __launch_bounds__(NT) void k_yinv(const cx<T> *__restrict__ q, cx<T> 
       buf, twl,
       [&](int, int, int n, int col) {
         const int idx = idx_of_bin(n, g);
-        return idx >= 0 ? src[(size_t)idx * g.kxp + col] : czero<T>();
+        return idx >= 0 && in_disk(kx0, signed_of(idx, g), N) ? src[(size_t)idx * g.kxp + col] : czero<T>();
       },
       [&](int, int, int n, int col, cx<T> v) { dst[(size_t)n * g.kxp + col] = v; });
 }
@@ -236,7 +252,7 @@ __global__ __launch_bounds__(XT) void k_xpass(const cx<T> *__restrict__ rin,
     pass_sync<WAVE>();
     fft_columns<T, N, 1, LPR, -1, false, WAVE>(wb, twl);
     cx<T> *dst = pxo + (size_t)d * N * plane + (size_t)z * plane + (size_t)ya * g.kxp;
-    for (int kx = lane; kx < g.kxp; kx += LPR) {
+    for (int kx = lane; kx < g.kxp; kx += LPR) {   // zeros past M keep every line whole
       cx<T> A = czero<T>(), B = czero<T>();
       if (kx <= M) {
         const cx<T> C = wb[kx];

@@ -63,10 +63,11 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *twl = buf + N * TX;
-  stage_twiddles<T, N, NT>(twl, tw);
   const int tid = threadIdx.x;
   const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y;
   const int my = signed_of(ky_idx, g);
+  if (!in_disk(kx0, my, N)) return;   // no column of this tile meets the ball (its PF lines are unwritten)
+  stage_twiddles<T, N, NT>(twl, tw);
   const T ky = T(g.k0 * my);
   const size_t comp = (size_t)K * K * g.kxp;           // one compact component
   const size_t pencil = (size_t)N * K * g.kxp;         // one PF component
@@ -84,7 +85,8 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
     for (int c = 0; c < 3; ++c) {
       const cx<T> *src = pf + (size_t)(3 + c) * pencil + (size_t)ky_idx * g.kxp + kx0;
       fft_io<T, N, TX, NT, -1, true, false>(
-          buf, twl, [&](int, int, int n, int col) { return src[(size_t)n * zstride + col]; },
+          buf, twl,
+          [&](int, int, int n, int col) { return src[(size_t)n * zstride + col]; },
           [&](int p, int q, int n, int col, cx<T> v) {
             const int mc = c == 0 ? kx0 + col : (c == 1 ? my : signed_of(idx_of_bin(n, g), g));
             const T kc = T(g.k0 * mc);
@@ -105,9 +107,10 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
         if (kz_idx < 0) continue;
         const int mz = signed_of(kz_idx, g);
         const int m2 = mx * mx + my * my + mz * mz;
+        if (!in_two_thirds(m2, N)) continue;   // zero, and stays zero
         const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
         const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
-        cx<T> nl = in_two_thirds(m2, N) ? cx<T>{G[p][q].im, -G[p][q].re} : czero<T>();   // -i k.F
+        cx<T> nl = {G[p][q].im, -G[p][q].re};   // -i k.F
         const size_t at = (size_t)kz_idx * zstride + (size_t)ky_idx * g.kxp + mx;
         nl = nl - scale(uh[2 * comp + at], Gm);   // the stage velocity: the velocity update comes after
         const T E = T(exp(sa.coef_p * k2d));
@@ -128,7 +131,8 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
     const cx<T> *src = pf + (size_t)c * pencil + (size_t)ky_idx * g.kxp + kx0;
     auto &Fc = F[c];
     fft_io<T, N, TX, NT, -1, true, false>(
-        buf, twl, [&](int, int, int n, int col) { return src[(size_t)n * zstride + col]; },
+        buf, twl,
+        [&](int, int, int n, int col) { return src[(size_t)n * zstride + col]; },
         [&](int p, int q, int, int, cx<T> v) { Fc[p][q] = v; });
   }
 
@@ -146,12 +150,12 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
       if (kz_idx < 0) continue;
       const int mz = signed_of(kz_idx, g);
       const int m2 = mx * mx + my * my + mz * mz;
+      if (!in_two_thirds(m2, N)) continue;   // zero, and stays zero
       const T kx = T(g.k0 * mx), kz = T(g.k0 * mz);
       const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
       const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
-      const bool keep = in_two_thirds(m2, N) && m2 > 0;
       cx<T> nl[3];
-      if (keep) {
+      if (m2 > 0) {
         const T k2 = T(k2d);
         const cx<T> kf = {kx * F[0][p][q].re + ky * F[1][p][q].re + kz * F[2][p][q].re,
                           kx * F[0][p][q].im + ky * F[1][p][q].im + kz * F[2][p][q].im};
@@ -216,11 +220,15 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *twl = buf + N * TX;
-  stage_twiddles<T, N, NT>(twl, tw);
   __shared__ double red[NT / 64][NRED];
   const int tid = threadIdx.x;
   const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y;
   const int my = signed_of(ky_idx, g);
+  if (!in_disk(line_start<T>(kx0), my, N)) {   // the tile's line misses the ball: never read
+    if (tid < NRED) partials[(size_t)(blockIdx.y * gridDim.x + blockIdx.x) * NRED + tid] = 0.0;
+    return;
+  }
+  stage_twiddles<T, N, NT>(twl, tw);
   const size_t comp = (size_t)K * K * g.kxp;
   const size_t pencil = (size_t)N * K * g.kxp;
   const size_t zstride = (size_t)K * g.kxp;
@@ -238,13 +246,13 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
 #pragma unroll
     for (int qq = 0; qq < R1; ++qq) {
       const int kz_idx = live ? idx_of_bin(j + qq * S1::NB, g) : -1;
-      if (kz_idx < 0) {
+      const int mz = kz_idx < 0 ? 0 : signed_of(kz_idx, g);
+      if (kz_idx < 0 || !in_two_thirds(mx * mx + my * my + mz * mz, N)) {   // zero outside the ball
 #pragma unroll
         for (int c = 0; c < 3; ++c) u[c][pp][qq] = czero<T>();
         p[pp][qq] = czero<T>();
         continue;
       }
-      const int mz = signed_of(kz_idx, g);
       const size_t at = (size_t)kz_idx * zstride + (size_t)ky_idx * g.kxp + mx;
       const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
       const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
