@@ -3,8 +3,11 @@
 // Tile width TX: 16 complex FP32 or 8 complex FP64 = 128 contiguous bytes per z or y row of a
 // tile.  The column buffer of a y or z block is N*TX complex values (64 KiB at N = 512).
 #pragma once
+#include <type_traits>
+
 #include "context.hpp"
 #include "tracers.hpp"
+#include "xpass_wave.hpp"
 
 namespace hdns {
 
@@ -33,6 +36,10 @@ struct PipelineImpl final : Pipeline {
   static constexpr int xr() { return XT / xpass_lanes<T, N, XT>(); }
   template <int XT>
   static constexpr size_t lds_x() { return ((size_t)xr<XT>() * N + N) * sizeof(cx<T>); }
+  // FP32 at N = 512: the register-resident x pass, one wavefront per row pair
+  static constexpr bool XWAVE = std::is_same<T, float>::value && N == 512;
+  template <int XT>
+  static constexpr size_t lds_xw() { return (size_t)(XT / 64) * (N + N / 8) * sizeof(cx<T>); }
 
   static void set_lds(const void *fn, size_t bytes) {
     (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
@@ -48,8 +55,13 @@ struct PipelineImpl final : Pipeline {
     set_lds((const void *)k_zf<T, N, ZT, ZN, false>, ZL);
     set_lds((const void *)k_zi<T, N, ZT, ZN, true>, ZL);
     set_lds((const void *)k_zi<T, N, ZT, ZN, false>, ZL);
-    set_lds((const void *)k_xpass<T, N, 7, kXThreads[V]>, lds_x<kXThreads[V]>());
-    set_lds((const void *)k_xpass<T, N, 6, kXThreads[V]>, lds_x<kXThreads[V]>());
+    if constexpr (XWAVE) {
+      set_lds((const void *)k_xpass_wave<T, N, 7, kXThreads[V]>, lds_xw<kXThreads[V]>());
+      set_lds((const void *)k_xpass_wave<T, N, 6, kXThreads[V]>, lds_xw<kXThreads[V]>());
+    } else {
+      set_lds((const void *)k_xpass<T, N, 7, kXThreads[V]>, lds_x<kXThreads[V]>());
+      set_lds((const void *)k_xpass<T, N, 6, kXThreads[V]>, lds_x<kXThreads[V]>());
+    }
   }
 
   PipelineImpl() {
@@ -115,6 +127,16 @@ struct PipelineImpl final : Pipeline {
 
   template <int XT>
   void xpass_v(Ctx &c, const XArgs &xa) {
+    if constexpr (XWAVE) {
+      const int blocks = N * N / (2 * (XT / 64));
+      if (c.cfg.has_scalar)
+        hipLaunchKernelGGL((k_xpass_wave<T, N, 7, XT>), dim3(blocks), dim3(XT), lds_xw<XT>(), c.stream,
+                           (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
+      else
+        hipLaunchKernelGGL((k_xpass_wave<T, N, 6, XT>), dim3(blocks), dim3(XT), lds_xw<XT>(), c.stream,
+                           (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
+      return;
+    }
     const int blocks = N * N / (2 * xr<XT>());
     if (c.cfg.has_scalar)
       hipLaunchKernelGGL((k_xpass<T, N, 7, XT>), dim3(blocks), dim3(XT), lds_x<XT>(), c.stream,

@@ -1,0 +1,240 @@
+// The x pass at N = 512 with one wavefront per row pair and the data in registers.
+//
+// A 512-point transform on the 64 lanes of a wavefront is three radix-8 Stockham passes in which
+// lane j is butterfly j of every pass: it reads elements j + 64 q (q = 0..7) and, after the last
+// pass, holds the outputs j + 64 q.  So the inverse transforms leave lane j with the physical
+// values at x = j + 64 q of every component, the products are formed lane-locally, and they feed
+// the forward transforms' first pass directly.  Only the two exchanges between passes go through
+// LDS (a wave-private, padded buffer).  The per-lane twiddles of the third pass are held in
+// registers for the whole kernel; the second pass's, W_64^{q (j mod 8)}, come from a 64-entry
+// table in LDS.
+//
+// The Hermitian pack of two real rows (row a in the real part, row b in the imaginary part) is
+// read straight from HBM into the first pass: element n of the packed spectrum is built from
+// (a, b) at kx = n (n <= M) or kx = N - n (n >= N - M), and is zero in between.  The forward
+// transforms' outputs are unpacked with the conjugate partner C[N - kx], which lives in lane
+// (64 - j) mod 64 and is fetched with a cross-lane permute.
+//
+// The components are streamed so that at most u, two partial Lamb components and the component
+// in flight are live: u, then phi and the three fluxes u phi, then w_x, w_y, w_z, each adding
+// its terms of u x w, each Lamb component transformed as soon as it is complete.
+#pragma once
+#include "pencil.hpp"
+
+namespace hdns {
+
+// wave-private exchange buffer: one pad slot per 8 values spreads a pass's strided accesses
+// over the LDS banks
+__device__ __forceinline__ int xpad(int i) { return i + (i >> 3); }
+
+template <int SIGN, typename T>
+__device__ __forceinline__ cx<T> twiddled(cx<T> v, cx<T> w) {
+  return v * cx<T>{w.re, SIGN < 0 ? w.im : -w.im};
+}
+
+// x: element j + 64 q of the input in, element j + 64 q of the transform out (lane j).
+// t64[m] = W_64^m, t3[q - 1] = W_512^{q j}.
+template <int SIGN, typename T>
+__device__ __forceinline__ void wave_fft512(cx<T> (&x)[8], cx<T> *wb, const cx<T> *t64, const cx<T> (&t3)[7],
+                                            int j) {
+  dft8<SIGN>(x);                                     // NS = 1: outputs 8 j + q
+#pragma unroll
+  for (int q = 0; q < 8; ++q) wb[xpad(8 * j + q)] = x[q];
+  pass_sync<true>();
+#pragma unroll
+  for (int q = 0; q < 8; ++q) x[q] = wb[xpad(j + 64 * q)];
+  pass_sync<true>();
+#pragma unroll
+  for (int q = 1; q < 8; ++q) x[q] = twiddled<SIGN>(x[q], t64[q * (j & 7)]);
+  dft8<SIGN>(x);                                     // NS = 8: outputs 64 (j/8) + j mod 8 + 8 q
+  const int base = (j >> 3) * 64 + (j & 7);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) wb[xpad(base + 8 * q)] = x[q];
+  pass_sync<true>();
+#pragma unroll
+  for (int q = 0; q < 8; ++q) x[q] = wb[xpad(j + 64 * q)];
+  pass_sync<true>();                                 // the buffer may be rewritten at once
+#pragma unroll
+  for (int q = 1; q < 8; ++q) x[q] = twiddled<SIGN>(x[q], t3[q - 1]);
+  dft8<SIGN>(x);                                     // NS = 64: outputs j + 64 q
+}
+
+// the value v of lane `lane` (all lanes active)
+__device__ __forceinline__ float lane_value(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(lane << 2, __float_as_int(v)));
+}
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(lane << 2, (int)(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_ds_bpermute(lane << 2, (int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <typename T, int N, int NCIN, int XT>
+__global__ __launch_bounds__(XT) void k_xpass_wave(const cx<T> *__restrict__ rin, cx<T> *__restrict__ pxo, Geo g,
+                                                   const cx<double> *__restrict__ tw, XArgs xa) {
+  static_assert(N == 512, "three radix-8 passes on 64 lanes");
+  static_assert(NCIN == 7 || NCIN == 6, "u, w and optionally phi");
+  constexpr int XR = XT / 64;          // row pairs per block, one per wavefront
+  constexpr int M = N / 3;
+  constexpr int NP = N + N / 8;        // padded exchange buffer
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ double red[XR];
+  __shared__ cx<T> t64[64];
+  const int w = threadIdx.x >> 6, j = threadIdx.x & 63;
+  cx<T> *wb = reinterpret_cast<cx<T> *>(smem_raw) + w * NP;
+  if (threadIdx.x < 64) t64[j] = {T(tw[8 * j].re), T(tw[8 * j].im)};
+  __syncthreads();
+
+  const int rows_per_plane = N / (2 * XR);
+  const int z = blockIdx.x / rows_per_plane;
+  const int ya = (blockIdx.x % rows_per_plane) * (2 * XR) + 2 * w;
+  const size_t plane = (size_t)N * g.kxp;           // one z plane of one component in R / PX
+  const size_t vol = (size_t)N * N * N;
+
+  cx<T> t3[7];
+#pragma unroll
+  for (int q = 1; q < 8; ++q) t3[q - 1] = {T(tw[q * j].re), T(tw[q * j].im)};
+
+  // component c of R to physical space: x[q] = (row a, row b) at x = j + 64 q
+  auto inverse = [&](int c, cx<T>(&x)[8]) {
+    const cx<T> *src = rin + (size_t)c * N * plane + (size_t)z * plane + (size_t)ya * g.kxp;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int n = j + 64 * q;
+      cx<T> v = czero<T>();
+      if (n <= M) {
+        cx<T> a = src[n], b = src[g.kxp + n];
+        if (n == 0) {  // the self-conjugate coefficient: only its real part is seen
+          a.im = T(0);
+          b.im = T(0);
+        }
+        v = {a.re - b.im, a.im + b.re};
+      } else if (n >= N - M) {
+        const cx<T> a = src[N - n], b = src[g.kxp + N - n];
+        v = {a.re + b.im, b.re - a.im};
+      }
+      x[q] = v;
+    }
+    wave_fft512<+1>(x, wb, t64, t3, j);
+  };
+
+  // product d to the spectrum, kx <= M kept, zeros up to KXP
+  auto forward = [&](int d, cx<T>(&x)[8]) {
+    wave_fft512<-1>(x, wb, t64, t3, j);
+    cx<T> *dst = pxo + (size_t)d * N * plane + (size_t)z * plane + (size_t)ya * g.kxp;
+    const int partner = (64 - j) & 63;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      // C[N - kx] for kx = j + 64 q: slot 7 - q of lane 64 - j, or slot (8 - q) mod 8 of lane 0
+      cx<T> cm = {lane_value(x[7 - q].re, partner), lane_value(x[7 - q].im, partner)};
+      if (j == 0) cm = x[(8 - q) & 7];
+      const int kx = j + 64 * q;
+      if (kx <= M) {
+        const cx<T> C = x[q];
+        dst[kx] = {T(0.5) * (C.re + cm.re), T(0.5) * (C.im - cm.im)};
+        dst[g.kxp + kx] = {T(0.5) * (C.im + cm.im), T(0.5) * (cm.re - C.re)};
+      } else if (kx < g.kxp) {
+        dst[kx] = czero<T>();
+        dst[g.kxp + kx] = czero<T>();
+      }
+    }
+  };
+
+  const size_t rowa = ((size_t)z * N + ya) * N, rowb = rowa + N;
+  T *phys = reinterpret_cast<T *>(xa.phys);
+  T *physw = reinterpret_cast<T *>(xa.physw);
+  auto store_phys = [&](T *base, const cx<T>(&x)[8]) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      base[rowa + j + 64 * q] = x[q].re;
+      base[rowb + j + 64 * q] = x[q].im;
+    }
+  };
+
+  cx<T> u[3][8];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    inverse(c, u[c]);
+    if (xa.store_u) store_phys(phys + c * vol, u[c]);
+  }
+
+  double mx = 0.0;
+  if (xa.want_max) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const T sa = u[0][q].re * u[0][q].re + u[1][q].re * u[1][q].re + u[2][q].re * u[2][q].re;
+      const T sb = u[0][q].im * u[0][q].im + u[1][q].im * u[1][q].im + u[2][q].im * u[2][q].im;
+      const double da = (double)sa, db = (double)sb;
+      if (da > mx || da != da) mx = da;
+      if (db > mx || db != db) mx = db;
+    }
+  }
+
+  if constexpr (NCIN == 7) {  // the fluxes u phi (multiply!(u, phi, u), src/Generator.jl:727)
+    cx<T> ph[8];
+    inverse(6, ph);
+    if (xa.store_phi) store_phys(phys + 3 * vol, ph);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      cx<T> p[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) p[q] = {u[d][q].re * ph[q].re, u[d][q].im * ph[q].im};
+      forward(3 + d, p);
+    }
+  }
+
+  // the negated Lamb vector L = u x w (cross!(w, u, w), src/Generator.jl:529):
+  //   L_0 = u_1 w_2 - u_2 w_1,  L_1 = u_2 w_0 - u_0 w_2,  L_2 = u_0 w_1 - u_1 w_0
+  cx<T> l0[8], l1[8];
+  {
+    cx<T> w0[8];
+    inverse(3, w0);
+    if (xa.store_w) store_phys(physw, w0);
+    cx<T> l2[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      l1[q] = {u[2][q].re * w0[q].re, u[2][q].im * w0[q].im};
+      l2[q] = {-(u[1][q].re * w0[q].re), -(u[1][q].im * w0[q].im)};
+    }
+    cx<T> w1[8];
+    inverse(4, w1);
+    if (xa.store_w) store_phys(physw + vol, w1);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      l0[q] = {-(u[2][q].re * w1[q].re), -(u[2][q].im * w1[q].im)};
+      l2[q] = {u[0][q].re * w1[q].re + l2[q].re, u[0][q].im * w1[q].im + l2[q].im};
+    }
+    forward(2, l2);
+  }
+  {
+    cx<T> w2[8];
+    inverse(5, w2);
+    if (xa.store_w) store_phys(physw + 2 * vol, w2);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      l0[q] = {u[1][q].re * w2[q].re + l0[q].re, u[1][q].im * w2[q].im + l0[q].im};
+      l1[q] = {l1[q].re - u[0][q].re * w2[q].re, l1[q].im - u[0][q].im * w2[q].im};
+    }
+    forward(0, l0);
+    forward(1, l1);
+  }
+
+  if (xa.want_max) {
+    // wave then block max, NaN-propagating, then one atomic per block
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_down(mx, off, 64);
+      if (o > mx || o != o) mx = o;
+    }
+    if (j == 0) red[w] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double b = red[0];
+      for (int i = 1; i < XR; ++i)
+        if (red[i] > b || red[i] != red[i]) b = red[i];
+      atomicMax(xa.max_bits, (unsigned long long)__double_as_longlong(b));
+    }
+  }
+}
+
+}  // namespace hdns
