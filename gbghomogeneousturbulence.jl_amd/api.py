@@ -213,6 +213,10 @@ def _check_reality(f: np.ndarray, name: str, dtype) -> None:
 
 # --- the run -------------------------------------------------------------------------------
 
+# the kernel families the autotuner schedules, in the order of hdns_schedule's entries
+SCHEDULE_FAMILIES = ("y_passes", "z_forward", "z_inverse", "x_pass")
+
+
 class Simulation:
     """A run: the device context (generator, registers, cache), the tracers and the clock.
 
@@ -404,17 +408,20 @@ class Simulation:
     def autotune(self, reps: int = 3) -> dict:
         """Time both launch schedules of every stage kernel family and keep the fastest (the state is
         left bitwise unchanged); returns {family: (ms_variant0, ms_variant1, chosen)}."""
-        ms = np.zeros(6)
-        chosen = np.zeros(3, dtype=np.int32)
+        ms = np.zeros(2 * len(SCHEDULE_FAMILIES))
+        chosen = np.zeros(len(SCHEDULE_FAMILIES), dtype=np.int32)
         self._call("hdns_autotune", int(reps), L.dptr(ms), chosen.ctypes.data_as(ctypes.POINTER(c_int32)))
         return {name: (float(ms[2 * i]), float(ms[2 * i + 1]), int(chosen[i]))
-                for i, name in enumerate(("y_passes", "z_passes", "x_pass"))}
+                for i, name in enumerate(SCHEDULE_FAMILIES)}
 
     def schedule(self, set_to=None):
-        """Read (and optionally set) the launch schedule: three entries (y, z, x) of 0 or 1."""
-        get = np.zeros(3, dtype=np.int32)
+        """Read (and optionally set) the launch schedule: one entry of 0 or 1 per kernel family
+        (y passes, z forward, z inverse, x pass)."""
+        get = np.zeros(len(SCHEDULE_FAMILIES), dtype=np.int32)
         P = ctypes.POINTER(c_int32)
         s = None if set_to is None else np.ascontiguousarray(set_to, dtype=np.int32)
+        if s is not None and s.shape != (len(SCHEDULE_FAMILIES),):
+            raise ValueError(f"a schedule has {len(SCHEDULE_FAMILIES)} entries, one per kernel family")
         self._call("hdns_schedule", s.ctypes.data_as(P) if s is not None else None, get.ctypes.data_as(P))
         return tuple(int(v) for v in get)
 

@@ -886,13 +886,13 @@ hdns_status hdns_autotune(hdns_ctx *ctx, int32_t reps, double *ms, int32_t *chos
   xa.phys = c.phys;
   xa.physw = c.physw;
   xa.max_bits = c.maxbits;
-  const int saved[3] = {c.sched[0], c.sched[1], c.sched[2]};
-  double best[3][2] = {{0, 0}, {0, 0}, {0, 0}};
-  for (int fam = 0; fam < 3; ++fam) {
+  constexpr int NF = Ctx::NFAM;
+  int saved[NF];
+  for (int f = 0; f < NF; ++f) saved[f] = c.sched[f];
+  double best[NF][2] = {};
+  for (int fam = 0; fam < NF; ++fam) {
     for (int v = 0; v < 2; ++v) {
-      c.sched[0] = saved[0];
-      c.sched[1] = saved[1];
-      c.sched[2] = saved[2];
+      for (int f = 0; f < NF; ++f) c.sched[f] = saved[f];
       c.sched[fam] = v;
       double total = 0.0;
       for (int r = 0; r <= reps; ++r) {  // the first round warms up
@@ -911,19 +911,19 @@ hdns_status hdns_autotune(hdns_ctx *ctx, int32_t reps, double *ms, int32_t *chos
           (void)hipEventElapsedTime(&t[k], e0, e1);
         }
         if (r == 0) continue;
-        total += fam == 0 ? t[1] + t[3] : fam == 1 ? t[0] + t[4] : t[2];
+        total += fam == 0 ? t[1] + t[3] : fam == 1 ? t[4] : fam == 2 ? t[0] : t[2];
       }
       best[fam][v] = total / reps;
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  for (int fam = 0; fam < 3; ++fam) c.sched[fam] = best[fam][1] < best[fam][0] ? 1 : 0;
+  for (int fam = 0; fam < NF; ++fam) c.sched[fam] = best[fam][1] < best[fam][0] ? 1 : 0;
   c.prof = prof;
   if (ms)
-    for (int i = 0; i < 6; ++i) ms[i] = best[i / 2][i % 2];
+    for (int i = 0; i < 2 * NF; ++i) ms[i] = best[i / 2][i % 2];
   if (chosen)
-    for (int i = 0; i < 3; ++i) chosen[i] = c.sched[i];
+    for (int i = 0; i < NF; ++i) chosen[i] = c.sched[i];
   hdns_status s = launched(ctx);
   if (s != HDNS_OK) return s;
   drop_graph(c);  // a captured step holds the old schedule
@@ -934,13 +934,13 @@ hdns_status hdns_schedule(hdns_ctx *ctx, const int32_t *set, int32_t *get) {
   if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
   Ctx &c = ctx->c;
   if (set) {
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < Ctx::NFAM; ++i)
       if (set[i] != 0 && set[i] != 1) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "schedule entries are 0 or 1");
-    for (int i = 0; i < 3; ++i) c.sched[i] = set[i];
+    for (int i = 0; i < Ctx::NFAM; ++i) c.sched[i] = set[i];
     drop_graph(c);
   }
   if (get)
-    for (int i = 0; i < 3; ++i) get[i] = c.sched[i];
+    for (int i = 0; i < Ctx::NFAM; ++i) get[i] = c.sched[i];
   return HDNS_OK;
 }
 

@@ -42,9 +42,11 @@ struct Ctx {
   size_t comp = 0;          // complex elements of one compact component
   size_t buf_a = 0, buf_b = 0;
   int stages = 0;
-  // launch schedule of the y, z and x passes (index into the two variants of each), set by
-  // hdns_autotune; the defaults are the fastest measured at 512^3 on MI355X
-  int sched[3] = {1, 0, 0};
+  // launch schedule of the y passes, the z forward pass, the z inverse pass and the x pass
+  // (index into the two variants of each), set by hdns_autotune; the defaults are the fastest
+  // measured at 512^3 on MI355X
+  static constexpr int NFAM = 4;
+  int sched[NFAM] = {1, 0, 0, 0};
   BandSpec band{};
   hipStream_t stream = nullptr;
   void *uh = nullptr, *hu = nullptr, *ph = nullptr, *hp = nullptr;

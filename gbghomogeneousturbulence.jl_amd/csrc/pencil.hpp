@@ -67,7 +67,7 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cx<T> *__restrict__ px, cx<T>
   const int kx0 = blockIdx.x * TX, z = blockIdx.y, c = blockIdx.z;
   const cx<T> *src = px + ((size_t)c * N + z) * (size_t)N * g.kxp + kx0;
   cx<T> *dst = pf + ((size_t)c * N + z) * (size_t)g.k * g.kxp + kx0;
-  static_assert(TX * sizeof(cx<T>) == 128, "a y tile is one 128-byte line of a row");
+  static_assert(TX * sizeof(cx<T>) <= 128, "a y tile is at most one 128-byte line of a row");
   fft_io<T, N, TX, NT, -1, true, false>(
       buf, twl, [&](int, int, int n, int col) { return src[(size_t)n * g.kxp + col]; },
       [&](int, int, int n, int col, cx<T> v) {

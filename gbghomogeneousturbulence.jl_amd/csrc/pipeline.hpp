@@ -14,17 +14,20 @@ namespace hdns {
 template <typename T>
 constexpr int tile_width_of() { return sizeof(T) == 4 ? 16 : 8; }
 
-// Two launch schedules per stage kernel, chosen per context by hdns_autotune:
+// Two launch schedules per stage kernel family, chosen per context by hdns_autotune:
 //   y passes: 256 or 512 threads per block;
-//   z passes: a TX-wide tile on 512 threads, or a TX/2-wide tile on 64*TX/2 threads (one radix-8
-//             butterfly per thread per pass at N = 512, half the registers, twice the blocks);
+//   z forward: (0) the scalar half and the velocity half as two kernels on 64*TX threads, one
+//              last-pass butterfly per thread at N = 512, the first two transformed velocity
+//              products staged in LDS where they fit (N <= 512); or (1) one fused kernel on 512;
+//   z inverse: a TX-wide tile on 512 threads, or a TX/2-wide tile on 64*TX/2 threads;
 //   x pass: 256 or 512 threads per block.
 constexpr int kYThreads[2] = {256, 512};
 constexpr int kXThreads[2] = {256, 512};
 template <typename T>
 struct ZSched {
-  static constexpr int tx[2] = {tile_width_of<T>(), tile_width_of<T>() / 2};
-  static constexpr int nt[2] = {512, 64 * (tile_width_of<T>() / 2)};
+  static constexpr int zf_nt[2] = {64 * tile_width_of<T>(), 512};
+  static constexpr int zi_tx[2] = {tile_width_of<T>(), tile_width_of<T>() / 2};
+  static constexpr int zi_nt[2] = {512, 64 * (tile_width_of<T>() / 2)};
 };
 
 template <typename T, int N>
@@ -45,14 +48,24 @@ struct PipelineImpl final : Pipeline {
     (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   }
 
+  // the z forward schedule 0 stages two velocity products in LDS when they fit beside the tile
+  static constexpr bool ZFL = zf_lds_bytes<N, TX, T>(true) <= 160 * 1024;
+
   template <int V>
   static void set_all() {
     set_lds((const void *)k_yfwd<T, N, TX, kYThreads[V]>, LDS_COL);
     set_lds((const void *)k_yinv<T, N, TX, kYThreads[V]>, LDS_COL);
-    constexpr int ZT = ZSched<T>::tx[V], ZN = ZSched<T>::nt[V];
+    if constexpr (V == 0) {
+      constexpr int ZN = ZSched<T>::zf_nt[0];
+      set_lds((const void *)k_zs<T, N, TX, ZN>, LDS_COL);
+      set_lds((const void *)k_zf<T, N, TX, ZN, false, ZFL>, zf_lds_bytes<N, TX, T>(ZFL));
+    } else {
+      constexpr int ZN = ZSched<T>::zf_nt[1];
+      set_lds((const void *)k_zf<T, N, TX, ZN, true>, LDS_COL);
+      set_lds((const void *)k_zf<T, N, TX, ZN, false>, LDS_COL);
+    }
+    constexpr int ZT = ZSched<T>::zi_tx[V], ZN = ZSched<T>::zi_nt[V];
     constexpr size_t ZL = ((size_t)N * ZT + N) * sizeof(cx<T>);
-    set_lds((const void *)k_zf<T, N, ZT, ZN, true>, ZL);
-    set_lds((const void *)k_zf<T, N, ZT, ZN, false>, ZL);
     set_lds((const void *)k_zi<T, N, ZT, ZN, true>, ZL);
     set_lds((const void *)k_zi<T, N, ZT, ZN, false>, ZL);
     if constexpr (XWAVE) {
@@ -71,7 +84,7 @@ struct PipelineImpl final : Pipeline {
 
   int tiles(const Ctx &c) const { return c.geo.kxp / TX; }
   int tile_width() const override { return TX; }
-  int z_blocks(const Ctx &c) const override { return c.geo.kxp / ZSched<T>::tx[c.sched[1]] * K; }
+  int z_blocks(const Ctx &c) const override { return c.geo.kxp / ZSched<T>::zi_tx[c.sched[2]] * K; }
 
   template <int NTY>
   void yfwd_v(Ctx &c) {
@@ -83,17 +96,25 @@ struct PipelineImpl final : Pipeline {
 
   template <int V>
   void zf_v(Ctx &c, const StageArgs &sa) {
-    constexpr int ZT = ZSched<T>::tx[V], ZN = ZSched<T>::nt[V];
-    constexpr size_t ZL = ((size_t)N * ZT + N) * sizeof(cx<T>);
-    const dim3 grid(c.geo.kxp / ZT, K);
-    if (c.cfg.has_scalar)
-      hipLaunchKernelGGL((k_zf<T, N, ZT, ZN, true>), grid, dim3(ZN), ZL, c.stream, (const cx<T> *)c.bufb,
-                         (cx<T> *)c.uh, (cx<T> *)c.hu, (cx<T> *)c.ph, (cx<T> *)c.hp, c.geo,
-                         (const cx<double> *)c.tw, sa);
-    else
-      hipLaunchKernelGGL((k_zf<T, N, ZT, ZN, false>), grid, dim3(ZN), ZL, c.stream, (const cx<T> *)c.bufb,
-                         (cx<T> *)c.uh, (cx<T> *)c.hu, (cx<T> *)c.ph, (cx<T> *)c.hp, c.geo,
-                         (const cx<double> *)c.tw, sa);
+    constexpr int ZN = ZSched<T>::zf_nt[V];
+    const dim3 grid(c.geo.kxp / TX, K);
+    const cx<T> *pf = (const cx<T> *)c.bufb;
+    cx<T> *uh = (cx<T> *)c.uh, *hu = (cx<T> *)c.hu, *ph = (cx<T> *)c.ph, *hp = (cx<T> *)c.hp;
+    const cx<double> *tw = (const cx<double> *)c.tw;
+    if constexpr (V == 0) {
+      if (c.cfg.has_scalar)   // the scalar half reads the stage u_z before the velocity half updates it
+        hipLaunchKernelGGL((k_zs<T, N, TX, ZN>), grid, dim3(ZN), LDS_COL, c.stream, pf, (const cx<T> *)uh, ph, hp,
+                           c.geo, tw, sa);
+      hipLaunchKernelGGL((k_zf<T, N, TX, ZN, false, ZFL>), grid, dim3(ZN), (zf_lds_bytes<N, TX, T>(ZFL)), c.stream,
+                         pf, uh, hu, ph, hp, c.geo, tw, sa);
+    } else {
+      if (c.cfg.has_scalar)
+        hipLaunchKernelGGL((k_zf<T, N, TX, ZN, true>), grid, dim3(ZN), LDS_COL, c.stream, pf, uh, hu, ph, hp, c.geo,
+                           tw, sa);
+      else
+        hipLaunchKernelGGL((k_zf<T, N, TX, ZN, false>), grid, dim3(ZN), LDS_COL, c.stream, pf, uh, hu, ph, hp, c.geo,
+                           tw, sa);
+    }
   }
   void zf(Ctx &c, const StageArgs &sa) override { c.sched[1] ? zf_v<1>(c, sa) : zf_v<0>(c, sa); }
 
@@ -105,7 +126,7 @@ struct PipelineImpl final : Pipeline {
 
   template <int V>
   void zi_v(Ctx &c) {
-    constexpr int ZT = ZSched<T>::tx[V], ZN = ZSched<T>::nt[V];
+    constexpr int ZT = ZSched<T>::zi_tx[V], ZN = ZSched<T>::zi_nt[V];
     constexpr size_t ZL = ((size_t)N * ZT + N) * sizeof(cx<T>);
     const dim3 grid(c.geo.kxp / ZT, K);
     if (c.cfg.has_scalar)
@@ -115,7 +136,7 @@ struct PipelineImpl final : Pipeline {
       hipLaunchKernelGGL((k_zi<T, N, ZT, ZN, false>), grid, dim3(ZN), ZL, c.stream, (const cx<T> *)c.uh,
                          (const cx<T> *)c.ph, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, c.band, c.partials);
   }
-  void zi(Ctx &c) override { c.sched[1] ? zi_v<1>(c) : zi_v<0>(c); }
+  void zi(Ctx &c) override { c.sched[2] ? zi_v<1>(c) : zi_v<0>(c); }
 
   template <int NTY>
   void yinv_v(Ctx &c) {
@@ -146,7 +167,7 @@ struct PipelineImpl final : Pipeline {
                          (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
   }
   void xpass(Ctx &c, const XArgs &xa) override {
-    c.sched[2] ? xpass_v<kXThreads[1]>(c, xa) : xpass_v<kXThreads[0]>(c, xa);
+    c.sched[3] ? xpass_v<kXThreads[1]>(c, xa) : xpass_v<kXThreads[0]>(c, xa);
   }
 
   void spectrum(Ctx &c, int which, double *bins) override {

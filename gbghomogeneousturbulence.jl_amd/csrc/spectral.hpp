@@ -42,6 +42,93 @@ constexpr int first_radix() { return N == 16 ? 4 : 8; }
 template <int N>
 constexpr int last_radix() { return (N == 64 || N == 512) ? 8 : 4; }
 
+// The scalar half of a stage (src/Generator.jl:648-730, src/Propagator.jl:366-391): the three
+// fluxes' forward z transforms, N_phi = -mask i k . F - G u_z at the stage velocity, and the
+// phi / h_phi update.  It runs before the velocity update, which overwrites u_z.
+template <typename T, int N, int TX, int NT>
+__device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, const cx<T> *__restrict__ pf,
+                                               const cx<T> *__restrict__ uh, cx<T> *__restrict__ ph,
+                                               cx<T> *__restrict__ hp, const Geo &g, const StageArgs &sa, int kx0,
+                                               int ky_idx, int my) {
+  constexpr int M = N / 3, K = 2 * M + 1;
+  constexpr int RL = last_radix<N>();
+  using SL = PassShape<N, RL, TX, NT>;
+  constexpr int PER = SL::PER;
+  constexpr int QS = 2;   // modes whose state loads are in flight together
+  const int tid = threadIdx.x;
+  const size_t comp = (size_t)K * K * g.kxp;
+  const size_t pencil = (size_t)N * K * g.kxp;
+  const size_t zstride = (size_t)K * g.kxp;
+  cx<T> G[PER][RL];
+#pragma unroll
+  for (int p = 0; p < PER; ++p)
+#pragma unroll
+    for (int q = 0; q < RL; ++q) G[p][q] = czero<T>();
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    __builtin_amdgcn_sched_barrier(0);   // one component's loads in flight at a time
+    const cx<T> *src = pf + (size_t)(3 + c) * pencil + (size_t)ky_idx * g.kxp + kx0;
+    fft_io<T, N, TX, NT, -1, true, false>(
+        buf, twl,
+        [&](int, int, int n, int col) { return src[(size_t)n * zstride + col]; },
+        [&](int p, int q, int n, int col, cx<T> v) {
+          const int mc = c == 0 ? kx0 + col : (c == 1 ? my : signed_of(idx_of_bin(n, g), g));
+          const T kc = T(g.k0 * mc);
+          G[p][q] = {G[p][q].re + kc * v.re, G[p][q].im + kc * v.im};
+        });
+  }
+  const T Gm = T(sa.mean_gradient);
+  const T dt = T(sa.dt), a = T(sa.a), b = T(sa.b);
+  __builtin_amdgcn_sched_barrier(0);   // the state loads stay behind the transforms
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const int bb = tid + p * NT;
+    if (!(SL::TOT % NT == 0 || bb < SL::TOT)) continue;
+    const int col = bb % TX, j = bb / TX;
+    const int mx = kx0 + col;
+#pragma unroll
+    for (int q0 = 0; q0 < RL; q0 += QS) {
+      // the group's loads first, unconditionally (a mode outside the ball, which is zero and
+      // stays zero, reads mode 0 and is not stored), so their latencies overlap
+      int at[QS];
+      cx<T> uz[QS], pv[QS], hv[QS];
+#pragma unroll
+      for (int i = 0; i < QS; ++i) {
+        const int kz_idx = idx_of_bin(j + (q0 + i) * SL::NB, g);
+        const int mz = kz_idx < 0 ? 0 : signed_of(kz_idx, g);
+        at[i] = kz_idx >= 0 && in_two_thirds(mx * mx + my * my + mz * mz, N)
+                    ? kz_idx * (int)zstride + ky_idx * g.kxp + mx
+                    : -1;
+        const int ld = at[i] < 0 ? 0 : at[i];
+        uz[i] = uh[2 * comp + ld];   // the stage velocity: the velocity update comes after
+        pv[i] = ph[ld];
+      }
+      if (sa.stage > 0) {
+#pragma unroll
+        for (int i = 0; i < QS; ++i) hv[i] = hp[at[i] < 0 ? 0 : at[i]];
+      } else {
+#pragma unroll
+        for (int i = 0; i < QS; ++i) hv[i] = czero<T>();
+      }
+#pragma unroll
+      for (int i = 0; i < QS; ++i) {
+        if (at[i] < 0) continue;
+        const int q = q0 + i;
+        const int mz = signed_of(idx_of_bin(j + q * SL::NB, g), g);
+        const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
+        const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
+        const cx<T> nl = cx<T>{G[p][q].im, -G[p][q].re} - scale(uz[i], Gm);   // -i k.F - G u_z
+        const T E = T(exp(sa.coef_p * k2d));
+        cx<T> h = scale(nl, dt);
+        if (sa.stage > 0) h = scale(hv[i], a) + h;
+        ph[at[i]] = scale(pv[i] + scale(h, b), E);
+        if (!sa.last) hp[at[i]] = scale(h, E);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
 // Every spectral operation of one stage (src/Propagator.jl:338-392), fused:
 //   N_u  = P mask FFT_z(PF_u), mean mode zeroed, + A 1_band u        (src/Generator.jl:484-534,
 //                                                                      src/Forcing.jl:1104-1117)
@@ -50,8 +137,16 @@ constexpr int last_radix() { return (N == 64 || N == 512) ? 8 : 4; }
 //   and at the last stage the state Leray projection                  (src/Propagator.jl:401).
 // E = exp(coef |k|^2) is evaluated in FP64 per mode and rounded to T.  A thread owns the modes
 // its last transform pass produces: kz bins n = j + q N/R of column col, for its butterflies
-// (col, j); the product tiles stream from HBM straight into the first pass.
-template <typename T, int N, int TX, int NT, bool SCALAR>
+// (col, j); the product tiles stream from HBM straight into the first pass.  FL: the first two
+// transformed velocity products wait in LDS (retained bins only, [kz_idx][col]) instead of
+// registers, so the stage update holds only the third and more state loads can be in flight;
+// each thread reads back only what it wrote itself.
+template <int N, int TX, typename T>
+constexpr size_t zf_lds_bytes(bool fl) {
+  return ((size_t)N * TX + N + (fl ? 2 * (size_t)(2 * (N / 3) + 1) * TX : 0)) * sizeof(cx<T>);
+}
+
+template <typename T, int N, int TX, int NT, bool SCALAR, bool FL = false>
 __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *__restrict__ uh,
                                            cx<T> *__restrict__ hu, cx<T> *__restrict__ ph,
                                            cx<T> *__restrict__ hp, Geo g, const cx<double> *__restrict__ tw,
@@ -60,6 +155,7 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
   constexpr int RL = last_radix<N>();
   using SL = PassShape<N, RL, TX, NT>;
   constexpr int PER = SL::PER;
+  constexpr int QG = PER > 1 ? 2 : (RL < 4 ? RL : 4);   // modes whose state loads are in flight together
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *twl = buf + N * TX;
@@ -73,71 +169,42 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
   const size_t pencil = (size_t)N * K * g.kxp;         // one PF component
   const size_t zstride = (size_t)K * g.kxp;
 
-  // The scalar first: its mean-gradient source reads the stage velocity before the velocity
-  // update overwrites it.
-  if constexpr (SCALAR) {
-    cx<T> G[PER][RL];
-#pragma unroll
-    for (int p = 0; p < PER; ++p)
-#pragma unroll
-      for (int q = 0; q < RL; ++q) G[p][q] = czero<T>();
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const cx<T> *src = pf + (size_t)(3 + c) * pencil + (size_t)ky_idx * g.kxp + kx0;
-      fft_io<T, N, TX, NT, -1, true, false>(
-          buf, twl,
-          [&](int, int, int n, int col) { return src[(size_t)n * zstride + col]; },
-          [&](int p, int q, int n, int col, cx<T> v) {
-            const int mc = c == 0 ? kx0 + col : (c == 1 ? my : signed_of(idx_of_bin(n, g), g));
-            const T kc = T(g.k0 * mc);
-            G[p][q] = {G[p][q].re + kc * v.re, G[p][q].im + kc * v.im};
-          });
-    }
-    const T Gm = T(sa.mean_gradient);
-    const T dt = T(sa.dt), a = T(sa.a), b = T(sa.b);
-#pragma unroll
-    for (int p = 0; p < PER; ++p) {
-      const int bb = tid + p * NT;
-      if (!(SL::TOT % NT == 0 || bb < SL::TOT)) continue;
-      const int col = bb % TX, j = bb / TX;
-      const int mx = kx0 + col;
-#pragma unroll
-      for (int q = 0; q < RL; ++q) {
-        const int kz_idx = idx_of_bin(j + q * SL::NB, g);
-        if (kz_idx < 0) continue;
-        const int mz = signed_of(kz_idx, g);
-        const int m2 = mx * mx + my * my + mz * mz;
-        if (!in_two_thirds(m2, N)) continue;   // zero, and stays zero
-        const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
-        const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
-        cx<T> nl = {G[p][q].im, -G[p][q].re};   // -i k.F
-        const size_t at = (size_t)kz_idx * zstride + (size_t)ky_idx * g.kxp + mx;
-        nl = nl - scale(uh[2 * comp + at], Gm);   // the stage velocity: the velocity update comes after
-        const T E = T(exp(sa.coef_p * k2d));
-        cx<T> h = scale(nl, dt);
-        if (sa.stage > 0) h = scale(hp[at], a) + h;
-        const cx<T> qn = scale(ph[at] + scale(h, b), E);
-        ph[at] = qn;
-        if (!sa.last) hp[at] = scale(h, E);
-        __builtin_amdgcn_sched_barrier(0);
-      }
-    }
-  }
+  if constexpr (SCALAR) zf_scalar_part<T, N, TX, NT>(buf, twl, pf, uh, ph, hp, g, sa, kx0, ky_idx, my);
 
-  // velocity products, forward z transform, straight into the owned modes
-  cx<T> F[3][PER][RL];
+  // velocity products, forward z transform, straight into the owned modes.  The compiler
+  // memory barrier keeps the scalar part's u_z loads from living on into the velocity part.
+  __asm__ volatile("" ::: "memory");
+  cx<T> F[FL ? 1 : 3][PER][RL];
+  cx<T> *fl = twl + N;   // FL: [2][K][TX]
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
+    __builtin_amdgcn_sched_barrier(0);   // one component's loads in flight at a time
     const cx<T> *src = pf + (size_t)c * pencil + (size_t)ky_idx * g.kxp + kx0;
-    auto &Fc = F[c];
     fft_io<T, N, TX, NT, -1, true, false>(
         buf, twl,
         [&](int, int, int n, int col) { return src[(size_t)n * zstride + col]; },
-        [&](int p, int q, int, int, cx<T> v) { Fc[p][q] = v; });
+        [&](int p, int q, int n, int col, cx<T> v) {
+          if constexpr (FL) {
+            if (c < 2) {
+              const int kz_idx = idx_of_bin(n, g);
+              if (kz_idx >= 0) fl[(c * K + kz_idx) * TX + col] = v;
+            } else {
+              F[0][p][q] = v;
+            }
+          } else {
+            F[c][p][q] = v;
+          }
+        });
   }
+  // the transformed product c of mode (p, q) of this thread, kz_idx its compact index
+  auto Fv = [&](int c, int p, int q, int kz_idx, int col) -> cx<T> {
+    if constexpr (FL) return c < 2 ? fl[(c * K + kz_idx) * TX + col] : F[0][p][q];
+    else return F[c][p][q];
+  };
 
   const T A = sa.band.on ? T(sa.scalars[0]) : T(0);
   const T a = T(sa.a), b = T(sa.b), dt = T(sa.dt);
+  __builtin_amdgcn_sched_barrier(0);   // the state loads stay behind the transforms
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
     const int bb = tid + p * NT;
@@ -145,57 +212,100 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
     const int col = bb % TX, j = bb / TX;
     const int mx = kx0 + col;
 #pragma unroll
-    for (int q = 0; q < RL; ++q) {
-      const int kz_idx = idx_of_bin(j + q * SL::NB, g);
-      if (kz_idx < 0) continue;
-      const int mz = signed_of(kz_idx, g);
-      const int m2 = mx * mx + my * my + mz * mz;
-      if (!in_two_thirds(m2, N)) continue;   // zero, and stays zero
-      const T kx = T(g.k0 * mx), kz = T(g.k0 * mz);
-      const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
-      const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
-      cx<T> nl[3];
-      if (m2 > 0) {
-        const T k2 = T(k2d);
-        const cx<T> kf = {kx * F[0][p][q].re + ky * F[1][p][q].re + kz * F[2][p][q].re,
-                          kx * F[0][p][q].im + ky * F[1][p][q].im + kz * F[2][p][q].im};
-        const cx<T> s = {kf.re / k2, kf.im / k2};
-        nl[0] = F[0][p][q] - scale(s, kx);
-        nl[1] = F[1][p][q] - scale(s, ky);
-        nl[2] = F[2][p][q] - scale(s, kz);
+    for (int q0 = 0; q0 < RL; q0 += QG) {
+      // the group's loads first, unconditionally (a mode outside the ball, which is zero and
+      // stays zero, reads mode 0 and is not stored), so their latencies overlap
+      int at[QG];
+      cx<T> u[QG][3], hv[QG][3];
+#pragma unroll
+      for (int i = 0; i < QG; ++i) {
+        const int kz_idx = idx_of_bin(j + (q0 + i) * SL::NB, g);
+        const int mz = kz_idx < 0 ? 0 : signed_of(kz_idx, g);
+        at[i] = kz_idx >= 0 && in_two_thirds(mx * mx + my * my + mz * mz, N)
+                    ? kz_idx * (int)zstride + ky_idx * g.kxp + mx
+                    : -1;
+        const int ld = at[i] < 0 ? 0 : at[i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[i][c] = uh[c * comp + ld];
+      }
+      if (sa.stage > 0) {
+#pragma unroll
+        for (int i = 0; i < QG; ++i)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) hv[i][c] = hu[c * comp + (at[i] < 0 ? 0 : at[i])];
       } else {
-        nl[0] = nl[1] = nl[2] = czero<T>();
-      }
-      const size_t at = (size_t)kz_idx * zstride + (size_t)ky_idx * g.kxp + mx;
-      cx<T> u[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) u[c] = uh[c * comp + at];
-      if (in_band(m2, sa.band)) {
+        for (int i = 0; i < QG; ++i)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) nl[c] = nl[c] + scale(u[c], A);
-      }
-      const T E = T(exp(sa.coef_u * k2d));
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        cx<T> h = scale(nl[c], dt);
-        if (sa.stage > 0) h = scale(hu[c * comp + at], a) + h;
-        u[c] = scale(u[c] + scale(h, b), E);
-        if (!sa.last) hu[c * comp + at] = scale(h, E);
-      }
-      if (sa.last && m2 > 0) {  // helmholtz_projection!(u, u): P without the mask
-        const T k2 = T(k2d);
-        const cx<T> kf = {kx * u[0].re + ky * u[1].re + kz * u[2].re, kx * u[0].im + ky * u[1].im + kz * u[2].im};
-        const cx<T> s = {kf.re / k2, kf.im / k2};
-        u[0] = u[0] - scale(s, kx);
-        u[1] = u[1] - scale(s, ky);
-        u[2] = u[2] - scale(s, kz);
+          for (int c = 0; c < 3; ++c) hv[i][c] = czero<T>();
       }
 #pragma unroll
-      for (int c = 0; c < 3; ++c) uh[c * comp + at] = u[c];
-      // keep each mode's loads inside its own iteration: hoisting all of them costs occupancy
+      for (int i = 0; i < QG; ++i) {
+        if (at[i] < 0) continue;
+        const int q = q0 + i;
+        const int kz_idx = idx_of_bin(j + q * SL::NB, g);
+        const int mz = signed_of(kz_idx, g);
+        const int m2 = mx * mx + my * my + mz * mz;
+        const T kx = T(g.k0 * mx), kz = T(g.k0 * mz);
+        const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
+        const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
+        cx<T> nl[3];
+        if (m2 > 0) {
+          const T k2 = T(k2d);
+          const cx<T> f0 = Fv(0, p, q, kz_idx, col), f1 = Fv(1, p, q, kz_idx, col), f2 = Fv(2, p, q, kz_idx, col);
+          const cx<T> kf = {kx * f0.re + ky * f1.re + kz * f2.re, kx * f0.im + ky * f1.im + kz * f2.im};
+          const cx<T> s = {kf.re / k2, kf.im / k2};
+          nl[0] = f0 - scale(s, kx);
+          nl[1] = f1 - scale(s, ky);
+          nl[2] = f2 - scale(s, kz);
+        } else {
+          nl[0] = nl[1] = nl[2] = czero<T>();
+        }
+        if (in_band(m2, sa.band)) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) nl[c] = nl[c] + scale(u[i][c], A);
+        }
+        const T E = T(exp(sa.coef_u * k2d));
+        cx<T> un[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          cx<T> h = scale(nl[c], dt);
+          if (sa.stage > 0) h = scale(hv[i][c], a) + h;
+          un[c] = scale(u[i][c] + scale(h, b), E);
+          if (!sa.last) hu[c * comp + at[i]] = scale(h, E);
+        }
+        if (sa.last && m2 > 0) {  // helmholtz_projection!(u, u): P without the mask
+          const T k2 = T(k2d);
+          const cx<T> kf = {kx * un[0].re + ky * un[1].re + kz * un[2].re,
+                            kx * un[0].im + ky * un[1].im + kz * un[2].im};
+          const cx<T> s = {kf.re / k2, kf.im / k2};
+          un[0] = un[0] - scale(s, kx);
+          un[1] = un[1] - scale(s, ky);
+          un[2] = un[2] - scale(s, kz);
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) uh[c * comp + at[i]] = un[c];
+      }
+      // the next group's loads stay behind this group's stores: hoisting them all costs occupancy
       __builtin_amdgcn_sched_barrier(0);
     }
   }
+}
+
+// The scalar half on its own (the velocity half then runs as k_zf without SCALAR): two smaller
+// kernels hold fewer registers than the fused one.
+template <typename T, int N, int TX, int NT>
+__global__ __launch_bounds__(NT) void k_zs(const cx<T> *__restrict__ pf, const cx<T> *__restrict__ uh,
+                                           cx<T> *__restrict__ ph, cx<T> *__restrict__ hp, Geo g,
+                                           const cx<double> *__restrict__ tw, StageArgs sa) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+  cx<T> *twl = buf + N * TX;
+  const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y;
+  const int my = signed_of(ky_idx, g);
+  if (!in_disk(kx0, my, N)) return;   // no column of this tile meets the ball (its PF lines are unwritten)
+  stage_twiddles<T, N, NT>(twl, tw);
+  zf_scalar_part<T, N, TX, NT>(buf, twl, pf, uh, ph, hp, g, sa, kx0, ky_idx, my);
 }
 
 // The inverse z pass of the state: u, w = i k x u and phi (with the 1/N^3 of the inverse
