@@ -20,7 +20,8 @@ constexpr int tile_width_of() { return sizeof(T) == 4 ? 16 : 8; }
 //              last-pass butterfly per thread at N = 512, the first two transformed velocity
 //              products staged in LDS where they fit (N <= 512); or (1) one fused kernel on 512;
 //   z inverse: a TX-wide tile on 512 threads, or a TX/2-wide tile on 64*TX/2 threads;
-//   x pass: 256 or 512 threads per block.
+//   x pass: 256 or 512 threads per block (FP32 at N = 512: the register-resident pass with or
+//           without the prefetch of the next component).
 constexpr int kYThreads[2] = {256, 512};
 constexpr int kXThreads[2] = {256, 512};
 template <typename T>
@@ -69,8 +70,8 @@ struct PipelineImpl final : Pipeline {
     set_lds((const void *)k_zi<T, N, ZT, ZN, true>, ZL);
     set_lds((const void *)k_zi<T, N, ZT, ZN, false>, ZL);
     if constexpr (XWAVE) {
-      set_lds((const void *)k_xpass_wave<T, N, 7, kXThreads[V]>, lds_xw<kXThreads[V]>());
-      set_lds((const void *)k_xpass_wave<T, N, 6, kXThreads[V]>, lds_xw<kXThreads[V]>());
+      set_lds((const void *)k_xpass_wave<T, N, 7, 256, V == 0>, lds_xw<256>());
+      set_lds((const void *)k_xpass_wave<T, N, 6, 256, V == 0>, lds_xw<256>());
     } else {
       set_lds((const void *)k_xpass<T, N, 7, kXThreads[V]>, lds_x<kXThreads[V]>());
       set_lds((const void *)k_xpass<T, N, 6, kXThreads[V]>, lds_x<kXThreads[V]>());
@@ -146,18 +147,21 @@ struct PipelineImpl final : Pipeline {
   }
   void yinv(Ctx &c) override { c.sched[0] ? yinv_v<kYThreads[1]>(c) : yinv_v<kYThreads[0]>(c); }
 
+  // FP32 at N = 512: schedule 0 prefetches the next component, schedule 1 does not (256 threads)
+  template <bool PF>
+  void xpass_wave_v(Ctx &c, const XArgs &xa) {
+    constexpr int XT = 256;
+    const int blocks = N * N / (2 * (XT / 64));
+    if (c.cfg.has_scalar)
+      hipLaunchKernelGGL((k_xpass_wave<T, N, 7, XT, PF>), dim3(blocks), dim3(XT), lds_xw<XT>(), c.stream,
+                         (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
+    else
+      hipLaunchKernelGGL((k_xpass_wave<T, N, 6, XT, PF>), dim3(blocks), dim3(XT), lds_xw<XT>(), c.stream,
+                         (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
+  }
+
   template <int XT>
   void xpass_v(Ctx &c, const XArgs &xa) {
-    if constexpr (XWAVE) {
-      const int blocks = N * N / (2 * (XT / 64));
-      if (c.cfg.has_scalar)
-        hipLaunchKernelGGL((k_xpass_wave<T, N, 7, XT>), dim3(blocks), dim3(XT), lds_xw<XT>(), c.stream,
-                           (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
-      else
-        hipLaunchKernelGGL((k_xpass_wave<T, N, 6, XT>), dim3(blocks), dim3(XT), lds_xw<XT>(), c.stream,
-                           (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
-      return;
-    }
     const int blocks = N * N / (2 * xr<XT>());
     if (c.cfg.has_scalar)
       hipLaunchKernelGGL((k_xpass<T, N, 7, XT>), dim3(blocks), dim3(XT), lds_x<XT>(), c.stream,
@@ -167,7 +171,8 @@ struct PipelineImpl final : Pipeline {
                          (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
   }
   void xpass(Ctx &c, const XArgs &xa) override {
-    c.sched[3] ? xpass_v<kXThreads[1]>(c, xa) : xpass_v<kXThreads[0]>(c, xa);
+    if constexpr (XWAVE) c.sched[3] ? xpass_wave_v<false>(c, xa) : xpass_wave_v<true>(c, xa);
+    else c.sched[3] ? xpass_v<kXThreads[1]>(c, xa) : xpass_v<kXThreads[0]>(c, xa);
   }
 
   void spectrum(Ctx &c, int which, double *bins) override {

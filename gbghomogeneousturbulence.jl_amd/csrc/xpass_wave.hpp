@@ -70,8 +70,10 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-template <typename T, int N, int NCIN, int XT>
-__global__ __launch_bounds__(XT) void k_xpass_wave(const cx<T> *__restrict__ rin, cx<T> *__restrict__ pxo, Geo g,
+// PF: fetch the next component while the current one transforms (more registers, two waves per
+// SIMD), or fetch each component just before its transform (three waves per SIMD).
+template <typename T, int N, int NCIN, int XT, bool PF>
+__global__ __launch_bounds__(XT, PF ? 2 : 3) void k_xpass_wave(const cx<T> *__restrict__ rin, cx<T> *__restrict__ pxo, Geo g,
                                                    const cx<double> *__restrict__ tw, XArgs xa) {
   static_assert(N == 512, "three radix-8 passes on 64 lanes");
   static_assert(NCIN == 7 || NCIN == 6, "u, w and optionally phi");
@@ -96,26 +98,48 @@ __global__ __launch_bounds__(XT) void k_xpass_wave(const cx<T> *__restrict__ rin
 #pragma unroll
   for (int q = 1; q < 8; ++q) t3[q - 1] = {T(tw[q * j].re), T(tw[q * j].im)};
 
-  // component c of R to physical space: x[q] = (row a, row b) at x = j + 64 q
-  auto inverse = [&](int c, cx<T>(&x)[8]) {
-    const cx<T> *src = rin + (size_t)c * N * plane + (size_t)z * plane + (size_t)ya * g.kxp;
+  // The coefficients of component c of R that the first pass of its inverse transform reads:
+  // slot q holds (a, b) at kx = j + 64 q (when <= M) or at kx = N - j - 64 q (when >= N - M),
+  // and zeros in between.  The next component's coefficients are fetched while the current
+  // one transforms, so its HBM latency hides behind the transform.
+  const cx<T> *rbase = rin + (size_t)z * plane + (size_t)ya * g.kxp;
+  cx<T> ra[8], rb[8];
+  auto fetch = [&](int c) {
+    const cx<T> *src = rbase + (size_t)c * N * plane;
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int n = j + 64 * q;
-      cx<T> v = czero<T>();
+      ra[q] = czero<T>();
+      rb[q] = czero<T>();
       if (n <= M) {
-        cx<T> a = src[n], b = src[g.kxp + n];
+        ra[q] = src[n];
+        rb[q] = src[g.kxp + n];
+      } else if (n >= N - M) {
+        ra[q] = src[N - n];
+        rb[q] = src[g.kxp + N - n];
+      }
+    }
+  };
+  // pack the fetched coefficients of component c as the spectrum of (row a) + i (row b), fetch
+  // component `next` (if any), and transform to physical space: x[q] = (row a, row b) at
+  // x = j + 64 q
+  auto inverse = [&](cx<T>(&x)[8], int c, int next) {
+    if (!PF) fetch(c);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int n = j + 64 * q;
+      cx<T> a = ra[q], b = rb[q];
+      if (n <= M) {
         if (n == 0) {  // the self-conjugate coefficient: only its real part is seen
           a.im = T(0);
           b.im = T(0);
         }
-        v = {a.re - b.im, a.im + b.re};
-      } else if (n >= N - M) {
-        const cx<T> a = src[N - n], b = src[g.kxp + N - n];
-        v = {a.re + b.im, b.re - a.im};
+        x[q] = {a.re - b.im, a.im + b.re};
+      } else {  // the mirrored coefficient, or zero
+        x[q] = {a.re + b.im, b.re - a.im};
       }
-      x[q] = v;
     }
+    if (PF && next >= 0) fetch(next);
     wave_fft512<+1>(x, wb, t64, t3, j);
   };
 
@@ -152,10 +176,12 @@ __global__ __launch_bounds__(XT) void k_xpass_wave(const cx<T> *__restrict__ rin
     }
   };
 
+  constexpr int W0 = 3, PHI = 6;   // components of R: u (0..2), w (3..5), phi (6)
   cx<T> u[3][8];
+  if (PF) fetch(0);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    inverse(c, u[c]);
+    inverse(u[c], c, c < 2 ? c + 1 : (NCIN == 7 ? PHI : W0));
     if (xa.store_u) store_phys(phys + c * vol, u[c]);
   }
 
@@ -173,7 +199,7 @@ __global__ __launch_bounds__(XT) void k_xpass_wave(const cx<T> *__restrict__ rin
 
   if constexpr (NCIN == 7) {  // the fluxes u phi (multiply!(u, phi, u), src/Generator.jl:727)
     cx<T> ph[8];
-    inverse(6, ph);
+    inverse(ph, PHI, W0);
     if (xa.store_phi) store_phys(phys + 3 * vol, ph);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -189,7 +215,7 @@ __global__ __launch_bounds__(XT) void k_xpass_wave(const cx<T> *__restrict__ rin
   cx<T> l0[8], l1[8];
   {
     cx<T> w0[8];
-    inverse(3, w0);
+    inverse(w0, W0, W0 + 1);
     if (xa.store_w) store_phys(physw, w0);
     cx<T> l2[8];
 #pragma unroll
@@ -198,7 +224,7 @@ __global__ __launch_bounds__(XT) void k_xpass_wave(const cx<T> *__restrict__ rin
       l2[q] = {-(u[1][q].re * w0[q].re), -(u[1][q].im * w0[q].im)};
     }
     cx<T> w1[8];
-    inverse(4, w1);
+    inverse(w1, W0 + 1, W0 + 2);
     if (xa.store_w) store_phys(physw + vol, w1);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -209,7 +235,7 @@ __global__ __launch_bounds__(XT) void k_xpass_wave(const cx<T> *__restrict__ rin
   }
   {
     cx<T> w2[8];
-    inverse(5, w2);
+    inverse(w2, W0 + 2, -1);
     if (xa.store_w) store_phys(physw + 2 * vol, w2);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
