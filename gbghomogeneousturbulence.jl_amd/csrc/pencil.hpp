@@ -14,6 +14,7 @@
 //   R  [c][z][y][kx]   after the inverse y pass
 // PX/Q share one buffer and PF/R another: each is dead before its twin is written.
 #pragma once
+#include "bufio.hpp"
 #include "fft.hpp"
 
 namespace hdns {

@@ -19,7 +19,8 @@ constexpr int tile_width_of() { return sizeof(T) == 4 ? 16 : 8; }
 //   z forward: (0) the scalar half and the velocity half as two kernels on 64*TX threads, one
 //              last-pass butterfly per thread at N = 512, the first two transformed velocity
 //              products staged in LDS where they fit (N <= 512); or (1) one fused kernel on 512;
-//   z inverse: a TX-wide tile on 512 threads, or a TX/2-wide tile on 64*TX/2 threads;
+//   z inverse: a TX-wide tile on 64*TX threads (one first-pass butterfly per thread at N = 512)
+//              or on 32*TX threads (two);
 //   x pass: 256 or 512 threads per block (FP32 at N = 512: the register-resident pass with or
 //           without the prefetch of the next component).
 constexpr int kYThreads[2] = {256, 512};
@@ -27,8 +28,8 @@ constexpr int kXThreads[2] = {256, 512};
 template <typename T>
 struct ZSched {
   static constexpr int zf_nt[2] = {64 * tile_width_of<T>(), 512};
-  static constexpr int zi_tx[2] = {tile_width_of<T>(), tile_width_of<T>() / 2};
-  static constexpr int zi_nt[2] = {512, 64 * (tile_width_of<T>() / 2)};
+  static constexpr int zi_tx[2] = {tile_width_of<T>(), tile_width_of<T>()};
+  static constexpr int zi_nt[2] = {64 * tile_width_of<T>(), 32 * tile_width_of<T>()};
 };
 
 template <typename T, int N>

@@ -389,7 +389,7 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
 
 #pragma unroll
   for (int c = 0; c < NCIN; ++c) {
-    cx<T> *dst = q + (size_t)c * pencil + (size_t)ky_idx * g.kxp + kx0;
+    const Buf<T> dst(q + (size_t)c * pencil + (size_t)ky_idx * g.kxp + kx0, pencil);
     fft_io<T, N, TX, NT, +1, true, false>(
         buf, twl,
         [&](int pp, int qq, int n, int col) {
@@ -409,7 +409,7 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
           }
           return scale(v, inv);
         },
-        [&](int, int, int n, int col, cx<T> v) { dst[(size_t)n * zstride + col] = v; });
+        [&](int, int, int n, int col, cx<T> v) { dst.store(v, (uint32_t)(n * (int)zstride + col), 0u); });
   }
 
   // deterministic block reduction: wave shuffles, then the waves in order
