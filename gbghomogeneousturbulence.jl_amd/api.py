@@ -27,6 +27,10 @@ class TwoThirds:
         return "TwoThirds()"
 
 
+# the transform lengths with a radix plan in the library: 2^k and 3 * 2^k
+SUPPORTED_GRIDS = (16, 32, 48, 64, 96, 128, 192, 256, 512, 1024)
+
+
 class FixedDomain:
     """The periodic box a run lives on (src/SimulationDomain.jl:199-328).
 
@@ -38,8 +42,9 @@ class FixedDomain:
     def __init__(self, *, grid_points_per_axis: int, side_length: float, dealiasing, element_type,
                  device: int = 0):
         n = int(grid_points_per_axis)
-        if n < 16 or n > 1024 or n & (n - 1):
-            raise NotImplementedError(f"grid_points_per_axis must be a power of two in 16..1024, got {n}")
+        if n not in SUPPORTED_GRIDS:
+            raise NotImplementedError(f"grid_points_per_axis must be a power of two in 16..1024, or 48, 96 or 192; "
+                                      f"got {n}")
         if not (side_length > 0 and math.isfinite(side_length)):
             raise ValueError("side_length must be positive and finite")
         if not isinstance(dealiasing, TwoThirds):

@@ -42,6 +42,8 @@ hdns_status launched(hdns_ctx *ctx) {
 }
 
 bool power_of_two(int n) { return n > 0 && (n & (n - 1)) == 0; }
+// the transform lengths with a radix plan: 2^k (16..1024) and 3 * 2^k (48, 96, 192)
+bool supported_grid(int n) { return (power_of_two(n) && n >= 16 && n <= 1024) || n == 48 || n == 96 || n == 192; }
 
 // The element-type value of a rational p/q: rounded once for FP64; for FP32 the nearest float
 // to the extended-precision quotient.
@@ -234,9 +236,10 @@ hdns_status hdns_ctx_create(const hdns_config *config, hdns_ctx **out) {
   if (!config || !out) return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "config and out must not be NULL");
   *out = nullptr;
   const hdns_config &k = *config;
-  if (!power_of_two(k.grid_points) || k.grid_points < 16 || k.grid_points > 1024)
+  if (!supported_grid(k.grid_points))
     return fail(nullptr, HDNS_ERR_UNSUPPORTED,
-                "grid_points must be a power of two in 16..1024, got " + std::to_string(k.grid_points));
+                "grid_points must be a power of two in 16..1024, or 48, 96 or 192; got " +
+                    std::to_string(k.grid_points));
   if (!(k.side_length > 0) || !isfinite(k.side_length))
     return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "side_length must be positive and finite");
   if (k.precision != HDNS_FP32_BULK && k.precision != HDNS_FP64)

@@ -223,6 +223,9 @@ __device__ __forceinline__ void fft_columns(cx<T> *buf, const cx<T> *tw) {
   else if constexpr (N == 256) run_passes<T, N, TX, NT, SIGN, IL, WAVE, 1, 8, 8, 4>(buf, tw);
   else if constexpr (N == 512) run_passes<T, N, TX, NT, SIGN, IL, WAVE, 1, 8, 8, 8>(buf, tw);
   else if constexpr (N == 1024) run_passes<T, N, TX, NT, SIGN, IL, WAVE, 1, 8, 8, 4, 4>(buf, tw);
+  else if constexpr (N == 48) run_passes<T, N, TX, NT, SIGN, IL, WAVE, 1, 4, 4, 3>(buf, tw);
+  else if constexpr (N == 96) run_passes<T, N, TX, NT, SIGN, IL, WAVE, 1, 8, 4, 3>(buf, tw);
+  else if constexpr (N == 192) run_passes<T, N, TX, NT, SIGN, IL, WAVE, 1, 8, 8, 3>(buf, tw);
   else static_assert(N == 16, "unsupported transform length");
 }
 
@@ -339,6 +342,10 @@ __device__ __forceinline__ void fft_io(cx<T> *buf, const cx<T> *tw, Load &&load,
     fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 8, 8>{}, buf, tw, load, store);
   else if constexpr (N == 1024)
     fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 8, 4, 4>{}, buf, tw, load, store);
+  else if constexpr (N == 48) fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<4, 4, 3>{}, buf, tw, load, store);
+  else if constexpr (N == 96) fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 4, 3>{}, buf, tw, load, store);
+  else if constexpr (N == 192)
+    fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 8, 3>{}, buf, tw, load, store);
   else static_assert(N == 16, "unsupported transform length");
 }
 

@@ -257,7 +257,7 @@ __global__ __launch_bounds__(XT) void k_xpass(const cx<T> *__restrict__ rin,
       cx<T> A = czero<T>(), B = czero<T>();
       if (kx <= M) {
         const cx<T> C = wb[kx];
-        const cx<T> Cm = wb[(N - kx) & (N - 1)];
+        const cx<T> Cm = wb[kx == 0 ? 0 : N - kx];
         A = {T(0.5) * (C.re + Cm.re), T(0.5) * (C.im - Cm.im)};
         B = {T(0.5) * (C.im + Cm.im), T(0.5) * (Cm.re - C.re)};
       }

@@ -217,6 +217,9 @@ std::unique_ptr<Pipeline> make_pipeline_t(int n) {
   switch (n) {
     case 16: return std::make_unique<PipelineImpl<T, 16>>();
     case 32: return std::make_unique<PipelineImpl<T, 32>>();
+    case 48: return std::make_unique<PipelineImpl<T, 48>>();
+    case 96: return std::make_unique<PipelineImpl<T, 96>>();
+    case 192: return std::make_unique<PipelineImpl<T, 192>>();
     case 64: return std::make_unique<PipelineImpl<T, 64>>();
     case 128: return std::make_unique<PipelineImpl<T, 128>>();
     case 256: return std::make_unique<PipelineImpl<T, 256>>();

@@ -38,9 +38,9 @@ struct StageArgs {
 
 // Radix of the first and the last pass of a length-N transform (fft_io's plans).
 template <int N>
-constexpr int first_radix() { return N == 16 ? 4 : 8; }
+constexpr int first_radix() { return (N == 16 || N == 48) ? 4 : 8; }
 template <int N>
-constexpr int last_radix() { return (N == 64 || N == 512) ? 8 : 4; }
+constexpr int last_radix() { return N % 3 == 0 ? 3 : ((N == 64 || N == 512) ? 8 : 4); }
 
 // The scalar half of a stage (src/Generator.jl:648-730, src/Propagator.jl:366-391): the three
 // fluxes' forward z transforms, N_phi = -mask i k . F - G u_z at the stage velocity, and the
@@ -54,7 +54,8 @@ __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, con
   constexpr int RL = last_radix<N>();
   using SL = PassShape<N, RL, TX, NT>;
   constexpr int PER = SL::PER;
-  constexpr int QS = 2;   // modes whose state loads are in flight together
+  constexpr int QS = RL % 2 == 0 ? 2 : RL;   // modes whose state loads are in flight together
+  static_assert(RL % QS == 0, "the load groups tile the last pass's outputs");
   const int tid = threadIdx.x;
   const size_t comp = (size_t)K * K * g.kxp;
   const size_t pencil = (size_t)N * K * g.kxp;
@@ -155,7 +156,8 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
   constexpr int RL = last_radix<N>();
   using SL = PassShape<N, RL, TX, NT>;
   constexpr int PER = SL::PER;
-  constexpr int QG = PER > 1 ? 2 : (RL < 4 ? RL : 4);   // modes whose state loads are in flight together
+  constexpr int QG = RL % 2 != 0 ? RL : (PER > 1 ? 2 : (RL < 4 ? RL : 4));   // state loads in flight together
+  static_assert(RL % QG == 0, "the load groups tile the last pass's outputs");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *twl = buf + N * TX;

@@ -4,12 +4,19 @@
 // Positions and samples are stored [3][M] (the reference's (M, 3) column-major PointSet matrix).
 // Grid point (i, j, k) sits at (i, j, k) * L/N; the field is [c][z][y][x].  The cell of x is
 // floor(x * (N/L)) with N/L rounded once to the element type and the product formed in it, the
-// contract the tests hold bit-exact.  N is a power of two, so periodic indices wrap with a mask.
+// contract the tests hold bit-exact.  Periodic indices wrap modulo N (a mask when N is a power of
+// two); any integer, including the cast of a non-finite coordinate, lands inside the grid.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace hdns {
+
+__device__ __forceinline__ int wrap_index(int i, int n) {
+  if ((n & (n - 1)) == 0) return i & (n - 1);
+  const int r = i % n;
+  return r < 0 ? r + n : r;
+}
 
 template <typename T>
 __device__ __forceinline__ T wrap_box(T x, T L) {
@@ -40,25 +47,24 @@ __device__ __forceinline__ Stencil stencil_of(T x, T y, T z, T inv_dx, int n, T 
   catmull_rom(sx - fx, wx);
   catmull_rom(sy - fy, wy);
   catmull_rom(sz - fz, wz);
-  return {((int)fx) & (n - 1), ((int)fy) & (n - 1), ((int)fz) & (n - 1)};
+  return {wrap_index((int)fx, n), wrap_index((int)fy, n), wrap_index((int)fz, n)};
 }
 
 // Separable 4^3 gather of NF components (stride vol apart) at one point.
 template <typename T, int NF>
 __device__ __forceinline__ void gather(const T *__restrict__ f, size_t vol, int n, const Stencil &s, const T *wx,
                                        const T *wy, const T *wz, T *out) {
-  const int msk = n - 1;
   int ii[4];
 #pragma unroll
-  for (int a = 0; a < 4; ++a) ii[a] = (s.i + a - 1) & msk;
+  for (int a = 0; a < 4; ++a) ii[a] = wrap_index(s.i + a - 1, n);
 #pragma unroll
   for (int q = 0; q < NF; ++q) out[q] = T(0);
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const int kk = (s.k + c - 1) & msk;
+    const int kk = wrap_index(s.k + c - 1, n);
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const int jj = (s.j + b - 1) & msk;
+      const int jj = wrap_index(s.j + b - 1, n);
       const size_t row = ((size_t)kk * n + jj) * n;
       const T wzy = wz[c] * wy[b];
 #pragma unroll
@@ -130,10 +136,9 @@ __global__ void k_cell_keys(const T *__restrict__ pos, int64_t *__restrict__ key
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= ta.m) return;
   const T inv_dx = T(ta.n / ta.side_length);
-  const int msk = ta.n - 1;
-  const int i = ((int)floor(pos[p] * inv_dx)) & msk;
-  const int j = ((int)floor(pos[ta.m + p] * inv_dx)) & msk;
-  const int k = ((int)floor(pos[2 * ta.m + p] * inv_dx)) & msk;
+  const int i = wrap_index((int)floor(pos[p] * inv_dx), ta.n);
+  const int j = wrap_index((int)floor(pos[ta.m + p] * inv_dx), ta.n);
+  const int k = wrap_index((int)floor(pos[2 * ta.m + p] * inv_dx), ta.n);
   keys[p] = ((int64_t)k * ta.n + j) * ta.n + i;
 }
 
@@ -141,10 +146,10 @@ __global__ void k_cell_keys(const T *__restrict__ pos, int64_t *__restrict__ key
 // The order only reorders the visits, so every gather gives the same values with or without it.
 template <typename T>
 __device__ __forceinline__ int tile_of(const T *pos, int64_t m, int64_t p, int n, T inv_dx) {
-  const int msk = n - 1, nt = n >> 3;
-  const int i = ((int)floor(pos[p] * inv_dx)) & msk;
-  const int j = ((int)floor(pos[m + p] * inv_dx)) & msk;
-  const int k = ((int)floor(pos[2 * m + p] * inv_dx)) & msk;
+  const int nt = n >> 3;
+  const int i = wrap_index((int)floor(pos[p] * inv_dx), n);
+  const int j = wrap_index((int)floor(pos[m + p] * inv_dx), n);
+  const int k = wrap_index((int)floor(pos[2 * m + p] * inv_dx), n);
   return ((k >> 3) * nt + (j >> 3)) * nt + (i >> 3);
 }
 

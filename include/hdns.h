@@ -45,7 +45,7 @@ enum { HDNS_FIELD_VELOCITY = 0, HDNS_FIELD_SCALAR = 1, HDNS_FIELD_VORTICITY = 2 
  * the scheme (src/Propagator.jl:264-320), and the tracer set (src/Particles.jl:135-176). */
 typedef struct {
   int32_t device;            /* HIP device ordinal */
-  int32_t grid_points;       /* N (the box is N^3); a power of two, 16..1024 */
+  int32_t grid_points;       /* N (the box is N^3): a power of two in 16..1024, or 48, 96, 192 */
   double side_length;        /* L */
   int32_t precision;         /* HDNS_FP32_BULK or HDNS_FP64 */
   int32_t scheme;            /* HDNS_IF_LOW_STORAGE_RK3 / _RK4 */

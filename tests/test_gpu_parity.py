@@ -35,7 +35,7 @@ def make_sim(g, n, dtype, *, scheme=None, forcing=None, G=1.0, nu=0.02, seed=7, 
     return sim, u, p, ocond
 
 
-@pytest.mark.parametrize("n", [16, 64])
+@pytest.mark.parametrize("n", [16, 48, 64])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_frame_tables_bit_exact(g, n, dtype):
     sim, *_ = make_sim(g, n, dtype)
@@ -46,9 +46,9 @@ def test_frame_tables_bit_exact(g, n, dtype):
     assert np.array_equal(mask.astype(bool), d.mask)
 
 
+@pytest.mark.parametrize("n", [32, 48])
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 3e-6)])
-def test_deposit_matches_the_oracle(g, dtype, tol):
-    n = 32
+def test_deposit_matches_the_oracle(g, n, dtype, tol):
     sim, u, p, oc = make_sim(g, n, dtype, keep_vorticity=True, forcing=0.1)
     d = dns.Domain(n)
     with sim:
@@ -68,10 +68,12 @@ def test_deposit_matches_the_oracle(g, dtype, tol):
         assert np.max(np.abs(e[: len(eo)] - eo)) <= 10 * tol * np.max(eo) + 1e-300
 
 
+@pytest.mark.parametrize("n", [32, 48, 96])
 @pytest.mark.parametrize("scheme", ["rk3", "rk4"])
 @pytest.mark.parametrize("forcing", [None, 0.1])
-def test_one_step_fp64_matches_the_oracle(g, scheme, forcing):
-    n = 32
+def test_one_step_fp64_matches_the_oracle(g, n, scheme, forcing):
+    """48 and 96 carry radix-3 passes, and 3 | N puts the boundary shell 9|m|^2 = N^2 inside the
+    ball (test/regressions/r6-inclusive-ball-exact.jl)."""
     sch = g.IFLowStorageRK4 if scheme == "rk4" else g.IFLowStorageRK3
     sim, u, p, oc = make_sim(g, n, np.float64, scheme=sch, forcing=forcing)
     d = dns.Domain(n)
@@ -131,9 +133,10 @@ def test_no_scalar_run_matches_the_oracle(g):
         assert rel(sim.state().velocity_spectral, uo) < 1e-12
 
 
+@pytest.mark.parametrize("n", [32, 48])
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 5e-5)])
-def test_tracers_match_the_oracle(g, dtype, tol):
-    n, m, dt = 32, 500, 2e-3
+def test_tracers_match_the_oracle(g, n, dtype, tol):
+    m, dt = 500, 2e-3
     pos = trc.uniform_positions(m, L, 5).T.copy()          # [3, M]
     pos = pos.astype(dtype).astype(np.float64)
     sim, u, p, oc = make_sim(g, n, dtype, tracers=g.Tracers(
