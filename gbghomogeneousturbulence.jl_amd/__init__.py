@@ -5,13 +5,13 @@ The folder name carries a dot, so it is imported by path; ``load_package()`` in
 ``__graft_entry__.py`` registers it as the module ``gbg_amd``.
 """
 from . import _lib
-from .api import (ConstantPower, FixedDomain, FlowConditions, Heun, IFLowStorageRK3, IFLowStorageRK4,
-                  NoForcing, ResolvedFields, Simulation, Tracers, TwoThirds, predicted_viscosity,
-                  recommended_cfl, truncate_to_band)
+from .api import (ConstantPower, Ensemble, EnsembleBox, EnsembleMember, FixedDomain, FlowConditions, Heun,
+                  IFLowStorageRK3, IFLowStorageRK4, NoForcing, ResolvedFields, Simulation, Tracers, TwoThirds,
+                  predicted_viscosity, recommended_cfl, truncate_to_band)
 from .seeding import random_phase_scalar, random_phase_velocity, uniform_positions
 
 __all__ = [
-    "ConstantPower", "FixedDomain", "FlowConditions", "Heun", "IFLowStorageRK3", "IFLowStorageRK4",
+    "ConstantPower", "Ensemble", "EnsembleBox", "EnsembleMember", "FixedDomain", "FlowConditions", "Heun", "IFLowStorageRK3", "IFLowStorageRK4",
     "NoForcing", "ResolvedFields", "Simulation", "Tracers", "TwoThirds", "predicted_viscosity",
     "recommended_cfl", "truncate_to_band", "random_phase_scalar", "random_phase_velocity",
     "uniform_positions",

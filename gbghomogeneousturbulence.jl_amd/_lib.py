@@ -77,6 +77,15 @@ _SIGS = {
     "hdns_profile_read": [c_void_p, _D, POINTER(c_int64), c_int32],
     "hdns_synchronize": [c_void_p],
     "hdns_stream": [c_void_p, POINTER(c_void_p)],
+    "hdns_ctx_create_batch": [POINTER(HdnsConfig), c_int32, _D, POINTER(c_void_p)],
+    "hdns_select_box": [c_void_p, c_int32],
+    "hdns_ensemble_create": [POINTER(HdnsConfig), c_int32, POINTER(c_void_p)],
+    "hdns_ensemble_destroy": [c_void_p],
+    "hdns_ensemble_member": [c_void_p, c_int32, POINTER(c_void_p)],
+    "hdns_ensemble_step": [c_void_p, c_double],
+    "hdns_ensemble_step_graph": [c_void_p, c_double],
+    "hdns_ensemble_advance_tracers": [c_void_p, c_double],
+    "hdns_ensemble_synchronize": [c_void_p],
 }
 
 
@@ -99,15 +108,20 @@ def lib() -> ctypes.CDLL:
             fn.restype = c_int32
         L.hdns_last_error.argtypes = [c_void_p]
         L.hdns_last_error.restype = c_char_p
+        L.hdns_ensemble_last_error.argtypes = [c_void_p]
+        L.hdns_ensemble_last_error.restype = c_char_p
         _lib = L
     return _lib
 
 
-def check(status: int, ctx=None) -> None:
+def check(status: int, ctx=None, ensemble=None) -> None:
     """Map a status to the reference's exception classes, with the library's message."""
     if status == OK:
         return
-    msg = (lib().hdns_last_error(ctx) or b"").decode(errors="replace")
+    if ensemble is not None:
+        msg = (lib().hdns_ensemble_last_error(ensemble) or b"").decode(errors="replace")
+    else:
+        msg = (lib().hdns_last_error(ctx) or b"").decode(errors="replace")
     if status == ERR_INVALID_ARGUMENT:
         raise ValueError(msg)
     if status == ERR_NONFINITE:

@@ -218,6 +218,33 @@ def _check_reality(f: np.ndarray, name: str, dtype) -> None:
 
 # --- the run -------------------------------------------------------------------------------
 
+def _config(domain: FixedDomain, conditions: FlowConditions, scheme, carries_scalar: bool, mean_gradient: float,
+            tracers, tracer_capacity, keep_physical: bool, keep_vorticity: bool) -> L.HdnsConfig:
+    """The C-ABI configuration of one box."""
+    cfg = L.HdnsConfig()
+    cfg.device = domain.device
+    cfg.grid_points = domain.grid_points_per_axis
+    cfg.side_length = domain.side_length
+    cfg.precision = domain.precision
+    cfg.scheme = scheme.code
+    cfg.dealiasing = 0
+    cfg.has_scalar = int(carries_scalar)
+    cfg.viscosity = conditions.viscosity
+    cfg.schmidt = conditions.schmidt
+    f = conditions.velocity_forcing
+    if isinstance(f, ConstantPower):
+        cfg.velocity_forcing = L.CONSTANT_POWER
+        cfg.injection_rate = f.injection_rate
+        cfg.lowest_shell = f.lowest_shell
+        cfg.highest_shell = -1 if f.highest_shell == math.inf else f.highest_shell
+    cfg.mean_gradient = float(mean_gradient)
+    cap = tracers.count if tracers is not None else 0
+    cfg.tracer_capacity = int(tracer_capacity if tracer_capacity is not None else cap)
+    keep = keep_physical or tracers is not None
+    cfg.store_physical = (1 | (2 if carries_scalar else 0) if keep else 0) | (4 if keep_vorticity else 0)
+    return cfg
+
+
 # the kernel families the autotuner schedules, in the order of hdns_schedule's entries
 SCHEDULE_FAMILIES = ("y_passes", "z_forward", "z_inverse", "x_pass")
 
@@ -260,27 +287,8 @@ class Simulation:
         self.time = 0.0
         self.step = 0
         self.carries_scalar = carries_scalar
-        cfg = L.HdnsConfig()
-        cfg.device = domain.device
-        cfg.grid_points = n
-        cfg.side_length = domain.side_length
-        cfg.precision = domain.precision
-        cfg.scheme = field_integration_scheme.code
-        cfg.dealiasing = 0
-        cfg.has_scalar = int(carries_scalar)
-        cfg.viscosity = conditions.viscosity
-        cfg.schmidt = conditions.schmidt
-        f = conditions.velocity_forcing
-        if isinstance(f, ConstantPower):
-            cfg.velocity_forcing = L.CONSTANT_POWER
-            cfg.injection_rate = f.injection_rate
-            cfg.lowest_shell = f.lowest_shell
-            cfg.highest_shell = -1 if f.highest_shell == math.inf else f.highest_shell
-        cfg.mean_gradient = float(mean_gradient)
-        cap = tracers.count if tracers is not None else 0
-        cfg.tracer_capacity = int(tracer_capacity if tracer_capacity is not None else cap)
-        keep = keep_physical or tracers is not None
-        cfg.store_physical = (1 | (2 if carries_scalar else 0) if keep else 0) | (4 if keep_vorticity else 0)
+        cfg = _config(domain, conditions, field_integration_scheme, carries_scalar, mean_gradient, tracers,
+                      tracer_capacity, keep_physical, keep_vorticity)
         h = c_void_p()
         L.check(L.lib().hdns_ctx_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._ctx = h
@@ -519,3 +527,153 @@ class Simulation:
     def __repr__(self) -> str:
         n = self.domain.grid_points_per_axis
         return f"Simulation({n}x{n}x{n} grid, time {self.time}, step {self.step})"
+
+
+# --- ensembles --------------------------------------------------------------------------------
+
+@dataclass
+class EnsembleMember:
+    """One box of an ensemble: its frame, conditions, initial state and optional tracers."""
+    domain: FixedDomain
+    conditions: FlowConditions
+    state: ResolvedFields
+    tracers: Tracers | None = None
+
+
+class EnsembleBox(Simulation):
+    """One box of an Ensemble, with the per-box reads of a Simulation (state, reductions,
+    max_speed, spectra, physical fields, tracers).  The ensemble advances it; a box has no clock
+    of its own."""
+
+    def __init__(self, ensemble: "Ensemble", index: int, member: EnsembleMember, scheme):
+        # no device context of its own: every call goes to the batch that holds this box
+        self._ens = ensemble
+        self.index = index
+        self.domain = member.domain
+        self.conditions = member.conditions
+        self.field_integration_scheme = scheme
+        self.tracers = member.tracers
+        self.carries_scalar = member.state.scalar_spectral is not None
+        self.time = 0.0
+        self.step = 0
+        self._ctx = None
+
+    def _member(self) -> c_void_p:
+        h = c_void_p()
+        L.check(L.lib().hdns_ensemble_member(self._ens._h, self.index, ctypes.byref(h)), ensemble=self._ens._h)
+        return h
+
+    def _call(self, name: str, *args) -> None:
+        if self._ens._h is None:
+            raise RuntimeError("this Ensemble has been closed")
+        h = self._member()
+        L.check(getattr(L.lib(), name)(h, *args), h)
+
+    def close(self) -> None:  # the ensemble owns the device memory
+        pass
+
+    def _advanced_by_the_ensemble(self, *args, **kwargs):
+        raise RuntimeError("an ensemble box is advanced by its Ensemble (Ensemble.step / advance_tracers)")
+
+    step_fields = advance = run = advect = deposit_tracers = autotune = schedule = _advanced_by_the_ensemble
+
+    def __repr__(self) -> str:
+        n = self.domain.grid_points_per_axis
+        return f"EnsembleBox({self.index}: {n}x{n}x{n}, nu {self.conditions.viscosity:g})"
+
+
+class Ensemble:
+    """Boxes of uneven sizes and viscosities advanced together (the library's hdns_ensemble).
+
+    Boxes of one size form a batch that every kernel advances in one launch; all batches share a
+    stream, and ``step(dt, graph=True)`` advances every box with one captured HIP graph.  The
+    boxes may differ in grid size, viscosity and initial state; scheme, precision, forcing, the
+    scalar and the tracer count are common.
+    """
+
+    def __init__(self, members: list[EnsembleMember], *, field_integration_scheme=IFLowStorageRK3,
+                 mean_gradient: float = 0.0, keep_physical: bool = False):
+        if not members:
+            raise ValueError("an ensemble needs at least one box")
+        self.field_integration_scheme = field_integration_scheme
+        cfgs = (L.HdnsConfig * len(members))()
+        for b, mb in enumerate(members):
+            carries = mb.state.scalar_spectral is not None
+            cfgs[b] = _config(mb.domain, mb.conditions, field_integration_scheme, carries, mean_gradient, mb.tracers,
+                              None, keep_physical, False)
+        h = c_void_p()
+        L.check(L.lib().hdns_ensemble_create(cfgs, len(members), ctypes.byref(h)), ensemble=None)
+        self._h = h
+        self.time = 0.0
+        self.step_count = 0
+        self.boxes = [EnsembleBox(self, b, mb, field_integration_scheme) for b, mb in enumerate(members)]
+        try:
+            for box, mb in zip(self.boxes, members):
+                n = mb.domain.grid_points_per_axis
+                u = np.ascontiguousarray(mb.state.velocity_spectral, dtype=np.complex128)
+                if u.shape != (3, n, n, n // 2 + 1):
+                    raise ValueError(f"box {box.index}: velocity_spectral must have shape {(3, n, n, n // 2 + 1)}")
+                _check_reality(u, "velocity", mb.domain.element_type)
+                p = None
+                if mb.state.scalar_spectral is not None:
+                    p = np.ascontiguousarray(mb.state.scalar_spectral, dtype=np.complex128)
+                    _check_reality(p, "scalar", mb.domain.element_type)
+                box._call("hdns_state_upload", L.dptr(u), L.dptr(p) if p is not None else None)
+                if mb.tracers is not None:
+                    box._call("hdns_tracers_upload", mb.tracers.count, L.dptr(mb.tracers.positions))
+            if members[0].tracers is not None:   # the trail's first sample, at t = 0
+                for box in self.boxes:
+                    box._call("hdns_rebuild_traversal")
+                    box._call("hdns_deposit_tracers")
+        except Exception:
+            self.close()
+            raise
+
+    def _ecall(self, name: str, *args) -> None:
+        if self._h is None:
+            raise RuntimeError("this Ensemble has been closed")
+        L.check(getattr(L.lib(), name)(self._h, *args), ensemble=self._h)
+
+    def __len__(self) -> int:
+        return len(self.boxes)
+
+    def box(self, b: int) -> EnsembleBox:
+        return self.boxes[b]
+
+    def step(self, dt: float, graph: bool = True) -> "Ensemble":
+        """One IF-RK step (+ the field deposit) of every box; ``graph`` replays one captured graph."""
+        if not (math.isfinite(dt) and dt > 0):
+            raise ValueError(f"the timestep must be positive and finite, got {dt}")
+        self._ecall("hdns_ensemble_step_graph" if graph else "hdns_ensemble_step", float(dt))
+        self.time += dt
+        self.step_count += 1
+        return self
+
+    def advance_tracers(self, dt: float) -> None:
+        """Traversal rebuild, Heun push and sample of every box's tracers (after a field step)."""
+        self._ecall("hdns_ensemble_advance_tracers", float(dt))
+
+    def synchronize(self) -> None:
+        self._ecall("hdns_ensemble_synchronize")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None:
+            L.lib().hdns_ensemble_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):  # pragma: no cover - interpreter teardown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __repr__(self) -> str:
+        sizes = sorted({b.domain.grid_points_per_axis for b in self.boxes})
+        return f"Ensemble({len(self.boxes)} boxes of sizes {sizes}, time {self.time})"
+

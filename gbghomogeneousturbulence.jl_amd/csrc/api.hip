@@ -152,10 +152,11 @@ void release(Ctx &c) {
   for (hipEvent_t e : c.prof_ev) (void)hipEventDestroy(e);
   c.prof_ev.clear();
   void *ptrs[] = {c.uh, c.hu, c.ph, c.hp, c.bufa, c.bufb, c.phys, c.physw, c.tw, c.partials, c.scalars,
-                  c.maxbits, c.spec_bins, c.pos, c.vel, c.phis, c.order, c.tile_counts, c.tile_cursor};
+                  c.maxbits, c.spec_bins, c.pos, c.vel, c.phis, c.order, c.tile_counts, c.tile_cursor,
+                  c.visc_d, c.kappa_d};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
-  if (c.stream) (void)hipStreamDestroy(c.stream);
+  if (c.stream && c.own_stream) (void)hipStreamDestroy(c.stream);
   c.stream = nullptr;
 }
 
@@ -212,10 +213,12 @@ hdns_status run_deposit(hdns_ctx *ctx) {
   const FinalizeArgs fa{c.pipe->z_blocks(c), 1.0 / pow((double)c.geo.n, 6.0),
                         c.cfg.velocity_forcing == HDNS_CONSTANT_POWER, c.cfg.injection_rate,
                         c.fp64 ? 2.220446049250313e-16 : 1.1920928955078125e-07};
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c.stream, (const double *)c.partials, c.scalars, fa);
+  hipLaunchKernelGGL(k_finalize, dim3(c.nbox), dim3(256), 0, c.stream, (const double *)c.partials, c.scalars, fa);
   c.pipe->yinv(c);
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c.stream, c.maxbits, 0ull, 1);
+  hipLaunchKernelGGL(k_fill_u64, dim3((c.nbox + 63) / 64), dim3(64), 0, c.stream, c.maxbits, 0ull, c.nbox);
   XArgs xa{};
+  xa.box_phys = c.box_phys;
+  xa.box_physw = c.box_physw;
   xa.store_u = (c.cfg.store_physical & 1) != 0;
   xa.store_phi = (c.cfg.store_physical & 2) != 0 && c.cfg.has_scalar;
   xa.store_w = (c.cfg.store_physical & 4) != 0;
@@ -232,9 +235,16 @@ hdns_status run_deposit(hdns_ctx *ctx) {
 
 extern "C" {
 
-hdns_status hdns_ctx_create(const hdns_config *config, hdns_ctx **out) {
+}  // extern "C"
+
+namespace {
+
+// A context of nbox boxes of one size (nbox = 1: a plain context), viscosities visc[nbox] (NULL:
+// the config's for every box).
+hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, hdns_ctx **out) {
   if (!config || !out) return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "config and out must not be NULL");
   *out = nullptr;
+  if (nbox < 1 || nbox > 65535) return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "a batch holds 1..65535 boxes");
   const hdns_config &k = *config;
   if (!supported_grid(k.grid_points))
     return fail(nullptr, HDNS_ERR_UNSUPPORTED,
@@ -248,10 +258,11 @@ hdns_status hdns_ctx_create(const hdns_config *config, hdns_ctx **out) {
     return fail(nullptr, HDNS_ERR_UNSUPPORTED, "scheme must be IFLowStorageRK3 or IFLowStorageRK4");
   if (k.dealiasing != HDNS_TWO_THIRDS) return fail(nullptr, HDNS_ERR_UNSUPPORTED, "only TwoThirds dealiasing is built");
   // src/Generator.jl:107-112
-  if (!(k.viscosity > 0))
-    return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
-                "`viscosity` must be positive: it sets the Reynolds number through Re = 1/nu, and a non-positive "
-                "value has no dissipation scale");
+  for (int b = 0; b < nbox; ++b)
+    if (!((visc ? visc[b] : k.viscosity) > 0))
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                  "`viscosity` must be positive: it sets the Reynolds number through Re = 1/nu, and a non-positive "
+                  "value has no dissipation scale");
   if (!(k.schmidt > 0))
     return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
                 "`schmidt` must be positive: the scalar diffusivity is nu/Sc, so a non-positive Schmidt number makes "
@@ -281,6 +292,9 @@ hdns_status hdns_ctx_create(const hdns_config *config, hdns_ctx **out) {
   if (!ctx) return fail(nullptr, HDNS_ERR_DEVICE, "out of host memory");
   Ctx &c = ctx->c;
   c.cfg = k;
+  c.nbox = nbox;
+  c.visc.assign(nbox, k.viscosity);
+  if (visc) c.visc.assign(visc, visc + nbox);
   if (c.cfg.tracer_capacity > 0) c.cfg.store_physical |= 1 | (c.cfg.has_scalar ? 2 : 0);
   c.fp64 = k.precision == HDNS_FP64;
   const int n = k.grid_points;
@@ -312,7 +326,14 @@ hdns_status hdns_ctx_create(const hdns_config *config, hdns_ctx **out) {
   const size_t nn = (size_t)n * n, nk = (size_t)n * c.geo.k;
   c.buf_a = std::max(ncf * nn, nci * nk) * c.geo.kxp;
   c.buf_b = std::max(ncf * nk, nci * nn) * c.geo.kxp;
+  c.geo.box_a = c.buf_a;
+  c.geo.box_b = c.buf_b;
+  c.geo.ncf = (int)ncf;
+  c.geo.nci = (int)nci;
   const size_t vol = nn * n;
+  c.box_phys = (k.has_scalar ? 4 : 3) * vol;
+  c.box_physw = 3 * vol;
+  const size_t B = (size_t)nbox;
 
   hdns_status s = HDNS_OK;
   e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
@@ -321,27 +342,36 @@ hdns_status hdns_ctx_create(const hdns_config *config, hdns_ctx **out) {
   c.nbins = (int)floor(sqrt(3.0) * n / 2) + 2;
   c.ntiles = (n / 8) * (n / 8) * (n / 8);
   c.m = 0;
-  if (s == HDNS_OK) s = alloc(ctx, &c.uh, 3 * c.comp * c.elem);
-  if (s == HDNS_OK) s = alloc(ctx, &c.hu, 3 * c.comp * c.elem);
-  if (s == HDNS_OK && k.has_scalar) s = alloc(ctx, &c.ph, c.comp * c.elem);
-  if (s == HDNS_OK && k.has_scalar) s = alloc(ctx, &c.hp, c.comp * c.elem);
-  if (s == HDNS_OK) s = alloc(ctx, &c.bufa, c.buf_a * c.elem);
-  if (s == HDNS_OK) s = alloc(ctx, &c.bufb, c.buf_b * c.elem);
-  if (s == HDNS_OK && (c.cfg.store_physical & 3)) s = alloc(ctx, &c.phys, (k.has_scalar ? 4 : 3) * vol * realb);
-  if (s == HDNS_OK && (c.cfg.store_physical & 4)) s = alloc(ctx, &c.physw, 3 * vol * realb);
+  if (s == HDNS_OK) s = alloc(ctx, &c.uh, B * 3 * c.comp * c.elem);
+  if (s == HDNS_OK) s = alloc(ctx, &c.hu, B * 3 * c.comp * c.elem);
+  if (s == HDNS_OK && k.has_scalar) s = alloc(ctx, &c.ph, B * c.comp * c.elem);
+  if (s == HDNS_OK && k.has_scalar) s = alloc(ctx, &c.hp, B * c.comp * c.elem);
+  if (s == HDNS_OK) s = alloc(ctx, &c.bufa, B * c.buf_a * c.elem);
+  if (s == HDNS_OK) s = alloc(ctx, &c.bufb, B * c.buf_b * c.elem);
+  if (s == HDNS_OK && (c.cfg.store_physical & 3)) s = alloc(ctx, &c.phys, B * c.box_phys * realb);
+  if (s == HDNS_OK && (c.cfg.store_physical & 4)) s = alloc(ctx, &c.physw, B * c.box_physw * realb);
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.tw, n * sizeof(cx<double>));
-  if (s == HDNS_OK) s = alloc(ctx, (void **)&c.partials, (size_t)nzb * NRED * sizeof(double));
-  if (s == HDNS_OK) s = alloc(ctx, (void **)&c.scalars, 8 * sizeof(double));
-  if (s == HDNS_OK) s = alloc(ctx, (void **)&c.maxbits, sizeof(unsigned long long));
+  if (s == HDNS_OK) s = alloc(ctx, (void **)&c.partials, B * nzb * NRED * sizeof(double));
+  if (s == HDNS_OK) s = alloc(ctx, (void **)&c.scalars, B * 8 * sizeof(double));
+  if (s == HDNS_OK) s = alloc(ctx, (void **)&c.maxbits, B * sizeof(unsigned long long));
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.spec_bins, c.nbins * sizeof(double));
+  if (s == HDNS_OK) s = alloc(ctx, (void **)&c.visc_d, B * sizeof(double));
+  if (s == HDNS_OK) s = alloc(ctx, (void **)&c.kappa_d, B * sizeof(double));
   if (s == HDNS_OK && k.tracer_capacity > 0) {
     const size_t cap = (size_t)k.tracer_capacity;
-    s = alloc(ctx, &c.pos, 3 * cap * realb);
-    if (s == HDNS_OK) s = alloc(ctx, &c.vel, 3 * cap * realb);
-    if (s == HDNS_OK && k.has_scalar) s = alloc(ctx, &c.phis, cap * realb);
-    if (s == HDNS_OK) s = alloc(ctx, (void **)&c.order, cap * sizeof(int));
-    if (s == HDNS_OK) s = alloc(ctx, (void **)&c.tile_counts, (size_t)c.ntiles * sizeof(int));
-    if (s == HDNS_OK) s = alloc(ctx, (void **)&c.tile_cursor, (size_t)c.ntiles * sizeof(int));
+    s = alloc(ctx, &c.pos, B * 3 * cap * realb);
+    if (s == HDNS_OK) s = alloc(ctx, &c.vel, B * 3 * cap * realb);
+    if (s == HDNS_OK && k.has_scalar) s = alloc(ctx, &c.phis, B * cap * realb);
+    if (s == HDNS_OK) s = alloc(ctx, (void **)&c.order, B * cap * sizeof(int));
+    if (s == HDNS_OK) s = alloc(ctx, (void **)&c.tile_counts, B * c.ntiles * sizeof(int));
+    if (s == HDNS_OK) s = alloc(ctx, (void **)&c.tile_cursor, B * c.ntiles * sizeof(int));
+  }
+  if (s == HDNS_OK) {
+    std::vector<double> kap(nbox);
+    for (int b = 0; b < nbox; ++b) kap[b] = c.visc[b] / k.schmidt;
+    e = hipMemcpy(c.visc_d, c.visc.data(), B * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(c.kappa_d, kap.data(), B * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, std::string("viscosity upload failed: ") + hipGetErrorString(e));
   }
   if (s == HDNS_OK) {
     // W_N^m = exp(-2 pi i m / N), each entry the FP64 rounding of an extended-precision value
@@ -356,15 +386,15 @@ hdns_status hdns_ctx_create(const hdns_config *config, hdns_ctx **out) {
   }
   if (s == HDNS_OK) {
     // every array the step reads starts at zero (padding columns included)
-    zero_f64(c, c.uh, 3 * c.comp * c.elem);
-    zero_f64(c, c.hu, 3 * c.comp * c.elem);
-    if (c.ph) zero_f64(c, c.ph, c.comp * c.elem);
-    if (c.hp) zero_f64(c, c.hp, c.comp * c.elem);
-    zero_f64(c, c.bufa, c.buf_a * c.elem);
-    zero_f64(c, c.bufb, c.buf_b * c.elem);
-    zero_f64(c, c.scalars, 8 * sizeof(double));
-    if (c.phys) zero_f64(c, c.phys, (k.has_scalar ? 4 : 3) * vol * realb);
-    if (c.physw) zero_f64(c, c.physw, 3 * vol * realb);
+    zero_f64(c, c.uh, B * 3 * c.comp * c.elem);
+    zero_f64(c, c.hu, B * 3 * c.comp * c.elem);
+    if (c.ph) zero_f64(c, c.ph, B * c.comp * c.elem);
+    if (c.hp) zero_f64(c, c.hp, B * c.comp * c.elem);
+    zero_f64(c, c.bufa, B * c.buf_a * c.elem);
+    zero_f64(c, c.bufb, B * c.buf_b * c.elem);
+    zero_f64(c, c.scalars, B * 8 * sizeof(double));
+    if (c.phys) zero_f64(c, c.phys, B * c.box_phys * realb);
+    if (c.physw) zero_f64(c, c.physw, B * c.box_physw * realb);
     s = launched(ctx);
     if (s == HDNS_OK) {
       e = hipStreamSynchronize(c.stream);
@@ -381,8 +411,30 @@ hdns_status hdns_ctx_create(const hdns_config *config, hdns_ctx **out) {
   return HDNS_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+hdns_status hdns_ctx_create(const hdns_config *config, hdns_ctx **out) { return create_ctx(config, 1, nullptr, out); }
+
+hdns_status hdns_ctx_create_batch(const hdns_config *config, int32_t nbox, const double *viscosities,
+                                  hdns_ctx **out) {
+  return create_ctx(config, nbox, viscosities, out);
+}
+
+hdns_status hdns_select_box(hdns_ctx *ctx, int32_t box) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  if (box < 0 || box >= ctx->c.nbox)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT,
+                "box " + std::to_string(box) + " is outside this batch of " + std::to_string(ctx->c.nbox));
+  ctx->c.box = box;
+  return HDNS_OK;
+}
+
 hdns_status hdns_ctx_destroy(hdns_ctx *ctx) {
   if (!ctx) return HDNS_OK;
+  if (ctx->c.in_ensemble)
+    return fail(ctx, HDNS_ERR_STATE, "this context belongs to an ensemble: destroy the ensemble instead");
   (void)hipSetDevice(ctx->c.cfg.device);
   if (ctx->c.stream) (void)hipStreamSynchronize(ctx->c.stream);
   release(ctx->c);
@@ -481,7 +533,8 @@ hdns_status hdns_state_upload(hdns_ctx *ctx, const double *u_hat, const double *
         }
       }
     }
-    void *dev = q < 3 ? (char *)c.uh + q * c.comp * c.elem : c.ph;
+    void *dev = q < 3 ? (char *)c.uh + ((size_t)c.box * 3 + q) * c.comp * c.elem
+                      : (char *)c.ph + (size_t)c.box * c.comp * c.elem;
     HIP_TRY(ctx, hipMemcpyAsync(dev, host.data(), host.size(), hipMemcpyHostToDevice, c.stream));
     HIP_TRY(ctx, hipStreamSynchronize(c.stream));
   }
@@ -502,7 +555,8 @@ hdns_status hdns_state_download(hdns_ctx *ctx, double *u_hat, double *phi_hat) {
   std::vector<unsigned char> host(c.comp * c.elem);
   for (int q = 0; q < ncomp; ++q) {
     double *dst = q < 3 ? u_hat + 2 * full * q : phi_hat;
-    const void *dev = q < 3 ? (char *)c.uh + q * c.comp * c.elem : c.ph;
+    const void *dev = q < 3 ? (char *)c.uh + ((size_t)c.box * 3 + q) * c.comp * c.elem
+                            : (char *)c.ph + (size_t)c.box * c.comp * c.elem;
     HIP_TRY(ctx, hipMemcpyAsync(host.data(), dev, host.size(), hipMemcpyDeviceToHost, c.stream));
     HIP_TRY(ctx, hipStreamSynchronize(c.stream));
     memset(dst, 0, 2 * full * sizeof(double));
@@ -538,12 +592,13 @@ void launch_step(Ctx &c, double dt) {
   const bool fp32 = !c.fp64;
   const Coeffs k = scheme_coefficients(c.cfg.scheme, fp32);
   const double dt_t = fp32 ? (double)(float)dt : dt;
-  const double kappa = c.cfg.viscosity / c.cfg.schmidt;
   const FinalizeArgs fa{c.pipe->z_blocks(c), 1.0 / pow((double)c.geo.n, 6.0),
                         c.cfg.velocity_forcing == HDNS_CONSTANT_POWER, c.cfg.injection_rate,
                         c.fp64 ? 2.220446049250313e-16 : 1.1920928955078125e-07};
   XArgs mid{};
   XArgs fin{};
+  fin.box_phys = c.box_phys;
+  fin.box_physw = c.box_physw;
   fin.store_u = (c.cfg.store_physical & 1) != 0;
   fin.store_phi = (c.cfg.store_physical & 2) != 0 && c.cfg.has_scalar;
   fin.store_w = (c.cfg.store_physical & 4) != 0;
@@ -559,9 +614,9 @@ void launch_step(Ctx &c, double dt) {
     sa.a = k.a[i];
     sa.b = k.b[i];
     sa.dt = dt_t;
-    const double sub = k.d[i] * dt;  // d_i in T, promoted: the exponent's substep (src/Propagator.jl:375)
-    sa.coef_u = -c.cfg.viscosity * sub;
-    sa.coef_p = -kappa * sub;
+    sa.sub = k.d[i] * dt;  // d_i in T, promoted: the exponent's substep (src/Propagator.jl:375)
+    sa.visc = c.visc_d;    // the exponent per |k|^2 is -D sub, D per box (nu, nu/Sc)
+    sa.kappa = c.kappa_d;
     sa.mean_gradient = c.cfg.mean_gradient;
     sa.band = c.band;
     sa.scalars = c.scalars;
@@ -570,9 +625,10 @@ void launch_step(Ctx &c, double dt) {
     if (last) TIMED(c, K_REALITY, c.pipe->reality(c));
     TIMED(c, K_ZI, c.pipe->zi(c));
     TIMED(c, K_FINALIZE,
-          hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, c.stream, (const double *)c.partials, c.scalars, fa));
+          hipLaunchKernelGGL(k_finalize, dim3(c.nbox), dim3(256), 0, c.stream, (const double *)c.partials, c.scalars,
+                             fa));
     TIMED(c, K_YINV, c.pipe->yinv(c));
-    if (last) hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c.stream, c.maxbits, 0ull, 1);
+    if (last) hipLaunchKernelGGL(k_fill_u64, dim3((c.nbox + 63) / 64), dim3(64), 0, c.stream, c.maxbits, 0ull, c.nbox);
     TIMED(c, K_XPASS, c.pipe->xpass(c, last ? fin : mid));
   }
 }
@@ -648,7 +704,7 @@ hdns_status hdns_max_speed(hdns_ctx *ctx, double *out) {
   Ctx &c = ctx->c;
   if (!c.deposited) return fail(ctx, HDNS_ERR_STATE, "max_speed has not been deposited");
   unsigned long long bits = 0;
-  HIP_TRY(ctx, hipMemcpyAsync(&bits, c.maxbits, sizeof bits, hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipMemcpyAsync(&bits, c.maxbits + c.box, sizeof bits, hipMemcpyDeviceToHost, c.stream));
   HIP_TRY(ctx, hipStreamSynchronize(c.stream));
   double v;
   memcpy(&v, &bits, sizeof v);
@@ -661,11 +717,11 @@ hdns_status hdns_reductions(hdns_ctx *ctx, double *out) {
   Ctx &c = ctx->c;
   if (!c.deposited) return fail(ctx, HDNS_ERR_STATE, "no deposit of the current state");
   double s[8];
-  HIP_TRY(ctx, hipMemcpyAsync(s, c.scalars, sizeof s, hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipMemcpyAsync(s, c.scalars + 8 * c.box, sizeof s, hipMemcpyDeviceToHost, c.stream));
   HIP_TRY(ctx, hipStreamSynchronize(c.stream));
-  const double kappa = c.cfg.viscosity / c.cfg.schmidt;
+  const double nu = c.visc[c.box], kappa = nu / c.cfg.schmidt;
   out[0] = 0.5 * s[1];
-  out[1] = c.cfg.viscosity * s[3];
+  out[1] = nu * s[3];
   out[2] = c.cfg.has_scalar ? s[4] : 0.0;
   out[3] = c.cfg.has_scalar ? 2.0 * kappa * s[5] : 0.0;
   out[4] = 0.5 * s[2];
@@ -700,16 +756,18 @@ hdns_status hdns_field_download(hdns_ctx *ctx, int32_t which, double *out) {
   Ctx &c = ctx->c;
   if (!c.deposited) return fail(ctx, HDNS_ERR_STATE, "no deposit of the current state");
   const size_t vol = (size_t)c.geo.n * c.geo.n * c.geo.n;
+  const size_t realb = c.fp64 ? 8 : 4;
+  const char *phys = (const char *)c.phys + c.box * c.box_phys * realb;
   const void *src = nullptr;
   int ncomp = 0;
   if (which == HDNS_FIELD_VELOCITY && (c.cfg.store_physical & 1)) {
-    src = c.phys;
+    src = phys;
     ncomp = 3;
   } else if (which == HDNS_FIELD_SCALAR && (c.cfg.store_physical & 2) && c.cfg.has_scalar) {
-    src = (const char *)c.phys + 3 * vol * (c.fp64 ? 8 : 4);
+    src = phys + 3 * vol * realb;
     ncomp = 1;
   } else if (which == HDNS_FIELD_VORTICITY && (c.cfg.store_physical & 4)) {
-    src = c.physw;
+    src = (const char *)c.physw + c.box * c.box_physw * realb;
     ncomp = 3;
   } else {
     return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "this context does not keep that physical field (store_physical)");
@@ -734,17 +792,22 @@ hdns_status hdns_tracers_upload(hdns_ctx *ctx, int64_t m, const double *position
     return fail(ctx, HDNS_ERR_INVALID_ARGUMENT,
                 "tracer count must be within the context's tracer_capacity " + std::to_string(c.cfg.tracer_capacity));
   if (m > 0 && !positions) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "positions must not be NULL");
+  if (c.nbox > 1 && c.m != 0 && m != c.m)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT,
+                "every box of a batch carries the same number of tracers (" + std::to_string(c.m) + ")");
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
   c.m = m;
   c.order_valid = false;
   if (m == 0) return HDNS_OK;
+  const size_t realb = c.fp64 ? 8 : 4;
+  char *pos = (char *)c.pos + (size_t)c.box * 3 * c.cfg.tracer_capacity * realb;
   if (c.fp64) {
-    HIP_TRY(ctx, hipMemcpyAsync(c.pos, positions, 3 * m * 8, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(ctx, hipMemcpyAsync(pos, positions, 3 * m * 8, hipMemcpyHostToDevice, c.stream));
     HIP_TRY(ctx, hipStreamSynchronize(c.stream));
   } else {
     std::vector<float> tmp(3 * m);
     for (size_t i = 0; i < tmp.size(); ++i) tmp[i] = (float)positions[i];
-    HIP_TRY(ctx, hipMemcpyAsync(c.pos, tmp.data(), 3 * m * 4, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(ctx, hipMemcpyAsync(pos, tmp.data(), 3 * m * 4, hipMemcpyHostToDevice, c.stream));
     HIP_TRY(ctx, hipStreamSynchronize(c.stream));
   }
   return HDNS_OK;
@@ -768,7 +831,10 @@ static hdns_status copy_real_out(hdns_ctx *ctx, const void *src, size_t count, d
 hdns_status hdns_tracers_download(hdns_ctx *ctx, double *positions) {
   if (!ctx || !positions) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "positions must not be NULL");
   HIP_TRY(ctx, hipSetDevice(ctx->c.cfg.device));
-  return copy_real_out(ctx, ctx->c.pos, 3 * (size_t)ctx->c.m, positions);
+  const Ctx &c = ctx->c;
+  const size_t realb = c.fp64 ? 8 : 4;
+  return copy_real_out(ctx, (const char *)c.pos + (size_t)c.box * 3 * c.cfg.tracer_capacity * realb, 3 * (size_t)c.m,
+                       positions);
 }
 
 hdns_status hdns_rebuild_traversal(hdns_ctx *ctx) {
@@ -811,9 +877,10 @@ hdns_status hdns_tracer_samples(hdns_ctx *ctx, double *velocity, double *scalar)
   if (!ctx || !velocity) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "velocity must not be NULL");
   Ctx &c = ctx->c;
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
-  hdns_status s = copy_real_out(ctx, c.vel, 3 * (size_t)c.m, velocity);
+  const size_t realb = c.fp64 ? 8 : 4, cap = (size_t)c.cfg.tracer_capacity;
+  hdns_status s = copy_real_out(ctx, (const char *)c.vel + c.box * 3 * cap * realb, 3 * (size_t)c.m, velocity);
   if (s != HDNS_OK || !scalar || !c.cfg.has_scalar) return s;
-  return copy_real_out(ctx, c.phis, (size_t)c.m, scalar);
+  return copy_real_out(ctx, (const char *)c.phis + c.box * cap * realb, (size_t)c.m, scalar);
 }
 
 hdns_status hdns_tracer_cells(hdns_ctx *ctx, int64_t *keys) {
@@ -836,7 +903,8 @@ hdns_status hdns_traversal_order(hdns_ctx *ctx, int32_t *order) {
   Ctx &c = ctx->c;
   if (!c.order_valid) return fail(ctx, HDNS_ERR_STATE, "no traversal order has been built for these positions");
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
-  HIP_TRY(ctx, hipMemcpyAsync(order, c.order, c.m * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipMemcpyAsync(order, c.order + (size_t)c.box * c.cfg.tracer_capacity, c.m * sizeof(int),
+                              hipMemcpyDeviceToHost, c.stream));
   HIP_TRY(ctx, hipStreamSynchronize(c.stream));
   return HDNS_OK;
 }
@@ -884,11 +952,16 @@ hdns_status hdns_autotune(hdns_ctx *ctx, int32_t reps, double *ms, int32_t *chos
   idle.stage = 0;
   idle.last = 0;
   idle.b = 1.0;
+  idle.sub = 0.0;   // exponent -D * 0: a unit exponential
+  idle.visc = c.visc_d;
+  idle.kappa = c.kappa_d;
   idle.scalars = c.scalars;
   XArgs xa{};
   xa.phys = c.phys;
   xa.physw = c.physw;
   xa.max_bits = c.maxbits;
+  xa.box_phys = c.box_phys;
+  xa.box_physw = c.box_physw;
   constexpr int NF = Ctx::NFAM;
   int saved[NF];
   for (int f = 0; f < NF; ++f) saved[f] = c.sched[f];
@@ -991,6 +1064,205 @@ hdns_status hdns_synchronize(hdns_ctx *ctx) {
 hdns_status hdns_stream(hdns_ctx *ctx, void **stream) {
   if (!ctx || !stream) return HDNS_ERR_INVALID_ARGUMENT;
   *stream = (void *)ctx->c.stream;
+  return HDNS_OK;
+}
+
+// --- ensembles ---------------------------------------------------------------------------
+
+}  // extern "C"
+
+struct hdns_ensemble {
+  std::vector<hdns_ctx *> classes;          // one batch context per grid size
+  std::vector<std::pair<int, int>> where;   // box -> (class, slot)
+  hipStream_t stream = nullptr;
+  int device = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  double graph_dt = 0.0;
+  std::vector<int> graph_sched;             // the schedules the captured step holds
+  std::string err;
+};
+
+namespace {
+
+hdns_status efail(hdns_ensemble *e, hdns_status code, const std::string &msg) {
+  if (e) e->err = msg;
+  else g_create_error = msg;
+  return code;
+}
+
+void drop_ensemble_graph(hdns_ensemble &e) {
+  if (e.graph_exec) (void)hipGraphExecDestroy(e.graph_exec);
+  if (e.graph) (void)hipGraphDestroy(e.graph);
+  e.graph_exec = nullptr;
+  e.graph = nullptr;
+}
+
+void release_ensemble(hdns_ensemble *e) {
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  drop_ensemble_graph(*e);
+  for (hdns_ctx *c : e->classes) {
+    c->c.in_ensemble = false;
+    hdns_ctx_destroy(c);
+  }
+  e->classes.clear();
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  e->stream = nullptr;
+}
+
+std::vector<int> ensemble_schedules(const hdns_ensemble &e) {
+  std::vector<int> s;
+  for (hdns_ctx *c : e.classes) s.insert(s.end(), c->c.sched, c->c.sched + Ctx::NFAM);
+  return s;
+}
+
+hdns_status ensemble_step_checks(hdns_ensemble *e, double dt) {
+  if (!e) return HDNS_ERR_INVALID_ARGUMENT;
+  if (!(isfinite(dt) && dt > 0)) return efail(e, HDNS_ERR_INVALID_ARGUMENT, "the timestep must be positive and finite");
+  for (hdns_ctx *c : e->classes)
+    if (!c->c.deposited) return efail(e, HDNS_ERR_STATE, "upload a state for every box before stepping the ensemble");
+  return HDNS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+hdns_status hdns_ensemble_create(const hdns_config *configs, int32_t nbox, hdns_ensemble **out) {
+  if (!configs || !out || nbox < 1) return efail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "configs, nbox >= 1 and out");
+  *out = nullptr;
+  const hdns_config &k0 = configs[0];
+  for (int b = 1; b < nbox; ++b) {
+    hdns_config k = configs[b];
+    k.grid_points = k0.grid_points;
+    k.viscosity = k0.viscosity;
+    if (memcmp(&k, &k0, sizeof k) != 0)
+      return efail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                   "the boxes of an ensemble may differ in grid_points and viscosity only (box " + std::to_string(b) +
+                       " differs in another field)");
+  }
+  hdns_ensemble *e = new (std::nothrow) hdns_ensemble();
+  if (!e) return efail(nullptr, HDNS_ERR_DEVICE, "out of host memory");
+  e->device = k0.device;
+  e->where.resize(nbox);
+  std::vector<int> sizes;
+  for (int b = 0; b < nbox; ++b)
+    if (std::find(sizes.begin(), sizes.end(), configs[b].grid_points) == sizes.end()) sizes.push_back(configs[b].grid_points);
+  hipError_t he = hipSetDevice(e->device);
+  if (he == hipSuccess) he = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+  if (he != hipSuccess) {
+    delete e;
+    return efail(nullptr, HDNS_ERR_DEVICE, std::string("ensemble stream: ") + hipGetErrorString(he));
+  }
+  for (size_t ci = 0; ci < sizes.size(); ++ci) {
+    std::vector<double> visc;
+    for (int b = 0; b < nbox; ++b)
+      if (configs[b].grid_points == sizes[ci]) {
+        e->where[b] = {(int)ci, (int)visc.size()};
+        visc.push_back(configs[b].viscosity);
+      }
+    hdns_config k = k0;
+    k.grid_points = sizes[ci];
+    k.viscosity = visc[0];
+    hdns_ctx *c = nullptr;
+    hdns_status s = create_ctx(&k, (int)visc.size(), visc.data(), &c);
+    if (s != HDNS_OK) {
+      const std::string msg = g_create_error;
+      release_ensemble(e);
+      delete e;
+      return efail(nullptr, s, "ensemble batch of " + std::to_string(sizes[ci]) + "^3 boxes: " + msg);
+    }
+    // the batch joins the ensemble's stream (its own is idle: creation synchronised it)
+    (void)hipStreamDestroy(c->c.stream);
+    c->c.stream = e->stream;
+    c->c.own_stream = false;
+    c->c.in_ensemble = true;
+    e->classes.push_back(c);
+  }
+  *out = e;
+  return HDNS_OK;
+}
+
+hdns_status hdns_ensemble_destroy(hdns_ensemble *ens) {
+  if (!ens) return HDNS_OK;
+  (void)hipSetDevice(ens->device);
+  release_ensemble(ens);
+  delete ens;
+  return HDNS_OK;
+}
+
+const char *hdns_ensemble_last_error(const hdns_ensemble *ens) { return ens ? ens->err.c_str() : g_create_error.c_str(); }
+
+hdns_status hdns_ensemble_member(hdns_ensemble *ens, int32_t box, hdns_ctx **ctx) {
+  if (!ens || !ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  if (box < 0 || box >= (int)ens->where.size())
+    return efail(ens, HDNS_ERR_INVALID_ARGUMENT, "box " + std::to_string(box) + " is outside the ensemble");
+  hdns_ctx *c = ens->classes[ens->where[box].first];
+  c->c.box = ens->where[box].second;
+  *ctx = c;
+  return HDNS_OK;
+}
+
+hdns_status hdns_ensemble_step(hdns_ensemble *ens, double dt) {
+  hdns_status s = ensemble_step_checks(ens, dt);
+  if (s != HDNS_OK) return s;
+  if (hipSetDevice(ens->device) != hipSuccess) return efail(ens, HDNS_ERR_DEVICE, "hipSetDevice failed");
+  for (hdns_ctx *c : ens->classes) launch_step(c->c, dt);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return efail(ens, HDNS_ERR_DEVICE, std::string("kernel launch failed: ") + hipGetErrorString(e));
+  return HDNS_OK;
+}
+
+hdns_status hdns_ensemble_step_graph(hdns_ensemble *ens, double dt) {
+  hdns_status s = ensemble_step_checks(ens, dt);
+  if (s != HDNS_OK) return s;
+  if (hipSetDevice(ens->device) != hipSuccess) return efail(ens, HDNS_ERR_DEVICE, "hipSetDevice failed");
+  const std::vector<int> sched = ensemble_schedules(*ens);
+  if (!ens->graph_exec || ens->graph_dt != dt || ens->graph_sched != sched) {
+    drop_ensemble_graph(*ens);
+    std::vector<bool> prof;
+    for (hdns_ctx *c : ens->classes) {
+      prof.push_back(c->c.prof);
+      c->c.prof = false;  // no event records inside the captured step
+    }
+    hipError_t e = hipStreamBeginCapture(ens->stream, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) return efail(ens, HDNS_ERR_DEVICE, std::string("capture: ") + hipGetErrorString(e));
+    for (hdns_ctx *c : ens->classes) launch_step(c->c, dt);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(ens->stream, &graph);
+    for (size_t i = 0; i < ens->classes.size(); ++i) ens->classes[i]->c.prof = prof[i];
+    if (e != hipSuccess) return efail(ens, HDNS_ERR_DEVICE, std::string("step capture failed: ") + hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return efail(ens, HDNS_ERR_DEVICE, std::string("graph instantiation failed: ") + hipGetErrorString(e));
+    }
+    ens->graph = graph;
+    ens->graph_exec = exec;
+    ens->graph_dt = dt;
+    ens->graph_sched = sched;
+  }
+  const hipError_t e = hipGraphLaunch(ens->graph_exec, ens->stream);
+  if (e != hipSuccess) return efail(ens, HDNS_ERR_DEVICE, std::string("graph launch: ") + hipGetErrorString(e));
+  return HDNS_OK;
+}
+
+hdns_status hdns_ensemble_advance_tracers(hdns_ensemble *ens, double dt) {
+  if (!ens) return HDNS_ERR_INVALID_ARGUMENT;
+  for (hdns_ctx *c : ens->classes) {
+    hdns_status s = hdns_rebuild_traversal(c);
+    if (s == HDNS_OK) s = hdns_advect(c, dt);
+    if (s == HDNS_OK) s = hdns_deposit_tracers(c);
+    if (s != HDNS_OK) return efail(ens, s, c->c.err);
+  }
+  return HDNS_OK;
+}
+
+hdns_status hdns_ensemble_synchronize(hdns_ensemble *ens) {
+  if (!ens) return HDNS_ERR_INVALID_ARGUMENT;
+  const hipError_t e = hipStreamSynchronize(ens->stream);
+  if (e != hipSuccess) return efail(ens, HDNS_ERR_DEVICE, std::string("synchronize: ") + hipGetErrorString(e));
   return HDNS_OK;
 }
 

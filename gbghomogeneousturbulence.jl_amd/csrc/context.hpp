@@ -37,6 +37,15 @@ std::unique_ptr<Pipeline> make_pipeline_f64(int n);
 struct Ctx {
   hdns_config cfg{};
   Geo geo{};
+  // a batch: nbox boxes of this size back to back in every array (1 for a plain context); the
+  // per-box entry points address box `box` (hdns_select_box)
+  int nbox = 1;
+  int box = 0;
+  std::vector<double> visc;   // per box: nu
+  double *visc_d = nullptr, *kappa_d = nullptr;   // per box on the device: nu, nu/Sc
+  size_t box_phys = 0, box_physw = 0;   // reals of phys / physw per box
+  bool own_stream = true;     // false inside an ensemble, which owns the shared stream
+  bool in_ensemble = false;
   int fp64 = 0;
   size_t elem = 0;          // bytes of one complex element
   size_t comp = 0;          // complex elements of one compact component

@@ -25,7 +25,15 @@ struct Geo {
   int kxp;    // padded kx extent of every compact array
   int k;      // 2m + 1 retained signed wavenumbers on y and z
   double k0;  // 2 pi / L
+  // A batch of boxes of one size lies back to back in every array; a kernel finds its box in the
+  // grid's last dimension.  Per box: buffer A and B hold box_a / box_b complex values, the state
+  // [3][comp] (u, h_u) or [comp] (phi, h_phi).
+  size_t box_a, box_b;
+  int ncf, nci;   // product and inverse components (the y passes' grid z is components x boxes)
 };
+
+// compact elements of one component
+__host__ __device__ __forceinline__ size_t comp_of(const Geo &g) { return (size_t)g.k * g.k * g.kxp; }
 
 // FFT bin of a compact signed-wavenumber index.
 __device__ __forceinline__ int bin_of(int idx, const Geo &g) { return idx <= g.m ? idx : g.n - g.k + idx; }
@@ -65,9 +73,9 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cx<T> *__restrict__ px, cx<T>
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *twl = buf + N * TX;
   stage_twiddles<T, N, NT>(twl, tw);
-  const int kx0 = blockIdx.x * TX, z = blockIdx.y, c = blockIdx.z;
-  const cx<T> *src = px + ((size_t)c * N + z) * (size_t)N * g.kxp + kx0;
-  cx<T> *dst = pf + ((size_t)c * N + z) * (size_t)g.k * g.kxp + kx0;
+  const int kx0 = blockIdx.x * TX, z = blockIdx.y, c = blockIdx.z % g.ncf, box = blockIdx.z / g.ncf;
+  const cx<T> *src = px + box * g.box_a + ((size_t)c * N + z) * (size_t)N * g.kxp + kx0;
+  cx<T> *dst = pf + box * g.box_b + ((size_t)c * N + z) * (size_t)g.k * g.kxp + kx0;
   static_assert(TX * sizeof(cx<T>) <= 128, "a y tile is at most one 128-byte line of a row");
   fft_io<T, N, TX, NT, -1, true, false>(
       buf, twl, [&](int, int, int n, int col) { return src[(size_t)n * g.kxp + col]; },
@@ -84,9 +92,9 @@ __global__ __launch_bounds__(NT) void k_yinv(const cx<T> *__restrict__ q, cx<T> 
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *twl = buf + N * TX;
   stage_twiddles<T, N, NT>(twl, tw);
-  const int kx0 = blockIdx.x * TX, z = blockIdx.y, c = blockIdx.z;
-  const cx<T> *src = q + ((size_t)c * N + z) * (size_t)g.k * g.kxp + kx0;
-  cx<T> *dst = r + ((size_t)c * N + z) * (size_t)N * g.kxp + kx0;
+  const int kx0 = blockIdx.x * TX, z = blockIdx.y, c = blockIdx.z % g.nci, box = blockIdx.z / g.nci;
+  const cx<T> *src = q + box * g.box_a + ((size_t)c * N + z) * (size_t)g.k * g.kxp + kx0;
+  cx<T> *dst = r + box * g.box_b + ((size_t)c * N + z) * (size_t)N * g.kxp + kx0;
   fft_io<T, N, TX, NT, +1, true, false>(
       buf, twl,
       [&](int, int, int n, int col) {
@@ -109,9 +117,10 @@ __global__ __launch_bounds__(NT) void k_yinv(const cx<T> *__restrict__ q, cx<T> 
 struct XArgs {
   int store_u, store_phi, store_w;
   int want_max;
-  void *phys;             // [4][N][N][N] real: u_x, u_y, u_z, phi
-  void *physw;            // [3][N][N][N] real: w
-  unsigned long long *max_bits;  // max |u|^2 as the bits of a non-negative double
+  void *phys;             // per box [4][N][N][N] real: u_x, u_y, u_z, phi
+  void *physw;            // per box [3][N][N][N] real: w
+  unsigned long long *max_bits;  // per box: max |u|^2 as the bits of a non-negative double
+  size_t box_phys, box_physw;    // reals per box in phys / physw
 };
 
 // lanes per row pair: a lane holds N/LPR values of each component
@@ -143,9 +152,11 @@ __global__ __launch_bounds__(XT) void k_xpass(const cx<T> *__restrict__ rin,
   __syncthreads();
 
   const int rows_per_plane = N / (2 * XR);
-  const int z = blockIdx.x / rows_per_plane;
+  const int z = blockIdx.x / rows_per_plane, box = blockIdx.y;
   const int w = threadIdx.x / LPR, lane = threadIdx.x % LPR;
   const int ya = (blockIdx.x % rows_per_plane) * (2 * XR) + 2 * w;
+  rin += box * g.box_b;
+  pxo += box * g.box_a;
   const size_t plane = (size_t)N * g.kxp;           // one z plane of one component in R / PX
   const size_t vol = (size_t)N * N * N;
   cx<T> *wb = buf + w * N;                         // this row pair's transform buffer
@@ -205,8 +216,8 @@ __global__ __launch_bounds__(XT) void k_xpass(const cx<T> *__restrict__ rin,
   }
   if (xa.store_u || xa.store_phi || xa.store_w) {
     const size_t rowa = ((size_t)z * N + ya) * N, rowb = rowa + N;
-    T *ph = reinterpret_cast<T *>(xa.phys);
-    T *pw = reinterpret_cast<T *>(xa.physw);
+    T *ph = reinterpret_cast<T *>(xa.phys) + box * xa.box_phys;
+    T *pw = reinterpret_cast<T *>(xa.physw) + box * xa.box_physw;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int x = lane + LPR * q;
@@ -279,7 +290,7 @@ __global__ __launch_bounds__(XT) void k_xpass(const cx<T> *__restrict__ rin,
       double b = red[0];
       for (int i = 1; i < NT / 64; ++i)
         if (red[i] > b || red[i] != red[i]) b = red[i];
-      atomicMax(xa.max_bits, (unsigned long long)__double_as_longlong(b));
+      atomicMax(xa.max_bits + box, (unsigned long long)__double_as_longlong(b));
     }
   }
 }

@@ -90,8 +90,7 @@ struct PipelineImpl final : Pipeline {
 
   template <int NTY>
   void yfwd_v(Ctx &c) {
-    const int nc = c.cfg.has_scalar ? 6 : 3;
-    hipLaunchKernelGGL((k_yfwd<T, N, TX, NTY>), dim3(tiles(c), N, nc), dim3(NTY), LDS_COL, c.stream,
+    hipLaunchKernelGGL((k_yfwd<T, N, TX, NTY>), dim3(tiles(c), N, c.geo.ncf * c.nbox), dim3(NTY), LDS_COL, c.stream,
                        (const cx<T> *)c.bufa, (cx<T> *)c.bufb, c.geo, (const cx<double> *)c.tw);
   }
   void yfwd(Ctx &c) override { c.sched[0] ? yfwd_v<kYThreads[1]>(c) : yfwd_v<kYThreads[0]>(c); }
@@ -99,7 +98,7 @@ struct PipelineImpl final : Pipeline {
   template <int V>
   void zf_v(Ctx &c, const StageArgs &sa) {
     constexpr int ZN = ZSched<T>::zf_nt[V];
-    const dim3 grid(c.geo.kxp / TX, K);
+    const dim3 grid(c.geo.kxp / TX, K, c.nbox);
     const cx<T> *pf = (const cx<T> *)c.bufb;
     cx<T> *uh = (cx<T> *)c.uh, *hu = (cx<T> *)c.hu, *ph = (cx<T> *)c.ph, *hp = (cx<T> *)c.hp;
     const cx<double> *tw = (const cx<double> *)c.tw;
@@ -122,7 +121,7 @@ struct PipelineImpl final : Pipeline {
 
   void reality(Ctx &c) override {
     const int n = K * K;
-    hipLaunchKernelGGL((k_reality<T>), dim3((n + 255) / 256), dim3(256), 0, c.stream, (cx<T> *)c.uh, (cx<T> *)c.ph,
+    hipLaunchKernelGGL((k_reality<T>), dim3((n + 255) / 256, c.nbox), dim3(256), 0, c.stream, (cx<T> *)c.uh, (cx<T> *)c.ph,
                        c.geo, c.cfg.has_scalar ? 4 : 3);
   }
 
@@ -130,7 +129,7 @@ struct PipelineImpl final : Pipeline {
   void zi_v(Ctx &c) {
     constexpr int ZT = ZSched<T>::zi_tx[V], ZN = ZSched<T>::zi_nt[V];
     constexpr size_t ZL = ((size_t)N * ZT + N) * sizeof(cx<T>);
-    const dim3 grid(c.geo.kxp / ZT, K);
+    const dim3 grid(c.geo.kxp / ZT, K, c.nbox);
     if (c.cfg.has_scalar)
       hipLaunchKernelGGL((k_zi<T, N, ZT, ZN, true>), grid, dim3(ZN), ZL, c.stream, (const cx<T> *)c.uh,
                          (const cx<T> *)c.ph, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, c.band, c.partials);
@@ -142,8 +141,7 @@ struct PipelineImpl final : Pipeline {
 
   template <int NTY>
   void yinv_v(Ctx &c) {
-    const int nc = c.cfg.has_scalar ? 7 : 6;
-    hipLaunchKernelGGL((k_yinv<T, N, TX, NTY>), dim3(tiles(c), N, nc), dim3(NTY), LDS_COL, c.stream,
+    hipLaunchKernelGGL((k_yinv<T, N, TX, NTY>), dim3(tiles(c), N, c.geo.nci * c.nbox), dim3(NTY), LDS_COL, c.stream,
                        (const cx<T> *)c.bufa, (cx<T> *)c.bufb, c.geo, (const cx<double> *)c.tw);
   }
   void yinv(Ctx &c) override { c.sched[0] ? yinv_v<kYThreads[1]>(c) : yinv_v<kYThreads[0]>(c); }
@@ -154,10 +152,10 @@ struct PipelineImpl final : Pipeline {
     constexpr int XT = 256;
     const int blocks = N * N / (2 * (XT / 64));
     if (c.cfg.has_scalar)
-      hipLaunchKernelGGL((k_xpass_wave<T, N, 7, XT, PF>), dim3(blocks), dim3(XT), lds_xw<XT>(), c.stream,
+      hipLaunchKernelGGL((k_xpass_wave<T, N, 7, XT, PF>), dim3(blocks, c.nbox), dim3(XT), lds_xw<XT>(), c.stream,
                          (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
     else
-      hipLaunchKernelGGL((k_xpass_wave<T, N, 6, XT, PF>), dim3(blocks), dim3(XT), lds_xw<XT>(), c.stream,
+      hipLaunchKernelGGL((k_xpass_wave<T, N, 6, XT, PF>), dim3(blocks, c.nbox), dim3(XT), lds_xw<XT>(), c.stream,
                          (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
   }
 
@@ -165,10 +163,10 @@ struct PipelineImpl final : Pipeline {
   void xpass_v(Ctx &c, const XArgs &xa) {
     const int blocks = N * N / (2 * xr<XT>());
     if (c.cfg.has_scalar)
-      hipLaunchKernelGGL((k_xpass<T, N, 7, XT>), dim3(blocks), dim3(XT), lds_x<XT>(), c.stream,
+      hipLaunchKernelGGL((k_xpass<T, N, 7, XT>), dim3(blocks, c.nbox), dim3(XT), lds_x<XT>(), c.stream,
                          (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
     else
-      hipLaunchKernelGGL((k_xpass<T, N, 6, XT>), dim3(blocks), dim3(XT), lds_x<XT>(), c.stream,
+      hipLaunchKernelGGL((k_xpass<T, N, 6, XT>), dim3(blocks, c.nbox), dim3(XT), lds_x<XT>(), c.stream,
                          (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
   }
   void xpass(Ctx &c, const XArgs &xa) override {
@@ -176,38 +174,43 @@ struct PipelineImpl final : Pipeline {
     else c.sched[3] ? xpass_v<kXThreads[1]>(c, xa) : xpass_v<kXThreads[0]>(c, xa);
   }
 
-  void spectrum(Ctx &c, int which, double *bins) override {
-    const cx<T> *f = which == HDNS_FIELD_SCALAR ? (const cx<T> *)c.ph : (const cx<T> *)c.uh;
+  void spectrum(Ctx &c, int which, double *bins) override {   // the selected box
+    const size_t comp = c.comp;
+    const cx<T> *f = which == HDNS_FIELD_SCALAR ? (const cx<T> *)c.ph + c.box * comp
+                                                : (const cx<T> *)c.uh + c.box * 3 * comp;
     const int nc = which == HDNS_FIELD_SCALAR ? 1 : 3;
     hipLaunchKernelGGL((k_spectrum<T>), dim3(1024), dim3(256), 0, c.stream, f, nc, c.geo, bins);
   }
-  TracerArgs targs(const Ctx &c) const { return TracerArgs{c.m, N, c.cfg.side_length}; }
+  TracerArgs targs(const Ctx &c) const {
+    return TracerArgs{c.m, N, c.cfg.side_length, c.cfg.tracer_capacity, c.box_phys, c.ntiles};
+  }
   int tblocks(const Ctx &c) const { return (int)((c.m + 255) / 256); }
   void advect(Ctx &c, double b1, double beta1, double c2) override {
     if (c.m == 0) return;
-    hipLaunchKernelGGL((k_advect<T>), dim3(tblocks(c)), dim3(256), 0, c.stream, (T *)c.pos, (const T *)c.vel,
+    hipLaunchKernelGGL((k_advect<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (T *)c.pos, (const T *)c.vel,
                        (const T *)c.phys, c.order_valid ? c.order : nullptr, targs(c), T(b1), T(beta1), T(c2));
   }
   void sample(Ctx &c) override {
     if (c.m == 0) return;
     const size_t vol = (size_t)N * N * N;
     const T *phi = c.cfg.has_scalar ? (const T *)c.phys + 3 * vol : nullptr;
-    hipLaunchKernelGGL((k_sample<T>), dim3(tblocks(c)), dim3(256), 0, c.stream, (const T *)c.pos, (T *)c.vel,
+    hipLaunchKernelGGL((k_sample<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const T *)c.pos, (T *)c.vel,
                        c.cfg.has_scalar ? (T *)c.phis : nullptr, (const T *)c.phys, phi,
                        c.order_valid ? c.order : nullptr, targs(c));
   }
-  void cell_keys(Ctx &c, int64_t *keys) override {
+  void cell_keys(Ctx &c, int64_t *keys) override {   // the selected box
     if (c.m == 0) return;
-    hipLaunchKernelGGL((k_cell_keys<T>), dim3(tblocks(c)), dim3(256), 0, c.stream, (const T *)c.pos, keys, targs(c));
+    hipLaunchKernelGGL((k_cell_keys<T>), dim3(tblocks(c)), dim3(256), 0, c.stream,
+                       (const T *)c.pos + (size_t)c.box * 3 * c.cfg.tracer_capacity, keys, targs(c));
   }
   void tile_sort(Ctx &c) override {
     if (c.m == 0) return;
-    hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, c.stream, c.tile_counts, 0, (int64_t)c.ntiles);
-    hipLaunchKernelGGL((k_tile_count<T>), dim3(tblocks(c)), dim3(256), 0, c.stream, (const T *)c.pos, c.tile_counts,
-                       targs(c));
-    hipLaunchKernelGGL(k_tile_scan, dim3(1), dim3(1024), 0, c.stream, (const int *)c.tile_counts, c.tile_cursor,
+    hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, c.stream, c.tile_counts, 0, (int64_t)c.ntiles * c.nbox);
+    hipLaunchKernelGGL((k_tile_count<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const T *)c.pos,
+                       c.tile_counts, targs(c));
+    hipLaunchKernelGGL(k_tile_scan, dim3(c.nbox), dim3(1024), 0, c.stream, (const int *)c.tile_counts, c.tile_cursor,
                        c.ntiles);
-    hipLaunchKernelGGL((k_tile_scatter<T>), dim3(tblocks(c)), dim3(256), 0, c.stream, (const T *)c.pos,
+    hipLaunchKernelGGL((k_tile_scatter<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const T *)c.pos,
                        c.tile_cursor, c.order, targs(c));
   }
 };

@@ -29,11 +29,12 @@ struct StageArgs {
   int last;           // 1 at the final stage
   double a, b;        // low-storage coefficients, already rounded to the element type
   double dt;          // the step, rounded to the element type for the accumulator
-  double coef_u;      // -nu * d_i * dt: the velocity exponent per |k|^2
-  double coef_p;      // -(nu/Sc) * d_i * dt
+  double sub;         // d_i * dt: the exponent's substep (the exponent per |k|^2 is -D * sub)
+  const double *visc;   // per box: nu
+  const double *kappa;  // per box: nu / Sc
   double mean_gradient;
   BandSpec band;
-  const double *scalars;  // [0] = the ConstantPower amplitude at this stage's state
+  const double *scalars;  // per box [8]: [0] = the ConstantPower amplitude at this stage's state
 };
 
 // Radix of the first and the last pass of a length-N transform (fft_io's plans).
@@ -49,7 +50,7 @@ template <typename T, int N, int TX, int NT>
 __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, const cx<T> *__restrict__ pf,
                                                const cx<T> *__restrict__ uh, cx<T> *__restrict__ ph,
                                                cx<T> *__restrict__ hp, const Geo &g, const StageArgs &sa, int kx0,
-                                               int ky_idx, int my) {
+                                               int ky_idx, int my, int box) {
   constexpr int M = N / 3, K = 2 * M + 1;
   constexpr int RL = last_radix<N>();
   using SL = PassShape<N, RL, TX, NT>;
@@ -80,6 +81,7 @@ __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, con
   }
   const T Gm = T(sa.mean_gradient);
   const T dt = T(sa.dt), a = T(sa.a), b = T(sa.b);
+  const double coef_p = -sa.kappa[box] * sa.sub;
   __builtin_amdgcn_sched_barrier(0);   // the state loads stay behind the transforms
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
@@ -119,7 +121,7 @@ __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, con
         const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
         const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
         const cx<T> nl = cx<T>{G[p][q].im, -G[p][q].re} - scale(uz[i], Gm);   // -i k.F - G u_z
-        const T E = T(exp(sa.coef_p * k2d));
+        const T E = T(exp(coef_p * k2d));
         cx<T> h = scale(nl, dt);
         if (sa.stage > 0) h = scale(hv[i], a) + h;
         ph[at[i]] = scale(pv[i] + scale(h, b), E);
@@ -162,7 +164,7 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *twl = buf + N * TX;
   const int tid = threadIdx.x;
-  const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y;
+  const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y, box = blockIdx.z;
   const int my = signed_of(ky_idx, g);
   if (!in_disk(kx0, my, N)) return;   // no column of this tile meets the ball (its PF lines are unwritten)
   stage_twiddles<T, N, NT>(twl, tw);
@@ -170,8 +172,15 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
   const size_t comp = (size_t)K * K * g.kxp;           // one compact component
   const size_t pencil = (size_t)N * K * g.kxp;         // one PF component
   const size_t zstride = (size_t)K * g.kxp;
+  pf += box * g.box_b;
+  uh += box * 3 * comp;
+  hu += box * 3 * comp;
+  if constexpr (SCALAR) {
+    ph += box * comp;
+    hp += box * comp;
+  }
 
-  if constexpr (SCALAR) zf_scalar_part<T, N, TX, NT>(buf, twl, pf, uh, ph, hp, g, sa, kx0, ky_idx, my);
+  if constexpr (SCALAR) zf_scalar_part<T, N, TX, NT>(buf, twl, pf, uh, ph, hp, g, sa, kx0, ky_idx, my, box);
 
   // velocity products, forward z transform, straight into the owned modes.  The compiler
   // memory barrier keeps the scalar part's u_z loads from living on into the velocity part.
@@ -204,8 +213,9 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
     else return F[c][p][q];
   };
 
-  const T A = sa.band.on ? T(sa.scalars[0]) : T(0);
+  const T A = sa.band.on ? T(sa.scalars[8 * box]) : T(0);
   const T a = T(sa.a), b = T(sa.b), dt = T(sa.dt);
+  const double coef_u = -sa.visc[box] * sa.sub;
   __builtin_amdgcn_sched_barrier(0);   // the state loads stay behind the transforms
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
@@ -267,7 +277,7 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
 #pragma unroll
           for (int c = 0; c < 3; ++c) nl[c] = nl[c] + scale(u[i][c], A);
         }
-        const T E = T(exp(sa.coef_u * k2d));
+        const T E = T(exp(coef_u * k2d));
         cx<T> un[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
@@ -303,11 +313,13 @@ __global__ __launch_bounds__(NT) void k_zs(const cx<T> *__restrict__ pf, const c
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *twl = buf + N * TX;
-  const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y;
+  const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y, box = blockIdx.z;
   const int my = signed_of(ky_idx, g);
   if (!in_disk(kx0, my, N)) return;   // no column of this tile meets the ball (its PF lines are unwritten)
   stage_twiddles<T, N, NT>(twl, tw);
-  zf_scalar_part<T, N, TX, NT>(buf, twl, pf, uh, ph, hp, g, sa, kx0, ky_idx, my);
+  const size_t comp = comp_of(g);
+  zf_scalar_part<T, N, TX, NT>(buf, twl, pf + box * g.box_b, uh + box * 3 * comp, ph + box * comp,
+                               hp + box * comp, g, sa, kx0, ky_idx, my, box);
 }
 
 // The inverse z pass of the state: u, w = i k x u and phi (with the 1/N^3 of the inverse
@@ -334,12 +346,17 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
   cx<T> *twl = buf + N * TX;
   __shared__ double red[NT / 64][NRED];
   const int tid = threadIdx.x;
-  const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y;
+  const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y, box = blockIdx.z;
   const int my = signed_of(ky_idx, g);
+  // the box's partials follow those of the boxes before it
+  const size_t part = ((size_t)box * gridDim.y * gridDim.x + (size_t)blockIdx.y * gridDim.x + blockIdx.x) * NRED;
   if (!in_disk(line_start<T>(kx0), my, N)) {   // the tile's line misses the ball: never read
-    if (tid < NRED) partials[(size_t)(blockIdx.y * gridDim.x + blockIdx.x) * NRED + tid] = 0.0;
+    if (tid < NRED) partials[part + tid] = 0.0;
     return;
   }
+  uh += box * 3 * comp_of(g);
+  if constexpr (SCALAR) ph += box * comp_of(g);
+  q += box * g.box_a;
   stage_twiddles<T, N, NT>(twl, tw);
   const size_t comp = (size_t)K * K * g.kxp;
   const size_t pencil = (size_t)N * K * g.kxp;
@@ -425,7 +442,7 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
   if (tid < NRED) {
     double s = 0.0;
     for (int w = 0; w < NT / 64; ++w) s += red[w][tid];
-    partials[(size_t)(blockIdx.y * gridDim.x + blockIdx.x) * NRED + tid] = s;
+    partials[part + tid] = s;
   }
 }
 
@@ -434,8 +451,10 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
 template <typename T>
 __global__ void k_reality(cx<T> *__restrict__ uh, cx<T> *__restrict__ ph, Geo g, int ncomp) {
   const int K = g.k;
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  const int l = blockIdx.x * blockDim.x + threadIdx.x, box = blockIdx.y;
   if (l >= K * K) return;
+  uh += box * 3 * comp_of(g);
+  if (ncomp > 3) ph += box * comp_of(g);
   const int kz_idx = l / K, ky_idx = l % K;
   const int pz = (K - kz_idx) % K, py = (K - ky_idx) % K;
   const int lp = pz * K + py;
@@ -466,7 +485,9 @@ struct FinalizeArgs {
 
 static __global__ void k_finalize(const double *__restrict__ partials, double *__restrict__ scalars, FinalizeArgs fa) {
   __shared__ double ss[256][NRED], cc[256][NRED];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, box = blockIdx.x;   // one block per box
+  partials += (size_t)box * fa.nblocks * NRED;
+  scalars += 8 * box;
   double s[NRED] = {0, 0, 0, 0, 0}, c[NRED] = {0, 0, 0, 0, 0};
   for (int b = tid; b < fa.nblocks; b += 256) {
 #pragma unroll

@@ -80,9 +80,12 @@ __device__ __forceinline__ void gather(const T *__restrict__ f, size_t vol, int 
 }
 
 struct TracerArgs {
-  int64_t m;
+  int64_t m;          // tracers per box
   int n;
   double side_length;
+  int64_t cap;        // tracer slots per box: box b's arrays start at b * cap (positions [3][m] inside)
+  size_t box_phys;    // reals of the physical fields per box
+  int ntiles;         // traversal tiles per box
 };
 
 // advect!(Heun) (src/ParticlePropagator.jl:288-315): X* = wrap(X + b1 v), f* = u(X*),
@@ -92,6 +95,11 @@ __global__ void k_advect(T *__restrict__ pos, const T *__restrict__ vel, const T
                          const int *__restrict__ order, TracerArgs ta, T b1, T beta1, T c2) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= ta.m) return;
+  const int box = blockIdx.y;
+  pos += box * 3 * ta.cap;
+  vel += box * 3 * ta.cap;
+  u += box * ta.box_phys;
+  if (order) order += box * ta.cap;
   const int64_t p = order ? order[t] : t;
   const T L = T(ta.side_length), inv_dx = T(ta.n / ta.side_length);
   const size_t vol = (size_t)ta.n * ta.n * ta.n;
@@ -116,6 +124,13 @@ __global__ void k_sample(const T *__restrict__ pos, T *__restrict__ vel, T *__re
                          TracerArgs ta) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= ta.m) return;
+  const int box = blockIdx.y;
+  pos += box * 3 * ta.cap;
+  vel += box * 3 * ta.cap;
+  if (phis) phis += box * ta.cap;
+  u += box * ta.box_phys;
+  if (phi) phi += box * ta.box_phys;
+  if (order) order += box * ta.cap;
   const int64_t p = order ? order[t] : t;
   const T inv_dx = T(ta.n / ta.side_length);
   const size_t vol = (size_t)ta.n * ta.n * ta.n;
@@ -134,7 +149,7 @@ __global__ void k_sample(const T *__restrict__ pos, T *__restrict__ vel, T *__re
 template <typename T>
 __global__ void k_cell_keys(const T *__restrict__ pos, int64_t *__restrict__ keys, TracerArgs ta) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (p >= ta.m) return;
+  if (p >= ta.m) return;   // one box per launch: the caller passes that box's arrays
   const T inv_dx = T(ta.n / ta.side_length);
   const int i = wrap_index((int)floor(pos[p] * inv_dx), ta.n);
   const int j = wrap_index((int)floor(pos[ta.m + p] * inv_dx), ta.n);
@@ -157,13 +172,18 @@ template <typename T>
 __global__ void k_tile_count(const T *__restrict__ pos, int *__restrict__ counts, TracerArgs ta) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= ta.m) return;
+  const int box = blockIdx.y;
+  pos += box * 3 * ta.cap;
+  counts += (size_t)box * ta.ntiles;
   atomicAdd(&counts[tile_of(pos, ta.m, p, ta.n, T(ta.n / ta.side_length))], 1);
 }
 
-// Exclusive scan of the tile counts into the cursors, one block of 1024 threads.
+// Exclusive scan of the tile counts into the cursors, one block of 1024 threads per box.
 static __global__ void k_tile_scan(const int *__restrict__ counts, int *__restrict__ cursor, int ntiles) {
   __shared__ int part[1024];
   const int tid = threadIdx.x;
+  counts += (size_t)blockIdx.x * ntiles;
+  cursor += (size_t)blockIdx.x * ntiles;
   const int chunk = (ntiles + 1023) / 1024;
   const int lo = tid * chunk, hi = min(ntiles, lo + chunk);
   int s = 0;
@@ -188,6 +208,10 @@ __global__ void k_tile_scatter(const T *__restrict__ pos, int *__restrict__ curs
                                TracerArgs ta) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= ta.m) return;
+  const int box = blockIdx.y;
+  pos += box * 3 * ta.cap;
+  cursor += (size_t)box * ta.ntiles;
+  order += box * ta.cap;
   const int slot = atomicAdd(&cursor[tile_of(pos, ta.m, p, ta.n, T(ta.n / ta.side_length))], 1);
   order[slot] = (int)p;
 }

@@ -89,8 +89,10 @@ __global__ __launch_bounds__(XT, PF ? 2 : 3) void k_xpass_wave(const cx<T> *__re
   __syncthreads();
 
   const int rows_per_plane = N / (2 * XR);
-  const int z = blockIdx.x / rows_per_plane;
+  const int z = blockIdx.x / rows_per_plane, box = blockIdx.y;
   const int ya = (blockIdx.x % rows_per_plane) * (2 * XR) + 2 * w;
+  rin += box * g.box_b;
+  pxo += box * g.box_a;
   const size_t plane = (size_t)N * g.kxp;           // one z plane of one component in R / PX
   const size_t vol = (size_t)N * N * N;
 
@@ -166,8 +168,8 @@ __global__ __launch_bounds__(XT, PF ? 2 : 3) void k_xpass_wave(const cx<T> *__re
   };
 
   const size_t rowa = ((size_t)z * N + ya) * N, rowb = rowa + N;
-  T *phys = reinterpret_cast<T *>(xa.phys);
-  T *physw = reinterpret_cast<T *>(xa.physw);
+  T *phys = reinterpret_cast<T *>(xa.phys) + box * xa.box_phys;
+  T *physw = reinterpret_cast<T *>(xa.physw) + box * xa.box_physw;
   auto store_phys = [&](T *base, const cx<T>(&x)[8]) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -258,7 +260,7 @@ __global__ __launch_bounds__(XT, PF ? 2 : 3) void k_xpass_wave(const cx<T> *__re
       double b = red[0];
       for (int i = 1; i < XR; ++i)
         if (red[i] > b || red[i] != red[i]) b = red[i];
-      atomicMax(xa.max_bits, (unsigned long long)__double_as_longlong(b));
+      atomicMax(xa.max_bits + box, (unsigned long long)__double_as_longlong(b));
     }
   }
 }

@@ -73,6 +73,17 @@ typedef struct hdns_ctx hdns_ctx;
  * for the device-resident part of a Simulation (src/SimulationRun.jl:750-931). */
 hdns_status hdns_ctx_create(const hdns_config *config, hdns_ctx **out);
 hdns_status hdns_ctx_destroy(hdns_ctx *ctx);
+/* A batch: nbox boxes of one size that differ in viscosity (viscosities[nbox]; NULL: the
+ * config's for all), stored back to back and advanced by the same launches (every kernel's grid
+ * carries a box dimension).  hdns_step, hdns_step_graph, hdns_advect, hdns_deposit_tracers and
+ * hdns_rebuild_traversal act on every box; the per-box entry points (state upload/download,
+ * reductions, max speed, spectra, physical fields, tracers) address the box chosen by
+ * hdns_select_box (box 0 at creation).  Every box carries the same number of tracers.  No
+ * counterpart in the reference, which advances one Simulation at a time; it is the building block
+ * of the ensemble below. */
+hdns_status hdns_ctx_create_batch(const hdns_config *config, int32_t nbox, const double *viscosities,
+                                  hdns_ctx **out);
+hdns_status hdns_select_box(hdns_ctx *ctx, int32_t box);
 /* Message of the last failure on this context (or of the last failed create when ctx is NULL). */
 const char *hdns_last_error(const hdns_ctx *ctx);
 /* Device bytes the context holds. */
@@ -161,6 +172,22 @@ hdns_status hdns_schedule(hdns_ctx *ctx, const int32_t *set, int32_t *get);
  * milliseconds and launches[8] counts since the last reset (synchronises the stream). */
 hdns_status hdns_profile_enable(hdns_ctx *ctx, int32_t on);
 hdns_status hdns_profile_read(hdns_ctx *ctx, double *ms, int64_t *launches, int32_t reset);
+
+/* Ensembles of boxes of uneven sizes and viscosities (no counterpart in the reference).  The
+ * configs may differ in grid_points and viscosity only.  Boxes of one size form a batch context;
+ * all batches share one stream, and hdns_ensemble_step_graph advances every box with one graph
+ * launch (captured once per dt).  hdns_ensemble_member returns box `box`'s batch context with that
+ * box selected, for the per-box entry points (the ensemble owns it: do not destroy it). */
+typedef struct hdns_ensemble hdns_ensemble;
+hdns_status hdns_ensemble_create(const hdns_config *configs, int32_t nbox, hdns_ensemble **out);
+hdns_status hdns_ensemble_destroy(hdns_ensemble *ens);
+const char *hdns_ensemble_last_error(const hdns_ensemble *ens);
+hdns_status hdns_ensemble_member(hdns_ensemble *ens, int32_t box, hdns_ctx **ctx);
+hdns_status hdns_ensemble_step(hdns_ensemble *ens, double dt);
+hdns_status hdns_ensemble_step_graph(hdns_ensemble *ens, double dt);
+/* The tracer half of an iteration for every box: traversal rebuild, Heun push, sample. */
+hdns_status hdns_ensemble_advance_tracers(hdns_ensemble *ens, double dt);
+hdns_status hdns_ensemble_synchronize(hdns_ensemble *ens);
 
 /* Synchronise the context's stream (for timing from the host). */
 hdns_status hdns_synchronize(hdns_ctx *ctx);

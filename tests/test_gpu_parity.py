@@ -258,3 +258,59 @@ def test_autotune_leaves_the_state_bitwise_unchanged_and_schedules_agree(g):
         a, b = sim.state(), ref.state()
         assert rel(a.velocity_spectral, b.velocity_spectral) < 1e-6
         assert rel(a.scalar_spectral, b.scalar_spectral) < 1e-6
+
+
+def _member(g, n, dtype, nu, seed, m=0):
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
+    cond = g.FlowConditions(viscosity=nu, schmidt=1.0)
+    state = g.ResolvedFields(g.random_phase_velocity(n, seed=seed), g.random_phase_scalar(n, seed=seed + 1))
+    tr = g.Tracers(dom, number_of_particles=m, seed=seed + 2) if m else None
+    return g.EnsembleMember(dom, cond, state, tr)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_ensemble_boxes_equal_their_standalone_runs_bitwise(g, dtype):
+    """Uneven sizes (3 | 48 included) and viscosities in one ensemble, IF-RK3, graph-captured: each
+    box's state, reductions, max|u| and tracers equal a standalone run of that box bit for bit."""
+    sizes = [32, 48, 32, 64, 48]
+    nus = [0.005, 0.012, 0.02, 0.008, 0.05]
+    m, dt, steps = 64, 2e-3, 3
+    members = [_member(g, n, dtype, nu, 100 + 7 * b, m) for b, (n, nu) in enumerate(zip(sizes, nus))]
+    with g.Ensemble(members, field_integration_scheme=g.IFLowStorageRK3, mean_gradient=1.0) as ens:
+        for s in range(steps):
+            ens.step(dt, graph=s != 1)   # graph replays around one eager step
+            ens.advance_tracers(dt)
+        for b, mb in enumerate(members):
+            alone = g.Simulation(mb.domain, conditions=mb.conditions, state=mb.state, tracers=mb.tracers,
+                                 field_integration_scheme=g.IFLowStorageRK3, mean_gradient=1.0)
+            with alone:
+                for _ in range(steps):
+                    alone.advance(dt, check=False)
+                box = ens.box(b)
+                a, e = alone.state(), box.state()
+                print(f"box {b} ({sizes[b]}^3, nu {nus[b]}): max|du| "
+                      f"{np.max(np.abs(a.velocity_spectral - e.velocity_spectral)):.3e}", flush=True)
+                assert np.array_equal(a.velocity_spectral, e.velocity_spectral), b
+                assert np.array_equal(a.scalar_spectral, e.scalar_spectral), b
+                assert alone.reductions() == box.reductions(), b
+                assert alone.max_speed() == box.max_speed(), b
+                assert np.array_equal(alone.tracer_positions(), box.tracer_positions()), b
+                va, sa = alone.tracer_samples()
+                ve, se = box.tracer_samples()
+                assert np.array_equal(va, ve) and np.array_equal(sa, se), b
+
+
+def test_ensemble_box_at_48_matches_the_fp64_oracle(g):
+    """A 48^3 box of a mixed ensemble (FP64, IF-RK3) against the oracle after two steps."""
+    members = [_member(g, 32, np.float64, 0.01, 11), _member(g, 48, np.float64, 0.02, 12)]
+    d = dns.Domain(48)
+    oc = dns.Conditions(viscosity=0.02, schmidt=1.0, forcing=None, mean_gradient=1.0, band_eps=float(np.finfo(np.float64).eps))
+    uo, po = members[1].state.velocity_spectral, members[1].state.scalar_spectral
+    dep = None
+    with g.Ensemble(members, field_integration_scheme=g.IFLowStorageRK3, mean_gradient=1.0) as ens:
+        for _ in range(2):
+            ens.step(2e-3)
+            uo, po, dep = dns.step(d, oc, "rk3", uo, po, 2e-3, dep)
+        got = ens.box(1).state()
+        assert rel(got.velocity_spectral, uo) < 1e-12
+        assert rel(got.scalar_spectral, po) < 1e-12
