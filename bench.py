@@ -153,6 +153,59 @@ def time_tracers(g, n: int, m: int, dtype, steps: int, H) -> dict:
             "tracer_steps_per_s": round(m / (per * 1e-3), 1)}
 
 
+def time_ensemble(g, steps: int, H, boxes_per_size: int = 32, tracers: int = 4096) -> dict:
+    """The batched ensemble: 32 boxes each of 32^3, 48^3 and 64^3, nu log-uniform in [0.005, 0.05],
+    own seeded random-phase fields and scalars, IF-RK3 FP32-bulk, 4096 tracers per box; one
+    graph launch per step, then the tracer half of the iteration."""
+    sizes = [32] * boxes_per_size + [48] * boxes_per_size + [64] * boxes_per_size
+    rng = np.random.default_rng(2029)
+    nus = np.exp(rng.uniform(np.log(0.005), np.log(0.05), len(sizes)))
+    members = []
+    for b, (n, nu) in enumerate(zip(sizes, nus)):
+        dom = g.FixedDomain(grid_points_per_axis=n, side_length=2 * math.pi, dealiasing=g.TwoThirds(),
+                            element_type=np.float32)
+        state = g.ResolvedFields(g.random_phase_velocity(n, seed=3000 + b), g.random_phase_scalar(n, seed=5000 + b))
+        members.append(g.EnsembleMember(dom, g.FlowConditions(viscosity=float(nu), schmidt=1.0), state,
+                                        g.Tracers(dom, number_of_particles=tracers, seed=7000 + b)))
+    dt = 1e-3
+    with g.Ensemble(members, field_integration_scheme=g.IFLowStorageRK3, mean_gradient=1.0) as ens:
+        stream = ctypes.c_void_p()
+        h = ctypes.c_void_p()
+        g._lib.check(g._lib.lib().hdns_ensemble_member(ens._h, 0, ctypes.byref(h)))
+        g._lib.check(g._lib.lib().hdns_stream(h, ctypes.byref(stream)))   # the ensemble's shared stream
+        for _ in range(3):
+            ens.step(dt)
+            ens.advance_tracers(dt)
+        e0, e1, e2 = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        for e in (e0, e1, e2):
+            H.hipEventCreate(ctypes.byref(e))
+        field_ms = 0.0
+        H.hipEventRecord(e0, stream)
+        for _ in range(steps):
+            H.hipEventRecord(e1, stream)
+            ens.step(dt)
+            H.hipEventRecord(e2, stream)
+            ens.advance_tracers(dt)
+            H.hipEventSynchronize(e2)
+            ms = ctypes.c_float()
+            H.hipEventElapsedTime(ctypes.byref(ms), e1, e2)
+            field_ms += ms.value
+        H.hipEventRecord(e1, stream)
+        H.hipEventSynchronize(e1)
+        total = ctypes.c_float()
+        H.hipEventElapsedTime(ctypes.byref(total), e0, e1)
+        for e in (e0, e1, e2):
+            H.hipEventDestroy(e)
+    points = sum(n ** 3 for n in sizes)
+    per = total.value / steps
+    return {"boxes": len(sizes), "sizes": {"32": boxes_per_size, "48": boxes_per_size, "64": boxes_per_size},
+            "scheme": "IFLowStorageRK3", "dtype": "f32", "graph": True, "steps": steps,
+            "ms_per_iteration": round(per, 4), "field_ms_per_step": round(field_ms / steps, 4),
+            "gridpoint_steps_per_s": round(points / (field_ms / steps * 1e-3), 1),
+            "tracer_steps_per_s": round(len(sizes) * tracers / (per * 1e-3), 1),
+            "tracers_per_box": tracers}
+
+
 def copy_calibration(H, nbytes: int = 1 << 30) -> None:
     """One device-to-device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
     H.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
@@ -221,6 +274,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=256)
     ap.add_argument("--no-tracers", action="store_true", help="skip the tracer throughput figure")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 96-box ensemble figure")
+    ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy (the reference kernel for counter scaling)")
     args = ap.parse_args()
@@ -266,6 +321,11 @@ def main() -> int:
     if not args.no_tracers:
         print("bench: tracers, 256^3 field, 2^20 tracers", file=sys.stderr, flush=True)
         tracers = [time_tracers(g, 256, 1 << 20, np.float32, 5, H), time_tracers(g, 256, 1 << 20, np.float64, 5, H)]
+    ensemble = None
+    if not args.no_ensemble:
+        print(f"bench: ensemble, 96 boxes (32^3, 48^3, 64^3), IF-RK3, {args.ensemble_steps} steps",
+              file=sys.stderr, flush=True)
+        ensemble = time_ensemble(g, args.ensemble_steps, H)
     if args.copy_calibration:
         copy_calibration(H)
     cpu = None
@@ -286,6 +346,7 @@ def main() -> int:
         "memory_GiB": round(r32["memory_bytes"] / 2 ** 30, 2),
         "autotuned_schedule": r32["schedule"],
         "tracers": tracers,
+        "ensemble": ensemble,
     }
     print(json.dumps(line), flush=True)
     return 0
