@@ -457,6 +457,38 @@ class Simulation:
         self._call("hdns_step_graph" if graph else "hdns_step", float(dt))
         self.fields_epoch += 1
 
+    # the adjoint (the rrule of step!)
+    def step_vjp(self, dt: float, lam_u: np.ndarray, lam_phi: np.ndarray | None = None):
+        """The transpose of one step's derivative at the current state (which is left unchanged):
+        the adjoints (lam_u, lam_phi) of the step's output, in the state layout, become those of
+        its input.  Adjoints are gradients under the real inner product of the half spectrum with
+        the Parseval weights (1 on kx = 0, 2 inside)."""
+        n = self.domain.grid_points_per_axis
+        lu = np.ascontiguousarray(lam_u, dtype=np.complex128)
+        if lu.shape != (3, n, n, n // 2 + 1):
+            raise ValueError("lam_u must have the velocity state's shape")
+        lp = None
+        if self.carries_scalar:
+            lp = np.ascontiguousarray(lam_phi if lam_phi is not None else np.zeros((n, n, n // 2 + 1)),
+                                      dtype=np.complex128)
+        out_u = np.zeros_like(lu)
+        out_p = np.zeros_like(lp) if lp is not None else None
+        self._call("hdns_step_vjp", float(dt), L.dptr(lu), L.dptr(lp) if lp is not None else None, L.dptr(out_u),
+                   L.dptr(out_p) if out_p is not None else None)
+        return out_u, out_p
+
+    def adjoint_gradient(self, dt: float, steps: int, checkpoint_every: int = 20):
+        """J = k + <phi^2> after `steps` steps from the current state, and its gradient with respect
+        to the current u_hat (and phi_hat), by the hand-written adjoint with checkpoints every
+        `checkpoint_every` steps.  The run's state is left unchanged.  Returns (J, grad_u, grad_phi)."""
+        n = self.domain.grid_points_per_axis
+        J = c_double()
+        gu = np.zeros((3, n, n, n // 2 + 1), dtype=np.complex128)
+        gp = np.zeros((n, n, n // 2 + 1), dtype=np.complex128) if self.carries_scalar else None
+        self._call("hdns_adjoint_gradient", float(dt), int(steps), int(checkpoint_every), ctypes.byref(J), L.dptr(gu),
+                   L.dptr(gp) if gp is not None else None)
+        return J.value, gu, gp
+
     def cfl_timestep(self) -> float:
         """The advective/resolution/max_dt bound (src/SimulationRun.jl:1051-1096)."""
         dx = self.domain.grid_spacing
@@ -576,6 +608,7 @@ class EnsembleBox(Simulation):
         raise RuntimeError("an ensemble box is advanced by its Ensemble (Ensemble.step / advance_tracers)")
 
     step_fields = advance = run = advect = deposit_tracers = autotune = schedule = _advanced_by_the_ensemble
+    step_vjp = adjoint_gradient = _advanced_by_the_ensemble
 
     def __repr__(self) -> str:
         n = self.domain.grid_points_per_axis

@@ -156,6 +156,16 @@ void release(Ctx &c) {
                   c.visc_d, c.kappa_d};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  AdjBufs &a = c.adj;
+  void *aptrs[] = {a.lq, a.lh, a.t6, a.t10, a.pa, a.pb, a.rscal, a.partials, a.proj};
+  for (void *p : aptrs)
+    if (p) (void)hipFree(p);
+  for (int i = 0; i < 5; ++i) {
+    if (a.rq[i]) (void)hipFree(a.rq[i]);
+    if (a.rphys[i]) (void)hipFree(a.rphys[i]);
+    if (a.rphysw[i]) (void)hipFree(a.rphysw[i]);
+  }
+  c.adj = AdjBufs{};
   if (c.stream && c.own_stream) (void)hipStreamDestroy(c.stream);
   c.stream = nullptr;
 }
@@ -207,7 +217,7 @@ void prof_end(Ctx &c) {
 // The inverse half of the pipeline for the current state: the field deposit!
 // (src/Propagator.jl:808-842): u, w, phi in physical space, their products for the next
 // stage 1, max|u|^2, and the reductions of the state.
-hdns_status run_deposit(hdns_ctx *ctx) {
+hdns_status run_deposit(hdns_ctx *ctx, void *phys_to = nullptr, void *physw_to = nullptr) {
   Ctx &c = ctx->c;
   c.pipe->zi(c);
   const FinalizeArgs fa{c.pipe->z_blocks(c), 1.0 / pow((double)c.geo.n, 6.0),
@@ -226,6 +236,13 @@ hdns_status run_deposit(hdns_ctx *ctx) {
   xa.phys = c.phys;
   xa.physw = c.physw;
   xa.max_bits = c.maxbits;
+  if (phys_to) {   // the adjoint's stage record: u, phi and w into the given arrays
+    xa.store_u = 1;
+    xa.store_phi = c.cfg.has_scalar;
+    xa.store_w = 1;
+    xa.phys = phys_to;
+    xa.physw = physw_to;
+  }
   c.pipe->xpass(c, xa);
   c.deposited = true;
   return launched(ctx);
@@ -588,7 +605,17 @@ hdns_status hdns_state_download(hdns_ctx *ctx, double *u_hat, double *phi_hat) {
 namespace {
 
 // Every launch of one step! + its closing deposit!, on the context's stream, no host sync.
-void launch_step(Ctx &c, double dt) {
+// Where a recording step keeps every stage's input, for the adjoint: the compact state
+// ([4][comp]: u, phi), the normalised sums and amplitude of finalize ([8]) and the physical
+// fields ([4][vol] u, phi and [3][vol] w; stage 0's are the caller's deposit).
+struct StageRecord {
+  void *q[5];
+  double *scal[5];
+  void *phys[5];
+  void *physw[5];
+};
+
+void launch_step(Ctx &c, double dt, const StageRecord *rec = nullptr) {
   const bool fp32 = !c.fp64;
   const Coeffs k = scheme_coefficients(c.cfg.scheme, fp32);
   const double dt_t = fp32 ? (double)(float)dt : dt;
@@ -620,6 +647,20 @@ void launch_step(Ctx &c, double dt) {
     sa.mean_gradient = c.cfg.mean_gradient;
     sa.band = c.band;
     sa.scalars = c.scalars;
+    XArgs xa = last ? fin : mid;
+    if (rec) {
+      const size_t cb = c.comp * c.elem;
+      (void)hipMemcpyAsync(rec->q[i], c.uh, 3 * cb, hipMemcpyDeviceToDevice, c.stream);
+      if (c.cfg.has_scalar) (void)hipMemcpyAsync((char *)rec->q[i] + 3 * cb, c.ph, cb, hipMemcpyDeviceToDevice, c.stream);
+      (void)hipMemcpyAsync(rec->scal[i], c.scalars, 8 * sizeof(double), hipMemcpyDeviceToDevice, c.stream);
+      if (!last) {   // the x pass of this stage recovers the next stage's physical fields
+        xa.store_u = 1;
+        xa.store_phi = c.cfg.has_scalar;
+        xa.store_w = 1;
+        xa.phys = rec->phys[i + 1];
+        xa.physw = rec->physw[i + 1];
+      }
+    }
     TIMED(c, K_YFWD, c.pipe->yfwd(c));
     TIMED(c, K_ZF, c.pipe->zf(c, sa));
     if (last) TIMED(c, K_REALITY, c.pipe->reality(c));
@@ -629,7 +670,7 @@ void launch_step(Ctx &c, double dt) {
                              fa));
     TIMED(c, K_YINV, c.pipe->yinv(c));
     if (last) hipLaunchKernelGGL(k_fill_u64, dim3((c.nbox + 63) / 64), dim3(64), 0, c.stream, c.maxbits, 0ull, c.nbox);
-    TIMED(c, K_XPASS, c.pipe->xpass(c, last ? fin : mid));
+    TIMED(c, K_XPASS, c.pipe->xpass(c, xa));
   }
 }
 
@@ -1065,6 +1106,255 @@ hdns_status hdns_stream(hdns_ctx *ctx, void **stream) {
   if (!ctx || !stream) return HDNS_ERR_INVALID_ARGUMENT;
   *stream = (void *)ctx->c.stream;
   return HDNS_OK;
+}
+
+// --- the adjoint step -------------------------------------------------------------------
+
+}  // extern "C"
+
+namespace {
+
+hdns_status adj_alloc(hdns_ctx *ctx) {
+  Ctx &c = ctx->c;
+  AdjBufs &a = c.adj;
+  if (a.ready) return HDNS_OK;
+  if (c.nbox != 1) return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint runs on a single-box context");
+  const size_t cb = c.comp * c.elem, realb = c.fp64 ? 8 : 4;
+  const size_t vol = (size_t)c.geo.n * c.geo.n * c.geo.n;
+  hdns_status s = HDNS_OK;
+  if (s == HDNS_OK) s = alloc(ctx, &a.lq, 4 * cb);
+  if (s == HDNS_OK) s = alloc(ctx, &a.lh, 4 * cb);
+  if (s == HDNS_OK) s = alloc(ctx, &a.t6, 6 * cb);
+  if (s == HDNS_OK) s = alloc(ctx, &a.t10, 10 * cb);
+  if (s == HDNS_OK) s = alloc(ctx, &a.pa, c.pipe->adj_pencil_a(c) * c.elem);
+  if (s == HDNS_OK) s = alloc(ctx, &a.pb, c.pipe->adj_pencil_b(c) * c.elem);
+  for (int i = 0; i < c.stages && s == HDNS_OK; ++i) {
+    s = alloc(ctx, &a.rq[i], 4 * cb);
+    if (s == HDNS_OK) s = alloc(ctx, &a.rphys[i], 4 * vol * realb);
+    if (s == HDNS_OK) s = alloc(ctx, &a.rphysw[i], 3 * vol * realb);
+  }
+  if (s == HDNS_OK) s = alloc(ctx, (void **)&a.rscal, 5 * 8 * sizeof(double));
+  if (s == HDNS_OK) s = alloc(ctx, (void **)&a.partials, ADJ_BLOCKS * sizeof(double));
+  if (s == HDNS_OK) s = alloc(ctx, (void **)&a.proj, sizeof(double));
+  if (s != HDNS_OK) return s;
+  // the scalar components stay zero when the context carries no scalar
+  zero_f64(c, a.lq, 4 * cb);
+  zero_f64(c, a.lh, 4 * cb);
+  for (int i = 0; i < c.stages; ++i) {
+    zero_f64(c, a.rq[i], 4 * cb);
+    zero_f64(c, a.rphys[i], 4 * vol * realb);
+  }
+  a.ready = true;
+  return launched(ctx);
+}
+
+// copy the state ([4][comp] at src, or to dst)
+void state_copy_in(Ctx &c, const void *src) {
+  const size_t cb = c.comp * c.elem;
+  (void)hipMemcpyAsync(c.uh, src, 3 * cb, hipMemcpyDeviceToDevice, c.stream);
+  if (c.cfg.has_scalar) (void)hipMemcpyAsync(c.ph, (const char *)src + 3 * cb, cb, hipMemcpyDeviceToDevice, c.stream);
+}
+void state_copy_out(Ctx &c, void *dst) {
+  const size_t cb = c.comp * c.elem;
+  (void)hipMemcpyAsync(dst, c.uh, 3 * cb, hipMemcpyDeviceToDevice, c.stream);
+  if (c.cfg.has_scalar) (void)hipMemcpyAsync((char *)dst + 3 * cb, c.ph, cb, hipMemcpyDeviceToDevice, c.stream);
+}
+
+// the transpose of one step at the current state: lam (in c.adj.lq, the output's adjoint) becomes
+// the input's.  The state is recorded (advanced by one step) and restored.
+hdns_status adj_step(hdns_ctx *ctx, double dt) {
+  Ctx &c = ctx->c;
+  AdjBufs &a = c.adj;
+  StageRecord rec{};
+  for (int i = 0; i < c.stages; ++i) {
+    rec.q[i] = a.rq[i];
+    rec.scal[i] = a.rscal + 8 * i;
+    rec.phys[i] = a.rphys[i];
+    rec.physw[i] = a.rphysw[i];
+  }
+  hdns_status s = run_deposit(ctx, a.rphys[0], a.rphysw[0]);
+  if (s != HDNS_OK) return s;
+  launch_step(c, dt, &rec);
+  state_copy_in(c, a.rq[0]);   // back to the step's input
+  s = run_deposit(ctx);
+  if (s != HDNS_OK) return s;
+  const bool fp32 = !c.fp64;
+  const Coeffs k = scheme_coefficients(c.cfg.scheme, fp32);
+  c.pipe->adj_begin(c);
+  for (int i = c.stages - 1; i >= 0; --i) {
+    AdjStage as{};
+    as.a = k.a[i];
+    as.b = k.b[i];
+    as.dt = fp32 ? (double)(float)dt : dt;
+    as.sub = k.d[i] * dt;
+    as.visc = c.visc[0];
+    as.kappa = c.visc[0] / c.cfg.schmidt;
+    as.mean_gradient = c.cfg.mean_gradient;
+    as.last = i == c.stages - 1;
+    as.scalar = c.cfg.has_scalar;
+    as.band = c.band;
+    as.scal = a.rscal + 8 * i;
+    as.proj = a.proj;
+    c.pipe->adj_stage(c, as, i);
+  }
+  return launched(ctx);
+}
+
+// host full layout <-> compact [4][comp] device array (components 0..2 from u, 3 from phi)
+hdns_status compact_in(hdns_ctx *ctx, const double *u_full, const double *phi_full, void *dev) {
+  Ctx &c = ctx->c;
+  const Geo &g = c.geo;
+  const int n = g.n, hx = n / 2 + 1;
+  const size_t full = (size_t)n * n * hx;
+  std::vector<unsigned char> host(4 * c.comp * c.elem, 0);
+  for (int q = 0; q < 4; ++q) {
+    const double *src = q < 3 ? u_full + 2 * full * q : phi_full;
+    if (!src) continue;
+    for (int kz = 0; kz < g.k; ++kz)
+      for (int ky = 0; ky < g.k; ++ky) {
+        const double *row = src + 2 * (((size_t)bin_index(kz, g) * n + bin_index(ky, g)) * hx);
+        const size_t at = q * c.comp + ((size_t)kz * g.k + ky) * g.kxp;
+        for (int ix = 0; ix <= g.m; ++ix) {
+          if (c.fp64) {
+            double *d = (double *)host.data() + 2 * (at + ix);
+            d[0] = row[2 * ix];
+            d[1] = row[2 * ix + 1];
+          } else {
+            float *d = (float *)host.data() + 2 * (at + ix);
+            d[0] = (float)row[2 * ix];
+            d[1] = (float)row[2 * ix + 1];
+          }
+        }
+      }
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(dev, host.data(), host.size(), hipMemcpyHostToDevice, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  return HDNS_OK;
+}
+
+hdns_status compact_out(hdns_ctx *ctx, const void *dev, double *u_full, double *phi_full) {
+  Ctx &c = ctx->c;
+  const Geo &g = c.geo;
+  const int n = g.n, hx = n / 2 + 1;
+  const size_t full = (size_t)n * n * hx;
+  std::vector<unsigned char> host(4 * c.comp * c.elem);
+  HIP_TRY(ctx, hipMemcpyAsync(host.data(), dev, host.size(), hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  for (int q = 0; q < 4; ++q) {
+    double *dst = q < 3 ? u_full + 2 * full * q : phi_full;
+    if (!dst) continue;
+    memset(dst, 0, 2 * full * sizeof(double));
+    for (int kz = 0; kz < g.k; ++kz)
+      for (int ky = 0; ky < g.k; ++ky) {
+        double *row = dst + 2 * (((size_t)bin_index(kz, g) * n + bin_index(ky, g)) * hx);
+        const size_t at = q * c.comp + ((size_t)kz * g.k + ky) * g.kxp;
+        for (int ix = 0; ix <= g.m; ++ix) {
+          if (c.fp64) {
+            const double *v = (const double *)host.data() + 2 * (at + ix);
+            row[2 * ix] = v[0];
+            row[2 * ix + 1] = v[1];
+          } else {
+            const float *v = (const float *)host.data() + 2 * (at + ix);
+            row[2 * ix] = v[0];
+            row[2 * ix + 1] = v[1];
+          }
+        }
+      }
+  }
+  return HDNS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+hdns_status hdns_step_vjp(hdns_ctx *ctx, double dt, const double *lam_u_out, const double *lam_phi_out,
+                          double *lam_u_in, double *lam_phi_in) {
+  hdns_status s = step_checks(ctx, dt);
+  if (s != HDNS_OK) return s;
+  if (!lam_u_out || !lam_u_in) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "the velocity adjoints must not be NULL");
+  Ctx &c = ctx->c;
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  s = adj_alloc(ctx);
+  if (s == HDNS_OK) s = compact_in(ctx, lam_u_out, c.cfg.has_scalar ? lam_phi_out : nullptr, c.adj.lq);
+  if (s == HDNS_OK) s = adj_step(ctx, dt);
+  if (s == HDNS_OK) s = compact_out(ctx, c.adj.lq, lam_u_in, c.cfg.has_scalar ? lam_phi_in : nullptr);
+  return s;
+}
+
+hdns_status hdns_adjoint_gradient(hdns_ctx *ctx, double dt, int32_t steps, int32_t checkpoint_every,
+                                  double *objective, double *grad_u, double *grad_phi) {
+  hdns_status s = step_checks(ctx, dt);
+  if (s != HDNS_OK) return s;
+  if (steps < 1 || checkpoint_every < 1)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "steps and checkpoint_every must be at least 1");
+  if (!objective || !grad_u) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "objective and grad_u must not be NULL");
+  Ctx &c = ctx->c;
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  s = adj_alloc(ctx);
+  if (s != HDNS_OK) return s;
+  const size_t sb = 4 * c.comp * c.elem;   // one saved state
+  const int every = std::min(checkpoint_every, (int)steps);
+  const int nck = (steps + every - 1) / every;
+  std::vector<void *> ck(nck, nullptr), seg(every, nullptr);
+  auto cleanup = [&]() {
+    for (void *p : ck)
+      if (p) (void)hipFree(p);
+    for (void *p : seg)
+      if (p) (void)hipFree(p);
+  };
+  for (int i = 0; i < nck && s == HDNS_OK; ++i) {
+    hipError_t e = hipMalloc(&ck[i], sb);
+    if (e != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, "checkpoint allocation failed");
+  }
+  for (int i = 0; i < every && s == HDNS_OK; ++i) {
+    hipError_t e = hipMalloc(&seg[i], sb);
+    if (e != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, "segment allocation failed");
+  }
+  if (s != HDNS_OK) {
+    cleanup();
+    return s;
+  }
+  // forward, keeping a checkpoint every `every` steps (checkpoint 0 is the initial state)
+  for (int t = 0; t < steps; ++t) {
+    if (t % every == 0) state_copy_out(c, ck[t / every]);
+    launch_step(c, dt);
+  }
+  double sc[8];
+  hipError_t e = hipMemcpyAsync(sc, c.scalars, sizeof sc, hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(ctx, HDNS_ERR_DEVICE, std::string("adjoint forward: ") + hipGetErrorString(e));
+  }
+  *objective = 0.5 * sc[1] + (c.cfg.has_scalar ? sc[4] : 0.0);
+  c.pipe->adj_objective(c);
+  // reverse sweep: each segment is recomputed from its checkpoint, then walked backwards
+  for (int sgi = nck - 1; sgi >= 0 && s == HDNS_OK; --sgi) {
+    const int t0 = sgi * every, t1 = std::min((int)steps, t0 + every);
+    state_copy_in(c, ck[sgi]);
+    s = run_deposit(ctx);   // a step starts from its state's products
+    for (int t = t0; t < t1 && s == HDNS_OK; ++t) {
+      state_copy_out(c, seg[t - t0]);
+      if (t + 1 < t1) launch_step(c, dt);
+    }
+    for (int t = t1 - 1; t >= t0 && s == HDNS_OK; --t) {
+      state_copy_in(c, seg[t - t0]);
+      s = adj_step(ctx, dt);
+    }
+  }
+  if (s == HDNS_OK) s = compact_out(ctx, c.adj.lq, grad_u, c.cfg.has_scalar ? grad_phi : nullptr);
+  // the context is left at the initial state
+  if (s == HDNS_OK) {
+    state_copy_in(c, ck[0]);
+    s = run_deposit(ctx);
+  }
+  if (s == HDNS_OK) {
+    e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, std::string("adjoint: ") + hipGetErrorString(e));
+  }
+  cleanup();
+  return s;
 }
 
 // --- ensembles ---------------------------------------------------------------------------

@@ -7,11 +7,26 @@
 #include <vector>
 
 #include "../../include/hdns.h"
-#include "spectral.hpp"
+#include "adjoint.hpp"
 
 namespace hdns {
 
 struct Ctx;
+
+
+// Device buffers of the adjoint step (allocated on first use): the adjoints of the state and the
+// register ([4][comp]: u, phi), the spectra of the physical adjoints and of the ten products, two
+// pencil buffers for ten components, and one step's stage record.
+struct AdjBufs {
+  void *lq = nullptr, *lh = nullptr;
+  void *t6 = nullptr, *t10 = nullptr;
+  void *pa = nullptr, *pb = nullptr;
+  void *rq[5] = {}, *rphys[5] = {}, *rphysw[5] = {};
+  double *rscal = nullptr;      // [5][8]
+  double *partials = nullptr;   // k_adj_pre's block sums
+  double *proj = nullptr;       // [1]
+  bool ready = false;
+};
 
 // The kernels of one (element type, grid size) pair, launched on the context's stream.
 struct Pipeline {
@@ -27,6 +42,13 @@ struct Pipeline {
   virtual void sample(Ctx &c) = 0;
   virtual void cell_keys(Ctx &c, int64_t *keys) = 0;
   virtual void tile_sort(Ctx &c) = 0;
+  // the adjoint step (one box): J's adjoint from the current state, the transposed closing
+  // projections, and one stage's transpose
+  virtual void adj_objective(Ctx &c) = 0;
+  virtual void adj_begin(Ctx &c) = 0;
+  virtual void adj_stage(Ctx &c, const AdjStage &as, int stage) = 0;
+  virtual size_t adj_pencil_a(const Ctx &c) const = 0;   // complex elements of the pencil buffers
+  virtual size_t adj_pencil_b(const Ctx &c) const = 0;
   virtual int tile_width() const = 0;  // kx columns per y block (the widest tile)
   virtual int z_blocks(const Ctx &c) const = 0;  // blocks of a z pass under the current schedule
 };
@@ -86,6 +108,7 @@ struct Ctx {
   // the captured step (hdns_step_graph), valid for one dt
   void *graph = nullptr, *graph_exec = nullptr;
   double graph_dt = 0.0;
+  AdjBufs adj;
   std::unique_ptr<Pipeline> pipe;
   std::string err;
 };

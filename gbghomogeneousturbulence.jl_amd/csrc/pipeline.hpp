@@ -3,8 +3,10 @@
 // Tile width TX: 16 complex FP32 or 8 complex FP64 = 128 contiguous bytes per z or y row of a
 // tile.  The column buffer of a y or z block is N*TX complex values (64 KiB at N = 512).
 #pragma once
+#include <algorithm>
 #include <type_traits>
 
+#include "adjoint.hpp"
 #include "context.hpp"
 #include "tracers.hpp"
 #include "xpass_wave.hpp"
@@ -79,9 +81,57 @@ struct PipelineImpl final : Pipeline {
     }
   }
 
+  static constexpr int ZNT_PLAIN = 256;   // threads of the plain z transforms of the adjoint
+  static constexpr int XT_ADJ = 256;
+
   PipelineImpl() {
     set_all<0>();
     set_all<1>();
+    set_lds((const void *)k_zinv_plain<T, N, TX, ZNT_PLAIN>, LDS_COL);
+    set_lds((const void *)k_zfwd_plain<T, N, TX, ZNT_PLAIN>, LDS_COL);
+    set_lds((const void *)k_xadj<T, N, XT_ADJ>, lds_x<XT_ADJ>());
+  }
+
+  // --- the adjoint step -------------------------------------------------------------------
+  size_t adj_pencil_a(const Ctx &c) const override {
+    return std::max((size_t)6 * N * K, (size_t)10 * N * N) * c.geo.kxp;   // Q (6) or PX (10)
+  }
+  size_t adj_pencil_b(const Ctx &c) const override {
+    return std::max((size_t)6 * N * N, (size_t)10 * N * K) * c.geo.kxp;   // R (6) or PF (10)
+  }
+  void adj_objective(Ctx &c) override {
+    hipLaunchKernelGGL((k_adj_objective<T>), dim3(ADJ_BLOCKS), dim3(256), 0, c.stream, (cx<T> *)c.adj.lq,
+                       (cx<T> *)c.adj.lh, (const cx<T> *)c.uh, (const cx<T> *)c.ph, c.geo, c.cfg.has_scalar);
+  }
+  void adj_begin(Ctx &c) override {   // the closing reality and Leray projections, transposed
+    cx<T> *lq = (cx<T> *)c.adj.lq;
+    hipLaunchKernelGGL((k_reality<T>), dim3((K * K + 255) / 256, 1), dim3(256), 0, c.stream, lq, lq + 3 * c.comp, c.geo,
+                       c.cfg.has_scalar ? 4 : 3);
+    hipLaunchKernelGGL((k_adj_leray<T>), dim3(ADJ_BLOCKS), dim3(256), 0, c.stream, lq, c.geo);
+  }
+  void adj_stage(Ctx &c, const AdjStage &as, int i) override {
+    AdjBufs &a = c.adj;
+    const cx<double> *tw = (const cx<double> *)c.tw;
+    hipLaunchKernelGGL((k_adj_pre<T>), dim3(ADJ_BLOCKS), dim3(256), 0, c.stream, (const cx<T> *)a.lq, (cx<T> *)a.lh,
+                       (const cx<T> *)a.rq[i], (cx<T> *)a.t6, c.geo, as, a.partials);
+    hipLaunchKernelGGL(k_adj_sum, dim3(1), dim3(256), 0, c.stream, (const double *)a.partials, ADJ_BLOCKS,
+                       1.0 / pow((double)N, 6.0), a.proj);
+    hipLaunchKernelGGL((k_zinv_plain<T, N, TX, ZNT_PLAIN>), dim3(tiles(c), K, 6), dim3(ZNT_PLAIN), LDS_COL, c.stream,
+                       (const cx<T> *)a.t6, (cx<T> *)a.pa, c.geo, tw, T(1));
+    Geo g6 = c.geo, g10 = c.geo;
+    g6.nci = 6;
+    g10.ncf = 10;
+    hipLaunchKernelGGL((k_yinv<T, N, TX, 256>), dim3(tiles(c), N, 6), dim3(256), LDS_COL, c.stream,
+                       (const cx<T> *)a.pa, (cx<T> *)a.pb, g6, tw);
+    hipLaunchKernelGGL((k_xadj<T, N, XT_ADJ>), dim3(N * N / (2 * xr<XT_ADJ>())), dim3(XT_ADJ), lds_x<XT_ADJ>(),
+                       c.stream, (const cx<T> *)a.pb, (const T *)a.rphys[i], (const T *)a.rphysw[i], (cx<T> *)a.pa,
+                       c.geo, tw, c.cfg.has_scalar);
+    hipLaunchKernelGGL((k_yfwd<T, N, TX, 256>), dim3(tiles(c), N, 10), dim3(256), LDS_COL, c.stream,
+                       (const cx<T> *)a.pa, (cx<T> *)a.pb, g10, tw);
+    hipLaunchKernelGGL((k_zfwd_plain<T, N, TX, ZNT_PLAIN>), dim3(tiles(c), K, 10), dim3(ZNT_PLAIN), LDS_COL, c.stream,
+                       (const cx<T> *)a.pb, (cx<T> *)a.t10, c.geo, tw, T(1.0 / ((double)N * N * N)));
+    hipLaunchKernelGGL((k_adj_post<T>), dim3(ADJ_BLOCKS), dim3(256), 0, c.stream, (cx<T> *)a.lq, (cx<T> *)a.lh,
+                       (const cx<T> *)a.rq[i], (const cx<T> *)a.t10, c.geo, as);
   }
 
   int tiles(const Ctx &c) const { return c.geo.kxp / TX; }

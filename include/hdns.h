@@ -173,6 +173,21 @@ hdns_status hdns_schedule(hdns_ctx *ctx, const int32_t *set, int32_t *get);
 hdns_status hdns_profile_enable(hdns_ctx *ctx, int32_t on);
 hdns_status hdns_profile_read(hdns_ctx *ctx, double *ms, int64_t *launches, int32_t reset);
 
+/* The adjoint of the step (no counterpart in the reference, which has no derivative: the north-star
+ * rrule of step!).  hdns_step_vjp applies the transpose of one step's derivative at the context's
+ * current state: lam_*_out (the adjoint of the step's output, full half-spectrum layout like
+ * hdns_state_download) becomes lam_*_in (that of its input); the state is unchanged.  Adjoints are
+ * gradients under the real inner product of the half spectrum with the Parseval weights (1 on
+ * kx = 0, 2 inside), which equals the full-spectrum product.  hdns_adjoint_gradient runs `steps`
+ * steps from the current state, keeping a checkpoint every checkpoint_every steps, and returns
+ * J = k + <phi^2> at the end and its gradient with respect to the initial u_hat (and phi_hat), by a
+ * reverse sweep that recomputes each segment from its checkpoint; the context is left at the
+ * initial state. */
+hdns_status hdns_step_vjp(hdns_ctx *ctx, double dt, const double *lam_u_out, const double *lam_phi_out,
+                          double *lam_u_in, double *lam_phi_in);
+hdns_status hdns_adjoint_gradient(hdns_ctx *ctx, double dt, int32_t steps, int32_t checkpoint_every,
+                                  double *objective, double *grad_u, double *grad_phi);
+
 /* Ensembles of boxes of uneven sizes and viscosities (no counterpart in the reference).  The
  * configs may differ in grid_points and viscosity only.  Boxes of one size form a batch context;
  * all batches share one stream, and hdns_ensemble_step_graph advances every box with one graph

@@ -26,6 +26,11 @@ product, s that of a flux):
            lam_u_z += -G mu_phi
 The stage recurrences (h1 = a h + dt N; q' = E (q + b h1); h' = E h1) transpose to
     lam_h1 = b E lam_q' + E lam_h';  lam_q = E lam_q' + dN^T (dt lam_h1);  lam_h = a lam_h1.
+
+The state lives in the retained modes (the 2/3 ball): a perturbation outside it is not a state.
+So the adjoints are restricted to the ball (the transposed products are masked).  Components
+outside the ball never flow back into it (every path into the products goes through the mask),
+so the restriction changes nothing inside.
 """
 from __future__ import annotations
 
@@ -124,8 +129,8 @@ def step_vjp(d: dns.Domain, cond: dns.Conditions, scheme: str, uh: np.ndarray, p
     rec, (a, b, dd) = _stage_record(d, cond, scheme, uh, ph, dt, dep)
     S = len(a)
     # the closing projections, transposed: u_out = R(P(u_S)), phi_out = R(phi_S)
-    lq_u = dns.leray(d, dns.reality_projection(d, lam_u))
-    lq_p = dns.reality_projection(d, lam_p) if lam_p is not None else None
+    lq_u = dns.leray(d, dns.reality_projection(d, lam_u)) * d.mask
+    lq_p = dns.reality_projection(d, lam_p) * d.mask if lam_p is not None else None
     lh_u = np.zeros_like(lq_u)
     lh_p = np.zeros_like(lq_p) if lq_p is not None else None
     for i in range(S - 1, -1, -1):
@@ -141,11 +146,11 @@ def step_vjp(d: dns.Domain, cond: dns.Conditions, scheme: str, uh: np.ndarray, p
         uh_i, ph_i, u, w, phi, amp, eb = rec[i]
         gu, gp = rhs_vjp(d, cond, uh_i, u, w, phi, amp, eb, dt * lh1_u,
                          dt * lh1_p if lh1_p is not None else None)
-        lq_u = new_u + gu
-        lh_u = a[i] * lh1_u
+        lq_u = (new_u + gu) * d.mask
+        lh_u = a[i] * lh1_u * d.mask
         if lq_p is not None:
-            lq_p = new_p + gp
-            lh_p = a[i] * lh1_p
+            lq_p = (new_p + gp) * d.mask
+            lh_p = a[i] * lh1_p * d.mask
     return lq_u, lq_p
 
 
