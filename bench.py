@@ -206,6 +206,77 @@ def time_ensemble(g, steps: int, H, boxes_per_size: int = 32, tracers: int = 409
             "tracers_per_box": tracers}
 
 
+def _parseval_weights(n: int) -> np.ndarray:
+    """1 on the kx = 0 and kx = N/2 planes, 2 inside: <a, b>_w = sum w Re(conj(a) b)."""
+    w = np.full(n // 2 + 1, 2.0)
+    w[0] = w[-1] = 1.0
+    return w
+
+
+def time_adjoint(g, n: int = 128, steps: int = 200, every: int = 20, directions: int = 16) -> dict:
+    """The adjoint workload: gradient of k(T) + <phi^2>(T) after `steps` forced IF-RK4 steps with
+    respect to u_hat(0), checkpoints every `every` steps, FP64 and FP32-bulk.  The FP64 gradient is
+    checked against fourth-order central differences of device runs along `directions` seeded
+    random-phase solenoidal directions (unit weighted norm, step 200 against |u_hat|_w ~ N^3)."""
+    dt = 2e-3
+    w = _parseval_weights(n)
+
+    def inner(a, b):
+        return float(np.sum(w * np.real(np.conj(a) * b)))
+
+    def run(dtype):
+        dom = g.FixedDomain(grid_points_per_axis=n, side_length=2 * math.pi, dealiasing=g.TwoThirds(),
+                            element_type=dtype)
+        cond = g.FlowConditions(viscosity=g.predicted_viscosity(dom, 0.1), schmidt=1.0,
+                                velocity_forcing=g.ConstantPower(dom, injection_rate=0.1, lowest_shell=1,
+                                                                 highest_shell=2))
+        u = g.random_phase_velocity(n, seed=2031)
+        p = np.zeros((n, n, n // 2 + 1), dtype=np.complex128)
+        return g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=1.0), u, p
+
+    out = {"grid": n, "steps": steps, "checkpoint_every": every, "scheme": "IFLowStorageRK4", "dt": dt,
+           "objective": "k(T) + <phi^2>(T)"}
+    grads = {}
+    for dtype, key in ((np.float64, "fp64"), (np.float32, "fp32_bulk")):
+        sim, u, p = run(dtype)
+        with sim:
+            sim.adjoint_gradient(dt, 2, 1)   # first-call allocations and code loads
+            t0 = time.perf_counter()
+            J, gu, _ = sim.adjoint_gradient(dt, steps, every)
+            tg = time.perf_counter() - t0
+            sim.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                sim.step_fields(dt, graph=True)
+            sim.synchronize()
+            tf = time.perf_counter() - t0
+            r = {"J": J, "gradient_s": round(tg, 4), "forward_ms_per_step": round(tf * 1e3 / steps, 4),
+                 "gradient_over_forward": round(tg / tf, 3)}
+            if key == "fp64" and directions:
+                def J_of(u0):
+                    sim.set_fields(g.ResolvedFields(u0, p))
+                    for _ in range(steps):
+                        sim.step_fields(dt, graph=True)
+                    red = sim.reductions()
+                    return red["kinetic_energy"] + red["scalar_variance"]
+
+                e, worst = 200.0, 0.0
+                for k in range(directions):
+                    v = g.random_phase_velocity(n, seed=9100 + k)
+                    v /= math.sqrt(inner(v, v))
+                    jp2, jp1, jm1, jm2 = (J_of(u + s_ * e * v) for s_ in (2, 1, -1, -2))
+                    fd = (-jp2 + 8 * jp1 - 8 * jm1 + jm2) / (12 * e)
+                    worst = max(worst, abs(inner(gu, v) - fd) / abs(fd))
+                r.update({"fd_directions": directions, "fd_max_rel": float(f"{worst:.3e}")})
+            grads[key] = (J, gu)
+            out[key] = r
+    J64, g64 = grads["fp64"]
+    J32, g32 = grads["fp32_bulk"]
+    out["fp32_bulk"]["J_rel_vs_fp64"] = float(f"{abs(J32 - J64) / J64:.3e}")
+    out["fp32_bulk"]["grad_rel_vs_fp64"] = float(f"{np.linalg.norm(g32 - g64) / np.linalg.norm(g64):.3e}")
+    return out
+
+
 def copy_calibration(H, nbytes: int = 1 << 30) -> None:
     """One device-to-device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
     H.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
@@ -276,6 +347,9 @@ def main() -> int:
     ap.add_argument("--no-tracers", action="store_true", help="skip the tracer throughput figure")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 96-box ensemble figure")
     ap.add_argument("--ensemble-steps", type=int, default=200)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the 128^3 adjoint figure")
+    ap.add_argument("--adjoint-directions", type=int, default=16,
+                    help="finite-difference directions for the adjoint check (0: time only)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy (the reference kernel for counter scaling)")
     args = ap.parse_args()
@@ -326,6 +400,11 @@ def main() -> int:
         print(f"bench: ensemble, 96 boxes (32^3, 48^3, 64^3), IF-RK3, {args.ensemble_steps} steps",
               file=sys.stderr, flush=True)
         ensemble = time_ensemble(g, args.ensemble_steps, H)
+    adjoint = None
+    if not args.no_adjoint:
+        print(f"bench: adjoint, 128^3, 200 IF-RK4 steps, checkpoints every 20, "
+              f"{args.adjoint_directions} FD directions", file=sys.stderr, flush=True)
+        adjoint = time_adjoint(g, directions=args.adjoint_directions)
     if args.copy_calibration:
         copy_calibration(H)
     cpu = None
@@ -347,6 +426,7 @@ def main() -> int:
         "autotuned_schedule": r32["schedule"],
         "tracers": tracers,
         "ensemble": ensemble,
+        "adjoint": adjoint,
     }
     print(json.dumps(line), flush=True)
     return 0

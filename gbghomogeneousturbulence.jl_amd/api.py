@@ -218,6 +218,23 @@ def _check_reality(f: np.ndarray, name: str, dtype) -> None:
 
 # --- the run -------------------------------------------------------------------------------
 
+def _state_arrays(state: ResolvedFields, domain: FixedDomain):
+    """The state's spectra as contiguous complex128 arrays, shape- and reality-checked
+    (src/SimulationRun.jl:786-787,1236-1248)."""
+    n = domain.grid_points_per_axis
+    u = np.ascontiguousarray(state.velocity_spectral, dtype=np.complex128)
+    if u.shape != (3, n, n, n // 2 + 1):
+        raise ValueError(f"velocity_spectral must have shape {(3, n, n, n // 2 + 1)}, got {u.shape}")
+    _check_reality(u, "velocity", domain.element_type)
+    p = None
+    if state.scalar_spectral is not None:
+        p = np.ascontiguousarray(state.scalar_spectral, dtype=np.complex128)
+        if p.shape != (n, n, n // 2 + 1):
+            raise ValueError(f"scalar_spectral must have shape {(n, n, n // 2 + 1)}, got {p.shape}")
+        _check_reality(p, "scalar", domain.element_type)
+    return u, p
+
+
 def _config(domain: FixedDomain, conditions: FlowConditions, scheme, carries_scalar: bool, mean_gradient: float,
             tracers, tracer_capacity, keep_physical: bool, keep_vorticity: bool) -> L.HdnsConfig:
     """The C-ABI configuration of one box."""
@@ -263,18 +280,8 @@ class Simulation:
                  particle_integration_scheme=None, mean_gradient: float = 0.0, cfl: float | None = None,
                  max_dt: float = math.inf, keep_vorticity: bool = False, keep_physical: bool = True,
                  tracer_capacity: int | None = None):
-        n = domain.grid_points_per_axis
         carries_scalar = state.scalar_spectral is not None
-        u = np.ascontiguousarray(state.velocity_spectral, dtype=np.complex128)
-        if u.shape != (3, n, n, n // 2 + 1):
-            raise ValueError(f"velocity_spectral must have shape {(3, n, n, n // 2 + 1)}, got {u.shape}")
-        _check_reality(u, "velocity", domain.element_type)
-        p = None
-        if carries_scalar:
-            p = np.ascontiguousarray(state.scalar_spectral, dtype=np.complex128)
-            if p.shape != (n, n, n // 2 + 1):
-                raise ValueError(f"scalar_spectral must have shape {(n, n, n // 2 + 1)}, got {p.shape}")
-            _check_reality(p, "scalar", domain.element_type)
+        u, p = _state_arrays(state, domain)
         if mean_gradient and not carries_scalar:
             raise ValueError("a mean scalar gradient needs a scalar in the state")
         self.domain = domain
@@ -451,6 +458,16 @@ class Simulation:
         return ms, cnt
 
     # the hot path
+    def set_fields(self, state: ResolvedFields) -> None:
+        """Adopt a new field state and deposit it, keeping the context (the adoption door of
+        src/SimulationRun.jl:786-787 without rebuilding the run).  Tracers keep their positions;
+        the clock and the trail are the caller's to reset."""
+        if (state.scalar_spectral is not None) != self.carries_scalar:
+            raise ValueError("the new state must carry a scalar exactly when the run does")
+        u, p = _state_arrays(state, self.domain)
+        self._call("hdns_state_upload", L.dptr(u), L.dptr(p) if p is not None else None)
+        self.fields_epoch += 1
+
     def step_fields(self, dt: float, graph: bool = False) -> None:
         """``step!`` then the field ``deposit!`` (src/Propagator.jl:304-419,808-842), asynchronously;
         ``graph`` replays the step as one captured HIP graph."""
@@ -642,15 +659,10 @@ class Ensemble:
         self.boxes = [EnsembleBox(self, b, mb, field_integration_scheme) for b, mb in enumerate(members)]
         try:
             for box, mb in zip(self.boxes, members):
-                n = mb.domain.grid_points_per_axis
-                u = np.ascontiguousarray(mb.state.velocity_spectral, dtype=np.complex128)
-                if u.shape != (3, n, n, n // 2 + 1):
-                    raise ValueError(f"box {box.index}: velocity_spectral must have shape {(3, n, n, n // 2 + 1)}")
-                _check_reality(u, "velocity", mb.domain.element_type)
-                p = None
-                if mb.state.scalar_spectral is not None:
-                    p = np.ascontiguousarray(mb.state.scalar_spectral, dtype=np.complex128)
-                    _check_reality(p, "scalar", mb.domain.element_type)
+                try:
+                    u, p = _state_arrays(mb.state, mb.domain)
+                except ValueError as e:
+                    raise ValueError(f"box {box.index}: {e}") from None
                 box._call("hdns_state_upload", L.dptr(u), L.dptr(p) if p is not None else None)
                 if mb.tracers is not None:
                     box._call("hdns_tracers_upload", mb.tracers.count, L.dptr(mb.tracers.positions))

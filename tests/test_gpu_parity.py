@@ -314,75 +314,3 @@ def test_ensemble_box_at_48_matches_the_fp64_oracle(g):
         got = ens.box(1).state()
         assert rel(got.velocity_spectral, uo) < 1e-12
         assert rel(got.scalar_spectral, po) < 1e-12
-
-
-# --- the adjoint -------------------------------------------------------------------------------
-
-from oracle import adjoint as oadj   # noqa: E402
-
-
-def _adjoint_case(g, n, dtype, forcing=0.1, seed=31):
-    sim, u, p, oc = make_sim(g, n, dtype, forcing=forcing, nu=0.02, seed=seed)
-    return sim, u, p, oc, dns.Domain(n)
-
-
-@pytest.mark.parametrize("scheme", ["rk3", "rk4"])
-def test_step_vjp_matches_the_oracle_adjoint(g, scheme):
-    n, dt = 32, 2e-3
-    sch = g.IFLowStorageRK4 if scheme == "rk4" else g.IFLowStorageRK3
-    sim, u, p, oc = make_sim(g, n, np.float64, scheme=sch, forcing=0.1, nu=0.02, seed=41)
-    d = dns.Domain(n)
-    lu, lp = oadj.solenoidal_direction(d, 1), dns.rfft3(np.random.default_rng(2).standard_normal((n, n, n))) * d.mask
-    with sim:
-        before = sim.state().velocity_spectral
-        gu, gp = sim.step_vjp(dt, lu, lp)
-        assert np.array_equal(sim.state().velocity_spectral, before)   # the state is left as it was
-        ou, op = oadj.step_vjp(d, oc, scheme, u, p, dt, lu, lp)
-        eu, ep = rel(gu, ou), rel(gp, op)
-        print(f"{scheme}: vjp rel err u {eu:.2e} phi {ep:.2e}", flush=True)
-        assert eu < 1e-11 and ep < 1e-11
-
-
-def test_adjoint_gradient_matches_the_oracle_and_finite_differences(g):
-    """The checkpointed device gradient of J = k + <phi^2> after 12 forced IF-RK4 steps: equal to the
-    oracle's to 1e-10, and to fourth-order central differences of device runs along 4 random
-    solenoidal directions to 1e-6."""
-    n, dt, steps, every = 32, 2e-3, 12, 5
-    sim, u, p, oc = make_sim(g, n, np.float64, forcing=0.1, nu=0.02, seed=43)
-    d = dns.Domain(n)
-    with sim:
-        J, gu, gp = sim.adjoint_gradient(dt, steps, every)
-        Jo, ou, op = oadj.gradient(d, oc, "rk4", u, p, dt, steps, every)
-        assert J == pytest.approx(Jo, rel=1e-12)
-        print(f"adjoint gradient vs oracle: u {rel(gu, ou):.2e} phi {rel(gp, op):.2e}", flush=True)
-        assert rel(gu, ou) < 1e-10 and rel(gp, op) < 1e-10
-    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
-    cond = g.FlowConditions(viscosity=0.02, schmidt=1.0,
-                            velocity_forcing=g.ConstantPower(dom, injection_rate=0.1, lowest_shell=1, highest_shell=2))
-
-    def J_of(u0):
-        with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u0, p), mean_gradient=1.0) as s:
-            for _ in range(steps):
-                s.step_fields(dt)
-            r = s.reductions()
-            return r["kinetic_energy"] + r["scalar_variance"]
-
-    for k in range(4):
-        du = oadj.solenoidal_direction(d, 60 + k)
-        e = 0.1   # |delta| = 1 against |u_hat| ~ N^3: the linear regime
-        fd = (-J_of(u + 2 * e * du) + 8 * J_of(u + e * du) - 8 * J_of(u - e * du) + J_of(u - 2 * e * du)) / (12 * e)
-        an = oadj.inner(d, gu, du)
-        print(f"direction {k}: adjoint {an:.12e} fd {fd:.12e} rel {abs(an - fd) / abs(fd):.2e}", flush=True)
-        assert abs(an - fd) <= 1e-6 * abs(fd)
-
-
-def test_fp32_bulk_adjoint_gradient_against_the_fp64_oracle(g):
-    n, dt, steps = 32, 2e-3, 6
-    sim, u, p, oc = make_sim(g, n, np.float32, forcing=0.1, nu=0.02, seed=47)
-    d = dns.Domain(n)
-    with sim:
-        J, gu, gp = sim.adjoint_gradient(dt, steps, 3)
-    Jo, ou, op = oadj.gradient(d, oc, "rk4", u, p, dt, steps, 3)
-    print(f"FP32-bulk adjoint: J {abs(J - Jo) / Jo:.2e}, grad u {rel(gu, ou):.2e}, phi {rel(gp, op):.2e}", flush=True)
-    assert J == pytest.approx(Jo, rel=1e-5)
-    assert rel(gu, ou) < 1e-4 and rel(gp, op) < 1e-4
