@@ -353,6 +353,21 @@ class Simulation:
         keys = ("kinetic_energy", "dissipation", "scalar_variance", "scalar_dissipation", "band_energy")
         return dict(zip(keys, map(float, out)))
 
+    def distance(self, other: "Simulation") -> dict:
+        """RMS difference of this run's state from `other`'s (same grid and device; the element
+        types may differ), over the grid points, with `other`'s RMS alongside and their ratios:
+        the FP32-bulk error report against an FP64 run of the same flow."""
+        if self._ctx is None or other._ctx is None:
+            raise ValueError("distance compares two open standalone runs (not ensemble boxes)")
+        out = np.zeros(4)
+        L.check(L.lib().hdns_state_distance(self._ctx, other._ctx, L.dptr(out)), self._ctx)
+        r = {"velocity_rms_difference": out[0], "velocity_rms": out[1],
+             "velocity_relative": out[0] / out[1] if out[1] > 0 else math.nan}
+        if self.carries_scalar:
+            r.update({"scalar_rms_difference": out[2], "scalar_rms": out[3],
+                      "scalar_relative": out[2] / out[3] if out[3] > 0 else math.nan})
+        return r
+
     def energy_spectrum(self, which: int = L.FIELD_VELOCITY):
         """(shell centres, E(kappa)) of the state (src/Observables.jl:502-514)."""
         n = self.domain.grid_points_per_axis

@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "compare.hpp"
 #include "context.hpp"
 
 using namespace hdns;
@@ -766,6 +767,76 @@ hdns_status hdns_reductions(hdns_ctx *ctx, double *out) {
   out[2] = c.cfg.has_scalar ? s[4] : 0.0;
   out[3] = c.cfg.has_scalar ? 2.0 * kappa * s[5] : 0.0;
   out[4] = 0.5 * s[2];
+  return HDNS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// sum over the velocity (ncomp = 3) or the scalar (ncomp = 1) of a's and b's selected boxes
+template <typename TA, typename TB>
+void launch_distance(const Ctx &ca, const Ctx &cb, bool scalar, double *partials) {
+  const size_t comp_a = ca.comp, comp_b = cb.comp;
+  const cx<TA> *a = scalar ? (const cx<TA> *)ca.ph + ca.box * comp_a : (const cx<TA> *)ca.uh + ca.box * 3 * comp_a;
+  const cx<TB> *b = scalar ? (const cx<TB> *)cb.ph + cb.box * comp_b : (const cx<TB> *)cb.uh + cb.box * 3 * comp_b;
+  hipLaunchKernelGGL((k_state_distance<TA, TB>), dim3(DIST_BLOCKS), dim3(256), 0, ca.stream, a, comp_a, ca.geo.kxp, b,
+                     comp_b, cb.geo.kxp, ca.geo.k, ca.geo.m, scalar ? 1 : 3, partials);
+}
+
+double neumaier(const double *v, size_t n) {
+  double s = 0.0, comp = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    const double t = s + v[i];
+    comp += std::fabs(s) >= std::fabs(v[i]) ? (s - t) + v[i] : (v[i] - t) + s;
+    s = t;
+  }
+  return s + comp;
+}
+
+}  // namespace
+
+extern "C" {
+
+hdns_status hdns_state_distance(hdns_ctx *a, hdns_ctx *b, double *out) {
+  if (!a || !b || !out) return fail(a, HDNS_ERR_INVALID_ARGUMENT, "a, b and out must not be NULL");
+  Ctx &ca = a->c, &cb = b->c;
+  if (ca.geo.n != cb.geo.n) return fail(a, HDNS_ERR_INVALID_ARGUMENT, "the two contexts must have the same grid");
+  if (ca.cfg.device != cb.cfg.device) return fail(a, HDNS_ERR_INVALID_ARGUMENT, "the two contexts must share a device");
+  if (ca.cfg.has_scalar != cb.cfg.has_scalar)
+    return fail(a, HDNS_ERR_INVALID_ARGUMENT, "both contexts must carry a scalar, or neither");
+  HIP_TRY(a, hipSetDevice(ca.cfg.device));
+  HIP_TRY(a, hipStreamSynchronize(cb.stream));   // b's last step has landed
+  double *partials = nullptr;
+  const size_t np = 3 * 2 * DIST_BLOCKS;
+  HIP_TRY(a, hipMalloc((void **)&partials, 2 * np * sizeof(double)));
+  std::vector<double> host(2 * np, 0.0);
+  const int ncomp[2] = {3, ca.cfg.has_scalar ? 1 : 0};
+  for (int f = 0; f < 2; ++f) {
+    if (!ncomp[f]) continue;
+    double *p = partials + f * np;
+    const bool sc = f == 1;
+    if (ca.fp64 && cb.fp64) launch_distance<double, double>(ca, cb, sc, p);
+    else if (ca.fp64) launch_distance<double, float>(ca, cb, sc, p);
+    else if (cb.fp64) launch_distance<float, double>(ca, cb, sc, p);
+    else launch_distance<float, float>(ca, cb, sc, p);
+  }
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(host.data(), partials, host.size() * sizeof(double), hipMemcpyDeviceToHost, ca.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ca.stream);
+  (void)hipFree(partials);
+  if (e != hipSuccess) return fail(a, HDNS_ERR_DEVICE, std::string("state distance: ") + hipGetErrorString(e));
+  // <f, f>_w / N^6 is the mean of |f|^2 over the grid points
+  const double norm = pow((double)ca.geo.n, 6.0);
+  for (int f = 0; f < 2; ++f) {
+    double dd = 0.0, bb = 0.0;
+    for (int c = 0; c < ncomp[f]; ++c) {
+      dd += neumaier(host.data() + f * np + (size_t)(2 * c) * DIST_BLOCKS, DIST_BLOCKS);
+      bb += neumaier(host.data() + f * np + (size_t)(2 * c + 1) * DIST_BLOCKS, DIST_BLOCKS);
+    }
+    out[2 * f] = sqrt(dd / norm);
+    out[2 * f + 1] = sqrt(bb / norm);
+  }
   return HDNS_OK;
 }
 

@@ -406,9 +406,13 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
     }
   }
 
+  // buffer stores need every offset of a Q component below 2^31 bytes (kxp is at most M+1
+  // rounded up to 16); beyond that (FP64 at 1024^3) the stores take 64-bit addresses
+  constexpr bool BUFQ = (size_t)N * K * ((M + 16) / 16 * 16) * sizeof(cx<T>) < 0x7fff0000u;
 #pragma unroll
   for (int c = 0; c < NCIN; ++c) {
-    const Buf<T> dst(q + (size_t)c * pencil + (size_t)ky_idx * g.kxp + kx0, pencil);
+    cx<T> *qc = q + (size_t)c * pencil + (size_t)ky_idx * g.kxp + kx0;
+    const Buf<T> dst(qc, pencil);
     fft_io<T, N, TX, NT, +1, true, false>(
         buf, twl,
         [&](int pp, int qq, int n, int col) {
@@ -428,7 +432,10 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
           }
           return scale(v, inv);
         },
-        [&](int, int, int n, int col, cx<T> v) { dst.store(v, (uint32_t)(n * (int)zstride + col), 0u); });
+        [&](int, int, int n, int col, cx<T> v) {
+          if constexpr (BUFQ) dst.store(v, (uint32_t)(n * (int)zstride + col), 0u);
+          else qc[(size_t)n * zstride + col] = v;
+        });
   }
 
   // deterministic block reduction: wave shuffles, then the waves in order

@@ -47,6 +47,16 @@ def _wn(n: int) -> np.ndarray:
     return np.where(i <= n // 2, i, i - n)
 
 
+def _ball_modes(n: int):
+    """Signed indices (mx, my, mz) and storage indices (iz, iy, ix) of the nonzero modes inside the
+    2/3 ball, in C order of the [N, N, N/2+1] layout, without materialising full index grids."""
+    mys = _wn(n)
+    m2 = (mys[:, None, None] ** 2 + mys[None, :, None] ** 2).astype(np.int32) + \
+        (np.arange(n // 2 + 1, dtype=np.int32) ** 2)[None, None, :]
+    iz, iy, ix = np.nonzero((9 * m2 <= n * n) & (m2 > 0))
+    return ix, mys[iy], mys[iz], (iz, iy, ix)
+
+
 def parseval_energy(u_hat: np.ndarray, n: int) -> float:
     """1/2 <u, u> of a spectral vector in the reference layout [3, N, N, N/2+1]."""
     w = np.full(n // 2 + 1, 2.0)
@@ -74,17 +84,12 @@ def random_phase_velocity(n: int, side_length: float = 2 * math.pi, energy: floa
     [3, N, N, N/2+1] (unnormalised forward-transform convention).
     """
     k0 = 2 * math.pi / side_length
-    mxs = np.arange(n // 2 + 1)
-    mys = _wn(n)
-    mzs = _wn(n)
-    MZ, MY, MX = np.meshgrid(mzs, mys, mxs, indexing="ij")
-    m2 = MX * MX + MY * MY + MZ * MZ
-    keep = (9 * m2 <= n * n) & (m2 > 0)
-    kx, ky, kz = (k0 * MX[keep], k0 * MY[keep], k0 * MZ[keep])
+    mx, my, mz, keep = _ball_modes(n)
+    kx, ky, kz = (k0 * mx, k0 * my, k0 * mz)
     kk = np.sqrt(kx * kx + ky * ky + kz * kz)
     kp = peak * k0
     amp = np.sqrt(kk ** 2 * np.exp(-2.0 * (kk / kp) ** 2))          # sqrt(E/(4 pi k^2)) up to a constant
-    ctr = _mode_counter(MX[keep], MY[keep], MZ[keep])
+    ctr = _mode_counter(mx, my, mz)
     t1 = 2 * math.pi * counter_uniform(seed, 0, ctr)
     t2 = 2 * math.pi * counter_uniform(seed, 1, ctr)
     g = 2 * math.pi * counter_uniform(seed, 2, ctr)
@@ -110,12 +115,10 @@ def random_phase_scalar(n: int, side_length: float = 2 * math.pi, variance: floa
                         peak: float = 4.0, seed: int = 2027) -> np.ndarray:
     """Random-phase scalar with the same spectral shape, zero mean, inside the 2/3 ball, <phi^2> = variance."""
     k0 = 2 * math.pi / side_length
-    MZ, MY, MX = np.meshgrid(_wn(n), _wn(n), np.arange(n // 2 + 1), indexing="ij")
-    m2 = MX * MX + MY * MY + MZ * MZ
-    keep = (9 * m2 <= n * n) & (m2 > 0)
-    kk = k0 * np.sqrt(m2[keep].astype(np.float64))
+    mx, my, mz, keep = _ball_modes(n)
+    kk = k0 * np.sqrt((mx * mx + my * my + mz * mz).astype(np.float64))
     amp = np.sqrt(kk ** 2 * np.exp(-2.0 * (kk / (peak * k0)) ** 2))
-    t = 2 * math.pi * counter_uniform(seed, 3, _mode_counter(MX[keep], MY[keep], MZ[keep]))
+    t = 2 * math.pi * counter_uniform(seed, 3, _mode_counter(mx, my, mz))
     p = np.zeros((n, n, n // 2 + 1), dtype=np.complex128)
     p[keep] = amp * np.exp(1j * t)
     hermitian_kx0(p, n)

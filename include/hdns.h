@@ -113,6 +113,13 @@ hdns_status hdns_max_speed(hdns_ctx *ctx, double *out);
 /* out[0..4] = k, eps, <phi^2>, eps_phi, band energy of the last deposit's state, in FP64 from
  * compensated device reductions (src/Observables.jl:84,155-158,345-400; src/Forcing.jl:1109). */
 hdns_status hdns_reductions(hdns_ctx *ctx, double *out);
+/* The distance between the states of two contexts on one grid and device (their selected boxes;
+ * the element types may differ): out[0] = rms |u_a - u_b|, out[1] = rms |u_b|, out[2] =
+ * rms |phi_a - phi_b|, out[3] = rms |phi_b| (0 without a scalar), over the grid points, in FP64
+ * (compensated sums).  No reference counterpart: it is the FP32-bulk error report against an FP64
+ * run of the same flow, which the reference's tests form on host arrays between two runs
+ * (test/box_scaling.jl:18,36).  Waits for b's stream and runs on a's. */
+hdns_status hdns_state_distance(hdns_ctx *a, hdns_ctx *b, double *out);
 /* Shell spectrum E(kappa_n), n = 0..nshells-1 (src/Observables.jl:502-514); which = velocity or
  * scalar.  nshells must be >= floor(sqrt(3)*N/2)+2. */
 hdns_status hdns_spectrum(hdns_ctx *ctx, int32_t which, double *out, int32_t nshells);

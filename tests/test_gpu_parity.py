@@ -5,6 +5,7 @@ the FP32 transform floor (about 1e-6 relative per step), and the velocity within
 of the FP64 oracle after 10 IF-RK4 steps.
 """
 import math
+import time
 
 import numpy as np
 import pytest
@@ -314,3 +315,73 @@ def test_ensemble_box_at_48_matches_the_fp64_oracle(g):
         got = ens.box(1).state()
         assert rel(got.velocity_spectral, uo) < 1e-12
         assert rel(got.scalar_spectral, po) < 1e-12
+
+
+def test_state_distance_matches_host_norms(g):
+    """hdns_state_distance: the RMS difference of two runs' states and the second's RMS, against the
+    same norms of the downloaded spectra (Parseval weights over the half spectrum)."""
+    n = 32
+    a, *_ = make_sim(g, n, np.float32, forcing=0.1)
+    b, *_ = make_sim(g, n, np.float64, forcing=0.1)
+    w = np.full(n // 2 + 1, 2.0)
+    w[0] = w[-1] = 1.0
+
+    def rms(f):
+        return math.sqrt(float(np.sum(w * np.abs(f) ** 2)) / float(n) ** 6)
+
+    with a, b:
+        for _ in range(3):
+            a.advance(2e-3)
+            b.advance(2e-3)
+        r = a.distance(b)
+        sa, sb = a.state(), b.state()
+        assert r["velocity_rms"] == pytest.approx(rms(sb.velocity_spectral), rel=1e-12)
+        assert r["velocity_rms_difference"] == pytest.approx(rms(sa.velocity_spectral - sb.velocity_spectral), rel=1e-9)
+        assert r["scalar_rms"] == pytest.approx(rms(sb.scalar_spectral), rel=1e-12)
+        assert r["scalar_rms_difference"] == pytest.approx(rms(sa.scalar_spectral - sb.scalar_spectral), rel=1e-9)
+        assert 0 < r["velocity_relative"] < 1e-5
+        assert b.distance(b)["velocity_rms_difference"] == 0.0
+
+
+def _spectrum_error(a, b, which):
+    _, ea = a.energy_spectrum(which)
+    _, eb = b.energy_spectrum(which)
+    live = eb > 1e-12 * eb.max()
+    return float(np.max(np.abs(ea[live] - eb[live]) / eb[live]))
+
+
+@pytest.mark.timeout(900)
+def test_fp32_bulk_against_fp64_at_1024_over_10_steps(g):
+    """1024^3 forced isotropic turbulence with a mean-gradient scalar (zero initial fluctuation),
+    IF-RK4: the FP32-bulk run against an FP64 run of the same flow on the same GPU, per step and per
+    field (state distance), per reduction and per spectrum.  Bar: velocity and scalar within 1e-5
+    relative after 10 steps."""
+    n, dt, steps = 1024, 5e-4, 10
+    t0 = time.perf_counter()
+    u = g.random_phase_velocity(n, seed=2026)
+    p = np.zeros((n, n, n // 2 + 1), dtype=np.complex128)
+    print(f"n=1024: seeded in {time.perf_counter() - t0:.1f} s", flush=True)
+    runs = []
+    for dtype in (np.float32, np.float64):
+        dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
+        cond = g.FlowConditions(viscosity=g.predicted_viscosity(dom, 0.1), schmidt=1.0,
+                                velocity_forcing=g.ConstantPower(dom, injection_rate=0.1, lowest_shell=1,
+                                                                 highest_shell=2))
+        runs.append(g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=1.0,
+                                 keep_physical=False))
+        print(f"n=1024 {np.dtype(dtype).name}: context {runs[-1].memory_bytes() / 2 ** 30:.1f} GiB, "
+              f"adopted at {time.perf_counter() - t0:.1f} s", flush=True)
+    del u, p
+    a, b = runs
+    with a, b:
+        keys = ("kinetic_energy", "dissipation", "scalar_variance", "scalar_dissipation")
+        for s in range(1, steps + 1):
+            a.advance(dt)
+            b.advance(dt)
+            r = a.distance(b)
+            ra, rb = a.reductions(), b.reductions()
+            red = " ".join(f"{k} {abs(ra[k] - rb[k]) / abs(rb[k]):.1e}" for k in keys)
+            print(f"step {s:2d}: u {r['velocity_relative']:.2e} phi {r['scalar_relative']:.2e} | {red} | "
+                  f"E(k) {_spectrum_error(a, b, 0):.1e} E_phi(k) {_spectrum_error(a, b, 1):.1e}", flush=True)
+        assert r["velocity_relative"] <= 1e-5
+        assert r["scalar_relative"] <= 1e-5
