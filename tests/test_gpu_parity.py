@@ -385,3 +385,41 @@ def test_fp32_bulk_against_fp64_at_1024_over_10_steps(g):
                   f"E(k) {_spectrum_error(a, b, 0):.1e} E_phi(k) {_spectrum_error(a, b, 1):.1e}", flush=True)
         assert r["velocity_relative"] <= 1e-5
         assert r["scalar_relative"] <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["dns16_rk4_forced", "dns16_rk3_unforced"])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 1e-5)])
+def test_device_reproduces_the_golden_fixtures(g, name, dtype, tol):
+    """The committed oracle fixtures (tests/golden): states after 1 and 10 steps, reductions, max|u|,
+    and for the forced case the tracers after 3 Heun steps."""
+    from test_golden import GOLDEN, load_case
+
+    d, cond, scheme, dt, z, unpack = load_case(name)
+    n = d.n
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
+    vf = g.ConstantPower(dom, injection_rate=cond.forcing[0], lowest_shell=cond.forcing[1],
+                         highest_shell=cond.forcing[2]) if cond.forcing else g.NoForcing()
+    fc = g.FlowConditions(viscosity=cond.viscosity, schmidt=cond.schmidt, velocity_forcing=vf)
+    t = np.load(GOLDEN / "tracers16_rk4_forced.npz") if name == "dns16_rk4_forced" else None
+    tracers = g.Tracers(dom, positions=t["X0"].T.copy()) if t is not None else None
+    sch = g.IFLowStorageRK4 if scheme == "rk4" else g.IFLowStorageRK3
+    with g.Simulation(dom, conditions=fc, state=g.ResolvedFields(unpack("u0"), unpack("p0")),
+                      mean_gradient=cond.mean_gradient, field_integration_scheme=sch, tracers=tracers) as sim:
+        if t is not None:
+            v, s = sim.tracer_samples()
+            assert rel(v, t["v0"].T) < tol and rel(s, t["s0"]) < tol
+        for step in range(1, 11):
+            sim.advance(dt)
+            if t is not None and step == 3:
+                assert np.max(np.abs(sim.tracer_positions() - t["X3"].T)) < 10 * tol
+                v, s = sim.tracer_samples()
+                assert rel(v, t["v3"].T) < 10 * tol and rel(s, t["s3"]) < 10 * tol
+            if step in (1, 10):
+                st = sim.state()
+                eu, ep = rel(st.velocity_spectral, unpack(f"u{step}")), rel(st.scalar_spectral, unpack(f"p{step}"))
+                print(f"{name} {np.dtype(dtype).name} step {step}: u {eu:.2e} phi {ep:.2e}", flush=True)
+                assert eu < tol and ep < tol
+                r = sim.reductions()
+                got = [r["kinetic_energy"], r["dissipation"], r["scalar_variance"], r["scalar_dissipation"]]
+                np.testing.assert_allclose(got, z[f"reductions{step}"], rtol=10 * tol)
+                assert sim.max_speed() == pytest.approx(float(z[f"max_speed{step}"]), rel=10 * tol)
