@@ -206,6 +206,69 @@ def time_ensemble(g, steps: int, H, boxes_per_size: int = 32, tracers: int = 409
             "tracers_per_box": tracers}
 
 
+def _forced_flow(g, n: int, dtype, tracers: int = 0):
+    """The configurations' flow: random-phase k^4 exp(-2(k/4)^2) velocity (k = 0.5, seed 2026),
+    ConstantPower eps = 0.1 on shells 1-2, nu for k_max eta = 1.5, a scalar with mean gradient
+    G = 1 along z and zero initial fluctuation, IF-RK4; `tracers` seeded uniform tracers."""
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=2 * math.pi, dealiasing=g.TwoThirds(), element_type=dtype)
+    cond = g.FlowConditions(viscosity=g.predicted_viscosity(dom, 0.1), schmidt=1.0,
+                            velocity_forcing=g.ConstantPower(dom, injection_rate=0.1, lowest_shell=1, highest_shell=2))
+    u = g.random_phase_velocity(n, seed=2026)
+    p = np.zeros((n, n, n // 2 + 1), dtype=np.complex128)
+    tr = g.Tracers(dom, number_of_particles=tracers, seed=2028) if tracers else None
+    return g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=1.0,
+                        field_integration_scheme=g.IFLowStorageRK4, tracers=tr, keep_physical=tracers > 0)
+
+
+def time_iterations(g, n: int, m: int, dtype, steps: int, dt: float) -> dict:
+    """Whole iterations of the run loop (advance!: step + deposit, traversal rebuild, Heun push and
+    sample of M tracers, the blow-up guards' readbacks), wall clock, after 2 untimed ones."""
+    t0 = time.perf_counter()
+    with _forced_flow(g, n, dtype, tracers=m) as sim:
+        setup = time.perf_counter() - t0
+        for _ in range(2):
+            sim.advance(dt)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.advance(dt)
+        sim.synchronize()
+        wall = time.perf_counter() - t0
+        mem = sim.memory_bytes()
+    per = wall / steps
+    return {"grid": n, "tracers": m, "dtype": np.dtype(dtype).name, "scheme": "IFLowStorageRK4", "dt": dt,
+            "iterations": steps, "ms_per_iteration": round(per * 1e3, 3),
+            "gridpoint_steps_per_s": round(n ** 3 / per, 1), "tracer_steps_per_s": round(m / per, 1),
+            "memory_GiB": round(mem / 2 ** 30, 2), "setup_s": round(setup, 1)}
+
+
+def accuracy_report(g, n: int, steps: int, dt: float) -> dict:
+    """FP32-bulk against a plain FP64 run of the same flow on the same GPU, per step: relative RMS
+    distance of u and phi, relative error of k, eps, <phi^2>, eps_phi, and the largest relative
+    error over the live shells of E(k) and E_phi(k)."""
+    keys = ("kinetic_energy", "dissipation", "scalar_variance", "scalar_dissipation")
+    out = {"grid": n, "steps": steps, "dt": dt, "per_step": []}
+
+    def spectrum_error(a, b, which):
+        _, ea = a.energy_spectrum(which)
+        _, eb = b.energy_spectrum(which)
+        live = eb > 1e-12 * eb.max()
+        return float(np.max(np.abs(ea[live] - eb[live]) / eb[live]))
+
+    with _forced_flow(g, n, np.float32) as a, _forced_flow(g, n, np.float64) as b:
+        for s in range(1, steps + 1):
+            a.advance(dt)
+            b.advance(dt)
+            d = a.distance(b)
+            ra, rb = a.reductions(), b.reductions()
+            row = {"step": s, "u": float(f"{d['velocity_relative']:.3e}"), "phi": float(f"{d['scalar_relative']:.3e}")}
+            row.update({k: float(f"{abs(ra[k] - rb[k]) / abs(rb[k]):.3e}") for k in keys})
+            row["E_u"] = float(f"{spectrum_error(a, b, 0):.3e}")
+            row["E_phi"] = float(f"{spectrum_error(a, b, 1):.3e}")
+            out["per_step"].append(row)
+    return out
+
+
 def _parseval_weights(n: int) -> np.ndarray:
     """1 on the kx = 0 and kx = N/2 planes, 2 inside: <a, b>_w = sum w Re(conj(a) b)."""
     w = np.full(n // 2 + 1, 2.0)
@@ -348,6 +411,9 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 96-box ensemble figure")
     ap.add_argument("--ensemble-steps", type=int, default=200)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the 128^3 adjoint figure")
+    ap.add_argument("--no-iterations", action="store_true",
+                    help="skip the whole-iteration figures (256^3 FP64 + 1M tracers, 1024^3 FP32-bulk + 16M tracers)")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP32-bulk vs FP64 error report")
     ap.add_argument("--adjoint-directions", type=int, default=16,
                     help="finite-difference directions for the adjoint check (0: time only)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -400,6 +466,16 @@ def main() -> int:
         print(f"bench: ensemble, 96 boxes (32^3, 48^3, 64^3), IF-RK3, {args.ensemble_steps} steps",
               file=sys.stderr, flush=True)
         ensemble = time_ensemble(g, args.ensemble_steps, H)
+    iterations = None
+    if not args.no_iterations:
+        print("bench: whole iterations, 256^3 FP64 + 2^20 tracers and 1024^3 FP32-bulk + 2^24 tracers",
+              file=sys.stderr, flush=True)
+        iterations = [time_iterations(g, 256, 1 << 20, np.float64, 20, 2e-3),
+                      time_iterations(g, 1024, 1 << 24, np.float32, 10, 5e-4)]
+    accuracy = None
+    if not args.no_accuracy:
+        print(f"bench: FP32-bulk vs FP64 error report at {n}^3, 10 steps", file=sys.stderr, flush=True)
+        accuracy = accuracy_report(g, n, 10, 1e-3)
     adjoint = None
     if not args.no_adjoint:
         print(f"bench: adjoint, 128^3, 200 IF-RK4 steps, checkpoints every 20, "
@@ -427,6 +503,8 @@ def main() -> int:
         "tracers": tracers,
         "ensemble": ensemble,
         "adjoint": adjoint,
+        "iterations": iterations,
+        "accuracy": accuracy,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -43,32 +43,46 @@ struct PipelineImpl final : Pipeline {
   static constexpr int xr() { return XT / xpass_lanes<T, N, XT>(); }
   template <int XT>
   static constexpr size_t lds_x() { return ((size_t)xr<XT>() * N + N) * sizeof(cx<T>); }
-  // FP32 at N = 512: the register-resident x pass, one wavefront per row pair
-  static constexpr bool XWAVE = std::is_same<T, float>::value && N == 512;
+  // FP32 at N = 512 and 1024: the register-resident x pass, one wavefront per row pair (512) or
+  // per row (1024, as a 512-point complex sequence)
+  static constexpr bool XWAVE = std::is_same<T, float>::value && (N == 512 || N == 1024);
+  static constexpr int XW_ROWS = N == 1024 ? 1 : 2;   // rows per wavefront
   template <int XT>
-  static constexpr size_t lds_xw() { return (size_t)(XT / 64) * (N + N / 8) * sizeof(cx<T>); }
+  static constexpr size_t lds_xw() { return (size_t)(XT / 64) * (512 + 512 / 8) * sizeof(cx<T>); }
 
   static void set_lds(const void *fn, size_t bytes) {
     (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   }
 
+  // The z passes take the schedules of ZSched, except at N >= 1024: there a full-width tile would
+  // give each thread two first-pass butterflies (which spill), so the tile is only as wide as one
+  // butterfly per thread allows on 1024 threads (FP32) or on 512 (FP64, whose values need the
+  // 256 registers a thread keeps at two waves per SIMD), in every z schedule.
+  static constexpr bool ZWIDE = N >= 1024;
+  static constexpr int ZNT_WIDE = sizeof(T) == 8 ? 512 : 1024;
+  static constexpr int ZTX = ZWIDE ? std::min(TX, ZNT_WIDE / (N / first_radix<N>())) : TX;
+  static constexpr int ZF_NT[2] = {ZWIDE ? ZNT_WIDE : ZSched<T>::zf_nt[0], ZWIDE ? ZNT_WIDE : ZSched<T>::zf_nt[1]};
+  static constexpr int ZI_TX[2] = {ZWIDE ? ZTX : ZSched<T>::zi_tx[0], ZWIDE ? ZTX : ZSched<T>::zi_tx[1]};
+  static constexpr int ZI_NT[2] = {ZWIDE ? ZNT_WIDE : ZSched<T>::zi_nt[0], ZWIDE ? ZNT_WIDE : ZSched<T>::zi_nt[1]};
+  static constexpr size_t LDS_Z = ((size_t)N * ZTX + N) * sizeof(cx<T>);   // z tile + twiddles
+
   // the z forward schedule 0 stages two velocity products in LDS when they fit beside the tile
-  static constexpr bool ZFL = zf_lds_bytes<N, TX, T>(true) <= 160 * 1024;
+  static constexpr bool ZFL = zf_lds_bytes<N, ZTX, T>(true) <= 160 * 1024;
 
   template <int V>
   static void set_all() {
     set_lds((const void *)k_yfwd<T, N, TX, kYThreads[V]>, LDS_COL);
     set_lds((const void *)k_yinv<T, N, TX, kYThreads[V]>, LDS_COL);
     if constexpr (V == 0) {
-      constexpr int ZN = ZSched<T>::zf_nt[0];
-      set_lds((const void *)k_zs<T, N, TX, ZN>, LDS_COL);
-      set_lds((const void *)k_zf<T, N, TX, ZN, false, ZFL>, zf_lds_bytes<N, TX, T>(ZFL));
+      constexpr int ZN = ZF_NT[0];
+      set_lds((const void *)k_zs<T, N, ZTX, ZN>, LDS_Z);
+      set_lds((const void *)k_zf<T, N, ZTX, ZN, false, ZFL>, zf_lds_bytes<N, ZTX, T>(ZFL));
     } else {
-      constexpr int ZN = ZSched<T>::zf_nt[1];
-      set_lds((const void *)k_zf<T, N, TX, ZN, true>, LDS_COL);
-      set_lds((const void *)k_zf<T, N, TX, ZN, false>, LDS_COL);
+      constexpr int ZN = ZF_NT[1];
+      set_lds((const void *)k_zf<T, N, ZTX, ZN, true>, LDS_Z);
+      set_lds((const void *)k_zf<T, N, ZTX, ZN, false>, LDS_Z);
     }
-    constexpr int ZT = ZSched<T>::zi_tx[V], ZN = ZSched<T>::zi_nt[V];
+    constexpr int ZT = ZI_TX[V], ZN = ZI_NT[V];
     constexpr size_t ZL = ((size_t)N * ZT + N) * sizeof(cx<T>);
     set_lds((const void *)k_zi<T, N, ZT, ZN, true>, ZL);
     set_lds((const void *)k_zi<T, N, ZT, ZN, false>, ZL);
@@ -136,7 +150,7 @@ struct PipelineImpl final : Pipeline {
 
   int tiles(const Ctx &c) const { return c.geo.kxp / TX; }
   int tile_width() const override { return TX; }
-  int z_blocks(const Ctx &c) const override { return c.geo.kxp / ZSched<T>::zi_tx[c.sched[2]] * K; }
+  int z_blocks(const Ctx &c) const override { return c.geo.kxp / ZI_TX[c.sched[2]] * K; }
 
   template <int NTY>
   void yfwd_v(Ctx &c) {
@@ -147,23 +161,23 @@ struct PipelineImpl final : Pipeline {
 
   template <int V>
   void zf_v(Ctx &c, const StageArgs &sa) {
-    constexpr int ZN = ZSched<T>::zf_nt[V];
-    const dim3 grid(c.geo.kxp / TX, K, c.nbox);
+    constexpr int ZN = ZF_NT[V];
+    const dim3 grid(c.geo.kxp / ZTX, K, c.nbox);
     const cx<T> *pf = (const cx<T> *)c.bufb;
     cx<T> *uh = (cx<T> *)c.uh, *hu = (cx<T> *)c.hu, *ph = (cx<T> *)c.ph, *hp = (cx<T> *)c.hp;
     const cx<double> *tw = (const cx<double> *)c.tw;
     if constexpr (V == 0) {
       if (c.cfg.has_scalar)   // the scalar half reads the stage u_z before the velocity half updates it
-        hipLaunchKernelGGL((k_zs<T, N, TX, ZN>), grid, dim3(ZN), LDS_COL, c.stream, pf, (const cx<T> *)uh, ph, hp,
+        hipLaunchKernelGGL((k_zs<T, N, ZTX, ZN>), grid, dim3(ZN), LDS_Z, c.stream, pf, (const cx<T> *)uh, ph, hp,
                            c.geo, tw, sa);
-      hipLaunchKernelGGL((k_zf<T, N, TX, ZN, false, ZFL>), grid, dim3(ZN), (zf_lds_bytes<N, TX, T>(ZFL)), c.stream,
+      hipLaunchKernelGGL((k_zf<T, N, ZTX, ZN, false, ZFL>), grid, dim3(ZN), (zf_lds_bytes<N, ZTX, T>(ZFL)), c.stream,
                          pf, uh, hu, ph, hp, c.geo, tw, sa);
     } else {
       if (c.cfg.has_scalar)
-        hipLaunchKernelGGL((k_zf<T, N, TX, ZN, true>), grid, dim3(ZN), LDS_COL, c.stream, pf, uh, hu, ph, hp, c.geo,
+        hipLaunchKernelGGL((k_zf<T, N, ZTX, ZN, true>), grid, dim3(ZN), LDS_Z, c.stream, pf, uh, hu, ph, hp, c.geo,
                            tw, sa);
       else
-        hipLaunchKernelGGL((k_zf<T, N, TX, ZN, false>), grid, dim3(ZN), LDS_COL, c.stream, pf, uh, hu, ph, hp, c.geo,
+        hipLaunchKernelGGL((k_zf<T, N, ZTX, ZN, false>), grid, dim3(ZN), LDS_Z, c.stream, pf, uh, hu, ph, hp, c.geo,
                            tw, sa);
     }
   }
@@ -177,7 +191,7 @@ struct PipelineImpl final : Pipeline {
 
   template <int V>
   void zi_v(Ctx &c) {
-    constexpr int ZT = ZSched<T>::zi_tx[V], ZN = ZSched<T>::zi_nt[V];
+    constexpr int ZT = ZI_TX[V], ZN = ZI_NT[V];
     constexpr size_t ZL = ((size_t)N * ZT + N) * sizeof(cx<T>);
     const dim3 grid(c.geo.kxp / ZT, K, c.nbox);
     if (c.cfg.has_scalar)
@@ -200,7 +214,7 @@ struct PipelineImpl final : Pipeline {
   template <bool PF>
   void xpass_wave_v(Ctx &c, const XArgs &xa) {
     constexpr int XT = 256;
-    const int blocks = N * N / (2 * (XT / 64));
+    const int blocks = N * N / (XW_ROWS * (XT / 64));
     if (c.cfg.has_scalar)
       hipLaunchKernelGGL((k_xpass_wave<T, N, 7, XT, PF>), dim3(blocks, c.nbox), dim3(XT), lds_xw<XT>(), c.stream,
                          (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);

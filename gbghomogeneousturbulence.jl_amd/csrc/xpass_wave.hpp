@@ -18,6 +18,14 @@
 // The components are streamed so that at most u, two partial Lamb components and the component
 // in flight are live: u, then phi and the three fluxes u phi, then w_x, w_y, w_z, each adding
 // its terms of u x w, each Lamb component transformed as soon as it is complete.
+//
+// At N = 1024 a wavefront takes one real row instead of a pair, as the 512-point complex sequence
+// z[n] = x[2n] + i x[2n+1] (the same three-pass transform, the same registers).  With H = 512 and
+// W = exp(-2 pi i / 1024):
+//   inverse:  Z[m] = (X[m] + conj X[H-m]) + i W^-m (X[m] - conj X[H-m]),  z = IFFT_H(Z);
+//   forward:  Z = FFT_H(z),  X[k] = (Z[k] + conj Z[H-k]) / 2 + W^k (Z[k] - conj Z[H-k]) / (2 i).
+// Lane j then holds the physical pairs x[2n], x[2n+1] at n = j + 64 q, and the products stay
+// lane-local exactly as for a row pair.
 #pragma once
 #include "pencil.hpp"
 
@@ -75,30 +83,38 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
 template <typename T, int N, int NCIN, int XT, bool PF>
 __global__ __launch_bounds__(XT, PF ? 2 : 3) void k_xpass_wave(const cx<T> *__restrict__ rin, cx<T> *__restrict__ pxo, Geo g,
                                                    const cx<double> *__restrict__ tw, XArgs xa) {
-  static_assert(N == 512, "three radix-8 passes on 64 lanes");
+  static_assert(N == 512 || N == 1024, "three radix-8 passes on 64 lanes");
   static_assert(NCIN == 7 || NCIN == 6, "u, w and optionally phi");
-  constexpr int XR = XT / 64;          // row pairs per block, one per wavefront
+  constexpr bool HALF = N == 1024;     // one real row per wavefront (else a row pair)
+  constexpr int H = 512;               // the wavefront's transform length
+  constexpr int RW = HALF ? 1 : 2;     // rows per wavefront
+  constexpr int XR = XT / 64;          // wavefronts per block
   constexpr int M = N / 3;
-  constexpr int NP = N + N / 8;        // padded exchange buffer
+  constexpr int NP = H + H / 8;        // padded exchange buffer
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double red[XR];
   __shared__ cx<T> t64[64];
   const int w = threadIdx.x >> 6, j = threadIdx.x & 63;
   cx<T> *wb = reinterpret_cast<cx<T> *>(smem_raw) + w * NP;
-  if (threadIdx.x < 64) t64[j] = {T(tw[8 * j].re), T(tw[8 * j].im)};
+  if (threadIdx.x < 64) t64[j] = {T(tw[(N / 64) * j].re), T(tw[(N / 64) * j].im)};   // W_64^j
   __syncthreads();
 
-  const int rows_per_plane = N / (2 * XR);
+  const int rows_per_plane = N / (RW * XR);
   const int z = blockIdx.x / rows_per_plane, box = blockIdx.y;
-  const int ya = (blockIdx.x % rows_per_plane) * (2 * XR) + 2 * w;
+  const int ya = (blockIdx.x % rows_per_plane) * (RW * XR) + RW * w;
   rin += box * g.box_b;
   pxo += box * g.box_a;
   const size_t plane = (size_t)N * g.kxp;           // one z plane of one component in R / PX
   const size_t vol = (size_t)N * N * N;
 
-  cx<T> t3[7];
+  cx<T> t3[7];                                      // W_512^{q j}
 #pragma unroll
-  for (int q = 1; q < 8; ++q) t3[q - 1] = {T(tw[q * j].re), T(tw[q * j].im)};
+  for (int q = 1; q < 8; ++q) t3[q - 1] = {T(tw[(N / H) * q * j].re), T(tw[(N / H) * q * j].im)};
+  cx<T> wq[HALF ? 8 : 1];                           // W_1024^{j + 64 q} (HALF)
+  if constexpr (HALF) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) wq[q] = {T(tw[j + 64 * q].re), T(tw[j + 64 * q].im)};
+  }
 
   // The coefficients of component c of R that the first pass of its inverse transform reads:
   // slot q holds (a, b) at kx = j + 64 q (when <= M) or at kx = N - j - 64 q (when >= N - M),
@@ -106,6 +122,8 @@ __global__ __launch_bounds__(XT, PF ? 2 : 3) void k_xpass_wave(const cx<T> *__re
   // one transforms, so its HBM latency hides behind the transform.
   const cx<T> *rbase = rin + (size_t)z * plane + (size_t)ya * g.kxp;
   cx<T> ra[8], rb[8];
+  // (HALF: slot q holds X[m] and X[H - m] of the row at m = j + 64 q, zero beyond M; X[H] is
+  // outside the ball)
   auto fetch = [&](int c) {
     const cx<T> *src = rbase + (size_t)c * N * plane;
 #pragma unroll
@@ -113,7 +131,10 @@ __global__ __launch_bounds__(XT, PF ? 2 : 3) void k_xpass_wave(const cx<T> *__re
       const int n = j + 64 * q;
       ra[q] = czero<T>();
       rb[q] = czero<T>();
-      if (n <= M) {
+      if constexpr (HALF) {
+        if (n <= M) ra[q] = src[n];
+        if (H - n <= M) rb[q] = src[H - n];
+      } else if (n <= M) {
         ra[q] = src[n];
         rb[q] = src[g.kxp + n];
       } else if (n >= N - M) {
@@ -131,6 +152,14 @@ __global__ __launch_bounds__(XT, PF ? 2 : 3) void k_xpass_wave(const cx<T> *__re
     for (int q = 0; q < 8; ++q) {
       const int n = j + 64 * q;
       cx<T> a = ra[q], b = rb[q];
+      if constexpr (HALF) {
+        if (n == 0) a.im = T(0);   // the self-conjugate coefficient: only its real part is seen
+        const cx<T> sm = {a.re + b.re, a.im - b.im};          // X[m] + conj X[H-m]
+        const cx<T> df = {a.re - b.re, a.im + b.im};          // X[m] - conj X[H-m]
+        const cx<T> p = twiddled<+1>(df, wq[q]);              // W^-m (..)
+        x[q] = {sm.re - p.im, sm.im + p.re};
+        continue;
+      }
       if (n <= M) {
         if (n == 0) {  // the self-conjugate coefficient: only its real part is seen
           a.im = T(0);
@@ -150,6 +179,26 @@ __global__ __launch_bounds__(XT, PF ? 2 : 3) void k_xpass_wave(const cx<T> *__re
     wave_fft512<-1>(x, wb, t64, t3, j);
     cx<T> *dst = pxo + (size_t)d * N * plane + (size_t)z * plane + (size_t)ya * g.kxp;
     const int partner = (64 - j) & 63;
+    if constexpr (HALF) {
+      // X[k] for k = j + 64 q <= M (q <= 5): Z[H - k] is slot 7 - q of lane 64 - j, or slot
+      // (8 - q) mod 8 of lane 0
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        cx<T> zp = {lane_value(x[7 - q].re, partner), lane_value(x[7 - q].im, partner)};
+        if (j == 0) zp = x[(8 - q) & 7];
+        const int kx = j + 64 * q;
+        if (kx <= M) {
+          const cx<T> Z = x[q];
+          const cx<T> e = {T(0.5) * (Z.re + zp.re), T(0.5) * (Z.im - zp.im)};     // (Z + conj Zp) / 2
+          const cx<T> o = {T(0.5) * (Z.im + zp.im), T(0.5) * (zp.re - Z.re)};     // (Z - conj Zp) / 2i
+          const cx<T> t = twiddled<-1>(o, wq[q]);
+          dst[kx] = {e.re + t.re, e.im + t.im};
+        } else if (kx < g.kxp) {
+          dst[kx] = czero<T>();
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       // C[N - kx] for kx = j + 64 q: slot 7 - q of lane 64 - j, or slot (8 - q) mod 8 of lane 0
@@ -173,8 +222,12 @@ __global__ __launch_bounds__(XT, PF ? 2 : 3) void k_xpass_wave(const cx<T> *__re
   auto store_phys = [&](T *base, const cx<T>(&x)[8]) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      base[rowa + j + 64 * q] = x[q].re;
-      base[rowb + j + 64 * q] = x[q].im;
+      if constexpr (HALF) {   // (x[2n], x[2n+1]) at n = j + 64 q
+        *reinterpret_cast<cx<T> *>(base + rowa + 2 * (j + 64 * q)) = x[q];
+      } else {
+        base[rowa + j + 64 * q] = x[q].re;
+        base[rowb + j + 64 * q] = x[q].im;
+      }
     }
   };
 
