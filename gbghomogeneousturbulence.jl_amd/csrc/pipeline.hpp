@@ -17,14 +17,15 @@ template <typename T>
 constexpr int tile_width_of() { return sizeof(T) == 4 ? 16 : 8; }
 
 // Two launch schedules per stage kernel family, chosen per context by hdns_autotune:
-//   y passes: 256 or 512 threads per block;
+//   y passes: 256 or 512 threads per block (512 or 1024 from N = 512 on);
 //   z forward: (0) the scalar half and the velocity half as two kernels on 64*TX threads, one
 //              last-pass butterfly per thread at N = 512, the first two transformed velocity
 //              products staged in LDS where they fit (N <= 512); or (1) one fused kernel on 512;
 //   z inverse: a TX-wide tile on 64*TX threads (one first-pass butterfly per thread at N = 512)
 //              or on 32*TX threads (two);
-//   x pass: 256 or 512 threads per block (FP32 at N = 512: the register-resident pass with or
-//           without the prefetch of the next component).
+//   x pass: 256 or 512 threads per block (FP32 at N = 512 and 1024: the register-resident pass
+//           with or without the prefetch of the next component).
+// At N >= 1024 the z schedules use narrower tiles (below).
 constexpr int kYThreads[2] = {256, 512};
 constexpr int kXThreads[2] = {256, 512};
 template <typename T>
@@ -59,6 +60,9 @@ struct PipelineImpl final : Pipeline {
   // butterfly per thread allows on 1024 threads (FP32) or on 512 (FP64, whose values need the
   // 256 registers a thread keeps at two waves per SIMD), in every z schedule.
   static constexpr bool ZWIDE = N >= 1024;
+  // y passes: from N = 512 on the two schedules are 512 and 1024 threads (a column tile of 64 or
+  // 128 KiB leaves room for two blocks per CU, or one)
+  static constexpr int YT[2] = {N >= 512 ? 512 : kYThreads[0], N >= 512 ? 1024 : kYThreads[1]};
   static constexpr int ZNT_WIDE = sizeof(T) == 8 ? 512 : 1024;
   static constexpr int ZTX = ZWIDE ? std::min(TX, ZNT_WIDE / (N / first_radix<N>())) : TX;
   static constexpr int ZF_NT[2] = {ZWIDE ? ZNT_WIDE : ZSched<T>::zf_nt[0], ZWIDE ? ZNT_WIDE : ZSched<T>::zf_nt[1]};
@@ -71,8 +75,8 @@ struct PipelineImpl final : Pipeline {
 
   template <int V>
   static void set_all() {
-    set_lds((const void *)k_yfwd<T, N, TX, kYThreads[V]>, LDS_COL);
-    set_lds((const void *)k_yinv<T, N, TX, kYThreads[V]>, LDS_COL);
+    set_lds((const void *)k_yfwd<T, N, TX, YT[V]>, LDS_COL);
+    set_lds((const void *)k_yinv<T, N, TX, YT[V]>, LDS_COL);
     if constexpr (V == 0) {
       constexpr int ZN = ZF_NT[0];
       set_lds((const void *)k_zs<T, N, ZTX, ZN>, LDS_Z);
@@ -157,7 +161,7 @@ struct PipelineImpl final : Pipeline {
     hipLaunchKernelGGL((k_yfwd<T, N, TX, NTY>), dim3(tiles(c), N, c.geo.ncf * c.nbox), dim3(NTY), LDS_COL, c.stream,
                        (const cx<T> *)c.bufa, (cx<T> *)c.bufb, c.geo, (const cx<double> *)c.tw);
   }
-  void yfwd(Ctx &c) override { c.sched[0] ? yfwd_v<kYThreads[1]>(c) : yfwd_v<kYThreads[0]>(c); }
+  void yfwd(Ctx &c) override { c.sched[0] ? yfwd_v<YT[1]>(c) : yfwd_v<YT[0]>(c); }
 
   template <int V>
   void zf_v(Ctx &c, const StageArgs &sa) {
@@ -208,7 +212,7 @@ struct PipelineImpl final : Pipeline {
     hipLaunchKernelGGL((k_yinv<T, N, TX, NTY>), dim3(tiles(c), N, c.geo.nci * c.nbox), dim3(NTY), LDS_COL, c.stream,
                        (const cx<T> *)c.bufa, (cx<T> *)c.bufb, c.geo, (const cx<double> *)c.tw);
   }
-  void yinv(Ctx &c) override { c.sched[0] ? yinv_v<kYThreads[1]>(c) : yinv_v<kYThreads[0]>(c); }
+  void yinv(Ctx &c) override { c.sched[0] ? yinv_v<YT[1]>(c) : yinv_v<YT[0]>(c); }
 
   // FP32 at N = 512: schedule 0 prefetches the next component, schedule 1 does not (256 threads)
   template <bool PF>
