@@ -166,6 +166,21 @@ def recommended_cfl(scheme, domain: FixedDomain) -> float:
     return scheme.base_cfl
 
 
+def cfl_bound(speed: float, dx: float, cfl: float, diffusivity: float, side_length: float,
+              max_dt: float = math.inf, step: int = 0) -> float:
+    """The adaptive step, the smallest of three bounds (src/SimulationRun.jl:1051-1096):
+    advective cfl dx / max|u| (cfl dx at rest), resolution 1 / (D kappa_min^2) with D the largest
+    diffusivity and kappa_min = 2 pi / L, and the caller's max_dt.  The start-up bound of
+    src/SimulationRun.jl:1118-1129 applies only to tracer schemes that integrate over more than one
+    sample, so it is absent under Heun.  A non-finite speed is the blow-up error."""
+    if not math.isfinite(speed):
+        raise RuntimeError(f"non-finite velocity at step {step}: the run appears to have blown up")
+    advective = cfl * dx / speed if speed > 0 else cfl * dx
+    kmin = 2 * math.pi / side_length
+    resolution = 1 / (diffusivity * kmin ** 2)
+    return min(advective, resolution, max_dt)
+
+
 class Heun:
     """ABM(1), the default tracer scheme (src/ParticlePropagator.jl:165)."""
 
@@ -210,7 +225,7 @@ def _check_reality(f: np.ndarray, name: str, dtype) -> None:
     s = f[..., 0]
     res = float(np.max(np.abs(s - np.conj(s[..., neg, :][..., :, neg])))) if s.size else 0.0
     scale = float(np.max(np.abs(f))) if f.size else 0.0
-    if res > math.sqrt(np.finfo(dtype).eps) * scale:
+    if not res <= math.sqrt(np.finfo(dtype).eps) * scale:   # NaN refused, as `residual <= ...` refuses it
         raise ValueError(f"Simulation: the given `state`'s {name} coefficients are not the spectrum of a real "
                          f"field -- the conjugate-symmetry residual is {res} against a field scale of {scale}. "
                          "Repair with `reality_projection!` or fix the conversion that produced them.")
@@ -523,16 +538,10 @@ class Simulation:
 
     def cfl_timestep(self) -> float:
         """The advective/resolution/max_dt bound (src/SimulationRun.jl:1051-1096)."""
-        dx = self.domain.grid_spacing
-        speed = self.max_speed()
-        if not math.isfinite(speed):
-            raise RuntimeError(f"non-finite velocity at step {self.step}: the run appears to have blown up")
-        advective = self.cfl * dx / speed if speed > 0 else self.cfl * dx
         c = self.conditions
         diffusivity = max(c.viscosity, c.scalar_diffusivity) if self.carries_scalar else c.viscosity
-        kmin = 2 * math.pi / self.domain.side_length
-        resolution = 1 / (diffusivity * kmin ** 2)
-        return min(advective, resolution, self.max_dt)
+        return cfl_bound(self.max_speed(), self.domain.grid_spacing, self.cfl, diffusivity,
+                         self.domain.side_length, self.max_dt, step=self.step)
 
     def advect(self, dt: float) -> None:
         """advect!(Heun, ...) (src/ParticlePropagator.jl:349-372) with its epoch check."""

@@ -1,0 +1,78 @@
+"""Host-side rows of the run API on the CPU: the viscosity rule, the CFL ceiling and the adaptive
+step's three bounds, the reality check of the adoption door and the band truncation
+(src/SimulationRun.jl:647-655,1051-1096,1236-1248; src/Propagator.jl:895-919;
+src/SimulationDomain.jl:520-537)."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import dns
+
+L = 2 * math.pi
+
+
+@pytest.mark.parametrize("n,nu", [(64, 1.347e-2), (128, 5.346e-3), (256, 2.121e-3), (512, 8.419e-4),
+                                  (1024, 3.341e-4)])
+def test_predicted_viscosity_gives_kmax_eta_of_one_and_a_half(g, n, nu):
+    """nu = eps^(1/3) eta^(4/3) with eta = 1.5 / kappa_max and kappa_max = (2/3)(N/2)(2 pi / L):
+    at eps = 0.1 the values quoted for the configurations, and k_max eta = 1.5 back from nu."""
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
+    got = g.predicted_viscosity(dom, 0.1)
+    assert got == pytest.approx(nu, rel=5e-4)
+    kmax = (2 / 3) * (n / 2)
+    eta = (got ** 3 / 0.1) ** 0.25
+    assert kmax * eta == pytest.approx(1.5, rel=1e-12)
+
+
+def test_recommended_cfl_is_the_base_ceiling_under_two_thirds(g):
+    dom = g.FixedDomain(grid_points_per_axis=32, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
+    assert g.recommended_cfl(g.IFLowStorageRK4, dom) == 0.96
+    assert g.recommended_cfl(g.IFLowStorageRK3, dom) == 0.5
+
+
+def test_cfl_bound_takes_the_smallest_of_its_three_bounds(g):
+    dx = L / 64
+    # advective binds in a turbulent flow
+    assert g.cfl_bound(2.0, dx, 0.96, 1e-2, L) == pytest.approx(0.96 * dx / 2.0, rel=1e-15)
+    # at rest the advective bound is cfl dx, and the resolution bound 1 / (D kappa_min^2) can bind
+    assert g.cfl_bound(0.0, dx, 0.96, 1e-2, L) == pytest.approx(0.96 * dx, rel=1e-15)
+    assert g.cfl_bound(0.0, dx, 0.96, 20.0, L) == pytest.approx(1 / (20.0 * 1.0 ** 2), rel=1e-15)
+    assert g.cfl_bound(1e-9, dx, 0.96, 10.0, 4 * math.pi) == pytest.approx(1 / (10.0 * 0.5 ** 2), rel=1e-15)
+    # the caller's ceiling
+    assert g.cfl_bound(2.0, dx, 0.96, 1e-2, L, max_dt=1e-4) == 1e-4
+    # a blown-up flow is an error, not a step
+    for bad in (math.nan, math.inf):
+        with pytest.raises(RuntimeError, match="blown up"):
+            g.cfl_bound(bad, dx, 0.96, 1e-2, L, step=7)
+
+
+def test_reality_check_of_the_adoption_door(g):
+    """A Hermitian kx = 0 plane passes; an O(1) anti-Hermitian pair, or a NaN, is refused; a
+    round-off disagreement is not a violation."""
+    from gbg_amd.api import _check_reality
+
+    n = 16
+    d = dns.Domain(n)
+    rng = np.random.default_rng(4)
+    f = dns.reality_projection(d, dns.rfft3(rng.standard_normal((n, n, n))) * d.mask)
+    _check_reality(f, "scalar", np.float64)
+    near = f.copy()
+    near[1, 2, 0] *= 1 + 1e-12
+    _check_reality(near, "scalar", np.float64)
+    bad = f.copy()
+    bad[1, 2, 0] += 1j * np.abs(f).max()
+    with pytest.raises(ValueError, match="not the spectrum of a real"):
+        _check_reality(bad, "scalar", np.float64)
+    nan = f.copy()
+    nan[3, 3, 3] = np.nan
+    with pytest.raises(ValueError):
+        _check_reality(nan, "scalar", np.float64)
+
+
+def test_truncate_to_band_keeps_exactly_the_closed_ball(g):
+    for n in (16, 48):
+        dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
+        f = np.ones((3, n, n, n // 2 + 1), dtype=np.complex128)
+        g.truncate_to_band(f, dom)
+        assert np.array_equal(f[0] != 0, dns.Domain(n).mask)
