@@ -278,7 +278,7 @@ def _config(domain: FixedDomain, conditions: FlowConditions, scheme, carries_sca
 
 
 # the kernel families the autotuner schedules, in the order of hdns_schedule's entries
-SCHEDULE_FAMILIES = ("y_passes", "z_forward", "z_inverse", "x_pass")
+SCHEDULE_FAMILIES = ("y_passes", "z_forward", "z_inverse", "x_pass", "y_radix_plan")
 
 
 class Simulation:
@@ -466,7 +466,7 @@ class Simulation:
 
     def schedule(self, set_to=None):
         """Read (and optionally set) the launch schedule: one entry of 0 or 1 per kernel family
-        (y passes, z forward, z inverse, x pass)."""
+        (y passes, z forward, z inverse, x pass, y radix plan)."""
         get = np.zeros(len(SCHEDULE_FAMILIES), dtype=np.int32)
         P = ctypes.POINTER(c_int32)
         s = None if set_to is None else np.ascontiguousarray(set_to, dtype=np.int32)

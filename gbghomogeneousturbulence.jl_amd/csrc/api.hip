@@ -1099,7 +1099,7 @@ hdns_status hdns_autotune(hdns_ctx *ctx, int32_t reps, double *ms, int32_t *chos
           (void)hipEventElapsedTime(&t[k], e0, e1);
         }
         if (r == 0) continue;
-        total += fam == 0 ? t[1] + t[3] : fam == 1 ? t[4] : fam == 2 ? t[0] : t[2];
+        total += (fam == 0 || fam == 4) ? t[1] + t[3] : fam == 1 ? t[4] : fam == 2 ? t[0] : t[2];
       }
       best[fam][v] = total / reps;
     }

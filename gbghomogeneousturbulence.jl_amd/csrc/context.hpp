@@ -76,8 +76,8 @@ struct Ctx {
   // launch schedule of the y passes, the z forward pass, the z inverse pass and the x pass
   // (index into the two variants of each), set by hdns_autotune; the defaults are the fastest
   // measured at 512^3 on MI355X
-  static constexpr int NFAM = 4;
-  int sched[NFAM] = {1, 0, 0, 0};
+  static constexpr int NFAM = 5;   // y threads, z forward, z inverse, x pass, y radix plan
+  int sched[NFAM] = {1, 0, 0, 0, 0};
   BandSpec band{};
   hipStream_t stream = nullptr;
   void *uh = nullptr, *hu = nullptr, *ph = nullptr, *hp = nullptr;

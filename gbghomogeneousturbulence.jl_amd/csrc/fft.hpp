@@ -349,4 +349,27 @@ __device__ __forceinline__ void fft_io(cx<T> *buf, const cx<T> *tw, Load &&load,
   else static_assert(N == 16, "unsupported transform length");
 }
 
+// The second radix plan of each length, led by a radix-16 pass (one pass fewer at 256 and 1024);
+// the autotuner picks between it and fft_io's plan for the y passes.
+template <int N>
+struct AltPlan;
+template <> struct AltPlan<16> { using type = Radices<2, 8>; };
+template <> struct AltPlan<32> { using type = Radices<16, 2>; };
+template <> struct AltPlan<64> { using type = Radices<16, 4>; };
+template <> struct AltPlan<128> { using type = Radices<16, 8>; };
+template <> struct AltPlan<256> { using type = Radices<16, 16>; };
+template <> struct AltPlan<512> { using type = Radices<16, 8, 4>; };
+template <> struct AltPlan<1024> { using type = Radices<16, 8, 8>; };
+template <> struct AltPlan<48> { using type = Radices<16, 3>; };
+template <> struct AltPlan<96> { using type = Radices<16, 2, 3>; };
+template <> struct AltPlan<192> { using type = Radices<16, 4, 3>; };
+
+// fft_io under radix plan PLAN (0: fft_io's, 1: AltPlan's)
+template <int PLAN, typename T, int N, int TX, int NT, int SIGN, bool IL, bool WAVE, bool SYNC_READ = false,
+          class Load, class Store>
+__device__ __forceinline__ void fft_io_planned(cx<T> *buf, const cx<T> *tw, Load &&load, Store &&store) {
+  if constexpr (PLAN == 0) fft_io<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(buf, tw, load, store);
+  else fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(typename AltPlan<N>::type{}, buf, tw, load, store);
+}
+
 }  // namespace hdns

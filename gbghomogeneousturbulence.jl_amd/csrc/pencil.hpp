@@ -66,7 +66,7 @@ __host__ __device__ __forceinline__ int line_start(int mx) { return mx & ~(int)(
 // ---------------------------------------------------------------------------------------------
 // y passes: a block transforms TX adjacent kx columns of one z plane of one component.
 
-template <typename T, int N, int TX, int NT>
+template <typename T, int N, int TX, int NT, int PLAN = 0>
 __global__ __launch_bounds__(NT) void k_yfwd(const cx<T> *__restrict__ px, cx<T> *__restrict__ pf, Geo g,
                                              const cx<double> *__restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -77,7 +77,7 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cx<T> *__restrict__ px, cx<T>
   const cx<T> *src = px + box * g.box_a + ((size_t)c * N + z) * (size_t)N * g.kxp + kx0;
   cx<T> *dst = pf + box * g.box_b + ((size_t)c * N + z) * (size_t)g.k * g.kxp + kx0;
   static_assert(TX * sizeof(cx<T>) <= 128, "a y tile is at most one 128-byte line of a row");
-  fft_io<T, N, TX, NT, -1, true, false>(
+  fft_io_planned<PLAN, T, N, TX, NT, -1, true, false>(
       buf, twl, [&](int, int, int n, int col) { return src[(size_t)n * g.kxp + col]; },
       [&](int, int, int n, int col, cx<T> v) {
         const int idx = idx_of_bin(n, g);
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cx<T> *__restrict__ px, cx<T>
       });
 }
 
-template <typename T, int N, int TX, int NT>
+template <typename T, int N, int TX, int NT, int PLAN = 0>
 __global__ __launch_bounds__(NT) void k_yinv(const cx<T> *__restrict__ q, cx<T> *__restrict__ r, Geo g,
                                              const cx<double> *__restrict__ tw) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(NT) void k_yinv(const cx<T> *__restrict__ q, cx<T> 
   const int kx0 = blockIdx.x * TX, z = blockIdx.y, c = blockIdx.z % g.nci, box = blockIdx.z / g.nci;
   const cx<T> *src = q + box * g.box_a + ((size_t)c * N + z) * (size_t)g.k * g.kxp + kx0;
   cx<T> *dst = r + box * g.box_b + ((size_t)c * N + z) * (size_t)N * g.kxp + kx0;
-  fft_io<T, N, TX, NT, +1, true, false>(
+  fft_io_planned<PLAN, T, N, TX, NT, +1, true, false>(
       buf, twl,
       [&](int, int, int n, int col) {
         const int idx = idx_of_bin(n, g);

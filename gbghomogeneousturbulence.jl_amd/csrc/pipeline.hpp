@@ -24,7 +24,8 @@ constexpr int tile_width_of() { return sizeof(T) == 4 ? 16 : 8; }
 //   z inverse: a TX-wide tile on 64*TX threads (one first-pass butterfly per thread at N = 512)
 //              or on 32*TX threads (two);
 //   x pass: 256 or 512 threads per block (FP32 at N = 512 and 1024: the register-resident pass
-//           with or without the prefetch of the next component).
+//           with or without the prefetch of the next component);
+//   y radix plan: fft_io's plan or AltPlan's, led by a radix-16 pass.
 // At N >= 1024 the z schedules use narrower tiles (below).
 constexpr int kYThreads[2] = {256, 512};
 constexpr int kXThreads[2] = {256, 512};
@@ -75,8 +76,10 @@ struct PipelineImpl final : Pipeline {
 
   template <int V>
   static void set_all() {
-    set_lds((const void *)k_yfwd<T, N, TX, YT[V]>, LDS_COL);
-    set_lds((const void *)k_yinv<T, N, TX, YT[V]>, LDS_COL);
+    set_lds((const void *)k_yfwd<T, N, TX, YT[V], 0>, LDS_COL);
+    set_lds((const void *)k_yinv<T, N, TX, YT[V], 0>, LDS_COL);
+    set_lds((const void *)k_yfwd<T, N, TX, YT[V], 1>, LDS_COL);
+    set_lds((const void *)k_yinv<T, N, TX, YT[V], 1>, LDS_COL);
     if constexpr (V == 0) {
       constexpr int ZN = ZF_NT[0];
       set_lds((const void *)k_zs<T, N, ZTX, ZN>, LDS_Z);
@@ -156,12 +159,15 @@ struct PipelineImpl final : Pipeline {
   int tile_width() const override { return TX; }
   int z_blocks(const Ctx &c) const override { return c.geo.kxp / ZI_TX[c.sched[2]] * K; }
 
-  template <int NTY>
+  template <int NTY, int PLAN>
   void yfwd_v(Ctx &c) {
-    hipLaunchKernelGGL((k_yfwd<T, N, TX, NTY>), dim3(tiles(c), N, c.geo.ncf * c.nbox), dim3(NTY), LDS_COL, c.stream,
-                       (const cx<T> *)c.bufa, (cx<T> *)c.bufb, c.geo, (const cx<double> *)c.tw);
+    hipLaunchKernelGGL((k_yfwd<T, N, TX, NTY, PLAN>), dim3(tiles(c), N, c.geo.ncf * c.nbox), dim3(NTY), LDS_COL,
+                       c.stream, (const cx<T> *)c.bufa, (cx<T> *)c.bufb, c.geo, (const cx<double> *)c.tw);
   }
-  void yfwd(Ctx &c) override { c.sched[0] ? yfwd_v<YT[1]>(c) : yfwd_v<YT[0]>(c); }
+  void yfwd(Ctx &c) override {
+    if (c.sched[4]) c.sched[0] ? yfwd_v<YT[1], 1>(c) : yfwd_v<YT[0], 1>(c);
+    else c.sched[0] ? yfwd_v<YT[1], 0>(c) : yfwd_v<YT[0], 0>(c);
+  }
 
   template <int V>
   void zf_v(Ctx &c, const StageArgs &sa) {
@@ -207,12 +213,15 @@ struct PipelineImpl final : Pipeline {
   }
   void zi(Ctx &c) override { c.sched[2] ? zi_v<1>(c) : zi_v<0>(c); }
 
-  template <int NTY>
+  template <int NTY, int PLAN>
   void yinv_v(Ctx &c) {
-    hipLaunchKernelGGL((k_yinv<T, N, TX, NTY>), dim3(tiles(c), N, c.geo.nci * c.nbox), dim3(NTY), LDS_COL, c.stream,
-                       (const cx<T> *)c.bufa, (cx<T> *)c.bufb, c.geo, (const cx<double> *)c.tw);
+    hipLaunchKernelGGL((k_yinv<T, N, TX, NTY, PLAN>), dim3(tiles(c), N, c.geo.nci * c.nbox), dim3(NTY), LDS_COL,
+                       c.stream, (const cx<T> *)c.bufa, (cx<T> *)c.bufb, c.geo, (const cx<double> *)c.tw);
   }
-  void yinv(Ctx &c) override { c.sched[0] ? yinv_v<YT[1]>(c) : yinv_v<YT[0]>(c); }
+  void yinv(Ctx &c) override {
+    if (c.sched[4]) c.sched[0] ? yinv_v<YT[1], 1>(c) : yinv_v<YT[0], 1>(c);
+    else c.sched[0] ? yinv_v<YT[1], 0>(c) : yinv_v<YT[0], 0>(c);
+  }
 
   // FP32 at N = 512: schedule 0 prefetches the next component, schedule 1 does not (256 threads)
   template <bool PF>

@@ -160,17 +160,19 @@ hdns_status hdns_scheme_coefficients(int32_t scheme, int32_t precision, double *
  * 4-point Gauss-Legendre (src/ParticlePropagator.jl:261-315).  Needs no device. */
 hdns_status hdns_heun_weights(double h, double *out3);
 
-/* Autotuner: time both launch schedules of each kernel family -- the y passes (256/512 threads),
- * the z forward pass (split scalar/velocity kernels with LDS-staged products, or one fused
- * kernel), the z inverse pass (full or half-width tiles) and the x pass (256/512) -- on this
+/* Autotuner: time both launch schedules of each kernel family -- the y passes (256/512 threads,
+ * 512/1024 from N = 512 on), the z forward pass (split scalar/velocity kernels with LDS-staged
+ * products, or one fused kernel), the z inverse pass (full or half-width tiles), the x pass
+ * (256/512 threads, or with/without prefetch) and the y passes' radix plan (the default plan, or
+ * one led by a radix-16 pass) -- on this
  * context's own buffers, keep the fastest of each, and redo the deposit.  The timed sequence
  * leaves the state bitwise unchanged.  No counterpart in the reference (whose field library has
- * no launch shapes); it tunes the kernels behind hdns_step.  ms[8] (optional) receives the average
- * ms of (y: v0, v1, z forward: v0, v1, z inverse: v0, v1, x: v0, v1), chosen[4] (optional) the
- * picks. */
+ * no launch shapes); it tunes the kernels behind hdns_step.  ms[10] (optional) receives the average
+ * ms of (y: v0, v1, z forward: v0, v1, z inverse: v0, v1, x: v0, v1, y radix plan: v0, v1),
+ * chosen[5] (optional) the picks. */
 hdns_status hdns_autotune(hdns_ctx *ctx, int32_t reps, double *ms, int32_t *chosen);
-/* Set (set != NULL) and/or read (get != NULL) the schedule: four entries of 0 or 1, in the order
- * y passes, z forward, z inverse, x pass. */
+/* Set (set != NULL) and/or read (get != NULL) the schedule: five entries of 0 or 1, in the order
+ * y passes, z forward, z inverse, x pass, y radix plan. */
 hdns_status hdns_schedule(hdns_ctx *ctx, const int32_t *set, int32_t *get);
 
 /* Per-kernel timing with HIP events on the context's stream, for the roofline report.  Kinds:

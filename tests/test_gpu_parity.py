@@ -196,6 +196,9 @@ def test_fp32_bulk_at_512_agrees_with_the_fp64_step(g, scalar):
     a, *_ = make_sim(g, n, np.float32, forcing=0.1, nu=8.42e-4, scalar=scalar)
     b, *_ = make_sim(g, n, np.float64, forcing=0.1, nu=8.42e-4, scalar=scalar)
     with a, b:
+        if not scalar:   # the FP32 run on the y passes' second radix plan (16.8.4)
+            a.schedule((1, 0, 0, 0, 1))
+            a._call("hdns_deposit")
         for k in ("kinetic_energy", "band_energy"):
             assert a.reductions()[k] == pytest.approx(b.reductions()[k], rel=1e-6)
         assert rel(a.physical(0), b.physical(0)) < 2e-6
@@ -242,23 +245,51 @@ def test_autotune_leaves_the_state_bitwise_unchanged_and_schedules_agree(g):
     with sim, ref:
         before = sim.state().velocity_spectral.copy()
         report = sim.autotune(reps=2)
-        assert set(report) == {"y_passes", "z_forward", "z_inverse", "x_pass"}
+        assert set(report) == {"y_passes", "z_forward", "z_inverse", "x_pass", "y_radix_plan"}
         assert np.array_equal(sim.state().velocity_spectral, before)
         # the chosen z tiling sets the block order of the FP64 partial sums, so equal to rounding
         ra, rb = sim.reductions(), ref.reductions()
         assert ra.keys() == rb.keys()
         for key in ra:
             assert abs(ra[key] - rb[key]) <= 1e-13 * abs(rb[key]), key
-        for sched in [(0, 0, 0, 0), (1, 1, 1, 1), (0, 1, 0, 1)]:
+        for sched in [(0, 0, 0, 0, 0), (1, 1, 1, 1, 1), (0, 1, 0, 1, 0)]:
             sim.schedule(sched)
             assert sim.schedule() == sched
-        sim.schedule((0, 0, 0, 0))
-        ref.schedule((1, 1, 1, 1))
+        sim.schedule((0, 0, 0, 0, 0))
+        ref.schedule((1, 1, 1, 1, 1))
         sim.step_fields(dt)
         ref.step_fields(dt)
         a, b = sim.state(), ref.state()
         assert rel(a.velocity_spectral, b.velocity_spectral) < 1e-6
         assert rel(a.scalar_spectral, b.scalar_spectral) < 1e-6
+
+
+@pytest.mark.parametrize("n", [16, 32, 48, 64, 96, 128])
+def test_y_radix_plan_led_by_radix_16_matches_the_oracle(g, n):
+    """The y passes' second radix plan (16.2, 16.4, 16.3, 16.8, 16.2.3 ..., 2.8 at 16) in one FP64
+    forced step with the scalar, against the oracle."""
+    sim, u, p, oc = make_sim(g, n, np.float64, forcing=0.1)
+    with sim:
+        sim.schedule((1, 0, 0, 0, 1))
+        sim._call("hdns_deposit")
+        sim.advance(2e-3)
+        uo, po, _ = dns.step(dns.Domain(n), oc, "rk4", u, p, 2e-3)
+        st = sim.state()
+        assert rel(st.velocity_spectral, uo) < 1e-12
+        assert rel(st.scalar_spectral, po) < 1e-12
+
+
+def test_y_radix_plans_agree_at_256(g):
+    """16.16 (one LDS exchange) against 8.8.4 at 256^3, FP64, one forced step."""
+    a, *_ = make_sim(g, 256, np.float64, forcing=0.1)
+    b, *_ = make_sim(g, 256, np.float64, forcing=0.1)
+    with a, b:
+        a.schedule((1, 0, 0, 0, 0))
+        b.schedule((1, 0, 0, 0, 1))
+        b._call("hdns_deposit")
+        a.advance(1e-3)
+        b.advance(1e-3)
+        assert rel(b.state().velocity_spectral, a.state().velocity_spectral) < 1e-13
 
 
 def _member(g, n, dtype, nu, seed, m=0):
@@ -373,6 +404,8 @@ def test_fp32_bulk_against_fp64_at_1024_over_10_steps(g):
               f"adopted at {time.perf_counter() - t0:.1f} s", flush=True)
     del u, p
     a, b = runs
+    a.schedule((1, 0, 0, 0, 1))   # the FP32 run on the y passes' second radix plan (16.8.8)
+    a._call("hdns_deposit")
     with a, b:
         keys = ("kinetic_energy", "dissipation", "scalar_variance", "scalar_dissipation")
         for s in range(1, steps + 1):
