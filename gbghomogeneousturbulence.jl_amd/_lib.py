@@ -60,6 +60,7 @@ _SIGS = {
     "hdns_reductions": [c_void_p, _D],
     "hdns_spectrum": [c_void_p, c_int32, _D, c_int32],
     "hdns_state_distance": [c_void_p, c_void_p, _D],
+    "hdns_derivative_skewness": [c_void_p, _D],
     "hdns_field_download": [c_void_p, c_int32, _D],
     "hdns_tracers_upload": [c_void_p, c_int64, _D],
     "hdns_tracers_download": [c_void_p, _D],

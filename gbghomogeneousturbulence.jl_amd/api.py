@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import warnings
 from ctypes import c_double, c_int32, c_int64, c_uint8, c_void_p
 from dataclasses import dataclass
 
@@ -166,6 +167,17 @@ def recommended_cfl(scheme, domain: FixedDomain) -> float:
     return scheme.base_cfl
 
 
+def integral_length(spectrum) -> float:
+    """L11 = (3 pi / 4) sum_{kappa > 0} E / kappa / sum_{kappa > 0} E of a measured spectrum
+    (kappa, E) (src/Observables.jl:567-581); no energy off the kappa = 0 shell is a ValueError."""
+    kappa, energy = (np.asarray(a, dtype=np.float64) for a in spectrum)
+    live = kappa > 0
+    total = float(np.sum(energy[live]))
+    if not total > 0:
+        raise ValueError("this velocity field carries no energy, so it has no integral scale")
+    return 3 * math.pi / 4 * float(np.sum(energy[live] / kappa[live])) / total
+
+
 def cfl_bound(speed: float, dx: float, cfl: float, diffusivity: float, side_length: float,
               max_dt: float = math.inf, step: int = 0) -> float:
     """The adaptive step, the smallest of three bounds (src/SimulationRun.jl:1051-1096):
@@ -297,6 +309,7 @@ class Simulation:
                  tracer_capacity: int | None = None):
         carries_scalar = state.scalar_spectral is not None
         u, p = _state_arrays(state, domain)
+        self.mean_gradient = float(mean_gradient)
         if mean_gradient and not carries_scalar:
             raise ValueError("a mean scalar gradient needs a scalar in the state")
         self.domain = domain
@@ -382,6 +395,17 @@ class Simulation:
             r.update({"scalar_rms_difference": out[2], "scalar_rms": out[3],
                       "scalar_relative": out[2] / out[3] if out[3] > 0 else math.nan})
         return r
+
+    def derivative_skewness(self) -> float:
+        """C * 3 <det S> / <S:S>^(3/2) of the current state, C = (4/7) sqrt(15/2): about 0 for a
+        random-phase field, about -0.5 for developed turbulence (src/Observables.jl:406,465-477)."""
+        out = np.zeros(1)
+        self._call("hdns_derivative_skewness", L.dptr(out))
+        return float(out[0])
+
+    def integral_length(self) -> float:
+        """L11 of the current state's energy spectrum (src/Observables.jl:556-557)."""
+        return integral_length(self.energy_spectrum())
 
     def energy_spectrum(self, which: int = L.FIELD_VELOCITY):
         """(shell centres, E(kappa)) of the state (src/Observables.jl:502-514)."""
@@ -589,12 +613,23 @@ class Simulation:
         self.step += 1
         return self
 
-    def run(self, callback=None, *, steps: int, dt: float | None = None) -> "Simulation":
-        """run!(callback, sim; dt) for a fixed number of iterations (src/SimulationRun.jl:1395-1405)."""
-        for _ in range(int(steps)):
-            if callback is not None and callback(self) is False:
-                break
+    def run(self, callback=None, *, steps: int | None = None, dt: float | None = None) -> "Simulation":
+        """run!(callback, sim; dt) (src/SimulationRun.jl:1395-1405): hand the state to `callback`,
+        advance while it returns True, stop when it returns False.  `steps` (an extension) caps the
+        number of iterations, and with no callback it is the run's length."""
+        if callback is None and steps is None:
+            raise ValueError("give a stopping criterion, or a number of steps")
+        done = 0
+        while steps is None or done < int(steps):
+            if callback is not None:
+                keep = callback(self)
+                if not isinstance(keep, bool):
+                    raise ValueError("the stopping criterion must return a Bool -- true to continue the run, "
+                                     f"false to stop it; it returned a {type(keep).__name__}")
+                if not keep:
+                    break
             self.advance(dt)
+            done += 1
         return self
 
     def __repr__(self) -> str:
@@ -746,3 +781,124 @@ class Ensemble:
         sizes = sorted({b.domain.grid_points_per_axis for b in self.boxes})
         return f"Ensemble({len(self.boxes)} boxes of sizes {sizes}, time {self.time})"
 
+
+
+# --- spin-up to stationarity ---------------------------------------------------------------
+
+class StationarityMonitor:
+    """The flatness test of the warm-up detector (src/SimulationRun.jl:1416-1485): signals stored
+    at least window/40 turnovers apart over a trailing window; a series is flat when the means of
+    the window's two halves differ by at most tolerance * |mean|."""
+
+    def __init__(self, *, min_turnovers, window_turnovers, tolerance, gate_scalar, gate_skewness, skewness_floor):
+        self.min_turnovers = float(min_turnovers)
+        self.window_turnovers = float(window_turnovers)
+        self.tolerance = float(tolerance)
+        self.gate_scalar = bool(gate_scalar)
+        self.gate_skewness = bool(gate_skewness)
+        self.skewness_floor = float(skewness_floor)
+        self.sample_gap = self.window_turnovers / 40
+        self.turn, self.eps, self.phi2, self.skew = [], [], [], []
+        self.converged = False
+
+    @staticmethod
+    def is_flat(turn, y, now, W, tolerance) -> bool:
+        mid = now - W / 2
+        s1 = s2 = 0.0
+        n1 = n2 = 0
+        for t, v in zip(turn, y):
+            if t < now - W:
+                continue
+            if t < mid:
+                s1 += v
+                n1 += 1
+            else:
+                s2 += v
+                n2 += 1
+        if n1 == 0 or n2 == 0:
+            return False
+        mean_all = (s1 + s2) / (n1 + n2)
+        return abs(s2 / n2 - s1 / n1) <= tolerance * abs(mean_all)
+
+    @staticmethod
+    def window_mean(turn, y, now, W) -> float:
+        vals = [v for t, v in zip(turn, y) if t >= now - W]
+        return sum(vals) / len(vals) if vals else 0.0
+
+    def due(self, elapsed_turn) -> bool:
+        return not self.turn or elapsed_turn - self.turn[-1] >= self.sample_gap
+
+    def push_and_test(self, elapsed_turn, eps, phi2, skew) -> bool:
+        if self.due(elapsed_turn):
+            for series, v in ((self.turn, elapsed_turn), (self.eps, eps), (self.phi2, phi2), (self.skew, skew)):
+                series.append(v)
+            while len(self.turn) > 2 and self.turn[0] < elapsed_turn - self.window_turnovers - self.sample_gap:
+                for series in (self.turn, self.eps, self.phi2, self.skew):
+                    series.pop(0)
+        if elapsed_turn < self.min_turnovers:
+            return False
+        if not self.turn[0] <= elapsed_turn - self.window_turnovers:
+            return False
+        W, tol = self.window_turnovers, self.tolerance
+        flat_v = self.is_flat(self.turn, self.eps, elapsed_turn, W, tol)
+        flat_s = not self.gate_scalar or self.is_flat(self.turn, self.phi2, elapsed_turn, W, tol)
+        flat_k = not self.gate_skewness or (self.is_flat(self.turn, self.skew, elapsed_turn, W, tol) and
+                                            abs(self.window_mean(self.turn, self.skew, elapsed_turn, W)) >= self.skewness_floor)
+        self.converged = flat_v and flat_s and flat_k
+        return self.converged
+
+
+class StationarityCriterion:
+    """run() callback that spins a run up to a statistically stationary state
+    (src/SimulationRun.jl:1560-1628): the eddy-turnover clock advances by dt * eps / k per
+    iteration; the run stops once the dissipation (and, for a forced scalar, <phi^2>; with
+    gate_skewness, the skewness, at least skewness_floor in size) is flat over the trailing window,
+    or at max_turnovers.  A scalar driven by the imposed mean gradient counts as forced."""
+
+    def __init__(self, *, gate_scalar: bool = True, gate_skewness: bool = False, min_turnovers: float = 5,
+                 window_turnovers: float = 4, tolerance: float = 0.05, skewness_floor: float = 0.1,
+                 max_turnovers: float = 60):
+        self.monitor = StationarityMonitor(min_turnovers=min_turnovers, window_turnovers=window_turnovers,
+                                           tolerance=tolerance, gate_scalar=gate_scalar, gate_skewness=gate_skewness,
+                                           skewness_floor=skewness_floor)
+        self.max_turnovers = float(max_turnovers)
+        self.gate_scalar = bool(gate_scalar)
+        self.elapsed_turn = 0.0
+        self.last_time = math.nan
+        self.started = False
+
+    def __call__(self, sim: Simulation) -> bool:
+        if not self.started:
+            forced = sim.carries_scalar and (getattr(sim, "mean_gradient", 0.0) != 0.0 or
+                                             not isinstance(sim.conditions.scalar_forcing, NoForcing))
+            self.monitor.gate_scalar = self.gate_scalar and forced
+            self.last_time = sim.time
+            self.started = True
+        r = sim.reductions()
+        k, eps = r["kinetic_energy"], r["dissipation"]
+        dt = sim.time - self.last_time
+        self.last_time = sim.time
+        if eps > 0:
+            self.elapsed_turn += dt * eps / k
+        else:
+            warnings.warn(f"StationarityCriterion: the dissipation rate is not positive (eps = {eps}), so the "
+                          "eddy-turnover clock cannot advance and stationarity can never be detected -- stopping. "
+                          "A run with no dissipation has no cascade to become stationary.")
+            return False
+        m = self.monitor
+        if m.due(self.elapsed_turn):
+            phi2 = (r["scalar_variance"] if sim.carries_scalar else math.nan) if m.gate_scalar else 0.0
+            skew = sim.derivative_skewness() if m.gate_skewness else 0.0
+            if m.push_and_test(self.elapsed_turn, eps, phi2, skew):
+                return False
+        if self.elapsed_turn >= self.max_turnovers:
+            warnings.warn(f"StationarityCriterion: hit the max_turnovers = {self.max_turnovers} cap without "
+                          "detecting stationarity")
+            return False
+        return True
+
+
+def warmup(sim: Simulation, *, dt: float | None = None, **kwargs) -> Simulation:
+    """warmup!(sim; dt, kwargs...) = run!(StationarityCriterion(; kwargs...), sim; dt)
+    (src/SimulationRun.jl:1630-1660)."""
+    return sim.run(StationarityCriterion(**kwargs), dt=dt)

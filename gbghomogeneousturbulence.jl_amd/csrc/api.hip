@@ -840,6 +840,37 @@ hdns_status hdns_state_distance(hdns_ctx *a, hdns_ctx *b, double *out) {
   return HDNS_OK;
 }
 
+hdns_status hdns_derivative_skewness(hdns_ctx *ctx, double *out) {
+  if (!ctx || !out) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "out must not be NULL");
+  Ctx &c = ctx->c;
+  if (!c.deposited) return fail(ctx, HDNS_ERR_STATE, "no deposit of the current state");
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  double s[8];
+  HIP_TRY(ctx, hipMemcpyAsync(s, c.scalars + 8 * c.box, sizeof s, hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  const double ss = 0.5 * s[3];   // <S:S> = (1/2) <|w|^2> = (1/2) sum |k|^2 |u_hat|^2 / N^6
+  if (!(ss > 0)) {   // a strain-free field (src/Observables.jl:472): no skewness to measure
+    *out = 0.0;
+    return HDNS_OK;
+  }
+  const int nb = c.pipe->strain_det_blocks(c);
+  double *partials = nullptr;
+  HIP_TRY(ctx, hipMalloc((void **)&partials, nb * sizeof(double)));
+  c.pipe->strain_det(c, partials);
+  std::vector<double> host(nb);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(host.data(), partials, nb * sizeof(double), hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  (void)hipFree(partials);
+  if (e != hipSuccess) return fail(ctx, HDNS_ERR_DEVICE, std::string("derivative skewness: ") + hipGetErrorString(e));
+  const double n3 = pow((double)c.geo.n, 3.0);
+  const double mean_det = neumaier(host.data(), host.size()) / n3;
+  constexpr double C_ISO = (4.0 / 7.0) * 2.7386127875258306;   // (4/7) sqrt(15/2) (src/Observables.jl:406)
+  *out = 3.0 * C_ISO * mean_det / pow(ss, 1.5);
+  // the strain passes used the pencil buffers, which hold the next step's products
+  return run_deposit(ctx);
+}
+
 hdns_status hdns_spectrum(hdns_ctx *ctx, int32_t which, double *out, int32_t nshells) {
   if (!ctx || !out) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "out must not be NULL");
   Ctx &c = ctx->c;

@@ -50,7 +50,11 @@ struct Pipeline {
   virtual size_t adj_pencil_a(const Ctx &c) const = 0;   // complex elements of the pencil buffers
   virtual size_t adj_pencil_b(const Ctx &c) const = 0;
   virtual int tile_width() const = 0;  // kx columns per y block (the widest tile)
-  virtual int z_blocks(const Ctx &c) const = 0;  // blocks of a z pass under the current schedule
+  virtual int z_blocks(const Ctx &c) const = 0;
+  // <det S> of the selected box's strain: writes one FP64 partial sum per block of its x pass into
+  // partials and returns their count (the pencil buffers are overwritten: redo the deposit)
+  virtual int strain_det(Ctx &c, double *partials) = 0;
+  virtual int strain_det_blocks(const Ctx &c) const = 0;  // blocks of a z pass under the current schedule
 };
 
 std::unique_ptr<Pipeline> make_pipeline_f32(int n);

@@ -8,6 +8,7 @@
 
 #include "adjoint.hpp"
 #include "context.hpp"
+#include "observe.hpp"
 #include "tracers.hpp"
 #include "xpass_wave.hpp"
 
@@ -111,6 +112,25 @@ struct PipelineImpl final : Pipeline {
     set_lds((const void *)k_zinv_plain<T, N, TX, ZNT_PLAIN>, LDS_COL);
     set_lds((const void *)k_zfwd_plain<T, N, TX, ZNT_PLAIN>, LDS_COL);
     set_lds((const void *)k_xadj<T, N, XT_ADJ>, lds_x<XT_ADJ>());
+    set_lds((const void *)k_zstrain<T, N, TX, ZNT_PLAIN>, LDS_COL);
+    set_lds((const void *)k_xdet<T, N, XT_ADJ>, lds_x<XT_ADJ>());
+  }
+
+  // --- the derivative skewness's strain determinant -------------------------------------------
+  int strain_det_blocks(const Ctx &) const override { return N * N / (2 * xr<XT_ADJ>()); }
+  int strain_det(Ctx &c, double *partials) override {
+    const cx<double> *tw = (const cx<double> *)c.tw;
+    cx<T> *pa = (cx<T> *)c.bufa + c.box * c.geo.box_a, *pb = (cx<T> *)c.bufb + c.box * c.geo.box_b;
+    hipLaunchKernelGGL((k_zstrain<T, N, TX, ZNT_PLAIN>), dim3(tiles(c), K, 6), dim3(ZNT_PLAIN), LDS_COL, c.stream,
+                       (const cx<T> *)c.uh + c.box * 3 * c.comp, pa, c.geo, tw, T(1.0 / ((double)N * N * N)));
+    Geo g6 = c.geo;
+    g6.nci = 6;
+    hipLaunchKernelGGL((k_yinv<T, N, TX, 256>), dim3(tiles(c), N, 6), dim3(256), LDS_COL, c.stream, (const cx<T> *)pa,
+                       pb, g6, tw);
+    const int blocks = strain_det_blocks(c);
+    hipLaunchKernelGGL((k_xdet<T, N, XT_ADJ>), dim3(blocks), dim3(XT_ADJ), lds_x<XT_ADJ>(), c.stream, (const cx<T> *)pb,
+                       c.geo, tw, partials);
+    return blocks;
   }
 
   // --- the adjoint step -------------------------------------------------------------------

@@ -120,6 +120,12 @@ hdns_status hdns_reductions(hdns_ctx *ctx, double *out);
  * run of the same flow, which the reference's tests form on host arrays between two runs
  * (test/box_scaling.jl:18,36).  Waits for b's stream and runs on a's. */
 hdns_status hdns_state_distance(hdns_ctx *a, hdns_ctx *b, double *out);
+/* The velocity-derivative skewness of the selected box's current state,
+ * S = C * 3 <det S> / <S:S>^(3/2) with C = (4/7) sqrt(15/2) and S = sym grad u
+ * (src/Observables.jl:406,465-477): the strain determinant on the device, summed in FP64;
+ * <S:S> = (1/2) <|w|^2> from the deposit's reductions; 0 for a strain-free field.  Reuses the
+ * pencil buffers and redoes the deposit. */
+hdns_status hdns_derivative_skewness(hdns_ctx *ctx, double *out);
 /* Shell spectrum E(kappa_n), n = 0..nshells-1 (src/Observables.jl:502-514); which = velocity or
  * scalar.  nshells must be >= floor(sqrt(3)*N/2)+2. */
 hdns_status hdns_spectrum(hdns_ctx *ctx, int32_t which, double *out, int32_t nshells);

@@ -76,3 +76,41 @@ def test_truncate_to_band_keeps_exactly_the_closed_ball(g):
         f = np.ones((3, n, n, n // 2 + 1), dtype=np.complex128)
         g.truncate_to_band(f, dom)
         assert np.array_equal(f[0] != 0, dns.Domain(n).mask)
+
+
+def test_integral_length_of_a_measured_spectrum(g):
+    """src/Observables.jl:567-581 through the API's spectrum form, after
+    test/observables_analytic.jl:71-73."""
+    assert g.integral_length(([0.0, 1.0, 2.0], [5.0, 2.0, 1.0])) == pytest.approx(
+        3 * math.pi / 4 * (2.0 / 1.0 + 1.0 / 2.0) / (2.0 + 1.0), rel=1e-14)
+    with pytest.raises(ValueError, match="no energy"):
+        g.integral_length(([0.0, 1.0], [5.0, 0.0]))
+
+
+def test_stationarity_monitor_flatness_and_gates(g):
+    """The warm-up detector's core on synthetic signals (src/SimulationRun.jl:1433-1477): a constant
+    dissipation is flat once min_turnovers have passed and the window is full; a drifting one is
+    not; the skewness gate needs a plateau of size at least skewness_floor."""
+    from gbg_amd.api import StationarityMonitor
+
+    def feed(m, f, upto, skew=lambda t: 0.0):
+        t, out = 0.0, False
+        while t <= upto + 1e-12:
+            out = m.push_and_test(t, f(t), 1.0, skew(t))
+            t += 0.05
+        return out
+
+    kw = dict(min_turnovers=5, window_turnovers=4, tolerance=0.05, gate_scalar=False, gate_skewness=False,
+              skewness_floor=0.1)
+    assert not feed(StationarityMonitor(**kw), lambda t: 0.1, 4.9)       # before min_turnovers
+    assert feed(StationarityMonitor(**kw), lambda t: 0.1, 6.0)
+    assert not feed(StationarityMonitor(**kw), lambda t: 0.1 * (1 + 0.1 * t), 8.0)   # drifts 5 % per half-window
+    gated = dict(kw, gate_skewness=True)
+    assert not feed(StationarityMonitor(**gated), lambda t: 0.1, 8.0, skew=lambda t: 0.02)   # below the floor
+    assert feed(StationarityMonitor(**gated), lambda t: 0.1, 8.0, skew=lambda t: -0.45)
+    # the flatness test itself: two half-window means against tolerance * |mean|
+    S = StationarityMonitor
+    assert S.is_flat([0, 1, 2, 3], [1.0, 1.0, 1.04, 1.04], 3.0, 4.0, 0.05)
+    assert not S.is_flat([0, 1, 2, 3], [1.0, 1.0, 1.2, 1.2], 3.0, 4.0, 0.05)
+    assert S.is_flat([0, 1, 2, 3], [0.0, 0.0, 0.0, 0.0], 3.0, 4.0, 0.05)    # identically zero is flat
+    assert not S.is_flat([2, 3], [1.0, 1.0], 3.0, 4.0, 0.05)                # one half empty
