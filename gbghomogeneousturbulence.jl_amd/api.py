@@ -118,6 +118,26 @@ class ConstantPower:
         return f"ConstantPower(injection_rate {self.injection_rate}, shells {self.lowest_shell}-{self.highest_shell})"
 
 
+class Linear:
+    """Linear (negative-damping) forcing f = A u on shells lowest..highest (src/Forcing.jl:600-642):
+    the rate is folded into the integrating factor, so band modes grow at A - nu |k|^2, with no
+    per-stage reduction.  highest_shell = Inf forces every mode but the mean (Lundgren's forcing).
+    The rate must be given (the reference's deferred form, which takes it from a predicted
+    stationary state, is not built)."""
+
+    def __init__(self, domain: FixedDomain, *, rate: float, lowest_shell: int = 1, highest_shell=2):
+        band = ConstantPower(domain, injection_rate=0.0, lowest_shell=lowest_shell, highest_shell=highest_shell)
+        rate = float(domain.element_type(rate))
+        if not math.isfinite(rate):
+            raise ValueError(f"the Linear `rate` must be finite, got {rate}")
+        self.rate = rate
+        self.lowest_shell = band.lowest_shell
+        self.highest_shell = band.highest_shell
+
+    def __repr__(self) -> str:
+        return f"Linear(rate {self.rate}, shells {self.lowest_shell}-{self.highest_shell})"
+
+
 class FlowConditions:
     """nu, Sc and the two forcings (src/Generator.jl:81-147), with the reference's guards."""
 
@@ -134,8 +154,8 @@ class FlowConditions:
         self.scalar_forcing = scalar_forcing if scalar_forcing is not None else NoForcing()
         if not isinstance(self.scalar_forcing, NoForcing):
             raise NotImplementedError("scalar forcings other than NoForcing are not built")
-        if not isinstance(self.velocity_forcing, (NoForcing, ConstantPower)):
-            raise NotImplementedError("velocity forcings other than NoForcing and ConstantPower are not built")
+        if not isinstance(self.velocity_forcing, (NoForcing, ConstantPower, Linear)):
+            raise NotImplementedError("velocity forcings other than NoForcing, ConstantPower and Linear are not built")
 
     @property
     def scalar_diffusivity(self) -> float:
@@ -276,9 +296,9 @@ def _config(domain: FixedDomain, conditions: FlowConditions, scheme, carries_sca
     cfg.viscosity = conditions.viscosity
     cfg.schmidt = conditions.schmidt
     f = conditions.velocity_forcing
-    if isinstance(f, ConstantPower):
-        cfg.velocity_forcing = L.CONSTANT_POWER
-        cfg.injection_rate = f.injection_rate
+    if isinstance(f, (ConstantPower, Linear)):
+        cfg.velocity_forcing = L.CONSTANT_POWER if isinstance(f, ConstantPower) else L.LINEAR
+        cfg.injection_rate = f.injection_rate if isinstance(f, ConstantPower) else f.rate
         cfg.lowest_shell = f.lowest_shell
         cfg.highest_shell = -1 if f.highest_shell == math.inf else f.highest_shell
     cfg.mean_gradient = float(mean_gradient)

@@ -285,9 +285,10 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
     return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
                 "`schmidt` must be positive: the scalar diffusivity is nu/Sc, so a non-positive Schmidt number makes "
                 "it infinite or negative");
-  if (k.velocity_forcing != HDNS_NO_FORCING && k.velocity_forcing != HDNS_CONSTANT_POWER)
-    return fail(nullptr, HDNS_ERR_UNSUPPORTED, "velocity_forcing must be NoForcing or ConstantPower");
-  if (k.velocity_forcing == HDNS_CONSTANT_POWER) {  // src/Forcing.jl:388-409
+  if (k.velocity_forcing != HDNS_NO_FORCING && k.velocity_forcing != HDNS_CONSTANT_POWER &&
+      k.velocity_forcing != HDNS_LINEAR)
+    return fail(nullptr, HDNS_ERR_UNSUPPORTED, "velocity_forcing must be NoForcing, ConstantPower or Linear");
+  if (k.velocity_forcing != HDNS_NO_FORCING) {  // src/Forcing.jl:388-409
     if (k.lowest_shell < 1)
       return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
                   "`lowest_shell` must be at least 1: shell 0 is the mean (kappa = 0) mode, which a band forcing "
@@ -297,7 +298,8 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
                   "`highest_shell` must be at least `lowest_shell`: the band would be empty and the forcing would "
                   "silently do nothing");
     if (!isfinite(k.injection_rate))
-      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "injection_rate must be finite");
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                  k.velocity_forcing == HDNS_LINEAR ? "the Linear rate must be finite" : "injection_rate must be finite");
   }
   if (!isfinite(k.mean_gradient)) return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "mean_gradient must be finite");
   if (k.tracer_capacity < 0) return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "tracer_capacity must be >= 0");
@@ -337,7 +339,7 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
   const size_t realb = c.fp64 ? 8 : 4;
   c.comp = (size_t)c.geo.k * c.geo.k * c.geo.kxp;
   c.stages = k.scheme == HDNS_IF_LOW_STORAGE_RK4 ? 5 : 3;
-  c.band.on = k.velocity_forcing == HDNS_CONSTANT_POWER;
+  c.band.on = k.velocity_forcing != HDNS_NO_FORCING;
   c.band.lo4 = (2 * k.lowest_shell - 1) * (2 * k.lowest_shell - 1);
   c.band.hi4 = k.highest_shell < 0 ? -1 : (2 * k.highest_shell + 1) * (2 * k.highest_shell + 1);
   const size_t ncf = k.has_scalar ? 6 : 3, nci = k.has_scalar ? 7 : 6;
@@ -648,6 +650,7 @@ void launch_step(Ctx &c, double dt, const StageRecord *rec = nullptr) {
     sa.mean_gradient = c.cfg.mean_gradient;
     sa.band = c.band;
     sa.scalars = c.scalars;
+    sa.lin_rate = c.cfg.velocity_forcing == HDNS_LINEAR ? c.cfg.injection_rate : 0.0;
     XArgs xa = last ? fin : mid;
     if (rec) {
       const size_t cb = c.comp * c.elem;
@@ -1221,6 +1224,8 @@ hdns_status adj_alloc(hdns_ctx *ctx) {
   AdjBufs &a = c.adj;
   if (a.ready) return HDNS_OK;
   if (c.nbox != 1) return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint runs on a single-box context");
+  if (c.cfg.velocity_forcing == HDNS_LINEAR)
+    return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint supports NoForcing and ConstantPower velocity forcing");
   const size_t cb = c.comp * c.elem, realb = c.fp64 ? 8 : 4;
   const size_t vol = (size_t)c.geo.n * c.geo.n * c.geo.n;
   hdns_status s = HDNS_OK;

@@ -14,7 +14,7 @@ namespace hdns {
 __host__ __device__ __forceinline__ bool in_two_thirds(int m2, int n) { return 9 * m2 <= n * n; }
 
 struct BandSpec {
-  int on;      // ConstantPower present
+  int on;      // a band forcing present (ConstantPower or Linear)
   int lo4;     // 4|m|^2 >= lo4 = (2*lowest - 1)^2
   int hi4;     // 4|m|^2 <  hi4 = (2*highest + 1)^2, or -1 for no upper edge
 };
@@ -35,6 +35,7 @@ struct StageArgs {
   double mean_gradient;
   BandSpec band;
   const double *scalars;  // per box [8]: [0] = the ConstantPower amplitude at this stage's state
+  double lin_rate;        // Linear forcing: the rate A folded into the band's exponent, else 0
 };
 
 // Radix of the first and the last pass of a length-N transform (fft_io's plans).
@@ -273,11 +274,14 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
         } else {
           nl[0] = nl[1] = nl[2] = czero<T>();
         }
-        if (in_band(m2, sa.band)) {
+        const bool banded = in_band(m2, sa.band);
+        if (banded) {
 #pragma unroll
           for (int c = 0; c < 3; ++c) nl[c] = nl[c] + scale(u[i][c], A);
         }
-        const T E = T(exp(coef_u * k2d));
+        // Linear forcing rides the integrating factor: exp((A - nu |k|^2) d_i dt) in the band
+        // (src/Generator.jl:565-581, src/Forcing.jl:600-640)
+        const T E = T(exp(coef_u * k2d + (banded ? sa.lin_rate * sa.sub : 0.0)));
         cx<T> un[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {

@@ -35,7 +35,7 @@ typedef enum {
 enum { HDNS_FP32_BULK = 0, HDNS_FP64 = 1 };
 enum { HDNS_IF_LOW_STORAGE_RK3 = 3, HDNS_IF_LOW_STORAGE_RK4 = 4 };
 enum { HDNS_TWO_THIRDS = 0 };
-enum { HDNS_NO_FORCING = 0, HDNS_CONSTANT_POWER = 1 };
+enum { HDNS_NO_FORCING = 0, HDNS_CONSTANT_POWER = 1, HDNS_LINEAR = 2 };
 enum { HDNS_CATMULL_ROM = 0 };
 enum { HDNS_HEUN = 0 };
 enum { HDNS_FIELD_VELOCITY = 0, HDNS_FIELD_SCALAR = 1, HDNS_FIELD_VORTICITY = 2 };
@@ -53,8 +53,8 @@ typedef struct {
   int32_t has_scalar;        /* evolve the passive scalar */
   double viscosity;          /* nu > 0 */
   double schmidt;            /* Sc > 0 */
-  int32_t velocity_forcing;  /* HDNS_NO_FORCING or HDNS_CONSTANT_POWER */
-  double injection_rate;     /* ConstantPower epsilon */
+  int32_t velocity_forcing;  /* HDNS_NO_FORCING, HDNS_CONSTANT_POWER or HDNS_LINEAR */
+  double injection_rate;     /* ConstantPower epsilon, or the Linear rate A (src/Forcing.jl:600-642) */
   int32_t lowest_shell;      /* >= 1 */
   int32_t highest_shell;     /* >= lowest_shell; -1 means Inf (every mode but the mean) */
   double mean_gradient;      /* G of an imposed mean scalar gradient along z (extension) */

@@ -192,11 +192,23 @@ class Conditions:
     schmidt: float = 1.0
     forcing: tuple | None = None
     mean_gradient: float = 0.0
+    linear: tuple | None = None       # Linear forcing (rate A, lowest_shell, highest_shell)
     band_eps: float = float(np.finfo(np.float64).eps)
 
     @property
     def scalar_diffusivity(self) -> float:
         return self.viscosity / self.schmidt
+
+
+def linear_factor(d: Domain, cond: Conditions, sub: float) -> np.ndarray:
+    """The velocity's integrating factor over a substep: exp(-nu |k|^2 sub), with a Linear forcing's
+    rate A folded in on its band, exp((A - nu |k|^2) sub) (src/Generator.jl:565-581,
+    src/Forcing.jl:600-642)."""
+    expo = -cond.viscosity * sub * d.k2
+    if cond.linear is not None:
+        rate, lo, hi = cond.linear
+        expo = expo + rate * sub * d.shell_band(lo, hi)
+    return np.exp(expo)
 
 
 def band_is_empty(band_value: float, whole_value: float, eps: float) -> bool:
@@ -335,7 +347,6 @@ def step(d: Domain, cond: Conditions, scheme: str, uh: np.ndarray, ph: np.ndarra
         dep = deposit(d, uh, ph)
     hu = None
     hp = None
-    nu = cond.viscosity
     kap = cond.scalar_diffusivity
     for i in range(S):
         if i == 0:
@@ -348,7 +359,7 @@ def step(d: Domain, cond: Conditions, scheme: str, uh: np.ndarray, ph: np.ndarra
         np_hat = scalar_rhs(d, cond, uh, u, phi) if ph is not None else None
         hu = dt * nu_hat if i == 0 else a[i] * hu + dt * nu_hat
         sub = dd[i] * dt
-        eu = np.exp(-nu * sub * d.k2)
+        eu = linear_factor(d, cond, sub)
         uh = eu * (uh + b[i] * hu)
         if i < S - 1:
             hu = eu * hu

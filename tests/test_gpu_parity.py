@@ -458,3 +458,47 @@ def test_device_reproduces_the_golden_fixtures(g, name, dtype, tol):
                 got = [r["kinetic_energy"], r["dissipation"], r["scalar_variance"], r["scalar_dissipation"]]
                 np.testing.assert_allclose(got, z[f"reductions{step}"], rtol=10 * tol)
                 assert sim.max_speed() == pytest.approx(float(z[f"max_speed{step}"]), rel=10 * tol)
+
+
+@pytest.mark.parametrize("highest", [2, math.inf])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 1e-5)])
+def test_linear_forcing_matches_the_oracle(g, highest, dtype, tol):
+    """Linear forcing (rate folded into the band's integrating factor), 10 IF-RK4 steps with the
+    scalar, against the oracle; checked after 1 and 10 steps."""
+    n, A, dt = 32, 0.4, 2e-3
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
+    cond = g.FlowConditions(viscosity=0.02, schmidt=1.0,
+                            velocity_forcing=g.Linear(dom, rate=A, lowest_shell=1, highest_shell=highest))
+    u, p = g.random_phase_velocity(n, seed=21), g.random_phase_scalar(n, seed=22)
+    oc = dns.Conditions(viscosity=0.02, schmidt=1.0, mean_gradient=1.0, linear=(float(dtype(A)), 1, highest))
+    d = dns.Domain(n)
+    uo, po, dep = u, p, None
+    with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=1.0) as sim:
+        for s in range(1, 11):
+            sim.advance(dt)
+            uo, po, dep = dns.step(d, oc, "rk4", uo, po, dt, dep)
+            if s in (1, 10):
+                st = sim.state()
+                assert rel(st.velocity_spectral, uo) < tol
+                assert rel(st.scalar_spectral, po) < tol
+
+
+def test_linear_forcing_grows_a_shear_wave_exactly_on_the_device(g):
+    """u_x = 0.8 sin y (shell 1, forced at A) + 0.5 sin 3z (shell 3): no nonlinear term, so the two
+    waves evolve as exp((A - nu) t) and exp(-9 nu t) exactly."""
+    n, nu, A, dt, steps = 32, 0.05, 0.3, 1e-2, 25
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
+    x = np.arange(n) * L / n
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    u = np.zeros((3, n, n, n))
+    u[0] = 0.8 * np.sin(Y) + 0.5 * np.sin(3 * Z)
+    uh = g.truncate_to_band(dns.rfft3(u), dom)
+    cond = g.FlowConditions(viscosity=nu, schmidt=1.0, velocity_forcing=g.Linear(dom, rate=A))
+    d = dns.Domain(n)
+    band = d.shell_band(1, 2)
+    expected = uh * np.where(band, np.exp((A - nu * d.k2) * steps * dt), np.exp(-nu * d.k2 * steps * dt))
+    with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(uh, None)) as sim:
+        for _ in range(steps):
+            sim.advance(dt)
+        got = sim.state().velocity_spectral
+    assert np.max(np.abs(got - expected)) <= 1e-12 * np.max(np.abs(expected))

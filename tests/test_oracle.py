@@ -294,3 +294,24 @@ def test_scalar_variance_budget():
     v1 = dns.reductions(d, cond, u1, p1)["scalar_variance"]
     mid = 0.5 * (rate(uh, ph) + rate(u1, p1))
     assert (v1 - v0) / dt == pytest.approx(mid, rel=1e-5)
+
+
+def test_linear_forcing_grows_band_modes_at_a_minus_nu_k2_exactly():
+    """Linear forcing folded into the integrating factor (src/Forcing.jl:600-642): the shear waves
+    u_x = a sin y (shell 1, forced) + b sin 3z (shell 3, not forced) carry no nonlinear term, so each
+    evolves exactly, at exp((A - nu) t) and exp(-9 nu t)."""
+    n, nu, A, dt, steps = 16, 0.05, 0.3, 1e-2, 25
+    d = dns.Domain(n)
+    x = np.arange(n) * 2 * math.pi / n
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    u = np.zeros((3, n, n, n))
+    u[0] = 0.8 * np.sin(Y) + 0.5 * np.sin(3 * Z)
+    uh = dns.rfft3(u) * d.mask
+    cond = dns.Conditions(viscosity=nu, linear=(A, 1, 2))
+    band = d.shell_band(1, 2)
+    expected = uh * np.where(band, np.exp((A - nu * d.k2) * steps * dt), np.exp(-nu * d.k2 * steps * dt))
+    for scheme in ("rk3", "rk4"):
+        v = uh
+        for _ in range(steps):
+            v, _, _ = dns.step(d, cond, scheme, v, None, dt)
+        assert np.max(np.abs(v - expected)) <= 1e-13 * np.max(np.abs(expected))
