@@ -51,18 +51,20 @@ def hip():
     return h
 
 
-def algorithmic_bytes(n: int, real_bytes: int, stages: int, scalar: bool = True) -> dict:
+def algorithmic_bytes(n: int, real_bytes: int, stages: int, scalar: bool = True, ssp: bool = False) -> dict:
     """Compulsory HBM bytes per launch of each kernel, counting retained modes only (kx <= M)."""
     m = n // 3
     k = 2 * m + 1
     kx = m + 1
     cb = 2 * real_bytes
     nf, ni, ns = (6, 7, 4) if scalar else (3, 6, 3)
-    acc_frac = (stages - 1) / stages            # accumulators: not read at stage 1, not written at the last
+    # accumulator passes per stage: the 2N schemes neither read it at the first stage nor write it at
+    # the last; IFSSPRK3 writes the initial state once and reads it at the two later stages
+    acc = 1.0 if ssp else 2 * (stages - 1) / stages
     comp = k * k * kx
     return {
         "y_forward": nf * n * kx * (n + k) * cb,
-        "z_forward_stage_update": (nf * n * k * kx + 2 * ns * comp + 2 * ns * comp * acc_frac) * cb,
+        "z_forward_stage_update": (nf * n * k * kx + 2 * ns * comp + ns * comp * acc) * cb,
         "z_inverse_reductions": (ns * comp + ni * n * k * kx) * cb,
         "y_inverse": ni * n * kx * (k + n) * cb,
         "x_pass_products": (ni + nf) * n * n * kx * cb,
@@ -71,7 +73,7 @@ def algorithmic_bytes(n: int, real_bytes: int, stages: int, scalar: bool = True)
     }
 
 
-def time_gpu(g, n: int, dtype, steps: int, warmup: int, H, seed: int = 2026):
+def time_gpu(g, n: int, dtype, steps: int, warmup: int, H, seed: int = 2026, scheme=None):
     dom = g.FixedDomain(grid_points_per_axis=n, side_length=2 * math.pi, dealiasing=g.TwoThirds(), element_type=dtype)
     nu = g.predicted_viscosity(dom, 0.1)
     u = g.random_phase_velocity(n, seed=seed)
@@ -80,7 +82,7 @@ def time_gpu(g, n: int, dtype, steps: int, warmup: int, H, seed: int = 2026):
                             velocity_forcing=g.ConstantPower(dom, injection_rate=0.1, lowest_shell=1, highest_shell=2))
     dt = 1e-3
     with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=1.0,
-                      field_integration_scheme=g.IFLowStorageRK4, keep_physical=False) as sim:
+                      field_integration_scheme=scheme or g.IFLowStorageRK4, keep_physical=False) as sim:
         del u, p
         tuned = sim.autotune(reps=2)
         for _ in range(warmup):
@@ -355,8 +357,8 @@ def copy_calibration(H, nbytes: int = 1 << 30) -> None:
     H.hipFree(b)
 
 
-def kernel_table(res: dict, n: int, real_bytes: int, stages: int) -> dict:
-    ab = algorithmic_bytes(n, real_bytes, stages)
+def kernel_table(res: dict, n: int, real_bytes: int, stages: int, ssp: bool = False) -> dict:
+    ab = algorithmic_bytes(n, real_bytes, stages, ssp=ssp)
     out = {}
     for i, name in enumerate(KIND_NAMES[:7]):
         cnt = int(res["kernel_launches"][i])
@@ -414,6 +416,8 @@ def main() -> int:
     ap.add_argument("--no-iterations", action="store_true",
                     help="skip the whole-iteration figures (256^3 FP64 + 1M tracers, 1024^3 FP32-bulk + 16M tracers)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP32-bulk vs FP64 error report")
+    ap.add_argument("--no-schemes", action="store_true",
+                    help="skip the other field schemes (IFLowStorageRK3, IFSSPRK3) on the same workload")
     ap.add_argument("--adjoint-directions", type=int, default=16,
                     help="finite-difference directions for the adjoint check (0: time only)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -457,6 +461,16 @@ def main() -> int:
                 roofline["traffic_source"] = "rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated: " + t["kernel"]
         except (ValueError, KeyError):
             pass
+    schemes = None
+    if not args.no_schemes:
+        print(f"bench: {n}^3 FP32-bulk, IFLowStorageRK3 and IFSSPRK3", file=sys.stderr, flush=True)
+        schemes = {}
+        for name in ("IFLowStorageRK3", "IFSSPRK3"):
+            k = max(5, args.steps // 2)
+            r = time_gpu(g, n, np.float32, k, 1, H, scheme=getattr(g, name))
+            ms = r["ms_total"] / k
+            schemes[name] = {"value": round(n ** 3 / (ms * 1e-3) / 1e6, 2), "ms_per_step": round(ms, 3),
+                             "steps": k, "kernels": kernel_table(r, n, 4, 3, ssp=name == "IFSSPRK3")}
     tracers = None
     if not args.no_tracers:
         print("bench: tracers, 256^3 field, 2^20 tracers", file=sys.stderr, flush=True)
@@ -496,7 +510,7 @@ def main() -> int:
         "config": {"workload": f"{n}^3 forced isotropic turbulence + passive scalar with mean gradient, IF-RK4, "
                                f"dt 1e-3, ConstantPower eps 0.1 shells 1-2, nu {r32['nu']:.4e}, tracers off",
                    "grid": n, "scheme": "IFLowStorageRK4", "parallelism": "single GPU"},
-        "roofline": roofline, "cpu_baseline": cpu, "fp64": fp64, "kernels": k32,
+        "roofline": roofline, "cpu_baseline": cpu, "fp64": fp64, "kernels": k32, "schemes": schemes,
         "wall_ms_per_step": round(r32["wall_s"] * 1e3 / args.steps, 3),
         "memory_GiB": round(r32["memory_bytes"] / 2 ** 30, 2),
         "autotuned_schedule": r32["schedule"],

@@ -182,6 +182,13 @@ class IFLowStorageRK4:
     base_cfl = 0.96
 
 
+class IFSSPRK3:
+    """Shu-Osher three-stage SSP IF scheme (src/Propagator.jl:547-691); its three-stage third-order
+    stability function shares IFLowStorageRK3's ceiling (:911)."""
+    code = L.SSPRK3
+    base_cfl = 0.5
+
+
 def recommended_cfl(scheme, domain: FixedDomain) -> float:
     """The advective CFL ceiling (src/Propagator.jl:906-919); TwoThirds leaves the base unscaled."""
     return scheme.base_cfl

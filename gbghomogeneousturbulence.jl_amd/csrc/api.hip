@@ -62,6 +62,10 @@ struct Coeffs {
 // rationals rounded to the element type; d are differences of the rounded abscissae formed in T.
 Coeffs scheme_coefficients(int scheme, bool fp32) {
   Coeffs k{};
+  if (scheme == HDNS_IF_SSP_RK3) {   // not a 2N scheme: its stages are set by ssp_stage
+    k.stages = 3;
+    return k;
+  }
   if (scheme == HDNS_IF_LOW_STORAGE_RK3) {
     k.stages = 3;
     const long double an[3][2] = {{0, 1}, {-5, 9}, {-153, 128}};
@@ -272,8 +276,8 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
     return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "side_length must be positive and finite");
   if (k.precision != HDNS_FP32_BULK && k.precision != HDNS_FP64)
     return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "precision must be HDNS_FP32_BULK or HDNS_FP64");
-  if (k.scheme != HDNS_IF_LOW_STORAGE_RK3 && k.scheme != HDNS_IF_LOW_STORAGE_RK4)
-    return fail(nullptr, HDNS_ERR_UNSUPPORTED, "scheme must be IFLowStorageRK3 or IFLowStorageRK4");
+  if (k.scheme != HDNS_IF_LOW_STORAGE_RK3 && k.scheme != HDNS_IF_LOW_STORAGE_RK4 && k.scheme != HDNS_IF_SSP_RK3)
+    return fail(nullptr, HDNS_ERR_UNSUPPORTED, "scheme must be IFLowStorageRK3, IFLowStorageRK4 or IFSSPRK3");
   if (k.dealiasing != HDNS_TWO_THIRDS) return fail(nullptr, HDNS_ERR_UNSUPPORTED, "only TwoThirds dealiasing is built");
   // src/Generator.jl:107-112
   for (int b = 0; b < nbox; ++b)
@@ -618,6 +622,21 @@ struct StageRecord {
   void *physw[5];
 };
 
+// IFSSPRK3's stage i (src/Propagator.jl:599-669) in the form q <- cb E^pb (q + dt N) + E^pa (ca h),
+// E = exp((A - nu |k|^2) dt / 2) and h the step's initial state, which stage 0 saves:
+//   q1 = E^2 (u + dt N);  q2 = 1/4 E^-1 (q1 + dt N) + E (3/4 u);  u' = 2/3 E (q2 + dt N) + E^2 (1/3 u).
+// The powers are evaluated as exp(p x), never as products of E.
+void ssp_stage(StageArgs &sa, int i, double dt, bool fp32) {
+  static const int pa[3] = {0, 1, 2}, pb[3] = {2, -1, 1};
+  static const long double ca[3][2] = {{0, 1}, {3, 4}, {1, 3}}, cb[3][2] = {{1, 1}, {1, 4}, {2, 3}};
+  sa.ssp = 1;
+  sa.sub = dt / 2;
+  sa.pa = pa[i];
+  sa.pb = pb[i];
+  sa.ca = round_t(ca[i][0], ca[i][1], fp32);
+  sa.cb = round_t(cb[i][0], cb[i][1], fp32);
+}
+
 void launch_step(Ctx &c, double dt, const StageRecord *rec = nullptr) {
   const bool fp32 = !c.fp64;
   const Coeffs k = scheme_coefficients(c.cfg.scheme, fp32);
@@ -651,6 +670,7 @@ void launch_step(Ctx &c, double dt, const StageRecord *rec = nullptr) {
     sa.band = c.band;
     sa.scalars = c.scalars;
     sa.lin_rate = c.cfg.velocity_forcing == HDNS_LINEAR ? c.cfg.injection_rate : 0.0;
+    if (c.cfg.scheme == HDNS_IF_SSP_RK3) ssp_stage(sa, i, dt, fp32);
     XArgs xa = last ? fin : mid;
     if (rec) {
       const size_t cb = c.comp * c.elem;
@@ -1226,6 +1246,8 @@ hdns_status adj_alloc(hdns_ctx *ctx) {
   if (c.nbox != 1) return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint runs on a single-box context");
   if (c.cfg.velocity_forcing == HDNS_LINEAR)
     return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint supports NoForcing and ConstantPower velocity forcing");
+  if (c.cfg.scheme == HDNS_IF_SSP_RK3)
+    return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint supports the low-storage schemes IFLowStorageRK3 and IFLowStorageRK4");
   const size_t cb = c.comp * c.elem, realb = c.fp64 ? 8 : 4;
   const size_t vol = (size_t)c.geo.n * c.geo.n * c.geo.n;
   hdns_status s = HDNS_OK;

@@ -85,6 +85,8 @@ struct PipelineImpl final : Pipeline {
       constexpr int ZN = ZF_NT[0];
       set_lds((const void *)k_zs<T, N, ZTX, ZN>, LDS_Z);
       set_lds((const void *)k_zf<T, N, ZTX, ZN, false, ZFL>, zf_lds_bytes<N, ZTX, T>(ZFL));
+      set_lds((const void *)k_zs<T, N, ZTX, ZN, true>, LDS_Z);
+      set_lds((const void *)k_zf<T, N, ZTX, ZN, false, ZFL, true>, zf_lds_bytes<N, ZTX, T>(ZFL));
     } else {
       constexpr int ZN = ZF_NT[1];
       set_lds((const void *)k_zf<T, N, ZTX, ZN, true>, LDS_Z);
@@ -189,7 +191,8 @@ struct PipelineImpl final : Pipeline {
     else c.sched[0] ? yfwd_v<YT[1], 0>(c) : yfwd_v<YT[0], 0>(c);
   }
 
-  template <int V>
+  // SSP: the IFSSPRK3 stage update, built for schedule 0 only (the scheme always runs it)
+  template <int V, bool SSP = false>
   void zf_v(Ctx &c, const StageArgs &sa) {
     constexpr int ZN = ZF_NT[V];
     const dim3 grid(c.geo.kxp / ZTX, K, c.nbox);
@@ -198,10 +201,10 @@ struct PipelineImpl final : Pipeline {
     const cx<double> *tw = (const cx<double> *)c.tw;
     if constexpr (V == 0) {
       if (c.cfg.has_scalar)   // the scalar half reads the stage u_z before the velocity half updates it
-        hipLaunchKernelGGL((k_zs<T, N, ZTX, ZN>), grid, dim3(ZN), LDS_Z, c.stream, pf, (const cx<T> *)uh, ph, hp,
+        hipLaunchKernelGGL((k_zs<T, N, ZTX, ZN, SSP>), grid, dim3(ZN), LDS_Z, c.stream, pf, (const cx<T> *)uh, ph, hp,
                            c.geo, tw, sa);
-      hipLaunchKernelGGL((k_zf<T, N, ZTX, ZN, false, ZFL>), grid, dim3(ZN), (zf_lds_bytes<N, ZTX, T>(ZFL)), c.stream,
-                         pf, uh, hu, ph, hp, c.geo, tw, sa);
+      hipLaunchKernelGGL((k_zf<T, N, ZTX, ZN, false, ZFL, SSP>), grid, dim3(ZN), (zf_lds_bytes<N, ZTX, T>(ZFL)),
+                         c.stream, pf, uh, hu, ph, hp, c.geo, tw, sa);
     } else {
       if (c.cfg.has_scalar)
         hipLaunchKernelGGL((k_zf<T, N, ZTX, ZN, true>), grid, dim3(ZN), LDS_Z, c.stream, pf, uh, hu, ph, hp, c.geo,
@@ -211,7 +214,10 @@ struct PipelineImpl final : Pipeline {
                            tw, sa);
     }
   }
-  void zf(Ctx &c, const StageArgs &sa) override { c.sched[1] ? zf_v<1>(c, sa) : zf_v<0>(c, sa); }
+  void zf(Ctx &c, const StageArgs &sa) override {
+    if (sa.ssp) zf_v<0, true>(c, sa);
+    else c.sched[1] ? zf_v<1>(c, sa) : zf_v<0>(c, sa);
+  }
 
   void reality(Ctx &c) override {
     const int n = K * K;

@@ -36,6 +36,12 @@ struct StageArgs {
   BandSpec band;
   const double *scalars;  // per box [8]: [0] = the ConstantPower amplitude at this stage's state
   double lin_rate;        // Linear forcing: the rate A folded into the band's exponent, else 0
+  // IFSSPRK3 (src/Propagator.jl:599-691), read by the kernels built with SSP:
+  // q <- cb E^pb (q + dt N) + E^pa (ca h), with E the factor over the half step sub = dt / 2 and
+  // h the step's initial state, which stage 0 saves
+  int ssp;
+  int pa, pb;
+  double ca, cb;          // the Shu-Osher weights, rounded to the element type
 };
 
 // Radix of the first and the last pass of a length-N transform (fft_io's plans).
@@ -47,7 +53,7 @@ constexpr int last_radix() { return N % 3 == 0 ? 3 : ((N == 64 || N == 512) ? 8 
 // The scalar half of a stage (src/Generator.jl:648-730, src/Propagator.jl:366-391): the three
 // fluxes' forward z transforms, N_phi = -mask i k . F - G u_z at the stage velocity, and the
 // phi / h_phi update.  It runs before the velocity update, which overwrites u_z.
-template <typename T, int N, int TX, int NT>
+template <typename T, int N, int TX, int NT, bool SSP>
 __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, const cx<T> *__restrict__ pf,
                                                const cx<T> *__restrict__ uh, cx<T> *__restrict__ ph,
                                                cx<T> *__restrict__ hp, const Geo &g, const StageArgs &sa, int kx0,
@@ -122,6 +128,16 @@ __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, con
         const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
         const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
         const cx<T> nl = cx<T>{G[p][q].im, -G[p][q].re} - scale(uz[i], Gm);   // -i k.F - G u_z
+        if constexpr (SSP) {
+          const cx<T> t = scale(pv[i] + scale(nl, dt), T(exp(sa.pb * coef_p * k2d)));
+          if (sa.stage == 0) {
+            ph[at[i]] = t;
+            hp[at[i]] = pv[i];
+          } else {
+            ph[at[i]] = scale(t, T(sa.cb)) + scale(scale(hv[i], T(sa.ca)), T(exp(sa.pa * coef_p * k2d)));
+          }
+          continue;
+        }
         const T E = T(exp(coef_p * k2d));
         cx<T> h = scale(nl, dt);
         if (sa.stage > 0) h = scale(hv[i], a) + h;
@@ -138,7 +154,8 @@ __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, con
 //                                                                      src/Forcing.jl:1104-1117)
 //   N_phi = -mask i k . FFT_z(PF_flux) - G u_z                        (src/Generator.jl:648-730)
 //   h <- a h + dt N;  q <- E (q + b h);  h <- E h  (not at the last)  (src/Propagator.jl:366-391)
-//   and at the last stage the state Leray projection                  (src/Propagator.jl:401).
+//   or, for IFSSPRK3, q <- cb E^pb (q + dt N) + E^pa (ca h), h = u_n   (src/Propagator.jl:599-669)
+//   and at the last stage the state Leray projection                  (src/Propagator.jl:401, :677).
 // E = exp(coef |k|^2) is evaluated in FP64 per mode and rounded to T.  A thread owns the modes
 // its last transform pass produces: kz bins n = j + q N/R of column col, for its butterflies
 // (col, j); the product tiles stream from HBM straight into the first pass.  FL: the first two
@@ -150,7 +167,7 @@ constexpr size_t zf_lds_bytes(bool fl) {
   return ((size_t)N * TX + N + (fl ? 2 * (size_t)(2 * (N / 3) + 1) * TX : 0)) * sizeof(cx<T>);
 }
 
-template <typename T, int N, int TX, int NT, bool SCALAR, bool FL = false>
+template <typename T, int N, int TX, int NT, bool SCALAR, bool FL = false, bool SSP = false>
 __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *__restrict__ uh,
                                            cx<T> *__restrict__ hu, cx<T> *__restrict__ ph,
                                            cx<T> *__restrict__ hp, Geo g, const cx<double> *__restrict__ tw,
@@ -181,7 +198,7 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
     hp += box * comp;
   }
 
-  if constexpr (SCALAR) zf_scalar_part<T, N, TX, NT>(buf, twl, pf, uh, ph, hp, g, sa, kx0, ky_idx, my, box);
+  if constexpr (SCALAR) zf_scalar_part<T, N, TX, NT, SSP>(buf, twl, pf, uh, ph, hp, g, sa, kx0, ky_idx, my, box);
 
   // velocity products, forward z transform, straight into the owned modes.  The compiler
   // memory barrier keeps the scalar part's u_z loads from living on into the velocity part.
@@ -281,14 +298,29 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
         }
         // Linear forcing rides the integrating factor: exp((A - nu |k|^2) d_i dt) in the band
         // (src/Generator.jl:565-581, src/Forcing.jl:600-640)
-        const T E = T(exp(coef_u * k2d + (banded ? sa.lin_rate * sa.sub : 0.0)));
+        const double xe = coef_u * k2d + (banded ? sa.lin_rate * sa.sub : 0.0);
         cx<T> un[3];
+        if constexpr (SSP) {
+          const T Eb = T(exp(sa.pb * xe)), Ea = sa.stage > 0 ? T(exp(sa.pa * xe)) : T(0);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          cx<T> h = scale(nl[c], dt);
-          if (sa.stage > 0) h = scale(hv[i][c], a) + h;
-          un[c] = scale(u[i][c] + scale(h, b), E);
-          if (!sa.last) hu[c * comp + at[i]] = scale(h, E);
+          for (int c = 0; c < 3; ++c) {
+            const cx<T> t = scale(u[i][c] + scale(nl[c], dt), Eb);
+            if (sa.stage == 0) {
+              un[c] = t;
+              hu[c * comp + at[i]] = u[i][c];
+            } else {
+              un[c] = scale(t, T(sa.cb)) + scale(scale(hv[i][c], T(sa.ca)), Ea);
+            }
+          }
+        } else {
+          const T E = T(exp(xe));
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            cx<T> h = scale(nl[c], dt);
+            if (sa.stage > 0) h = scale(hv[i][c], a) + h;
+            un[c] = scale(u[i][c] + scale(h, b), E);
+            if (!sa.last) hu[c * comp + at[i]] = scale(h, E);
+          }
         }
         if (sa.last && m2 > 0) {  // helmholtz_projection!(u, u): P without the mask
           const T k2 = T(k2d);
@@ -310,7 +342,7 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
 
 // The scalar half on its own (the velocity half then runs as k_zf without SCALAR): two smaller
 // kernels hold fewer registers than the fused one.
-template <typename T, int N, int TX, int NT>
+template <typename T, int N, int TX, int NT, bool SSP = false>
 __global__ __launch_bounds__(NT) void k_zs(const cx<T> *__restrict__ pf, const cx<T> *__restrict__ uh,
                                            cx<T> *__restrict__ ph, cx<T> *__restrict__ hp, Geo g,
                                            const cx<double> *__restrict__ tw, StageArgs sa) {
@@ -322,7 +354,7 @@ __global__ __launch_bounds__(NT) void k_zs(const cx<T> *__restrict__ pf, const c
   if (!in_disk(kx0, my, N)) return;   // no column of this tile meets the ball (its PF lines are unwritten)
   stage_twiddles<T, N, NT>(twl, tw);
   const size_t comp = comp_of(g);
-  zf_scalar_part<T, N, TX, NT>(buf, twl, pf + box * g.box_b, uh + box * 3 * comp, ph + box * comp,
+  zf_scalar_part<T, N, TX, NT, SSP>(buf, twl, pf + box * g.box_b, uh + box * 3 * comp, ph + box * comp,
                                hp + box * comp, g, sa, kx0, ky_idx, my, box);
 }
 

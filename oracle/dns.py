@@ -339,6 +339,8 @@ def step(d: Domain, cond: Conditions, scheme: str, uh: np.ndarray, ph: np.ndarra
     projection of the state (:401) and the reality projection of both fields (:411-412).
     Returns (u_hat, phi_hat, deposit).
     """
+    if scheme == "ssprk3":
+        return ssp_step(d, cond, uh, ph, dt, dep, coeff_dtype)
     a, b, dd = scheme_coefficients(scheme, coeff_dtype)
     S = len(a)
     uh = uh.copy()
@@ -374,6 +376,56 @@ def step(d: Domain, cond: Conditions, scheme: str, uh: np.ndarray, ph: np.ndarra
     if ph is not None:
         ph = reality_projection(d, ph)
     return uh, ph, deposit(d, uh, ph)
+
+
+def ssp_step(d: Domain, cond: Conditions, uh: np.ndarray, ph: np.ndarray | None, dt: float,
+             dep: Deposit | None = None, coeff_dtype=np.float64):
+    """One IFSSPRK3 ``step!`` (Shu-Osher), then the closing deposit (src/Propagator.jl:599-691).
+
+    With E(t) the integrating factor over t and N(q) the right-hand side at the stage state q:
+        q1    = E(dt) (u + dt N(u))
+        q2    = 3/4 E(dt/2) u + 1/4 E(-dt/2) (q1 + dt N(q1))
+        u_new = 1/3 E(dt) u + 2/3 E(dt/2) (q2 + dt N(q2))
+    for u_hat and phi_hat in lockstep (the scalar's factor uses nu/Sc), then the state Leray
+    projection (:677) and the reality projection of both fields (:687-688).  The weights are the
+    element type's roundings of 3/4, 1/4, 1/3, 2/3.
+    """
+    w34, w14, w13, w23 = (_round(Fraction(p, q), coeff_dtype) for p, q in ((3, 4), (1, 4), (1, 3), (2, 3)))
+    if dep is None:
+        dep = deposit(d, uh, ph)
+    kap = cond.scalar_diffusivity
+
+    def factors(t):
+        return linear_factor(d, cond, t), np.exp(-kap * t * d.k2)
+
+    def rhs(q, p, first):
+        if first:
+            u, w, phi = dep.u, dep.w, dep.phi
+        else:
+            u, w = irfft3(q, d.n), irfft3(curl(d, q), d.n)
+            phi = irfft3(p, d.n) if p is not None else None
+        nu_hat = velocity_rhs(d, cond, q, u, w)
+        np_hat = scalar_rhs(d, cond, q, u, phi) if p is not None else None
+        return nu_hat, np_hat
+
+    eu_full, ep_full = factors(dt)
+    eu_half, ep_half = factors(dt / 2)
+    eu_back, ep_back = factors(-dt / 2)
+    n0, m0 = rhs(uh, ph, True)
+    q = eu_full * (uh + dt * n0)
+    p = ep_full * (ph + dt * m0) if ph is not None else None
+    n1, m1 = rhs(q, p, False)
+    q = w14 * (eu_back * (q + dt * n1)) + eu_half * (w34 * uh)
+    if ph is not None:
+        p = w14 * (ep_back * (p + dt * m1)) + ep_half * (w34 * ph)
+    n2, m2 = rhs(q, p, False)
+    q = eu_full * (w13 * uh) + w23 * (eu_half * (q + dt * n2))
+    if ph is not None:
+        p = ep_full * (w13 * ph) + w23 * (ep_half * (p + dt * m2))
+    q = reality_projection(d, leray(d, q))
+    if p is not None:
+        p = reality_projection(d, p)
+    return q, p, deposit(d, q, p)
 
 
 # --- observables --------------------------------------------------------------------------

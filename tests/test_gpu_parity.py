@@ -69,13 +69,17 @@ def test_deposit_matches_the_oracle(g, n, dtype, tol):
         assert np.max(np.abs(e[: len(eo)] - eo)) <= 10 * tol * np.max(eo) + 1e-300
 
 
+def _scheme(g, name):
+    return {"rk3": g.IFLowStorageRK3, "rk4": g.IFLowStorageRK4, "ssprk3": g.IFSSPRK3}[name]
+
+
 @pytest.mark.parametrize("n", [32, 48, 96])
-@pytest.mark.parametrize("scheme", ["rk3", "rk4"])
+@pytest.mark.parametrize("scheme", ["rk3", "rk4", "ssprk3"])
 @pytest.mark.parametrize("forcing", [None, 0.1])
 def test_one_step_fp64_matches_the_oracle(g, n, scheme, forcing):
     """48 and 96 carry radix-3 passes, and 3 | N puts the boundary shell 9|m|^2 = N^2 inside the
     ball (test/regressions/r6-inclusive-ball-exact.jl)."""
-    sch = g.IFLowStorageRK4 if scheme == "rk4" else g.IFLowStorageRK3
+    sch = _scheme(g, scheme)
     sim, u, p, oc = make_sim(g, n, np.float64, scheme=sch, forcing=forcing)
     d = dns.Domain(n)
     with sim:
@@ -497,8 +501,70 @@ def test_linear_forcing_grows_a_shear_wave_exactly_on_the_device(g):
     d = dns.Domain(n)
     band = d.shell_band(1, 2)
     expected = uh * np.where(band, np.exp((A - nu * d.k2) * steps * dt), np.exp(-nu * d.k2 * steps * dt))
-    with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(uh, None)) as sim:
-        for _ in range(steps):
+    for scheme in ("rk4", "ssprk3"):
+        with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(uh, None),
+                          field_integration_scheme=_scheme(g, scheme)) as sim:
+            for _ in range(steps):
+                sim.advance(dt)
+            got = sim.state().velocity_spectral
+        assert np.max(np.abs(got - expected)) <= 1e-12 * np.max(np.abs(expected)), scheme
+
+
+@pytest.mark.parametrize("forcing", ["constant_power", "linear"])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 1e-5)])
+def test_ssprk3_ten_steps_match_the_oracle(g, forcing, dtype, tol):
+    """IFSSPRK3 (src/Propagator.jl:599-691) with the scalar and the mean gradient, 10 steps against
+    the oracle's Shu-Osher stages; checked after 1 and 10 steps."""
+    n, dt, nu = 32, 2e-3, 0.02
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
+    if forcing == "linear":
+        vf = g.Linear(dom, rate=0.4, lowest_shell=1, highest_shell=2)
+        okw = {"linear": (float(dtype(0.4)), 1, 2)}
+    else:
+        vf = g.ConstantPower(dom, injection_rate=0.1, lowest_shell=1, highest_shell=2)
+        okw = {"forcing": (float(dtype(0.1)), 1, 2), "band_eps": float(np.finfo(dtype).eps)}
+    cond = g.FlowConditions(viscosity=nu, schmidt=1.0, velocity_forcing=vf)
+    u, p = g.random_phase_velocity(n, seed=31), g.random_phase_scalar(n, seed=32)
+    oc = dns.Conditions(viscosity=nu, schmidt=1.0, mean_gradient=1.0, **okw)
+    d = dns.Domain(n)
+    uo, po, dep = u, p, None
+    with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=1.0,
+                      field_integration_scheme=g.IFSSPRK3) as sim:
+        for s in range(1, 11):
             sim.advance(dt)
-        got = sim.state().velocity_spectral
-    assert np.max(np.abs(got - expected)) <= 1e-12 * np.max(np.abs(expected))
+            uo, po, dep = dns.step(d, oc, "ssprk3", uo, po, dt, dep)
+            if s in (1, 10):
+                st = sim.state()
+                eu, ep = rel(st.velocity_spectral, uo), rel(st.scalar_spectral, po)
+                print(f"ssprk3 {forcing} {np.dtype(dtype).name} step {s}: u {eu:.2e} phi {ep:.2e}", flush=True)
+                assert eu < tol and ep < tol
+                assert sim.max_speed() == pytest.approx(dep.max_speed, rel=10 * tol)
+
+
+@pytest.mark.parametrize("scalar", [True, False])
+def test_ssprk3_fp32_bulk_at_512_agrees_with_the_fp64_step(g, scalar):
+    """The 512^3 kernels (the LDS-staged z forward variant, the split scalar kernel) on the
+    IFSSPRK3 stages: the FP32-bulk state within 3e-6 of the FP64 state after 3 steps, by the
+    device distance."""
+    n, dt = 512, 1e-3
+    sims = []
+    for dtype in (np.float32, np.float64):
+        sim, *_ = make_sim(g, n, dtype, scheme=g.IFSSPRK3, forcing=0.1, nu=2e-3, scalar=scalar)
+        sims.append(sim)
+    with sims[0], sims[1]:
+        for _ in range(3):
+            sims[0].advance(dt)
+            sims[1].advance(dt)
+        r = sims[0].distance(sims[1])
+        print(f"ssprk3 512 scalar={scalar}:", {k: f"{v:.2e}" for k, v in r.items() if k.endswith("relative")},
+              flush=True)
+        assert r["velocity_relative"] < 3e-6
+        if scalar:
+            assert r["scalar_relative"] < 3e-6
+
+
+def test_the_adjoint_refuses_ssprk3(g):
+    sim, *_ = make_sim(g, 16, np.float64, scheme=g.IFSSPRK3)
+    with sim:
+        with pytest.raises(Exception, match="low-storage"):
+            sim.adjoint_gradient(1e-3, 2)

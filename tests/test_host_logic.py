@@ -29,6 +29,7 @@ def test_recommended_cfl_is_the_base_ceiling_under_two_thirds(g):
     dom = g.FixedDomain(grid_points_per_axis=32, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
     assert g.recommended_cfl(g.IFLowStorageRK4, dom) == 0.96
     assert g.recommended_cfl(g.IFLowStorageRK3, dom) == 0.5
+    assert g.recommended_cfl(g.IFSSPRK3, dom) == 0.5   # src/Propagator.jl:911, the RK3 ceiling
 
 
 def test_cfl_bound_takes_the_smallest_of_its_three_bounds(g):

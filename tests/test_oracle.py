@@ -221,7 +221,7 @@ def _integrate(scheme, n, steps, T, uh, ph, cond):
     return u, p
 
 
-@pytest.mark.parametrize("scheme,lo,hi", [("rk3", 5.0, 12.0), ("rk4", 10.0, 24.0)])
+@pytest.mark.parametrize("scheme,lo,hi", [("rk3", 5.0, 12.0), ("rk4", 10.0, 24.0), ("ssprk3", 5.0, 12.0)])
 def test_temporal_self_convergence(scheme, lo, hi):
     n = 16
     d, uh = _band_limited(n, 1, shell_frac=0.5)
@@ -310,8 +310,77 @@ def test_linear_forcing_grows_band_modes_at_a_minus_nu_k2_exactly():
     cond = dns.Conditions(viscosity=nu, linear=(A, 1, 2))
     band = d.shell_band(1, 2)
     expected = uh * np.where(band, np.exp((A - nu * d.k2) * steps * dt), np.exp(-nu * d.k2 * steps * dt))
-    for scheme in ("rk3", "rk4"):
+    for scheme in ("rk3", "rk4", "ssprk3"):
         v = uh
         for _ in range(steps):
             v, _, _ = dns.step(d, cond, scheme, v, None, dt)
         assert np.max(np.abs(v - expected)) <= 1e-13 * np.max(np.abs(expected))
+
+
+# --- IFSSPRK3 ------------------------------------------------------------------------------------
+
+def _lawson_ssprk3(d, cond, uh, ph, dt):
+    """IFSSPRK3 written independently of the oracle's Shu-Osher arrangement: the classical SSPRK3
+    Butcher tableau (c = 0, 1, 1/2; b = 1/6, 1/6, 2/3) applied to the transformed variable
+    v = E(-tau) q, whose equation is v' = E(-tau) N(E(tau) v); then q = E(dt) v, the state Leray
+    projection and the reality projection."""
+    kap = cond.scalar_diffusivity
+
+    def g(tau, v, w):
+        q = dns.linear_factor(d, cond, tau) * v
+        p = np.exp(-kap * tau * d.k2) * w
+        dep = dns.deposit(d, q, p)
+        nu = dns.velocity_rhs(d, cond, q, dep.u, dep.w)
+        npp = dns.scalar_rhs(d, cond, q, dep.u, dep.phi)
+        return dns.linear_factor(d, cond, -tau) * nu, np.exp(kap * tau * d.k2) * npp
+
+    k1 = g(0.0, uh, ph)
+    k2 = g(dt, uh + dt * k1[0], ph + dt * k1[1])
+    k3 = g(dt / 2, uh + dt / 4 * (k1[0] + k2[0]), ph + dt / 4 * (k1[1] + k2[1]))
+    v = uh + dt / 6 * (k1[0] + k2[0] + 4 * k3[0])
+    w = ph + dt / 6 * (k1[1] + k2[1] + 4 * k3[1])
+    q = dns.linear_factor(d, cond, dt) * v
+    p = np.exp(-kap * dt * d.k2) * w
+    return dns.reality_projection(d, dns.leray(d, q)), dns.reality_projection(d, p)
+
+
+@pytest.mark.parametrize("forcing", ["none", "constant_power", "linear"])
+def test_ssprk3_equals_the_lawson_butcher_form(forcing):
+    """The Shu-Osher stages of src/Propagator.jl:599-691 against the Butcher form of the same
+    scheme on the transformed variable: a swapped weight, a wrong factor sign or a wrong stage
+    time fails this at 1e-13."""
+    n = 16
+    d, uh = _band_limited(n, 4, shell_frac=0.6)
+    _, ph = _band_limited(n, 5, shell_frac=0.6)
+    ph = ph[0]
+    kw = {"constant_power": {"forcing": (0.2, 1, 2)}, "linear": {"linear": (0.4, 1, 2)}}.get(forcing, {})
+    cond = dns.Conditions(viscosity=0.03, schmidt=0.8, mean_gradient=0.7, **kw)
+    dt = 0.05
+    got_u, got_p, _ = dns.step(d, cond, "ssprk3", uh, ph, dt)
+    want_u, want_p = _lawson_ssprk3(d, cond, uh, ph, dt)
+    assert np.max(np.abs(got_u - want_u)) <= 1e-13 * np.max(np.abs(want_u))
+    assert np.max(np.abs(got_p - want_p)) <= 1e-13 * np.max(np.abs(want_p))
+
+
+def test_ssprk3_and_rk3_are_distinct_tableaux_of_one_order():
+    """test/integrators.jl:71-76: the two third-order schemes agree as dt -> 0 while their
+    single-dt states differ."""
+    n = 16
+    d, uh = _band_limited(n, 1, shell_frac=0.5)
+    uh = uh * math.sqrt(0.5 / dns.reductions(d, dns.Conditions(viscosity=1), uh, None)["kinetic_energy"])
+    cond = dns.Conditions(viscosity=0.05)
+    T = 0.4
+    u1, _ = _integrate("rk3", n, 8, T, uh, None, cond)
+    u4, _ = _integrate("rk3", n, 32, T, uh, None, cond)
+    w1, _ = _integrate("ssprk3", n, 8, T, uh, None, cond)
+    w4, _ = _integrate("ssprk3", n, 32, T, uh, None, cond)
+    assert np.linalg.norm(u1 - w1) > 0
+    assert np.linalg.norm(u4 - w4) < np.linalg.norm(u1 - w1)
+
+
+def test_ssprk3_weights_round_to_the_element_type():
+    """T(3//4), T(1//4), T(1//3), T(2//3) (src/Propagator.jl:643-667): exact in FP64 for the
+    quarters, correctly rounded for the thirds."""
+    assert dns._round(Fraction(1, 3), np.float32) == float(np.float32(1 / 3))
+    assert dns._round(Fraction(2, 3), np.float32) == float(np.float32(2 / 3))
+    assert dns._round(Fraction(3, 4), np.float32) == 0.75
