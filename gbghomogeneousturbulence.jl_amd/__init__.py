@@ -6,14 +6,16 @@ The folder name carries a dot, so it is imported by path; ``load_package()`` in
 """
 from . import _lib
 from .api import (ConstantPower, Ensemble, EnsembleBox, EnsembleMember, FixedDomain, FlowConditions, Heun,
-                  IFLowStorageRK3, IFLowStorageRK4, IFSSPRK3, Linear, NoForcing, ResolvedFields, Simulation, Tracers, TwoThirds,
+                  IFLowStorageRK3, IFLowStorageRK4, IFSSPRK3, Linear, NoForcing, VarianceController,
+                  VarianceRescaling, ResolvedFields, Simulation, Tracers, TwoThirds,
                   StationarityCriterion, cfl_bound, integral_length, predicted_viscosity,
                   recommended_cfl, truncate_to_band, warmup)
 from .seeding import random_phase_scalar, random_phase_velocity, uniform_positions
 
 __all__ = [
-    "ConstantPower", "Ensemble", "EnsembleBox", "EnsembleMember", "FixedDomain", "FlowConditions", "Heun", "IFLowStorageRK3", "IFLowStorageRK4",
-    "IFSSPRK3",     "Linear", "NoForcing", "ResolvedFields", "Simulation", "Tracers", "TwoThirds", "predicted_viscosity",
+    "ConstantPower", "Ensemble", "EnsembleBox", "EnsembleMember", "FixedDomain", "FlowConditions", "Heun",
+    "IFLowStorageRK3", "IFLowStorageRK4", "IFSSPRK3", "Linear", "NoForcing", "VarianceController",
+    "VarianceRescaling", "ResolvedFields", "Simulation", "Tracers", "TwoThirds", "predicted_viscosity",
     "recommended_cfl", "cfl_bound", "integral_length", "StationarityCriterion", "warmup", "truncate_to_band", "random_phase_scalar", "random_phase_velocity",
     "uniform_positions",
 ]

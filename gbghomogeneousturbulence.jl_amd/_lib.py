@@ -19,7 +19,8 @@ HEADER = HERE.parent / "include" / "hdns.h"
 OK, ERR_INVALID_ARGUMENT, ERR_STATE, ERR_NONFINITE, ERR_DEVICE, ERR_UNSUPPORTED = range(6)
 FP32_BULK, FP64 = 0, 1
 RK3, RK4, SSPRK3 = 3, 4, 5
-NO_FORCING, CONSTANT_POWER, LINEAR = 0, 1, 2
+NO_FORCING, CONSTANT_POWER, LINEAR, VARIANCE_CONTROLLER, VARIANCE_RESCALING = 0, 1, 2, 3, 4
+OVER_BAND, OVER_TOTAL = 0, 1
 FIELD_VELOCITY, FIELD_SCALAR, FIELD_VORTICITY = 0, 1, 2
 
 
@@ -32,7 +33,8 @@ class HdnsConfig(ctypes.Structure):
         ("lowest_shell", c_int32), ("highest_shell", c_int32),
         ("mean_gradient", c_double), ("tracer_capacity", c_int64),
         ("interpolation", c_int32), ("particle_scheme", c_int32),
-        ("store_physical", c_int32), ("reserved", c_int32),
+        ("store_physical", c_int32), ("forcing_over", c_int32),
+        ("forcing_target", c_double), ("forcing_gain", c_double), ("forcing_time", c_double),
     ]
 
 

@@ -138,6 +138,91 @@ class Linear:
         return f"Linear(rate {self.rate}, shells {self.lowest_shell}-{self.highest_shell})"
 
 
+# (1/highest_shell, k~): the stationary kinetic energy in box units k / (eps L)^(2/3) that the
+# reference's placeholder model assigns to a band starting at shell 1, piecewise linear in
+# 1/highest_shell (src/Forcing.jl:887, :945-953)
+_PLACEHOLDER_KINETIC_ENERGY_IN_BOX_UNITS = ((0.0, 0.47), (0.25, 0.56), (0.5, 0.72))
+
+
+def _placeholder_kinetic_energy_in_box_units(highest_shell) -> float:
+    x = 0.0 if highest_shell == math.inf else 1.0 / highest_shell
+    table = _PLACEHOLDER_KINETIC_ENERGY_IN_BOX_UNITS
+    if x >= table[-1][0]:
+        return table[-1][1]
+    for (x0, y0), (x1, y1) in zip(table, table[1:]):
+        if x0 <= x <= x1:
+            return y0 + (y1 - y0) * (x - x0) / (x1 - x0)
+    raise AssertionError("unreachable")
+
+
+class VarianceController:
+    """Hold the whole field's kinetic energy at `target` by feedback (src/Forcing.jl:722-871,
+    :1041-1065): A = (eps - G (k - k_inf) / t_l) / (2 k_band) on shells lowest..highest, re-measured
+    at every stage's own field on the device, so that dk/dt = -G (k - k_inf) / t_l.  gain = 0 is the
+    bare marginal law.  turnover_time defaults to 2 k_inf / (3 eps_inf) with eps_inf =
+    (k_inf / k~)^(3/2) / L from the placeholder model of the band (bands from shell 1 to 2 or more)."""
+
+    def __init__(self, domain: FixedDomain, *, target: float, gain: float = 1.0, turnover_time=None,
+                 lowest_shell: int = 1, highest_shell=2):
+        band = ConstantPower(domain, injection_rate=0.0, lowest_shell=lowest_shell, highest_shell=highest_shell)
+        cast = domain.element_type
+        target, gain = float(cast(target)), float(cast(gain))
+        if not (math.isfinite(target) and target > 0):
+            raise ValueError(f"`target` must be finite and positive, got {target}: it is the variance the "
+                             "controller holds, and the rate divides by the band's own variance")
+        if not (math.isfinite(gain) and gain >= 0):
+            raise ValueError(f"`gain` must be finite and non-negative, got {gain}: a negative gain makes the "
+                             "restoring term push the variance AWAY from the target. The value `gain = 0` is "
+                             "the bare marginal law, no restoring at all, and is allowed.")
+        if turnover_time is None:
+            if band.lowest_shell != 1:
+                raise ValueError("`turnover_time` has no default here: the model covers bands that start at "
+                                 f"shell 1, but this VarianceController starts at shell {band.lowest_shell}. "
+                                 "Pass `turnover_time`.")
+            if not (band.highest_shell == math.inf or band.highest_shell >= 2):
+                raise ValueError("`turnover_time` has no default here: the model has no value for a band ending "
+                                 f"at shell {band.highest_shell}. Pass `turnover_time`.")
+            kt = float(cast(_placeholder_kinetic_energy_in_box_units(band.highest_shell)))
+            eps = float(cast((target / kt) ** 1.5 / float(cast(domain.side_length))))
+            tl = float(cast(2 * target / (3 * eps)))
+        else:
+            tl = float(cast(turnover_time))
+        if not (math.isfinite(tl) and tl > 0):
+            raise ValueError(f"`turnover_time` must be finite and positive, got {tl}")
+        self.target, self.gain, self.turnover_time = target, gain, tl
+        self.turnover_time_derived = turnover_time is None
+        self.lowest_shell = band.lowest_shell
+        self.highest_shell = band.highest_shell
+
+    def __repr__(self) -> str:
+        return (f"VarianceController(target {self.target}, gain {self.gain}, turnover_time {self.turnover_time}, "
+                f"shells {self.lowest_shell}-{self.highest_shell})")
+
+
+class VarianceRescaling:
+    """Sullivan-Mahalingam-Kerr rescaling (src/Forcing.jl:467-597): after each step the band's modes
+    are rescaled by one factor that holds the band's kinetic energy (over = "band") or the whole
+    field's (over = "total") at `target`; an empty band is left alone.  First order in time by
+    construction (a post-step projection)."""
+
+    def __init__(self, domain: FixedDomain, *, target: float, over: str, lowest_shell: int = 1, highest_shell=2):
+        if over not in ("band", "total"):
+            raise ValueError(f"VarianceRescaling `over` must be 'band' or 'total'; got {over!r}")
+        band = ConstantPower(domain, injection_rate=0.0, lowest_shell=lowest_shell, highest_shell=highest_shell)
+        target = float(domain.element_type(target))
+        if not (math.isfinite(target) and target >= 0):
+            raise ValueError(f"VarianceRescaling `target` must be finite and non-negative, got {target}: it is "
+                             "the kinetic energy or scalar variance the forcing holds, and the rescale takes "
+                             "its square root")
+        self.target = target
+        self.over = over
+        self.lowest_shell = band.lowest_shell
+        self.highest_shell = band.highest_shell
+
+    def __repr__(self) -> str:
+        return f"VarianceRescaling(target {self.target}, over {self.over}, shells {self.lowest_shell}-{self.highest_shell})"
+
+
 class FlowConditions:
     """nu, Sc and the two forcings (src/Generator.jl:81-147), with the reference's guards."""
 
@@ -154,8 +239,10 @@ class FlowConditions:
         self.scalar_forcing = scalar_forcing if scalar_forcing is not None else NoForcing()
         if not isinstance(self.scalar_forcing, NoForcing):
             raise NotImplementedError("scalar forcings other than NoForcing are not built")
-        if not isinstance(self.velocity_forcing, (NoForcing, ConstantPower, Linear)):
-            raise NotImplementedError("velocity forcings other than NoForcing, ConstantPower and Linear are not built")
+        if not isinstance(self.velocity_forcing,
+                          (NoForcing, ConstantPower, Linear, VarianceController, VarianceRescaling)):
+            raise NotImplementedError("the velocity forcings built are NoForcing, ConstantPower, Linear, "
+                                      "VarianceController and VarianceRescaling")
 
     @property
     def scalar_diffusivity(self) -> float:
@@ -303,11 +390,19 @@ def _config(domain: FixedDomain, conditions: FlowConditions, scheme, carries_sca
     cfg.viscosity = conditions.viscosity
     cfg.schmidt = conditions.schmidt
     f = conditions.velocity_forcing
+    if isinstance(f, (ConstantPower, Linear, VarianceController, VarianceRescaling)):
+        cfg.lowest_shell = f.lowest_shell
+        cfg.highest_shell = -1 if f.highest_shell == math.inf else f.highest_shell
     if isinstance(f, (ConstantPower, Linear)):
         cfg.velocity_forcing = L.CONSTANT_POWER if isinstance(f, ConstantPower) else L.LINEAR
         cfg.injection_rate = f.injection_rate if isinstance(f, ConstantPower) else f.rate
-        cfg.lowest_shell = f.lowest_shell
-        cfg.highest_shell = -1 if f.highest_shell == math.inf else f.highest_shell
+    elif isinstance(f, VarianceController):
+        cfg.velocity_forcing = L.VARIANCE_CONTROLLER
+        cfg.forcing_target, cfg.forcing_gain, cfg.forcing_time = f.target, f.gain, f.turnover_time
+    elif isinstance(f, VarianceRescaling):
+        cfg.velocity_forcing = L.VARIANCE_RESCALING
+        cfg.forcing_target = f.target
+        cfg.forcing_over = L.OVER_TOTAL if f.over == "total" else L.OVER_BAND
     cfg.mean_gradient = float(mean_gradient)
     cap = tracers.count if tracers is not None else 0
     cfg.tracer_capacity = int(tracer_capacity if tracer_capacity is not None else cap)

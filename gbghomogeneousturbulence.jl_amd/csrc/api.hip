@@ -12,6 +12,7 @@
 
 #include "compare.hpp"
 #include "context.hpp"
+#include "forcing.hpp"
 
 using namespace hdns;
 
@@ -156,7 +157,7 @@ void release(Ctx &c) {
   drop_graph(c);
   for (hipEvent_t e : c.prof_ev) (void)hipEventDestroy(e);
   c.prof_ev.clear();
-  void *ptrs[] = {c.uh, c.hu, c.ph, c.hp, c.bufa, c.bufb, c.phys, c.physw, c.tw, c.partials, c.scalars,
+  void *ptrs[] = {c.uh, c.hu, c.ph, c.hp, c.bufa, c.bufb, c.phys, c.physw, c.tw, c.partials, c.scalars, c.rpart,
                   c.maxbits, c.spec_bins, c.pos, c.vel, c.phis, c.order, c.tile_counts, c.tile_cursor,
                   c.visc_d, c.kappa_d};
   for (void *p : ptrs)
@@ -219,15 +220,49 @@ void prof_end(Ctx &c) {
     prof_end(c);             \
   } while (0)
 
+double eps_of(const Ctx &c) { return c.fp64 ? 2.220446049250313e-16 : 1.1920928955078125e-07; }
+
+FinalizeArgs finalize_args(const Ctx &c) {
+  FinalizeArgs fa{};
+  fa.nblocks = c.pipe->z_blocks(c);
+  fa.inv_norm = 1.0 / pow((double)c.geo.n, 6.0);
+  fa.forcing = c.cfg.velocity_forcing == HDNS_CONSTANT_POWER ? 1 : (c.cfg.velocity_forcing == HDNS_VARIANCE_CONTROLLER ? 2 : 0);
+  fa.injection_rate = c.cfg.injection_rate;
+  fa.eps_t = eps_of(c);
+  fa.target = c.cfg.forcing_target;
+  fa.gain = c.cfg.forcing_gain;
+  fa.turnover_time = c.cfg.forcing_time;
+  fa.visc = c.visc_d;
+  return fa;
+}
+
+// VarianceRescaling's finalize! on the advanced state of every box (src/Forcing.jl:575-596).
+void launch_rescale(Ctx &c) {
+  const RescaleArgs ra{c.cfg.forcing_target, c.cfg.forcing_over == HDNS_OVER_TOTAL, 1.0 / pow((double)c.geo.n, 6.0),
+                       eps_of(c)};
+  const int r = c.cfg.highest_shell < 0 ? c.geo.m : std::min(c.cfg.highest_shell, c.geo.m);
+  const size_t cube = (size_t)(2 * r + 1) * (2 * r + 1) * (r + 1);
+  const dim3 sums(RESCALE_BLOCKS, 1, c.nbox), apply((unsigned)((cube + 255) / 256), 1, c.nbox);
+  if (c.fp64) {
+    hipLaunchKernelGGL(k_band_sums<double>, sums, dim3(256), 0, c.stream, (const cx<double> *)c.uh, c.geo, c.band, c.rpart);
+    hipLaunchKernelGGL(k_rescale_factor, dim3(c.nbox), dim3(256), 0, c.stream, (const double *)c.rpart, c.scalars, ra);
+    hipLaunchKernelGGL(k_rescale_apply<double>, apply, dim3(256), 0, c.stream, (cx<double> *)c.uh, c.geo, c.band, r,
+                       (const double *)c.scalars);
+  } else {
+    hipLaunchKernelGGL(k_band_sums<float>, sums, dim3(256), 0, c.stream, (const cx<float> *)c.uh, c.geo, c.band, c.rpart);
+    hipLaunchKernelGGL(k_rescale_factor, dim3(c.nbox), dim3(256), 0, c.stream, (const double *)c.rpart, c.scalars, ra);
+    hipLaunchKernelGGL(k_rescale_apply<float>, apply, dim3(256), 0, c.stream, (cx<float> *)c.uh, c.geo, c.band, r,
+                       (const double *)c.scalars);
+  }
+}
+
 // The inverse half of the pipeline for the current state: the field deposit!
 // (src/Propagator.jl:808-842): u, w, phi in physical space, their products for the next
 // stage 1, max|u|^2, and the reductions of the state.
 hdns_status run_deposit(hdns_ctx *ctx, void *phys_to = nullptr, void *physw_to = nullptr) {
   Ctx &c = ctx->c;
   c.pipe->zi(c);
-  const FinalizeArgs fa{c.pipe->z_blocks(c), 1.0 / pow((double)c.geo.n, 6.0),
-                        c.cfg.velocity_forcing == HDNS_CONSTANT_POWER, c.cfg.injection_rate,
-                        c.fp64 ? 2.220446049250313e-16 : 1.1920928955078125e-07};
+  const FinalizeArgs fa = finalize_args(c);
   hipLaunchKernelGGL(k_finalize, dim3(c.nbox), dim3(256), 0, c.stream, (const double *)c.partials, c.scalars, fa);
   c.pipe->yinv(c);
   hipLaunchKernelGGL(k_fill_u64, dim3((c.nbox + 63) / 64), dim3(64), 0, c.stream, c.maxbits, 0ull, c.nbox);
@@ -289,9 +324,9 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
     return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
                 "`schmidt` must be positive: the scalar diffusivity is nu/Sc, so a non-positive Schmidt number makes "
                 "it infinite or negative");
-  if (k.velocity_forcing != HDNS_NO_FORCING && k.velocity_forcing != HDNS_CONSTANT_POWER &&
-      k.velocity_forcing != HDNS_LINEAR)
-    return fail(nullptr, HDNS_ERR_UNSUPPORTED, "velocity_forcing must be NoForcing, ConstantPower or Linear");
+  if (k.velocity_forcing < HDNS_NO_FORCING || k.velocity_forcing > HDNS_VARIANCE_RESCALING)
+    return fail(nullptr, HDNS_ERR_UNSUPPORTED,
+                "velocity_forcing must be NoForcing, ConstantPower, Linear, VarianceController or VarianceRescaling");
   if (k.velocity_forcing != HDNS_NO_FORCING) {  // src/Forcing.jl:388-409
     if (k.lowest_shell < 1)
       return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
@@ -301,9 +336,29 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
       return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
                   "`highest_shell` must be at least `lowest_shell`: the band would be empty and the forcing would "
                   "silently do nothing");
-    if (!isfinite(k.injection_rate))
+    if ((k.velocity_forcing == HDNS_CONSTANT_POWER || k.velocity_forcing == HDNS_LINEAR) && !isfinite(k.injection_rate))
       return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
                   k.velocity_forcing == HDNS_LINEAR ? "the Linear rate must be finite" : "injection_rate must be finite");
+  }
+  if (k.velocity_forcing == HDNS_VARIANCE_CONTROLLER) {   // src/Forcing.jl:828-871
+    if (!(isfinite(k.forcing_target) && k.forcing_target > 0))
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                  "`target` must be finite and positive: it is the variance the controller holds, and the rate "
+                  "divides by the band's own variance");
+    if (!(isfinite(k.forcing_gain) && k.forcing_gain >= 0))
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                  "`gain` must be finite and non-negative: a negative gain makes the restoring term push the "
+                  "variance away from the target (gain = 0 is the bare marginal law)");
+    if (!(isfinite(k.forcing_time) && k.forcing_time > 0))
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "`turnover_time` must be finite and positive");
+  }
+  if (k.velocity_forcing == HDNS_VARIANCE_RESCALING) {    // src/Forcing.jl:555-573
+    if (k.forcing_over != HDNS_OVER_BAND && k.forcing_over != HDNS_OVER_TOTAL)
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "VarianceRescaling `over` must be band or total");
+    if (!(isfinite(k.forcing_target) && k.forcing_target >= 0))
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                  "VarianceRescaling `target` must be finite and non-negative: it is the kinetic energy the "
+                  "forcing holds, and the rescale takes its square root");
   }
   if (!isfinite(k.mean_gradient)) return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "mean_gradient must be finite");
   if (k.tracer_capacity < 0) return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "tracer_capacity must be >= 0");
@@ -377,6 +432,8 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.tw, n * sizeof(cx<double>));
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.partials, B * nzb * NRED * sizeof(double));
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.scalars, B * 8 * sizeof(double));
+  if (s == HDNS_OK && k.velocity_forcing == HDNS_VARIANCE_RESCALING)
+    s = alloc(ctx, (void **)&c.rpart, B * RESCALE_BLOCKS * 2 * sizeof(double));
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.maxbits, B * sizeof(unsigned long long));
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.spec_bins, c.nbins * sizeof(double));
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.visc_d, B * sizeof(double));
@@ -641,9 +698,7 @@ void launch_step(Ctx &c, double dt, const StageRecord *rec = nullptr) {
   const bool fp32 = !c.fp64;
   const Coeffs k = scheme_coefficients(c.cfg.scheme, fp32);
   const double dt_t = fp32 ? (double)(float)dt : dt;
-  const FinalizeArgs fa{c.pipe->z_blocks(c), 1.0 / pow((double)c.geo.n, 6.0),
-                        c.cfg.velocity_forcing == HDNS_CONSTANT_POWER, c.cfg.injection_rate,
-                        c.fp64 ? 2.220446049250313e-16 : 1.1920928955078125e-07};
+  const FinalizeArgs fa = finalize_args(c);
   XArgs mid{};
   XArgs fin{};
   fin.box_phys = c.box_phys;
@@ -688,6 +743,7 @@ void launch_step(Ctx &c, double dt, const StageRecord *rec = nullptr) {
     TIMED(c, K_YFWD, c.pipe->yfwd(c));
     TIMED(c, K_ZF, c.pipe->zf(c, sa));
     if (last) TIMED(c, K_REALITY, c.pipe->reality(c));
+    if (last && c.cfg.velocity_forcing == HDNS_VARIANCE_RESCALING) launch_rescale(c);
     TIMED(c, K_ZI, c.pipe->zi(c));
     TIMED(c, K_FINALIZE,
           hipLaunchKernelGGL(k_finalize, dim3(c.nbox), dim3(256), 0, c.stream, (const double *)c.partials, c.scalars,
@@ -1244,7 +1300,7 @@ hdns_status adj_alloc(hdns_ctx *ctx) {
   AdjBufs &a = c.adj;
   if (a.ready) return HDNS_OK;
   if (c.nbox != 1) return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint runs on a single-box context");
-  if (c.cfg.velocity_forcing == HDNS_LINEAR)
+  if (c.cfg.velocity_forcing != HDNS_NO_FORCING && c.cfg.velocity_forcing != HDNS_CONSTANT_POWER)
     return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint supports NoForcing and ConstantPower velocity forcing");
   if (c.cfg.scheme == HDNS_IF_SSP_RK3)
     return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint supports the low-storage schemes IFLowStorageRK3 and IFLowStorageRK4");

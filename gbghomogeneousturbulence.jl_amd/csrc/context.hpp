@@ -89,7 +89,8 @@ struct Ctx {
   void *phys = nullptr, *physw = nullptr;
   cx<double> *tw = nullptr;
   double *partials = nullptr;
-  double *scalars = nullptr;
+  double *scalars = nullptr;   // per box [8]: A, the five reductions, the rescale factor - 1, spare
+  double *rpart = nullptr;     // VarianceRescaling's block sums
   unsigned long long *maxbits = nullptr;
   double *spec_bins = nullptr;
   int nbins = 0;

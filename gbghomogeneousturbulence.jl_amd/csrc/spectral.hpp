@@ -14,7 +14,7 @@ namespace hdns {
 __host__ __device__ __forceinline__ bool in_two_thirds(int m2, int n) { return 9 * m2 <= n * n; }
 
 struct BandSpec {
-  int on;      // a band forcing present (ConstantPower or Linear)
+  int on;      // a band forcing present (every forcing but NoForcing)
   int lo4;     // 4|m|^2 >= lo4 = (2*lowest - 1)^2
   int hi4;     // 4|m|^2 <  hi4 = (2*highest + 1)^2, or -1 for no upper edge
 };
@@ -521,9 +521,11 @@ __global__ void k_reality(cx<T> *__restrict__ uh, cx<T> *__restrict__ ph, Geo g,
 struct FinalizeArgs {
   int nblocks;
   double inv_norm;       // 1/N^6
-  int forcing;
+  int forcing;           // the amplitude A: 0 none, 1 ConstantPower, 2 VarianceController
   double injection_rate;
   double eps_t;          // eps of the run's element type
+  double target, gain, turnover_time;   // VarianceController k_inf, G, t_l
+  const double *visc;    // per box: nu
 };
 
 static __global__ void k_finalize(const double *__restrict__ partials, double *__restrict__ scalars, FinalizeArgs fa) {
@@ -560,11 +562,16 @@ static __global__ void k_finalize(const double *__restrict__ partials, double *_
   }
   __syncthreads();
   if (tid == 0) {
+    // ConstantPower: A = eps_in / (2 k_band) (src/Forcing.jl:1091-1117); VarianceController:
+    // A = (eps - G (k - k_inf) / t_l) / (2 k_band) with eps = nu <u, |k|^2 u> (src/Forcing.jl:1041-1065);
+    // both 0 for an empty band (src/Forcing.jl:110-111, :1093-1103)
     double A = 0.0;
     if (fa.forcing) {
       const double band = 0.5 * scalars[2], whole = 0.5 * scalars[1];
       const bool empty = !(isfinite(band) && band > 10.0 * fa.eps_t * whole);
-      A = empty ? 0.0 : fa.injection_rate / (2.0 * band);
+      double num = fa.injection_rate;
+      if (fa.forcing == 2) num = fa.visc[box] * scalars[3] - fa.gain * (whole - fa.target) / fa.turnover_time;
+      A = empty ? 0.0 : num / (2.0 * band);
     }
     scalars[0] = A;
   }

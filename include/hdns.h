@@ -36,13 +36,18 @@ enum { HDNS_FP32_BULK = 0, HDNS_FP64 = 1 };
 /* IFLowStorageRK3, IFLowStorageRK4 (src/Propagator.jl:264-320) and IFSSPRK3 (:596-691) */
 enum { HDNS_IF_LOW_STORAGE_RK3 = 3, HDNS_IF_LOW_STORAGE_RK4 = 4, HDNS_IF_SSP_RK3 = 5 };
 enum { HDNS_TWO_THIRDS = 0 };
-enum { HDNS_NO_FORCING = 0, HDNS_CONSTANT_POWER = 1, HDNS_LINEAR = 2 };
+/* NoForcing, ConstantPower, Linear, VarianceController and VarianceRescaling (src/Forcing.jl:464,
+ * :714-719, :600-642, :817-871, :566-597) */
+enum { HDNS_NO_FORCING = 0, HDNS_CONSTANT_POWER = 1, HDNS_LINEAR = 2, HDNS_VARIANCE_CONTROLLER = 3,
+       HDNS_VARIANCE_RESCALING = 4 };
+/* VarianceRescaling's `over`: hold the band's kinetic energy, or the whole field's */
+enum { HDNS_OVER_BAND = 0, HDNS_OVER_TOTAL = 1 };
 enum { HDNS_CATMULL_ROM = 0 };
 enum { HDNS_HEUN = 0 };
 enum { HDNS_FIELD_VELOCITY = 0, HDNS_FIELD_SCALAR = 1, HDNS_FIELD_VORTICITY = 2 };
 
 /* Everything a run fixes at construction: FixedDomain (src/SimulationDomain.jl:295-328),
- * FlowConditions (src/Generator.jl:81-147), ConstantPower (src/Forcing.jl:714-719,1069-1080),
+ * FlowConditions (src/Generator.jl:81-147), the velocity forcings (src/Forcing.jl:566-1117),
  * the scheme (src/Propagator.jl:264-320, :596-691), and the tracer set (src/Particles.jl:135-176). */
 typedef struct {
   int32_t device;            /* HIP device ordinal */
@@ -54,7 +59,7 @@ typedef struct {
   int32_t has_scalar;        /* evolve the passive scalar */
   double viscosity;          /* nu > 0 */
   double schmidt;            /* Sc > 0 */
-  int32_t velocity_forcing;  /* HDNS_NO_FORCING, HDNS_CONSTANT_POWER or HDNS_LINEAR */
+  int32_t velocity_forcing;  /* HDNS_NO_FORCING, _CONSTANT_POWER, _LINEAR, _VARIANCE_CONTROLLER, _VARIANCE_RESCALING */
   double injection_rate;     /* ConstantPower epsilon, or the Linear rate A (src/Forcing.jl:600-642) */
   int32_t lowest_shell;      /* >= 1 */
   int32_t highest_shell;     /* >= lowest_shell; -1 means Inf (every mode but the mean) */
@@ -63,7 +68,10 @@ typedef struct {
   int32_t interpolation;     /* HDNS_CATMULL_ROM */
   int32_t particle_scheme;   /* HDNS_HEUN */
   int32_t store_physical;    /* bit 0: keep physical u, bit 1: keep physical phi, bit 2: keep physical w */
-  int32_t reserved;
+  int32_t forcing_over;      /* VarianceRescaling: HDNS_OVER_BAND or HDNS_OVER_TOTAL */
+  double forcing_target;     /* VarianceController k_inf > 0, or VarianceRescaling's target >= 0 */
+  double forcing_gain;       /* VarianceController G >= 0 (0: the marginal law) */
+  double forcing_time;       /* VarianceController turnover time t_l > 0 */
 } hdns_config;
 
 typedef struct hdns_ctx hdns_ctx;

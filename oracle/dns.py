@@ -193,6 +193,10 @@ class Conditions:
     forcing: tuple | None = None
     mean_gradient: float = 0.0
     linear: tuple | None = None       # Linear forcing (rate A, lowest_shell, highest_shell)
+    # VarianceController (target k_inf, gain G, turnover_time t_l, lowest_shell, highest_shell)
+    controller: tuple | None = None
+    # VarianceRescaling (target, over "band" | "total", lowest_shell, highest_shell)
+    rescaling: tuple | None = None
     band_eps: float = float(np.finfo(np.float64).eps)
 
     @property
@@ -227,6 +231,36 @@ def constant_power_amplitude(d: Domain, cond: Conditions, uh: np.ndarray) -> flo
     return rate / (2 * band_obs)
 
 
+def controller_amplitude(d: Domain, cond: Conditions, uh: np.ndarray) -> float:
+    """A = (eps - G (k - k_inf) / t_l) / (2 k_band) at the stage field, with eps = nu <u, |k|^2 u>,
+    k the whole field's and k_band the band's kinetic energy; 0 for an empty band
+    (src/Forcing.jl:1041-1065, the clamp :1093-1103)."""
+    target, gain, tl, lo, hi = cond.controller
+    band = d.shell_band(lo, hi)
+    whole = 0.5 * weighted_inner(d, uh)
+    band_obs = 0.5 * weighted_inner(d, uh, band)
+    eps = cond.viscosity * weighted_inner(d, uh, d.k2)
+    restoring = gain * (whole - target) / tl
+    if band_is_empty(band_obs, whole, cond.band_eps):
+        return 0.0
+    return (eps - restoring) / (2 * band_obs)
+
+
+def variance_rescaling(d: Domain, cond: Conditions, uh: np.ndarray) -> np.ndarray:
+    """The post-step rescale of the band (src/Forcing.jl:575-596): s = sqrt(target_band / k_band)
+    with target_band = target (over "band") or max(0, target - k + k_band) (over "total"); band
+    modes become u + (s - 1) u; an empty band is left alone."""
+    target, over, lo, hi = cond.rescaling
+    band = d.shell_band(lo, hi)
+    band_obs = 0.5 * weighted_inner(d, uh, band)
+    whole = 0.5 * weighted_inner(d, uh)
+    if band_is_empty(band_obs, whole, cond.band_eps):
+        return uh
+    target_band = max(0.0, target - whole + band_obs) if over == "total" else target
+    scale = math.sqrt(target_band / band_obs)
+    return uh + (scale - 1) * band * uh
+
+
 def velocity_rhs(d: Domain, cond: Conditions, uh: np.ndarray, u: np.ndarray, w: np.ndarray) -> np.ndarray:
     """N_u = P mask FFT(u x w), mean mode zeroed, plus the ConstantPower source.
 
@@ -242,6 +276,9 @@ def velocity_rhs(d: Domain, cond: Conditions, uh: np.ndarray, u: np.ndarray, w: 
         amp = constant_power_amplitude(d, cond, uh)
         band = d.shell_band(cond.forcing[1], cond.forcing[2])
         nl = nl + amp * band * uh
+    if cond.controller is not None:
+        amp = controller_amplitude(d, cond, uh)
+        nl = nl + amp * d.shell_band(cond.controller[3], cond.controller[4]) * uh
     return nl
 
 
@@ -371,6 +408,8 @@ def step(d: Domain, cond: Conditions, scheme: str, uh: np.ndarray, ph: np.ndarra
             ph = ep * (ph + b[i] * hp)
             if i < S - 1:
                 hp = ep * hp
+    if cond.rescaling is not None:   # finalize! before the state projections (:394-401)
+        uh = variance_rescaling(d, cond, uh)
     uh = leray(d, uh)
     uh = reality_projection(d, uh)
     if ph is not None:
@@ -422,6 +461,8 @@ def ssp_step(d: Domain, cond: Conditions, uh: np.ndarray, ph: np.ndarray | None,
     q = eu_full * (w13 * uh) + w23 * (eu_half * (q + dt * n2))
     if ph is not None:
         p = ep_full * (w13 * ph) + w23 * (ep_half * (p + dt * m2))
+    if cond.rescaling is not None:   # finalize! before the state projections (:674-677)
+        q = variance_rescaling(d, cond, q)
     q = reality_projection(d, leray(d, q))
     if p is not None:
         p = reality_projection(d, p)

@@ -115,3 +115,33 @@ def test_stationarity_monitor_flatness_and_gates(g):
     assert not S.is_flat([0, 1, 2, 3], [1.0, 1.0, 1.2, 1.2], 3.0, 4.0, 0.05)
     assert S.is_flat([0, 1, 2, 3], [0.0, 0.0, 0.0, 0.0], 3.0, 4.0, 0.05)    # identically zero is flat
     assert not S.is_flat([2, 3], [1.0, 1.0], 3.0, 4.0, 0.05)                # one half empty
+
+
+def test_variance_controller_default_turnover_time_follows_the_placeholder_model(g):
+    """t_l = 2 k_inf / (3 eps_inf), eps_inf = (k_inf / k~)^(3/2) / L, k~ piecewise linear in
+    1/highest_shell through (0, 0.47), (1/4, 0.56), (1/2, 0.72) (src/Forcing.jl:852-862, :887, :945-953)."""
+    dom = g.FixedDomain(grid_points_per_axis=32, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
+    for hi, kt in ((2, 0.72), (4, 0.56), (math.inf, 0.47), (3, 0.56 + 0.16 * (1 / 3 - 0.25) / 0.25)):
+        f = g.VarianceController(dom, target=1.5, highest_shell=hi)
+        eps = (1.5 / kt) ** 1.5 / L
+        assert f.turnover_time == pytest.approx(2 * 1.5 / (3 * eps), rel=1e-12)
+        assert f.turnover_time_derived
+    # test/forcing.jl:369-371: a band the model does not cover needs the time stated
+    with pytest.raises(ValueError, match="turnover_time"):
+        g.VarianceController(dom, target=1.5, lowest_shell=2, highest_shell=3)
+    assert g.VarianceController(dom, target=1.5, lowest_shell=2, highest_shell=3, turnover_time=1.0).turnover_time == 1.0
+    # test/forcing.jl:407-408 and the target guard
+    for bad in ({"gain": -1.0}, {"gain": math.inf}, {"target": 0.0}):
+        kw = {"target": 1.5, **bad}
+        with pytest.raises(ValueError):
+            g.VarianceController(dom, **kw)
+
+
+def test_variance_rescaling_guards(g):
+    dom = g.FixedDomain(grid_points_per_axis=16, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float32)
+    with pytest.raises(ValueError, match="over"):
+        g.VarianceRescaling(dom, target=0.5, over="shell")
+    with pytest.raises(ValueError, match="non-negative"):
+        g.VarianceRescaling(dom, target=-1.0, over="band")
+    f = g.VarianceRescaling(dom, target=0.1, over="total", highest_shell=math.inf)
+    assert f.target == float(np.float32(0.1)) and f.highest_shell == math.inf

@@ -2,6 +2,7 @@
 header declares, its coefficient and weight tables equal the oracle's, and its argument guards fire
 before any device call."""
 import ctypes
+from pathlib import Path
 import math
 
 import numpy as np
@@ -59,7 +60,21 @@ def test_create_rejects_bad_configurations_before_touching_a_device(g, kw, code)
     assert L.hdns_last_error(None)
 
 
-def test_config_struct_layout_matches_the_header(g):
-    # the C struct as hipcc lays it out: 8-byte aligned doubles, 4-byte ints
-    assert ctypes.sizeof(g._lib.HdnsConfig) == 104
-    assert g._lib.HdnsConfig.tracer_capacity.offset == 80
+def test_config_struct_layout_matches_the_header(g, tmp_path):
+    """Every field's offset and the size of hdns_config as a C compiler lays out include/hdns.h."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    fields = [f for f, _ in g._lib.HdnsConfig._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"hdns.h\"\nint main(void) {\n"
+                   + "".join(f'  printf("%zu\\n", offsetof(hdns_config, {f}));\n' for f in fields)
+                   + '  printf("%zu\\n", sizeof(hdns_config));\n  return 0;\n}\n')
+    exe = tmp_path / "layout"
+    inc = Path(__file__).resolve().parent.parent / "include"
+    subprocess.run([cc, "-I", str(inc), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    want = [getattr(g._lib.HdnsConfig, f).offset for f in fields] + [ctypes.sizeof(g._lib.HdnsConfig)]
+    assert got == want

@@ -384,3 +384,125 @@ def test_ssprk3_weights_round_to_the_element_type():
     assert dns._round(Fraction(1, 3), np.float32) == float(np.float32(1 / 3))
     assert dns._round(Fraction(2, 3), np.float32) == float(np.float32(2 / 3))
     assert dns._round(Fraction(3, 4), np.float32) == 0.75
+
+
+# --- VarianceController and VarianceRescaling ------------------------------------------------------
+
+def _physical_mean_square(d, fh):
+    """<|f|^2> over the grid points of the physical field whose spectrum is fh (no Parseval)."""
+    f = dns.irfft3(fh, d.n)
+    return float(np.mean(np.sum(f * f, axis=0)))
+
+
+def test_controller_amplitude_is_bassenne_eq20_in_physical_space():
+    """test/forcing.jl:322-347: A = (eps - G (k - k_inf)/t_l) / (2 k_band), the restoring term on the
+    WHOLE field's k and the normalisation on the band's, each measured here in physical space
+    (eps = nu <|grad u|^2> from the physical gradient)."""
+    n, nu = 16, 0.03
+    d, uh = _band_limited(n, 11, shell_frac=0.7)
+    uh = uh * math.sqrt(0.4 / dns.reductions(d, dns.Conditions(viscosity=1), uh, None)["kinetic_energy"])
+    target, gain, tl = 1.5, 1.0, 0.8
+    cond = dns.Conditions(viscosity=nu, controller=(target, gain, tl, 1, 2))
+    band = d.shell_band(1, 2)
+    k = 0.5 * _physical_mean_square(d, uh)
+    kb = 0.5 * _physical_mean_square(d, uh * band)
+    kv = d.kvec
+    grad = sum(_physical_mean_square(d, 1j * kv[i] * uh) for i in range(3))
+    eps = nu * grad
+    want = (eps - gain * (k - target) / tl) / (2 * kb)
+    got = dns.controller_amplitude(d, cond, uh)
+    assert got == pytest.approx(want, rel=1e-12)
+    wrong = (eps - gain * (kb - target) / tl) / (2 * kb)
+    assert abs(got - wrong) > 1e-3 * abs(wrong)
+
+
+def _controlled_run(scheme, gain, dt, steps, k0=0.5, target=1.5, tl=0.8):
+    n = 16
+    d, uh = _band_limited(n, 3, shell_frac=0.5)
+    uh = uh * math.sqrt(k0 / dns.reductions(d, dns.Conditions(viscosity=1), uh, None)["kinetic_energy"])
+    cond = dns.Conditions(viscosity=0.05, controller=(target, gain, tl, 1, 2))
+    u, dep = uh, None
+    ks = []
+    for _ in range(steps):
+        u, _, dep = dns.step(d, cond, scheme, u, None, dt, dep)
+        ks.append(dns.reductions(d, cond, u, None)["kinetic_energy"])
+    return np.array(ks)
+
+
+def test_controller_relaxes_k_exponentially():
+    """The dealiased nonlinear term carries no energy at N = 16 (no product mode aliases into the
+    ball), so dk/dt = -eps + 2 A k_band = -G (k - k_inf)/t_l: k(t) = k_inf + (k0 - k_inf) exp(-G t/t_l)."""
+    dt, steps, gain, tl = 5e-3, 100, 2.0, 0.8
+    ks = _controlled_run("rk4", gain, dt, steps, tl=tl)
+    t = dt * np.arange(1, steps + 1)
+    want = 1.5 + (0.5 - 1.5) * np.exp(-gain * t / tl)
+    assert np.max(np.abs(ks - want)) < 1e-9
+
+
+def test_controller_marginal_law_conserves_k():
+    """test/forcing.jl:389-405: gain = 0 reinjects eps exactly, so k stays at its initial 0.5, far
+    from the target 1.5."""
+    ks = _controlled_run("rk3", 0.0, 5e-3, 100)
+    assert np.max(np.abs(ks - 0.5)) < 1e-8   # the scheme's uncorrected O(dt^3) bias: 4.3e-9 here
+
+
+def test_controller_keeps_third_order():
+    """src/Forcing.jl:521-526: the controller enters as a per-stage source, so IFLowStorageRK3 keeps
+    its order (ratio about 8 between successive halvings)."""
+    n = 16
+    d, uh = _band_limited(n, 1, shell_frac=0.5)
+    uh = uh * math.sqrt(0.5 / dns.reductions(d, dns.Conditions(viscosity=1), uh, None)["kinetic_energy"])
+    cond = dns.Conditions(viscosity=0.05, controller=(0.8, 1.0, 0.5, 1, 2))
+    runs = [_integrate("rk3", n, k, 0.4, uh, None, cond)[0] for k in (8, 16, 32)]
+    ratio = np.linalg.norm(runs[0] - runs[1]) / np.linalg.norm(runs[1] - runs[2])
+    assert 5.0 < ratio < 12.0, ratio
+
+
+@pytest.mark.parametrize("n", [16, 24])
+@pytest.mark.parametrize("over", ["band", "total"])
+def test_rescaling_every_mode_scales_the_field_to_the_target(n, over):
+    """test/forcing.jl:440-456: with highest_shell = Inf the band is every mode but the mean and both
+    modes scale the whole field by sqrt(target / k)."""
+    d, uh = _band_limited(n, n, shell_frac=0.8)
+    cond = dns.Conditions(viscosity=0.01, rescaling=(1.3, over, 1, math.inf))
+    k = dns.reductions(d, cond, uh, None)["kinetic_energy"]
+    got = dns.variance_rescaling(d, cond, uh)
+    assert dns.reductions(d, cond, got, None)["kinetic_energy"] == pytest.approx(1.3, rel=1e-12)
+    want = uh * math.sqrt(1.3 / k)
+    assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want))
+
+
+def test_rescaling_band_and_total_hit_their_targets():
+    n = 16
+    d, uh = _band_limited(n, 8, shell_frac=0.8)
+    band = d.shell_band(1, 2)
+    k = dns.reductions(d, dns.Conditions(viscosity=1), uh, None)["kinetic_energy"]
+    kb = 0.5 * dns.weighted_inner(d, uh, band)
+    # over band: the band's energy becomes the target, the rest is untouched
+    got = dns.variance_rescaling(d, dns.Conditions(viscosity=1, rescaling=(0.2, "band", 1, 2)), uh)
+    assert 0.5 * dns.weighted_inner(d, got, band) == pytest.approx(0.2, rel=1e-12)
+    assert np.array_equal(got * ~band, uh * ~band)
+    # over total: the whole field's energy becomes the target
+    tgt = k + 0.1
+    got = dns.variance_rescaling(d, dns.Conditions(viscosity=1, rescaling=(tgt, "total", 1, 2)), uh)
+    assert dns.reductions(d, dns.Conditions(viscosity=1), got, None)["kinetic_energy"] == pytest.approx(tgt, rel=1e-12)
+    # over total with the outside alone above the target: the band is zeroed
+    got = dns.variance_rescaling(d, dns.Conditions(viscosity=1, rescaling=(0.5 * (k - kb), "total", 1, 2)), uh)
+    assert np.max(np.abs(got * band)) == 0.0
+    # an empty band is left alone
+    v = uh * ~band
+    got = dns.variance_rescaling(d, dns.Conditions(viscosity=1, rescaling=(0.3, "band", 1, 2)), v)
+    assert np.array_equal(got, v)
+
+
+@pytest.mark.parametrize("scheme", ["rk3", "ssprk3"])
+def test_rescaling_after_each_step_holds_the_target(scheme):
+    """finalize! runs after the stages and before the state projections, which keep the band's
+    energy: the held value is exact after every step."""
+    n = 16
+    d, uh = _band_limited(n, 2, shell_frac=0.6)
+    cond = dns.Conditions(viscosity=0.02, rescaling=(0.7, "total", 1, 2))
+    u, dep = uh, None
+    for _ in range(5):
+        u, _, dep = dns.step(d, cond, scheme, u, None, 1e-2, dep)
+        assert dns.reductions(d, cond, u, None)["kinetic_energy"] == pytest.approx(0.7, rel=1e-13)
