@@ -59,7 +59,7 @@ def algorithmic_bytes(n: int, real_bytes: int, stages: int, scalar: bool = True,
     cb = 2 * real_bytes
     nf, ni, ns = (6, 7, 4) if scalar else (3, 6, 3)
     # accumulator passes per stage: the 2N schemes neither read it at the first stage nor write it at
-    # the last; IFSSPRK3 writes the initial state once and reads it at the two later stages
+    # the last; the Shu-Osher schemes write the initial state once and read it at every later stage
     acc = 1.0 if ssp else 2 * (stages - 1) / stages
     comp = k * k * kx
     return {
@@ -417,7 +417,7 @@ def main() -> int:
                     help="skip the whole-iteration figures (256^3 FP64 + 1M tracers, 1024^3 FP32-bulk + 16M tracers)")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the FP32-bulk vs FP64 error report")
     ap.add_argument("--no-schemes", action="store_true",
-                    help="skip the other field schemes (IFLowStorageRK3, IFSSPRK3) on the same workload")
+                    help="skip the other field schemes (IFLowStorageRK3, IFSSPRK3, IFHeun) on the same workload")
     ap.add_argument("--adjoint-directions", type=int, default=16,
                     help="finite-difference directions for the adjoint check (0: time only)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -463,14 +463,14 @@ def main() -> int:
             pass
     schemes = None
     if not args.no_schemes:
-        print(f"bench: {n}^3 FP32-bulk, IFLowStorageRK3 and IFSSPRK3", file=sys.stderr, flush=True)
+        print(f"bench: {n}^3 FP32-bulk, IFLowStorageRK3, IFSSPRK3 and IFHeun", file=sys.stderr, flush=True)
         schemes = {}
-        for name in ("IFLowStorageRK3", "IFSSPRK3"):
+        for name, stages in (("IFLowStorageRK3", 3), ("IFSSPRK3", 3), ("IFHeun", 2)):
             k = max(5, args.steps // 2)
             r = time_gpu(g, n, np.float32, k, 1, H, scheme=getattr(g, name))
             ms = r["ms_total"] / k
             schemes[name] = {"value": round(n ** 3 / (ms * 1e-3) / 1e6, 2), "ms_per_step": round(ms, 3),
-                             "steps": k, "kernels": kernel_table(r, n, 4, 3, ssp=name == "IFSSPRK3")}
+                             "steps": k, "kernels": kernel_table(r, n, 4, stages, ssp=name != "IFLowStorageRK3")}
     tracers = None
     if not args.no_tracers:
         print("bench: tracers, 256^3 field, 2^20 tracers", file=sys.stderr, flush=True)

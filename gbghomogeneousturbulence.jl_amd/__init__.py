@@ -6,7 +6,7 @@ The folder name carries a dot, so it is imported by path; ``load_package()`` in
 """
 from . import _lib
 from .api import (ConstantPower, Ensemble, EnsembleBox, EnsembleMember, FixedDomain, FlowConditions, Heun,
-                  IFLowStorageRK3, IFLowStorageRK4, IFSSPRK3, Linear, NoForcing, VarianceController,
+                  IFHeun, IFLowStorageRK3, IFLowStorageRK4, IFSSPRK3, Linear, NoForcing, VarianceController,
                   VarianceRescaling, ResolvedFields, Simulation, Tracers, TwoThirds,
                   StationarityCriterion, cfl_bound, integral_length, predicted_viscosity,
                   recommended_cfl, truncate_to_band, warmup)
@@ -14,7 +14,7 @@ from .seeding import random_phase_scalar, random_phase_velocity, uniform_positio
 
 __all__ = [
     "ConstantPower", "Ensemble", "EnsembleBox", "EnsembleMember", "FixedDomain", "FlowConditions", "Heun",
-    "IFLowStorageRK3", "IFLowStorageRK4", "IFSSPRK3", "Linear", "NoForcing", "VarianceController",
+    "IFHeun", "IFLowStorageRK3", "IFLowStorageRK4", "IFSSPRK3", "Linear", "NoForcing", "VarianceController",
     "VarianceRescaling", "ResolvedFields", "Simulation", "Tracers", "TwoThirds", "predicted_viscosity",
     "recommended_cfl", "cfl_bound", "integral_length", "StationarityCriterion", "warmup", "truncate_to_band", "random_phase_scalar", "random_phase_velocity",
     "uniform_positions",

@@ -18,7 +18,7 @@ HEADER = HERE.parent / "include" / "hdns.h"
 
 OK, ERR_INVALID_ARGUMENT, ERR_STATE, ERR_NONFINITE, ERR_DEVICE, ERR_UNSUPPORTED = range(6)
 FP32_BULK, FP64 = 0, 1
-RK3, RK4, SSPRK3 = 3, 4, 5
+RK3, RK4, SSPRK3, HEUN = 3, 4, 5, 6
 NO_FORCING, CONSTANT_POWER, LINEAR, VARIANCE_CONTROLLER, VARIANCE_RESCALING = 0, 1, 2, 3, 4
 OVER_BAND, OVER_TOTAL = 0, 1
 FIELD_VELOCITY, FIELD_SCALAR, FIELD_VORTICITY = 0, 1, 2

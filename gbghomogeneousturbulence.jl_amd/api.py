@@ -276,6 +276,13 @@ class IFSSPRK3:
     base_cfl = 0.5
 
 
+class IFHeun:
+    """Two-stage second-order IF Heun (src/Propagator.jl:420-544); unstable on the imaginary axis
+    without viscous damping, hence its lower ceiling (:906)."""
+    code = L.HEUN
+    base_cfl = 0.25
+
+
 def recommended_cfl(scheme, domain: FixedDomain) -> float:
     """The advective CFL ceiling (src/Propagator.jl:906-919); TwoThirds leaves the base unscaled."""
     return scheme.base_cfl

@@ -63,8 +63,8 @@ struct Coeffs {
 // rationals rounded to the element type; d are differences of the rounded abscissae formed in T.
 Coeffs scheme_coefficients(int scheme, bool fp32) {
   Coeffs k{};
-  if (scheme == HDNS_IF_SSP_RK3) {   // not a 2N scheme: its stages are set by ssp_stage
-    k.stages = 3;
+  if (scheme == HDNS_IF_SSP_RK3 || scheme == HDNS_IF_HEUN) {   // not 2N schemes: shu_osher_stage sets their stages
+    k.stages = scheme == HDNS_IF_HEUN ? 2 : 3;
     return k;
   }
   if (scheme == HDNS_IF_LOW_STORAGE_RK3) {
@@ -311,8 +311,9 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
     return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "side_length must be positive and finite");
   if (k.precision != HDNS_FP32_BULK && k.precision != HDNS_FP64)
     return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "precision must be HDNS_FP32_BULK or HDNS_FP64");
-  if (k.scheme != HDNS_IF_LOW_STORAGE_RK3 && k.scheme != HDNS_IF_LOW_STORAGE_RK4 && k.scheme != HDNS_IF_SSP_RK3)
-    return fail(nullptr, HDNS_ERR_UNSUPPORTED, "scheme must be IFLowStorageRK3, IFLowStorageRK4 or IFSSPRK3");
+  if (k.scheme != HDNS_IF_LOW_STORAGE_RK3 && k.scheme != HDNS_IF_LOW_STORAGE_RK4 && k.scheme != HDNS_IF_SSP_RK3 &&
+      k.scheme != HDNS_IF_HEUN)
+    return fail(nullptr, HDNS_ERR_UNSUPPORTED, "scheme must be IFLowStorageRK3, IFLowStorageRK4, IFSSPRK3 or IFHeun");
   if (k.dealiasing != HDNS_TWO_THIRDS) return fail(nullptr, HDNS_ERR_UNSUPPORTED, "only TwoThirds dealiasing is built");
   // src/Generator.jl:107-112
   for (int b = 0; b < nbox; ++b)
@@ -397,7 +398,7 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
   c.elem = c.fp64 ? 16 : 8;
   const size_t realb = c.fp64 ? 8 : 4;
   c.comp = (size_t)c.geo.k * c.geo.k * c.geo.kxp;
-  c.stages = k.scheme == HDNS_IF_LOW_STORAGE_RK4 ? 5 : 3;
+  c.stages = k.scheme == HDNS_IF_LOW_STORAGE_RK4 ? 5 : (k.scheme == HDNS_IF_HEUN ? 2 : 3);
   c.band.on = k.velocity_forcing != HDNS_NO_FORCING;
   c.band.lo4 = (2 * k.lowest_shell - 1) * (2 * k.lowest_shell - 1);
   c.band.hi4 = k.highest_shell < 0 ? -1 : (2 * k.highest_shell + 1) * (2 * k.highest_shell + 1);
@@ -679,17 +680,24 @@ struct StageRecord {
   void *physw[5];
 };
 
-// IFSSPRK3's stage i (src/Propagator.jl:599-669) in the form q <- cb E^pb (q + dt N) + E^pa (ca h),
-// E = exp((A - nu |k|^2) dt / 2) and h the step's initial state, which stage 0 saves:
+// Stage i of a scheme in Shu-Osher form, q <- cb E^pb (q + dt N) + E^pa (ca h), with
+// E = exp((A - nu |k|^2) dt / 2) and h the step's initial state, which stage 0 saves.
+// IFSSPRK3 (src/Propagator.jl:599-669):
 //   q1 = E^2 (u + dt N);  q2 = 1/4 E^-1 (q1 + dt N) + E (3/4 u);  u' = 2/3 E (q2 + dt N) + E^2 (1/3 u).
+// IFHeun (src/Propagator.jl:456-517), whose u' = E u + dt/2 (E N1 + N2) is SSPRK(2,2):
+//   q1 = E^2 (u + dt N);  u' = 1/2 (q1 + dt N) + E^2 (1/2 u).
 // The powers are evaluated as exp(p x), never as products of E.
-void ssp_stage(StageArgs &sa, int i, double dt, bool fp32) {
-  static const int pa[3] = {0, 1, 2}, pb[3] = {2, -1, 1};
-  static const long double ca[3][2] = {{0, 1}, {3, 4}, {1, 3}}, cb[3][2] = {{1, 1}, {1, 4}, {2, 3}};
+void shu_osher_stage(StageArgs &sa, int scheme, int i, double dt, bool fp32) {
+  static const int pa3[3] = {0, 1, 2}, pb3[3] = {2, -1, 1}, pa2[2] = {0, 2}, pb2[2] = {2, 0};
+  static const long double ca3[3][2] = {{0, 1}, {3, 4}, {1, 3}}, cb3[3][2] = {{1, 1}, {1, 4}, {2, 3}};
+  static const long double ca2[2][2] = {{0, 1}, {1, 2}}, cb2[2][2] = {{1, 1}, {1, 2}};
+  const bool heun = scheme == HDNS_IF_HEUN;
   sa.ssp = 1;
   sa.sub = dt / 2;
-  sa.pa = pa[i];
-  sa.pb = pb[i];
+  sa.pa = heun ? pa2[i] : pa3[i];
+  sa.pb = heun ? pb2[i] : pb3[i];
+  const long double(*ca)[2] = heun ? ca2 : ca3;
+  const long double(*cb)[2] = heun ? cb2 : cb3;
   sa.ca = round_t(ca[i][0], ca[i][1], fp32);
   sa.cb = round_t(cb[i][0], cb[i][1], fp32);
 }
@@ -725,7 +733,7 @@ void launch_step(Ctx &c, double dt, const StageRecord *rec = nullptr) {
     sa.band = c.band;
     sa.scalars = c.scalars;
     sa.lin_rate = c.cfg.velocity_forcing == HDNS_LINEAR ? c.cfg.injection_rate : 0.0;
-    if (c.cfg.scheme == HDNS_IF_SSP_RK3) ssp_stage(sa, i, dt, fp32);
+    if (c.cfg.scheme == HDNS_IF_SSP_RK3 || c.cfg.scheme == HDNS_IF_HEUN) shu_osher_stage(sa, c.cfg.scheme, i, dt, fp32);
     XArgs xa = last ? fin : mid;
     if (rec) {
       const size_t cb = c.comp * c.elem;
@@ -1302,7 +1310,7 @@ hdns_status adj_alloc(hdns_ctx *ctx) {
   if (c.nbox != 1) return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint runs on a single-box context");
   if (c.cfg.velocity_forcing != HDNS_NO_FORCING && c.cfg.velocity_forcing != HDNS_CONSTANT_POWER)
     return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint supports NoForcing and ConstantPower velocity forcing");
-  if (c.cfg.scheme == HDNS_IF_SSP_RK3)
+  if (c.cfg.scheme == HDNS_IF_SSP_RK3 || c.cfg.scheme == HDNS_IF_HEUN)
     return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint supports the low-storage schemes IFLowStorageRK3 and IFLowStorageRK4");
   const size_t cb = c.comp * c.elem, realb = c.fp64 ? 8 : 4;
   const size_t vol = (size_t)c.geo.n * c.geo.n * c.geo.n;

@@ -191,7 +191,7 @@ struct PipelineImpl final : Pipeline {
     else c.sched[0] ? yfwd_v<YT[1], 0>(c) : yfwd_v<YT[0], 0>(c);
   }
 
-  // SSP: the IFSSPRK3 stage update, built for schedule 0 only (the scheme always runs it)
+  // SSP: the Shu-Osher stage update (IFSSPRK3, IFHeun), built for schedule 0 only (those schemes always run it)
   template <int V, bool SSP = false>
   void zf_v(Ctx &c, const StageArgs &sa) {
     constexpr int ZN = ZF_NT[V];

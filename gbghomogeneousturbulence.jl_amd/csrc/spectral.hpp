@@ -36,7 +36,7 @@ struct StageArgs {
   BandSpec band;
   const double *scalars;  // per box [8]: [0] = the ConstantPower amplitude at this stage's state
   double lin_rate;        // Linear forcing: the rate A folded into the band's exponent, else 0
-  // IFSSPRK3 (src/Propagator.jl:599-691), read by the kernels built with SSP:
+  // Shu-Osher form (IFSSPRK3 src/Propagator.jl:599-691, IFHeun :456-517), read by the kernels built with SSP:
   // q <- cb E^pb (q + dt N) + E^pa (ca h), with E the factor over the half step sub = dt / 2 and
   // h the step's initial state, which stage 0 saves
   int ssp;
@@ -154,7 +154,8 @@ __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, con
 //                                                                      src/Forcing.jl:1104-1117)
 //   N_phi = -mask i k . FFT_z(PF_flux) - G u_z                        (src/Generator.jl:648-730)
 //   h <- a h + dt N;  q <- E (q + b h);  h <- E h  (not at the last)  (src/Propagator.jl:366-391)
-//   or, for IFSSPRK3, q <- cb E^pb (q + dt N) + E^pa (ca h), h = u_n   (src/Propagator.jl:599-669)
+//   or, for IFSSPRK3 and IFHeun, q <- cb E^pb (q + dt N) + E^pa (ca h), h = u_n
+//                                                                      (src/Propagator.jl:456-517, :599-669)
 //   and at the last stage the state Leray projection                  (src/Propagator.jl:401, :677).
 // E = exp(coef |k|^2) is evaluated in FP64 per mode and rounded to T.  A thread owns the modes
 // its last transform pass produces: kz bins n = j + q N/R of column col, for its butterflies

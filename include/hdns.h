@@ -33,8 +33,8 @@ typedef enum {
 } hdns_status;
 
 enum { HDNS_FP32_BULK = 0, HDNS_FP64 = 1 };
-/* IFLowStorageRK3, IFLowStorageRK4 (src/Propagator.jl:264-320) and IFSSPRK3 (:596-691) */
-enum { HDNS_IF_LOW_STORAGE_RK3 = 3, HDNS_IF_LOW_STORAGE_RK4 = 4, HDNS_IF_SSP_RK3 = 5 };
+/* IFLowStorageRK3, IFLowStorageRK4 (src/Propagator.jl:264-320), IFSSPRK3 (:596-691) and IFHeun (:420-544) */
+enum { HDNS_IF_LOW_STORAGE_RK3 = 3, HDNS_IF_LOW_STORAGE_RK4 = 4, HDNS_IF_SSP_RK3 = 5, HDNS_IF_HEUN = 6 };
 enum { HDNS_TWO_THIRDS = 0 };
 /* NoForcing, ConstantPower, Linear, VarianceController and VarianceRescaling (src/Forcing.jl:464,
  * :714-719, :600-642, :817-871, :566-597) */
@@ -48,13 +48,13 @@ enum { HDNS_FIELD_VELOCITY = 0, HDNS_FIELD_SCALAR = 1, HDNS_FIELD_VORTICITY = 2 
 
 /* Everything a run fixes at construction: FixedDomain (src/SimulationDomain.jl:295-328),
  * FlowConditions (src/Generator.jl:81-147), the velocity forcings (src/Forcing.jl:566-1117),
- * the scheme (src/Propagator.jl:264-320, :596-691), and the tracer set (src/Particles.jl:135-176). */
+ * the scheme (src/Propagator.jl:264-320, :456-691), and the tracer set (src/Particles.jl:135-176). */
 typedef struct {
   int32_t device;            /* HIP device ordinal */
   int32_t grid_points;       /* N (the box is N^3): a power of two in 16..1024, or 48, 96, 192 */
   double side_length;        /* L */
   int32_t precision;         /* HDNS_FP32_BULK or HDNS_FP64 */
-  int32_t scheme;            /* HDNS_IF_LOW_STORAGE_RK3 / _RK4, HDNS_IF_SSP_RK3 */
+  int32_t scheme;            /* HDNS_IF_LOW_STORAGE_RK3 / _RK4, HDNS_IF_SSP_RK3, HDNS_IF_HEUN */
   int32_t dealiasing;        /* HDNS_TWO_THIRDS */
   int32_t has_scalar;        /* evolve the passive scalar */
   double viscosity;          /* nu > 0 */

@@ -378,6 +378,8 @@ def step(d: Domain, cond: Conditions, scheme: str, uh: np.ndarray, ph: np.ndarra
     """
     if scheme == "ssprk3":
         return ssp_step(d, cond, uh, ph, dt, dep, coeff_dtype)
+    if scheme == "heun":
+        return heun_step(d, cond, uh, ph, dt, dep)
     a, b, dd = scheme_coefficients(scheme, coeff_dtype)
     S = len(a)
     uh = uh.copy()
@@ -462,6 +464,31 @@ def ssp_step(d: Domain, cond: Conditions, uh: np.ndarray, ph: np.ndarray | None,
     if ph is not None:
         p = ep_full * (w13 * ph) + w23 * (ep_half * (p + dt * m2))
     if cond.rescaling is not None:   # finalize! before the state projections (:674-677)
+        q = variance_rescaling(d, cond, q)
+    q = reality_projection(d, leray(d, q))
+    if p is not None:
+        p = reality_projection(d, p)
+    return q, p, deposit(d, q, p)
+
+
+def heun_step(d: Domain, cond: Conditions, uh: np.ndarray, ph: np.ndarray | None, dt: float,
+              dep: Deposit | None = None):
+    """One IFHeun ``step!``, then the closing deposit (src/Propagator.jl:456-544):
+        u* = E(dt) (u + dt N(u)),   u_new = E(dt) u + dt/2 (E(dt) N(u) + N(u*))
+    for u_hat and phi_hat in lockstep, then finalize! and the state projections (:519-538)."""
+    if dep is None:
+        dep = deposit(d, uh, ph)
+    eu, ep = linear_factor(d, cond, dt), np.exp(-cond.scalar_diffusivity * dt * d.k2)
+    n1 = velocity_rhs(d, cond, uh, dep.u, dep.w)
+    m1 = scalar_rhs(d, cond, uh, dep.u, dep.phi) if ph is not None else None
+    us = eu * (uh + dt * n1)
+    ps = ep * (ph + dt * m1) if ph is not None else None
+    u, w = irfft3(us, d.n), irfft3(curl(d, us), d.n)
+    n2 = velocity_rhs(d, cond, us, u, w)
+    m2 = scalar_rhs(d, cond, us, u, irfft3(ps, d.n)) if ph is not None else None
+    q = eu * uh + dt / 2 * (eu * n1) + dt / 2 * n2
+    p = ep * ph + dt / 2 * (ep * m1) + dt / 2 * m2 if ph is not None else None
+    if cond.rescaling is not None:
         q = variance_rescaling(d, cond, q)
     q = reality_projection(d, leray(d, q))
     if p is not None:

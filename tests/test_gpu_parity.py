@@ -70,11 +70,11 @@ def test_deposit_matches_the_oracle(g, n, dtype, tol):
 
 
 def _scheme(g, name):
-    return {"rk3": g.IFLowStorageRK3, "rk4": g.IFLowStorageRK4, "ssprk3": g.IFSSPRK3}[name]
+    return {"rk3": g.IFLowStorageRK3, "rk4": g.IFLowStorageRK4, "ssprk3": g.IFSSPRK3, "heun": g.IFHeun}[name]
 
 
 @pytest.mark.parametrize("n", [32, 48, 96])
-@pytest.mark.parametrize("scheme", ["rk3", "rk4", "ssprk3"])
+@pytest.mark.parametrize("scheme", ["rk3", "rk4", "ssprk3", "heun"])
 @pytest.mark.parametrize("forcing", [None, 0.1])
 def test_one_step_fp64_matches_the_oracle(g, n, scheme, forcing):
     """48 and 96 carry radix-3 passes, and 3 | N puts the boundary shell 9|m|^2 = N^2 inside the
@@ -501,7 +501,7 @@ def test_linear_forcing_grows_a_shear_wave_exactly_on_the_device(g):
     d = dns.Domain(n)
     band = d.shell_band(1, 2)
     expected = uh * np.where(band, np.exp((A - nu * d.k2) * steps * dt), np.exp(-nu * d.k2 * steps * dt))
-    for scheme in ("rk4", "ssprk3"):
+    for scheme in ("rk4", "ssprk3", "heun"):
         with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(uh, None),
                           field_integration_scheme=_scheme(g, scheme)) as sim:
             for _ in range(steps):
@@ -510,11 +510,12 @@ def test_linear_forcing_grows_a_shear_wave_exactly_on_the_device(g):
         assert np.max(np.abs(got - expected)) <= 1e-12 * np.max(np.abs(expected)), scheme
 
 
+@pytest.mark.parametrize("scheme", ["ssprk3", "heun"])
 @pytest.mark.parametrize("forcing", ["constant_power", "linear"])
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 1e-5)])
-def test_ssprk3_ten_steps_match_the_oracle(g, forcing, dtype, tol):
-    """IFSSPRK3 (src/Propagator.jl:599-691) with the scalar and the mean gradient, 10 steps against
-    the oracle's Shu-Osher stages; checked after 1 and 10 steps."""
+def test_shu_osher_schemes_ten_steps_match_the_oracle(g, scheme, forcing, dtype, tol):
+    """IFSSPRK3 (src/Propagator.jl:599-691) and IFHeun (:456-544) with the scalar and the mean
+    gradient, 10 steps against the oracle's own arrangement of each; checked after 1 and 10 steps."""
     n, dt, nu = 32, 2e-3, 0.02
     dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
     if forcing == "linear":
@@ -529,14 +530,14 @@ def test_ssprk3_ten_steps_match_the_oracle(g, forcing, dtype, tol):
     d = dns.Domain(n)
     uo, po, dep = u, p, None
     with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=1.0,
-                      field_integration_scheme=g.IFSSPRK3) as sim:
+                      field_integration_scheme=_scheme(g, scheme)) as sim:
         for s in range(1, 11):
             sim.advance(dt)
-            uo, po, dep = dns.step(d, oc, "ssprk3", uo, po, dt, dep)
+            uo, po, dep = dns.step(d, oc, scheme, uo, po, dt, dep)
             if s in (1, 10):
                 st = sim.state()
                 eu, ep = rel(st.velocity_spectral, uo), rel(st.scalar_spectral, po)
-                print(f"ssprk3 {forcing} {np.dtype(dtype).name} step {s}: u {eu:.2e} phi {ep:.2e}", flush=True)
+                print(f"{scheme} {forcing} {np.dtype(dtype).name} step {s}: u {eu:.2e} phi {ep:.2e}", flush=True)
                 assert eu < tol and ep < tol
                 assert sim.max_speed() == pytest.approx(dep.max_speed, rel=10 * tol)
 

@@ -30,6 +30,7 @@ def test_recommended_cfl_is_the_base_ceiling_under_two_thirds(g):
     assert g.recommended_cfl(g.IFLowStorageRK4, dom) == 0.96
     assert g.recommended_cfl(g.IFLowStorageRK3, dom) == 0.5
     assert g.recommended_cfl(g.IFSSPRK3, dom) == 0.5   # src/Propagator.jl:911, the RK3 ceiling
+    assert g.recommended_cfl(g.IFHeun, dom) == 0.25    # src/Propagator.jl:906
 
 
 def test_cfl_bound_takes_the_smallest_of_its_three_bounds(g):

@@ -221,7 +221,8 @@ def _integrate(scheme, n, steps, T, uh, ph, cond):
     return u, p
 
 
-@pytest.mark.parametrize("scheme,lo,hi", [("rk3", 5.0, 12.0), ("rk4", 10.0, 24.0), ("ssprk3", 5.0, 12.0)])
+@pytest.mark.parametrize("scheme,lo,hi", [("rk3", 5.0, 12.0), ("rk4", 10.0, 24.0), ("ssprk3", 5.0, 12.0),
+                                           ("heun", 3.0, 6.0)])
 def test_temporal_self_convergence(scheme, lo, hi):
     n = 16
     d, uh = _band_limited(n, 1, shell_frac=0.5)
@@ -310,7 +311,7 @@ def test_linear_forcing_grows_band_modes_at_a_minus_nu_k2_exactly():
     cond = dns.Conditions(viscosity=nu, linear=(A, 1, 2))
     band = d.shell_band(1, 2)
     expected = uh * np.where(band, np.exp((A - nu * d.k2) * steps * dt), np.exp(-nu * d.k2 * steps * dt))
-    for scheme in ("rk3", "rk4", "ssprk3"):
+    for scheme in ("rk3", "rk4", "ssprk3", "heun"):
         v = uh
         for _ in range(steps):
             v, _, _ = dns.step(d, cond, scheme, v, None, dt)
@@ -506,3 +507,53 @@ def test_rescaling_after_each_step_holds_the_target(scheme):
     for _ in range(5):
         u, _, dep = dns.step(d, cond, scheme, u, None, 1e-2, dep)
         assert dns.reductions(d, cond, u, None)["kinetic_energy"] == pytest.approx(0.7, rel=1e-13)
+
+
+def _lawson_heun(d, cond, uh, ph, dt):
+    """IFHeun as the Butcher tableau (c = 0, 1; b = 1/2, 1/2) on v = E(-tau) q, independent of the
+    reference's predictor/corrector arrangement."""
+    kap = cond.scalar_diffusivity
+
+    def g(tau, v, w):
+        q = dns.linear_factor(d, cond, tau) * v
+        p = np.exp(-kap * tau * d.k2) * w
+        dep = dns.deposit(d, q, p)
+        return (dns.linear_factor(d, cond, -tau) * dns.velocity_rhs(d, cond, q, dep.u, dep.w),
+                np.exp(kap * tau * d.k2) * dns.scalar_rhs(d, cond, q, dep.u, dep.phi))
+
+    k1 = g(0.0, uh, ph)
+    k2 = g(dt, uh + dt * k1[0], ph + dt * k1[1])
+    q = dns.linear_factor(d, cond, dt) * (uh + dt / 2 * (k1[0] + k2[0]))
+    p = np.exp(-kap * dt * d.k2) * (ph + dt / 2 * (k1[1] + k2[1]))
+    return dns.reality_projection(d, dns.leray(d, q)), dns.reality_projection(d, p)
+
+
+@pytest.mark.parametrize("forcing", ["none", "constant_power", "linear"])
+def test_heun_equals_the_lawson_butcher_form(forcing):
+    n = 16
+    d, uh = _band_limited(n, 4, shell_frac=0.6)
+    _, ph = _band_limited(n, 5, shell_frac=0.6)
+    ph = ph[0]
+    kw = {"constant_power": {"forcing": (0.2, 1, 2)}, "linear": {"linear": (0.4, 1, 2)}}.get(forcing, {})
+    cond = dns.Conditions(viscosity=0.03, schmidt=0.8, mean_gradient=0.7, **kw)
+    got_u, got_p, _ = dns.step(d, cond, "heun", uh, ph, 0.05)
+    want_u, want_p = _lawson_heun(d, cond, uh, ph, 0.05)
+    assert np.max(np.abs(got_u - want_u)) <= 1e-13 * np.max(np.abs(want_u))
+    assert np.max(np.abs(got_p - want_p)) <= 1e-13 * np.max(np.abs(want_p))
+
+
+def test_heun_is_the_shu_osher_ssprk2():
+    """The device runs IFHeun as SSPRK(2,2): u' = 1/2 E u + 1/2 (u* + dt N(u*)), u* = E (u + dt N(u));
+    the rearrangement equals the reference's E u + dt/2 (E N1 + N2) to roundoff."""
+    n = 16
+    d, uh = _band_limited(n, 6, shell_frac=0.6)
+    cond = dns.Conditions(viscosity=0.03, forcing=(0.2, 1, 2))
+    dt = 0.05
+    e = dns.linear_factor(d, cond, dt)
+    dep = dns.deposit(d, uh, None)
+    us = e * (uh + dt * dns.velocity_rhs(d, cond, uh, dep.u, dep.w))
+    ds = dns.deposit(d, us, None)
+    so = 0.5 * (e * uh) + 0.5 * (us + dt * dns.velocity_rhs(d, cond, us, ds.u, ds.w))
+    want = dns.reality_projection(d, dns.leray(d, so))
+    got, _, _ = dns.step(d, cond, "heun", uh, None, dt)
+    assert np.max(np.abs(got - want)) <= 1e-14 * np.max(np.abs(want))
