@@ -5,7 +5,7 @@ The folder name carries a dot, so it is imported by path; ``load_package()`` in
 ``__graft_entry__.py`` registers it as the module ``gbg_amd``.
 """
 from . import _lib
-from .api import (ConstantPower, Ensemble, EnsembleBox, EnsembleMember, FixedDomain, FlowConditions, Heun,
+from .api import (AB, ABM, ConstantPower, Ensemble, EnsembleBox, EnsembleMember, FixedDomain, FlowConditions, Heun,
                   IFHeun, IFLowStorageRK3, IFLowStorageRK4, IFSSPRK3, Linear, NoForcing, VarianceController,
                   VarianceRescaling, ResolvedFields, Simulation, Tracers, TwoThirds,
                   StationarityCriterion, cfl_bound, integral_length, predicted_viscosity,
@@ -13,7 +13,7 @@ from .api import (ConstantPower, Ensemble, EnsembleBox, EnsembleMember, FixedDom
 from .seeding import random_phase_scalar, random_phase_velocity, uniform_positions
 
 __all__ = [
-    "ConstantPower", "Ensemble", "EnsembleBox", "EnsembleMember", "FixedDomain", "FlowConditions", "Heun",
+    "AB", "ABM", "ConstantPower", "Ensemble", "EnsembleBox", "EnsembleMember", "FixedDomain", "FlowConditions", "Heun",
     "IFHeun", "IFLowStorageRK3", "IFLowStorageRK4", "IFSSPRK3", "Linear", "NoForcing", "VarianceController",
     "VarianceRescaling", "ResolvedFields", "Simulation", "Tracers", "TwoThirds", "predicted_viscosity",
     "recommended_cfl", "cfl_bound", "integral_length", "StationarityCriterion", "warmup", "truncate_to_band", "random_phase_scalar", "random_phase_velocity",

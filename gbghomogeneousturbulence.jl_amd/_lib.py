@@ -21,6 +21,7 @@ FP32_BULK, FP64 = 0, 1
 RK3, RK4, SSPRK3, HEUN = 3, 4, 5, 6
 NO_FORCING, CONSTANT_POWER, LINEAR, VARIANCE_CONTROLLER, VARIANCE_RESCALING = 0, 1, 2, 3, 4
 OVER_BAND, OVER_TOTAL = 0, 1
+P_HEUN, P_ABM, P_AB = 0, 1, 2
 FIELD_VELOCITY, FIELD_SCALAR, FIELD_VORTICITY = 0, 1, 2
 
 
@@ -35,6 +36,7 @@ class HdnsConfig(ctypes.Structure):
         ("interpolation", c_int32), ("particle_scheme", c_int32),
         ("store_physical", c_int32), ("forcing_over", c_int32),
         ("forcing_target", c_double), ("forcing_gain", c_double), ("forcing_time", c_double),
+        ("particle_steps", c_int32), ("reserved", c_int32),
     ]
 
 
@@ -70,6 +72,9 @@ _SIGS = {
     "hdns_advect": [c_void_p, c_double],
     "hdns_deposit_tracers": [c_void_p],
     "hdns_tracer_samples": [c_void_p, _D, _D],
+    "hdns_tracer_trail": [c_void_p, POINTER(c_int32), POINTER(c_int32), _D],
+    "hdns_tracer_trail_sample": [c_void_p, c_int32, _D],
+    "hdns_tracer_time_derivative": [c_void_p, c_int32, _D],
     "hdns_tracer_cells": [c_void_p, POINTER(c_int64)],
     "hdns_traversal_order": [c_void_p, POINTER(c_int32)],
     "hdns_frame_tables": [c_void_p, POINTER(c_int32), POINTER(c_int32), POINTER(c_int32), POINTER(c_uint8)],

@@ -303,9 +303,8 @@ def cfl_bound(speed: float, dx: float, cfl: float, diffusivity: float, side_leng
               max_dt: float = math.inf, step: int = 0) -> float:
     """The adaptive step, the smallest of three bounds (src/SimulationRun.jl:1051-1096):
     advective cfl dx / max|u| (cfl dx at rest), resolution 1 / (D kappa_min^2) with D the largest
-    diffusivity and kappa_min = 2 pi / L, and the caller's max_dt.  The start-up bound of
-    src/SimulationRun.jl:1118-1129 applies only to tracer schemes that integrate over more than one
-    sample, so it is absent under Heun.  A non-finite speed is the blow-up error."""
+    diffusivity and kappa_min = 2 pi / L, and the caller's max_dt.  The fourth, the start-up bound
+    of a multistep tracer scheme, is ``startup_bound``.  A non-finite speed is the blow-up error."""
     if not math.isfinite(speed):
         raise RuntimeError(f"non-finite velocity at step {step}: the run appears to have blown up")
     advective = cfl * dx / speed if speed > 0 else cfl * dx
@@ -314,11 +313,75 @@ def cfl_bound(speed: float, dx: float, cfl: float, diffusivity: float, side_leng
     return min(advective, resolution, max_dt)
 
 
-class Heun:
-    """ABM(1), the default tracer scheme (src/ParticlePropagator.jl:165)."""
+def startup_growth(order: int) -> float:
+    """How much the step may grow per iteration during the start-up (src/SimulationRun.jl:1116)."""
+    return 2.0 if order <= 5 else (1.5 if order <= 7 else 1.25)
+
+
+def startup_bound(scheme, trail_times, dt: float, viscosity: float, dissipation: float) -> float:
+    """The start-up bound of shortened_first_steps (src/SimulationRun.jl:1099-1129): inf for a
+    scheme on one sample; with one sample in the trail tau (dt / tau)^(p/3), tau = sqrt(nu / eps)
+    the Kolmogorov time of the current state; after that the growth factor times the last step.
+    ``dt`` is the smallest of the other bounds; ``trail_times`` newest first."""
+    if scheme is None or scheme.steps <= 1:
+        return math.inf
+    if len(trail_times) > 1:
+        return startup_growth(scheme.order) * (trail_times[0] - trail_times[1])
+    tau = math.sqrt(viscosity / dissipation) if dissipation > 0 else math.inf
+    return tau * (dt / tau) ** (scheme.order / 3) if math.isfinite(tau) and tau > 0 else math.inf
+
+
+class ABM:
+    """The k-step Adams-Bashforth-Moulton predictor-corrector for the tracers, of order k + 1,
+    k = 1..7 (src/ParticlePropagator.jl:58-117): an AB step over the trail's k newest samples, the
+    velocity gathered at the predicted positions, and the Adams-Moulton corrector; the weights come
+    from the samples' actual times.  Its trail keeps k + 1 samples.  Until it holds k, the steps are
+    ABM(1), ABM(2), ..."""
+    code = L.P_ABM
+
+    def __init__(self, k: int):
+        if not (isinstance(k, int) and 1 <= k <= 7):
+            raise ValueError(f"`ABM(k)` takes a number of steps `k` from 1 to 7, got {k}: the four-point "
+                             "Gauss-Legendre rule that integrates the weights is exact only through the "
+                             "degree `k = 7` needs, so a longer scheme would be given silently wrong coefficients")
+        self.steps = k
+        self.corrects = True
+        self.order = k + 1
+        self.trail_capacity = k + 1
+
+    def __repr__(self) -> str:
+        return f"ABM({self.steps})"
+
+
+class Heun(ABM):
+    """ABM(1), the default tracer scheme (src/ParticlePropagator.jl:118-165)."""
+    code = L.P_HEUN
+
+    def __init__(self):
+        super().__init__(1)
 
     def __repr__(self) -> str:
         return "Heun()"
+
+
+class AB:
+    """Variable-step Adams-Bashforth of order k for the tracers, k = 1..8, TurTLE's scheme at k = 4
+    (src/ParticlePropagator.jl:168-234): one gather per step (the closing sample).  Its trail keeps
+    k samples.  Until it holds k, the steps are ABM(1), ABM(2), ..."""
+    code = L.P_AB
+
+    def __init__(self, k: int):
+        if not (isinstance(k, int) and 1 <= k <= 8):
+            raise ValueError(f"`AB(k)` takes a number of steps `k` from 1 to 8, got {k}: the four-point "
+                             "Gauss-Legendre rule that integrates the weights is exact only through degree 7, "
+                             "so a higher order would be given silently wrong coefficients")
+        self.steps = k
+        self.corrects = False
+        self.order = k
+        self.trail_capacity = k
+
+    def __repr__(self) -> str:
+        return f"AB({self.steps})"
 
 
 # --- state and tracers ---------------------------------------------------------------------
@@ -384,7 +447,8 @@ def _state_arrays(state: ResolvedFields, domain: FixedDomain):
 
 
 def _config(domain: FixedDomain, conditions: FlowConditions, scheme, carries_scalar: bool, mean_gradient: float,
-            tracers, tracer_capacity, keep_physical: bool, keep_vorticity: bool) -> L.HdnsConfig:
+            tracers, tracer_capacity, keep_physical: bool, keep_vorticity: bool,
+            particle_scheme=None) -> L.HdnsConfig:
     """The C-ABI configuration of one box."""
     cfg = L.HdnsConfig()
     cfg.device = domain.device
@@ -411,6 +475,9 @@ def _config(domain: FixedDomain, conditions: FlowConditions, scheme, carries_sca
         cfg.forcing_target = f.target
         cfg.forcing_over = L.OVER_TOTAL if f.over == "total" else L.OVER_BAND
     cfg.mean_gradient = float(mean_gradient)
+    ps = particle_scheme if particle_scheme is not None else Heun()
+    cfg.particle_scheme = ps.code
+    cfg.particle_steps = ps.steps
     cap = tracers.count if tracers is not None else 0
     cfg.tracer_capacity = int(tracer_capacity if tracer_capacity is not None else cap)
     keep = keep_physical or tracers is not None
@@ -435,7 +502,7 @@ class Simulation:
                  field_integration_scheme=IFLowStorageRK4, tracers: Tracers | None = None,
                  particle_integration_scheme=None, mean_gradient: float = 0.0, cfl: float | None = None,
                  max_dt: float = math.inf, keep_vorticity: bool = False, keep_physical: bool = True,
-                 tracer_capacity: int | None = None):
+                 tracer_capacity: int | None = None, shortened_first_steps: bool = False):
         carries_scalar = state.scalar_spectral is not None
         u, p = _state_arrays(state, domain)
         self.mean_gradient = float(mean_gradient)
@@ -445,6 +512,10 @@ class Simulation:
         self.conditions = conditions
         self.field_integration_scheme = field_integration_scheme
         self.particle_integration_scheme = particle_integration_scheme or (Heun() if tracers is not None else None)
+        if self.particle_integration_scheme is not None and not isinstance(self.particle_integration_scheme, (ABM, AB)):
+            raise ValueError("the particle integration scheme must be Heun(), ABM(k) or AB(k), got "
+                             f"{self.particle_integration_scheme!r}")
+        self.shortened_first_steps = bool(shortened_first_steps)
         self.tracers = tracers
         self.cfl = recommended_cfl(field_integration_scheme, domain) if cfl is None else float(cfl)
         self.max_dt = float(max_dt)
@@ -452,7 +523,7 @@ class Simulation:
         self.step = 0
         self.carries_scalar = carries_scalar
         cfg = _config(domain, conditions, field_integration_scheme, carries_scalar, mean_gradient, tracers,
-                      tracer_capacity, keep_physical, keep_vorticity)
+                      tracer_capacity, keep_physical, keep_vorticity, self.particle_integration_scheme)
         h = c_void_p()
         L.check(L.lib().hdns_ctx_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._ctx = h
@@ -690,14 +761,39 @@ class Simulation:
         return J.value, gu, gp
 
     def cfl_timestep(self) -> float:
-        """The advective/resolution/max_dt bound (src/SimulationRun.jl:1051-1096)."""
+        """The advective/resolution/max_dt bound and, under shortened_first_steps, the start-up
+        bound of a multistep tracer scheme (src/SimulationRun.jl:1051-1129)."""
         c = self.conditions
         diffusivity = max(c.viscosity, c.scalar_diffusivity) if self.carries_scalar else c.viscosity
-        return cfl_bound(self.max_speed(), self.domain.grid_spacing, self.cfl, diffusivity,
-                         self.domain.side_length, self.max_dt, step=self.step)
+        dt = cfl_bound(self.max_speed(), self.domain.grid_spacing, self.cfl, diffusivity,
+                       self.domain.side_length, self.max_dt, step=self.step)
+        if self.shortened_first_steps and self.tracers is not None:
+            eps = self.reductions()["dissipation"]
+            dt = min(startup_bound(self.particle_integration_scheme, self.trail_times(), dt, c.viscosity, eps), dt)
+        return dt
+
+    def trail_times(self) -> list:
+        """The times of the velocity trail's samples at the tracers, newest first (src/Particles.jl:404-526)."""
+        filled, cap = c_int32(), c_int32()
+        times = np.zeros(8)
+        self._call("hdns_tracer_trail", ctypes.byref(filled), ctypes.byref(cap), L.dptr(times))
+        return [float(t) for t in times[: filled.value]]
+
+    def trail_sample(self, offset: int = 0) -> np.ndarray:
+        """The trail's velocity sample `offset` steps back from the newest (offset <= 0), [3, M]."""
+        v = np.zeros((3, self.tracers.count))
+        self._call("hdns_tracer_trail_sample", int(offset), L.dptr(v))
+        return v
+
+    def time_derivative(self, order: int = 1) -> np.ndarray:
+        """time_derivative!(out, trail; order) (src/Particles.jl:597-631): d^m v/dt^m at the newest
+        sample of the polynomial through every sample held; order 1 is the Lagrangian acceleration."""
+        out = np.zeros((3, self.tracers.count))
+        self._call("hdns_tracer_time_derivative", int(order), L.dptr(out))
+        return out
 
     def advect(self, dt: float) -> None:
-        """advect!(Heun, ...) (src/ParticlePropagator.jl:349-372) with its epoch check."""
+        """advect!(scheme, ...) (src/ParticlePropagator.jl:349-372) with its epoch check."""
         if self.tracers is None:
             raise ValueError("this run carries no tracers")
         if self.trail_epoch != (self.fields_epoch - 1, self.particles_epoch):

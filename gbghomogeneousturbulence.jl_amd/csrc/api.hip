@@ -114,6 +114,31 @@ Coeffs scheme_coefficients(int scheme, bool fp32) {
   return k;
 }
 
+// Fornberg's weights (Math. Comp. 51 (1988) 699-706) for the derivative of order `order` at 0 of
+// the polynomial through values at the n nodes (src/Particles.jl:533-565): out[j] multiplies
+// node j's value.
+void derivative_weights(const double *x, int n, int order, double *out) {
+  double c[MAX_TRAIL][MAX_TRAIL] = {};
+  c[0][0] = 1.0;
+  double prod_prev = 1.0;
+  for (int i = 1; i < n; ++i) {
+    const int top = std::min(i, order);
+    double prod = 1.0;
+    for (int j = 0; j < i; ++j) {
+      const double gap = x[i] - x[j];
+      prod *= gap;
+      if (j == i - 1) {   // the new node's row, from the previous node's before that row moves
+        for (int k = top; k > 0; --k) c[i][k] = prod_prev * (k * c[i - 1][k - 1] - x[i - 1] * c[i - 1][k]) / prod;
+        c[i][0] = -prod_prev * x[i - 1] * c[i - 1][0] / prod;
+      }
+      for (int k = top; k > 0; --k) c[j][k] = (x[i] * c[j][k] - k * c[j][k - 1]) / gap;
+      c[j][0] = x[i] * c[j][0] / gap;
+    }
+    prod_prev = prod;
+  }
+  for (int j = 0; j < n; ++j) out[j] = c[j][order];
+}
+
 // src/ParticlePropagator.jl:261-280: the integral over [0, h] of each Lagrange basis polynomial
 // on the sample times, by 4-point Gauss-Legendre.
 void adams_weights(const double *nodes, int n, double h, double *out) {
@@ -363,8 +388,21 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
   }
   if (!isfinite(k.mean_gradient)) return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "mean_gradient must be finite");
   if (k.tracer_capacity < 0) return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "tracer_capacity must be >= 0");
-  if (k.tracer_capacity > 0 && (k.interpolation != HDNS_CATMULL_ROM || k.particle_scheme != HDNS_HEUN))
-    return fail(nullptr, HDNS_ERR_UNSUPPORTED, "tracers support CatmullRom interpolation and the Heun scheme");
+  if (k.tracer_capacity > 0 && k.interpolation != HDNS_CATMULL_ROM)
+    return fail(nullptr, HDNS_ERR_UNSUPPORTED, "tracers support CatmullRom interpolation");
+  if (k.particle_scheme != HDNS_HEUN && k.particle_scheme != HDNS_ABM && k.particle_scheme != HDNS_AB)
+    return fail(nullptr, HDNS_ERR_UNSUPPORTED, "the particle scheme must be Heun, ABM(k) or AB(k)");
+  // src/ParticlePropagator.jl:106-115, :225-234
+  if (k.particle_scheme == HDNS_ABM && !(k.particle_steps >= 1 && k.particle_steps <= 7))
+    return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                "`ABM(k)` takes a number of steps `k` from 1 to 7, got " + std::to_string(k.particle_steps) +
+                    ": the four-point Gauss-Legendre rule that integrates the weights is exact only through the "
+                    "degree `k = 7` needs");
+  if (k.particle_scheme == HDNS_AB && !(k.particle_steps >= 1 && k.particle_steps <= 8))
+    return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                "`AB(k)` takes a number of steps `k` from 1 to 8, got " + std::to_string(k.particle_steps) +
+                    ": the four-point Gauss-Legendre rule that integrates the weights is exact only through "
+                    "degree 7");
   if (k.tracer_capacity > (int64_t)INT32_MAX)
     return fail(nullptr, HDNS_ERR_UNSUPPORTED, "tracer_capacity above 2^31-1 is not supported");
 
@@ -399,6 +437,9 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
   const size_t realb = c.fp64 ? 8 : 4;
   c.comp = (size_t)c.geo.k * c.geo.k * c.geo.kxp;
   c.stages = k.scheme == HDNS_IF_LOW_STORAGE_RK4 ? 5 : (k.scheme == HDNS_IF_HEUN ? 2 : 3);
+  // trail_capacity (src/ParticlePropagator.jl:248-249): ABM(k) k + 1, AB(k) k; Heun is ABM(1)
+  c.trail_cap = k.particle_scheme == HDNS_AB ? k.particle_steps
+                                             : (k.particle_scheme == HDNS_ABM ? k.particle_steps + 1 : 2);
   c.band.on = k.velocity_forcing != HDNS_NO_FORCING;
   c.band.lo4 = (2 * k.lowest_shell - 1) * (2 * k.lowest_shell - 1);
   c.band.hi4 = k.highest_shell < 0 ? -1 : (2 * k.highest_shell + 1) * (2 * k.highest_shell + 1);
@@ -442,7 +483,7 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
   if (s == HDNS_OK && k.tracer_capacity > 0) {
     const size_t cap = (size_t)k.tracer_capacity;
     s = alloc(ctx, &c.pos, B * 3 * cap * realb);
-    if (s == HDNS_OK) s = alloc(ctx, &c.vel, B * 3 * cap * realb);
+    if (s == HDNS_OK) s = alloc(ctx, &c.vel, B * (size_t)c.trail_cap * 3 * cap * realb);
     if (s == HDNS_OK && k.has_scalar) s = alloc(ctx, &c.phis, B * cap * realb);
     if (s == HDNS_OK) s = alloc(ctx, (void **)&c.order, B * cap * sizeof(int));
     if (s == HDNS_OK) s = alloc(ctx, (void **)&c.tile_counts, B * c.ntiles * sizeof(int));
@@ -1028,6 +1069,8 @@ hdns_status hdns_tracers_upload(hdns_ctx *ctx, int64_t m, const double *position
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
   c.m = m;
   c.order_valid = false;
+  c.trail_filled = 0;   // new positions: the trail starts again
+  c.advected = false;
   if (m == 0) return HDNS_OK;
   const size_t realb = c.fp64 ? 8 : 4;
   char *pos = (char *)c.pos + (size_t)c.box * 3 * c.cfg.tracer_capacity * realb;
@@ -1086,10 +1129,37 @@ hdns_status hdns_advect(hdns_ctx *ctx, double dt) {
                 "increasing times");
   if (c.cfg.tracer_capacity == 0) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "this context carries no tracers");
   if (!c.deposited) return fail(ctx, HDNS_ERR_STATE, "no deposited velocity to advect the tracers with");
+  if (c.trail_filled == 0)
+    return fail(ctx, HDNS_ERR_STATE, "the trail holds no velocity sample at these positions: deposit the tracers first");
+  if (c.advected)
+    return fail(ctx, HDNS_ERR_STATE,
+                "the positions moved since the trail's newest sample: deposit the tracers before the next push");
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
-  double w[3];
-  hdns_heun_weights(dt, w);
-  c.pipe->advect(c, w[0], w[1], w[2]);
+  // src/ParticlePropagator.jl:349-372: the scheme's own step once the trail holds its k samples,
+  // else the predictor-corrector the samples present allow
+  const int k = c.cfg.particle_scheme == HDNS_HEUN ? 1 : c.cfg.particle_steps;
+  const bool full = c.trail_filled >= k;
+  AdamsArgs aa{};
+  aa.J = full ? k : c.trail_filled;
+  aa.corrected = !full || c.cfg.particle_scheme != HDNS_AB;
+  double nodes[MAX_TRAIL + 1];
+  const size_t per_slot = (size_t)3 * c.cfg.tracer_capacity;
+  for (int j = 0; j < aa.J; ++j) {
+    const int slot = (c.trail_newest - j + c.trail_cap) % c.trail_cap;
+    aa.slot[j] = (int64_t)(slot * per_slot);
+    nodes[j + 1] = c.trail_t[slot] - c.trail_t[c.trail_newest];   // 0, then negative
+  }
+  adams_weights(nodes + 1, aa.J, dt, aa.b);
+  if (aa.corrected) {
+    double beta[MAX_TRAIL + 1];
+    nodes[0] = dt;
+    adams_weights(nodes, aa.J + 1, dt, beta);
+    aa.c[0] = beta[0];
+    for (int j = 0; j < aa.J; ++j) aa.c[j + 1] = beta[j + 1] - aa.b[j];
+  }
+  c.pipe->advect(c, aa);
+  c.advected = true;
+  c.pending_dt = dt;
   return launched(ctx);
 }
 
@@ -1099,8 +1169,79 @@ hdns_status hdns_deposit_tracers(hdns_ctx *ctx) {
   if (c.cfg.tracer_capacity == 0) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "this context carries no tracers");
   if (!c.deposited) return fail(ctx, HDNS_ERR_STATE, "no deposited velocity to sample at the tracers");
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  // the ring (src/Particles.jl:454-490): a pushed set of positions takes the next slot, retiring the
+  // oldest sample of a full trail; unmoved positions refresh the newest sample in place
+  if (c.trail_filled == 0) {
+    c.trail_newest = 0;
+    c.trail_t[0] = 0.0;
+    c.trail_filled = 1;
+  } else if (c.advected) {
+    const double t = c.trail_t[c.trail_newest] + c.pending_dt;
+    c.trail_newest = (c.trail_newest + 1) % c.trail_cap;
+    c.trail_t[c.trail_newest] = t;
+    c.trail_filled = std::min(c.trail_filled + 1, c.trail_cap);
+  }
+  c.advected = false;
   c.pipe->sample(c);
   return launched(ctx);
+}
+
+hdns_status hdns_tracer_trail(hdns_ctx *ctx, int32_t *filled, int32_t *capacity, double *times) {
+  if (!ctx || !filled || !capacity) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "filled and capacity must not be NULL");
+  Ctx &c = ctx->c;
+  *filled = c.trail_filled;
+  *capacity = c.trail_cap;
+  if (times)
+    for (int j = 0; j < c.trail_filled; ++j) times[j] = c.trail_t[(c.trail_newest - j + c.trail_cap) % c.trail_cap];
+  return HDNS_OK;
+}
+
+hdns_status hdns_tracer_trail_sample(hdns_ctx *ctx, int32_t offset, double *velocity) {
+  if (!ctx || !velocity) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "velocity must not be NULL");
+  Ctx &c = ctx->c;
+  if (offset > 0 || offset <= -c.trail_filled)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT,
+                "a trail is indexed by the offset in steps from its newest sample (0, -1, ...); it holds " +
+                    std::to_string(c.trail_filled) + " sample(s), so it has no trail[" + std::to_string(offset) + "]");
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  const size_t realb = c.fp64 ? 8 : 4, cap = (size_t)c.cfg.tracer_capacity;
+  const int slot = (c.trail_newest + offset + c.trail_cap) % c.trail_cap;
+  const char *src = (const char *)c.vel + ((size_t)c.box * c.trail_cap + slot) * 3 * cap * realb;
+  return copy_real_out(ctx, src, 3 * (size_t)c.m, velocity);
+}
+
+hdns_status hdns_tracer_time_derivative(hdns_ctx *ctx, int32_t order, double *out) {
+  if (!ctx || !out) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "out must not be NULL");
+  Ctx &c = ctx->c;
+  if (order < 0) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "`order` must be non-negative, got " + std::to_string(order));
+  if (c.trail_filled < order + 1)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT,
+                "a derivative of order " + std::to_string(order) + " needs at least " + std::to_string(order + 1) +
+                    " samples, and the trail holds " + std::to_string(c.trail_filled));
+  if (c.m == 0) return HDNS_OK;
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  TrailCombine tc{};
+  tc.n = c.trail_filled;
+  double nodes[MAX_TRAIL];
+  const size_t per_slot = (size_t)3 * c.cfg.tracer_capacity;
+  for (int j = 0; j < tc.n; ++j) {
+    const int slot = (c.trail_newest - j + c.trail_cap) % c.trail_cap;
+    tc.slot[j] = (int64_t)(slot * per_slot);
+    nodes[j] = c.trail_t[slot] - c.trail_t[c.trail_newest];
+  }
+  if (order == 0) {   // the newest value itself
+    tc.n = 1;
+    tc.w[0] = 1.0;
+  } else {
+    derivative_weights(nodes, tc.n, order, tc.w);
+  }
+  const size_t realb = c.fp64 ? 8 : 4;
+  void *dev = nullptr;
+  HIP_TRY(ctx, hipMalloc(&dev, 3 * (size_t)c.m * realb));
+  c.pipe->trail_combine(c, tc, dev);
+  const hdns_status s = copy_real_out(ctx, dev, 3 * (size_t)c.m, out);
+  (void)hipFree(dev);
+  return s;
 }
 
 hdns_status hdns_tracer_samples(hdns_ctx *ctx, double *velocity, double *scalar) {
@@ -1108,7 +1249,8 @@ hdns_status hdns_tracer_samples(hdns_ctx *ctx, double *velocity, double *scalar)
   Ctx &c = ctx->c;
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
   const size_t realb = c.fp64 ? 8 : 4, cap = (size_t)c.cfg.tracer_capacity;
-  hdns_status s = copy_real_out(ctx, (const char *)c.vel + c.box * 3 * cap * realb, 3 * (size_t)c.m, velocity);
+  const char *newest = (const char *)c.vel + ((size_t)c.box * c.trail_cap + c.trail_newest) * 3 * cap * realb;
+  hdns_status s = copy_real_out(ctx, newest, 3 * (size_t)c.m, velocity);
   if (s != HDNS_OK || !scalar || !c.cfg.has_scalar) return s;
   return copy_real_out(ctx, (const char *)c.phis + c.box * cap * realb, (size_t)c.m, scalar);
 }

@@ -8,6 +8,7 @@
 
 #include "../../include/hdns.h"
 #include "adjoint.hpp"
+#include "tracers.hpp"
 
 namespace hdns {
 
@@ -38,7 +39,9 @@ struct Pipeline {
   virtual void yinv(Ctx &c) = 0;
   virtual void xpass(Ctx &c, const XArgs &xa) = 0;
   virtual void spectrum(Ctx &c, int which, double *bins) = 0;
-  virtual void advect(Ctx &c, double b1, double beta1, double c2) = 0;
+  virtual void advect(Ctx &c, const AdamsArgs &aa) = 0;
+  // out [3][m] of the selected box <- sum_j w_j (trail sample j), in the element type
+  virtual void trail_combine(Ctx &c, const TrailCombine &tc, void *out) = 0;
   virtual void sample(Ctx &c) = 0;
   virtual void cell_keys(Ctx &c, int64_t *keys) = 0;
   virtual void tile_sort(Ctx &c) = 0;
@@ -100,6 +103,14 @@ struct Ctx {
   int *order = nullptr, *tile_counts = nullptr, *tile_cursor = nullptr;
   int ntiles = 0;
   bool order_valid = false;
+  // the velocity trail at the tracers (src/Particles.jl:404-526): a ring of trail_cap slots on the
+  // device, [box][slot][3][cap]; the times and the ring's bookkeeping on the host
+  int trail_cap = 2;           // k + 1 for ABM(k) (Heun: 2), k for AB(k)
+  int trail_newest = 0;        // the newest sample's slot
+  int trail_filled = 0;        // samples held
+  double trail_t[MAX_TRAIL] = {0};
+  bool advected = false;       // the positions moved since the newest sample
+  double pending_dt = 0.0;     // that push's step: the next sample's time is the newest's plus it
   bool deposited = false;
   size_t bytes = 0;
   // optional per-kernel timing with events on the context's stream

@@ -285,19 +285,27 @@ struct PipelineImpl final : Pipeline {
     hipLaunchKernelGGL((k_spectrum<T>), dim3(1024), dim3(256), 0, c.stream, f, nc, c.geo, bins);
   }
   TracerArgs targs(const Ctx &c) const {
-    return TracerArgs{c.m, N, c.cfg.side_length, c.cfg.tracer_capacity, c.box_phys, c.ntiles};
+    return TracerArgs{c.m, N, c.cfg.side_length, c.cfg.tracer_capacity, c.box_phys, c.ntiles,
+                      (int64_t)c.trail_cap * 3 * c.cfg.tracer_capacity};
   }
   int tblocks(const Ctx &c) const { return (int)((c.m + 255) / 256); }
-  void advect(Ctx &c, double b1, double beta1, double c2) override {
+  void advect(Ctx &c, const AdamsArgs &aa) override {
     if (c.m == 0) return;
     hipLaunchKernelGGL((k_advect<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (T *)c.pos, (const T *)c.vel,
-                       (const T *)c.phys, c.order_valid ? c.order : nullptr, targs(c), T(b1), T(beta1), T(c2));
+                       (const T *)c.phys, c.order_valid ? c.order : nullptr, targs(c), aa);
+  }
+  void trail_combine(Ctx &c, const TrailCombine &tc, void *out) override {   // the selected box
+    if (c.m == 0) return;
+    const TracerArgs ta = targs(c);
+    hipLaunchKernelGGL((k_trail_combine<T>), dim3(tblocks(c)), dim3(256), 0, c.stream,
+                       (const T *)c.vel + (size_t)c.box * ta.vstride, ta, tc, (T *)out);
   }
   void sample(Ctx &c) override {
     if (c.m == 0) return;
     const size_t vol = (size_t)N * N * N;
     const T *phi = c.cfg.has_scalar ? (const T *)c.phys + 3 * vol : nullptr;
-    hipLaunchKernelGGL((k_sample<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const T *)c.pos, (T *)c.vel,
+    T *slot = (T *)c.vel + (size_t)c.trail_newest * 3 * c.cfg.tracer_capacity;
+    hipLaunchKernelGGL((k_sample<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const T *)c.pos, slot,
                        c.cfg.has_scalar ? (T *)c.phis : nullptr, (const T *)c.phys, phi,
                        c.order_valid ? c.order : nullptr, targs(c));
   }

@@ -1,5 +1,6 @@
-// Tracer kernels: the Catmull-Rom gather, the Heun push with the periodic wrap, the closing
-// sample, the canonical cell keys, and the counting sort that builds the traversal order.
+// Tracer kernels: the Catmull-Rom gather, the Adams pushes (Heun = ABM(1), ABM(k), AB(k)) with the
+// periodic wrap, the closing sample into the velocity trail, the trail's time derivative, the
+// canonical cell keys, and the counting sort that builds the traversal order.
 //
 // Positions and samples are stored [3][M] (the reference's (M, 3) column-major PointSet matrix).
 // Grid point (i, j, k) sits at (i, j, k) * L/N; the field is [c][z][y][x].  The cell of x is
@@ -86,38 +87,96 @@ struct TracerArgs {
   int64_t cap;        // tracer slots per box: box b's arrays start at b * cap (positions [3][m] inside)
   size_t box_phys;    // reals of the physical fields per box
   int ntiles;         // traversal tiles per box
+  int64_t vstride;    // reals of the velocity trail per box: [slot][3][cap]
 };
 
-// advect!(Heun) (src/ParticlePropagator.jl:288-315): X* = wrap(X + b1 v), f* = u(X*),
-// X <- wrap(X* + beta1 f* + (beta2 - b1) v).  The coefficients arrive rounded to T.
+constexpr int MAX_TRAIL = 8;   // samples an Adams step integrates over, and the longest trail
+
+// One Adams step over the trail's newest J samples f_j (newest first), with weights already
+// formed on the host from the samples' times (src/ParticlePropagator.jl:287-316):
+//   X* = wrap(X + sum_j b_j f_j);  when corrected, f* = u(X*) and
+//   X <- wrap(X* + c_0 f* + sum_j c_{j+1} f_j),  c_0 = beta_*, c_{j+1} = beta_{j+1} - b_j.
+// Each sum is added left to right in the order written (one fused displacement each).
+struct AdamsArgs {
+  int J;
+  int corrected;
+  int64_t slot[MAX_TRAIL];   // reals from a box's trail start to sample j: slot * 3 * cap
+  double b[MAX_TRAIL];
+  double c[MAX_TRAIL + 1];
+};
+
 template <typename T>
-__global__ void k_advect(T *__restrict__ pos, const T *__restrict__ vel, const T *__restrict__ u,
-                         const int *__restrict__ order, TracerArgs ta, T b1, T beta1, T c2) {
+__global__ void k_advect(T *__restrict__ pos, const T *__restrict__ trail, const T *__restrict__ u,
+                         const int *__restrict__ order, TracerArgs ta, AdamsArgs aa) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= ta.m) return;
   const int box = blockIdx.y;
   pos += box * 3 * ta.cap;
-  vel += box * 3 * ta.cap;
+  trail += box * ta.vstride;
   u += box * ta.box_phys;
   if (order) order += box * ta.cap;
   const int64_t p = order ? order[t] : t;
   const T L = T(ta.side_length), inv_dx = T(ta.n / ta.side_length);
   const size_t vol = (size_t)ta.n * ta.n * ta.n;
-  T X[3], V[3], Xs[3];
+  T F[MAX_TRAIL][3], acc[3], Xs[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    X[d] = pos[d * ta.m + p];
-    V[d] = vel[d * ta.m + p];
-    Xs[d] = wrap_box(X[d] + b1 * V[d], L);
+  for (int d = 0; d < 3; ++d) acc[d] = pos[d * ta.m + p];
+#pragma unroll
+  for (int j = 0; j < MAX_TRAIL; ++j) {
+    if (j >= aa.J) break;
+    const T bj = T(aa.b[j]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      F[j][d] = trail[aa.slot[j] + d * ta.m + p];
+      acc[d] = acc[d] + bj * F[j][d];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) Xs[d] = wrap_box(acc[d], L);
+  if (!aa.corrected) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pos[d * ta.m + p] = Xs[d];
+    return;
   }
   T wx[4], wy[4], wz[4], fs[3];
   const Stencil s = stencil_of(Xs[0], Xs[1], Xs[2], inv_dx, ta.n, wx, wy, wz);
   gather<T, 3>(u, vol, ta.n, s, wx, wy, wz, fs);
+  const T c0 = T(aa.c[0]);
 #pragma unroll
-  for (int d = 0; d < 3; ++d) pos[d * ta.m + p] = wrap_box((Xs[d] + beta1 * fs[d]) + c2 * V[d], L);
+  for (int d = 0; d < 3; ++d) acc[d] = Xs[d] + c0 * fs[d];
+#pragma unroll
+  for (int j = 0; j < MAX_TRAIL; ++j) {
+    if (j >= aa.J) break;
+    const T cj = T(aa.c[j + 1]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[d] = acc[d] + cj * F[j][d];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) pos[d * ta.m + p] = wrap_box(acc[d], L);
 }
 
-// The particle deposit! (src/ParticlePropagator.jl:378-386): u (and phi) at the positions.
+// sum_j w_j f_j over n trail samples of the selected box, added left to right (time_derivative!,
+// src/Particles.jl:597-631); out [3][m].
+struct TrailCombine {
+  int n;
+  int64_t slot[MAX_TRAIL];
+  double w[MAX_TRAIL];
+};
+
+template <typename T>
+__global__ void k_trail_combine(const T *__restrict__ trail, TracerArgs ta, TrailCombine tc, T *__restrict__ out) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= ta.m) return;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    T acc = T(tc.w[0]) * trail[tc.slot[0] + d * ta.m + t];
+    for (int j = 1; j < tc.n; ++j) acc = acc + T(tc.w[j]) * trail[tc.slot[j] + d * ta.m + t];
+    out[d * ta.m + t] = acc;
+  }
+}
+
+// The particle deposit! (src/ParticlePropagator.jl:378-386): u (and phi) at the positions, into
+// the trail slot `vel` points at (inside box 0's trail; box b's is b * vstride further).
 template <typename T>
 __global__ void k_sample(const T *__restrict__ pos, T *__restrict__ vel, T *__restrict__ phis,
                          const T *__restrict__ u, const T *__restrict__ phi, const int *__restrict__ order,
@@ -126,7 +185,7 @@ __global__ void k_sample(const T *__restrict__ pos, T *__restrict__ vel, T *__re
   if (t >= ta.m) return;
   const int box = blockIdx.y;
   pos += box * 3 * ta.cap;
-  vel += box * 3 * ta.cap;
+  vel += box * ta.vstride;
   if (phis) phis += box * ta.cap;
   u += box * ta.box_phys;
   if (phi) phi += box * ta.box_phys;

@@ -43,7 +43,8 @@ enum { HDNS_NO_FORCING = 0, HDNS_CONSTANT_POWER = 1, HDNS_LINEAR = 2, HDNS_VARIA
 /* VarianceRescaling's `over`: hold the band's kinetic energy, or the whole field's */
 enum { HDNS_OVER_BAND = 0, HDNS_OVER_TOTAL = 1 };
 enum { HDNS_CATMULL_ROM = 0 };
-enum { HDNS_HEUN = 0 };
+/* the particle schemes (src/ParticlePropagator.jl:48-253): Heun = ABM(1), ABM(k) k = 1..7, AB(k) k = 1..8 */
+enum { HDNS_HEUN = 0, HDNS_ABM = 1, HDNS_AB = 2 };
 enum { HDNS_FIELD_VELOCITY = 0, HDNS_FIELD_SCALAR = 1, HDNS_FIELD_VORTICITY = 2 };
 
 /* Everything a run fixes at construction: FixedDomain (src/SimulationDomain.jl:295-328),
@@ -66,12 +67,14 @@ typedef struct {
   double mean_gradient;      /* G of an imposed mean scalar gradient along z (extension) */
   int64_t tracer_capacity;   /* 0: no tracers */
   int32_t interpolation;     /* HDNS_CATMULL_ROM */
-  int32_t particle_scheme;   /* HDNS_HEUN */
+  int32_t particle_scheme;   /* HDNS_HEUN, HDNS_ABM or HDNS_AB, with particle_steps k */
   int32_t store_physical;    /* bit 0: keep physical u, bit 1: keep physical phi, bit 2: keep physical w */
   int32_t forcing_over;      /* VarianceRescaling: HDNS_OVER_BAND or HDNS_OVER_TOTAL */
   double forcing_target;     /* VarianceController k_inf > 0, or VarianceRescaling's target >= 0 */
   double forcing_gain;       /* VarianceController G >= 0 (0: the marginal law) */
   double forcing_time;       /* VarianceController turnover time t_l > 0 */
+  int32_t particle_steps;    /* k of ABM(k) (1..7) or AB(k) (1..8); ignored for HDNS_HEUN */
+  int32_t reserved;
 } hdns_config;
 
 typedef struct hdns_ctx hdns_ctx;
@@ -147,12 +150,24 @@ hdns_status hdns_tracers_upload(hdns_ctx *ctx, int64_t m, const double *position
 hdns_status hdns_tracers_download(hdns_ctx *ctx, double *positions);
 /* rebuild_traversal! (src/Particles.jl:268-271): counting sort of the tracers by cell tile. */
 hdns_status hdns_rebuild_traversal(hdns_ctx *ctx);
-/* advect!(Heun, ...) (src/ParticlePropagator.jl:288-315,349-372): predictor, gather at X*,
- * corrector, wrap.  Needs the deposited velocity one step ahead of the positions. */
+/* advect!(scheme, ...) (src/ParticlePropagator.jl:287-372): the Adams predictor over the trail's
+ * newest J samples with weights from their actual times, and for ABM (or while the trail holds
+ * fewer samples than the scheme integrates over) the gather at X* and the corrector; wrap.  Needs
+ * the deposited velocity one step ahead of the positions and the newest sample at the positions. */
 hdns_status hdns_advect(hdns_ctx *ctx, double dt);
-/* Particle deposit! (src/ParticlePropagator.jl:378-386): sample u (and phi) at the positions
- * into the newest trail slot. */
+/* Particle deposit! (src/ParticlePropagator.jl:378-386): sample u (and phi) at the positions.
+ * After an advect the sample is new, at the newest sample's time plus that step's dt (the first
+ * sample after an upload is at t = 0); otherwise it replaces the newest sample. */
 hdns_status hdns_deposit_tracers(hdns_ctx *ctx);
+/* The run's velocity trail (src/Particles.jl:404-526): *filled samples, their times newest first
+ * (times[capacity], capacity = k + 1 for ABM(k), k for AB(k), 2 for Heun). */
+hdns_status hdns_tracer_trail(hdns_ctx *ctx, int32_t *filled, int32_t *capacity, double *times);
+/* The velocity sample `offset` steps back from the newest (offset <= 0), [3][M]: trail[offset]. */
+hdns_status hdns_tracer_trail_sample(hdns_ctx *ctx, int32_t offset, double *velocity);
+/* time_derivative!(out, trail; order) (src/Particles.jl:597-631): d^m v/dt^m at the newest
+ * sample's time of the polynomial through every sample held, Fornberg weights from the times;
+ * order 1 is the Lagrangian acceleration.  out [3][M]. */
+hdns_status hdns_tracer_time_derivative(hdns_ctx *ctx, int32_t order, double *out);
 /* The newest velocity sample [3][M] and, when the run carries a scalar, phi at the tracers [M]. */
 hdns_status hdns_tracer_samples(hdns_ctx *ctx, double *velocity, double *scalar);
 /* Canonical cell key (k*N + j)*N + i of every tracer, computed on the device. */

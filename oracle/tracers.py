@@ -1,4 +1,5 @@
-"""Tracer restatement: Catmull-Rom gather, Heun push, wrap, Adams weights, cell keys.
+"""Tracer restatement: Catmull-Rom gather, Heun and Adams pushes, wrap, Adams and Fornberg
+weights, cell keys.
 
 Test infrastructure only (see oracle/__init__.py).  The reference delegates the gather and
 the displacement to its field library (``interpolate!``, ``displace!``); their contracts are
@@ -115,6 +116,70 @@ def heun_advect(X: np.ndarray, v_prev: np.ndarray, u_new: np.ndarray, h: float, 
     Xn = wrap(Xs + beta1 * fs + (beta2 - b1) * v_prev, side_length)
     vn = interpolate(u_new, Xn, side_length)
     return Xn, vn
+
+
+def adams_advect(X: np.ndarray, samples, times, u_new: np.ndarray, h: float, side_length: float,
+                 corrected: bool) -> np.ndarray:
+    """One AB(J) step, or with ``corrected`` one ABM(J) step, of the positions X[M, 3]
+    (src/ParticlePropagator.jl:287-316): ``samples`` are the trail's velocity samples [M, 3],
+    newest first, taken at ``times``; the Adams weights come from the samples' actual times.
+
+        X* = wrap(X + sum_j b_j f_j)
+        X' = wrap(X* + beta_* f* + sum_j (beta_{j+1} - b_j) f_j),  f* = u_new(X*)   (corrected)
+
+    Each sum is added left to right in the order written, as one fused displacement.
+    """
+    nodes = tuple(float(t) - float(times[0]) for t in times)
+    b = adams_weights(nodes, h)
+    acc = X
+    for bj, f in zip(b, samples):
+        acc = acc + bj * f
+    Xs = wrap(acc, side_length)
+    if not corrected:
+        return Xs
+    fs = interpolate(u_new, Xs, side_length)
+    beta = adams_weights((h,) + nodes, h)
+    acc = Xs + beta[0] * fs
+    for j, f in enumerate(samples):
+        acc = acc + (beta[j + 1] - b[j]) * f
+    return wrap(acc, side_length)
+
+
+def derivative_weights(nodes, order: int) -> np.ndarray:
+    """Fornberg's finite-difference weights at 0 on arbitrary ``nodes`` (Math. Comp. 51 (1988)
+    699-706): w[j, m] multiplies the value at node j in the derivative of order m, m = 0..order.
+    The reference computes the same weights for time_derivative! (src/Particles.jl:533-565)."""
+    x = [float(v) for v in nodes]
+    n = len(x)
+    c = np.zeros((n, order + 1))
+    c[0, 0] = 1.0
+    prod_prev = 1.0
+    for i in range(1, n):
+        top = min(i, order)
+        prod = 1.0
+        for j in range(i):
+            gap = x[i] - x[j]
+            prod *= gap
+            if j == i - 1:   # the new node's row, from the previous node's (before that row moves)
+                for k in range(top, 0, -1):
+                    c[i, k] = prod_prev * (k * c[i - 1, k - 1] - x[i - 1] * c[i - 1, k]) / prod
+                c[i, 0] = -prod_prev * x[i - 1] * c[i - 1, 0] / prod
+            for k in range(top, 0, -1):
+                c[j, k] = (x[i] * c[j, k] - k * c[j, k - 1]) / gap
+            c[j, 0] = x[i] * c[j, 0] / gap
+        prod_prev = prod
+    return c
+
+
+def time_derivative(samples, times, order: int = 1) -> np.ndarray:
+    """d^m f/dt^m at the newest sample's time of the polynomial through every sample
+    (src/Particles.jl:597-631); ``samples`` newest first."""
+    nodes = [float(t) - float(times[0]) for t in times]
+    w = derivative_weights(nodes, order)
+    out = np.zeros_like(np.asarray(samples[0], dtype=np.float64))
+    for j, f in enumerate(samples):
+        out = out + w[j, order] * f
+    return out
 
 
 def uniform_positions(m: int, side_length: float, seed: int) -> np.ndarray:

@@ -146,3 +146,34 @@ def test_variance_rescaling_guards(g):
         g.VarianceRescaling(dom, target=-1.0, over="band")
     f = g.VarianceRescaling(dom, target=0.1, over="total", highest_shell=math.inf)
     assert f.target == float(np.float32(0.1)) and f.highest_shell == math.inf
+
+
+def test_particle_scheme_guards_and_their_trail_capacities(g):
+    """src/ParticlePropagator.jl:106-115, :225-249: ABM(k) k = 1..7 keeps k + 1 samples and is of
+    order k + 1; AB(k) k = 1..8 keeps k and is of order k; Heun is ABM(1)."""
+    for k in (0, 8, 2.0):
+        with pytest.raises(ValueError, match="ABM"):
+            g.ABM(k)
+    for k in (0, 9):
+        with pytest.raises(ValueError, match="AB"):
+            g.AB(k)
+    assert (g.ABM(3).trail_capacity, g.ABM(3).order, g.ABM(3).corrects) == (4, 4, True)
+    assert (g.AB(4).trail_capacity, g.AB(4).order, g.AB(4).corrects) == (4, 4, False)
+    h = g.Heun()
+    assert (h.steps, h.trail_capacity, h.order, repr(h)) == (1, 2, 2, "Heun()")
+
+
+def test_startup_bound(g):
+    """src/SimulationRun.jl:1099-1129: absent for one-sample schemes; tau (dt/tau)^(p/3) with one
+    sample in the trail; then the growth factor (2 through order 5, 1.5 through 7, 1.25 at 8) times
+    the last step."""
+    from gbg_amd.api import startup_bound
+    nu, eps, dt = 0.01, 0.1, 0.02
+    tau = math.sqrt(nu / eps)
+    assert startup_bound(g.Heun(), [0.0], dt, nu, eps) == math.inf
+    assert startup_bound(g.AB(4), [0.0], dt, nu, eps) == pytest.approx(tau * (dt / tau) ** (4 / 3), rel=1e-15)
+    assert startup_bound(g.ABM(5), [0.0], dt, nu, eps) == pytest.approx(tau * (dt / tau) ** 2, rel=1e-15)
+    assert startup_bound(g.AB(4), [0.3, 0.2], dt, nu, eps) == pytest.approx(0.2, rel=1e-12)
+    assert startup_bound(g.AB(7), [0.3, 0.2], dt, nu, eps) == pytest.approx(0.15, rel=1e-12)
+    assert startup_bound(g.AB(8), [0.3, 0.2], dt, nu, eps) == pytest.approx(0.125, rel=1e-12)
+    assert startup_bound(g.AB(4), [0.0], dt, nu, 0.0) == math.inf       # a flow at rest

@@ -92,3 +92,104 @@ def test_cell_keys_in_float32_use_float32_arithmetic():
     s = X.astype(np.float32) * np.float32(n / L)
     assert np.array_equal(cell, np.mod(np.floor(s).astype(np.int64), n))
     assert trc.cell_keys(X, n, L, np.float32)[0] == (cell[0, 2] * n + cell[0, 1]) * n + cell[0, 0]
+
+
+# --- the Adams family, the trail and its time derivative ----------------------------------------
+
+def test_adams_weights_integrate_polynomials_of_degree_below_the_node_count_exactly():
+    """sum_j w_j p(t_j) = int_0^h p for deg p <= J - 1 on uneven nodes, with and without h among
+    them (the AB and AM weights, src/ParticlePropagator.jl:256-280); J up to 8."""
+    rng = np.random.default_rng(3)
+    h = 0.07
+    for J in range(1, 9):
+        past = -np.cumsum(rng.uniform(0.5, 1.5, J - 1)) * h
+        for nodes in ((0.0,) + tuple(past), (h, 0.0) + tuple(past[:-1]) if J > 1 else (h,)):
+            w = trc.adams_weights(nodes, h)
+            for deg in range(len(nodes)):
+                coef = rng.standard_normal(deg + 1)
+                p = np.polynomial.Polynomial(coef)
+                want = p.integ()(h) - p.integ()(0.0)
+                got = sum(wj * p(t) for wj, t in zip(w, nodes))
+                assert got == pytest.approx(want, rel=1e-11, abs=1e-14), (J, deg)
+
+
+def test_fornberg_weights_differentiate_polynomials_exactly_and_give_the_classical_stencils():
+    w = trc.derivative_weights([0.0, -1.0, -2.0], 2)
+    assert np.allclose(w[:, 1], [1.5, -2.0, 0.5], atol=1e-15)            # second-order backward
+    assert np.allclose(w[:, 2], [1.0, -2.0, 1.0], atol=1e-15)
+    rng = np.random.default_rng(4)
+    for n in range(2, 9):
+        nodes = np.concatenate([[0.0], -np.cumsum(rng.uniform(0.5, 1.5, n - 1)) * 0.1])
+        w = trc.derivative_weights(nodes, min(3, n - 1))
+        p = np.polynomial.Polynomial(rng.standard_normal(n))
+        for m in range(min(3, n - 1) + 1):
+            want = p.deriv(m)(0.0) if m else p(0.0)
+            assert float(w[:, m] @ p(nodes)) == pytest.approx(want, rel=1e-8, abs=1e-9), (n, m)
+
+
+def _strain_field(n, s, xc):
+    xg = np.arange(n) * L / n
+    u = np.zeros((3, n, n, n))
+    u[0] = s * (xg - xc)[None, None, :]
+    return u
+
+
+def run_trail(kind, k, X, u, dts, samples=None, times=None):
+    """The run's loop for a steady field u: advect over the trail, then sample at the new positions
+    (src/ParticlePropagator.jl:349-372, :378-386, src/Particles.jl:454-490); the trail keeps
+    k + 1 samples under ABM(k), k under AB(k)."""
+    cap = k if kind == "ab" else k + 1
+    samples = [trc.interpolate(u, X, L)] if samples is None else list(samples)
+    times = [0.0] if times is None else list(times)
+    for dt in dts:
+        full = len(samples) >= k
+        J = k if full else len(samples)
+        X = trc.adams_advect(X, samples[:J], times[:J], u, dt, L, corrected=(not full) or kind == "abm")
+        samples = [trc.interpolate(u, X, L)] + samples[: cap - 1]
+        times = [times[0] + dt] + times[: cap - 1]
+    return X, samples, times
+
+
+@pytest.mark.parametrize("kind,k,order", [("ab", 2, 2), ("ab", 3, 3), ("ab", 4, 4), ("ab", 5, 5),
+                                          ("abm", 1, 2), ("abm", 2, 3), ("abm", 3, 4), ("abm", 4, 5)])
+def test_adams_schemes_reach_their_order_from_an_exact_history(kind, k, order):
+    """x' = s (x - xc) with Catmull-Rom exact on the linear field: from a trail of exact samples at
+    the exact past positions every step is the full scheme, and the error falls as dt^order."""
+    n, s, xc, x0, T = 16, 1.0, math.pi, math.pi + 0.4, 0.5
+    u = _strain_field(n, s, xc)
+    errs = []
+    for steps in (16, 32, 64):
+        h = T / steps
+        ts = [-j * h for j in range(k)]
+        xs = [xc + (x0 - xc) * math.exp(s * t) for t in ts]
+        samples = [np.array([[s * (x - xc), 0.0, 0.0]]) for x in xs]
+        X = np.array([[x0, 1.0, 1.0]])
+        X, _, _ = run_trail(kind, k, X, u, [h] * steps, samples, ts)
+        errs.append(abs(X[0, 0] - (xc + (x0 - xc) * math.exp(s * T))))
+    r = 2.0 ** order
+    assert 0.75 * r < errs[0] / errs[1] < 1.3 * r and 0.75 * r < errs[1] / errs[2] < 1.3 * r, errs
+
+
+def test_abm1_through_the_trail_is_the_heun_push():
+    n = 16
+    rng = np.random.default_rng(9)
+    u = rng.standard_normal((3, n, n, n))
+    X = trc.uniform_positions(50, L, 3)
+    v = trc.interpolate(u, X, L)
+    want, _ = trc.heun_advect(X, v, u, 0.01, L)
+    got, _, _ = run_trail("abm", 1, X, u, [0.01])
+    assert np.array_equal(got, want)
+
+
+def test_time_derivative_of_a_trail_converges_to_the_acceleration():
+    """Along x' = s (x - xc) the acceleration is s^2 (x - xc); the derivative of the polynomial
+    through d uneven samples errs by O(dt^(d-1))."""
+    s, xc, x0 = 1.3, 0.0, 0.4
+    errs = []
+    for h in (0.04, 0.02, 0.01):
+        ts = [0.0, -h, -2.3 * h, -3.1 * h]
+        samples = [np.array([s * (x0 * math.exp(s * t) - xc)]) for t in ts]
+        a = trc.time_derivative(samples, ts, 1)
+        errs.append(abs(a[0] - s * s * x0))
+    assert 6.0 < errs[0] / errs[1] < 10.0 and 6.0 < errs[1] / errs[2] < 10.0
+    assert trc.time_derivative(samples, ts, 0)[0] == samples[0][0]
