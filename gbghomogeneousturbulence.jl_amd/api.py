@@ -772,6 +772,28 @@ class Simulation:
             dt = min(startup_bound(self.particle_integration_scheme, self.trail_times(), dt, c.viscosity, eps), dt)
         return dt
 
+    def redeposit(self) -> None:
+        """The field deposit! of the current state again (after a schedule change, say), and the
+        tracer sample at the unmoved positions with it."""
+        self._call("hdns_deposit")
+        self.fields_epoch += 1
+        if self.tracers is not None:
+            self._call("hdns_deposit_tracers")
+            self.trail_epoch = (self.fields_epoch, self.particles_epoch)
+
+    def restore_trail(self, times, samples, t_next: float) -> None:
+        """Put a stored trail back (read_simulation, ext/GBGHomogeneousTurbulenceHDF5Ext.jl:535-544):
+        the earlier samples [n, 3, M] newest first with their times, then the newest measured again
+        at the positions at t_next."""
+        samples = np.ascontiguousarray(samples, dtype=np.float64)
+        times = np.ascontiguousarray(times, dtype=np.float64)
+        if samples.shape[0] != times.shape[0]:
+            raise ValueError("one time per stored sample")
+        self._call("hdns_tracer_trail_restore", int(times.shape[0]), L.dptr(times), L.dptr(samples), float(t_next))
+        self._call("hdns_deposit_tracers")
+        self.trail_epoch = (self.fields_epoch, self.particles_epoch)
+        self.trail_time = float(t_next)
+
     def trail_times(self) -> list:
         """The times of the velocity trail's samples at the tracers, newest first (src/Particles.jl:404-526)."""
         filled, cap = c_int32(), c_int32()

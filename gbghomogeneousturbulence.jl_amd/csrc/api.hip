@@ -1069,8 +1069,9 @@ hdns_status hdns_tracers_upload(hdns_ctx *ctx, int64_t m, const double *position
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
   c.m = m;
   c.order_valid = false;
-  c.trail_filled = 0;   // new positions: the trail starts again
+  c.trail_filled = 0;   // new positions: the trail starts again, at t = 0
   c.advected = false;
+  c.pending_t = 0.0;
   if (m == 0) return HDNS_OK;
   const size_t realb = c.fp64 ? 8 : 4;
   char *pos = (char *)c.pos + (size_t)c.box * 3 * c.cfg.tracer_capacity * realb;
@@ -1159,7 +1160,7 @@ hdns_status hdns_advect(hdns_ctx *ctx, double dt) {
   }
   c.pipe->advect(c, aa);
   c.advected = true;
-  c.pending_dt = dt;
+  c.pending_t = c.trail_t[c.trail_newest] + dt;
   return launched(ctx);
 }
 
@@ -1171,14 +1172,13 @@ hdns_status hdns_deposit_tracers(hdns_ctx *ctx) {
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
   // the ring (src/Particles.jl:454-490): a pushed set of positions takes the next slot, retiring the
   // oldest sample of a full trail; unmoved positions refresh the newest sample in place
-  if (c.trail_filled == 0) {
+  if (c.trail_filled == 0) {   // the first sample: t = 0 after an upload, the stored clock after a restore
     c.trail_newest = 0;
-    c.trail_t[0] = 0.0;
+    c.trail_t[0] = c.pending_t;
     c.trail_filled = 1;
   } else if (c.advected) {
-    const double t = c.trail_t[c.trail_newest] + c.pending_dt;
     c.trail_newest = (c.trail_newest + 1) % c.trail_cap;
-    c.trail_t[c.trail_newest] = t;
+    c.trail_t[c.trail_newest] = c.pending_t;
     c.trail_filled = std::min(c.trail_filled + 1, c.trail_cap);
   }
   c.advected = false;
@@ -1193,6 +1193,45 @@ hdns_status hdns_tracer_trail(hdns_ctx *ctx, int32_t *filled, int32_t *capacity,
   *capacity = c.trail_cap;
   if (times)
     for (int j = 0; j < c.trail_filled; ++j) times[j] = c.trail_t[(c.trail_newest - j + c.trail_cap) % c.trail_cap];
+  return HDNS_OK;
+}
+
+hdns_status hdns_tracer_trail_restore(hdns_ctx *ctx, int32_t n, const double *times, const double *samples,
+                                      double t_next) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  Ctx &c = ctx->c;
+  if (c.cfg.tracer_capacity == 0) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "this context carries no tracers");
+  if (n < 0 || n > c.trail_cap - 1)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT,
+                "a restored trail holds at most capacity - 1 = " + std::to_string(c.trail_cap - 1) +
+                    " earlier samples (the next deposit measures the newest)");
+  if (n > 0 && (!times || !samples)) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "times and samples must not be NULL");
+  if (!isfinite(t_next)) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "a sample's time must be finite");
+  for (int j = 0; j < n; ++j)   // newest first, strictly decreasing, all before t_next
+    if (!isfinite(times[j]) || !(times[j] < (j == 0 ? t_next : times[j - 1])))
+      return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "the restored sample times must be finite and decrease from t_next");
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  const size_t realb = c.fp64 ? 8 : 4, cap = (size_t)c.cfg.tracer_capacity, count = 3 * (size_t)c.m;
+  // oldest in slot 0: the newest restored sample in slot n - 1, and the next deposit takes slot n
+  std::vector<float> tmp(c.fp64 ? 0 : count);
+  for (int j = 0; j < n; ++j) {
+    const int slot = n - 1 - j;
+    char *dst = (char *)c.vel + ((size_t)c.box * c.trail_cap + slot) * 3 * cap * realb;
+    const double *src = samples + (size_t)j * count;
+    if (c.fp64) {
+      HIP_TRY(ctx, hipMemcpyAsync(dst, src, count * 8, hipMemcpyHostToDevice, c.stream));
+    } else {
+      for (size_t i = 0; i < count; ++i) tmp[i] = (float)src[i];
+      HIP_TRY(ctx, hipMemcpyAsync(dst, tmp.data(), count * 4, hipMemcpyHostToDevice, c.stream));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+    c.trail_t[slot] = times[j];
+  }
+  c.trail_filled = n;
+  c.trail_newest = n > 0 ? n - 1 : c.trail_cap - 1;
+  c.advected = n > 0;
+  c.pending_t = t_next;
+  if (n == 0) c.trail_filled = 0;   // the next deposit starts the trail, at t_next
   return HDNS_OK;
 }
 

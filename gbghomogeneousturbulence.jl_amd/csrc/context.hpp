@@ -110,7 +110,7 @@ struct Ctx {
   int trail_filled = 0;        // samples held
   double trail_t[MAX_TRAIL] = {0};
   bool advected = false;       // the positions moved since the newest sample
-  double pending_dt = 0.0;     // that push's step: the next sample's time is the newest's plus it
+  double pending_t = 0.0;      // the time the next sample carries (the newest's plus that push's step)
   bool deposited = false;
   size_t bytes = 0;
   // optional per-kernel timing with events on the context's stream

@@ -162,6 +162,11 @@ hdns_status hdns_deposit_tracers(hdns_ctx *ctx);
 /* The run's velocity trail (src/Particles.jl:404-526): *filled samples, their times newest first
  * (times[capacity], capacity = k + 1 for ABM(k), k for AB(k), 2 for Heun). */
 hdns_status hdns_tracer_trail(hdns_ctx *ctx, int32_t *filled, int32_t *capacity, double *times);
+/* Restore a stored trail (read_simulation, ext/GBGHomogeneousTurbulenceHDF5Ext.jl:535-544): the n
+ * earlier samples [n][3][M] newest first with their times, n <= capacity - 1; the next
+ * hdns_deposit_tracers measures the newest sample again at the positions, at time t_next. */
+hdns_status hdns_tracer_trail_restore(hdns_ctx *ctx, int32_t n, const double *times, const double *samples,
+                                      double t_next);
 /* The velocity sample `offset` steps back from the newest (offset <= 0), [3][M]: trail[offset]. */
 hdns_status hdns_tracer_trail_sample(hdns_ctx *ctx, int32_t offset, double *velocity);
 /* time_derivative!(out, trail; order) (src/Particles.jl:597-631): d^m v/dt^m at the newest

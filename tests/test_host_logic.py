@@ -177,3 +177,30 @@ def test_startup_bound(g):
     assert startup_bound(g.AB(7), [0.3, 0.2], dt, nu, eps) == pytest.approx(0.15, rel=1e-12)
     assert startup_bound(g.AB(8), [0.3, 0.2], dt, nu, eps) == pytest.approx(0.125, rel=1e-12)
     assert startup_bound(g.AB(4), [0.0], dt, nu, 0.0) == math.inf       # a flow at rest
+
+
+def test_checkpoint_store_round_trips_groups_attributes_and_datasets(g, tmp_path):
+    """The container of the checkpoint schema: nested groups, JSON attributes (inf included),
+    complex and real datasets; loading executes nothing from the file (allow_pickle=False)."""
+    from gbg_amd import checkpoint as ck
+    s = ck.Store()
+    top = ck._tagged(s.root, "simulation", "Simulation")
+    top.attrs.update(time=0.25, step=3, schedule=[1, 0, 0, 0, 1])
+    f = ck._tagged(top, "velocity_forcing", "Linear")
+    f.attrs.update(rate=0.3, lowest_shell=1, highest_shell=math.inf)
+    z = (np.arange(12) + 1j * np.arange(12)).reshape(3, 4)
+    st = ck._tagged(top, "state", "ResolvedFields")
+    st["velocity_spectral"] = z
+    s.save(tmp_path / "c.npz")
+    r = ck.Store.load(tmp_path / "c.npz")
+    top2 = r.root.group("simulation", "t")
+    assert ck._require_tag(top2, "t", "Simulation")["schedule"] == [1, 0, 0, 0, 1]
+    assert top2.group("velocity_forcing", "t").attrs["highest_shell"] == math.inf
+    assert np.array_equal(top2.group("state", "t")["velocity_spectral"], z)
+    with pytest.raises(ValueError, match="holds a"):
+        ck._require_tag(top2.group("state", "t"), "t", "Simulation")
+    with pytest.raises(ValueError, match="no object"):
+        r.root.group("missing", "read_simulation")
+    with pytest.raises(ValueError, match="newer"):
+        top2.attrs["version"] = 99
+        ck._require_tag(top2, "t", "Simulation")
