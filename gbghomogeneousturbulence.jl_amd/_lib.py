@@ -37,6 +37,9 @@ class HdnsConfig(ctypes.Structure):
         ("store_physical", c_int32), ("forcing_over", c_int32),
         ("forcing_target", c_double), ("forcing_gain", c_double), ("forcing_time", c_double),
         ("particle_steps", c_int32), ("reserved", c_int32),
+        ("scalar_forcing", c_int32), ("scalar_lowest_shell", c_int32), ("scalar_highest_shell", c_int32),
+        ("scalar_over", c_int32), ("scalar_rate", c_double), ("scalar_target", c_double),
+        ("scalar_gain", c_double), ("scalar_time", c_double),
     ]
 
 

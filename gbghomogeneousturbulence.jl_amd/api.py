@@ -237,8 +237,19 @@ class FlowConditions:
         self.schmidt = float(schmidt)
         self.velocity_forcing = velocity_forcing if velocity_forcing is not None else NoForcing()
         self.scalar_forcing = scalar_forcing if scalar_forcing is not None else NoForcing()
-        if not isinstance(self.scalar_forcing, NoForcing):
-            raise NotImplementedError("scalar forcings other than NoForcing are not built")
+        if not isinstance(self.scalar_forcing,
+                          (NoForcing, ConstantPower, Linear, VarianceController, VarianceRescaling)):
+            raise NotImplementedError("the scalar forcings built are NoForcing, ConstantPower, Linear, "
+                                      "VarianceController and VarianceRescaling")
+        if velocity_forcing is not None and velocity_forcing is scalar_forcing and not isinstance(velocity_forcing, NoForcing):
+            raise ValueError("one forcing instance is bound to both `velocity_forcing` and `scalar_forcing`; "
+                             "build two (src/Generator.jl:111-115)")
+        sf = self.scalar_forcing
+        if isinstance(sf, VarianceController) and sf.turnover_time_derived and sf.gain > 0:
+            # src/Generator.jl:120-127
+            raise ValueError("this `VarianceController` is bound as `scalar_forcing` but its turnover_time was "
+                             "derived from the kinetic-energy model, which is meaningless for a scalar variance; "
+                             "pass the flow's turnover time 2k/(3 eps) explicitly")
         if not isinstance(self.velocity_forcing,
                           (NoForcing, ConstantPower, Linear, VarianceController, VarianceRescaling)):
             raise NotImplementedError("the velocity forcings built are NoForcing, ConstantPower, Linear, "
@@ -474,6 +485,22 @@ def _config(domain: FixedDomain, conditions: FlowConditions, scheme, carries_sca
         cfg.velocity_forcing = L.VARIANCE_RESCALING
         cfg.forcing_target = f.target
         cfg.forcing_over = L.OVER_TOTAL if f.over == "total" else L.OVER_BAND
+    sf = conditions.scalar_forcing
+    if not carries_scalar and not isinstance(sf, NoForcing):
+        raise ValueError(f"a scalar forcing ({sf!r}) needs a state that carries the scalar")
+    if carries_scalar and not isinstance(sf, NoForcing):
+        cfg.scalar_lowest_shell = sf.lowest_shell
+        cfg.scalar_highest_shell = -1 if sf.highest_shell == math.inf else sf.highest_shell
+        if isinstance(sf, ConstantPower):
+            cfg.scalar_forcing, cfg.scalar_rate = L.CONSTANT_POWER, sf.injection_rate
+        elif isinstance(sf, Linear):
+            cfg.scalar_forcing, cfg.scalar_rate = L.LINEAR, sf.rate
+        elif isinstance(sf, VarianceController):
+            cfg.scalar_forcing = L.VARIANCE_CONTROLLER
+            cfg.scalar_target, cfg.scalar_gain, cfg.scalar_time = sf.target, sf.gain, sf.turnover_time
+        else:
+            cfg.scalar_forcing, cfg.scalar_target = L.VARIANCE_RESCALING, sf.target
+            cfg.scalar_over = L.OVER_TOTAL if sf.over == "total" else L.OVER_BAND
     cfg.mean_gradient = float(mean_gradient)
     ps = particle_scheme if particle_scheme is not None else Heun()
     cfg.particle_scheme = ps.code

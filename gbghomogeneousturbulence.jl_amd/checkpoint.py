@@ -239,7 +239,7 @@ def read_simulation(path, name: str = "simulation", **simulation_kwargs):
     a = _require_tag(g, "read_simulation", "Simulation")
     domain = read_domain(g, "domain")
     vf = read_forcing(g, "velocity_forcing", domain)
-    read_forcing(g, "scalar_forcing", domain)   # NoForcing is the only scalar forcing built
+    sf = read_forcing(g, "scalar_forcing", domain)
     scheme = _FIELD_SCHEMES.get(a["field_integration_scheme"])
     if scheme is None:
         raise ValueError(f"read_simulation: unrecognized field scheme tag {a['field_integration_scheme']!r}")
@@ -262,7 +262,7 @@ def read_simulation(path, name: str = "simulation", **simulation_kwargs):
         shortened = bool(a["shortened_first_steps"])
         trail = g.group("velocity_at_particles_trail", "read_simulation")
         _require_tag(trail, "read_simulation", "ParticleTrail")
-    cond = api.FlowConditions(viscosity=a["viscosity"], schmidt=a["schmidt"], velocity_forcing=vf)
+    cond = api.FlowConditions(viscosity=a["viscosity"], schmidt=a["schmidt"], velocity_forcing=vf, scalar_forcing=sf)
     sim = api.Simulation(domain, conditions=cond, state=state, field_integration_scheme=scheme, tracers=tracers,
                          particle_integration_scheme=ps, mean_gradient=float(a.get("mean_gradient", 0.0)),
                          cfl=float(a["cfl"]), max_dt=float(a["max_dt"]), shortened_first_steps=shortened,

@@ -261,24 +261,65 @@ FinalizeArgs finalize_args(const Ctx &c) {
   return fa;
 }
 
-// VarianceRescaling's finalize! on the advanced state of every box (src/Forcing.jl:575-596).
-void launch_rescale(Ctx &c) {
-  const RescaleArgs ra{c.cfg.forcing_target, c.cfg.forcing_over == HDNS_OVER_TOTAL, 1.0 / pow((double)c.geo.n, 6.0),
-                       eps_of(c)};
-  const int r = c.cfg.highest_shell < 0 ? c.geo.m : std::min(c.cfg.highest_shell, c.geo.m);
+BandArgs band_args(const Ctx &c, int mode, double factor, int slot) {
+  BandArgs ba{};
+  ba.mode = mode;
+  ba.factor = factor;
+  ba.slot = slot;
+  ba.inv_norm = 1.0 / pow((double)c.geo.n, 6.0);
+  ba.eps_t = eps_of(c);
+  return ba;
+}
+
+// The band sums of one field (u: 3 components at c.uh, or phi at c.ph) and the per-box factor.
+void launch_band_factor(Ctx &c, bool scalar, const BandSpec &band, const BandArgs &ba) {
+  const dim3 sums(RESCALE_BLOCKS, 1, c.nbox);
+  const int nc = scalar ? 1 : 3;
+  if (c.fp64)
+    hipLaunchKernelGGL(k_band_sums<double>, sums, dim3(256), 0, c.stream,
+                       (const cx<double> *)(scalar ? c.ph : c.uh), nc, c.geo, band, c.rpart);
+  else
+    hipLaunchKernelGGL(k_band_sums<float>, sums, dim3(256), 0, c.stream, (const cx<float> *)(scalar ? c.ph : c.uh),
+                       nc, c.geo, band, c.rpart);
+  hipLaunchKernelGGL(k_band_factor, dim3(c.nbox), dim3(256), 0, c.stream, (const double *)c.rpart, c.scalars, ba);
+}
+
+// VarianceRescaling's finalize! on the advanced state of every box (src/Forcing.jl:575-596): the
+// velocity's factor in scalars[6], the scalar's in scalars[7].
+void launch_rescale(Ctx &c, bool scalar) {
+  const hdns_config &k = c.cfg;
+  BandArgs ba = band_args(c, 0, scalar ? 1.0 : 0.5, scalar ? 7 : 6);
+  ba.target = scalar ? k.scalar_target : k.forcing_target;
+  ba.over_total = (scalar ? k.scalar_over : k.forcing_over) == HDNS_OVER_TOTAL;
+  const BandSpec &band = scalar ? c.sband : c.band;
+  launch_band_factor(c, scalar, band, ba);
+  const int hi = scalar ? k.scalar_highest_shell : k.highest_shell;
+  const int r = hi < 0 ? c.geo.m : std::min(hi, c.geo.m);
   const size_t cube = (size_t)(2 * r + 1) * (2 * r + 1) * (r + 1);
-  const dim3 sums(RESCALE_BLOCKS, 1, c.nbox), apply((unsigned)((cube + 255) / 256), 1, c.nbox);
-  if (c.fp64) {
-    hipLaunchKernelGGL(k_band_sums<double>, sums, dim3(256), 0, c.stream, (const cx<double> *)c.uh, c.geo, c.band, c.rpart);
-    hipLaunchKernelGGL(k_rescale_factor, dim3(c.nbox), dim3(256), 0, c.stream, (const double *)c.rpart, c.scalars, ra);
-    hipLaunchKernelGGL(k_rescale_apply<double>, apply, dim3(256), 0, c.stream, (cx<double> *)c.uh, c.geo, c.band, r,
-                       (const double *)c.scalars);
-  } else {
-    hipLaunchKernelGGL(k_band_sums<float>, sums, dim3(256), 0, c.stream, (const cx<float> *)c.uh, c.geo, c.band, c.rpart);
-    hipLaunchKernelGGL(k_rescale_factor, dim3(c.nbox), dim3(256), 0, c.stream, (const double *)c.rpart, c.scalars, ra);
-    hipLaunchKernelGGL(k_rescale_apply<float>, apply, dim3(256), 0, c.stream, (cx<float> *)c.uh, c.geo, c.band, r,
-                       (const double *)c.scalars);
-  }
+  const dim3 apply((unsigned)((cube + 255) / 256), 1, c.nbox);
+  const int nc = scalar ? 1 : 3;
+  if (c.fp64)
+    hipLaunchKernelGGL(k_rescale_apply<double>, apply, dim3(256), 0, c.stream, (cx<double> *)(scalar ? c.ph : c.uh),
+                       nc, c.geo, band, r, (const double *)c.scalars, ba.slot);
+  else
+    hipLaunchKernelGGL(k_rescale_apply<float>, apply, dim3(256), 0, c.stream, (cx<float> *)(scalar ? c.ph : c.uh), nc,
+                       c.geo, band, r, (const double *)c.scalars, ba.slot);
+}
+
+// A ConstantPower or VarianceController scalar forcing's amplitude at the state the deposit just
+// reduced, into scalars[7] (src/Forcing.jl:1041-1117, variance factor 1, diffusivity nu/Sc); the
+// controller's dissipation reads the deposit's sum w |k|^2 |phi|^2 in scalars[5].
+void launch_scalar_amplitude(Ctx &c) {
+  const hdns_config &k = c.cfg;
+  if (!c.sband.on || (k.scalar_forcing != HDNS_CONSTANT_POWER && k.scalar_forcing != HDNS_VARIANCE_CONTROLLER)) return;
+  BandArgs ba = band_args(c, k.scalar_forcing == HDNS_CONSTANT_POWER ? 1 : 2, 1.0, 7);
+  ba.rate = k.scalar_rate;
+  ba.target = k.scalar_target;
+  ba.gain = k.scalar_gain;
+  ba.turnover_time = k.scalar_time;
+  ba.diffusivity = c.kappa_d;
+  ba.dirichlet_slot = 5;
+  launch_band_factor(c, true, c.sband, ba);
 }
 
 // The inverse half of the pipeline for the current state: the field deposit!
@@ -289,6 +330,7 @@ hdns_status run_deposit(hdns_ctx *ctx, void *phys_to = nullptr, void *physw_to =
   c.pipe->zi(c);
   const FinalizeArgs fa = finalize_args(c);
   hipLaunchKernelGGL(k_finalize, dim3(c.nbox), dim3(256), 0, c.stream, (const double *)c.partials, c.scalars, fa);
+  launch_scalar_amplitude(c);
   c.pipe->yinv(c);
   hipLaunchKernelGGL(k_fill_u64, dim3((c.nbox + 63) / 64), dim3(64), 0, c.stream, c.maxbits, 0ull, c.nbox);
   XArgs xa{};
@@ -378,6 +420,34 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
     if (!(isfinite(k.forcing_time) && k.forcing_time > 0))
       return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "`turnover_time` must be finite and positive");
   }
+  if (k.scalar_forcing < HDNS_NO_FORCING || k.scalar_forcing > HDNS_VARIANCE_RESCALING)
+    return fail(nullptr, HDNS_ERR_UNSUPPORTED,
+                "scalar_forcing must be NoForcing, ConstantPower, Linear, VarianceController or VarianceRescaling");
+  if (k.scalar_forcing != HDNS_NO_FORCING) {
+    if (!k.has_scalar)
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "a scalar forcing needs a run that carries the scalar");
+    if (k.scalar_lowest_shell < 1)
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                  "`lowest_shell` must be at least 1: shell 0 is the mean (kappa = 0) mode, which a band forcing "
+                  "cannot legitimately drive");
+    if (k.scalar_highest_shell != -1 && k.scalar_highest_shell < k.scalar_lowest_shell)
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                  "`highest_shell` must be at least `lowest_shell`: the band would be empty and the forcing would "
+                  "silently do nothing");
+    if ((k.scalar_forcing == HDNS_CONSTANT_POWER || k.scalar_forcing == HDNS_LINEAR) && !isfinite(k.scalar_rate))
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "the scalar forcing's rate must be finite");
+    if (k.scalar_forcing == HDNS_VARIANCE_CONTROLLER &&
+        !(isfinite(k.scalar_target) && k.scalar_target > 0 && isfinite(k.scalar_gain) && k.scalar_gain >= 0 &&
+          isfinite(k.scalar_time) && k.scalar_time > 0))
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                  "the scalar VarianceController needs a finite positive target, a finite non-negative gain and a "
+                  "finite positive turnover_time");
+    if (k.scalar_forcing == HDNS_VARIANCE_RESCALING &&
+        (!(isfinite(k.scalar_target) && k.scalar_target >= 0) ||
+         (k.scalar_over != HDNS_OVER_BAND && k.scalar_over != HDNS_OVER_TOTAL)))
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                  "the scalar VarianceRescaling needs a finite non-negative target and `over` band or total");
+  }
   if (k.velocity_forcing == HDNS_VARIANCE_RESCALING) {    // src/Forcing.jl:555-573
     if (k.forcing_over != HDNS_OVER_BAND && k.forcing_over != HDNS_OVER_TOTAL)
       return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "VarianceRescaling `over` must be band or total");
@@ -443,6 +513,9 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
   c.band.on = k.velocity_forcing != HDNS_NO_FORCING;
   c.band.lo4 = (2 * k.lowest_shell - 1) * (2 * k.lowest_shell - 1);
   c.band.hi4 = k.highest_shell < 0 ? -1 : (2 * k.highest_shell + 1) * (2 * k.highest_shell + 1);
+  c.sband.on = k.has_scalar && k.scalar_forcing != HDNS_NO_FORCING;
+  c.sband.lo4 = (2 * k.scalar_lowest_shell - 1) * (2 * k.scalar_lowest_shell - 1);
+  c.sband.hi4 = k.scalar_highest_shell < 0 ? -1 : (2 * k.scalar_highest_shell + 1) * (2 * k.scalar_highest_shell + 1);
   const size_t ncf = k.has_scalar ? 6 : 3, nci = k.has_scalar ? 7 : 6;
   const size_t nn = (size_t)n * n, nk = (size_t)n * c.geo.k;
   c.buf_a = std::max(ncf * nn, nci * nk) * c.geo.kxp;
@@ -474,8 +547,12 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.tw, n * sizeof(cx<double>));
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.partials, B * nzb * NRED * sizeof(double));
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.scalars, B * 8 * sizeof(double));
-  if (s == HDNS_OK && k.velocity_forcing == HDNS_VARIANCE_RESCALING)
+  if (s == HDNS_OK && (k.velocity_forcing == HDNS_VARIANCE_RESCALING || c.sband.on))
     s = alloc(ctx, (void **)&c.rpart, B * RESCALE_BLOCKS * 2 * sizeof(double));
+  if (s == HDNS_OK) {   // the amplitudes and factors start at zero (a Linear scalar forcing reads A_phi = 0)
+    const hipError_t z = hipMemset(c.scalars, 0, B * 8 * sizeof(double));
+    if (z != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, std::string("clearing the scalars: ") + hipGetErrorString(z));
+  }
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.maxbits, B * sizeof(unsigned long long));
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.spec_bins, c.nbins * sizeof(double));
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.visc_d, B * sizeof(double));
@@ -774,6 +851,8 @@ void launch_step(Ctx &c, double dt, const StageRecord *rec = nullptr) {
     sa.band = c.band;
     sa.scalars = c.scalars;
     sa.lin_rate = c.cfg.velocity_forcing == HDNS_LINEAR ? c.cfg.injection_rate : 0.0;
+    sa.sband = c.sband;
+    sa.slin_rate = c.cfg.scalar_forcing == HDNS_LINEAR ? c.cfg.scalar_rate : 0.0;
     if (c.cfg.scheme == HDNS_IF_SSP_RK3 || c.cfg.scheme == HDNS_IF_HEUN) shu_osher_stage(sa, c.cfg.scheme, i, dt, fp32);
     XArgs xa = last ? fin : mid;
     if (rec) {
@@ -792,11 +871,13 @@ void launch_step(Ctx &c, double dt, const StageRecord *rec = nullptr) {
     TIMED(c, K_YFWD, c.pipe->yfwd(c));
     TIMED(c, K_ZF, c.pipe->zf(c, sa));
     if (last) TIMED(c, K_REALITY, c.pipe->reality(c));
-    if (last && c.cfg.velocity_forcing == HDNS_VARIANCE_RESCALING) launch_rescale(c);
+    if (last && c.cfg.velocity_forcing == HDNS_VARIANCE_RESCALING) launch_rescale(c, false);
+    if (last && c.sband.on && c.cfg.scalar_forcing == HDNS_VARIANCE_RESCALING) launch_rescale(c, true);
     TIMED(c, K_ZI, c.pipe->zi(c));
     TIMED(c, K_FINALIZE,
           hipLaunchKernelGGL(k_finalize, dim3(c.nbox), dim3(256), 0, c.stream, (const double *)c.partials, c.scalars,
                              fa));
+    launch_scalar_amplitude(c);
     TIMED(c, K_YINV, c.pipe->yinv(c));
     if (last) hipLaunchKernelGGL(k_fill_u64, dim3((c.nbox + 63) / 64), dim3(64), 0, c.stream, c.maxbits, 0ull, c.nbox);
     TIMED(c, K_XPASS, c.pipe->xpass(c, xa));
@@ -1491,6 +1572,8 @@ hdns_status adj_alloc(hdns_ctx *ctx) {
   if (c.nbox != 1) return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint runs on a single-box context");
   if (c.cfg.velocity_forcing != HDNS_NO_FORCING && c.cfg.velocity_forcing != HDNS_CONSTANT_POWER)
     return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint supports NoForcing and ConstantPower velocity forcing");
+  if (c.cfg.scalar_forcing != HDNS_NO_FORCING)
+    return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint supports no scalar forcing (the mean gradient drives the scalar)");
   if (c.cfg.scheme == HDNS_IF_SSP_RK3 || c.cfg.scheme == HDNS_IF_HEUN)
     return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint supports the low-storage schemes IFLowStorageRK3 and IFLowStorageRK4");
   const size_t cb = c.comp * c.elem, realb = c.fp64 ? 8 : 4;

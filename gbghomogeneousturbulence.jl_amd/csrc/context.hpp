@@ -86,6 +86,7 @@ struct Ctx {
   static constexpr int NFAM = 5;   // y threads, z forward, z inverse, x pass, y radix plan
   int sched[NFAM] = {1, 0, 0, 0, 0};
   BandSpec band{};
+  BandSpec sband{};   // the scalar forcing's band
   hipStream_t stream = nullptr;
   void *uh = nullptr, *hu = nullptr, *ph = nullptr, *hp = nullptr;
   void *bufa = nullptr, *bufb = nullptr;

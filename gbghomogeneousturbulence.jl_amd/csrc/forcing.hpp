@@ -1,10 +1,14 @@
-// The post-step forcing projection VarianceRescaling (src/Forcing.jl:575-596): after the last
-// stage, the band's modes are rescaled by one factor per box that holds the band's kinetic energy
-// (over band) or the whole field's (over total) at the target.  Three small launches on the state
-// after the step's projections, before the closing deposit reads it:
-//   k_band_sums      per-block sums of w |u|^2 over the whole state and over the band (FP64);
-//   k_rescale_factor one block per box: the compensated totals, the empty-band floor and the factor;
-//   k_rescale_apply  u <- u + (s - 1) u on the band's modes, inside the band's bounding cube.
+// Band factors measured on the state of one field, on the device (no host readback):
+//   * VarianceRescaling's post-step rescale (src/Forcing.jl:575-596): after the last stage the band's
+//     modes are rescaled by one factor per box that holds the band's value (over band) or the whole
+//     field's (over total) at the target;
+//   * the amplitude of a ConstantPower or VarianceController scalar forcing at a stage's scalar
+//     (src/Forcing.jl:1041-1117), where the velocity's comes from the deposit's reductions.
+// The variance factor is 1/2 for the velocity (kinetic energy) and 1 for the scalar (src/Forcing.jl:456).
+// Three small launches:
+//   k_band_sums      per-block sums of w |f|^2 over the whole field and over the band (FP64);
+//   k_band_factor    one block per box: the compensated totals, the empty-band floor and the factor;
+//   k_rescale_apply  f <- f + (s - 1) f on the band's modes, inside the band's bounding cube.
 // The rescale multiplies whole modes by one real number, so it commutes with the Leray and the
 // reality projections that the reference applies after it.
 #pragma once
@@ -16,21 +20,29 @@ namespace hdns {
 
 constexpr int RESCALE_BLOCKS = 256;
 
-struct RescaleArgs {
-  double target;     // the held value: the band's or the whole field's kinetic energy
-  int over_total;    // 1: hold the whole field's value
-  double inv_norm;   // 1/N^6
-  double eps_t;      // eps of the run's element type (the empty-band floor)
+struct BandArgs {
+  int mode;           // 0: rescale factor s - 1; 1: ConstantPower amplitude; 2: VarianceController amplitude
+  double factor;      // 1/2 (velocity) or 1 (scalar)
+  int slot;           // scalars[8 box + slot] receives the result
+  double target;      // the held value (rescale, controller)
+  int over_total;     // rescale: hold the whole field's value
+  double rate;        // ConstantPower: the injection rate
+  double gain, turnover_time;   // controller
+  const double *diffusivity;    // controller: per box, the field's diffusivity
+  int dirichlet_slot;           // controller: scalars[8 box + this] = sum w |k|^2 |f|^2 / N^6
+  double inv_norm;    // 1/N^6
+  double eps_t;       // eps of the run's element type (the empty-band floor)
 };
 
-// partials [box][RESCALE_BLOCKS][2]: sum w |u|^2 over every retained mode, and over the band
+// partials [box][RESCALE_BLOCKS][2]: sum w |f|^2 over every retained mode, and over the band, of
+// the ncomp compact components of f (box b's start at b * ncomp * comp)
 template <typename T>
-__global__ __launch_bounds__(256) void k_band_sums(const cx<T> *__restrict__ uh, Geo g, BandSpec band,
+__global__ __launch_bounds__(256) void k_band_sums(const cx<T> *__restrict__ f, int ncomp, Geo g, BandSpec band,
                                                    double *__restrict__ partials) {
   __shared__ double red[4][2];
   const int box = blockIdx.z;
   const size_t comp = comp_of(g);
-  uh += (size_t)box * 3 * comp;
+  f += (size_t)box * ncomp * comp;
   const size_t modes = (size_t)g.k * g.k * (g.m + 1);
   double whole = 0.0, inband = 0.0;
   for (size_t l = blockIdx.x * (size_t)blockDim.x + threadIdx.x; l < modes; l += (size_t)gridDim.x * blockDim.x) {
@@ -39,9 +51,8 @@ __global__ __launch_bounds__(256) void k_band_sums(const cx<T> *__restrict__ uh,
     const int ky_idx = (int)(row % g.k), kz_idx = (int)(row / g.k);
     const size_t at = row * g.kxp + kx;
     double e = 0.0;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const cx<T> v = uh[c * comp + at];
+    for (int c = 0; c < ncomp; ++c) {
+      const cx<T> v = f[c * comp + at];
       e += (double)v.re * (double)v.re + (double)v.im * (double)v.im;
     }
     const double w = kx == 0 ? 1.0 : 2.0;
@@ -65,11 +76,13 @@ __global__ __launch_bounds__(256) void k_band_sums(const cx<T> *__restrict__ uh,
   }
 }
 
-// One block per box: scalars[8 box + 6] = s - 1, with s = sqrt(target_band / k_band), target_band
-// the target (over band) or max(0, target - k + k_band) (over total); 0 for an empty band, which
-// is left alone (src/Forcing.jl:110-111, :575-596).
-__global__ __launch_bounds__(256) void k_rescale_factor(const double *__restrict__ partials, double *__restrict__ scalars,
-                                                        RescaleArgs ra) {
+// One block per box.  With W = factor * whole and B = factor * band (an empty band, the relative
+// floor of src/Forcing.jl:110-111, gives 0):
+//   mode 0: s - 1, s = sqrt(target_band / B), target_band = target or max(0, target - W + B);
+//   mode 1: A = rate / (2 B);
+//   mode 2: A = (2 factor D <f, |k|^2 f> - G (W - target) / t_l) / (2 B).
+__global__ __launch_bounds__(256) void k_band_factor(const double *__restrict__ partials, double *__restrict__ scalars,
+                                                     BandArgs ba) {
   __shared__ double red[256][2], cor[256][2];
   const int box = blockIdx.x, tid = threadIdx.x;
   partials += (size_t)box * RESCALE_BLOCKS * 2;
@@ -97,24 +110,32 @@ __global__ __launch_bounds__(256) void k_rescale_factor(const double *__restrict
         S = u;
         C += cor[t][i];
       }
-      tot[i] = (S + C) * ra.inv_norm;
+      tot[i] = (S + C) * ba.inv_norm;
     }
     const double whole_raw = tot[0], band_raw = tot[1];
-    double f = 0.0;
-    if (isfinite(band_raw) && band_raw > 10.0 * ra.eps_t * whole_raw) {
-      const double band = 0.5 * band_raw, whole = 0.5 * whole_raw;
-      const double target_band = ra.over_total ? fmax(0.0, ra.target - whole + band) : ra.target;
-      f = sqrt(target_band / band) - 1.0;
+    const double W = ba.factor * whole_raw, B = ba.factor * band_raw;
+    double out = 0.0;
+    if (isfinite(band_raw) && band_raw > 10.0 * ba.eps_t * whole_raw) {
+      if (ba.mode == 0) {
+        const double target_band = ba.over_total ? fmax(0.0, ba.target - W + B) : ba.target;
+        out = sqrt(target_band / B) - 1.0;
+      } else if (ba.mode == 1) {
+        out = ba.rate / (2.0 * B);
+      } else {
+        const double dissipation = 2.0 * ba.factor * ba.diffusivity[box] * scalars[8 * box + ba.dirichlet_slot];
+        out = (dissipation - ba.gain * (W - ba.target) / ba.turnover_time) / (2.0 * B);
+      }
     }
-    scalars[8 * box + 6] = f;
+    scalars[8 * box + ba.slot] = out;
   }
 }
 
-// u <- u + (s - 1) u on the band's modes; the grid covers the cube |m| <= r around the origin
-// (r = the band's highest shell, or M for a band with no upper edge) for every box.
+// f <- f + (s - 1) f on the band's modes of the ncomp components; the grid covers the cube
+// |m| <= r around the origin (r = the band's highest shell, or M for a band with no upper edge)
+// for every box.
 template <typename T>
-__global__ __launch_bounds__(256) void k_rescale_apply(cx<T> *__restrict__ uh, Geo g, BandSpec band, int r,
-                                                       const double *__restrict__ scalars) {
+__global__ __launch_bounds__(256) void k_rescale_apply(cx<T> *__restrict__ f, int ncomp, Geo g, BandSpec band, int r,
+                                                       const double *__restrict__ scalars, int slot) {
   const int box = blockIdx.z;
   const size_t l = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   const int side = 2 * r + 1;
@@ -122,14 +143,13 @@ __global__ __launch_bounds__(256) void k_rescale_apply(cx<T> *__restrict__ uh, G
   const int kx = (int)(l % (r + 1));
   const int my = (int)((l / (r + 1)) % side) - r, mz = (int)(l / ((size_t)(r + 1) * side)) - r;
   if (!in_band(kx * kx + my * my + mz * mz, band)) return;
-  const T f = T(scalars[8 * box + 6]);
+  const T s1 = T(scalars[8 * box + slot]);
   const int ky_idx = my < 0 ? my + g.k : my, kz_idx = mz < 0 ? mz + g.k : mz;
   const size_t comp = comp_of(g);
-  cx<T> *u = uh + (size_t)box * 3 * comp + ((size_t)kz_idx * g.k + ky_idx) * g.kxp + kx;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
+  cx<T> *u = f + (size_t)box * ncomp * comp + ((size_t)kz_idx * g.k + ky_idx) * g.kxp + kx;
+  for (int c = 0; c < ncomp; ++c) {
     const cx<T> v = u[c * comp];
-    u[c * comp] = {v.re + f * v.re, v.im + f * v.im};
+    u[c * comp] = {v.re + s1 * v.re, v.im + s1 * v.im};
   }
 }
 

@@ -86,6 +86,8 @@ struct PipelineImpl final : Pipeline {
       set_lds((const void *)k_zs<T, N, ZTX, ZN>, LDS_Z);
       set_lds((const void *)k_zf<T, N, ZTX, ZN, false, ZFL>, zf_lds_bytes<N, ZTX, T>(ZFL));
       set_lds((const void *)k_zs<T, N, ZTX, ZN, true>, LDS_Z);
+      set_lds((const void *)k_zs<T, N, ZTX, ZN, false, true>, LDS_Z);
+      set_lds((const void *)k_zs<T, N, ZTX, ZN, true, true>, LDS_Z);
       set_lds((const void *)k_zf<T, N, ZTX, ZN, false, ZFL, true>, zf_lds_bytes<N, ZTX, T>(ZFL));
     } else {
       constexpr int ZN = ZF_NT[1];
@@ -191,8 +193,9 @@ struct PipelineImpl final : Pipeline {
     else c.sched[0] ? yfwd_v<YT[1], 0>(c) : yfwd_v<YT[0], 0>(c);
   }
 
-  // SSP: the Shu-Osher stage update (IFSSPRK3, IFHeun), built for schedule 0 only (those schemes always run it)
-  template <int V, bool SSP = false>
+  // SSP: the Shu-Osher stage update (IFSSPRK3, IFHeun); SF: a scalar forcing.  Both are built for
+  // schedule 0 only, which those runs always take.
+  template <int V, bool SSP = false, bool SF = false>
   void zf_v(Ctx &c, const StageArgs &sa) {
     constexpr int ZN = ZF_NT[V];
     const dim3 grid(c.geo.kxp / ZTX, K, c.nbox);
@@ -201,8 +204,8 @@ struct PipelineImpl final : Pipeline {
     const cx<double> *tw = (const cx<double> *)c.tw;
     if constexpr (V == 0) {
       if (c.cfg.has_scalar)   // the scalar half reads the stage u_z before the velocity half updates it
-        hipLaunchKernelGGL((k_zs<T, N, ZTX, ZN, SSP>), grid, dim3(ZN), LDS_Z, c.stream, pf, (const cx<T> *)uh, ph, hp,
-                           c.geo, tw, sa);
+        hipLaunchKernelGGL((k_zs<T, N, ZTX, ZN, SSP, SF>), grid, dim3(ZN), LDS_Z, c.stream, pf, (const cx<T> *)uh, ph,
+                           hp, c.geo, tw, sa);
       hipLaunchKernelGGL((k_zf<T, N, ZTX, ZN, false, ZFL, SSP>), grid, dim3(ZN), (zf_lds_bytes<N, ZTX, T>(ZFL)),
                          c.stream, pf, uh, hu, ph, hp, c.geo, tw, sa);
     } else {
@@ -215,7 +218,11 @@ struct PipelineImpl final : Pipeline {
     }
   }
   void zf(Ctx &c, const StageArgs &sa) override {
-    if (sa.ssp) zf_v<0, true>(c, sa);
+    // the scalar forcings acting inside the stage (VarianceRescaling acts after the step)
+    const bool sf = c.cfg.has_scalar && c.cfg.scalar_forcing != HDNS_NO_FORCING &&
+                    c.cfg.scalar_forcing != HDNS_VARIANCE_RESCALING;
+    if (sa.ssp) sf ? zf_v<0, true, true>(c, sa) : zf_v<0, true>(c, sa);
+    else if (sf) zf_v<0, false, true>(c, sa);
     else c.sched[1] ? zf_v<1>(c, sa) : zf_v<0>(c, sa);
   }
 

@@ -36,6 +36,11 @@ struct StageArgs {
   BandSpec band;
   const double *scalars;  // per box [8]: [0] = the ConstantPower amplitude at this stage's state
   double lin_rate;        // Linear forcing: the rate A folded into the band's exponent, else 0
+  // the scalar forcing, read by the scalar kernels built with SF: the source A_phi 1_band phi_hat
+  // (A_phi = scalars[8 box + 7], 0 but for ConstantPower and VarianceController) and a Linear rate
+  // folded into the scalar's integrating factor on its band
+  BandSpec sband;
+  double slin_rate;
   // Shu-Osher form (IFSSPRK3 src/Propagator.jl:599-691, IFHeun :456-517), read by the kernels built with SSP:
   // q <- cb E^pb (q + dt N) + E^pa (ca h), with E the factor over the half step sub = dt / 2 and
   // h the step's initial state, which stage 0 saves
@@ -53,7 +58,7 @@ constexpr int last_radix() { return N % 3 == 0 ? 3 : ((N == 64 || N == 512) ? 8 
 // The scalar half of a stage (src/Generator.jl:648-730, src/Propagator.jl:366-391): the three
 // fluxes' forward z transforms, N_phi = -mask i k . F - G u_z at the stage velocity, and the
 // phi / h_phi update.  It runs before the velocity update, which overwrites u_z.
-template <typename T, int N, int TX, int NT, bool SSP>
+template <typename T, int N, int TX, int NT, bool SSP, bool SF>
 __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, const cx<T> *__restrict__ pf,
                                                const cx<T> *__restrict__ uh, cx<T> *__restrict__ ph,
                                                cx<T> *__restrict__ hp, const Geo &g, const StageArgs &sa, int kx0,
@@ -89,6 +94,7 @@ __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, con
   const T Gm = T(sa.mean_gradient);
   const T dt = T(sa.dt), a = T(sa.a), b = T(sa.b);
   const double coef_p = -sa.kappa[box] * sa.sub;
+  const T Aphi = SF ? T(sa.scalars[8 * box + 7]) : T(0);
   __builtin_amdgcn_sched_barrier(0);   // the state loads stay behind the transforms
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
@@ -127,18 +133,25 @@ __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, con
         const int mz = signed_of(idx_of_bin(j + q * SL::NB, g), g);
         const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
         const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
-        const cx<T> nl = cx<T>{G[p][q].im, -G[p][q].re} - scale(uz[i], Gm);   // -i k.F - G u_z
+        cx<T> nl = cx<T>{G[p][q].im, -G[p][q].re} - scale(uz[i], Gm);   // -i k.F - G u_z
+        double xe = coef_p * k2d;   // the scalar's integrating-factor exponent over the substep
+        if constexpr (SF) {
+          if (in_band(mx * mx + my * my + mz * mz, sa.sband)) {   // src/Forcing.jl:1104-1117, :600-642
+            nl = nl + scale(pv[i], Aphi);
+            xe += sa.slin_rate * sa.sub;
+          }
+        }
         if constexpr (SSP) {
-          const cx<T> t = scale(pv[i] + scale(nl, dt), T(exp(sa.pb * coef_p * k2d)));
+          const cx<T> t = scale(pv[i] + scale(nl, dt), T(exp(sa.pb * xe)));
           if (sa.stage == 0) {
             ph[at[i]] = t;
             hp[at[i]] = pv[i];
           } else {
-            ph[at[i]] = scale(t, T(sa.cb)) + scale(scale(hv[i], T(sa.ca)), T(exp(sa.pa * coef_p * k2d)));
+            ph[at[i]] = scale(t, T(sa.cb)) + scale(scale(hv[i], T(sa.ca)), T(exp(sa.pa * xe)));
           }
           continue;
         }
-        const T E = T(exp(coef_p * k2d));
+        const T E = T(exp(xe));
         cx<T> h = scale(nl, dt);
         if (sa.stage > 0) h = scale(hv[i], a) + h;
         ph[at[i]] = scale(pv[i] + scale(h, b), E);
@@ -199,7 +212,7 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
     hp += box * comp;
   }
 
-  if constexpr (SCALAR) zf_scalar_part<T, N, TX, NT, SSP>(buf, twl, pf, uh, ph, hp, g, sa, kx0, ky_idx, my, box);
+  if constexpr (SCALAR) zf_scalar_part<T, N, TX, NT, SSP, false>(buf, twl, pf, uh, ph, hp, g, sa, kx0, ky_idx, my, box);
 
   // velocity products, forward z transform, straight into the owned modes.  The compiler
   // memory barrier keeps the scalar part's u_z loads from living on into the velocity part.
@@ -343,7 +356,7 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
 
 // The scalar half on its own (the velocity half then runs as k_zf without SCALAR): two smaller
 // kernels hold fewer registers than the fused one.
-template <typename T, int N, int TX, int NT, bool SSP = false>
+template <typename T, int N, int TX, int NT, bool SSP = false, bool SF = false>
 __global__ __launch_bounds__(NT) void k_zs(const cx<T> *__restrict__ pf, const cx<T> *__restrict__ uh,
                                            cx<T> *__restrict__ ph, cx<T> *__restrict__ hp, Geo g,
                                            const cx<double> *__restrict__ tw, StageArgs sa) {
@@ -355,7 +368,7 @@ __global__ __launch_bounds__(NT) void k_zs(const cx<T> *__restrict__ pf, const c
   if (!in_disk(kx0, my, N)) return;   // no column of this tile meets the ball (its PF lines are unwritten)
   stage_twiddles<T, N, NT>(twl, tw);
   const size_t comp = comp_of(g);
-  zf_scalar_part<T, N, TX, NT, SSP>(buf, twl, pf + box * g.box_b, uh + box * 3 * comp, ph + box * comp,
+  zf_scalar_part<T, N, TX, NT, SSP, SF>(buf, twl, pf + box * g.box_b, uh + box * 3 * comp, ph + box * comp,
                                hp + box * comp, g, sa, kx0, ky_idx, my, box);
 }
 

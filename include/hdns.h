@@ -75,6 +75,16 @@ typedef struct {
   double forcing_time;       /* VarianceController turnover time t_l > 0 */
   int32_t particle_steps;    /* k of ABM(k) (1..7) or AB(k) (1..8); ignored for HDNS_HEUN */
   int32_t reserved;
+  /* the scalar forcing (src/Generator.jl:652-653): the velocity forcings' strategies with the
+   * variance factor 1 and the diffusivity nu/Sc (src/Forcing.jl:456); needs has_scalar */
+  int32_t scalar_forcing;        /* HDNS_NO_FORCING, _CONSTANT_POWER, _LINEAR, _VARIANCE_CONTROLLER, _VARIANCE_RESCALING */
+  int32_t scalar_lowest_shell;   /* >= 1 */
+  int32_t scalar_highest_shell;  /* >= scalar_lowest_shell; -1 means Inf */
+  int32_t scalar_over;           /* VarianceRescaling: HDNS_OVER_BAND or HDNS_OVER_TOTAL */
+  double scalar_rate;            /* ConstantPower's injection rate, or the Linear rate */
+  double scalar_target;          /* VarianceController's or VarianceRescaling's target */
+  double scalar_gain;            /* VarianceController G >= 0 */
+  double scalar_time;            /* VarianceController turnover time > 0 */
 } hdns_config;
 
 typedef struct hdns_ctx hdns_ctx;

@@ -197,6 +197,11 @@ class Conditions:
     controller: tuple | None = None
     # VarianceRescaling (target, over "band" | "total", lowest_shell, highest_shell)
     rescaling: tuple | None = None
+    # the scalar forcing (src/Generator.jl:652-653, the same strategies with the variance factor 1
+    # and the diffusivity nu/Sc, src/Forcing.jl:456): one of
+    #   ("constant_power", injection_rate, lo, hi), ("linear", rate, lo, hi),
+    #   ("controller", target, gain, turnover_time, lo, hi), ("rescaling", target, over, lo, hi)
+    scalar_forcing: tuple | None = None
     band_eps: float = float(np.finfo(np.float64).eps)
 
     @property
@@ -213,6 +218,59 @@ def linear_factor(d: Domain, cond: Conditions, sub: float) -> np.ndarray:
         rate, lo, hi = cond.linear
         expo = expo + rate * sub * d.shell_band(lo, hi)
     return np.exp(expo)
+
+
+def scalar_factor(d: Domain, cond: Conditions, sub: float) -> np.ndarray:
+    """The scalar's integrating factor over a substep: exp(-(nu/Sc) |k|^2 sub), with a Linear scalar
+    forcing's rate folded in on its band (src/Generator.jl:583-595)."""
+    expo = -cond.scalar_diffusivity * sub * d.k2
+    sf = cond.scalar_forcing
+    if sf is not None and sf[0] == "linear":
+        expo = expo + sf[1] * sub * d.shell_band(sf[2], sf[3])
+    return np.exp(expo)
+
+
+def scalar_amplitude(d: Domain, cond: Conditions, ph: np.ndarray) -> float:
+    """A of a ConstantPower or VarianceController scalar forcing at the stage scalar, with the
+    variance factor 1: eps / (2 <phi^2>_band), or (2 kappa <phi, |k|^2 phi> - G (<phi^2> - target) / t_l)
+    / (2 <phi^2>_band); 0 for an empty band (src/Forcing.jl:1041-1117)."""
+    sf = cond.scalar_forcing
+    lo, hi = sf[-2], sf[-1]
+    band_obs = weighted_inner(d, ph, d.shell_band(lo, hi))
+    whole = weighted_inner(d, ph)
+    if sf[0] == "constant_power":
+        num = sf[1]
+    else:
+        _, target, gain, tl, _, _ = sf
+        num = 2 * cond.scalar_diffusivity * weighted_inner(d, ph, d.k2) - gain * (whole - target) / tl
+    if band_is_empty(band_obs, whole, cond.band_eps):
+        return 0.0
+    return num / (2 * band_obs)
+
+
+def scalar_rescaling(d: Domain, cond: Conditions, ph: np.ndarray) -> np.ndarray:
+    """VarianceRescaling of the scalar after the step (src/Forcing.jl:575-596, variance factor 1)."""
+    _, target, over, lo, hi = cond.scalar_forcing
+    band = d.shell_band(lo, hi)
+    band_obs = weighted_inner(d, ph, band)
+    whole = weighted_inner(d, ph)
+    if band_is_empty(band_obs, whole, cond.band_eps):
+        return ph
+    target_band = max(0.0, target - whole + band_obs) if over == "total" else target
+    return ph + (math.sqrt(target_band / band_obs) - 1) * band * ph
+
+
+def finalize_fields(d: Domain, cond: Conditions, uh: np.ndarray, ph: np.ndarray | None):
+    """finalize! (the post-step projections of the forcings), then the state Leray projection and
+    the reality projection of both fields (src/Propagator.jl:394-412)."""
+    if cond.rescaling is not None:
+        uh = variance_rescaling(d, cond, uh)
+    if ph is not None and cond.scalar_forcing is not None and cond.scalar_forcing[0] == "rescaling":
+        ph = scalar_rescaling(d, cond, ph)
+    uh = reality_projection(d, leray(d, uh))
+    if ph is not None:
+        ph = reality_projection(d, ph)
+    return uh, ph
 
 
 def band_is_empty(band_value: float, whole_value: float, eps: float) -> bool:
@@ -282,12 +340,14 @@ def velocity_rhs(d: Domain, cond: Conditions, uh: np.ndarray, u: np.ndarray, w: 
     return nl
 
 
-def scalar_rhs(d: Domain, cond: Conditions, uh: np.ndarray, u: np.ndarray, phi: np.ndarray) -> np.ndarray:
-    """N_phi = -mask i kappa . FFT(u phi) - G u_z_hat.
+def scalar_rhs(d: Domain, cond: Conditions, uh: np.ndarray, u: np.ndarray, phi: np.ndarray,
+               ph: np.ndarray | None = None) -> np.ndarray:
+    """N_phi = -mask i kappa . FFT(u phi) - G u_z_hat, plus a scalar forcing's source A 1_band phi_hat.
 
     src/Generator.jl:648-661 (scalar_rhs!), :663-666 (mask strategies, alpha = -1), :708-730
     (_scalar_flux_divergence!, _scalar_flux!).  The -G u_z term is the mean-gradient extension
-    (the reference has none), evaluated at the stage velocity like every other source.
+    (the reference has none), evaluated at the stage velocity like every other source.  ``ph`` is
+    the stage scalar in spectral space, needed by a ConstantPower or VarianceController scalar forcing.
     """
     flux = u * phi[None]
     f = rfft3(flux)
@@ -296,6 +356,9 @@ def scalar_rhs(d: Domain, cond: Conditions, uh: np.ndarray, u: np.ndarray, phi: 
     nl = -(d.mask * div)
     if cond.mean_gradient != 0.0:
         nl = nl - cond.mean_gradient * uh[2]
+    sf = cond.scalar_forcing
+    if sf is not None and sf[0] in ("constant_power", "controller"):
+        nl = nl + scalar_amplitude(d, cond, ph) * d.shell_band(sf[-2], sf[-1]) * ph
     return nl
 
 
@@ -388,7 +451,6 @@ def step(d: Domain, cond: Conditions, scheme: str, uh: np.ndarray, ph: np.ndarra
         dep = deposit(d, uh, ph)
     hu = None
     hp = None
-    kap = cond.scalar_diffusivity
     for i in range(S):
         if i == 0:
             u, w, phi = dep.u, dep.w, dep.phi
@@ -397,7 +459,7 @@ def step(d: Domain, cond: Conditions, scheme: str, uh: np.ndarray, ph: np.ndarra
             w = irfft3(curl(d, uh), d.n)
             phi = irfft3(ph, d.n) if ph is not None else None
         nu_hat = velocity_rhs(d, cond, uh, u, w)
-        np_hat = scalar_rhs(d, cond, uh, u, phi) if ph is not None else None
+        np_hat = scalar_rhs(d, cond, uh, u, phi, ph) if ph is not None else None
         hu = dt * nu_hat if i == 0 else a[i] * hu + dt * nu_hat
         sub = dd[i] * dt
         eu = linear_factor(d, cond, sub)
@@ -406,16 +468,11 @@ def step(d: Domain, cond: Conditions, scheme: str, uh: np.ndarray, ph: np.ndarra
             hu = eu * hu
         if ph is not None:
             hp = dt * np_hat if i == 0 else a[i] * hp + dt * np_hat
-            ep = np.exp(-kap * sub * d.k2)
+            ep = scalar_factor(d, cond, sub)
             ph = ep * (ph + b[i] * hp)
             if i < S - 1:
                 hp = ep * hp
-    if cond.rescaling is not None:   # finalize! before the state projections (:394-401)
-        uh = variance_rescaling(d, cond, uh)
-    uh = leray(d, uh)
-    uh = reality_projection(d, uh)
-    if ph is not None:
-        ph = reality_projection(d, ph)
+    uh, ph = finalize_fields(d, cond, uh, ph)
     return uh, ph, deposit(d, uh, ph)
 
 
@@ -434,10 +491,9 @@ def ssp_step(d: Domain, cond: Conditions, uh: np.ndarray, ph: np.ndarray | None,
     w34, w14, w13, w23 = (_round(Fraction(p, q), coeff_dtype) for p, q in ((3, 4), (1, 4), (1, 3), (2, 3)))
     if dep is None:
         dep = deposit(d, uh, ph)
-    kap = cond.scalar_diffusivity
 
     def factors(t):
-        return linear_factor(d, cond, t), np.exp(-kap * t * d.k2)
+        return linear_factor(d, cond, t), scalar_factor(d, cond, t)
 
     def rhs(q, p, first):
         if first:
@@ -446,7 +502,7 @@ def ssp_step(d: Domain, cond: Conditions, uh: np.ndarray, ph: np.ndarray | None,
             u, w = irfft3(q, d.n), irfft3(curl(d, q), d.n)
             phi = irfft3(p, d.n) if p is not None else None
         nu_hat = velocity_rhs(d, cond, q, u, w)
-        np_hat = scalar_rhs(d, cond, q, u, phi) if p is not None else None
+        np_hat = scalar_rhs(d, cond, q, u, phi, p) if p is not None else None
         return nu_hat, np_hat
 
     eu_full, ep_full = factors(dt)
@@ -463,11 +519,7 @@ def ssp_step(d: Domain, cond: Conditions, uh: np.ndarray, ph: np.ndarray | None,
     q = eu_full * (w13 * uh) + w23 * (eu_half * (q + dt * n2))
     if ph is not None:
         p = ep_full * (w13 * ph) + w23 * (ep_half * (p + dt * m2))
-    if cond.rescaling is not None:   # finalize! before the state projections (:674-677)
-        q = variance_rescaling(d, cond, q)
-    q = reality_projection(d, leray(d, q))
-    if p is not None:
-        p = reality_projection(d, p)
+    q, p = finalize_fields(d, cond, q, p)   # finalize! before the state projections (:674-688)
     return q, p, deposit(d, q, p)
 
 
@@ -478,21 +530,17 @@ def heun_step(d: Domain, cond: Conditions, uh: np.ndarray, ph: np.ndarray | None
     for u_hat and phi_hat in lockstep, then finalize! and the state projections (:519-538)."""
     if dep is None:
         dep = deposit(d, uh, ph)
-    eu, ep = linear_factor(d, cond, dt), np.exp(-cond.scalar_diffusivity * dt * d.k2)
+    eu, ep = linear_factor(d, cond, dt), scalar_factor(d, cond, dt)
     n1 = velocity_rhs(d, cond, uh, dep.u, dep.w)
-    m1 = scalar_rhs(d, cond, uh, dep.u, dep.phi) if ph is not None else None
+    m1 = scalar_rhs(d, cond, uh, dep.u, dep.phi, ph) if ph is not None else None
     us = eu * (uh + dt * n1)
     ps = ep * (ph + dt * m1) if ph is not None else None
     u, w = irfft3(us, d.n), irfft3(curl(d, us), d.n)
     n2 = velocity_rhs(d, cond, us, u, w)
-    m2 = scalar_rhs(d, cond, us, u, irfft3(ps, d.n)) if ph is not None else None
+    m2 = scalar_rhs(d, cond, us, u, irfft3(ps, d.n), ps) if ph is not None else None
     q = eu * uh + dt / 2 * (eu * n1) + dt / 2 * n2
     p = ep * ph + dt / 2 * (ep * m1) + dt / 2 * m2 if ph is not None else None
-    if cond.rescaling is not None:
-        q = variance_rescaling(d, cond, q)
-    q = reality_projection(d, leray(d, q))
-    if p is not None:
-        p = reality_projection(d, p)
+    q, p = finalize_fields(d, cond, q, p)
     return q, p, deposit(d, q, p)
 
 

@@ -152,3 +152,85 @@ def test_the_adjoint_refuses_the_variance_forcings(g, kind):
                       state=g.ResolvedFields(g.random_phase_velocity(n, seed=1), None)) as sim:
         with pytest.raises(Exception, match="NoForcing and ConstantPower"):
             sim.adjoint_gradient(1e-3, 2)
+
+
+# --- scalar forcings ---------------------------------------------------------------------------
+
+def _scalar_forcing(g, dom, kind):
+    """The library's scalar forcing and the oracle's tuple for it."""
+    if kind == "constant_power":
+        f = g.ConstantPower(dom, injection_rate=0.05, lowest_shell=1, highest_shell=3)
+        return f, ("constant_power", f.injection_rate, 1, 3)
+    if kind == "linear":
+        f = g.Linear(dom, rate=0.5, lowest_shell=1, highest_shell=2)
+        return f, ("linear", f.rate, 1, 2)
+    if kind == "controller":
+        f = g.VarianceController(dom, target=0.9, gain=2.0, turnover_time=0.6, highest_shell=math.inf)
+        return f, ("controller", f.target, f.gain, f.turnover_time, 1, math.inf)
+    over = kind.split("_")[1]   # the seeded scalar holds <phi^2> = 0.5, 0.058 of it on shells 1-2
+    f = g.VarianceRescaling(dom, target=0.1 if over == "band" else 0.55, over=over, lowest_shell=1, highest_shell=2)
+    return f, ("rescaling", f.target, over, 1, 2)
+
+
+@pytest.mark.parametrize("kind", ["constant_power", "linear", "controller", "rescaling_band", "rescaling_total"])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 1e-5)])
+@pytest.mark.parametrize("scheme", ["rk4", "ssprk3"])
+def test_scalar_forcings_match_the_oracle(g, kind, dtype, tol, scheme):
+    """Each scalar forcing (variance factor 1, diffusivity nu/Sc) beside a ConstantPower velocity
+    forcing and a mean gradient, 10 steps against the oracle, checked after 1 and 10."""
+    n, dt = 32, 2e-3
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
+    sf, osf = _scalar_forcing(g, dom, kind)
+    vf = g.ConstantPower(dom, injection_rate=0.1)
+    cond = g.FlowConditions(viscosity=0.02, schmidt=0.7, velocity_forcing=vf, scalar_forcing=sf)
+    u, p = g.random_phase_velocity(n, seed=81), g.random_phase_scalar(n, seed=82)
+    oc = dns.Conditions(viscosity=0.02, schmidt=0.7, mean_gradient=0.5, forcing=(vf.injection_rate, 1, 2),
+                        scalar_forcing=osf, band_eps=float(np.finfo(dtype).eps))
+    d = dns.Domain(n)
+    sch = {"rk4": g.IFLowStorageRK4, "ssprk3": g.IFSSPRK3}[scheme]
+    uo, po, dep = u, p, None
+    with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=0.5,
+                      field_integration_scheme=sch) as sim:
+        for s in range(1, 11):
+            sim.advance(dt)
+            uo, po, dep = dns.step(d, oc, scheme, uo, po, dt, dep)
+            if s in (1, 10):
+                st = sim.state()
+                eu, ep = rel(st.velocity_spectral, uo), rel(st.scalar_spectral, po)
+                print(f"scalar {kind} {scheme} {np.dtype(dtype).name} step {s}: u {eu:.2e} phi {ep:.2e}", flush=True)
+                assert eu < tol and ep < tol
+        if kind == "rescaling_total":
+            assert sim.reductions()["scalar_variance"] == pytest.approx(0.55, rel=1e-12 if dtype is np.float64 else 1e-5)
+
+
+def test_scalar_linear_forcing_grows_its_band_exactly_on_the_device(g):
+    """u = 0: phi = 0.6 (sin y + 0.5 cos 3z) evolves as exp((A - kappa) t) on shell 1 and
+    exp(-9 kappa t) on shell 3, under every field scheme."""
+    n, nu, sc, A, dt, steps = 32, 0.05, 0.5, 0.4, 1e-2, 20
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
+    x = np.arange(n) * L / n
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    ph = g.truncate_to_band(dns.rfft3(0.6 * (np.sin(Y) + 0.5 * np.cos(3 * Z))), dom)
+    d = dns.Domain(n)
+    kap = nu / sc
+    want = ph * np.where(d.shell_band(1, 2), np.exp((A - kap * d.k2) * steps * dt), np.exp(-kap * d.k2 * steps * dt))
+    for scheme in (g.IFLowStorageRK3, g.IFLowStorageRK4, g.IFSSPRK3, g.IFHeun):
+        cond = g.FlowConditions(viscosity=nu, schmidt=sc, scalar_forcing=g.Linear(dom, rate=A))
+        with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(np.zeros((3,) + ph.shape, complex), ph),
+                          field_integration_scheme=scheme) as sim:
+            for _ in range(steps):
+                sim.advance(dt)
+            got = sim.state().scalar_spectral
+        assert np.max(np.abs(got - want)) <= 1e-12 * np.max(np.abs(want)), scheme.__name__
+
+
+def test_a_scalar_forcing_needs_the_scalar_and_the_adjoint_refuses_it(g):
+    n = 16
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
+    cond = g.FlowConditions(viscosity=0.02, schmidt=1.0, scalar_forcing=g.ConstantPower(dom, injection_rate=0.1))
+    with pytest.raises(ValueError, match="carries the scalar"):
+        g.Simulation(dom, conditions=cond, state=g.ResolvedFields(g.random_phase_velocity(n, seed=1), None))
+    with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(g.random_phase_velocity(n, seed=1),
+                                                                   g.random_phase_scalar(n, seed=2))) as sim:
+        with pytest.raises(Exception, match="scalar forcing"):
+            sim.adjoint_gradient(1e-3, 2)

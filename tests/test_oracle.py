@@ -557,3 +557,69 @@ def test_heun_is_the_shu_osher_ssprk2():
     want = dns.reality_projection(d, dns.leray(d, so))
     got, _, _ = dns.step(d, cond, "heun", uh, None, dt)
     assert np.max(np.abs(got - want)) <= 1e-14 * np.max(np.abs(want))
+
+
+# --- scalar forcings ---------------------------------------------------------------------------
+
+def _scalar_wave(n, amp=0.6):
+    """phi = amp (sin y + 0.5 cos 3z): shells 1 and 3, with u = 0 nothing advects it."""
+    x = np.arange(n) * TWO_PI / n
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    return dns.rfft3(amp * (np.sin(Y) + 0.5 * np.cos(3 * Z)))
+
+
+def test_scalar_constant_power_injects_exactly_its_rate():
+    """<phi, A 1_band phi> = eps_phi / 2 with the variance factor 1 (src/Forcing.jl:456, :1104-1117):
+    d<phi^2>/dt gets exactly eps_phi from the source."""
+    n = 16
+    d = dns.Domain(n)
+    _, f = _band_limited(n, 12, shell_frac=0.8)
+    ph = f[0]
+    cond = dns.Conditions(viscosity=0.02, scalar_forcing=("constant_power", 0.3, 1, 2))
+    src = dns.scalar_amplitude(d, cond, ph) * d.shell_band(1, 2) * ph
+    rate = 2 * sum(np.real(np.sum(d.weights * np.conj(ph) * src, axis=None)) for _ in [0]) / d.norm
+    assert rate == pytest.approx(0.3, rel=1e-12)
+
+
+@pytest.mark.parametrize("scheme", ["rk3", "rk4", "ssprk3", "heun"])
+def test_scalar_linear_forcing_grows_its_band_exactly(scheme):
+    """With u = 0 the scalar is linear: shell 1 grows as exp((A - kappa) t), shell 3 decays as
+    exp(-9 kappa t) (the rate folded into the scalar integrating factor)."""
+    n, nu, sc, A, dt, steps = 16, 0.05, 0.5, 0.4, 1e-2, 20
+    d = dns.Domain(n)
+    ph = _scalar_wave(n)
+    kap = nu / sc
+    cond = dns.Conditions(viscosity=nu, schmidt=sc, scalar_forcing=("linear", A, 1, 2))
+    want = ph * np.where(d.shell_band(1, 2), np.exp((A - kap * d.k2) * steps * dt), np.exp(-kap * d.k2 * steps * dt))
+    u, p, dep = np.zeros((3,) + ph.shape, dtype=complex), ph, None
+    for _ in range(steps):
+        u, p, dep = dns.step(d, cond, scheme, u, p, dt, dep)
+    assert np.max(np.abs(p - want)) <= 1e-13 * np.max(np.abs(want))
+
+
+def test_scalar_controller_relaxes_the_variance_exponentially():
+    """u = 0: d<phi^2>/dt = -2 kappa <phi, |k|^2 phi> + 2 A <phi^2>_band = -G (<phi^2> - target)/t_l."""
+    n, nu, sc, dt, steps, G, tl, target = 16, 0.05, 0.5, 5e-3, 60, 2.0, 0.8, 0.6
+    d = dns.Domain(n)
+    ph = _scalar_wave(n)
+    cond = dns.Conditions(viscosity=nu, schmidt=sc, scalar_forcing=("controller", target, G, tl, 1, 2))
+    v0 = dns.weighted_inner(d, ph)
+    u, p, dep = np.zeros((3,) + ph.shape, dtype=complex), ph, None
+    for s in range(1, steps + 1):
+        u, p, dep = dns.step(d, cond, "rk4", u, p, dt, dep)
+        want = target + (v0 - target) * math.exp(-G * s * dt / tl)
+        assert dns.weighted_inner(d, p) == pytest.approx(want, rel=2e-8)   # IF-RK4's truncation error
+
+
+@pytest.mark.parametrize("over", ["band", "total"])
+def test_scalar_rescaling_holds_its_target_after_every_step(over):
+    n = 16
+    d, uh = _band_limited(n, 13, shell_frac=0.6)
+    _, f = _band_limited(n, 14, shell_frac=0.6)
+    ph = f[0] * math.sqrt(0.3 / dns.weighted_inner(d, f[0]))
+    cond = dns.Conditions(viscosity=0.02, mean_gradient=0.5, scalar_forcing=("rescaling", 0.45, over, 1, 2))
+    u, p, dep = uh, ph, None
+    for _ in range(4):
+        u, p, dep = dns.step(d, cond, "rk3", u, p, 1e-2, dep)
+        held = dns.weighted_inner(d, p, d.shell_band(1, 2)) if over == "band" else dns.weighted_inner(d, p)
+        assert held == pytest.approx(0.45, rel=1e-13)
