@@ -975,15 +975,19 @@ class Ensemble:
     """
 
     def __init__(self, members: list[EnsembleMember], *, field_integration_scheme=IFLowStorageRK3,
-                 mean_gradient: float = 0.0, keep_physical: bool = False):
+                 mean_gradient: float = 0.0, keep_physical: bool = False, particle_integration_scheme=None):
         if not members:
             raise ValueError("an ensemble needs at least one box")
         self.field_integration_scheme = field_integration_scheme
+        self.particle_integration_scheme = particle_integration_scheme or Heun()
+        if not isinstance(self.particle_integration_scheme, (ABM, AB)):
+            raise ValueError("the particle integration scheme must be Heun(), ABM(k) or AB(k), got "
+                             f"{self.particle_integration_scheme!r}")
         cfgs = (L.HdnsConfig * len(members))()
         for b, mb in enumerate(members):
             carries = mb.state.scalar_spectral is not None
             cfgs[b] = _config(mb.domain, mb.conditions, field_integration_scheme, carries, mean_gradient, mb.tracers,
-                              None, keep_physical, False)
+                              None, keep_physical, False, self.particle_integration_scheme)
         h = c_void_p()
         L.check(L.lib().hdns_ensemble_create(cfgs, len(members), ctypes.byref(h)), ensemble=None)
         self._h = h
@@ -1028,7 +1032,8 @@ class Ensemble:
         return self
 
     def advance_tracers(self, dt: float) -> None:
-        """Traversal rebuild, Heun push and sample of every box's tracers (after a field step)."""
+        """Traversal rebuild, push (the ensemble's particle scheme over each batch's trail) and sample
+        of every box's tracers (after a field step)."""
         self._ecall("hdns_ensemble_advance_tracers", float(dt))
 
     def synchronize(self) -> None:
