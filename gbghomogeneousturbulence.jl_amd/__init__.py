@@ -11,11 +11,12 @@ from .api import (AB, ABM, ConstantPower, Ensemble, EnsembleBox, EnsembleMember,
                   StationarityCriterion, cfl_bound, integral_length, predicted_viscosity,
                   recommended_cfl, truncate_to_band, warmup)
 from .seeding import random_phase_scalar, random_phase_velocity, uniform_positions
+from .stochastic import StochasticOU
 
 __all__ = [
     "AB", "ABM", "ConstantPower", "Ensemble", "EnsembleBox", "EnsembleMember", "FixedDomain", "FlowConditions", "Heun",
     "IFHeun", "IFLowStorageRK3", "IFLowStorageRK4", "IFSSPRK3", "Linear", "NoForcing", "VarianceController",
     "VarianceRescaling", "ResolvedFields", "Simulation", "Tracers", "TwoThirds", "predicted_viscosity",
     "recommended_cfl", "cfl_bound", "integral_length", "StationarityCriterion", "warmup", "truncate_to_band", "random_phase_scalar", "random_phase_velocity",
-    "uniform_positions",
+    "uniform_positions", "StochasticOU",
 ]

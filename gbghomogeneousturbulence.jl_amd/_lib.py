@@ -19,7 +19,7 @@ HEADER = HERE.parent / "include" / "hdns.h"
 OK, ERR_INVALID_ARGUMENT, ERR_STATE, ERR_NONFINITE, ERR_DEVICE, ERR_UNSUPPORTED = range(6)
 FP32_BULK, FP64 = 0, 1
 RK3, RK4, SSPRK3, HEUN = 3, 4, 5, 6
-NO_FORCING, CONSTANT_POWER, LINEAR, VARIANCE_CONTROLLER, VARIANCE_RESCALING = 0, 1, 2, 3, 4
+NO_FORCING, CONSTANT_POWER, LINEAR, VARIANCE_CONTROLLER, VARIANCE_RESCALING, STOCHASTIC_OU = 0, 1, 2, 3, 4, 5
 OVER_BAND, OVER_TOTAL = 0, 1
 P_HEUN, P_ABM, P_AB = 0, 1, 2
 FIELD_VELOCITY, FIELD_SCALAR, FIELD_VORTICITY = 0, 1, 2
@@ -78,6 +78,7 @@ _SIGS = {
     "hdns_tracer_trail": [c_void_p, POINTER(c_int32), POINTER(c_int32), _D],
     "hdns_tracer_trail_sample": [c_void_p, c_int32, _D],
     "hdns_tracer_trail_restore": [c_void_p, c_int32, _D, _D, c_double],
+    "hdns_band_source_upload": [c_void_p, c_int32, _D],
     "hdns_tracer_time_derivative": [c_void_p, c_int32, _D],
     "hdns_tracer_cells": [c_void_p, POINTER(c_int64)],
     "hdns_traversal_order": [c_void_p, POINTER(c_int32)],

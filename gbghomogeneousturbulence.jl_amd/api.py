@@ -17,6 +17,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib as L
+from .stochastic import OUProcess, StochasticOU
 
 
 # --- frame ---------------------------------------------------------------------------------
@@ -238,9 +239,9 @@ class FlowConditions:
         self.velocity_forcing = velocity_forcing if velocity_forcing is not None else NoForcing()
         self.scalar_forcing = scalar_forcing if scalar_forcing is not None else NoForcing()
         if not isinstance(self.scalar_forcing,
-                          (NoForcing, ConstantPower, Linear, VarianceController, VarianceRescaling)):
+                          (NoForcing, ConstantPower, Linear, VarianceController, VarianceRescaling, StochasticOU)):
             raise NotImplementedError("the scalar forcings built are NoForcing, ConstantPower, Linear, "
-                                      "VarianceController and VarianceRescaling")
+                                      "VarianceController, VarianceRescaling and StochasticOU")
         if velocity_forcing is not None and velocity_forcing is scalar_forcing and not isinstance(velocity_forcing, NoForcing):
             raise ValueError("one forcing instance is bound to both `velocity_forcing` and `scalar_forcing`; "
                              "build two (src/Generator.jl:111-115)")
@@ -251,9 +252,9 @@ class FlowConditions:
                              "derived from the kinetic-energy model, which is meaningless for a scalar variance; "
                              "pass the flow's turnover time 2k/(3 eps) explicitly")
         if not isinstance(self.velocity_forcing,
-                          (NoForcing, ConstantPower, Linear, VarianceController, VarianceRescaling)):
+                          (NoForcing, ConstantPower, Linear, VarianceController, VarianceRescaling, StochasticOU)):
             raise NotImplementedError("the velocity forcings built are NoForcing, ConstantPower, Linear, "
-                                      "VarianceController and VarianceRescaling")
+                                      "VarianceController, VarianceRescaling and StochasticOU")
 
     @property
     def scalar_diffusivity(self) -> float:
@@ -472,10 +473,12 @@ def _config(domain: FixedDomain, conditions: FlowConditions, scheme, carries_sca
     cfg.viscosity = conditions.viscosity
     cfg.schmidt = conditions.schmidt
     f = conditions.velocity_forcing
-    if isinstance(f, (ConstantPower, Linear, VarianceController, VarianceRescaling)):
+    if isinstance(f, (ConstantPower, Linear, VarianceController, VarianceRescaling, StochasticOU)):
         cfg.lowest_shell = f.lowest_shell
         cfg.highest_shell = -1 if f.highest_shell == math.inf else f.highest_shell
-    if isinstance(f, (ConstantPower, Linear)):
+    if isinstance(f, StochasticOU):
+        cfg.velocity_forcing = L.STOCHASTIC_OU
+    elif isinstance(f, (ConstantPower, Linear)):
         cfg.velocity_forcing = L.CONSTANT_POWER if isinstance(f, ConstantPower) else L.LINEAR
         cfg.injection_rate = f.injection_rate if isinstance(f, ConstantPower) else f.rate
     elif isinstance(f, VarianceController):
@@ -491,7 +494,9 @@ def _config(domain: FixedDomain, conditions: FlowConditions, scheme, carries_sca
     if carries_scalar and not isinstance(sf, NoForcing):
         cfg.scalar_lowest_shell = sf.lowest_shell
         cfg.scalar_highest_shell = -1 if sf.highest_shell == math.inf else sf.highest_shell
-        if isinstance(sf, ConstantPower):
+        if isinstance(sf, StochasticOU):
+            cfg.scalar_forcing = L.STOCHASTIC_OU
+        elif isinstance(sf, ConstantPower):
             cfg.scalar_forcing, cfg.scalar_rate = L.CONSTANT_POWER, sf.injection_rate
         elif isinstance(sf, Linear):
             cfg.scalar_forcing, cfg.scalar_rate = L.LINEAR, sf.rate
@@ -555,6 +560,12 @@ class Simulation:
         L.check(L.lib().hdns_ctx_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._ctx = h
         self._cfg = cfg
+        # StochasticOU forcing states, stationary at the start (src/Forcing.jl:1388-1392)
+        self.ou = {}
+        if isinstance(conditions.velocity_forcing, StochasticOU):
+            self.ou[0] = OUProcess(conditions.velocity_forcing, 1)
+        if carries_scalar and isinstance(conditions.scalar_forcing, StochasticOU):
+            self.ou[1] = OUProcess(conditions.scalar_forcing, 0)
         self.fields_epoch = 0
         self.particles_epoch = 0
         self.trail_epoch = None
@@ -751,7 +762,11 @@ class Simulation:
 
     def step_fields(self, dt: float, graph: bool = False) -> None:
         """``step!`` then the field ``deposit!`` (src/Propagator.jl:304-419,808-842), asynchronously;
-        ``graph`` replays the step as one captured HIP graph."""
+        ``graph`` replays the step as one captured HIP graph.  A StochasticOU forcing's process first
+        takes its transition over dt and its band force goes to the device (prepare_forcing!)."""
+        for field, proc in self.ou.items():
+            proc.advance(dt)
+            self._call("hdns_band_source_upload", int(field), L.dptr(proc.band_source(self.domain.grid_points_per_axis)))
         self._call("hdns_step_graph" if graph else "hdns_step", float(dt))
         self.fields_epoch += 1
 
@@ -985,6 +1000,10 @@ class Ensemble:
                              f"{self.particle_integration_scheme!r}")
         cfgs = (L.HdnsConfig * len(members))()
         for b, mb in enumerate(members):
+            if isinstance(mb.conditions.velocity_forcing, StochasticOU) or isinstance(mb.conditions.scalar_forcing,
+                                                                                     StochasticOU):
+                raise NotImplementedError("an ensemble box cannot carry a StochasticOU forcing (its process is "
+                                          "kept per standalone run)")
             carries = mb.state.scalar_spectral is not None
             cfgs[b] = _config(mb.domain, mb.conditions, field_integration_scheme, carries, mean_gradient, mb.tracers,
                               None, keep_physical, False, self.particle_integration_scheme)

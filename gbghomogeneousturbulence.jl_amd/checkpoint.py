@@ -162,6 +162,9 @@ def write_forcing(parent: Group, name: str, f) -> Group:
         g = _tagged(parent, name, "VarianceController")
         g.attrs.update(target=f.target, gain=f.gain, turnover_time=f.turnover_time,
                        turnover_time_derived=f.turnover_time_derived)
+    elif isinstance(f, api.StochasticOU):
+        g = _tagged(parent, name, "StochasticOU")
+        g.attrs.update(forcing_variance=f.forcing_variance, correlation_time=f.correlation_time, seed=f.seed)
     else:
         raise ValueError(f"write_forcing: no schema for {f!r}")
     g.attrs.update(lowest_shell=f.lowest_shell, highest_shell=f.highest_shell)
@@ -183,6 +186,9 @@ def read_forcing(parent: Group, name: str, domain):
         return api.ConstantPower(domain, injection_rate=a["injection_rate"], lowest_shell=lo, highest_shell=hi)
     if tag == "VarianceRescaling":
         return api.VarianceRescaling(domain, target=a["target"], over=a["over"], lowest_shell=lo, highest_shell=hi)
+    if tag == "StochasticOU":
+        return api.StochasticOU(domain, forcing_variance=a["forcing_variance"], correlation_time=a["correlation_time"],
+                                lowest_shell=lo, highest_shell=hi, seed=int(a.get("seed", 0)))
     if tag == "VarianceController":
         derived = bool(a.get("turnover_time_derived", False))
         return api.VarianceController(domain, target=a["target"], gain=a["gain"],
@@ -211,6 +217,14 @@ def write_simulation(path, sim, name: str = "simulation") -> None:
     st["velocity_spectral"] = fields.velocity_spectral
     if sim.carries_scalar:
         st["scalar_spectral"] = fields.scalar_spectral
+    # the forcing states (src/Forcing.jl StochasticOUState; ext/GBGHomogeneousTurbulenceHDF5Ext.jl:153-168):
+    # the process and its draw count, with which the stored seed fixes every later draw
+    for field, key in ((0, "modeled_velocity_forcing"), (1, "modeled_scalar_forcing")):
+        proc = sim.ou.get(field)
+        if proc is not None:
+            og = _tagged(st, key, "StochasticOUState")
+            og.attrs["draws"] = proc.draws
+            og["process"] = proc.process
     write_forcing(g, "velocity_forcing", c.velocity_forcing)
     write_forcing(g, "scalar_forcing", c.scalar_forcing)
     if sim.tracers is not None:
@@ -263,11 +277,22 @@ def read_simulation(path, name: str = "simulation", **simulation_kwargs):
         trail = g.group("velocity_at_particles_trail", "read_simulation")
         _require_tag(trail, "read_simulation", "ParticleTrail")
     cond = api.FlowConditions(viscosity=a["viscosity"], schmidt=a["schmidt"], velocity_forcing=vf, scalar_forcing=sf)
+    stored_ou = {}
+    for field, key in ((0, "modeled_velocity_forcing"), (1, "modeled_scalar_forcing")):
+        if key in sg:
+            og = sg.group(key, "read_simulation")
+            oa = _require_tag(og, "read_simulation", "StochasticOUState")
+            stored_ou[field] = (og["process"], int(oa["draws"]))
     sim = api.Simulation(domain, conditions=cond, state=state, field_integration_scheme=scheme, tracers=tracers,
                          particle_integration_scheme=ps, mean_gradient=float(a.get("mean_gradient", 0.0)),
                          cfl=float(a["cfl"]), max_dt=float(a["max_dt"]), shortened_first_steps=shortened,
                          **simulation_kwargs)
     try:
+        for field, (process, draws) in stored_ou.items():
+            if field not in sim.ou or sim.ou[field].process.shape != process.shape:
+                raise ValueError(f"read_simulation: group {name!r} stores a forcing state that does not fit its "
+                                 "stored forcing")
+            sim.ou[field] = api.OUProcess(sim.ou[field].forcing, sim.ou[field].order, process, draws)
         if "schedule" in a:
             sim.schedule(np.asarray(a["schedule"], dtype=np.int32))
             sim.redeposit()

@@ -183,6 +183,7 @@ void release(Ctx &c) {
   for (hipEvent_t e : c.prof_ev) (void)hipEventDestroy(e);
   c.prof_ev.clear();
   void *ptrs[] = {c.uh, c.hu, c.ph, c.hp, c.bufa, c.bufb, c.phys, c.physw, c.tw, c.partials, c.scalars, c.rpart,
+                  c.usrc, c.psrc,
                   c.maxbits, c.spec_bins, c.pos, c.vel, c.phis, c.order, c.tile_counts, c.tile_cursor,
                   c.visc_d, c.kappa_d};
   for (void *p : ptrs)
@@ -259,6 +260,28 @@ FinalizeArgs finalize_args(const Ctx &c) {
   fa.turnover_time = c.cfg.forcing_time;
   fa.visc = c.visc_d;
   return fa;
+}
+
+// modes in the cube |m| <= r (kx >= 0) of a StochasticOU band with highest shell r
+size_t ou_cube(int r) { return (size_t)(2 * r + 1) * (2 * r + 1) * (r + 1); }
+
+// StochasticOU's band force added after the stage update of each forced field.
+void launch_band_source(Ctx &c, const StageArgs &sa) {
+  for (int f = 0; f < 2; ++f) {
+    void *src = f == 0 ? c.usrc : c.psrc;
+    if (!src) continue;
+    const int r = f == 0 ? c.cfg.highest_shell : c.cfg.scalar_highest_shell;
+    const dim3 grid((unsigned)((ou_cube(r) + 255) / 256), 1, c.nbox);
+    const BandSpec &band = f == 0 ? c.band : c.sband;
+    if (c.fp64)
+      hipLaunchKernelGGL(k_band_source<double>, grid, dim3(256), 0, c.stream, (cx<double> *)(f == 0 ? c.uh : c.ph),
+                         (cx<double> *)(f == 0 ? c.hu : c.hp), f == 0 ? 3 : 1, c.geo, band, r,
+                         (const cx<double> *)src, sa, f);
+    else
+      hipLaunchKernelGGL(k_band_source<float>, grid, dim3(256), 0, c.stream, (cx<float> *)(f == 0 ? c.uh : c.ph),
+                         (cx<float> *)(f == 0 ? c.hu : c.hp), f == 0 ? 3 : 1, c.geo, band, r,
+                         (const cx<double> *)src, sa, f);
+  }
 }
 
 BandArgs band_args(const Ctx &c, int mode, double factor, int slot) {
@@ -392,9 +415,23 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
     return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
                 "`schmidt` must be positive: the scalar diffusivity is nu/Sc, so a non-positive Schmidt number makes "
                 "it infinite or negative");
-  if (k.velocity_forcing < HDNS_NO_FORCING || k.velocity_forcing > HDNS_VARIANCE_RESCALING)
+  if (k.velocity_forcing < HDNS_NO_FORCING || k.velocity_forcing > HDNS_STOCHASTIC_OU)
     return fail(nullptr, HDNS_ERR_UNSUPPORTED,
-                "velocity_forcing must be NoForcing, ConstantPower, Linear, VarianceController or VarianceRescaling");
+                "velocity_forcing must be NoForcing, ConstantPower, Linear, VarianceController, VarianceRescaling or "
+                "StochasticOU");
+  // src/Forcing.jl:1315-1333: an additive force needs a finite band below the dealias cutoff
+  for (int f = 0; f < 2; ++f) {
+    const bool ou = (f == 0 ? k.velocity_forcing : k.scalar_forcing) == HDNS_STOCHASTIC_OU;
+    const int hi = f == 0 ? k.highest_shell : k.scalar_highest_shell;
+    if (ou && hi < 0)
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                  "StochasticOU needs a finite `highest_shell`: it adds its process rather than multiplying the "
+                  "field, so its band must stay below the dealias cutoff");
+    if (ou && 9 * (2 * hi + 1) * (2 * hi + 1) > 4 * k.grid_points * k.grid_points)
+      return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
+                  "the StochasticOU band reaches past the dealias cutoff; an additive force would fill modes the "
+                  "dealiasing holds empty. Lower the shells, or enlarge the grid.");
+  }
   if (k.velocity_forcing != HDNS_NO_FORCING) {  // src/Forcing.jl:388-409
     if (k.lowest_shell < 1)
       return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
@@ -420,9 +457,10 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
     if (!(isfinite(k.forcing_time) && k.forcing_time > 0))
       return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "`turnover_time` must be finite and positive");
   }
-  if (k.scalar_forcing < HDNS_NO_FORCING || k.scalar_forcing > HDNS_VARIANCE_RESCALING)
+  if (k.scalar_forcing < HDNS_NO_FORCING || k.scalar_forcing > HDNS_STOCHASTIC_OU)
     return fail(nullptr, HDNS_ERR_UNSUPPORTED,
-                "scalar_forcing must be NoForcing, ConstantPower, Linear, VarianceController or VarianceRescaling");
+                "scalar_forcing must be NoForcing, ConstantPower, Linear, VarianceController, VarianceRescaling or "
+                "StochasticOU");
   if (k.scalar_forcing != HDNS_NO_FORCING) {
     if (!k.has_scalar)
       return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "a scalar forcing needs a run that carries the scalar");
@@ -549,6 +587,16 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.scalars, B * 8 * sizeof(double));
   if (s == HDNS_OK && (k.velocity_forcing == HDNS_VARIANCE_RESCALING || c.sband.on))
     s = alloc(ctx, (void **)&c.rpart, B * RESCALE_BLOCKS * 2 * sizeof(double));
+  for (int f = 0; f < 2 && s == HDNS_OK; ++f) {   // StochasticOU: the band force of the step, zero until uploaded
+    if ((f == 0 ? k.velocity_forcing : (k.has_scalar ? k.scalar_forcing : HDNS_NO_FORCING)) != HDNS_STOCHASTIC_OU) continue;
+    const size_t bytes = B * (f == 0 ? 3 : 1) * ou_cube(f == 0 ? k.highest_shell : k.scalar_highest_shell) * sizeof(cx<double>);
+    void *&dst = f == 0 ? c.usrc : c.psrc;
+    s = alloc(ctx, &dst, bytes);
+    if (s == HDNS_OK) {
+      const hipError_t z = hipMemset(dst, 0, bytes);
+      if (z != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, std::string("clearing the band force: ") + hipGetErrorString(z));
+    }
+  }
   if (s == HDNS_OK) {   // the amplitudes and factors start at zero (a Linear scalar forcing reads A_phi = 0)
     const hipError_t z = hipMemset(c.scalars, 0, B * 8 * sizeof(double));
     if (z != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, std::string("clearing the scalars: ") + hipGetErrorString(z));
@@ -870,6 +918,7 @@ void launch_step(Ctx &c, double dt, const StageRecord *rec = nullptr) {
     }
     TIMED(c, K_YFWD, c.pipe->yfwd(c));
     TIMED(c, K_ZF, c.pipe->zf(c, sa));
+    launch_band_source(c, sa);
     if (last) TIMED(c, K_REALITY, c.pipe->reality(c));
     if (last && c.cfg.velocity_forcing == HDNS_VARIANCE_RESCALING) launch_rescale(c, false);
     if (last && c.sband.on && c.cfg.scalar_forcing == HDNS_VARIANCE_RESCALING) launch_rescale(c, true);
@@ -1313,6 +1362,22 @@ hdns_status hdns_tracer_trail_restore(hdns_ctx *ctx, int32_t n, const double *ti
   c.advected = n > 0;
   c.pending_t = t_next;
   if (n == 0) c.trail_filled = 0;   // the next deposit starts the trail, at t_next
+  return HDNS_OK;
+}
+
+hdns_status hdns_band_source_upload(hdns_ctx *ctx, int32_t field, const double *source) {
+  if (!ctx || !source) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "source must not be NULL");
+  Ctx &c = ctx->c;
+  if (field != 0 && field != 1) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "field is 0 (velocity) or 1 (scalar)");
+  void *dst = field == 0 ? c.usrc : c.psrc;
+  if (!dst)
+    return fail(ctx, HDNS_ERR_STATE, std::string("the ") + (field == 0 ? "velocity" : "scalar") +
+                                         " is not driven by a StochasticOU forcing");
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  const int nc = field == 0 ? 3 : 1;
+  const size_t bytes = nc * ou_cube(field == 0 ? c.cfg.highest_shell : c.cfg.scalar_highest_shell) * sizeof(cx<double>);
+  HIP_TRY(ctx, hipMemcpyAsync((char *)dst + (size_t)c.box * bytes, source, bytes, hipMemcpyHostToDevice, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
   return HDNS_OK;
 }
 

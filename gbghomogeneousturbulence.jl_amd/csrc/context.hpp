@@ -95,6 +95,7 @@ struct Ctx {
   double *partials = nullptr;
   double *scalars = nullptr;   // per box [8]: A, the five reductions, the rescale factor - 1, spare
   double *rpart = nullptr;     // VarianceRescaling's block sums
+  void *usrc = nullptr, *psrc = nullptr;   // StochasticOU's band force of the step, u and phi (FP64)
   unsigned long long *maxbits = nullptr;
   double *spec_bins = nullptr;
   int nbins = 0;

@@ -154,3 +154,48 @@ __global__ __launch_bounds__(256) void k_rescale_apply(cx<T> *__restrict__ f, in
 }
 
 }  // namespace hdns
+
+namespace hdns {
+
+// StochasticOU's additive force f on the band (src/Forcing.jl:1404-1405), held over the step and
+// added at every stage.  The stage update is linear in its source, so the force is added to what
+// the update just wrote, with the stage's own integrating factor E, evaluated as the update does:
+//   2N schemes:         q += E (b (dt f)),  and h += E (dt f) except at the last stage;
+//   Shu-Osher schemes:  q += E^pb (cb (dt f)).
+// f is transverse (velocity) and a real field's spectrum, so the state projections that follow
+// the last stage leave it as it is.  src: [box][ncomp][2r+1][2r+1][r+1], FP64.
+template <typename T>
+__global__ __launch_bounds__(256) void k_band_source(cx<T> *__restrict__ q, cx<T> *__restrict__ h, int ncomp, Geo g,
+                                                     BandSpec band, int r, const cx<double> *__restrict__ src,
+                                                     StageArgs sa, int scalar) {
+  const int box = blockIdx.z;
+  const size_t l = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const int side = 2 * r + 1;
+  const size_t cube = (size_t)side * side * (r + 1);
+  if (l >= cube) return;
+  const int mx = (int)(l % (r + 1));
+  const int my = (int)((l / (r + 1)) % side) - r, mz = (int)(l / ((size_t)(r + 1) * side)) - r;
+  if (!in_band(mx * mx + my * my + mz * mz, band)) return;
+  const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
+  const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
+  const double xe = -(scalar ? sa.kappa[box] : sa.visc[box]) * sa.sub * k2d;
+  const T dt = T(sa.dt);
+  const int ky_idx = my < 0 ? my + g.k : my, kz_idx = mz < 0 ? mz + g.k : mz;
+  const size_t comp = comp_of(g), at = ((size_t)kz_idx * g.k + ky_idx) * g.kxp + mx;
+  q += (size_t)box * ncomp * comp + at;
+  h += (size_t)box * ncomp * comp + at;
+  src += (size_t)box * ncomp * cube + l;
+  for (int c = 0; c < ncomp; ++c) {
+    const cx<double> fd = src[c * cube];
+    const cx<T> df = scale(cx<T>{T(fd.re), T(fd.im)}, dt);
+    if (sa.ssp) {
+      q[c * comp] = q[c * comp] + scale(scale(df, T(sa.cb)), T(exp(sa.pb * xe)));
+    } else {
+      const T E = T(exp(xe));
+      q[c * comp] = q[c * comp] + scale(scale(df, T(sa.b)), E);
+      if (!sa.last) h[c * comp] = h[c * comp] + scale(df, E);
+    }
+  }
+}
+
+}  // namespace hdns

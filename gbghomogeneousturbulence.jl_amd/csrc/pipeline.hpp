@@ -218,9 +218,10 @@ struct PipelineImpl final : Pipeline {
     }
   }
   void zf(Ctx &c, const StageArgs &sa) override {
-    // the scalar forcings acting inside the stage (VarianceRescaling acts after the step)
+    // the scalar forcings acting inside the stage update (VarianceRescaling acts after the step,
+    // StochasticOU's force is added after the update)
     const bool sf = c.cfg.has_scalar && c.cfg.scalar_forcing != HDNS_NO_FORCING &&
-                    c.cfg.scalar_forcing != HDNS_VARIANCE_RESCALING;
+                    c.cfg.scalar_forcing != HDNS_VARIANCE_RESCALING && c.cfg.scalar_forcing != HDNS_STOCHASTIC_OU;
     if (sa.ssp) sf ? zf_v<0, true, true>(c, sa) : zf_v<0, true>(c, sa);
     else if (sf) zf_v<0, false, true>(c, sa);
     else c.sched[1] ? zf_v<1>(c, sa) : zf_v<0>(c, sa);

@@ -36,10 +36,10 @@ enum { HDNS_FP32_BULK = 0, HDNS_FP64 = 1 };
 /* IFLowStorageRK3, IFLowStorageRK4 (src/Propagator.jl:264-320), IFSSPRK3 (:596-691) and IFHeun (:420-544) */
 enum { HDNS_IF_LOW_STORAGE_RK3 = 3, HDNS_IF_LOW_STORAGE_RK4 = 4, HDNS_IF_SSP_RK3 = 5, HDNS_IF_HEUN = 6 };
 enum { HDNS_TWO_THIRDS = 0 };
-/* NoForcing, ConstantPower, Linear, VarianceController and VarianceRescaling (src/Forcing.jl:464,
- * :714-719, :600-642, :817-871, :566-597) */
+/* NoForcing, ConstantPower, Linear, VarianceController, VarianceRescaling and StochasticOU
+ * (src/Forcing.jl:464, :714-719, :600-642, :817-871, :566-597, :1120-1413) */
 enum { HDNS_NO_FORCING = 0, HDNS_CONSTANT_POWER = 1, HDNS_LINEAR = 2, HDNS_VARIANCE_CONTROLLER = 3,
-       HDNS_VARIANCE_RESCALING = 4 };
+       HDNS_VARIANCE_RESCALING = 4, HDNS_STOCHASTIC_OU = 5 };
 /* VarianceRescaling's `over`: hold the band's kinetic energy, or the whole field's */
 enum { HDNS_OVER_BAND = 0, HDNS_OVER_TOTAL = 1 };
 enum { HDNS_CATMULL_ROM = 0 };
@@ -179,6 +179,13 @@ hdns_status hdns_tracer_trail_restore(hdns_ctx *ctx, int32_t n, const double *ti
                                       double t_next);
 /* The velocity sample `offset` steps back from the newest (offset <= 0), [3][M]: trail[offset]. */
 hdns_status hdns_tracer_trail_sample(hdns_ctx *ctx, int32_t offset, double *velocity);
+/* The additive band force of a StochasticOU forcing for the next steps (src/Forcing.jl:1404-1405,
+ * add_forcing_source!): field 0 (velocity, [3][2r+1][2r+1][r+1]) or 1 (scalar, [2r+1][2r+1][r+1])
+ * of the selected box, complex, in the unnormalised spectral convention of the run, with r the
+ * forcing's highest shell: entry [c][mz + r][my + r][mx] is mode (mx, my, mz).  Modes outside the
+ * band are ignored.  The force is held over a step and added at every stage; the host advances the
+ * Ornstein-Uhlenbeck process once per step and uploads it before the step. */
+hdns_status hdns_band_source_upload(hdns_ctx *ctx, int32_t field, const double *source);
 /* time_derivative!(out, trail; order) (src/Particles.jl:597-631): d^m v/dt^m at the newest
  * sample's time of the polynomial through every sample held, Fornberg weights from the times;
  * order 1 is the Lagrangian acceleration.  out [3][M]. */

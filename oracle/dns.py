@@ -202,6 +202,10 @@ class Conditions:
     #   ("constant_power", injection_rate, lo, hi), ("linear", rate, lo, hi),
     #   ("controller", target, gain, turnover_time, lo, hi), ("rescaling", target, over, lo, hi)
     scalar_forcing: tuple | None = None
+    # an additive force held over the step (StochasticOU's band force, src/Forcing.jl:1404-1405), in
+    # the unnormalised spectral convention: [3, N, N, N/2+1] for the velocity, [N, N, N/2+1] for phi
+    velocity_source: np.ndarray | None = None
+    scalar_source: np.ndarray | None = None
     band_eps: float = float(np.finfo(np.float64).eps)
 
     @property
@@ -337,6 +341,8 @@ def velocity_rhs(d: Domain, cond: Conditions, uh: np.ndarray, u: np.ndarray, w: 
     if cond.controller is not None:
         amp = controller_amplitude(d, cond, uh)
         nl = nl + amp * d.shell_band(cond.controller[3], cond.controller[4]) * uh
+    if cond.velocity_source is not None:
+        nl = nl + cond.velocity_source
     return nl
 
 
@@ -359,6 +365,8 @@ def scalar_rhs(d: Domain, cond: Conditions, uh: np.ndarray, u: np.ndarray, phi: 
     sf = cond.scalar_forcing
     if sf is not None and sf[0] in ("constant_power", "controller"):
         nl = nl + scalar_amplitude(d, cond, ph) * d.shell_band(sf[-2], sf[-1]) * ph
+    if cond.scalar_source is not None:
+        nl = nl + cond.scalar_source
     return nl
 
 

@@ -15,6 +15,7 @@ def _run(g, dtype, scheme, forcing, particles):
     n = 32
     dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
     vf = {"constant_power": lambda: g.ConstantPower(dom, injection_rate=0.1),
+          "stochastic_ou": lambda: g.StochasticOU(dom, forcing_variance=0.3, correlation_time=0.05, seed=17),
           "controller": lambda: g.VarianceController(dom, target=0.6, turnover_time=0.7),
           "rescaling": lambda: g.VarianceRescaling(dom, target=0.55, over="total")}[forcing]()
     cond = g.FlowConditions(viscosity=0.02, schmidt=0.9, velocity_forcing=vf)
@@ -36,6 +37,7 @@ def _snapshot(sim):
     (np.float32, "IFSSPRK3", "constant_power", ("AB", 3)),
     (np.float64, "IFLowStorageRK4", "controller", ("ABM", 2)),
     (np.float32, "IFLowStorageRK3", "rescaling", ("ABM", 1)),
+    (np.float32, "IFLowStorageRK4", "stochastic_ou", ("AB", 2)),
 ])
 def test_a_restored_run_continues_bit_for_bit(g, tmp_path, dtype, scheme, forcing, particles):
     from gbg_amd import checkpoint as ck

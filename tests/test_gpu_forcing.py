@@ -234,3 +234,52 @@ def test_a_scalar_forcing_needs_the_scalar_and_the_adjoint_refuses_it(g):
                                                                    g.random_phase_scalar(n, seed=2))) as sim:
         with pytest.raises(Exception, match="scalar forcing"):
             sim.adjoint_gradient(1e-3, 2)
+
+
+# --- StochasticOU ------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("field", ["velocity", "scalar"])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 1e-5)])
+@pytest.mark.parametrize("scheme", ["rk4", "ssprk3"])
+def test_stochastic_ou_matches_the_oracle_on_its_own_draws(g, field, dtype, tol, scheme):
+    """The band force the host's process holds each step (src/Forcing.jl:1395-1405), added at every
+    stage on the device, against the oracle given the same force: 10 steps of uneven dt."""
+    n = 32
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
+    ou = g.StochasticOU(dom, forcing_variance=0.5, correlation_time=0.05, lowest_shell=1, highest_shell=2, seed=99)
+    kw = {"velocity_forcing": ou} if field == "velocity" else {"scalar_forcing": ou}
+    cond = g.FlowConditions(viscosity=0.02, schmidt=0.8, **kw)
+    u, p = g.random_phase_velocity(n, seed=91), g.random_phase_scalar(n, seed=92)
+    d = dns.Domain(n)
+    uo, po, dep = u, p, None
+    sch = {"rk4": g.IFLowStorageRK4, "ssprk3": g.IFSSPRK3}[scheme]
+    with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=0.5,
+                      field_integration_scheme=sch) as sim:
+        for s in range(1, 11):
+            dt = 2e-3 * (1 + 0.3 * math.sin(s))
+            sim.advance(dt)
+            src = sim.ou[0 if field == "velocity" else 1].full_source(n)
+            okw = {"velocity_source": src} if field == "velocity" else {"scalar_source": src}
+            oc = dns.Conditions(viscosity=0.02, schmidt=0.8, mean_gradient=0.5, band_eps=float(np.finfo(dtype).eps), **okw)
+            uo, po, dep = dns.step(d, oc, scheme, uo, po, dt, dep)
+            if s in (1, 10):
+                st = sim.state()
+                eu, ep = rel(st.velocity_spectral, uo), rel(st.scalar_spectral, po)
+                print(f"ou {field} {scheme} {np.dtype(dtype).name} step {s}: u {eu:.2e} phi {ep:.2e}", flush=True)
+                assert eu < tol and ep < tol
+
+
+def test_stochastic_ou_injects_its_force_into_a_fluid_at_rest(g):
+    """From rest, with nu = 0 in the band's short window, the first step moves the band modes by
+    dt f to first order and by nothing else: the device's increment over one small step, divided
+    by dt, is the uploaded force (to O(dt))."""
+    n, dt = 32, 1e-6
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
+    ou = g.StochasticOU(dom, forcing_variance=1.0, correlation_time=10.0, seed=5)
+    zero = np.zeros((3, n, n, n // 2 + 1), dtype=np.complex128)
+    with g.Simulation(dom, conditions=g.FlowConditions(viscosity=1e-3, schmidt=1.0, velocity_forcing=ou),
+                      state=g.ResolvedFields(zero, None)) as sim:
+        sim.advance(dt)
+        got = sim.state().velocity_spectral / dt
+        want = sim.ou[0].full_source(n)
+    assert rel(got, want) < 1e-4

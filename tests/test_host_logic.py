@@ -204,3 +204,58 @@ def test_checkpoint_store_round_trips_groups_attributes_and_datasets(g, tmp_path
     with pytest.raises(ValueError, match="newer"):
         top2.attrs["version"] = 99
         ck._require_tag(top2, "t", "Simulation")
+
+
+def _ou(g, order, **kw):
+    from gbg_amd.stochastic import OUProcess
+    dom = g.FixedDomain(grid_points_per_axis=32, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
+    f = g.StochasticOU(dom, forcing_variance=kw.get("var", 0.3), correlation_time=kw.get("tl", 0.5),
+                       lowest_shell=1, highest_shell=kw.get("hi", 2), seed=kw.get("seed", 12345))
+    return f, OUProcess(f, order)
+
+
+def test_stochastic_ou_draws_are_transverse_real_band_limited_and_keyed(g):
+    """src/Forcing.jl:1360-1385: each increment is a real field's spectrum cut to the band and, for
+    the velocity, transverse to kappa; the stream is a pure function of (seed, draw, field)."""
+    f, p = _ou(g, 1)
+    nc = f.companion_points
+    xi = p.increment(7)
+    m = np.fft.fftfreq(nc, 1.0 / nc)
+    kz, ky, kx = m[:, None, None], m[None, :, None], np.arange(nc // 2 + 1)[None, None, :]
+    assert np.max(np.abs(kx * xi[0] + ky * xi[1] + kz * xi[2])) < 1e-12 * np.max(np.abs(xi))
+    assert np.all(xi[:, ~f.band] == 0)
+    plane = xi[..., 0]                                       # kx = 0: xi(-k) = conj xi(k)
+    neg = (-np.arange(nc)) % nc
+    assert np.max(np.abs(plane - np.conj(plane[:, neg][:, :, neg]))) < 1e-12
+    assert np.array_equal(p.increment(7), xi)
+    assert not np.array_equal(p.increment(8), xi)
+    _, s = _ou(g, 0)
+    assert not np.allclose(s.increment(7), xi[0])           # the scalar draws its own stream
+
+
+def test_stochastic_ou_transition_is_exact_and_stationary(g):
+    """a <- alpha a + sqrt(sigma^2 (1 - alpha^2)) xi exactly, and the physical mean square of the
+    force averages sigma^2 over the stream (2000 draws, 3 % bar)."""
+    f, p = _ou(g, 1, var=0.3, tl=0.5)
+    before, draws = p.process.copy(), p.draws
+    p.advance(0.05)
+    alpha = math.exp(-0.05 / 0.5)
+    want = alpha * before + math.sqrt(0.3 * (1 - alpha ** 2)) * p.increment(draws)
+    assert np.max(np.abs(p.process - want)) <= 1e-14 * np.max(np.abs(want))
+    nc = f.companion_points
+    w = np.full(nc // 2 + 1, 2.0)
+    w[0] = 1.0
+    ms = [float(np.sum(w * np.abs(p.increment(j)) ** 2)) / nc ** 6 for j in range(2000)]
+    assert np.mean(ms) == pytest.approx(1.0, rel=0.03)
+
+
+def test_stochastic_ou_guards(g):
+    dom = g.FixedDomain(grid_points_per_axis=16, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
+    with pytest.raises(ValueError, match="finite `highest_shell`"):
+        g.StochasticOU(dom, forcing_variance=0.1, correlation_time=0.5, highest_shell=math.inf)
+    with pytest.raises(ValueError, match="dealias cutoff"):
+        g.StochasticOU(dom, forcing_variance=0.1, correlation_time=0.5, highest_shell=6)
+    with pytest.raises(ValueError, match="correlation_time"):
+        g.StochasticOU(dom, forcing_variance=0.1, correlation_time=0.0)
+    with pytest.raises(ValueError, match="forcing_variance"):
+        g.StochasticOU(dom, forcing_variance=-1.0, correlation_time=0.5)
