@@ -9,7 +9,8 @@ from .api import (AB, ABM, ConstantPower, Ensemble, EnsembleBox, EnsembleMember,
                   IFHeun, IFLowStorageRK3, IFLowStorageRK4, IFSSPRK3, Linear, NoForcing, VarianceController,
                   VarianceRescaling, ResolvedFields, Simulation, Tracers, TwoThirds,
                   StationarityCriterion, cfl_bound, integral_length, predicted_viscosity,
-                  recommended_cfl, truncate_to_band, warmup)
+                  recommended_cfl, truncate_to_band, warmup, reynolds_box, reynolds_taylor, taylor_length,
+                  kolmogorov_length, kolmogorov_time, kolmogorov_velocity)
 from .seeding import random_phase_scalar, random_phase_velocity, uniform_positions
 from .stochastic import StochasticOU
 
@@ -18,5 +19,6 @@ __all__ = [
     "IFHeun", "IFLowStorageRK3", "IFLowStorageRK4", "IFSSPRK3", "Linear", "NoForcing", "VarianceController",
     "VarianceRescaling", "ResolvedFields", "Simulation", "Tracers", "TwoThirds", "predicted_viscosity",
     "recommended_cfl", "cfl_bound", "integral_length", "StationarityCriterion", "warmup", "truncate_to_band", "random_phase_scalar", "random_phase_velocity",
-    "uniform_positions", "StochasticOU",
+    "uniform_positions", "StochasticOU", "reynolds_box", "reynolds_taylor", "taylor_length", "kolmogorov_length",
+    "kolmogorov_time", "kolmogorov_velocity",
 ]

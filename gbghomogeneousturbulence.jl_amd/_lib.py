@@ -79,6 +79,7 @@ _SIGS = {
     "hdns_tracer_trail_sample": [c_void_p, c_int32, _D],
     "hdns_tracer_trail_restore": [c_void_p, c_int32, _D, _D, c_double],
     "hdns_band_source_upload": [c_void_p, c_int32, _D],
+    "hdns_below_scale": [c_void_p, _D, _D, _D],
     "hdns_tracer_time_derivative": [c_void_p, c_int32, _D],
     "hdns_tracer_cells": [c_void_p, POINTER(c_int64)],
     "hdns_traversal_order": [c_void_p, POINTER(c_int32)],

@@ -300,6 +300,51 @@ def recommended_cfl(scheme, domain: FixedDomain) -> float:
     return scheme.base_cfl
 
 
+# --- derived scales of (k, eps, nu) (src/Observables.jl:611-760) ---------------------------------
+
+def _pos(name: str, v: float, allow_zero: bool = False) -> float:
+    v = float(v)
+    if not (math.isfinite(v) and (v >= 0 if allow_zero else v > 0)):
+        raise ValueError(f"`{name}` must be finite and {'non-negative' if allow_zero else 'positive'}, got {v}")
+    return v
+
+
+def reynolds_box(side_length: float, dissipation: float, viscosity: float) -> float:
+    """Re_box = eps^(1/3) L^(4/3) / nu (src/Observables.jl:611-617)."""
+    L, e, nu = _pos("side_length", side_length), _pos("dissipation", dissipation, True), _pos("viscosity", viscosity)
+    return e ** (1 / 3) * L ** (4 / 3) / nu
+
+
+def reynolds_taylor(kinetic_energy: float, dissipation: float, viscosity: float) -> float:
+    """Re_lambda = sqrt(20/3) k / sqrt(eps nu) (src/Observables.jl:647-653)."""
+    k, e, nu = _pos("kinetic_energy", kinetic_energy, True), _pos("dissipation", dissipation), _pos("viscosity", viscosity)
+    return math.sqrt(20 / 3) * k / math.sqrt(e * nu)
+
+
+def taylor_length(kinetic_energy: float, dissipation: float, viscosity: float) -> float:
+    """lambda = sqrt(10 nu k / eps) (src/Observables.jl:682-688)."""
+    k, e, nu = _pos("kinetic_energy", kinetic_energy, True), _pos("dissipation", dissipation), _pos("viscosity", viscosity)
+    return math.sqrt(10 * nu * k / e)
+
+
+def kolmogorov_length(dissipation: float, viscosity: float) -> float:
+    """eta = (nu^3 / eps)^(1/4) (src/Observables.jl:704-709)."""
+    e, nu = _pos("dissipation", dissipation), _pos("viscosity", viscosity)
+    return math.sqrt(math.sqrt(nu ** 3 / e))
+
+
+def kolmogorov_time(dissipation: float, viscosity: float) -> float:
+    """tau_eta = sqrt(nu / eps) (src/Observables.jl:724-729)."""
+    e, nu = _pos("dissipation", dissipation), _pos("viscosity", viscosity)
+    return math.sqrt(nu / e)
+
+
+def kolmogorov_velocity(dissipation: float, viscosity: float) -> float:
+    """u_eta = (eps nu)^(1/4) (src/Observables.jl:745-750)."""
+    e, nu = _pos("dissipation", dissipation), _pos("viscosity", viscosity)
+    return math.sqrt(math.sqrt(e * nu))
+
+
 def integral_length(spectrum) -> float:
     """L11 = (3 pi / 4) sum_{kappa > 0} E / kappa / sum_{kappa > 0} E of a measured spectrum
     (kappa, E) (src/Observables.jl:567-581); no energy off the kappa = 0 shell is a ValueError."""
@@ -640,6 +685,15 @@ class Simulation:
         out = np.zeros(1)
         self._call("hdns_derivative_skewness", L.dptr(out))
         return float(out[0])
+
+    def kinetic_energy_and_dissipation_below_scale(self) -> dict:
+        """(scales, kinetic_energy, dissipation) below every side l = n L / N of a top-hat box
+        (src/Observables.jl:195-264): k(l) = k - (1/2) <|<u>_l|^2>, eps(l) = eps - nu <|<w>_l|^2>, with
+        the box mean's transfer W_l = prod_a sinc(k_a l / 2).  O(N^4) work: an occasional diagnostic."""
+        n = self.domain.grid_points_per_axis
+        sc, k, e = np.zeros(n), np.zeros(n), np.zeros(n)
+        self._call("hdns_below_scale", L.dptr(sc), L.dptr(k), L.dptr(e))
+        return {"scales": sc, "kinetic_energy": k, "dissipation": e}
 
     def integral_length(self) -> float:
         """L11 of the current state's energy spectrum (src/Observables.jl:556-557)."""

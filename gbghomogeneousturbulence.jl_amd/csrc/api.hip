@@ -13,6 +13,7 @@
 #include "compare.hpp"
 #include "context.hpp"
 #include "forcing.hpp"
+#include "observe.hpp"
 
 using namespace hdns;
 
@@ -1127,6 +1128,61 @@ hdns_status hdns_derivative_skewness(hdns_ctx *ctx, double *out) {
   *out = 3.0 * C_ISO * mean_det / pow(ss, 1.5);
   // the strain passes used the pencil buffers, which hold the next step's products
   return run_deposit(ctx);
+}
+
+hdns_status hdns_below_scale(hdns_ctx *ctx, double *scales, double *kinetic_energy, double *dissipation) {
+  if (!ctx || !scales || !kinetic_energy || !dissipation)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "scales, kinetic_energy and dissipation must not be NULL");
+  Ctx &c = ctx->c;
+  if (!c.deposited) return fail(ctx, HDNS_ERR_STATE, "no deposit of the current state");
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  const int n = c.geo.n, m = c.geo.m;
+  double s[8];
+  HIP_TRY(ctx, hipMemcpyAsync(s, c.scalars + 8 * c.box, sizeof s, hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  // sinc^2(pi |m| j / N) per side j = 1..N: the box mean's per-axis transfer at k_a l / 2
+  std::vector<double> t((size_t)n * (m + 1));
+  for (int j = 1; j <= n; ++j)
+    for (int a = 0; a <= m; ++a) {
+      const double x = M_PI * a * j / n;
+      const double sc = a == 0 ? 1.0 : sin(x) / x;
+      t[(size_t)(j - 1) * (m + 1) + a] = sc * sc;
+    }
+  const int groups = (n + BS_SIDES - 1) / BS_SIDES;
+  double *dt = nullptr, *dp = nullptr;
+  HIP_TRY(ctx, hipMalloc((void **)&dt, t.size() * sizeof(double)));
+  hipError_t e = hipMalloc((void **)&dp, (size_t)n * BS_BLOCKS * 2 * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpyAsync(dt, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, c.stream);
+  if (e == hipSuccess) {
+    const dim3 grid(BS_BLOCKS, groups);
+    const size_t off = (size_t)c.box * 3 * c.comp;
+    if (c.fp64)
+      hipLaunchKernelGGL(k_below_scale<double>, grid, dim3(256), 0, c.stream, (const cx<double> *)c.uh + off, c.geo,
+                         (const double *)dt, n, dp);
+    else
+      hipLaunchKernelGGL(k_below_scale<float>, grid, dim3(256), 0, c.stream, (const cx<float> *)c.uh + off, c.geo,
+                         (const double *)dt, n, dp);
+    e = hipGetLastError();
+  }
+  std::vector<double> host((size_t)n * BS_BLOCKS * 2);
+  if (e == hipSuccess) e = hipMemcpyAsync(host.data(), dp, host.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+  (void)hipFree(dt);
+  (void)hipFree(dp);
+  if (e != hipSuccess) return fail(ctx, HDNS_ERR_DEVICE, std::string("below-scale sums: ") + hipGetErrorString(e));
+  const double inv = 1.0 / pow((double)n, 6.0), nu = c.visc[c.box];
+  std::vector<double> col(BS_BLOCKS);
+  for (int j = 0; j < n; ++j) {
+    double r[2];
+    for (int q = 0; q < 2; ++q) {
+      for (int b = 0; b < BS_BLOCKS; ++b) col[b] = host[((size_t)j * BS_BLOCKS + b) * 2 + q];
+      r[q] = neumaier(col.data(), col.size()) * inv;
+    }
+    scales[j] = (j + 1) * c.cfg.side_length / n;
+    kinetic_energy[j] = (s[1] - r[0]) / 2;   // k(l) = (whole - retained) / 2
+    dissipation[j] = nu * (s[3] - r[1]);      // eps(l) = nu (dirichlet - retained)
+  }
+  return HDNS_OK;
 }
 
 hdns_status hdns_spectrum(hdns_ctx *ctx, int32_t which, double *out, int32_t nshells) {

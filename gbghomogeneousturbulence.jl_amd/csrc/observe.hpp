@@ -110,3 +110,69 @@ __global__ __launch_bounds__(XT) void k_xdet(const cx<T> *__restrict__ rin, Geo 
 }
 
 }  // namespace hdns
+
+namespace hdns {
+
+// kinetic_energy_and_dissipation_below_scale (src/Observables.jl:195-264): for every side
+// l = n L / N, the retained sums sum w W_l^2 |u|^2 and sum w |k|^2 W_l^2 |u|^2 of the box-mean
+// window W_l(k) = prod_a sinc(k_a l / 2).  t[(n - 1) (M + 1) + |m|] = sinc^2(pi |m| n / N) is the
+// per-axis factor; each block takes BS_SIDES sides, so one read of a mode serves all of them.
+// partials: [side][blocks][2], FP64.
+constexpr int BS_SIDES = 8;
+constexpr int BS_BLOCKS = 128;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_below_scale(const cx<T> *__restrict__ uh, Geo g, const double *__restrict__ t,
+                                                     int nsides, double *__restrict__ partials) {
+  __shared__ double red[4][BS_SIDES][2];
+  const size_t comp = comp_of(g);
+  const int s0 = blockIdx.y * BS_SIDES;
+  const size_t modes = (size_t)g.k * g.k * (g.m + 1);
+  double acc[BS_SIDES][2];
+#pragma unroll
+  for (int i = 0; i < BS_SIDES; ++i) acc[i][0] = acc[i][1] = 0.0;
+  for (size_t l = blockIdx.x * (size_t)blockDim.x + threadIdx.x; l < modes; l += (size_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(l % (g.m + 1));
+    const size_t row = l / (g.m + 1);
+    const int my = signed_of((int)(row % g.k), g), mz = signed_of((int)(row / g.k), g);
+    const size_t at = row * g.kxp + kx;
+    double e = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const cx<T> v = uh[c * comp + at];
+      e += (double)v.re * (double)v.re + (double)v.im * (double)v.im;
+    }
+    if (e == 0.0) continue;
+    const double w = kx == 0 ? 1.0 : 2.0;
+    const double kxd = g.k0 * kx, kyd = g.k0 * my, kzd = g.k0 * mz;
+    const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
+    const int ay = my < 0 ? -my : my, az = mz < 0 ? -mz : mz;
+#pragma unroll
+    for (int i = 0; i < BS_SIDES; ++i) {
+      if (s0 + i >= nsides) break;
+      const double *ts = t + (size_t)(s0 + i) * (g.m + 1);
+      const double w2 = ts[kx] * ts[ay] * ts[az];
+      acc[i][0] += w * w2 * e;
+      acc[i][1] += w * k2d * w2 * e;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < BS_SIDES; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      double v = acc[i][j];
+      for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+      if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][i][j] = v;
+    }
+  __syncthreads();
+  if (threadIdx.x < 2 * BS_SIDES) {
+    const int i = threadIdx.x >> 1, j = threadIdx.x & 1;
+    if (s0 + i < nsides) {
+      double v = 0.0;
+      for (int wv = 0; wv < 4; ++wv) v += red[wv][i][j];
+      partials[((size_t)(s0 + i) * gridDim.x + blockIdx.x) * 2 + j] = v;
+    }
+  }
+}
+
+}  // namespace hdns

@@ -179,6 +179,12 @@ hdns_status hdns_tracer_trail_restore(hdns_ctx *ctx, int32_t n, const double *ti
                                       double t_next);
 /* The velocity sample `offset` steps back from the newest (offset <= 0), [3][M]: trail[offset]. */
 hdns_status hdns_tracer_trail_sample(hdns_ctx *ctx, int32_t offset, double *velocity);
+/* kinetic_energy_and_dissipation_below_scale (src/Observables.jl:195-264) of the selected box's
+ * current state: for every side l = n L / N (n = 1..N), scales[n-1] = l, the kinetic energy and the
+ * dissipation below l, k(l) = (k_all - sum w W_l^2 |u|^2) / 2 and eps(l) = nu (sum w |k|^2 |u|^2 -
+ * sum w |k|^2 W_l^2 |u|^2), W_l(k) = prod_a sinc(k_a l / 2), FP64 sums.  An occasional diagnostic:
+ * O(N^4) work. */
+hdns_status hdns_below_scale(hdns_ctx *ctx, double *scales, double *kinetic_energy, double *dissipation);
 /* The additive band force of a StochasticOU forcing for the next steps (src/Forcing.jl:1404-1405,
  * add_forcing_source!): field 0 (velocity, [3][2r+1][2r+1][r+1]) or 1 (scalar, [2r+1][2r+1][r+1])
  * of the selected box, complex, in the unnormalised spectral convention of the run, with r the

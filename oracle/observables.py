@@ -59,3 +59,22 @@ def integral_length(kappa: np.ndarray, energy: np.ndarray) -> float:
     if not total > 0:
         raise ValueError("this velocity field carries no energy, so it has no integral scale")
     return 3.0 * math.pi / 4.0 * float(np.sum(energy[live] / kappa[live])) / total
+
+
+def below_scale(d: dns.Domain, uh: np.ndarray, viscosity: float):
+    """kinetic_energy_and_dissipation_below_scale (src/Observables.jl:195-264): for l = n L / N,
+    k(l) = (sum w |u|^2 - sum w W_l^2 |u|^2) / (2 N^6) and eps(l) = nu (sum w |k|^2 |u|^2 -
+    sum w |k|^2 W_l^2 |u|^2) / N^6, W_l(k) = prod_a sinc(k_a l / 2) (sin x / x)."""
+    whole = dns.weighted_inner(d, uh)
+    dirichlet = dns.weighted_inner(d, uh, d.k2)
+    scales, ks, es = [], [], []
+    for n in range(1, d.n + 1):
+        l = n * d.side_length / d.n
+        w2 = 1.0
+        for k in d.kvec:
+            x = k * l / 2
+            w2 = w2 * np.where(x == 0, 1.0, np.sin(x) / np.where(x == 0, 1.0, x)) ** 2
+        scales.append(l)
+        ks.append((whole - dns.weighted_inner(d, uh, w2)) / 2)
+        es.append(viscosity * (dirichlet - dns.weighted_inner(d, uh, d.k2 * w2)))
+    return np.array(scales), np.array(ks), np.array(es)

@@ -83,3 +83,21 @@ def test_warmup_spins_a_forced_run_up_to_a_flat_dissipation(g):
         assert any(s != 0 for s in m.skew)
         with pytest.raises(ValueError, match="must return a Bool"):
             sim.run(lambda s: 1, dt=1e-2)
+
+
+@pytest.mark.parametrize("n", [32, 48])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-12), (np.float32, 2e-5)])
+def test_below_scale_matches_the_oracle(g, n, dtype, tol):
+    """kinetic_energy_and_dissipation_below_scale on the device (k_below_scale) against the oracle,
+    every side l = L/N .. L; the bar is relative to k and eps (the small sides are differences)."""
+    from oracle import observables as obs
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=2 * np.pi, dealiasing=g.TwoThirds(), element_type=dtype)
+    u = g.random_phase_velocity(n, seed=17)
+    with g.Simulation(dom, conditions=g.FlowConditions(viscosity=0.01, schmidt=1.0), state=g.ResolvedFields(u, None)) as sim:
+        got = sim.kinetic_energy_and_dissipation_below_scale()
+        st = sim.state().velocity_spectral
+    d = dns.Domain(n)
+    sc, kl, el = obs.below_scale(d, st, 0.01)
+    assert np.allclose(got["scales"], sc, rtol=1e-15)
+    assert np.max(np.abs(got["kinetic_energy"] - kl)) <= tol * kl[-1]
+    assert np.max(np.abs(got["dissipation"] - el)) <= tol * el[-1]

@@ -259,3 +259,22 @@ def test_stochastic_ou_guards(g):
         g.StochasticOU(dom, forcing_variance=0.1, correlation_time=0.0)
     with pytest.raises(ValueError, match="forcing_variance"):
         g.StochasticOU(dom, forcing_variance=-1.0, correlation_time=0.5)
+
+
+def test_derived_scales_closed_forms_and_guards(g):
+    """src/Observables.jl:611-760: Re_box = eps^(1/3) L^(4/3) / nu, Re_lambda = sqrt(20/3) k /
+    sqrt(eps nu) = u' lambda / nu, lambda = sqrt(10 nu k / eps), eta = (nu^3/eps)^(1/4),
+    tau_eta = sqrt(nu/eps), u_eta = (eps nu)^(1/4), and eta = u_eta tau_eta, eta u_eta / nu = 1."""
+    k, eps, nu, Lb = 1.5, 0.1, 1e-3, 2 * math.pi
+    lam = g.taylor_length(k, eps, nu)
+    up = math.sqrt(2 * k / 3)
+    assert g.reynolds_taylor(k, eps, nu) == pytest.approx(up * lam / nu, rel=1e-14)
+    assert lam == pytest.approx(math.sqrt(15 * nu * up ** 2 / eps), rel=1e-14)
+    assert g.reynolds_box(Lb, eps, nu) == pytest.approx((eps * Lb ** 4) ** (1 / 3) / nu, rel=1e-14)
+    eta, tau, ue = g.kolmogorov_length(eps, nu), g.kolmogorov_time(eps, nu), g.kolmogorov_velocity(eps, nu)
+    assert eta == pytest.approx(ue * tau, rel=1e-14) and eta * ue / nu == pytest.approx(1.0, rel=1e-14)
+    with pytest.raises(ValueError, match="dissipation"):
+        g.kolmogorov_length(0.0, nu)
+    with pytest.raises(ValueError, match="viscosity"):
+        g.reynolds_taylor(k, eps, -1.0)
+    assert g.reynolds_box(Lb, 0.0, nu) == 0.0
