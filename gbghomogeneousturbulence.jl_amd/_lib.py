@@ -36,7 +36,7 @@ class HdnsConfig(ctypes.Structure):
         ("interpolation", c_int32), ("particle_scheme", c_int32),
         ("store_physical", c_int32), ("forcing_over", c_int32),
         ("forcing_target", c_double), ("forcing_gain", c_double), ("forcing_time", c_double),
-        ("particle_steps", c_int32), ("reserved", c_int32),
+        ("particle_steps", c_int32), ("unwrapped", c_int32),
         ("scalar_forcing", c_int32), ("scalar_lowest_shell", c_int32), ("scalar_highest_shell", c_int32),
         ("scalar_over", c_int32), ("scalar_rate", c_double), ("scalar_target", c_double),
         ("scalar_gain", c_double), ("scalar_time", c_double),
@@ -71,6 +71,8 @@ _SIGS = {
     "hdns_field_download": [c_void_p, c_int32, _D],
     "hdns_tracers_upload": [c_void_p, c_int64, _D],
     "hdns_tracers_download": [c_void_p, _D],
+    "hdns_tracers_unwrapped_download": [c_void_p, _D],
+    "hdns_tracers_unwrapped_upload": [c_void_p, c_int64, _D],
     "hdns_rebuild_traversal": [c_void_p],
     "hdns_advect": [c_void_p, c_double],
     "hdns_deposit_tracers": [c_void_p],

@@ -450,11 +450,20 @@ class ResolvedFields:
     scalar_spectral: np.ndarray | None = None
 
 
+def unwrapped_positions(run) -> np.ndarray:
+    """unwrapped_positions (src/Particles.jl:317-323) of a Simulation's or an EnsembleBox's tracers:
+    the trajectory with its winding kept, [3, M]; raises when the set was built without
+    ``unwrapped=True``."""
+    return run.unwrapped_positions()
+
+
 class Tracers:
-    """A tracer set (src/Particles.jl:135-176): positions [3, M] in [0, L)."""
+    """A tracer set (src/Particles.jl:135-176): positions [3, M] in [0, L).  With ``unwrapped=True``
+    the run also tracks the unwrapped positions, which start equal to the positions and take every
+    push's displacement without the periodic reduction (src/Particles.jl:131-138,171-175)."""
 
     def __init__(self, domain: FixedDomain, *, positions: np.ndarray | None = None,
-                 number_of_particles: int | None = None, seed: int = 2028):
+                 number_of_particles: int | None = None, seed: int = 2028, unwrapped: bool = False):
         if (positions is None) == (number_of_particles is None):
             raise ValueError("give exactly one of `positions` (a [3, M] array) or `number_of_particles`")
         if positions is None:
@@ -465,6 +474,7 @@ class Tracers:
             raise ValueError("positions must have shape [3, M]")
         self.positions = positions
         self.interpolation_scheme = "CatmullRom"
+        self.unwrapped = bool(unwrapped)
 
     @property
     def count(self) -> int:
@@ -557,6 +567,7 @@ def _config(domain: FixedDomain, conditions: FlowConditions, scheme, carries_sca
     cfg.particle_steps = ps.steps
     cap = tracers.count if tracers is not None else 0
     cfg.tracer_capacity = int(tracer_capacity if tracer_capacity is not None else cap)
+    cfg.unwrapped = int(tracers is not None and tracers.unwrapped)
     keep = keep_physical or tracers is not None
     cfg.store_physical = (1 | (2 if carries_scalar else 0) if keep else 0) | (4 if keep_vorticity else 0)
     return cfg
@@ -727,6 +738,25 @@ class Simulation:
         out = np.zeros((3, self.tracers.count))
         self._call("hdns_tracers_download", L.dptr(out))
         return out
+
+    def unwrapped_positions(self) -> np.ndarray:
+        """unwrapped_positions (src/Particles.jl:317-323): the winding-preserving positions [3, M];
+        raises on a tracer set built without ``unwrapped=True``."""
+        if self.tracers is None or not self.tracers.unwrapped:
+            raise ValueError("this tracer set carries no unwrapped positions: build it with `unwrapped=True` "
+                             "(a `Tracers` keyword) to track winding alongside the wrapped positions")
+        out = np.zeros((3, self.tracers.count))
+        self._call("hdns_tracers_unwrapped_download", L.dptr(out))
+        return out
+
+    def set_unwrapped_positions(self, positions: np.ndarray) -> None:
+        """Replace the unwrapped record (a checkpoint's stored one), [3, M]."""
+        if self.tracers is None or not self.tracers.unwrapped:
+            raise ValueError("this tracer set carries no unwrapped positions: build it with `unwrapped=True`")
+        x = np.ascontiguousarray(positions, dtype=np.float64)
+        if x.shape != (3, self.tracers.count):
+            raise ValueError(f"unwrapped positions must have shape {(3, self.tracers.count)}, got {x.shape}")
+        self._call("hdns_tracers_unwrapped_upload", self.tracers.count, L.dptr(x))
 
     def tracer_samples(self):
         """(velocity [3, M], scalar [M] or None) of the newest trail sample."""

@@ -233,6 +233,8 @@ def write_simulation(path, sim, name: str = "simulation") -> None:
                        particle_scheme_steps=ps.steps, shortened_first_steps=sim.shortened_first_steps)
         tr = _tagged(g, "tracers", "Tracers")
         tr["positions"] = sim.tracer_positions()
+        if sim.tracers.unwrapped:   # the unwrapped record, when tracked (ext/GBGHomogeneousTurbulenceHDF5Ext.jl:229-230)
+            tr["unwrapped_positions"] = sim.unwrapped_positions()
         tr.attrs["interpolation_scheme"] = "CatmullRom"
         # the newest sample is measured again at the stored positions on read; the earlier ones
         # are stored, oldest first, with their times (src/Particles.jl trail order)
@@ -261,14 +263,15 @@ def read_simulation(path, name: str = "simulation", **simulation_kwargs):
     _require_tag(sg, "read_simulation", "ResolvedFields")
     scalar = bool(a["scalar"])
     state = api.ResolvedFields(sg["velocity_spectral"], sg["scalar_spectral"] if scalar else None)
-    tracers, ps, shortened, trail = None, None, False, None
+    tracers, ps, shortened, trail, unwrapped = None, None, False, None, None
     if "tracers" in g:
         if "velocity_at_particles_trail" not in g:
             raise ValueError(f"read_simulation: group {name!r} carries tracers without the times of its velocity "
                              "samples, so the run cannot be continued from it")
         tg = g.group("tracers", "read_simulation")
         _require_tag(tg, "read_simulation", "Tracers")
-        tracers = api.Tracers(domain, positions=tg["positions"])
+        tracers = api.Tracers(domain, positions=tg["positions"], unwrapped="unwrapped_positions" in tg)
+        unwrapped = tg["unwrapped_positions"] if tracers.unwrapped else None
         k = int(a["particle_scheme_steps"])
         ps = {"AB": api.AB, "ABM": api.ABM}[a["particle_integration_scheme"]](k)
         if isinstance(ps, api.ABM) and k == 1:
@@ -293,6 +296,8 @@ def read_simulation(path, name: str = "simulation", **simulation_kwargs):
                 raise ValueError(f"read_simulation: group {name!r} stores a forcing state that does not fit its "
                                  "stored forcing")
             sim.ou[field] = api.OUProcess(sim.ou[field].forcing, sim.ou[field].order, process, draws)
+        if unwrapped is not None:
+            sim.set_unwrapped_positions(unwrapped)
         if "schedule" in a:
             sim.schedule(np.asarray(a["schedule"], dtype=np.int32))
             sim.redeposit()

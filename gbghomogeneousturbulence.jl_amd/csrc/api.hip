@@ -185,7 +185,7 @@ void release(Ctx &c) {
   c.prof_ev.clear();
   void *ptrs[] = {c.uh, c.hu, c.ph, c.hp, c.bufa, c.bufb, c.phys, c.physw, c.tw, c.partials, c.scalars, c.rpart,
                   c.usrc, c.psrc,
-                  c.maxbits, c.spec_bins, c.pos, c.vel, c.phis, c.order, c.tile_counts, c.tile_cursor,
+                  c.maxbits, c.spec_bins, c.pos, c.upos, c.vel, c.phis, c.order, c.tile_counts, c.tile_cursor,
                   c.visc_d, c.kappa_d};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -497,6 +497,9 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
   }
   if (!isfinite(k.mean_gradient)) return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "mean_gradient must be finite");
   if (k.tracer_capacity < 0) return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "tracer_capacity must be >= 0");
+  if (k.unwrapped != 0 && k.unwrapped != 1) return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "unwrapped must be 0 or 1");
+  if (k.unwrapped && k.tracer_capacity == 0)
+    return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "unwrapped positions need tracers (tracer_capacity > 0)");
   if (k.tracer_capacity > 0 && k.interpolation != HDNS_CATMULL_ROM)
     return fail(nullptr, HDNS_ERR_UNSUPPORTED, "tracers support CatmullRom interpolation");
   if (k.particle_scheme != HDNS_HEUN && k.particle_scheme != HDNS_ABM && k.particle_scheme != HDNS_AB)
@@ -609,6 +612,7 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
   if (s == HDNS_OK && k.tracer_capacity > 0) {
     const size_t cap = (size_t)k.tracer_capacity;
     s = alloc(ctx, &c.pos, B * 3 * cap * realb);
+    if (s == HDNS_OK && k.unwrapped) s = alloc(ctx, &c.upos, B * 3 * cap * realb);
     if (s == HDNS_OK) s = alloc(ctx, &c.vel, B * (size_t)c.trail_cap * 3 * cap * realb);
     if (s == HDNS_OK && k.has_scalar) s = alloc(ctx, &c.phis, B * cap * realb);
     if (s == HDNS_OK) s = alloc(ctx, (void **)&c.order, B * cap * sizeof(int));
@@ -1242,6 +1246,24 @@ hdns_status hdns_field_download(hdns_ctx *ctx, int32_t which, double *out) {
   return HDNS_OK;
 }
 
+// count reals from the host into the selected box's slice of a [box][3][cap] tracer array,
+// rounded to the context's element type
+static hdns_status copy_real_in(hdns_ctx *ctx, void *base, size_t count, const double *in) {
+  Ctx &c = ctx->c;
+  const size_t realb = c.fp64 ? 8 : 4;
+  char *dst = (char *)base + (size_t)c.box * 3 * c.cfg.tracer_capacity * realb;
+  if (c.fp64) {
+    HIP_TRY(ctx, hipMemcpyAsync(dst, in, count * 8, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  } else {
+    std::vector<float> tmp(count);
+    for (size_t i = 0; i < count; ++i) tmp[i] = (float)in[i];
+    HIP_TRY(ctx, hipMemcpyAsync(dst, tmp.data(), count * 4, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  }
+  return HDNS_OK;
+}
+
 hdns_status hdns_tracers_upload(hdns_ctx *ctx, int64_t m, const double *positions) {
   if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
   Ctx &c = ctx->c;
@@ -1259,18 +1281,10 @@ hdns_status hdns_tracers_upload(hdns_ctx *ctx, int64_t m, const double *position
   c.advected = false;
   c.pending_t = 0.0;
   if (m == 0) return HDNS_OK;
-  const size_t realb = c.fp64 ? 8 : 4;
-  char *pos = (char *)c.pos + (size_t)c.box * 3 * c.cfg.tracer_capacity * realb;
-  if (c.fp64) {
-    HIP_TRY(ctx, hipMemcpyAsync(pos, positions, 3 * m * 8, hipMemcpyHostToDevice, c.stream));
-    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
-  } else {
-    std::vector<float> tmp(3 * m);
-    for (size_t i = 0; i < tmp.size(); ++i) tmp[i] = (float)positions[i];
-    HIP_TRY(ctx, hipMemcpyAsync(pos, tmp.data(), 3 * m * 4, hipMemcpyHostToDevice, c.stream));
-    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
-  }
-  return HDNS_OK;
+  const hdns_status s = copy_real_in(ctx, c.pos, 3 * (size_t)m, positions);
+  if (s != HDNS_OK || !c.upos) return s;
+  // the unwrapped record starts equal to the positions (src/Particles.jl:171-175)
+  return copy_real_in(ctx, c.upos, 3 * (size_t)m, positions);
 }
 
 static hdns_status copy_real_out(hdns_ctx *ctx, const void *src, size_t count, double *out) {
@@ -1295,6 +1309,33 @@ hdns_status hdns_tracers_download(hdns_ctx *ctx, double *positions) {
   const size_t realb = c.fp64 ? 8 : 4;
   return copy_real_out(ctx, (const char *)c.pos + (size_t)c.box * 3 * c.cfg.tracer_capacity * realb, 3 * (size_t)c.m,
                        positions);
+}
+
+static const char *NO_UNWRAPPED =
+    "this tracer set carries no unwrapped positions: build it with `unwrapped = true` to track winding alongside "
+    "the wrapped positions";
+
+hdns_status hdns_tracers_unwrapped_download(hdns_ctx *ctx, double *positions) {
+  if (!ctx || !positions) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "positions must not be NULL");
+  const Ctx &c = ctx->c;
+  if (!c.upos) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, NO_UNWRAPPED);
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  const size_t realb = c.fp64 ? 8 : 4;
+  return copy_real_out(ctx, (const char *)c.upos + (size_t)c.box * 3 * c.cfg.tracer_capacity * realb,
+                       3 * (size_t)c.m, positions);
+}
+
+hdns_status hdns_tracers_unwrapped_upload(hdns_ctx *ctx, int64_t m, const double *positions) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  Ctx &c = ctx->c;
+  if (!c.upos) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, NO_UNWRAPPED);
+  if (m != c.m)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT,
+                "the unwrapped record holds one position per tracer: " + std::to_string(c.m) + " expected, got " +
+                    std::to_string(m));
+  if (m > 0 && !positions) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "positions must not be NULL");
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  return copy_real_in(ctx, c.upos, 3 * (size_t)m, positions);
 }
 
 hdns_status hdns_rebuild_traversal(hdns_ctx *ctx) {

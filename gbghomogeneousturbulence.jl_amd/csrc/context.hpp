@@ -102,6 +102,7 @@ struct Ctx {
   // tracers
   int64_t m = 0;
   void *pos = nullptr, *vel = nullptr, *phis = nullptr;
+  void *upos = nullptr;        // the unwrapped record [box][3][cap], when tracked; else null
   int *order = nullptr, *tile_counts = nullptr, *tile_cursor = nullptr;
   int ntiles = 0;
   bool order_valid = false;

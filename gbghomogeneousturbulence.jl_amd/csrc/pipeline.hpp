@@ -299,8 +299,8 @@ struct PipelineImpl final : Pipeline {
   int tblocks(const Ctx &c) const { return (int)((c.m + 255) / 256); }
   void advect(Ctx &c, const AdamsArgs &aa) override {
     if (c.m == 0) return;
-    hipLaunchKernelGGL((k_advect<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (T *)c.pos, (const T *)c.vel,
-                       (const T *)c.phys, c.order_valid ? c.order : nullptr, targs(c), aa);
+    hipLaunchKernelGGL((k_advect<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (T *)c.pos, (T *)c.upos,
+                       (const T *)c.vel, (const T *)c.phys, c.order_valid ? c.order : nullptr, targs(c), aa);
   }
   void trail_combine(Ctx &c, const TrailCombine &tc, void *out) override {   // the selected box
     if (c.m == 0) return;

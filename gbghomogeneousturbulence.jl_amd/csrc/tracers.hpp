@@ -96,7 +96,9 @@ constexpr int MAX_TRAIL = 8;   // samples an Adams step integrates over, and the
 // formed on the host from the samples' times (src/ParticlePropagator.jl:287-316):
 //   X* = wrap(X + sum_j b_j f_j);  when corrected, f* = u(X*) and
 //   X <- wrap(X* + c_0 f* + sum_j c_{j+1} f_j),  c_0 = beta_*, c_{j+1} = beta_{j+1} - b_j.
-// Each sum is added left to right in the order written (one fused displacement each).
+// Each sum is added left to right in the order written (one fused displacement each).  The
+// unwrapped record U, when tracked (upos non-null), takes the same terms in the same order and is
+// never reduced to the box (src/ParticlePropagator.jl:299,313).
 struct AdamsArgs {
   int J;
   int corrected;
@@ -106,21 +108,26 @@ struct AdamsArgs {
 };
 
 template <typename T>
-__global__ void k_advect(T *__restrict__ pos, const T *__restrict__ trail, const T *__restrict__ u,
+__global__ void k_advect(T *__restrict__ pos, T *__restrict__ upos, const T *__restrict__ trail, const T *__restrict__ u,
                          const int *__restrict__ order, TracerArgs ta, AdamsArgs aa) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= ta.m) return;
   const int box = blockIdx.y;
   pos += box * 3 * ta.cap;
+  if (upos) upos += box * 3 * ta.cap;
   trail += box * ta.vstride;
   u += box * ta.box_phys;
   if (order) order += box * ta.cap;
   const int64_t p = order ? order[t] : t;
   const T L = T(ta.side_length), inv_dx = T(ta.n / ta.side_length);
   const size_t vol = (size_t)ta.n * ta.n * ta.n;
-  T F[MAX_TRAIL][3], acc[3], Xs[3];
+  T F[MAX_TRAIL][3], acc[3], Xs[3], ua[3] = {T(0), T(0), T(0)};
 #pragma unroll
   for (int d = 0; d < 3; ++d) acc[d] = pos[d * ta.m + p];
+  if (upos) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ua[d] = upos[d * ta.m + p];
+  }
 #pragma unroll
   for (int j = 0; j < MAX_TRAIL; ++j) {
     if (j >= aa.J) break;
@@ -129,6 +136,7 @@ __global__ void k_advect(T *__restrict__ pos, const T *__restrict__ trail, const
     for (int d = 0; d < 3; ++d) {
       F[j][d] = trail[aa.slot[j] + d * ta.m + p];
       acc[d] = acc[d] + bj * F[j][d];
+      ua[d] = ua[d] + bj * F[j][d];
     }
   }
 #pragma unroll
@@ -136,6 +144,10 @@ __global__ void k_advect(T *__restrict__ pos, const T *__restrict__ trail, const
   if (!aa.corrected) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) pos[d * ta.m + p] = Xs[d];
+    if (upos) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) upos[d * ta.m + p] = ua[d];
+    }
     return;
   }
   T wx[4], wy[4], wz[4], fs[3];
@@ -143,16 +155,26 @@ __global__ void k_advect(T *__restrict__ pos, const T *__restrict__ trail, const
   gather<T, 3>(u, vol, ta.n, s, wx, wy, wz, fs);
   const T c0 = T(aa.c[0]);
 #pragma unroll
-  for (int d = 0; d < 3; ++d) acc[d] = Xs[d] + c0 * fs[d];
+  for (int d = 0; d < 3; ++d) {
+    acc[d] = Xs[d] + c0 * fs[d];
+    ua[d] = ua[d] + c0 * fs[d];
+  }
 #pragma unroll
   for (int j = 0; j < MAX_TRAIL; ++j) {
     if (j >= aa.J) break;
     const T cj = T(aa.c[j + 1]);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) acc[d] = acc[d] + cj * F[j][d];
+    for (int d = 0; d < 3; ++d) {
+      acc[d] = acc[d] + cj * F[j][d];
+      ua[d] = ua[d] + cj * F[j][d];
+    }
   }
 #pragma unroll
   for (int d = 0; d < 3; ++d) pos[d * ta.m + p] = wrap_box(acc[d], L);
+  if (upos) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) upos[d * ta.m + p] = ua[d];
+  }
 }
 
 // sum_j w_j f_j over n trail samples of the selected box, added left to right (time_derivative!,

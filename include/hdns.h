@@ -74,7 +74,7 @@ typedef struct {
   double forcing_gain;       /* VarianceController G >= 0 (0: the marginal law) */
   double forcing_time;       /* VarianceController turnover time t_l > 0 */
   int32_t particle_steps;    /* k of ABM(k) (1..7) or AB(k) (1..8); ignored for HDNS_HEUN */
-  int32_t reserved;
+  int32_t unwrapped;         /* 1: also track the unwrapped positions (src/Particles.jl:131-138); needs tracers */
   /* the scalar forcing (src/Generator.jl:652-653): the velocity forcings' strategies with the
    * variance factor 1 and the diffusivity nu/Sc (src/Forcing.jl:456); needs has_scalar */
   int32_t scalar_forcing;        /* HDNS_NO_FORCING, _CONSTANT_POWER, _LINEAR, _VARIANCE_CONTROLLER, _VARIANCE_RESCALING */
@@ -158,6 +158,14 @@ hdns_status hdns_field_download(hdns_ctx *ctx, int32_t which, double *out);
 /* Tracers (src/Particles.jl:135-176): set M <= tracer_capacity positions, layout [3][M]. */
 hdns_status hdns_tracers_upload(hdns_ctx *ctx, int64_t m, const double *positions);
 hdns_status hdns_tracers_download(hdns_ctx *ctx, double *positions);
+/* unwrapped_positions(tracers) (src/Particles.jl:317-323): the unwrapped record, [3][M], which
+ * hdns_tracers_upload starts equal to the positions and every push advances by the same
+ * displacement terms without the wrap (src/ParticlePropagator.jl:299,313).  Fails on a context
+ * built without unwrapped = 1. */
+hdns_status hdns_tracers_unwrapped_download(hdns_ctx *ctx, double *positions);
+/* Set the unwrapped record of the current M tracers (read_tracers, ext/GBGHomogeneousTurbulenceHDF5Ext.jl:236-241),
+ * after hdns_tracers_upload has set the positions. */
+hdns_status hdns_tracers_unwrapped_upload(hdns_ctx *ctx, int64_t m, const double *positions);
 /* rebuild_traversal! (src/Particles.jl:268-271): counting sort of the tracers by cell tile. */
 hdns_status hdns_rebuild_traversal(hdns_ctx *ctx);
 /* advect!(scheme, ...) (src/ParticlePropagator.jl:287-372): the Adams predictor over the trail's

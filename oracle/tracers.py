@@ -119,7 +119,7 @@ def heun_advect(X: np.ndarray, v_prev: np.ndarray, u_new: np.ndarray, h: float, 
 
 
 def adams_advect(X: np.ndarray, samples, times, u_new: np.ndarray, h: float, side_length: float,
-                 corrected: bool) -> np.ndarray:
+                 corrected: bool, unwrapped: np.ndarray | None = None):
     """One AB(J) step, or with ``corrected`` one ABM(J) step, of the positions X[M, 3]
     (src/ParticlePropagator.jl:287-316): ``samples`` are the trail's velocity samples [M, 3],
     newest first, taken at ``times``; the Adams weights come from the samples' actual times.
@@ -127,22 +127,30 @@ def adams_advect(X: np.ndarray, samples, times, u_new: np.ndarray, h: float, sid
         X* = wrap(X + sum_j b_j f_j)
         X' = wrap(X* + beta_* f* + sum_j (beta_{j+1} - b_j) f_j),  f* = u_new(X*)   (corrected)
 
-    Each sum is added left to right in the order written, as one fused displacement.
+    Each sum is added left to right in the order written, as one fused displacement.  Given the
+    unwrapped record U[M, 3] (src/ParticlePropagator.jl:299,313), returns (X', U') with U taking the
+    same terms in the same order and never wrapped.
     """
     nodes = tuple(float(t) - float(times[0]) for t in times)
     b = adams_weights(nodes, h)
-    acc = X
+    acc, uacc = X, unwrapped
     for bj, f in zip(b, samples):
         acc = acc + bj * f
+        if uacc is not None:
+            uacc = uacc + bj * f
     Xs = wrap(acc, side_length)
     if not corrected:
-        return Xs
+        return Xs if unwrapped is None else (Xs, uacc)
     fs = interpolate(u_new, Xs, side_length)
     beta = adams_weights((h,) + nodes, h)
     acc = Xs + beta[0] * fs
+    if uacc is not None:
+        uacc = uacc + beta[0] * fs
     for j, f in enumerate(samples):
         acc = acc + (beta[j + 1] - b[j]) * f
-    return wrap(acc, side_length)
+        if uacc is not None:
+            uacc = uacc + (beta[j + 1] - b[j]) * f
+    return wrap(acc, side_length) if unwrapped is None else (wrap(acc, side_length), uacc)
 
 
 def derivative_weights(nodes, order: int) -> np.ndarray:

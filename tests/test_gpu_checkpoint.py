@@ -1,7 +1,7 @@
 """Checkpoint and restart on the device: a run written after 3 steps and read back continues bit
 for bit like the run that never stopped (the reference's test/hdf5_checkpoint.jl:202-230 asks the
 same of an IFSSPRK3 run), through the state, the clock, the forcing, the tracer positions and the
-velocity trail with its times."""
+velocity trail with its times, and the unwrapped positions (test/hdf5_checkpoint.jl:114-135)."""
 import math
 
 import numpy as np
@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 L = 2 * math.pi
 
 
-def _run(g, dtype, scheme, forcing, particles):
+def _run(g, dtype, scheme, forcing, particles, unwrapped=True):
     n = 32
     dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
     vf = {"constant_power": lambda: g.ConstantPower(dom, injection_rate=0.1),
@@ -19,7 +19,7 @@ def _run(g, dtype, scheme, forcing, particles):
           "controller": lambda: g.VarianceController(dom, target=0.6, turnover_time=0.7),
           "rescaling": lambda: g.VarianceRescaling(dom, target=0.55, over="total")}[forcing]()
     cond = g.FlowConditions(viscosity=0.02, schmidt=0.9, velocity_forcing=vf)
-    tr = g.Tracers(dom, number_of_particles=300, seed=5)
+    tr = g.Tracers(dom, number_of_particles=300, seed=5, unwrapped=unwrapped)
     return g.Simulation(dom, conditions=cond, state=g.ResolvedFields(g.random_phase_velocity(n, seed=71),
                                                                      g.random_phase_scalar(n, seed=72)),
                         field_integration_scheme=scheme, tracers=tr, particle_integration_scheme=particles,
@@ -29,7 +29,8 @@ def _run(g, dtype, scheme, forcing, particles):
 def _snapshot(sim):
     st = sim.state()
     times = sim.trail_times()
-    return {"u": st.velocity_spectral, "phi": st.scalar_spectral, "x": sim.tracer_positions(), "times": times,
+    return {"u": st.velocity_spectral, "phi": st.scalar_spectral, "x": sim.tracer_positions(),
+            "unwrapped": sim.unwrapped_positions(), "times": times,
             "trail": [sim.trail_sample(-j) for j in range(len(times))], "clock": (sim.time, sim.step)}
 
 
@@ -63,6 +64,7 @@ def test_a_restored_run_continues_bit_for_bit(g, tmp_path, dtype, scheme, forcin
     assert np.array_equal(got["u"], want["u"])
     assert np.array_equal(got["phi"], want["phi"])
     assert np.array_equal(got["x"], want["x"])
+    assert np.array_equal(got["unwrapped"], want["unwrapped"])
     assert got["times"] == want["times"]
     for a_, b_ in zip(got["trail"], want["trail"]):
         assert np.array_equal(a_, b_)
@@ -82,3 +84,17 @@ def test_the_checkpoint_keeps_the_settings(g, tmp_path):
         assert b.mean_gradient == 0.8 and b.conditions.schmidt == 0.9
         assert list(b.schedule()) == [0, 1, 1, 0, 1]
         assert b.trail_times() == [1e-3, 0.0]
+
+
+def test_an_untracked_set_stays_untracked(g, tmp_path):
+    """test/hdf5_checkpoint.jl:74-78: a set without the unwrapped record reads back without it."""
+    from gbg_amd import checkpoint as ck
+    with _run(g, np.float64, g.IFLowStorageRK4, "constant_power", g.Heun(), unwrapped=False) as a:
+        a.advance(1e-3)
+        with pytest.raises(ValueError, match="no unwrapped positions"):
+            a.unwrapped_positions()
+        ck.write_simulation(tmp_path / "p.npz", a)
+    with ck.read_simulation(tmp_path / "p.npz") as b:
+        assert not b.tracers.unwrapped
+        with pytest.raises(ValueError, match="no unwrapped positions"):
+            g.unwrapped_positions(b)

@@ -99,6 +99,66 @@ def test_multistep_order_on_the_decaying_shear_wave(g, kind, k, order):
     assert 0.7 * r < errs[0] / errs[1] < 1.4 * r and 0.7 * r < errs[1] / errs[2] < 1.4 * r
 
 
+@pytest.mark.parametrize("kind,k", [("ab", 3), ("abm", 1)])
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-11), (np.float32, 5e-4)])
+def test_unwrapped_positions_keep_the_winding(g, kind, k, dtype, tol):
+    """src/ParticlePropagator.jl:299,313 and src/Particles.jl:171-175: the unwrapped record starts
+    at the positions and takes every push's terms without the wrap.  A mean flow (+2 along x, -2
+    along z) carries tracers seeded by the faces across them; the record matches the oracle's, its
+    winding is a whole number of boxes, and reduced to the box it is the wrapped position."""
+    n, m = 32, 200
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
+    rng = np.random.default_rng(3)
+    pos = np.stack([rng.uniform(L - 0.15, L, m), rng.uniform(0, L, m), rng.uniform(0, 0.15, m)])
+    pos = pos.astype(dtype).astype(np.float64)
+    u = g.random_phase_velocity(n, seed=53)
+    u[0, 0, 0, 0] += 2.0 * n ** 3
+    u[2, 0, 0, 0] -= 2.0 * n ** 3
+    oc = dns.Conditions(viscosity=0.02, schmidt=1.0)
+    d = dns.Domain(n)
+    cap = k if kind == "ab" else k + 1
+    dts = [1e-2 * (1 + 0.2 * math.cos(i)) for i in range(10)]
+    with g.Simulation(dom, conditions=g.FlowConditions(viscosity=0.02, schmidt=1.0), state=g.ResolvedFields(u, None),
+                      tracers=g.Tracers(dom, positions=pos, unwrapped=True),
+                      particle_integration_scheme=_scheme(g, kind, k)) as sim:
+        assert np.array_equal(sim.unwrapped_positions(), sim.tracer_positions())
+        uo, dep = u, dns.deposit(d, u, None)
+        X, U = pos.T.copy(), pos.T.copy()
+        samples, times = [trc.interpolate(dep.u, X, L)], [0.0]
+        for dt in dts:
+            uo, _, dep = dns.step(d, oc, "rk4", uo, None, dt, dep)
+            full = len(samples) >= k
+            J = k if full else len(samples)
+            X, U = trc.adams_advect(X, samples[:J], times[:J], dep.u, dt, L,
+                                    corrected=(not full) or kind == "abm", unwrapped=U)
+            samples = [trc.interpolate(dep.u, X, L)] + samples[: cap - 1]
+            times = [times[0] + dt] + times[: cap - 1]
+            sim.advance(dt)
+        x, uw = sim.tracer_positions(), g.unwrapped_positions(sim)
+    crossed = int(np.count_nonzero((uw < 0) | (uw >= L)))
+    print(f"{kind}({k}) {np.dtype(dtype).name}: unwrapped {np.max(np.abs(uw - U.T)):.2e}, "
+          f"{crossed} crossings", flush=True)
+    assert crossed > m                                   # most tracers crossed the x and the z faces
+    assert np.max(np.abs(uw - U.T)) < tol
+    assert np.max(np.abs(x - X.T)) < tol
+    winding = (uw - x) / L
+    assert np.max(np.abs(winding - np.round(winding))) < (1e-12 if dtype is np.float64 else 1e-5)
+    assert np.all(np.round(winding[0]) >= 0) and np.all(np.round(winding[2]) <= 0)
+
+
+def test_unwrapped_positions_are_opt_in(g):
+    n = 16
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
+    with g.Simulation(dom, conditions=g.FlowConditions(viscosity=0.02, schmidt=1.0),
+                      state=g.ResolvedFields(g.random_phase_velocity(n, seed=1), None),
+                      tracers=g.Tracers(dom, number_of_particles=10)) as sim:
+        with pytest.raises(ValueError, match="no unwrapped positions"):
+            g.unwrapped_positions(sim)
+        buf = np.zeros((3, 10))
+        with pytest.raises(Exception, match="no unwrapped positions"):   # the C-ABI refuses it too
+            sim._call("hdns_tracers_unwrapped_download", g._lib.dptr(buf))
+
+
 def test_shortened_first_steps_start_small_and_grow(g):
     """src/SimulationRun.jl:1099-1129 under AB(4): the first step is tau (dt/tau)^(4/3) with tau the
     Kolmogorov time of the initial state; each later one at most twice the last, up to the CFL step."""

@@ -193,3 +193,34 @@ def test_time_derivative_of_a_trail_converges_to_the_acceleration():
         errs.append(abs(a[0] - s * s * x0))
     assert 6.0 < errs[0] / errs[1] < 10.0 and 6.0 < errs[1] / errs[2] < 10.0
     assert trc.time_derivative(samples, ts, 0)[0] == samples[0][0]
+
+
+@pytest.mark.parametrize("kind,k", [("ab", 3), ("abm", 2), ("abm", 1)])
+def test_unwrapped_record_of_a_uniform_flow_is_the_straight_line(kind, k):
+    """src/ParticlePropagator.jl:299,313: the unwrapped record takes every displacement term
+    without the wrap, so in a uniform flow c it is X0 + c t across several windings (the Adams
+    weights of each step sum to the step), and reduced to the box it is the wrapped position."""
+    n = 16
+    c = np.array([7.0, 0.0, -3.5])
+    u = np.zeros((3, n, n, n))
+    u[0], u[2] = c[0], c[2]
+    X0 = trc.uniform_positions(20, L, 5)
+    cap = k if kind == "ab" else k + 1
+    dts = [0.05 * (1 + 0.3 * math.sin(i)) for i in range(40)]
+    X, U = X0.copy(), X0.copy()
+    samples, times = [trc.interpolate(u, X, L)], [0.0]
+    for dt in dts:
+        full = len(samples) >= k
+        J = k if full else len(samples)
+        X, U = trc.adams_advect(X, samples[:J], times[:J], u, dt, L, corrected=(not full) or kind == "abm",
+                                unwrapped=U)
+        samples = [trc.interpolate(u, X, L)] + samples[: cap - 1]
+        times = [times[0] + dt] + times[: cap - 1]
+    T = sum(dts)
+    assert np.max(np.abs(U - (X0 + c * T))) < 1e-11
+    assert np.array_equal(U[:, 1], X0[:, 1])
+    winding = (U - X) / L
+    assert np.max(np.abs(winding - np.round(winding))) < 1e-12
+    assert set(np.round(winding[:, 0])) >= {2.0} and set(np.round(winding[:, 2])) <= {-1.0, -2.0}
+    assert np.array_equal(trc.adams_advect(X0, samples[:1], times[:1], u, 0.1, L, corrected=True),
+                          trc.adams_advect(X0, samples[:1], times[:1], u, 0.1, L, corrected=True, unwrapped=X0)[0])
