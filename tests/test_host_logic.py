@@ -278,3 +278,15 @@ def test_derived_scales_closed_forms_and_guards(g):
     with pytest.raises(ValueError, match="viscosity"):
         g.reynolds_taylor(k, eps, -1.0)
     assert g.reynolds_box(Lb, 0.0, nu) == 0.0
+
+
+def test_the_seed_is_the_same_field_on_every_box(g):
+    """test/box_scaling.jl:34-43: the random-phase seed on a box of side 0.02 is the 2 pi seed with
+    the velocity scaled by sigma = 0.02 / 2 pi (energy by sigma^2) and the scalar unchanged."""
+    sigma = 0.02 / (2 * math.pi)
+    u = g.random_phase_velocity(16, side_length=2 * math.pi, energy=1.5, seed=4)
+    us = g.random_phase_velocity(16, side_length=0.02, energy=1.5 * sigma ** 2, seed=4)
+    assert np.max(np.abs(u - us / sigma)) / np.max(np.abs(u)) < 1e-13
+    p = g.random_phase_scalar(16, side_length=2 * math.pi, seed=5)
+    ps = g.random_phase_scalar(16, side_length=0.02, seed=5)
+    assert np.max(np.abs(p - ps)) / np.max(np.abs(p)) < 1e-13
