@@ -64,8 +64,14 @@ def test_fixed_steps_scale_with_the_box(g, name):
             for _ in range(4):
                 sim.advance(0.002)
             st = sim.state()
-            runs.append((st.velocity_spectral, st.scalar_spectral, sim.reductions()))
-    (uR, pR, rR), (uL, pL, rL) = runs
+            kappa, e = sim.energy_spectrum()
+            # test/validation.jl:26-35: the shells are 2 pi / L wide, and sum E dkappa is k
+            dk = 2 * math.pi / side
+            assert kappa[1] == pytest.approx(dk, rel=1e-15)
+            assert float(np.sum(e)) * dk == pytest.approx(sim.reductions()["kinetic_energy"], rel=1e-12)
+            runs.append((st.velocity_spectral, st.scalar_spectral, sim.reductions(), e))
+    (uR, pR, rR, eR), (uL, pL, rL, eL) = runs
+    assert rel(eR, eL / SIGMA ** 3) < 1e-13   # E(kappa) dkappa is an energy: E scales by sigma^3
     eu = rel(uR, uL / SIGMA)
     print(f"{name}: velocity {eu:.2e}", flush=True)
     assert eu < 1e-13
