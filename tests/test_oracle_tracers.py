@@ -224,3 +224,126 @@ def test_unwrapped_record_of_a_uniform_flow_is_the_straight_line(kind, k):
     assert set(np.round(winding[:, 0])) >= {2.0} and set(np.round(winding[:, 2])) <= {-1.0, -2.0}
     assert np.array_equal(trc.adams_advect(X0, samples[:1], times[:1], u, 0.1, L, corrected=True),
                           trc.adams_advect(X0, samples[:1], times[:1], u, 0.1, L, corrected=True, unwrapped=X0)[0])
+
+
+# --- the start-up and the stability of the Adams family (test/particles.jl:30-85, 233-318) ---------
+# x' = cos(w t) s (x - xc): Catmull-Rom is exact on the linear field, and the exact path is
+# xc + (x0 - xc) exp(s sin(w t) / w).
+S_, W_, XC_, X0_ = 1.0, 3.0, math.pi, math.pi + 0.4
+
+
+def _strain_exact(t):
+    return XC_ + (X0_ - XC_) * math.exp(S_ * math.sin(W_ * t) / W_)
+
+
+def _strain_grid(t, n=16):
+    u = np.zeros((3, n, n, n))
+    u[0] = math.cos(W_ * t) * S_ * (np.arange(n) * L / n - XC_)[None, None, :]
+    return u
+
+
+def _strain_error(kind, k, steps, exact_start=False):
+    """The largest error along a pass of the run's loop through `steps` (time-dependent field)."""
+    cap = k if kind == "ab" else k + 1
+    X = np.array([[X0_, 1.0, 1.0]])
+    if exact_start:     # the exact earlier samples at -h, -2h, ... for the first step h
+        h = steps[0]
+        times = [-j * h for j in range(min(k, cap))]
+        samples = [np.array([[math.cos(W_ * t) * S_ * (_strain_exact(t) - XC_), 0.0, 0.0]]) for t in times]
+    else:
+        times, samples = [0.0], [trc.interpolate(_strain_grid(0.0), X, L)]
+    t, worst = 0.0, 0.0
+    for dt in steps:
+        full = len(samples) >= k
+        J = k if full else len(samples)
+        u_new = _strain_grid(t + dt)
+        X = trc.adams_advect(X, samples[:J], times[:J], u_new, dt, L, corrected=(not full) or kind == "abm")
+        t += dt
+        samples = [trc.interpolate(u_new, X, L)] + samples[: cap - 1]
+        times = [t] + times[: cap - 1]
+        worst = max(worst, abs(X[0, 0] - _strain_exact(t)))
+    return worst
+
+
+def _uniform(n, T=2.0):
+    return [T / n] * n
+
+
+def _shortened(g, scheme, n, T=2.0):
+    """The schedule shortened_first_steps gives with tau = 1/w (src/SimulationRun.jl:1099-1129),
+    built with the host's own rule: nu / eps = tau^2."""
+    dt, steps, t, trail = T / n, [], 0.0, [0.0]
+    while t < T - 1e-12:
+        h = min(dt, g.api.startup_bound(scheme, trail, dt, 1.0, W_ * W_), T - t)
+        steps.append(h)
+        t += h
+        trail = [t] + trail[:1]
+    return steps
+
+
+def _order(errors):
+    return math.log2(errors[-2] / errors[-1])
+
+
+def test_the_ordinary_start_up_leaves_an_error_of_third_order():
+    counts = (64, 128, 256)
+    ref = [_strain_error("abm", 4, _uniform(n)) for n in counts]
+    assert 2.7 < _order(ref) < 3.3
+    for kind, k in (("ab", 6), ("ab", 8), ("abm", 7)):
+        errs = [_strain_error(kind, k, _uniform(n)) for n in counts]
+        assert 2.7 < _order(errs) < 3.3
+        assert errs[-1] == pytest.approx(ref[-1], rel=0.05)
+    errs = [_strain_error("ab", 2, _uniform(n)) for n in counts]
+    assert 1.7 < _order(errs) < 2.3
+
+
+def test_shortened_first_steps_restore_the_full_order(g):
+    for kind, k in [("ab", k) for k in range(3, 9)] + [("abm", k) for k in range(2, 8)]:
+        scheme = g.AB(k) if kind == "ab" else g.ABM(k)
+        errs = [_strain_error(kind, k, _shortened(g, scheme, n)) for n in (32, 64, 128)]
+        assert _order(errs) > scheme.order - 0.6, (kind, k, errs)
+        assert errs[1] < 4 * _strain_error(kind, k, _uniform(64), exact_start=True), (kind, k)
+    # at order 8 the doubling that serves order 5 loses the accuracy again
+    dt, steps, t = 2.0 / 64, [], 0.0
+    while t < 2.0 - 1e-12:
+        h = min(dt, (1 / W_) * (dt * W_) ** (8 / 3)) if not steps else min(dt, 2.0 * steps[-1])
+        h = min(h, 2.0 - t)
+        steps.append(h)
+        t += h
+    assert _strain_error("ab", 8, steps) > 20 * _strain_error("ab", 8, _shortened(g, g.AB(8), 64))
+
+
+def _real_axis_limit(k, corrected):
+    """The largest h|lambda| at which the scheme at uniform step damps x' = lambda x: the recurrence
+    y_{n+1} = sum_j c_j(z) y_{n+1-j} is stable when every root of its characteristic polynomial
+    lies in the closed unit disc."""
+    nodes = tuple(-float(i) for i in range(k))
+    b = np.array(trc.adams_weights(nodes, 1.0))
+    beta = np.array(trc.adams_weights((1.0,) + nodes, 1.0)) if corrected else None
+
+    def stable(z):
+        c = z * beta[1:] + z * z * beta[0] * b if corrected else z * b
+        c = c.copy()
+        c[0] += 1 + z * beta[0] if corrected else 1
+        return np.max(np.abs(np.roots(np.concatenate([[1.0], -c])))) <= 1 + 1e-9
+
+    lo, hi = 0.0, 3.0
+    for x in np.arange(0.005, 3.0, 0.005):
+        if not stable(-x):
+            hi = x
+            break
+        lo = x
+    for _ in range(30):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if stable(-mid) else (lo, mid)
+    return lo
+
+
+def test_the_stability_limits_on_the_real_axis_are_the_textbook_ones():
+    for k, want in ((1, 2.0), (2, 1.0), (3, 6 / 11), (4, 3 / 10), (5, 90 / 551)):
+        assert _real_axis_limit(k, False) == pytest.approx(want, abs=2e-3), k
+    assert _real_axis_limit(1, True) == pytest.approx(2.0, abs=2e-3)       # Heun: 1 + z + z^2/2
+    assert _real_axis_limit(6, False) == pytest.approx(0.088, abs=2e-3)
+    assert _real_axis_limit(8, False) == pytest.approx(0.024, abs=2e-3)
+    for k, want in ((3, 1.93), (5, 1.04), (7, 0.58)):
+        assert _real_axis_limit(k, True) == pytest.approx(want, abs=0.02), k
