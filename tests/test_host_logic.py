@@ -290,3 +290,59 @@ def test_the_seed_is_the_same_field_on_every_box(g):
     p = g.random_phase_scalar(16, side_length=2 * math.pi, seed=5)
     ps = g.random_phase_scalar(16, side_length=0.02, seed=5)
     assert np.max(np.abs(p - ps)) / np.max(np.abs(p)) < 1e-13
+
+
+@pytest.mark.parametrize("order", [1, 0])
+def test_stochastic_ou_mean_square_spread_shares_and_lag(g, order):
+    """test/forcing.jl:StochasticOU: over draws ten correlation times apart the force's physical mean
+    square averages sigma^2 (2 %) and spreads as a chi^2 over D real degrees of freedom, sqrt(2/D)
+    (10 %; D = 2 modes for the velocity after the transverse projection, modes for the scalar), which
+    an increment normalised by its own realized norm fails.  At dt = T_L / 10 every band mode holds
+    the same share (20 %) and the lag-one correlation is exp(-dt / T_L) (0.01)."""
+    f, p = _ou(g, order, var=0.4, tl=0.5)
+    nc = f.companion_points
+    w = np.full(nc // 2 + 1, 2.0)
+    w[0] = 1.0
+
+    def mean_square(a):
+        return float(np.sum(w * np.abs(a) ** 2)) / nc ** 6
+
+    samples = []
+    for _ in range(4000):
+        p.advance(5.0)
+        samples.append(mean_square(p.process))
+    avg = float(np.mean(samples))
+    assert abs(avg / 0.4 - 1) < 0.02
+    dof = (2 if order == 1 else 1) * f.modes
+    spread = float(np.std(samples, ddof=1)) / avg
+    assert abs(spread / math.sqrt(2 / dof) - 1) < 0.1, (spread, math.sqrt(2 / dof))
+    alpha = math.exp(-0.05 / 0.5)
+    per_mode = np.zeros(p.process.shape[-3:])
+    lagged = power = 0.0
+    prev = p.process.copy()
+    for _ in range(20000):
+        p.advance(0.05)
+        a = p.process
+        per_mode += np.sum(np.abs(a) ** 2, axis=0) if order == 1 else np.abs(a) ** 2
+        lagged += float(np.real(np.sum(a * np.conj(prev))))
+        power += float(np.sum(np.abs(prev) ** 2))
+        prev = a.copy()
+    forced = per_mode[f.band]
+    assert np.all(per_mode[~f.band] == 0)
+    assert forced.max() / forced.min() - 1 < 0.2
+    assert abs(lagged / power - alpha) < 0.01
+
+
+def test_stochastic_ou_every_seed_bit_reaches_the_stream(g):
+    """test/forcing.jl: a negative seed builds, draws and differs from its magnitude, and a seed's
+    high bits change the stream."""
+    dom = g.FixedDomain(grid_points_per_axis=16, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
+
+    def draw(seed):
+        return g.api.OUProcess(g.StochasticOU(dom, forcing_variance=0.3, correlation_time=0.4, seed=seed), 1).process
+
+    assert np.array_equal(draw(7), draw(7))
+    assert not np.array_equal(draw(7), draw(8))
+    assert not np.array_equal(draw(-7), draw(7))
+    assert not np.array_equal(draw(1), draw(1 + (1 << 40)))
+    assert not np.array_equal(draw(1), draw(1 - (1 << 63)))
