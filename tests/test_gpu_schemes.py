@@ -82,3 +82,32 @@ def test_the_scalar_mean_is_conserved_to_the_bit(g, dtype):
     held = complex(np.complex64(p[0, 0, 0])) if dtype is np.float32 else p[0, 0, 0]   # the upload's rounding
     assert st.scalar_spectral[0, 0, 0] == held
     assert np.max(np.abs(st.scalar_spectral[1:4, 1:4, 1:4])) > 0   # the rest of the field moved
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_a_resting_flow_stays_at_rest_under_the_zero_speed_step(g, dtype):
+    """test/regressions/03-numerics-and-types-03-cfl-zero-speed.jl: at max|u| = 0 the adaptive
+    step falls back to cfl dx; a resting unforced flow is a fixed point of every advance, the clock
+    moves, and the scalar's decay over an interval does not depend on how it is split."""
+    n, nu = 16, 1e-3
+    dom = _dom(g, n, dtype)
+    u = np.zeros((3, n, n, n // 2 + 1), dtype=np.complex128)
+    p = g.random_phase_scalar(n, seed=99)
+    runs = []
+    for splits in (1, 2):
+        with g.Simulation(dom, conditions=g.FlowConditions(viscosity=nu, schmidt=1.0), state=g.ResolvedFields(u, p),
+                          field_integration_scheme=g.IFLowStorageRK3) as sim:
+            assert sim.max_speed() == 0
+            dt = sim.cfl_timestep()
+            assert dt == pytest.approx(sim.cfl * dom.grid_spacing, rel=1e-15)
+            for _ in range(20 // splits):
+                sim.advance(dt * splits)
+            st = sim.state()
+            assert not np.any(st.velocity_spectral)
+            assert sim.reductions()["kinetic_energy"] == 0
+            assert math.isfinite(sim.time) and sim.time == pytest.approx(20 * dt, rel=1e-12)
+            runs.append(st.scalar_spectral)
+    err = float(np.max(np.abs(runs[0] - runs[1])) / np.max(np.abs(runs[1])))
+    # FP32: 20 steps x 3 stage factors round up to 60 x 2^-24 = 3.6e-6 apart; a step-dependent
+    # decay would differ at order one
+    assert err < (1e-14 if dtype is np.float64 else 5e-6)
