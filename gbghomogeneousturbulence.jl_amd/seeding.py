@@ -106,6 +106,9 @@ def random_phase_velocity(n: int, side_length: float = 2 * math.pi, energy: floa
         u[c][keep] = amp * (alpha * e1[c] + beta * e2[c])
     hermitian_kx0(u, n)
     e = parseval_energy(u, n)
+    if not e > 0 and energy > 0:
+        raise ValueError(f"the {n}^3 grid's dealias ball holds no nonzero mode, so no field of kinetic energy "
+                         f"{energy} can be seeded on it; use a finer grid")
     if e > 0:
         u *= math.sqrt(energy / e)
     return u
@@ -123,6 +126,9 @@ def random_phase_scalar(n: int, side_length: float = 2 * math.pi, variance: floa
     p[keep] = amp * np.exp(1j * t)
     hermitian_kx0(p, n)
     v = 2.0 * parseval_energy(p[None], n)
+    if not v > 0 and variance > 0:
+        raise ValueError(f"the {n}^3 grid's dealias ball holds no nonzero mode, so no scalar of variance "
+                         f"{variance} can be seeded on it; use a finer grid")
     if v > 0:
         p *= math.sqrt(variance / v)
     return p

@@ -346,3 +346,14 @@ def test_stochastic_ou_every_seed_bit_reaches_the_stream(g):
     assert not np.array_equal(draw(-7), draw(7))
     assert not np.array_equal(draw(1), draw(1 + (1 << 40)))
     assert not np.array_equal(draw(1), draw(1 - (1 << 63)))
+
+
+def test_a_seed_on_a_grid_with_an_empty_ball_is_refused(g):
+    """test/regressions/05-edge-cases-05.jl: a grid whose dealias ball holds no nonzero mode must
+    not silently seed a zero field; the requested energy is realized otherwise."""
+    with pytest.raises(ValueError, match="no nonzero mode"):
+        g.random_phase_velocity(2, energy=0.5)
+    with pytest.raises(ValueError, match="no nonzero mode"):
+        g.random_phase_scalar(2, variance=0.5)
+    u = g.random_phase_velocity(16, energy=0.37, seed=3)
+    assert g.seeding.parseval_energy(u, 16) == pytest.approx(0.37, rel=1e-14)
