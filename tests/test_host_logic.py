@@ -357,3 +357,18 @@ def test_a_seed_on_a_grid_with_an_empty_ball_is_refused(g):
         g.random_phase_scalar(2, variance=0.5)
     u = g.random_phase_velocity(16, energy=0.37, seed=3)
     assert g.seeding.parseval_energy(u, 16) == pytest.approx(0.37, rel=1e-14)
+
+
+def test_integral_length_ignores_a_uniform_mean_flow(g):
+    """test/regressions/04-r5-free-modules-04.jl: the kappa = 0 shell is out of both sums, so a
+    mean-flow-only field is refused rather than given L11 = 0, and adding a mean flow leaves L11
+    as it was.  L11 of energy on one shell kappa is 3 pi / (4 kappa)."""
+    kappa = np.arange(6.0)
+    with pytest.raises(ValueError, match="no energy"):
+        g.integral_length((kappa, np.array([0.7, 0, 0, 0, 0, 0])))
+    e = np.array([0.0, 0.2, 0.5, 0.1, 0.05, 0.0])
+    plain = g.integral_length((kappa, e))
+    e_mean = e.copy()
+    e_mean[0] = 3.0
+    assert g.integral_length((kappa, e_mean)) == plain
+    assert g.integral_length((kappa, np.array([0, 0, 1.0, 0, 0, 0]))) == pytest.approx(3 * math.pi / 8, rel=1e-15)
