@@ -298,11 +298,11 @@ def test_y_radix_plans_agree_at_256(g):
         assert rel(b.state().velocity_spectral, a.state().velocity_spectral) < 1e-13
 
 
-def _member(g, n, dtype, nu, seed, m=0):
+def _member(g, n, dtype, nu, seed, m=0, unwrapped=False):
     dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
     cond = g.FlowConditions(viscosity=nu, schmidt=1.0)
     state = g.ResolvedFields(g.random_phase_velocity(n, seed=seed), g.random_phase_scalar(n, seed=seed + 1))
-    tr = g.Tracers(dom, number_of_particles=m, seed=seed + 2) if m else None
+    tr = g.Tracers(dom, number_of_particles=m, seed=seed + 2, unwrapped=unwrapped) if m else None
     return g.EnsembleMember(dom, cond, state, tr)
 
 
@@ -340,10 +340,10 @@ def test_ensemble_boxes_equal_their_standalone_runs_bitwise(g, dtype):
 
 def test_ensemble_tracers_under_ab3_equal_their_standalone_runs_bitwise(g):
     """The multistep push over each batch's trail: 4 steps of AB(3) in a mixed ensemble equal the
-    standalone runs bit for bit, positions and the whole trail."""
+    standalone runs bit for bit, positions, the unwrapped record and the whole trail."""
     sizes, nus = [32, 48, 32], [0.01, 0.02, 0.03]
     m, dt, steps = 64, 2e-3, 4
-    members = [_member(g, n, np.float32, nu, 300 + b, m) for b, (n, nu) in enumerate(zip(sizes, nus))]
+    members = [_member(g, n, np.float32, nu, 300 + b, m, unwrapped=True) for b, (n, nu) in enumerate(zip(sizes, nus))]
     with g.Ensemble(members, field_integration_scheme=g.IFLowStorageRK3, mean_gradient=1.0,
                     particle_integration_scheme=g.AB(3)) as ens:
         for s in range(steps):
@@ -357,6 +357,7 @@ def test_ensemble_tracers_under_ab3_equal_their_standalone_runs_bitwise(g):
                     alone.advance(dt, check=False)
                 box = ens.box(b)
                 assert np.array_equal(alone.tracer_positions(), box.tracer_positions()), b
+                assert np.array_equal(alone.unwrapped_positions(), box.unwrapped_positions()), b
                 va, _ = alone.tracer_samples()
                 ve, _ = box.tracer_samples()
                 assert np.array_equal(va, ve), b
