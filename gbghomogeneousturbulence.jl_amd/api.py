@@ -262,7 +262,15 @@ class FlowConditions:
 
 
 def predicted_viscosity(domain: FixedDomain, dissipation: float, kmax_eta: float = 1.5) -> float:
-    """nu = eps^{1/3} eta^{4/3} with eta = kmax_eta / kappa_max (src/SimulationRun.jl:647-655)."""
+    """nu = eps^{1/3} eta^{4/3} with eta = kmax_eta / kappa_max (src/SimulationRun.jl:647-655); a
+    non-positive or non-finite kmax_eta or dissipation is refused by name (src/SimulationRun.jl:648-649)."""
+    kmax_eta, dissipation = float(kmax_eta), float(dissipation)
+    if not (math.isfinite(kmax_eta) and kmax_eta > 0):
+        raise ValueError(f"`kmax_eta` must be finite and positive, got {kmax_eta}: it is the resolution criterion "
+                         "kappa_max eta, from which the viscosity is solved")
+    if not (math.isfinite(dissipation) and dissipation > 0):
+        raise ValueError(f"`dissipation` must be finite and positive, got {dissipation}: it is the dissipation "
+                         "rate eps the viscosity is solved for")
     eta = kmax_eta / domain.max_retained_wavenumber()
     return dissipation ** (1.0 / 3.0) * eta ** (4.0 / 3.0)
 

@@ -372,3 +372,13 @@ def test_integral_length_ignores_a_uniform_mean_flow(g):
     e_mean[0] = 3.0
     assert g.integral_length((kappa, e_mean)) == plain
     assert g.integral_length((kappa, np.array([0, 0, 1.0, 0, 0, 0]))) == pytest.approx(3 * math.pi / 8, rel=1e-15)
+
+
+@pytest.mark.parametrize("bad", [0.0, -1.0, math.inf, math.nan])
+def test_predicted_viscosity_refuses_a_bad_resolution_criterion_by_name(g, bad):
+    """test/regressions/04-r4-free-hot-path-04.jl: kappa_max eta must be finite and positive."""
+    dom = g.FixedDomain(grid_points_per_axis=16, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
+    with pytest.raises(ValueError, match="kmax_eta"):
+        g.predicted_viscosity(dom, 0.1, kmax_eta=bad)
+    with pytest.raises(ValueError, match="dissipation"):
+        g.predicted_viscosity(dom, bad if bad != 0.0 else -0.0, kmax_eta=1.5)
