@@ -382,3 +382,45 @@ def test_predicted_viscosity_refuses_a_bad_resolution_criterion_by_name(g, bad):
         g.predicted_viscosity(dom, 0.1, kmax_eta=bad)
     with pytest.raises(ValueError, match="dissipation"):
         g.predicted_viscosity(dom, bad if bad != 0.0 else -0.0, kmax_eta=1.5)
+
+
+def _amplification_2n(scheme, z):
+    """R(z) of a 2N scheme on y' = z y over one unit step: h <- a_i h + z q, q <- q + b_i h."""
+    a, b, _ = dns.scheme_coefficients(scheme)
+    q, h = 1.0 + 0j, 0j
+    for ai, bi in zip(a, b):
+        h = ai * h + z * q
+        q = q + bi * h
+    return q
+
+
+def test_the_cfl_ceiling_keeps_the_advective_rate_inside_the_stability_limit(g):
+    """test/observables_analytic.jl: C kappa_max dx is the advective rate times the step.  For the
+    three-stage third-order schemes it stays below sqrt(3), the imaginary-axis limit they share; for
+    IF-RK4 below the Carpenter-Kennedy scheme's own limit, found here from its coefficients.  At
+    that rate the amplification factor of the scheme's own coefficients is at most 1."""
+    dom = g.FixedDomain(grid_points_per_axis=64, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
+    dx = dom.grid_spacing
+
+    def rate(scheme):
+        return g.recommended_cfl(scheme, dom) * dom.max_retained_wavenumber() * dx
+
+    def ssp3(z):
+        return 1 + z + z * z / 2 + z ** 3 / 6
+
+    def limit(amp):
+        lo, hi = 0.0, 4.0
+        for _ in range(60):
+            mid = 0.5 * (lo + hi)
+            ok = all(abs(amp(1j * y)) <= 1 + 1e-12 for y in np.linspace(0, mid, 400))
+            lo, hi = (mid, hi) if ok else (lo, mid)
+        return lo
+
+    rk3 = limit(lambda z: _amplification_2n("rk3", z))
+    assert rk3 == pytest.approx(math.sqrt(3), rel=1e-6)          # every three-stage third-order scheme
+    assert limit(ssp3) == pytest.approx(math.sqrt(3), rel=1e-6)
+    for scheme, lim in ((g.IFLowStorageRK3, rk3), (g.IFSSPRK3, math.sqrt(3)),
+                        (g.IFLowStorageRK4, limit(lambda z: _amplification_2n("rk4", z)))):
+        assert rate(scheme) < lim, scheme
+    assert abs(_amplification_2n("rk3", 1j * rate(g.IFLowStorageRK3))) <= 1
+    assert abs(_amplification_2n("rk4", 1j * rate(g.IFLowStorageRK4))) <= 1
