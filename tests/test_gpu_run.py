@@ -110,3 +110,30 @@ def test_velocity_slice_heun_with_band_rescaling_and_an_inert_scalar(g, dtype, b
     assert not np.any(st.scalar_spectral)
     print(f"{np.dtype(dtype).name}: k {energies[0]:.4f} -> {energies[-1]:.4f}, band worst {worst_band:.2e}", flush=True)
     assert worst_band < band_tol
+
+
+def test_an_unforced_decaying_run_keeps_a_bounded_step_and_a_meaningful_clock(g):
+    """test/regressions/02-r5-free-modules-02.jl: as an unforced flow decays, max|u| falls and the
+    advective bound grows without limit, but the resolution bound 1 / (nu kappa_min^2) caps the
+    step; the clock stays finite and strictly increasing, and the energy never grows."""
+    n, nu = 16, 0.05
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=2 * math.pi, dealiasing=g.TwoThirds(),
+                        element_type=np.float64)
+    cap = 1.0 / nu
+    with g.Simulation(dom, conditions=g.FlowConditions(viscosity=nu, schmidt=1.0),
+                      state=g.ResolvedFields(g.random_phase_velocity(n, seed=7), None),
+                      field_integration_scheme=g.IFLowStorageRK3) as sim:
+        energies, times, steps = [sim.reductions()["kinetic_energy"]], [sim.time], []
+        for _ in range(400):
+            dt = sim.cfl_timestep()
+            steps.append(dt)
+            sim.advance(dt)
+            energies.append(sim.reductions()["kinetic_energy"])
+            times.append(sim.time)
+    print(f"k {energies[0]:.3e} -> {energies[-1]:.3e}, dt {steps[0]:.3e} -> {steps[-1]:.3e}, t {times[-1]:.3f}",
+          flush=True)
+    assert all(0 < s <= cap * (1 + 1e-15) for s in steps)
+    assert all(math.isfinite(t) for t in times) and all(b > a for a, b in zip(times, times[1:]))
+    assert all(b <= a * (1 + 1e-12) for a, b in zip(energies, energies[1:]))
+    assert energies[-1] < 1e-3 * energies[0]
+    assert steps[-1] == pytest.approx(cap, rel=1e-12)     # the resolution bound has taken over
