@@ -101,3 +101,14 @@ def test_below_scale_matches_the_oracle(g, n, dtype, tol):
     assert np.allclose(got["scales"], sc, rtol=1e-15)
     assert np.max(np.abs(got["kinetic_energy"] - kl)) <= tol * kl[-1]
     assert np.max(np.abs(got["dissipation"] - el)) <= tol * el[-1]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_skewness_of_a_zero_field_is_zero(g, dtype):
+    """test/regressions/04-edge-cases-04.jl: the strain-free branch gives 0, not 0/0."""
+    n = 16
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=2 * np.pi, dealiasing=g.TwoThirds(), element_type=dtype)
+    u = np.zeros((3, n, n, n // 2 + 1), dtype=np.complex128)
+    with g.Simulation(dom, conditions=g.FlowConditions(viscosity=0.02, schmidt=1.0),
+                      state=g.ResolvedFields(u, None)) as sim:
+        assert sim.derivative_skewness() == 0.0
