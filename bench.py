@@ -77,7 +77,7 @@ def algorithmic_bytes(n: int, real_bytes: int, stages: int, scalar: bool = True,
 
 def time_gpu(g, n: int, dtype, steps: int, warmup: int, H, seed: int = 2026, scheme=None):
     dom = g.FixedDomain(grid_points_per_axis=n, side_length=2 * math.pi, dealiasing=g.TwoThirds(), element_type=dtype)
-    nu = g.predicted_viscosity(dom, 0.1)
+    nu = g.predicted_viscosity(dom, 0.1, kmax_eta=1.5)
     u = g.random_phase_velocity(n, seed=seed)
     p = np.zeros((n, n, n // 2 + 1), dtype=np.complex128)
     cond = g.FlowConditions(viscosity=nu, schmidt=1.0,
@@ -145,7 +145,7 @@ def time_tracers(g, n: int, m: int, dtype, steps: int, H) -> dict:
     dom = g.FixedDomain(grid_points_per_axis=n, side_length=2 * math.pi, dealiasing=g.TwoThirds(), element_type=dtype)
     u = g.random_phase_velocity(n, seed=2026)
     p = np.zeros((n, n, n // 2 + 1), dtype=np.complex128)
-    cond = g.FlowConditions(viscosity=g.predicted_viscosity(dom, 0.1), schmidt=1.0)
+    cond = g.FlowConditions(viscosity=g.predicted_viscosity(dom, 0.1, kmax_eta=1.5), schmidt=1.0)
     tr = g.Tracers(dom, number_of_particles=m, seed=2028)
     dt = 1e-3
     total = 0.0
@@ -233,7 +233,7 @@ def _forced_flow(g, n: int, dtype, tracers: int = 0):
     ConstantPower eps = 0.1 on shells 1-2, nu for k_max eta = 1.5, a scalar with mean gradient
     G = 1 along z and zero initial fluctuation, IF-RK4; `tracers` seeded uniform tracers."""
     dom = g.FixedDomain(grid_points_per_axis=n, side_length=2 * math.pi, dealiasing=g.TwoThirds(), element_type=dtype)
-    cond = g.FlowConditions(viscosity=g.predicted_viscosity(dom, 0.1), schmidt=1.0,
+    cond = g.FlowConditions(viscosity=g.predicted_viscosity(dom, 0.1, kmax_eta=1.5), schmidt=1.0,
                             velocity_forcing=g.ConstantPower(dom, injection_rate=0.1, lowest_shell=1, highest_shell=2))
     u = g.random_phase_velocity(n, seed=2026)
     p = np.zeros((n, n, n // 2 + 1), dtype=np.complex128)
@@ -312,7 +312,7 @@ def time_adjoint(g, n: int = 128, steps: int = 200, every: int = 20, directions:
     def run(dtype):
         dom = g.FixedDomain(grid_points_per_axis=n, side_length=2 * math.pi, dealiasing=g.TwoThirds(),
                             element_type=dtype)
-        cond = g.FlowConditions(viscosity=g.predicted_viscosity(dom, 0.1), schmidt=1.0,
+        cond = g.FlowConditions(viscosity=g.predicted_viscosity(dom, 0.1, kmax_eta=1.5), schmidt=1.0,
                                 velocity_forcing=g.ConstantPower(dom, injection_rate=0.1, lowest_shell=1,
                                                                  highest_shell=2))
         u = g.random_phase_velocity(n, seed=2031)
@@ -407,7 +407,7 @@ def cpu_baseline(n_sample: int) -> dict:
     p = np.zeros((n_sample, n_sample, n_sample // 2 + 1), dtype=np.complex128)
     dom = g.FixedDomain(grid_points_per_axis=n_sample, side_length=2 * math.pi, dealiasing=g.TwoThirds(),
                         element_type=np.float64)
-    cond = dns.Conditions(viscosity=g.predicted_viscosity(dom, 0.1), schmidt=1.0, forcing=(0.1, 1, 2),
+    cond = dns.Conditions(viscosity=g.predicted_viscosity(dom, 0.1, kmax_eta=1.5), schmidt=1.0, forcing=(0.1, 1, 2),
                           mean_gradient=1.0)
     dep = dns.deposit(d, u, p)
     t0 = time.perf_counter()

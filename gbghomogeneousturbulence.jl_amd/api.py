@@ -261,8 +261,9 @@ class FlowConditions:
         return self.viscosity / self.schmidt
 
 
-def predicted_viscosity(domain: FixedDomain, dissipation: float, kmax_eta: float = 1.5) -> float:
-    """nu = eps^{1/3} eta^{4/3} with eta = kmax_eta / kappa_max (src/SimulationRun.jl:647-655); a
+def predicted_viscosity(domain: FixedDomain, dissipation: float, kmax_eta: float = 2.0) -> float:
+    """nu = eps^{1/3} eta^{4/3} with eta = kmax_eta / kappa_max (src/SimulationRun.jl:647-655); the
+    resolution criterion defaults to 2 as the reference's does (src/SimulationRun.jl:648).  A
     non-positive or non-finite kmax_eta or dissipation is refused by name (src/SimulationRun.jl:648-649)."""
     kmax_eta, dissipation = float(kmax_eta), float(dissipation)
     if not (math.isfinite(kmax_eta) and kmax_eta > 0):
@@ -919,10 +920,17 @@ class Simulation:
         """Put a stored trail back (read_simulation, ext/GBGHomogeneousTurbulenceHDF5Ext.jl:535-544):
         the earlier samples [n, 3, M] newest first with their times, then the newest measured again
         at the positions at t_next."""
+        if self.tracers is None:
+            raise ValueError("this run carries no tracers")
         samples = np.ascontiguousarray(samples, dtype=np.float64)
         times = np.ascontiguousarray(times, dtype=np.float64)
-        if samples.shape[0] != times.shape[0]:
-            raise ValueError("one time per stored sample")
+        if times.ndim != 1:
+            raise ValueError("the trail's times are one number per stored sample")
+        want = (times.shape[0], 3, self.tracers.count)
+        # ext/GBGHomogeneousTurbulenceHDF5Ext.jl:268-283: a sample that does not fit the trail is refused
+        if samples.shape != want:
+            raise ValueError(f"the stored samples do not fit the trail they are read into: expected shape {want} "
+                             f"(samples, components, tracers), got {samples.shape}")
         self._call("hdns_tracer_trail_restore", int(times.shape[0]), L.dptr(times), L.dptr(samples), float(t_next))
         self._call("hdns_deposit_tracers")
         self.trail_epoch = (self.fields_epoch, self.particles_epoch)
@@ -968,12 +976,34 @@ class Simulation:
         self.trail_epoch = (self.fields_epoch, self.particles_epoch)
         self.trail_time = t
 
+    def _check_trail_between_iterations(self) -> None:
+        """The checked read at an iteration's start (src/Cache.jl:342-349)."""
+        ahead = self.fields_epoch - self.particles_epoch
+        if ahead != 0:
+            raise RuntimeError(f"the fields have taken {ahead} step(s) that the tracers have not, so the two belong "
+                               "to different times. A run that has let them part cannot be repaired; build a new "
+                               "one from its state and tracers")
+        if self.trail_epoch != (self.fields_epoch, self.particles_epoch):
+            raise RuntimeError("the tracers were pushed and their new velocity sample was never taken: deposit "
+                               "the tracers before the next iteration")
+
     def advance(self, dt: float | None = None, check: bool = True) -> "Simulation":
         """One iteration (src/SimulationRun.jl:1155-1229): dt, step + deposit, traversal rebuild,
         tracer push and sample, then the blow-up guards and the clock."""
+        ps = self.particle_integration_scheme
+        if dt is not None and self.shortened_first_steps and ps is not None and ps.steps > 1:
+            # src/SimulationRun.jl:1158-1161
+            raise ValueError("this run was built with `shortened_first_steps = true`, which chooses the timestep "
+                             "itself while the particle scheme starts, and a fixed `dt` was given: drop `dt`, or "
+                             "build the run without `shortened_first_steps`")
         dt_ = self.cfl_timestep() if dt is None else float(dt)
         if not (math.isfinite(dt_) and dt_ > 0):
             raise ValueError(f"the timestep must be positive and finite, got {dt_}")
+        # src/SimulationRun.jl:1177-1182 (trail_between_iterations, src/Cache.jl:342-349): a run whose
+        # fields were stepped alone, or whose last push was never closed by a deposit, is refused
+        # before anything moves
+        if self.tracers is not None:
+            self._check_trail_between_iterations()
         t_new = self.time + dt_
         if not t_new > self.time:
             raise ValueError(f"the timestep {dt_} is too small to advance the clock, which reads {self.time}")

@@ -247,6 +247,21 @@ def write_simulation(path, sim, name: str = "simulation") -> None:
     store.save(path)
 
 
+def stored_trail_samples(trail, m: int) -> np.ndarray:
+    """The earlier velocity samples of a stored trail group, newest first, [n, 3, m].  Each stored
+    sample must hold one velocity per tracer; one that does not is refused by its number
+    (ext/GBGHomogeneousTurbulenceHDF5Ext.jl:268-283)."""
+    n = len(trail.attrs["times"])
+    for i in range(1, n + 1):
+        got = np.shape(trail[f"sample{i}"])
+        if got != (3, m):
+            raise ValueError(f"read_simulation: the stored sample {i} does not fit the trail it is read into: "
+                             f"expected shape {(3, m)} (components, tracers), got {got}")
+    if n == 0:
+        return np.zeros((0, 3, m))
+    return np.stack([np.asarray(trail[f"sample{i}"], dtype=np.float64) for i in range(n, 0, -1)])
+
+
 def read_simulation(path, name: str = "simulation", **simulation_kwargs):
     """read_simulation (ext/GBGHomogeneousTurbulenceHDF5Ext.jl:458-545): a Simulation that continues
     the stored run; `simulation_kwargs` pass device options (keep_vorticity, tracer_capacity, ...)."""
@@ -279,6 +294,7 @@ def read_simulation(path, name: str = "simulation", **simulation_kwargs):
         shortened = bool(a["shortened_first_steps"])
         trail = g.group("velocity_at_particles_trail", "read_simulation")
         _require_tag(trail, "read_simulation", "ParticleTrail")
+        trail_samples = stored_trail_samples(trail, tracers.count)
     cond = api.FlowConditions(viscosity=a["viscosity"], schmidt=a["schmidt"], velocity_forcing=vf, scalar_forcing=sf)
     stored_ou = {}
     for field, key in ((0, "modeled_velocity_forcing"), (1, "modeled_scalar_forcing")):
@@ -306,8 +322,7 @@ def read_simulation(path, name: str = "simulation", **simulation_kwargs):
         if trail is not None:
             times = list(trail.attrs["times"])                # oldest first
             n = len(times)
-            samples = np.stack([trail[f"sample{i}"] for i in range(n, 0, -1)]) if n else np.zeros((0, 3, 0))
-            sim.restore_trail(times[::-1], samples, sim.time)
+            sim.restore_trail(times[::-1], trail_samples, sim.time)
     except Exception:
         sim.close()
         raise

@@ -426,7 +426,7 @@ def test_fp32_bulk_against_fp64_at_1024_over_10_steps(g):
     runs = []
     for dtype in (np.float32, np.float64):
         dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
-        cond = g.FlowConditions(viscosity=g.predicted_viscosity(dom, 0.1), schmidt=1.0,
+        cond = g.FlowConditions(viscosity=g.predicted_viscosity(dom, 0.1, kmax_eta=1.5), schmidt=1.0,
                                 velocity_forcing=g.ConstantPower(dom, injection_rate=0.1, lowest_shell=1,
                                                                  highest_shell=2))
         runs.append(g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=1.0,

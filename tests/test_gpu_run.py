@@ -77,6 +77,30 @@ def test_tracer_trail_epochs_throw_or_serve(g):
         assert np.all(np.isfinite(v)) and np.all(np.isfinite(s))
 
 
+def test_advance_refuses_a_parted_run_before_anything_moves(g):
+    """src/SimulationRun.jl:1177-1182 (trail_between_iterations, src/Cache.jl:342-349): after a
+    field step taken alone, or a push never closed by a deposit, advance() raises before the fields
+    move: the state, the positions and the clock are left bit for bit as they were."""
+    n, dt = 32, 2e-3
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
+    for parting in ("fields", "push"):
+        sim, *_ = make_sim(g, n, np.float64, tracers=g.Tracers(dom, number_of_particles=64, seed=9))
+        with sim:
+            sim.advance(dt)
+            sim.step_fields(dt)
+            if parting == "push":
+                sim.advect(dt)
+            st0, x0 = sim.state(), sim.tracer_positions()
+            match = "taken 1 step" if parting == "fields" else "never taken"
+            with pytest.raises(RuntimeError, match=match):
+                sim.advance(dt)
+            st1, x1 = sim.state(), sim.tracer_positions()
+            assert np.array_equal(st0.velocity_spectral, st1.velocity_spectral)
+            assert np.array_equal(st0.scalar_spectral, st1.scalar_spectral)
+            assert np.array_equal(x0, x1)
+            assert sim.step == 1 and sim.time == dt
+
+
 @pytest.mark.parametrize("dtype,band_tol", [(np.float64, 1e-12), (np.float32, 2e-6)])
 def test_velocity_slice_heun_with_band_rescaling_and_an_inert_scalar(g, dtype, band_tol):
     """test/velocity_slice.jl: forced 32^3 HIT by IF-Heun, 300 steps at cfl 0.5, VarianceRescaling

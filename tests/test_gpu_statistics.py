@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 def _forced(g, n, dtype, seed=2026):
     dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
-    cond = g.FlowConditions(viscosity=g.predicted_viscosity(dom, 0.1), schmidt=1.0,
+    cond = g.FlowConditions(viscosity=g.predicted_viscosity(dom, 0.1, kmax_eta=1.5), schmidt=1.0,
                             velocity_forcing=g.ConstantPower(dom, injection_rate=0.1, lowest_shell=1, highest_shell=2))
     u = g.random_phase_velocity(n, seed=seed)
     p = np.zeros((n, n, n // 2 + 1), dtype=np.complex128)

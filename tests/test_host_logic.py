@@ -18,7 +18,7 @@ def test_predicted_viscosity_gives_kmax_eta_of_one_and_a_half(g, n, nu):
     """nu = eps^(1/3) eta^(4/3) with eta = 1.5 / kappa_max and kappa_max = (2/3)(N/2)(2 pi / L):
     at eps = 0.1 the values quoted for the configurations, and k_max eta = 1.5 back from nu."""
     dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=np.float64)
-    got = g.predicted_viscosity(dom, 0.1)
+    got = g.predicted_viscosity(dom, 0.1, kmax_eta=1.5)
     assert got == pytest.approx(nu, rel=5e-4)
     kmax = (2 / 3) * (n / 2)
     eta = (got ** 3 / 0.1) ** 0.25
@@ -424,3 +424,38 @@ def test_the_cfl_ceiling_keeps_the_advective_rate_inside_the_stability_limit(g):
         assert rate(scheme) < lim, scheme
     assert abs(_amplification_2n("rk3", 1j * rate(g.IFLowStorageRK3))) <= 1
     assert abs(_amplification_2n("rk4", 1j * rate(g.IFLowStorageRK4))) <= 1
+
+
+def test_a_stored_trail_sample_that_does_not_fit_is_refused_by_number(g, tmp_path):
+    """read_simulation refuses a trail sample whose shape is not (3, M) before any run is built
+    (ext/GBGHomogeneousTurbulenceHDF5Ext.jl:268-283); a fitting trail comes back newest first."""
+    from gbg_amd import checkpoint as ck
+    s = ck.Store()
+    tr = ck._tagged(s.root, "velocity_at_particles_trail", "ParticleTrail")
+    tr.attrs["times"] = [0.0, 0.1]
+    a, b = np.arange(12.0).reshape(3, 4), -np.arange(12.0).reshape(3, 4)
+    tr["sample1"], tr["sample2"] = a, b
+    s.save(tmp_path / "t.npz")
+    trail = ck.Store.load(tmp_path / "t.npz").root.group("velocity_at_particles_trail", "t")
+    got = ck.stored_trail_samples(trail, 4)
+    assert got.shape == (2, 3, 4) and np.array_equal(got[0], b) and np.array_equal(got[1], a)
+    with pytest.raises(ValueError, match="stored sample 1 does not fit"):
+        ck.stored_trail_samples(trail, 5)
+    tr["sample2"] = np.zeros((4, 3))   # an (M, 3) array must not be read as (3, M)
+    s.save(tmp_path / "u.npz")
+    trail = ck.Store.load(tmp_path / "u.npz").root.group("velocity_at_particles_trail", "t")
+    with pytest.raises(ValueError, match="stored sample 2 does not fit"):
+        ck.stored_trail_samples(trail, 4)
+
+
+def test_a_fixed_dt_under_a_shortened_multistep_start_is_refused(g):
+    """src/SimulationRun.jl:1158-1161: a fixed dt and shortened_first_steps are two rules for the
+    same steps; a multistep particle scheme refuses the pair before anything moves."""
+    sim = g.Simulation.__new__(g.Simulation)   # the guard runs before any device call
+    sim.shortened_first_steps = True
+    sim.particle_integration_scheme = g.AB(3)
+    with pytest.raises(ValueError, match="shortened_first_steps = true"):
+        sim.advance(0.01)
+    sim.particle_integration_scheme = g.ABM(2)
+    with pytest.raises(ValueError, match="drop `dt`"):
+        sim.advance(0.01)
