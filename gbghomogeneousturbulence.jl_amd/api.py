@@ -599,7 +599,8 @@ class Simulation:
                  field_integration_scheme=IFLowStorageRK4, tracers: Tracers | None = None,
                  particle_integration_scheme=None, mean_gradient: float = 0.0, cfl: float | None = None,
                  max_dt: float = math.inf, keep_vorticity: bool = False, keep_physical: bool = True,
-                 tracer_capacity: int | None = None, shortened_first_steps: bool = False):
+                 tracer_capacity: int | None = None, shortened_first_steps: bool = False,
+                 traversal_mode: str = "automatic"):
         carries_scalar = state.scalar_spectral is not None
         u, p = _state_arrays(state, domain)
         self.mean_gradient = float(mean_gradient)
@@ -613,6 +614,12 @@ class Simulation:
             raise ValueError("the particle integration scheme must be Heun(), ABM(k) or AB(k), got "
                              f"{self.particle_integration_scheme!r}")
         self.shortened_first_steps = bool(shortened_first_steps)
+        # src/SimulationRun.jl:242-244: the gathers at the tracers visit them in a tile order rebuilt
+        # every iteration (automatic), rebuilt by the caller (manual), or in tracer order (none)
+        traversal_mode = str(traversal_mode).lstrip(":")
+        if traversal_mode not in ("automatic", "manual", "none"):
+            raise ValueError(f"`traversal_mode` must be `:automatic`, `:manual` or `:none`; got {traversal_mode!r}")
+        self.traversal_mode = traversal_mode
         self.tracers = tracers
         self.cfl = recommended_cfl(field_integration_scheme, domain) if cfl is None else float(cfl)
         self.max_dt = float(max_dt)
@@ -639,7 +646,8 @@ class Simulation:
             self._call("hdns_state_upload", L.dptr(u), L.dptr(p) if p is not None else None)
             if tracers is not None:
                 self._call("hdns_tracers_upload", tracers.count, L.dptr(tracers.positions))
-                self._call("hdns_rebuild_traversal")
+                if self.traversal_mode != "none":   # src/SimulationRun.jl:921-922
+                    self._call("hdns_rebuild_traversal")
                 self._call("hdns_deposit_tracers")
                 self.trail_epoch = (self.fields_epoch, self.particles_epoch)
                 self.trail_time = 0.0
@@ -779,6 +787,17 @@ class Simulation:
         keys = np.zeros(self.tracers.count, dtype=np.int64)
         self._call("hdns_tracer_cells", keys.ctypes.data_as(ctypes.POINTER(c_int64)))
         return keys
+
+    def rebuild_traversal(self) -> "Simulation":
+        """rebuild_traversal!(sim) (src/SimulationRun.jl:1257-1266): rebuild the tracer traversal
+        order from the current positions; a run without one refuses."""
+        if self.tracers is None or self.traversal_mode == "none":
+            why = "without tracers" if self.tracers is None else "with `traversal_mode = :none`"
+            raise ValueError(f"this run carries no particle traversal order to rebuild: it was built {why}. Build it "
+                             "with `traversal_mode = :manual` to rebuild by hand, or leave the default, which "
+                             "rebuilds every iteration.")
+        self._call("hdns_rebuild_traversal")
+        return self
 
     def traversal_order(self) -> np.ndarray:
         order = np.zeros(self.tracers.count, dtype=np.int32)
@@ -978,14 +997,15 @@ class Simulation:
 
     def _check_trail_between_iterations(self) -> None:
         """The checked read at an iteration's start (src/Cache.jl:342-349)."""
-        ahead = self.fields_epoch - self.particles_epoch
-        if ahead != 0:
+        fe, pe = self.trail_epoch if self.trail_epoch is not None else (None, None)
+        if pe != self.particles_epoch:
+            raise RuntimeError("the tracers were pushed and their new velocity sample was never taken: deposit "
+                               "the tracers before the next iteration")
+        if fe != self.fields_epoch:
+            ahead = self.fields_epoch - fe
             raise RuntimeError(f"the fields have taken {ahead} step(s) that the tracers have not, so the two belong "
                                "to different times. A run that has let them part cannot be repaired; build a new "
                                "one from its state and tracers")
-        if self.trail_epoch != (self.fields_epoch, self.particles_epoch):
-            raise RuntimeError("the tracers were pushed and their new velocity sample was never taken: deposit "
-                               "the tracers before the next iteration")
 
     def advance(self, dt: float | None = None, check: bool = True) -> "Simulation":
         """One iteration (src/SimulationRun.jl:1155-1229): dt, step + deposit, traversal rebuild,
@@ -1009,7 +1029,8 @@ class Simulation:
             raise ValueError(f"the timestep {dt_} is too small to advance the clock, which reads {self.time}")
         self.step_fields(dt_)
         if self.tracers is not None:
-            self._call("hdns_rebuild_traversal")
+            if self.traversal_mode == "automatic":   # from the start-of-step positions (src/SimulationRun.jl:1197-1200)
+                self._call("hdns_rebuild_traversal")
             self.advect(dt_)
             self.deposit_tracers(t_new)
         if check:

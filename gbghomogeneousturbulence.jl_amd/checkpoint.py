@@ -295,6 +295,10 @@ def read_simulation(path, name: str = "simulation", **simulation_kwargs):
         trail = g.group("velocity_at_particles_trail", "read_simulation")
         _require_tag(trail, "read_simulation", "ParticleTrail")
         trail_samples = stored_trail_samples(trail, tracers.count)
+    mode = str(simulation_kwargs.get("traversal_mode", "automatic")).lstrip(":")
+    if tracers is None and mode != "automatic":   # ext/GBGHomogeneousTurbulenceHDF5Ext.jl:507-511
+        raise ValueError(f"read_simulation: `traversal_mode = :{mode}` was given, but group {name!r} stores no "
+                         "tracers to order the gathers over")
     cond = api.FlowConditions(viscosity=a["viscosity"], schmidt=a["schmidt"], velocity_forcing=vf, scalar_forcing=sf)
     stored_ou = {}
     for field, key in ((0, "modeled_velocity_forcing"), (1, "modeled_scalar_forcing")):

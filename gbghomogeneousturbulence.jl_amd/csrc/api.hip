@@ -14,6 +14,7 @@
 #include "context.hpp"
 #include "forcing.hpp"
 #include "observe.hpp"
+#include "sort.hpp"
 
 using namespace hdns;
 
@@ -185,7 +186,7 @@ void release(Ctx &c) {
   c.prof_ev.clear();
   void *ptrs[] = {c.uh, c.hu, c.ph, c.hp, c.bufa, c.bufb, c.phys, c.physw, c.tw, c.partials, c.scalars, c.rpart,
                   c.usrc, c.psrc,
-                  c.maxbits, c.spec_bins, c.pos, c.upos, c.vel, c.phis, c.order, c.tile_counts, c.tile_cursor,
+                  c.maxbits, c.spec_bins, c.pos, c.upos, c.vel, c.phis, c.order, c.skeys, c.svals, c.sort_temp,
                   c.visc_d, c.kappa_d};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -606,7 +607,7 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
     if (z != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, std::string("clearing the scalars: ") + hipGetErrorString(z));
   }
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.maxbits, B * sizeof(unsigned long long));
-  if (s == HDNS_OK) s = alloc(ctx, (void **)&c.spec_bins, c.nbins * sizeof(double));
+  if (s == HDNS_OK) s = alloc(ctx, (void **)&c.spec_bins, (size_t)c.nbins * (1 + SPEC_CHUNKS) * sizeof(double));
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.visc_d, B * sizeof(double));
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.kappa_d, B * sizeof(double));
   if (s == HDNS_OK && k.tracer_capacity > 0) {
@@ -616,8 +617,15 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
     if (s == HDNS_OK) s = alloc(ctx, &c.vel, B * (size_t)c.trail_cap * 3 * cap * realb);
     if (s == HDNS_OK && k.has_scalar) s = alloc(ctx, &c.phis, B * cap * realb);
     if (s == HDNS_OK) s = alloc(ctx, (void **)&c.order, B * cap * sizeof(int));
-    if (s == HDNS_OK) s = alloc(ctx, (void **)&c.tile_counts, B * c.ntiles * sizeof(int));
-    if (s == HDNS_OK) s = alloc(ctx, (void **)&c.tile_cursor, B * c.ntiles * sizeof(int));
+    if (s == HDNS_OK) s = alloc(ctx, (void **)&c.skeys, 2 * B * cap * sizeof(uint32_t));
+    if (s == HDNS_OK) s = alloc(ctx, (void **)&c.svals, 2 * B * cap * sizeof(int32_t));
+    if (s == HDNS_OK) {   // the sort's scratch for every tracer slot of every box
+      while ((1ull << c.sort_key_bits) < (unsigned long long)B * c.ntiles) ++c.sort_key_bits;
+      e = tile_sort_pairs(nullptr, &c.sort_temp_bytes, c.skeys, c.skeys + B * cap, c.svals, c.svals + B * cap,
+                          (int64_t)(B * cap), c.sort_key_bits, nullptr, nullptr);
+      if (e != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, std::string("sizing the traversal sort: ") + hipGetErrorString(e));
+    }
+    if (s == HDNS_OK) s = alloc(ctx, &c.sort_temp, std::max<size_t>(c.sort_temp_bytes, 256));
   }
   if (s == HDNS_OK) {
     std::vector<double> kap(nbox);
@@ -1189,6 +1197,22 @@ hdns_status hdns_below_scale(hdns_ctx *ctx, double *scales, double *kinetic_ener
   return HDNS_OK;
 }
 
+// The spectrum's sums into the device buffers partials [nbins][SPEC_CHUNKS] and bins [nbins], then
+// E(kappa_n) = bins / (N^6 * width) into out[0..nshells) (zeros past nbins), in place.
+static hdns_status spectrum_into(hdns_ctx *ctx, int32_t which, double *partials, double *bins, double *out,
+                                 int32_t nshells) {
+  Ctx &c = ctx->c;
+  c.pipe->spectrum(c, which, partials, bins);
+  const hdns_status s = launched(ctx);
+  if (s != HDNS_OK) return s;
+  HIP_TRY(ctx, hipMemcpyAsync(out, bins, c.nbins * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  const double norm = pow((double)c.geo.n, 6.0);
+  const double width = which == HDNS_FIELD_VELOCITY ? 2.0 * c.geo.k0 : c.geo.k0;
+  for (int i = 0; i < nshells; ++i) out[i] = i < c.nbins ? out[i] / norm / width : 0.0;
+  return HDNS_OK;
+}
+
 hdns_status hdns_spectrum(hdns_ctx *ctx, int32_t which, double *out, int32_t nshells) {
   if (!ctx || !out) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "out must not be NULL");
   Ctx &c = ctx->c;
@@ -1199,17 +1223,7 @@ hdns_status hdns_spectrum(hdns_ctx *ctx, int32_t which, double *out, int32_t nsh
   if (nshells < c.nbins)
     return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "nshells must be at least " + std::to_string(c.nbins));
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
-  zero_f64(c, c.spec_bins, c.nbins * sizeof(double));
-  c.pipe->spectrum(c, which, c.spec_bins);
-  hdns_status s = launched(ctx);
-  if (s != HDNS_OK) return s;
-  std::vector<double> bins(c.nbins);
-  HIP_TRY(ctx, hipMemcpyAsync(bins.data(), c.spec_bins, c.nbins * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
-  const double norm = pow((double)c.geo.n, 6.0);
-  const double width = which == HDNS_FIELD_VELOCITY ? 2.0 * c.geo.k0 : c.geo.k0;
-  for (int i = 0; i < nshells; ++i) out[i] = i < c.nbins ? bins[i] / norm / width : 0.0;
-  return HDNS_OK;
+  return spectrum_into(ctx, which, c.spec_bins + c.nbins, c.spec_bins, out, nshells);
 }
 
 hdns_status hdns_field_download(hdns_ctx *ctx, int32_t which, double *out) {
@@ -1343,7 +1357,8 @@ hdns_status hdns_rebuild_traversal(hdns_ctx *ctx) {
   Ctx &c = ctx->c;
   if (c.cfg.tracer_capacity == 0) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "this context carries no tracers");
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
-  c.pipe->tile_sort(c);
+  const hipError_t e = c.pipe->tile_sort(c);
+  if (e != hipSuccess) return fail(ctx, HDNS_ERR_DEVICE, std::string("traversal sort: ") + hipGetErrorString(e));
   c.order_valid = c.m > 0;
   return launched(ctx);
 }

@@ -38,13 +38,14 @@ struct Pipeline {
   virtual void zi(Ctx &c) = 0;
   virtual void yinv(Ctx &c) = 0;
   virtual void xpass(Ctx &c, const XArgs &xa) = 0;
-  virtual void spectrum(Ctx &c, int which, double *bins) = 0;
+  // the shell spectrum's sums of the selected box: partials [nbins][SPEC_CHUNKS], then bins [nbins]
+  virtual void spectrum(Ctx &c, int which, double *partials, double *bins) = 0;
   virtual void advect(Ctx &c, const AdamsArgs &aa) = 0;
   // out [3][m] of the selected box <- sum_j w_j (trail sample j), in the element type
   virtual void trail_combine(Ctx &c, const TrailCombine &tc, void *out) = 0;
   virtual void sample(Ctx &c) = 0;
   virtual void cell_keys(Ctx &c, int64_t *keys) = 0;
-  virtual void tile_sort(Ctx &c) = 0;
+  virtual hipError_t tile_sort(Ctx &c) = 0;   // the traversal order of every box
   // the adjoint step (one box): J's adjoint from the current state, the transposed closing
   // projections, and one stage's transpose
   virtual void adj_objective(Ctx &c) = 0;
@@ -97,13 +98,19 @@ struct Ctx {
   double *rpart = nullptr;     // VarianceRescaling's block sums
   void *usrc = nullptr, *psrc = nullptr;   // StochasticOU's band force of the step, u and phi (FP64)
   unsigned long long *maxbits = nullptr;
-  double *spec_bins = nullptr;
+  double *spec_bins = nullptr;   // [nbins] shell sums, then [nbins][SPEC_CHUNKS] partials
   int nbins = 0;
   // tracers
   int64_t m = 0;
   void *pos = nullptr, *vel = nullptr, *phis = nullptr;
   void *upos = nullptr;        // the unwrapped record [box][3][cap], when tracked; else null
-  int *order = nullptr, *tile_counts = nullptr, *tile_cursor = nullptr;
+  int *order = nullptr;
+  // the traversal sort's double buffers [2][box][cap] of keys and tracer indices, and its scratch
+  uint32_t *skeys = nullptr;
+  int32_t *svals = nullptr;
+  void *sort_temp = nullptr;
+  size_t sort_temp_bytes = 0;
+  int sort_key_bits = 1;       // bits of the largest key, nbox * ntiles - 1
   int ntiles = 0;
   bool order_valid = false;
   // the velocity trail at the tracers (src/Particles.jl:404-526): a ring of trail_cap slots on the

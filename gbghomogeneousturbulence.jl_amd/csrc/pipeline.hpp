@@ -9,6 +9,7 @@
 #include "adjoint.hpp"
 #include "context.hpp"
 #include "observe.hpp"
+#include "sort.hpp"
 #include "tracers.hpp"
 #include "xpass_wave.hpp"
 
@@ -285,12 +286,15 @@ struct PipelineImpl final : Pipeline {
     else c.sched[3] ? xpass_v<kXThreads[1]>(c, xa) : xpass_v<kXThreads[0]>(c, xa);
   }
 
-  void spectrum(Ctx &c, int which, double *bins) override {   // the selected box
+  void spectrum(Ctx &c, int which, double *partials, double *bins) override {   // the selected box
     const size_t comp = c.comp;
     const cx<T> *f = which == HDNS_FIELD_SCALAR ? (const cx<T> *)c.ph + c.box * comp
                                                 : (const cx<T> *)c.uh + c.box * 3 * comp;
     const int nc = which == HDNS_FIELD_SCALAR ? 1 : 3;
-    hipLaunchKernelGGL((k_spectrum<T>), dim3(1024), dim3(256), 0, c.stream, f, nc, c.geo, bins);
+    hipLaunchKernelGGL((k_spectrum_shells<T>), dim3(c.nbins, SPEC_CHUNKS), dim3(SPEC_THREADS), 0, c.stream, f, nc,
+                       c.geo, partials);
+    hipLaunchKernelGGL(k_spectrum_sum, dim3((c.nbins + 255) / 256), dim3(256), 0, c.stream, (const double *)partials,
+                       c.nbins, bins);
   }
   TracerArgs targs(const Ctx &c) const {
     return TracerArgs{c.m, N, c.cfg.side_length, c.cfg.tracer_capacity, c.box_phys, c.ntiles,
@@ -322,15 +326,21 @@ struct PipelineImpl final : Pipeline {
     hipLaunchKernelGGL((k_cell_keys<T>), dim3(tblocks(c)), dim3(256), 0, c.stream,
                        (const T *)c.pos + (size_t)c.box * 3 * c.cfg.tracer_capacity, keys, targs(c));
   }
-  void tile_sort(Ctx &c) override {
-    if (c.m == 0) return;
-    hipLaunchKernelGGL(k_fill_i32, dim3(256), dim3(256), 0, c.stream, c.tile_counts, 0, (int64_t)c.ntiles * c.nbox);
-    hipLaunchKernelGGL((k_tile_count<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const T *)c.pos,
-                       c.tile_counts, targs(c));
-    hipLaunchKernelGGL(k_tile_scan, dim3(c.nbox), dim3(1024), 0, c.stream, (const int *)c.tile_counts, c.tile_cursor,
-                       c.ntiles);
-    hipLaunchKernelGGL((k_tile_scatter<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const T *)c.pos,
-                       c.tile_cursor, c.order, targs(c));
+  hipError_t tile_sort(Ctx &c) override {
+    if (c.m == 0) return hipSuccess;
+    const size_t per = (size_t)c.nbox * c.cfg.tracer_capacity;
+    uint32_t *ka = c.skeys, *kb = c.skeys + per;
+    int32_t *va = c.svals, *vb = c.svals + per;
+    hipLaunchKernelGGL((k_tile_keys<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const T *)c.pos, ka, va,
+                       targs(c));
+    bool in_b = false;
+    size_t bytes = c.sort_temp_bytes;
+    const hipError_t e = tile_sort_pairs(c.sort_temp, &bytes, ka, kb, va, vb, (int64_t)c.nbox * c.m, c.sort_key_bits,
+                                         c.stream, &in_b);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_order_place, dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const int32_t *)(in_b ? vb : va),
+                       c.order, c.m, (int64_t)c.cfg.tracer_capacity);
+    return hipSuccess;
   }
 };
 

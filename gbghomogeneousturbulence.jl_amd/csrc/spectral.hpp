@@ -592,30 +592,65 @@ static __global__ void k_finalize(const double *__restrict__ partials, double *_
 }
 
 // Shell-binned spectrum of the compact state (src/Observables.jl:502-514): shell n >= 0 of a mode
-// is the integer with n - 1/2 <= |m| < n + 1/2.  Atomic FP64 accumulation: a diagnostic off the
-// step, reproducible to the last bit only up to summation order.
+// is the integer with n - 1/2 <= |m| < n + 1/2, i.e. (2n - 1)^2 <= 4|m|^2 < (2n + 1)^2.  Each
+// block sums one shell over a fixed share of its (my, mz) rows: along a row the shell's modes are
+// one run of kx, found from integer bounds.  A thread adds its rows in order and the block adds
+// its threads in a fixed tree, so every partial, and the spectrum, is the same to the last bit on
+// every call.  Parseval weights: 1 on the kx = 0 and kx = N/2 planes, 2 inside.
+constexpr int SPEC_CHUNKS = 16;    // blocks per shell
+constexpr int SPEC_THREADS = 256;
+
 template <typename T>
-__global__ void k_spectrum(const cx<T> *__restrict__ f, int ncomp, Geo g, double *__restrict__ bins) {
-  const int K = g.k;
-  const size_t total = (size_t)K * K * g.kxp;
-  const size_t comp = total;
-  for (size_t l = blockIdx.x * (size_t)blockDim.x + threadIdx.x; l < total; l += (size_t)gridDim.x * blockDim.x) {
-    const int mx = (int)(l % g.kxp);
-    if (mx > g.m) continue;
-    const int ky_idx = (int)((l / g.kxp) % K), kz_idx = (int)(l / ((size_t)g.kxp * K));
-    const int my = signed_of(ky_idx, g), mz = signed_of(kz_idx, g);
-    const int four = 4 * (mx * mx + my * my + mz * mz);
-    int n = (int)floor(sqrt((double)four) * 0.5 + 0.5);
-    while (n > 0 && (2 * n - 1) * (2 * n - 1) > four) --n;
-    while ((2 * n + 1) * (2 * n + 1) <= four) ++n;
-    double e2 = 0.0;
-    for (int c = 0; c < ncomp; ++c) {
-      const cx<T> v = f[c * comp + l];
-      e2 += (double)v.re * (double)v.re + (double)v.im * (double)v.im;
+__global__ void __launch_bounds__(SPEC_THREADS) k_spectrum_shells(const cx<T> *__restrict__ f, int ncomp, Geo g,
+                                                                   double *__restrict__ partials) {
+  const int n = blockIdx.x, s = blockIdx.y, t = threadIdx.x;
+  const int L = min(n, g.m), side = 2 * L + 1;
+  const long rows = (long)side * side;
+  const int hi2 = (2 * n + 1) * (2 * n + 1), lo2 = n > 0 ? (2 * n - 1) * (2 * n - 1) : 0;
+  const size_t comp = comp_of(g);
+  double acc = 0.0;
+  for (long j = s + (long)SPEC_CHUNKS * t; j < rows; j += (long)SPEC_CHUNKS * SPEC_THREADS) {
+    const int my = (int)(j % side) - L, mz = (int)(j / side) - L;
+    const int r4 = 4 * (my * my + mz * mz);
+    if (r4 >= hi2) continue;
+    int lo = 0;
+    if (r4 < lo2) {   // the least mx with 4 (mx^2 + r^2) >= (2n - 1)^2
+      lo = (int)sqrt((double)(lo2 - r4) * 0.25);
+      while (4 * lo * lo + r4 < lo2) ++lo;
+      while (lo > 0 && 4 * (lo - 1) * (lo - 1) + r4 >= lo2) --lo;
     }
-    const double w = mx == 0 ? 1.0 : 2.0;
-    if (e2 != 0.0) atomicAdd(&bins[n], w * e2);
+    int hi = (int)sqrt((double)(hi2 - r4) * 0.25);   // the greatest mx with 4 (mx^2 + r^2) < (2n + 1)^2
+    while (hi >= 0 && 4 * hi * hi + r4 >= hi2) --hi;
+    while (4 * (hi + 1) * (hi + 1) + r4 < hi2) ++hi;
+    hi = min(hi, g.m);
+    const int ky_idx = my >= 0 ? my : my + g.k, kz_idx = mz >= 0 ? mz : mz + g.k;
+    const size_t row = ((size_t)kz_idx * g.k + ky_idx) * g.kxp;
+    for (int mx = lo; mx <= hi; ++mx) {
+      double e2 = 0.0;
+      for (int c = 0; c < ncomp; ++c) {
+        const cx<T> v = f[c * comp + row + mx];
+        e2 += (double)v.re * (double)v.re + (double)v.im * (double)v.im;
+      }
+      acc += (mx == 0 || 2 * mx == g.n) ? e2 : 2.0 * e2;
+    }
   }
+  __shared__ double red[SPEC_THREADS];
+  red[t] = acc;
+  __syncthreads();
+  for (int off = SPEC_THREADS / 2; off > 0; off >>= 1) {
+    if (t < off) red[t] += red[t + off];
+    __syncthreads();
+  }
+  if (t == 0) partials[(size_t)n * SPEC_CHUNKS + s] = red[0];
+}
+
+// bins[n] = the shell's partials added in chunk order.
+static __global__ void k_spectrum_sum(const double *__restrict__ partials, int nbins, double *__restrict__ bins) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nbins) return;
+  double s = 0.0;
+  for (int c = 0; c < SPEC_CHUNKS; ++c) s += partials[(size_t)n * SPEC_CHUNKS + c];
+  bins[n] = s;
 }
 
 // The frame tables as the device computes them: signed indices of the three axes and the mask

@@ -1,6 +1,6 @@
 // Tracer kernels: the Catmull-Rom gather, the Adams pushes (Heun = ABM(1), ABM(k), AB(k)) with the
 // periodic wrap, the closing sample into the velocity trail, the trail's time derivative, the
-// canonical cell keys, and the counting sort that builds the traversal order.
+// canonical cell keys, and the keys and placement of the traversal order's stable sort.
 //
 // Positions and samples are stored [3][M] (the reference's (M, 3) column-major PointSet matrix).
 // Grid point (i, j, k) sits at (i, j, k) * L/N; the field is [c][z][y][x].  The cell of x is
@@ -238,8 +238,10 @@ __global__ void k_cell_keys(const T *__restrict__ pos, int64_t *__restrict__ key
   keys[p] = ((int64_t)k * ta.n + j) * ta.n + i;
 }
 
-// Traversal order (src/Particles.jl:220-271): a counting sort of the tracers by 8^3-cell tile.
-// The order only reorders the visits, so every gather gives the same values with or without it.
+// Traversal order (src/Particles.jl:220-271): the tracers sorted by 8^3-cell tile, and within a
+// tile by index.  The sort key of tracer p of box b is b * ntiles + tile, so one stable sort of the
+// (key, p) pairs of every box leaves box b's order in its own run of m entries.  The order only
+// reorders the visits, so every gather gives the same values with or without it.
 template <typename T>
 __device__ __forceinline__ int tile_of(const T *pos, int64_t m, int64_t p, int n, T inv_dx) {
   const int nt = n >> 3;
@@ -250,55 +252,23 @@ __device__ __forceinline__ int tile_of(const T *pos, int64_t m, int64_t p, int n
 }
 
 template <typename T>
-__global__ void k_tile_count(const T *__restrict__ pos, int *__restrict__ counts, TracerArgs ta) {
+__global__ void k_tile_keys(const T *__restrict__ pos, uint32_t *__restrict__ keys, int32_t *__restrict__ vals,
+                            TracerArgs ta) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= ta.m) return;
   const int box = blockIdx.y;
   pos += box * 3 * ta.cap;
-  counts += (size_t)box * ta.ntiles;
-  atomicAdd(&counts[tile_of(pos, ta.m, p, ta.n, T(ta.n / ta.side_length))], 1);
+  const size_t i = (size_t)box * ta.m + p;
+  keys[i] = (uint32_t)box * (uint32_t)ta.ntiles + (uint32_t)tile_of(pos, ta.m, p, ta.n, T(ta.n / ta.side_length));
+  vals[i] = (int32_t)p;
 }
 
-// Exclusive scan of the tile counts into the cursors, one block of 1024 threads per box.
-static __global__ void k_tile_scan(const int *__restrict__ counts, int *__restrict__ cursor, int ntiles) {
-  __shared__ int part[1024];
-  const int tid = threadIdx.x;
-  counts += (size_t)blockIdx.x * ntiles;
-  cursor += (size_t)blockIdx.x * ntiles;
-  const int chunk = (ntiles + 1023) / 1024;
-  const int lo = tid * chunk, hi = min(ntiles, lo + chunk);
-  int s = 0;
-  for (int i = lo; i < hi; ++i) s += counts[i];
-  part[tid] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = tid >= off ? part[tid - off] : 0;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
-  }
-  int run = part[tid] - s;
-  for (int i = lo; i < hi; ++i) {
-    cursor[i] = run;
-    run += counts[i];
-  }
-}
-
-template <typename T>
-__global__ void k_tile_scatter(const T *__restrict__ pos, int *__restrict__ cursor, int *__restrict__ order,
-                               TracerArgs ta) {
+// Box b's sorted run [b m, (b + 1) m) into its order slots [b cap, b cap + m).
+static __global__ void k_order_place(const int32_t *__restrict__ sorted, int *__restrict__ order, int64_t m, int64_t cap) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (p >= ta.m) return;
+  if (p >= m) return;
   const int box = blockIdx.y;
-  pos += box * 3 * ta.cap;
-  cursor += (size_t)box * ta.ntiles;
-  order += box * ta.cap;
-  const int slot = atomicAdd(&cursor[tile_of(pos, ta.m, p, ta.n, T(ta.n / ta.side_length))], 1);
-  order[slot] = (int)p;
-}
-
-static __global__ void k_fill_i32(int *p, int v, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+  order[box * cap + p] = sorted[box * m + p];
 }
 
 }  // namespace hdns

@@ -149,7 +149,8 @@ hdns_status hdns_state_distance(hdns_ctx *a, hdns_ctx *b, double *out);
  * pencil buffers and redoes the deposit. */
 hdns_status hdns_derivative_skewness(hdns_ctx *ctx, double *out);
 /* Shell spectrum E(kappa_n), n = 0..nshells-1 (src/Observables.jl:502-514); which = velocity or
- * scalar.  nshells must be >= floor(sqrt(3)*N/2)+2. */
+ * scalar.  nshells must be >= floor(sqrt(3)*N/2)+2.  The FP64 shell sums run in a fixed order, so
+ * two calls on one state agree to the last bit. */
 hdns_status hdns_spectrum(hdns_ctx *ctx, int32_t which, double *out, int32_t nshells);
 /* A deposited physical field in the reference layout (velocity_physical / scalar_physical /
  * vorticity_physical, src/Cache.jl:203-240); the context must keep it (store_physical). */
@@ -166,7 +167,8 @@ hdns_status hdns_tracers_unwrapped_download(hdns_ctx *ctx, double *positions);
 /* Set the unwrapped record of the current M tracers (read_tracers, ext/GBGHomogeneousTurbulenceHDF5Ext.jl:236-241),
  * after hdns_tracers_upload has set the positions. */
 hdns_status hdns_tracers_unwrapped_upload(hdns_ctx *ctx, int64_t m, const double *positions);
-/* rebuild_traversal! (src/Particles.jl:268-271): counting sort of the tracers by cell tile. */
+/* rebuild_traversal! (src/Particles.jl:268-271): the tracers sorted by 8^3-cell tile, and within a
+ * tile by index (a stable sort), so the order depends on the positions alone. */
 hdns_status hdns_rebuild_traversal(hdns_ctx *ctx);
 /* advect!(scheme, ...) (src/ParticlePropagator.jl:287-372): the Adams predictor over the trail's
  * newest J samples with weights from their actual times, and for ABM (or while the trail holds

@@ -199,3 +199,51 @@ def test_trail_guards(g):
         sim.advect(1e-3)
         with pytest.raises(Exception):   # pushed twice without a sample in between
             sim._call("hdns_advect", 1e-3)
+
+
+def _tracked(g, mode, scheme, dtype, steps=6, m=300, n=16):
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
+    u, p = g.random_phase_velocity(n, seed=3), g.random_phase_scalar(n, seed=4)
+    sim = g.Simulation(dom, conditions=g.FlowConditions(viscosity=0.05, schmidt=1.0), state=g.ResolvedFields(u, p),
+                       mean_gradient=1.0, tracers=g.Tracers(dom, number_of_particles=m, seed=5, unwrapped=True),
+                       particle_integration_scheme=scheme, traversal_mode=mode)
+    for _ in range(steps):
+        sim.advance(0.01)
+    return sim
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_the_traversal_order_is_a_value_preserving_accelerator(g, dtype):
+    """After test/particles.jl:389-424: a run with the traversal order, rebuilt every iteration or
+    left stale under manual, is bitwise the run without it (positions, unwrapped positions, every
+    trail sample and time); a rebuilt order equals a freshly built one from the same positions and
+    sorts the tracers by tile, and within a tile by index."""
+    n, m = 16, 300
+    for scheme in (g.Heun(), g.AB(3), g.ABM(3)):
+        off = _tracked(g, "none", scheme, dtype)
+        with off:
+            with pytest.raises(ValueError, match="traversal_mode = :none"):
+                off.rebuild_traversal()
+            x_off, w_off, t_off = off.tracer_positions(), off.unwrapped_positions(), off.trail_times()
+            trail_off = [off.trail_sample(-j) for j in range(len(t_off))]
+            for mode in ("automatic", "manual"):
+                with _tracked(g, mode, scheme, dtype) as on:
+                    assert np.array_equal(on.tracer_positions(), x_off)
+                    assert np.array_equal(on.unwrapped_positions(), w_off)
+                    assert on.trail_times() == t_off
+                    for j, v in enumerate(trail_off):
+                        assert np.array_equal(on.trail_sample(-j), v)
+    with _tracked(g, "manual", g.Heun(), dtype) as man:
+        man.rebuild_traversal()
+        rebuilt = man.traversal_order()
+        x = man.tracer_positions()
+        dom = man.domain
+        with g.Simulation(dom, conditions=man.conditions, state=man.state(), mean_gradient=1.0,
+                          tracers=g.Tracers(dom, positions=x)) as fresh:
+            assert np.array_equal(fresh.tracer_positions(), x)
+            assert np.array_equal(rebuilt, fresh.traversal_order())
+        assert np.array_equal(np.sort(rebuilt), np.arange(m))
+        keys = trc.cell_keys(x.T, n, L, dtype)
+        i, j, k = keys % n, keys // n % n, keys // (n * n)
+        tiles = (k // 8 * (n // 8) + j // 8) * (n // 8) + i // 8
+        assert np.array_equal(rebuilt, np.argsort(tiles, kind="stable"))

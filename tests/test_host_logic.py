@@ -459,3 +459,46 @@ def test_a_fixed_dt_under_a_shortened_multistep_start_is_refused(g):
     sim.particle_integration_scheme = g.ABM(2)
     with pytest.raises(ValueError, match="drop `dt`"):
         sim.advance(0.01)
+
+
+@pytest.mark.parametrize("n", [64, 48])
+def test_the_seeded_spectrum_follows_k4_exp_gaussian_with_half_unit_energy(g, n):
+    """The benchmark seed: every mode off the kx = 0 and Nyquist planes carries |u_hat|^2 in exact
+    proportion to k^2 exp(-2 (k/4)^2), i.e. E(k) ~ k^4 exp(-2 (k/4)^2) once the 4 pi k^2 modes of a
+    shell are summed; the field is solenoidal, zero-mean, inside the closed 2/3 ball, and its kinetic
+    energy is 1/2.  The shell spectrum, its log-slope and its peak follow the same law."""
+    u = g.random_phase_velocity(n, seed=2026)
+    d = dns.Domain(n)
+    shape = (n, n, n // 2 + 1)                                     # [z][y][x]
+    mx = np.broadcast_to(d.mx[None, None, :], shape)
+    my = np.broadcast_to(d.my[None, :, None], shape)
+    mz = np.broadcast_to(d.mz[:, None, None], shape)
+    k2 = (mx * mx + my * my + mz * mz).astype(np.float64)
+    e_mode = np.sum(np.abs(u) ** 2, axis=0)
+    assert np.all(e_mode[~d.mask] == 0) and e_mode[0, 0, 0] == 0
+    div = mx * u[0] + my * u[1] + mz * u[2]
+    assert np.max(np.abs(div)) < 1e-12 * np.max(np.abs(u))
+    law = k2 * np.exp(-2.0 * k2 / 16.0)
+    interior = d.mask & (mx > 0) & (mx < n // 2)
+    ratio = e_mode[interior] / law[interior]
+    assert np.max(np.abs(ratio / ratio[0] - 1)) < 1e-12           # exact, mode by mode
+    from gbg_amd.seeding import parseval_energy
+    assert parseval_energy(u, n) == pytest.approx(0.5, rel=1e-14)
+    # the shell spectrum E(kappa_n), shells n - 1/2 <= |m| < n + 1/2, Parseval weights
+    w = np.where((mx == 0) | (mx == n // 2), 1.0, 2.0)
+    shell = np.floor(np.sqrt(k2) + 0.5).astype(int)
+    E = np.bincount(shell.ravel(), weights=(0.5 * w * e_mode).ravel()) / float(n) ** 6
+    assert np.sum(E) == pytest.approx(0.5, rel=1e-13)
+    # against the law integrated over each shell, C * int_{n-1/2}^{n+1/2} k^4 exp(-2 (k/4)^2) dk: the
+    # per-shell ratio carries only the lattice's mode count (a shell holds about, not exactly,
+    # 4 pi n^2 modes), so a fit of ln(E/I) = c + a ln n + b (n/4)^2 is flat: a and b near 0.  A
+    # k^2 law would give a = -2, a peak at 3 would give b = 0.78.
+    from scipy.integrate import quad
+    kk = np.arange(1, 2 * n // 7)                                 # whole shells inside the ball
+    law_k = lambda k: k ** 4 * math.exp(-2.0 * (k / 4.0) ** 2)
+    integ = np.array([quad(law_k, k - 0.5, k + 0.5)[0] for k in kk])
+    ratio = E[kk] / integ
+    assert np.max(np.abs(ratio / np.exp(np.mean(np.log(ratio))) - 1)) < 0.4
+    fit = np.linalg.lstsq(np.stack([np.ones(len(kk)), np.log(kk), (kk / 4.0) ** 2], 1), np.log(ratio), rcond=None)[0]
+    assert abs(fit[1]) < 0.3 and abs(fit[2]) < 0.05
+    assert int(np.argmax(E)) == 4                                 # the peak at k_p = 4
