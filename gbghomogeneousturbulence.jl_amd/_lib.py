@@ -66,6 +66,7 @@ _SIGS = {
     "hdns_max_speed": [c_void_p, _D],
     "hdns_reductions": [c_void_p, _D],
     "hdns_spectrum": [c_void_p, c_int32, _D, c_int32],
+    "hdns_spectrum_noalloc": [c_void_p, c_int32, _D, c_int32],
     "hdns_state_distance": [c_void_p, c_void_p, _D],
     "hdns_derivative_skewness": [c_void_p, _D],
     "hdns_field_download": [c_void_p, c_int32, _D],

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -390,6 +391,37 @@ namespace {
 
 // A context of nbox boxes of one size (nbox = 1: a plain context), viscosities visc[nbox] (NULL:
 // the config's for every box).
+// The allocation-free spectrum's scratch (hdns_spectrum_noalloc): the shell partials
+// [nshells][SPEC_CHUNKS] and the shell sums [nshells] for up to NOALLOC_MAX_SHELLS shells, 272 KiB,
+// in a file-scope device array that every GPU the library runs on holds until the process ends.
+// Host code never passes the array's name to a kernel (on the host it is only a stand-in for the
+// symbol): each device's own address is taken once, at the first context created on that device,
+// and kept in g_scratch_ptr.
+constexpr int NOALLOC_MAX_SHELLS = 2048;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[(size_t)NOALLOC_MAX_SHELLS * (SPEC_CHUNKS + 1)];
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// With `device` current: the device address of this device's scratch, taken once.
+static hipError_t bind_scratch(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!ptr) return hipErrorInvalidSymbol;
+  g_scratch_ptr[device] = (double *)ptr;
+  return hipSuccess;
+}
+
+static double *scratch_of(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
 hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, hdns_ctx **out) {
   if (!config || !out) return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "config and out must not be NULL");
   *out = nullptr;
@@ -534,6 +566,12 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
   if (e != hipSuccess) {
     delete ctx;
     return fail(nullptr, HDNS_ERR_DEVICE, std::string("hipSetDevice failed: ") + hipGetErrorString(e));
+  }
+  e = bind_scratch(k.device);
+  if (e != hipSuccess) {
+    delete ctx;
+    return fail(nullptr, HDNS_ERR_DEVICE, std::string("hipGetSymbolAddress of the scratch array failed: ") +
+                                              hipGetErrorString(e));
   }
   c.pipe = c.fp64 ? make_pipeline_f64(n) : make_pipeline_f32(n);
   if (!c.pipe) {
@@ -1224,6 +1262,26 @@ hdns_status hdns_spectrum(hdns_ctx *ctx, int32_t which, double *out, int32_t nsh
     return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "nshells must be at least " + std::to_string(c.nbins));
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
   return spectrum_into(ctx, which, c.spec_bins + c.nbins, c.spec_bins, out, nshells);
+}
+
+hdns_status hdns_spectrum_noalloc(hdns_ctx *ctx, int32_t which, double *out, int32_t nshells) {
+  if (!ctx || !out) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "out must not be NULL");
+  Ctx &c = ctx->c;
+  if (which != HDNS_FIELD_VELOCITY && which != HDNS_FIELD_SCALAR)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "which must be the velocity or the scalar");
+  if (which == HDNS_FIELD_SCALAR && !c.cfg.has_scalar)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "this context carries no scalar");
+  if (nshells < c.nbins)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "nshells must be at least " + std::to_string(c.nbins));
+  if (c.nbins > NOALLOC_MAX_SHELLS)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT,
+                "the allocation-free spectrum holds at most " + std::to_string(NOALLOC_MAX_SHELLS) + " shells; this "
+                "grid has " + std::to_string(c.nbins) + ": use hdns_spectrum");
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  double *ptr = scratch_of(c.cfg.device);
+  if (!ptr) return fail(ctx, HDNS_ERR_STATE, "the scratch array of this device has no address: no context was created on it");
+  // every partial and every sum is written before it is read, so nothing needs clearing
+  return spectrum_into(ctx, which, ptr + NOALLOC_MAX_SHELLS, ptr, out, nshells);
 }
 
 hdns_status hdns_field_download(hdns_ctx *ctx, int32_t which, double *out) {

@@ -152,6 +152,13 @@ hdns_status hdns_derivative_skewness(hdns_ctx *ctx, double *out);
  * scalar.  nshells must be >= floor(sqrt(3)*N/2)+2.  The FP64 shell sums run in a fixed order, so
  * two calls on one state agree to the last bit. */
 hdns_status hdns_spectrum(hdns_ctx *ctx, int32_t which, double *out, int32_t nshells);
+/* hdns_spectrum without any allocation: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): the same kernels and the same output, bit for bit, with the partial sums in a
+ * file-scope scratch array of 272 KiB per device (at most 2048 shells, i.e. N <= 2362; a larger
+ * grid is an HDNS_ERR_INVALID_ARGUMENT).  Every call on one device shares that device's array:
+ * calls that overlap on one GPU, from any context or thread, race. */
+hdns_status hdns_spectrum_noalloc(hdns_ctx *ctx, int32_t which, double *out, int32_t nshells);
 /* A deposited physical field in the reference layout (velocity_physical / scalar_physical /
  * vorticity_physical, src/Cache.jl:203-240); the context must keep it (store_physical). */
 hdns_status hdns_field_download(hdns_ctx *ctx, int32_t which, double *out);
