@@ -416,13 +416,15 @@ def _spectrum_error(a, b, which):
 @pytest.mark.timeout(900)
 def test_fp32_bulk_against_fp64_at_1024_over_10_steps(g):
     """1024^3 forced isotropic turbulence with a mean-gradient scalar (zero initial fluctuation),
-    IF-RK4: the FP32-bulk run against an FP64 run of the same flow on the same GPU, per step and per
-    field (state distance), per reduction and per spectrum.  Bar: velocity and scalar within 1e-5
-    relative after 10 steps."""
-    n, dt, steps = 1024, 5e-4, 10
+    IF-RK4, 2^24 Heun tracers from seed 2028: the FP32-bulk run against an FP64 run of the same flow
+    on the same GPU, per step and per field (state distance), per reduction and per spectrum, and
+    for the tracers' positions and samples.  Bars after 10 steps: velocity and scalar within 1e-5
+    relative; positions within 2e-5 (a few FP32 roundings of 2 pi) and samples within 1e-5."""
+    n, dt, steps, m = 1024, 5e-4, 10, 1 << 24
     t0 = time.perf_counter()
     u = g.random_phase_velocity(n, seed=2026)
     p = np.zeros((n, n, n // 2 + 1), dtype=np.complex128)
+    pos = g.uniform_positions(m, L, 2028, np.float32).astype(np.float64)   # one set for both runs
     print(f"n=1024: seeded in {time.perf_counter() - t0:.1f} s", flush=True)
     runs = []
     for dtype in (np.float32, np.float64):
@@ -431,7 +433,7 @@ def test_fp32_bulk_against_fp64_at_1024_over_10_steps(g):
                                 velocity_forcing=g.ConstantPower(dom, injection_rate=0.1, lowest_shell=1,
                                                                  highest_shell=2))
         runs.append(g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=1.0,
-                                 keep_physical=False))
+                                 tracers=g.Tracers(dom, positions=pos)))
         print(f"n=1024 {np.dtype(dtype).name}: context {runs[-1].memory_bytes() / 2 ** 30:.1f} GiB, "
               f"adopted at {time.perf_counter() - t0:.1f} s", flush=True)
     del u, p
@@ -450,6 +452,10 @@ def test_fp32_bulk_against_fp64_at_1024_over_10_steps(g):
                   f"E(k) {_spectrum_error(a, b, 0):.1e} E_phi(k) {_spectrum_error(a, b, 1):.1e}", flush=True)
         assert r["velocity_relative"] <= 1e-5
         assert r["scalar_relative"] <= 1e-5
+        dx = np.abs((a.tracer_positions() - b.tracer_positions() + L / 2) % L - L / 2).max()
+        (va, sa), (vb, sb) = a.tracer_samples(), b.tracer_samples()
+        print(f"2^24 tracers after {steps} steps: |dX| {dx:.2e}, v {rel(va, vb):.2e}, phi {rel(sa, sb):.2e}", flush=True)
+        assert dx < 2e-5 and rel(va, vb) < 1e-5 and rel(sa, sb) < 1e-5
 
 
 @pytest.mark.parametrize("name", ["dns16_rk4_forced", "dns16_rk3_unforced"])
