@@ -7,7 +7,7 @@ The folder name carries a dot, so it is imported by path; ``load_package()`` in
 from . import _lib
 from .api import (AB, ABM, ConstantPower, Ensemble, EnsembleBox, EnsembleMember, FixedDomain, FlowConditions, Heun,
                   IFHeun, IFLowStorageRK3, IFLowStorageRK4, IFSSPRK3, Linear, NoForcing, VarianceController,
-                  VarianceRescaling, ResolvedFields, Simulation, Tracers, TwoThirds,
+                  VarianceRescaling, ResolvedFields, Simulation, Tracers, TwoThirds, HouLi, PhaseShift,
                   StationarityCriterion, cfl_bound, integral_length, predicted_viscosity,
                   recommended_cfl, truncate_to_band, warmup, reynolds_box, reynolds_taylor, taylor_length,
                   kolmogorov_length, kolmogorov_time, kolmogorov_velocity, unwrapped_positions)
@@ -17,7 +17,8 @@ from .stochastic import StochasticOU
 __all__ = [
     "AB", "ABM", "ConstantPower", "Ensemble", "EnsembleBox", "EnsembleMember", "FixedDomain", "FlowConditions", "Heun",
     "IFHeun", "IFLowStorageRK3", "IFLowStorageRK4", "IFSSPRK3", "Linear", "NoForcing", "VarianceController",
-    "VarianceRescaling", "ResolvedFields", "Simulation", "Tracers", "TwoThirds", "predicted_viscosity",
+    "VarianceRescaling", "ResolvedFields", "Simulation", "Tracers", "TwoThirds", "HouLi", "PhaseShift",
+    "predicted_viscosity",
     "recommended_cfl", "cfl_bound", "integral_length", "StationarityCriterion", "warmup", "truncate_to_band", "random_phase_scalar", "random_phase_velocity",
     "uniform_positions", "StochasticOU", "reynolds_box", "reynolds_taylor", "taylor_length", "kolmogorov_length",
     "kolmogorov_time", "kolmogorov_velocity", "unwrapped_positions",

@@ -40,6 +40,7 @@ class HdnsConfig(ctypes.Structure):
         ("scalar_forcing", c_int32), ("scalar_lowest_shell", c_int32), ("scalar_highest_shell", c_int32),
         ("scalar_over", c_int32), ("scalar_rate", c_double), ("scalar_target", c_double),
         ("scalar_gain", c_double), ("scalar_time", c_double),
+        ("filter_order", c_int32), ("filter_strength", c_double),
     ]
 
 

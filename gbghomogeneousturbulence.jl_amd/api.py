@@ -23,14 +23,65 @@ from .stochastic import OUProcess, StochasticOU
 # --- frame ---------------------------------------------------------------------------------
 
 class TwoThirds:
-    """The sharp 2/3 rule (src/SimulationDomain.jl:100-140): keep the closed ball |k| <= (2/3) k_Ny."""
+    """The sharp 2/3 rule (src/SimulationDomain.jl:61-74): keep the closed ball |k| <= (2/3) k_Ny."""
+    code = 0
+    retained_fraction = 2.0 / 3.0     # src/SimulationDomain.jl:181
+    resolved_fraction = 2.0 / 3.0     # src/SimulationDomain.jl:192
 
     def __repr__(self) -> str:
         return "TwoThirds()"
 
 
+class HouLi:
+    """The smooth Hou-Li filter (src/SimulationDomain.jl:76-138): every mode of the nonlinear terms
+    is multiplied by exp(-strength (|k|/k_Ny)^order).  ``strength=None`` is the element type's
+    -log(eps): 36.04 for FP64, 15.94 for FP32.  ``materialized`` is accepted for the reference's
+    signature; the library always evaluates the filter once per |m|^2 into a table, which is the
+    materialized form (the two advance a run bit-identically in the reference)."""
+    code = 1
+    retained_fraction = 1.0           # src/SimulationDomain.jl:182
+    resolved_fraction = 4.0 / 5.0     # src/SimulationDomain.jl:193
+
+    def __init__(self, order: int = 36, strength: float | None = None, materialized: bool = False):
+        if int(order) != order or int(order) < 1:
+            raise ValueError(f"the HouLi filter's `order` must be a positive integer, got {order}")
+        if strength is not None and not math.isfinite(float(strength)):
+            raise ValueError(f"the HouLi filter's `strength` must be finite, got {strength}")
+        self.order = int(order)
+        self.strength = None if strength is None else float(strength)
+        self.materialized = bool(materialized)
+
+    def strength_for(self, element_type) -> float:
+        """The strength a run of `element_type` uses (src/SimulationDomain.jl:367)."""
+        return -math.log(float(np.finfo(element_type).eps)) if self.strength is None else self.strength
+
+    def __repr__(self) -> str:
+        return f"HouLi(order={self.order}, strength={self.strength})"
+
+
+class PhaseShift:
+    """Phase-shift dealiasing (src/SimulationDomain.jl:140-176): the nonlinear terms are the average
+    of the products on the grid and on the grid displaced by half a cell along every axis, truncated
+    to the closed ball |k| <= (sqrt 2 / 3) N (2 pi/L) = (2 sqrt 2 / 3) k_Ny."""
+    code = 2
+    retained_fraction = 2.0 * math.sqrt(2.0) / 3.0   # src/SimulationDomain.jl:184
+    resolved_fraction = 2.0 * math.sqrt(2.0) / 3.0   # src/SimulationDomain.jl:194
+
+    def __repr__(self) -> str:
+        return "PhaseShift()"
+
+
+def _ball9(n: int, dealiasing) -> float:
+    """The bound of 9 |m|^2 that a strategy's closed ball keeps (inf under HouLi)."""
+    if isinstance(dealiasing, HouLi):
+        return math.inf
+    return 2 * n * n if isinstance(dealiasing, PhaseShift) else n * n
+
+
 # the transform lengths with a radix plan in the library: 2^k and 3 * 2^k
 SUPPORTED_GRIDS = (16, 32, 48, 64, 96, 128, 192, 256, 512, 1024)
+# the grids the HouLi and PhaseShift pipelines are built for
+WIDE_GRIDS = (16, 32, 48, 64, 128, 256)
 
 
 class FixedDomain:
@@ -49,8 +100,11 @@ class FixedDomain:
                                       f"got {n}")
         if not (side_length > 0 and math.isfinite(side_length)):
             raise ValueError("side_length must be positive and finite")
-        if not isinstance(dealiasing, TwoThirds):
-            raise NotImplementedError("only TwoThirds dealiasing is built")
+        if not isinstance(dealiasing, (TwoThirds, HouLi, PhaseShift)):
+            raise ValueError(f"`dealiasing` must be TwoThirds(), HouLi(...) or PhaseShift(), got {dealiasing!r}")
+        if not isinstance(dealiasing, TwoThirds) and n not in WIDE_GRIDS:
+            raise NotImplementedError(f"HouLi and PhaseShift dealiasing are built for grid_points_per_axis in "
+                                      f"{WIDE_GRIDS}; got {n}")
         if element_type not in (np.float32, np.float64):
             raise ValueError("element_type must be np.float32 or np.float64")
         self.grid_points_per_axis = n
@@ -67,9 +121,19 @@ class FixedDomain:
     def grid_spacing(self) -> float:
         return self.side_length / self.grid_points_per_axis
 
+    def _cutoff(self, fraction: float) -> float:
+        # src/SimulationDomain.jl:306-311: T(fraction) * T(N/2) * T(2 pi/L), in the element type
+        t = self.element_type
+        return float(t(fraction) * t(self.grid_points_per_axis / 2) * t(2 * math.pi / self.side_length))
+
     def max_retained_wavenumber(self) -> float:
-        """(2/3)(N/2)(2 pi/L) (src/SimulationDomain.jl:181-195)."""
-        return (2.0 / 3.0) * (self.grid_points_per_axis / 2) * (2 * math.pi / self.side_length)
+        """f_ret (N/2)(2 pi/L): the band the strategy keeps energetic (src/SimulationDomain.jl:391-418)."""
+        return self._cutoff(self.dealiasing.retained_fraction)
+
+    def max_resolved_wavenumber(self) -> float:
+        """f_res (N/2)(2 pi/L): the band whose dynamics is Navier-Stokes, the resolution criterion's
+        kappa_max (src/SimulationDomain.jl:420-457); 4/5 of Nyquist under HouLi."""
+        return self._cutoff(self.dealiasing.resolved_fraction)
 
     def __repr__(self) -> str:
         n = self.grid_points_per_axis
@@ -77,12 +141,19 @@ class FixedDomain:
 
 
 def truncate_to_band(f: np.ndarray, domain: FixedDomain) -> np.ndarray:
-    """Zero every coefficient outside the 2/3 ball, in place (src/SimulationDomain.jl:520-537)."""
+    """Apply the dealias kernel to a spectral field, in place (src/SimulationDomain.jl:520-537): zero
+    every coefficient outside the closed ball (TwoThirds, PhaseShift), or multiply each mode by the
+    HouLi filter exp(-strength (|k|/k_Ny)^order)."""
     n = domain.grid_points_per_axis
     i = np.arange(n)
     w = np.where(i <= n // 2, i, i - n)
     m2 = np.arange(n // 2 + 1)[None, None, :] ** 2 + w[None, :, None] ** 2 + w[:, None, None] ** 2
-    f[..., 9 * m2 > n * n] = 0
+    dl = domain.dealiasing
+    if isinstance(dl, HouLi):
+        s = dl.strength_for(domain.element_type)
+        f *= np.exp(-s * (np.sqrt(m2.astype(np.float64)) / (n / 2)) ** dl.order)
+        return f
+    f[..., 9 * m2 > _ball9(n, dl)] = 0
     return f
 
 
@@ -272,7 +343,7 @@ def predicted_viscosity(domain: FixedDomain, dissipation: float, kmax_eta: float
     if not (math.isfinite(dissipation) and dissipation > 0):
         raise ValueError(f"`dissipation` must be finite and positive, got {dissipation}: it is the dissipation "
                          "rate eps the viscosity is solved for")
-    eta = kmax_eta / domain.max_retained_wavenumber()
+    eta = kmax_eta / domain.max_resolved_wavenumber()
     return dissipation ** (1.0 / 3.0) * eta ** (4.0 / 3.0)
 
 
@@ -305,8 +376,11 @@ class IFHeun:
 
 
 def recommended_cfl(scheme, domain: FixedDomain) -> float:
-    """The advective CFL ceiling (src/Propagator.jl:906-919); TwoThirds leaves the base unscaled."""
-    return scheme.base_cfl
+    """The advective CFL ceiling (src/Propagator.jl:906-919): the scheme's base ceiling times
+    (2/3)/f_ret, so TwoThirds leaves it unscaled and HouLi, which keeps modes to Nyquist, takes 2/3 of it."""
+    if isinstance(domain.dealiasing, TwoThirds):
+        return scheme.base_cfl   # the factor is exactly 1
+    return scheme.base_cfl * (2.0 / 3.0) / domain.dealiasing.retained_fraction
 
 
 # --- derived scales of (k, eps, nu) (src/Observables.jl:611-760) ---------------------------------
@@ -532,7 +606,11 @@ def _config(domain: FixedDomain, conditions: FlowConditions, scheme, carries_sca
     cfg.side_length = domain.side_length
     cfg.precision = domain.precision
     cfg.scheme = scheme.code
-    cfg.dealiasing = 0
+    dl = domain.dealiasing
+    cfg.dealiasing = dl.code
+    if isinstance(dl, HouLi):
+        cfg.filter_order = dl.order
+        cfg.filter_strength = math.nan if dl.strength is None else dl.strength
     cfg.has_scalar = int(carries_scalar)
     cfg.viscosity = conditions.viscosity
     cfg.schmidt = conditions.schmidt

@@ -13,7 +13,7 @@ CSRC = HERE / "csrc"
 INCLUDE = HERE.parent / "include"
 LIB = HERE / "libhdns.so"
 OBJDIR = HERE / "_build"
-SOURCES = ["api.hip", "inst_f32.hip", "inst_f64.hip", "sort.hip"]
+SOURCES = ["api.hip", "inst_f32.hip", "inst_f64.hip", "inst_f32_wide.hip", "inst_f64_wide.hip", "sort.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -41,7 +41,7 @@ def build(verbose: bool = False) -> Path:
         subprocess.run(cmd, check=True)
         return obj
 
-    jobs = int(os.environ.get("MAX_JOBS", "4"))
+    jobs = int(os.environ.get("MAX_JOBS", "6"))
     with ThreadPoolExecutor(max_workers=max(1, min(jobs, len(SOURCES)))) as pool:
         objs = list(pool.map(compile_one, SOURCES))
     tmp = LIB.with_suffix(".so.tmp")

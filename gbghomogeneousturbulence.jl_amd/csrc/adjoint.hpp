@@ -48,7 +48,7 @@ __device__ __forceinline__ Mode mode_of(size_t l, const Geo &g) {
   m.m2 = m.mx * m.mx + m.my * m.my + m.mz * m.mz;
   const double kx = g.k0 * m.mx, ky = g.k0 * m.my, kz = g.k0 * m.mz;
   m.k2 = (kx * kx + ky * ky) + kz * kz;
-  m.in = m.mx <= g.m && in_two_thirds(m.m2, g.n);
+  m.in = m.mx <= g.m && keeps(m.m2, g);
   return m;
 }
 
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void k_adj_pre(const cx<T> *__restrict__ lq, c
     t6[5 * comp + l] = m.in ? cx<T>{-kz * mp.im, kz * mp.re} : czero<T>();
     // <mu_u, 1_band u_i>_w
     if (m.in && in_band(m.m2, as.band)) {
-      const double w = m.mx == 0 ? 1.0 : 2.0;
+      const double w = parseval_w(m.mx, g.n);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const cx<T> u = qi[c * comp + l];
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(NT) void k_zinv_plain(const cx<T> *__restrict__ src
   cx<T> *twl = buf + N * TX;
   const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y, c = blockIdx.z;
   const int my = signed_of(ky_idx, g);
-  if (!in_disk(line_start<T>(kx0), my, N)) return;
+  if (!in_disk(line_start<T>(kx0), my, g)) return;
   stage_twiddles<T, N, NT>(twl, tw);
   const size_t comp = comp_of(g), pencil = (size_t)N * g.k * g.kxp, zstride = (size_t)g.k * g.kxp;
   const cx<T> *s = src + c * comp + (size_t)ky_idx * g.kxp + kx0;
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(NT) void k_zinv_plain(const cx<T> *__restrict__ src
         const int kz_idx = idx_of_bin(n, g);
         if (kz_idx < 0) return czero<T>();
         const int mx = kx0 + col, mz = signed_of(kz_idx, g);
-        if (mx > g.m || !in_two_thirds(mx * mx + my * my + mz * mz, N)) return czero<T>();
+        if (mx > g.m || !keeps(mx * mx + my * my + mz * mz, g)) return czero<T>();
         return scale(s[(size_t)kz_idx * zstride + col], scale_by);
       },
       [&](int, int, int n, int col, cx<T> v) { dst[(size_t)n * zstride + col] = v; });
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(NT) void k_zfwd_plain(const cx<T> *__restrict__ pf,
   const int my = signed_of(ky_idx, g);
   const size_t comp = comp_of(g), pencil = (size_t)N * g.k * g.kxp, zstride = (size_t)g.k * g.kxp;
   cx<T> *d = dst + c * comp + (size_t)ky_idx * g.kxp + kx0;
-  if (!in_disk(kx0, my, N)) {
+  if (!in_disk(kx0, my, g)) {
     for (int e = threadIdx.x; e < g.k * TX; e += NT) d[(size_t)(e / TX) * zstride + e % TX] = czero<T>();
     return;
   }
@@ -286,7 +286,7 @@ __global__ __launch_bounds__(NT) void k_zfwd_plain(const cx<T> *__restrict__ pf,
         const int kz_idx = idx_of_bin(n, g);
         if (kz_idx < 0) return;
         const int mx = kx0 + col, mz = signed_of(kz_idx, g);
-        const bool in = mx <= g.m && in_two_thirds(mx * mx + my * my + mz * mz, N);
+        const bool in = mx <= g.m && keeps(mx * mx + my * my + mz * mz, g);
         d[(size_t)kz_idx * zstride + col] = in ? scale(v, scale_by) : czero<T>();
       });
 }

@@ -49,6 +49,8 @@ hdns_status launched(hdns_ctx *ctx) {
 bool power_of_two(int n) { return n > 0 && (n & (n - 1)) == 0; }
 // the transform lengths with a radix plan: 2^k (16..1024) and 3 * 2^k (48, 96, 192)
 bool supported_grid(int n) { return (power_of_two(n) && n >= 16 && n <= 1024) || n == 48 || n == 96 || n == 192; }
+// The grids the HouLi and PhaseShift pipelines are built for (make_pipeline_wide_t).
+bool wide_grid(int n) { return (power_of_two(n) && n >= 16 && n <= 256) || n == 48; }
 
 // The element-type value of a rational p/q: rounded once for FP64; for FP32 the nearest float
 // to the extended-precision quotient.
@@ -188,7 +190,7 @@ void release(Ctx &c) {
   void *ptrs[] = {c.uh, c.hu, c.ph, c.hp, c.bufa, c.bufb, c.phys, c.physw, c.tw, c.partials, c.scalars, c.rpart,
                   c.usrc, c.psrc,
                   c.maxbits, c.spec_bins, c.pos, c.upos, c.vel, c.phis, c.order, c.skeys, c.svals, c.sort_temp,
-                  c.visc_d, c.kappa_d};
+                  c.visc_d, c.kappa_d, c.filter, c.shifted};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   AdjBufs &a = c.adj;
@@ -353,6 +355,7 @@ void launch_scalar_amplitude(Ctx &c) {
 // stage 1, max|u|^2, and the reductions of the state.
 hdns_status run_deposit(hdns_ctx *ctx, void *phys_to = nullptr, void *physw_to = nullptr) {
   Ctx &c = ctx->c;
+  if (c.shifted) c.pipe->shifted(c);   // PhaseShift: the displaced products of this state, for its next stage
   c.pipe->zi(c);
   const FinalizeArgs fa = finalize_args(c);
   hipLaunchKernelGGL(k_finalize, dim3(c.nbox), dim3(256), 0, c.stream, (const double *)c.partials, c.scalars, fa);
@@ -438,7 +441,16 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
   if (k.scheme != HDNS_IF_LOW_STORAGE_RK3 && k.scheme != HDNS_IF_LOW_STORAGE_RK4 && k.scheme != HDNS_IF_SSP_RK3 &&
       k.scheme != HDNS_IF_HEUN)
     return fail(nullptr, HDNS_ERR_UNSUPPORTED, "scheme must be IFLowStorageRK3, IFLowStorageRK4, IFSSPRK3 or IFHeun");
-  if (k.dealiasing != HDNS_TWO_THIRDS) return fail(nullptr, HDNS_ERR_UNSUPPORTED, "only TwoThirds dealiasing is built");
+  if (k.dealiasing != HDNS_TWO_THIRDS && k.dealiasing != HDNS_HOU_LI && k.dealiasing != HDNS_PHASE_SHIFT)
+    return fail(nullptr, HDNS_ERR_UNSUPPORTED, "dealiasing must be TwoThirds, HouLi or PhaseShift");
+  if (k.dealiasing != HDNS_TWO_THIRDS && !wide_grid(k.grid_points))
+    return fail(nullptr, HDNS_ERR_UNSUPPORTED,
+                "HouLi and PhaseShift dealiasing are built for grid_points a power of two in 16..256, or 48; got " +
+                    std::to_string(k.grid_points));
+  if (k.dealiasing == HDNS_HOU_LI && k.filter_order < 1)
+    return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "the HouLi filter's `order` must be a positive integer");
+  if (k.dealiasing == HDNS_HOU_LI && isinf(k.filter_strength))
+    return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT, "the HouLi filter's `strength` must be finite (NaN: the default)");
   // src/Generator.jl:107-112
   for (int b = 0; b < nbox; ++b)
     if (!((visc ? visc[b] : k.viscosity) > 0))
@@ -461,7 +473,7 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
       return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
                   "StochasticOU needs a finite `highest_shell`: it adds its process rather than multiplying the "
                   "field, so its band must stay below the dealias cutoff");
-    if (ou && 9 * (2 * hi + 1) * (2 * hi + 1) > 4 * k.grid_points * k.grid_points)
+    if (ou && 9LL * (2 * hi + 1) * (2 * hi + 1) > 4LL * band_ball9(k.grid_points, k.dealiasing))
       return fail(nullptr, HDNS_ERR_INVALID_ARGUMENT,
                   "the StochasticOU band reaches past the dealias cutoff; an additive force would fill modes the "
                   "dealiasing holds empty. Lower the shells, or enlarge the grid.");
@@ -573,15 +585,17 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
     return fail(nullptr, HDNS_ERR_DEVICE, std::string("hipGetSymbolAddress of the scratch array failed: ") +
                                               hipGetErrorString(e));
   }
-  c.pipe = c.fp64 ? make_pipeline_f64(n) : make_pipeline_f32(n);
+  if (k.dealiasing == HDNS_TWO_THIRDS) c.pipe = c.fp64 ? make_pipeline_f64(n) : make_pipeline_f32(n);
+  else c.pipe = c.fp64 ? make_pipeline_f64_wide(n, k.dealiasing) : make_pipeline_f32_wide(n, k.dealiasing);
   if (!c.pipe) {
     delete ctx;
     return fail(nullptr, HDNS_ERR_UNSUPPORTED, "no pipeline for this grid size");
   }
   const int tx = c.pipe->tile_width();
   c.geo.n = n;
-  c.geo.m = n / 3;
-  c.geo.k = 2 * c.geo.m + 1;
+  c.geo.m = band_m(n, k.dealiasing);
+  c.geo.k = band_k(n, k.dealiasing);
+  c.geo.ball9 = band_ball9(n, k.dealiasing);
   c.geo.kxp = ((c.geo.m + 1 + tx - 1) / tx) * tx;
   c.geo.k0 = 2.0 * M_PI / k.side_length;
   c.elem = c.fp64 ? 16 : 8;
@@ -646,6 +660,22 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
   }
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.maxbits, B * sizeof(unsigned long long));
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.spec_bins, (size_t)c.nbins * (1 + SPEC_CHUNKS) * sizeof(double));
+  if (s == HDNS_OK && k.dealiasing == HDNS_HOU_LI) {
+    // src/SimulationDomain.jl:365-376: exp(-strength (|k| / k_Ny)^order), |k| / k_Ny = |m| / (N/2), per
+    // |m|^2 up to 3 (N/2)^2; the default strength -log(eps(T)) damps the Nyquist mode to eps(T)
+    const int nh = n / 2;
+    const size_t len = (size_t)3 * nh * nh + 1;
+    const double strength = isnan(k.filter_strength) ? -log(eps_of(c)) : k.filter_strength;
+    std::vector<double> f(len);
+    for (size_t m2 = 0; m2 < len; ++m2) f[m2] = exp(-strength * pow(sqrt((double)m2) / (double)nh, (double)k.filter_order));
+    s = alloc(ctx, (void **)&c.filter, len * sizeof(double));
+    if (s == HDNS_OK) {
+      e = hipMemcpy(c.filter, f.data(), len * sizeof(double), hipMemcpyHostToDevice);
+      if (e != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, std::string("filter upload failed: ") + hipGetErrorString(e));
+    }
+  }
+  if (s == HDNS_OK && k.dealiasing == HDNS_PHASE_SHIFT)   // the displaced products' transforms
+    s = alloc(ctx, &c.shifted, B * ncf * c.comp * c.elem);
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.visc_d, B * sizeof(double));
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.kappa_d, B * sizeof(double));
   if (s == HDNS_OK && k.tracer_capacity > 0) {
@@ -796,13 +826,14 @@ hdns_status hdns_state_upload(hdns_ctx *ctx, const double *u_hat, const double *
         const int sy = iy <= n / 2 ? iy : iy - n;
         const double *row = src + 2 * (((size_t)iz * n + iy) * hx);
         for (int ix = 0; ix < hx; ++ix) {
-          if (in_two_thirds(ix * ix + sy * sy + sz * sz, n)) continue;
+          if (keeps(ix * ix + sy * sy + sz * sz, g)) continue;
           if (row[2 * ix] != 0.0 || row[2 * ix + 1] != 0.0) {
             char msg[400];
             snprintf(msg, sizeof msg,
-                     "the %s spectrum has content outside the 2/3 dealiasing ball (mode %d,%d,%d); the step keeps "
+                     "the %s spectrum has content outside the %s dealiasing ball (mode %d,%d,%d); the step keeps "
                      "only the retained modes, so truncate the state to the band first (truncate_to_band!)",
-                     q < 3 ? "velocity" : "scalar", ix, sy, sz);
+                     q < 3 ? "velocity" : "scalar", c.cfg.dealiasing == HDNS_PHASE_SHIFT ? "phase-shift" : "2/3",
+                     ix, sy, sz);
             return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, msg);
           }
         }
@@ -952,6 +983,8 @@ void launch_step(Ctx &c, double dt, const StageRecord *rec = nullptr) {
     sa.lin_rate = c.cfg.velocity_forcing == HDNS_LINEAR ? c.cfg.injection_rate : 0.0;
     sa.sband = c.sband;
     sa.slin_rate = c.cfg.scalar_forcing == HDNS_LINEAR ? c.cfg.scalar_rate : 0.0;
+    sa.filter = c.filter;
+    sa.shifted = c.shifted;
     if (c.cfg.scheme == HDNS_IF_SSP_RK3 || c.cfg.scheme == HDNS_IF_HEUN) shu_osher_stage(sa, c.cfg.scheme, i, dt, fp32);
     XArgs xa = last ? fin : mid;
     if (rec) {
@@ -973,6 +1006,7 @@ void launch_step(Ctx &c, double dt, const StageRecord *rec = nullptr) {
     if (last) TIMED(c, K_REALITY, c.pipe->reality(c));
     if (last && c.cfg.velocity_forcing == HDNS_VARIANCE_RESCALING) launch_rescale(c, false);
     if (last && c.sband.on && c.cfg.scalar_forcing == HDNS_VARIANCE_RESCALING) launch_rescale(c, true);
+    if (c.shifted) TIMED(c, K_OTHER, c.pipe->shifted(c));   // PhaseShift: the next stage's displaced products
     TIMED(c, K_ZI, c.pipe->zi(c));
     TIMED(c, K_FINALIZE,
           hipLaunchKernelGGL(k_finalize, dim3(c.nbox), dim3(256), 0, c.stream, (const double *)c.partials, c.scalars,
@@ -1090,7 +1124,7 @@ void launch_distance(const Ctx &ca, const Ctx &cb, bool scalar, double *partials
   const cx<TA> *a = scalar ? (const cx<TA> *)ca.ph + ca.box * comp_a : (const cx<TA> *)ca.uh + ca.box * 3 * comp_a;
   const cx<TB> *b = scalar ? (const cx<TB> *)cb.ph + cb.box * comp_b : (const cx<TB> *)cb.uh + cb.box * 3 * comp_b;
   hipLaunchKernelGGL((k_state_distance<TA, TB>), dim3(DIST_BLOCKS), dim3(256), 0, ca.stream, a, comp_a, ca.geo.kxp, b,
-                     comp_b, cb.geo.kxp, ca.geo.k, ca.geo.m, scalar ? 1 : 3, partials);
+                     comp_b, cb.geo.kxp, ca.geo.k, ca.geo.m, ca.geo.n, scalar ? 1 : 3, partials);
 }
 
 double neumaier(const double *v, size_t n) {
@@ -1110,7 +1144,8 @@ extern "C" {
 hdns_status hdns_state_distance(hdns_ctx *a, hdns_ctx *b, double *out) {
   if (!a || !b || !out) return fail(a, HDNS_ERR_INVALID_ARGUMENT, "a, b and out must not be NULL");
   Ctx &ca = a->c, &cb = b->c;
-  if (ca.geo.n != cb.geo.n) return fail(a, HDNS_ERR_INVALID_ARGUMENT, "the two contexts must have the same grid");
+  if (ca.geo.n != cb.geo.n || ca.geo.m != cb.geo.m)
+    return fail(a, HDNS_ERR_INVALID_ARGUMENT, "the two contexts must have the same grid and dealiasing");
   if (ca.cfg.device != cb.cfg.device) return fail(a, HDNS_ERR_INVALID_ARGUMENT, "the two contexts must share a device");
   if (ca.cfg.has_scalar != cb.cfg.has_scalar)
     return fail(a, HDNS_ERR_INVALID_ARGUMENT, "both contexts must carry a scalar, or neither");
@@ -1151,6 +1186,8 @@ hdns_status hdns_state_distance(hdns_ctx *a, hdns_ctx *b, double *out) {
 
 hdns_status hdns_derivative_skewness(hdns_ctx *ctx, double *out) {
   if (!ctx || !out) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "out must not be NULL");
+  if (ctx->c.cfg.dealiasing != HDNS_TWO_THIRDS)
+    return fail(ctx, HDNS_ERR_UNSUPPORTED, "the derivative skewness is built for TwoThirds dealiasing only");
   Ctx &c = ctx->c;
   if (!c.deposited) return fail(ctx, HDNS_ERR_STATE, "no deposit of the current state");
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
@@ -1184,6 +1221,8 @@ hdns_status hdns_below_scale(hdns_ctx *ctx, double *scales, double *kinetic_ener
   if (!ctx || !scales || !kinetic_energy || !dissipation)
     return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "scales, kinetic_energy and dissipation must not be NULL");
   Ctx &c = ctx->c;
+  if (c.cfg.dealiasing != HDNS_TWO_THIRDS)
+    return fail(ctx, HDNS_ERR_UNSUPPORTED, "the below-scale energies are built for TwoThirds dealiasing only");
   if (!c.deposited) return fail(ctx, HDNS_ERR_STATE, "no deposit of the current state");
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
   const int n = c.geo.n, m = c.geo.m;
@@ -1647,7 +1686,7 @@ hdns_status hdns_frame_tables(hdns_ctx *ctx, int32_t *mx, int32_t *my, int32_t *
   HIP_TRY(ctx, hipMalloc((void **)&d, (hx + 2 * n) * sizeof(int)));
   hipError_t e = hipMalloc((void **)&dm, total);
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_frame_tables, dim3(1024), dim3(256), 0, c.stream, n, d, d + hx, d + hx + n, dm);
+    hipLaunchKernelGGL(k_frame_tables, dim3(1024), dim3(256), 0, c.stream, n, c.geo.ball9, d, d + hx, d + hx + n, dm);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpyAsync(mx, d, hx * sizeof(int), hipMemcpyDeviceToHost, c.stream);
@@ -1805,6 +1844,8 @@ hdns_status adj_alloc(hdns_ctx *ctx) {
   AdjBufs &a = c.adj;
   if (a.ready) return HDNS_OK;
   if (c.nbox != 1) return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint runs on a single-box context");
+  if (c.cfg.dealiasing != HDNS_TWO_THIRDS)
+    return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint is built for TwoThirds dealiasing only");
   if (c.cfg.velocity_forcing != HDNS_NO_FORCING && c.cfg.velocity_forcing != HDNS_CONSTANT_POWER)
     return fail(ctx, HDNS_ERR_UNSUPPORTED, "the adjoint supports NoForcing and ConstantPower velocity forcing");
   if (c.cfg.scalar_forcing != HDNS_NO_FORCING)

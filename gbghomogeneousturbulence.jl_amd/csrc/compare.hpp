@@ -10,12 +10,12 @@ namespace hdns {
 constexpr int DIST_BLOCKS = 1024;
 
 // Per-block sums of w |a - b|^2 and w |b|^2 over the retained cube of `ncomp` compact components
-// (the Parseval weight w is 1 on kx = 0 and 2 inside; kx = N/2 lies outside the ball).  The
+// (the Parseval weight w is 1 on the kx = 0 and kx = N/2 planes and 2 inside).  The
 // element types and padded row widths of the two layouts may differ.  partials: [ncomp][2][blocks].
 template <typename TA, typename TB>
 __global__ __launch_bounds__(256) void k_state_distance(const cx<TA> *__restrict__ a, size_t comp_a, int kxp_a,
                                                         const cx<TB> *__restrict__ b, size_t comp_b, int kxp_b,
-                                                        int k, int m, int ncomp, double *__restrict__ partials) {
+                                                        int k, int m, int n, int ncomp, double *__restrict__ partials) {
   __shared__ double red[4][8][2];
   const size_t modes = (size_t)k * k * (m + 1);
   for (int c = 0; c < ncomp; ++c) {
@@ -25,7 +25,7 @@ __global__ __launch_bounds__(256) void k_state_distance(const cx<TA> *__restrict
       const size_t row = l / (m + 1);
       const cx<TA> va = a[c * comp_a + row * kxp_a + kx];
       const cx<TB> vb = b[c * comp_b + row * kxp_b + kx];
-      const double w = kx == 0 ? 1.0 : 2.0;
+      const double w = (kx == 0 || 2 * kx == n) ? 1.0 : 2.0;
       const double dr = (double)va.re - (double)vb.re, di = (double)va.im - (double)vb.im;
       dd += w * (dr * dr + di * di);
       bb += w * ((double)vb.re * (double)vb.re + (double)vb.im * (double)vb.im);

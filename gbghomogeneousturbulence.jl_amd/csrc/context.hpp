@@ -59,10 +59,15 @@ struct Pipeline {
   // partials and returns their count (the pencil buffers are overwritten: redo the deposit)
   virtual int strain_det(Ctx &c, double *partials) = 0;
   virtual int strain_det_blocks(const Ctx &c) const = 0;  // blocks of a z pass under the current schedule
+  // PhaseShift: the products on the half-cell-displaced grid, transformed, into c.shifted
+  virtual void shifted(Ctx &c) = 0;
 };
 
 std::unique_ptr<Pipeline> make_pipeline_f32(int n);
 std::unique_ptr<Pipeline> make_pipeline_f64(int n);
+// HouLi (dealias 1) and PhaseShift (2) pipelines, built for the grids WIDE_GRIDS lists
+std::unique_ptr<Pipeline> make_pipeline_f32_wide(int n, int dealias);
+std::unique_ptr<Pipeline> make_pipeline_f64_wide(int n, int dealias);
 
 struct Ctx {
   hdns_config cfg{};
@@ -98,6 +103,8 @@ struct Ctx {
   double *rpart = nullptr;     // VarianceRescaling's block sums
   void *usrc = nullptr, *psrc = nullptr;   // StochasticOU's band force of the step, u and phi (FP64)
   unsigned long long *maxbits = nullptr;
+  double *filter = nullptr;      // HouLi: the filter of each |m|^2 (FP64), 3 (N/2)^2 + 1 entries
+  void *shifted = nullptr;        // PhaseShift: the displaced products' transforms [box][ncf][comp]
   double *spec_bins = nullptr;   // [nbins] shell sums, then [nbins][SPEC_CHUNKS] partials
   int nbins = 0;
   // tracers

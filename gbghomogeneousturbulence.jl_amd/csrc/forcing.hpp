@@ -55,7 +55,7 @@ __global__ __launch_bounds__(256) void k_band_sums(const cx<T> *__restrict__ f, 
       const cx<T> v = f[c * comp + at];
       e += (double)v.re * (double)v.re + (double)v.im * (double)v.im;
     }
-    const double w = kx == 0 ? 1.0 : 2.0;
+    const double w = parseval_w(kx, g.n);
     const int my = signed_of(ky_idx, g), mz = signed_of(kz_idx, g);
     whole += w * e;
     if (in_band(kx * kx + my * my + mz * mz, band)) inband += w * e;

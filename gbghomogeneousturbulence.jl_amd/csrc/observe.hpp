@@ -22,7 +22,7 @@ __global__ __launch_bounds__(NT) void k_zstrain(const cx<T> *__restrict__ uh, cx
   cx<T> *twl = buf + N * TX;
   const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y, s = blockIdx.z;
   const int my = signed_of(ky_idx, g);
-  if (!in_disk(line_start<T>(kx0), my, N)) return;
+  if (!in_disk(line_start<T>(kx0), my, g)) return;
   stage_twiddles<T, N, NT>(twl, tw);
   const int ci = strain_i(s), cj = strain_j(s);
   const size_t comp = comp_of(g), pencil = (size_t)N * g.k * g.kxp, zstride = (size_t)g.k * g.kxp;
@@ -35,7 +35,7 @@ __global__ __launch_bounds__(NT) void k_zstrain(const cx<T> *__restrict__ uh, cx
         const int kz_idx = idx_of_bin(n, g);
         if (kz_idx < 0) return czero<T>();
         const int mx = kx0 + col, mz = signed_of(kz_idx, g);
-        if (mx > g.m || !in_two_thirds(mx * mx + my * my + mz * mz, N)) return czero<T>();
+        if (mx > g.m || !keeps(mx * mx + my * my + mz * mz, g)) return czero<T>();
         const int mv[3] = {mx, my, mz};
         const T ki = T(g.k0 * mv[ci]), kj = T(g.k0 * mv[cj]);
         const cx<T> a = uj[(size_t)kz_idx * zstride + col], b = ui[(size_t)kz_idx * zstride + col];
@@ -143,7 +143,7 @@ __global__ __launch_bounds__(256) void k_below_scale(const cx<T> *__restrict__ u
       e += (double)v.re * (double)v.re + (double)v.im * (double)v.im;
     }
     if (e == 0.0) continue;
-    const double w = kx == 0 ? 1.0 : 2.0;
+    const double w = parseval_w(kx, g.n);
     const double kxd = g.k0 * kx, kyd = g.k0 * my, kzd = g.k0 * mz;
     const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
     const int ay = my < 0 ? -my : my, az = mz < 0 ? -mz : mz;

@@ -1,10 +1,13 @@
 // The pencil kernels of the DNS stage: the two y passes, the x pass that forms the physical
 // products, and the two z passes that carry every spectral pointwise operation of the stage.
 //
-// Retained modes only.  Under the 2/3 rule every field the step keeps lies inside the cube
-// |mx|, |my|, |mz| <= M = floor(N/3), so spectral data is stored compactly as
+// Retained modes only.  Every field the step keeps lies inside the cube |mx|, |my|, |mz| <= M of
+// its dealiasing strategy: M = floor(N/3) under the 2/3 rule, the largest m with 9 m^2 <= 2 N^2
+// under PhaseShift (the ball of radius sqrt(2) N/3), and N/2 under HouLi, which keeps every mode.
+// Spectral data is stored compactly as
 //   [c][kz_idx][ky_idx][kx]   kx in [0, KXP) (padded past M with zeros),
-//                             ky_idx, kz_idx in [0, K), K = 2M + 1, m = idx <= M ? idx : idx - K,
+//                             ky_idx, kz_idx in [0, K), K = min(2M + 1, N), m = idx <= M ? idx : idx - K
+//                             (under HouLi K = N and the Nyquist index N/2 carries m = +N/2),
 // and the transforms are pruned: a z pass only ever touches the K retained ky columns, a y pass
 // only the kx < KXP columns, and the x pass zero-fills M < kx < KXP.  The half-transformed
 // pencils are
@@ -19,12 +22,33 @@
 
 namespace hdns {
 
+// The dealiasing strategies (src/SimulationDomain.jl:44-195): the sharp 2/3 ball, Hou-Li's smooth
+// filter (every mode kept), and the phase-shifted evaluation with the ball of radius sqrt(2) N/3.
+enum { DEALIAS_TWO_THIRDS = 0, DEALIAS_HOU_LI = 1, DEALIAS_PHASE_SHIFT = 2 };
+
+// Largest retained |m| per axis, and the retained signed wavenumbers per y/z axis.
+__host__ __device__ constexpr int band_m(int n, int da) {
+  if (da == DEALIAS_HOU_LI) return n / 2;
+  if (da == DEALIAS_PHASE_SHIFT) {
+    int m = 0;
+    while (9 * (m + 1) * (m + 1) <= 2 * n * n) ++m;
+    return m;
+  }
+  return n / 3;
+}
+__host__ __device__ constexpr int band_k(int n, int da) { return 2 * band_m(n, da) + 1 > n ? n : 2 * band_m(n, da) + 1; }
+// The ball a strategy keeps, as the bound of 9 |m|^2 (no bound under HouLi).
+__host__ __device__ constexpr int band_ball9(int n, int da) {
+  return da == DEALIAS_HOU_LI ? 0x7fffffff : (da == DEALIAS_PHASE_SHIFT ? 2 * n * n : n * n);
+}
+
 struct Geo {
   int n;      // grid points per axis
-  int m;      // largest retained |m| per axis, floor(n/3)
+  int m;      // largest retained |m| per axis (band_m)
   int kxp;    // padded kx extent of every compact array
-  int k;      // 2m + 1 retained signed wavenumbers on y and z
+  int k;      // retained signed wavenumbers on y and z (band_k)
   double k0;  // 2 pi / L
+  int ball9;  // a mode is kept iff 9 |m|^2 <= ball9 (band_ball9)
   // A batch of boxes of one size lies back to back in every array; a kernel finds its box in the
   // grid's last dimension.  Per box: buffer A and B hold box_a / box_b complex values, the state
   // [3][comp] (u, h_u) or [comp] (phi, h_phi).
@@ -49,15 +73,22 @@ __device__ __forceinline__ int idx_of_bin(int b, const Geo &g) {
 template <typename T>
 __device__ __forceinline__ cx<T> czero() { return {T(0), T(0)}; }
 
-// The (kx, ky) column of a z pencil meets the closed 2/3 ball 9|m|^2 <= N^2 only when
-// 9 (mx^2 + my^2) <= N^2.  Every retained state mode outside the ball is zero and stays zero
+// The (kx, ky) column of a z pencil meets the retained ball 9|m|^2 <= ball9 only when
+// 9 (mx^2 + my^2) <= ball9.  Every retained state mode outside the ball is zero and stays zero
 // (the state is adopted truncated, and the mask, the Leray projection, the forcing and the
 // integrating factor all keep zeros zero), so the passes skip the (kx, ky) columns outside this
 // disk, at the granularity of whole 128-byte lines (partly written lines cost more than they
 // save): a ky row of PF or Q is written and read only in the lines whose first column lies in
 // the disk.  In a written Q line the columns outside the disk hold exact zeros (the transform of
 // zero modes), so the inverse y pass reads the whole line.
-__host__ __device__ __forceinline__ bool in_disk(int mx, int my, int n) { return 9 * (mx * mx + my * my) <= n * n; }
+__host__ __device__ __forceinline__ bool in_disk(int mx, int my, const Geo &g) {
+  return 9 * (mx * mx + my * my) <= g.ball9;
+}
+// The retained ball: a mode the strategy keeps (9|m|^2 <= N^2 under the 2/3 rule, the exact-integer
+// closed ball of src/SimulationDomain.jl:342-376).
+__host__ __device__ __forceinline__ bool keeps(int m2, const Geo &g) { return 9 * m2 <= g.ball9; }
+// Parseval weight of a half-spectrum mode: 1 on the self-conjugate kx = 0 and kx = N/2 planes, 2 inside.
+__host__ __device__ __forceinline__ double parseval_w(int mx, int n) { return (mx == 0 || 2 * mx == n) ? 1.0 : 2.0; }
 
 // First column of the 128-byte line that holds column mx.
 template <typename T>
@@ -81,7 +112,7 @@ __global__ __launch_bounds__(NT) void k_yfwd(const cx<T> *__restrict__ px, cx<T>
       buf, twl, [&](int, int, int n, int col) { return src[(size_t)n * g.kxp + col]; },
       [&](int, int, int n, int col, cx<T> v) {
         const int idx = idx_of_bin(n, g);
-        if (idx >= 0 && in_disk(kx0, signed_of(idx, g), N)) dst[(size_t)idx * g.kxp + col] = v;
+        if (idx >= 0 && in_disk(kx0, signed_of(idx, g), g)) dst[(size_t)idx * g.kxp + col] = v;
       });
 }
 
@@ -99,7 +130,7 @@ __global__ __launch_bounds__(NT) void k_yinv(const cx<T> *__restrict__ q, cx<T> 
       buf, twl,
       [&](int, int, int n, int col) {
         const int idx = idx_of_bin(n, g);
-        return idx >= 0 && in_disk(kx0, signed_of(idx, g), N) ? src[(size_t)idx * g.kxp + col] : czero<T>();
+        return idx >= 0 && in_disk(kx0, signed_of(idx, g), g) ? src[(size_t)idx * g.kxp + col] : czero<T>();
       },
       [&](int, int, int n, int col, cx<T> v) { dst[(size_t)n * g.kxp + col] = v; });
 }
@@ -131,7 +162,7 @@ constexpr int xpass_lanes() {
   return l < 64 ? 64 : (l > XT ? XT : l);
 }
 
-template <typename T, int N, int NCIN, int XT>
+template <typename T, int N, int NCIN, int XT, int DA = DEALIAS_TWO_THIRDS>
 __global__ __launch_bounds__(XT) void k_xpass(const cx<T> *__restrict__ rin,
                                                               cx<T> *__restrict__ pxo, Geo g,
                                                               const cx<double> *__restrict__ tw, XArgs xa) {
@@ -140,7 +171,7 @@ __global__ __launch_bounds__(XT) void k_xpass(const cx<T> *__restrict__ rin,
   constexpr int NT = XT;
   constexpr int XR = NT / LPR;                     // row pairs per block
   constexpr int E = (N + LPR - 1) / LPR;           // physical elements per lane per row pair
-  constexpr int M = N / 3;
+  constexpr int M = band_m(N, DA);
   constexpr int PF = (M + 1 + LPR - 1) / LPR;      // retained coefficients per lane per row
   static_assert(NCIN == 7 || NCIN == 6, "u, w and optionally phi");
   constexpr int NCOUT = NCIN == 7 ? 6 : 3;
@@ -184,12 +215,12 @@ __global__ __launch_bounds__(XT) void k_xpass(const cx<T> *__restrict__ rin,
       const int kx = lane + LPR * i;
       if (kx <= M) {
         cx<T> a = pa[i], b = pb[i];
-        if (kx == 0) {  // the self-conjugate coefficient: only its real part is seen
+        if (kx == 0 || 2 * kx == N) {  // the self-conjugate coefficients: only their real parts are seen
           a.im = T(0);
           b.im = T(0);
         }
         wb[kx] = {a.re - b.im, a.im + b.re};
-        if (kx > 0) wb[N - kx] = {a.re + b.im, b.re - a.im};
+        if (kx > 0 && 2 * kx != N) wb[N - kx] = {a.re + b.im, b.re - a.im};
       }
     }
     if (c + 1 < NCIN) fetch(c + 1);

@@ -38,10 +38,14 @@ struct ZSched {
   static constexpr int zi_nt[2] = {64 * tile_width_of<T>(), 32 * tile_width_of<T>()};
 };
 
-template <typename T, int N>
+// DA: the dealiasing strategy (DEALIAS_TWO_THIRDS, _HOU_LI, _PHASE_SHIFT), which fixes the retained
+// cube (M, K) and the nonlinear terms' dealiasing in the z forward pass.  The adjoint and the
+// strain-determinant diagnostics are built for the 2/3 rule only.
+template <typename T, int N, int DA = DEALIAS_TWO_THIRDS>
 struct PipelineImpl final : Pipeline {
   static constexpr int TX = tile_width_of<T>();
-  static constexpr int M = N / 3, K = 2 * M + 1;
+  static constexpr int M = band_m(N, DA), K = band_k(N, DA);
+  static constexpr bool TWO_THIRDS = DA == DEALIAS_TWO_THIRDS;
   static constexpr size_t LDS_COL = ((size_t)N * TX + N) * sizeof(cx<T>);   // tile + twiddles
   template <int XT>
   static constexpr int xr() { return XT / xpass_lanes<T, N, XT>(); }
@@ -49,7 +53,7 @@ struct PipelineImpl final : Pipeline {
   static constexpr size_t lds_x() { return ((size_t)xr<XT>() * N + N) * sizeof(cx<T>); }
   // FP32 at N = 512 and 1024: the register-resident x pass, one wavefront per row pair (512) or
   // per row (1024, as a 512-point complex sequence)
-  static constexpr bool XWAVE = std::is_same<T, float>::value && (N == 512 || N == 1024);
+  static constexpr bool XWAVE = TWO_THIRDS && std::is_same<T, float>::value && (N == 512 || N == 1024);
   static constexpr int XW_ROWS = N == 1024 ? 1 : 2;   // rows per wavefront
   template <int XT>
   static constexpr size_t lds_xw() { return (size_t)(XT / 64) * (512 + 512 / 8) * sizeof(cx<T>); }
@@ -74,7 +78,8 @@ struct PipelineImpl final : Pipeline {
   static constexpr size_t LDS_Z = ((size_t)N * ZTX + N) * sizeof(cx<T>);   // z tile + twiddles
 
   // the z forward schedule 0 stages two velocity products in LDS when they fit beside the tile
-  static constexpr bool ZFL = zf_lds_bytes<N, ZTX, T>(true) <= 160 * 1024;
+  static constexpr bool ZFL = zf_lds_bytes<N, ZTX, T, DA>(true) <= 160 * 1024;
+  static constexpr size_t ZFL_BYTES = zf_lds_bytes<N, ZTX, T, DA>(ZFL);
 
   template <int V>
   static void set_all() {
@@ -84,27 +89,31 @@ struct PipelineImpl final : Pipeline {
     set_lds((const void *)k_yinv<T, N, TX, YT[V], 1>, LDS_COL);
     if constexpr (V == 0) {
       constexpr int ZN = ZF_NT[0];
-      set_lds((const void *)k_zs<T, N, ZTX, ZN>, LDS_Z);
-      set_lds((const void *)k_zf<T, N, ZTX, ZN, false, ZFL>, zf_lds_bytes<N, ZTX, T>(ZFL));
-      set_lds((const void *)k_zs<T, N, ZTX, ZN, true>, LDS_Z);
-      set_lds((const void *)k_zs<T, N, ZTX, ZN, false, true>, LDS_Z);
-      set_lds((const void *)k_zs<T, N, ZTX, ZN, true, true>, LDS_Z);
-      set_lds((const void *)k_zf<T, N, ZTX, ZN, false, ZFL, true>, zf_lds_bytes<N, ZTX, T>(ZFL));
+      set_lds((const void *)k_zs<T, N, ZTX, ZN, false, false, DA>, LDS_Z);
+      set_lds((const void *)k_zf<T, N, ZTX, ZN, false, ZFL, false, DA>, ZFL_BYTES);
+      set_lds((const void *)k_zs<T, N, ZTX, ZN, true, false, DA>, LDS_Z);
+      set_lds((const void *)k_zs<T, N, ZTX, ZN, false, true, DA>, LDS_Z);
+      set_lds((const void *)k_zs<T, N, ZTX, ZN, true, true, DA>, LDS_Z);
+      set_lds((const void *)k_zf<T, N, ZTX, ZN, false, ZFL, true, DA>, ZFL_BYTES);
     } else {
       constexpr int ZN = ZF_NT[1];
-      set_lds((const void *)k_zf<T, N, ZTX, ZN, true>, LDS_Z);
-      set_lds((const void *)k_zf<T, N, ZTX, ZN, false>, LDS_Z);
+      set_lds((const void *)k_zf<T, N, ZTX, ZN, true, false, false, DA>, LDS_Z);
+      set_lds((const void *)k_zf<T, N, ZTX, ZN, false, false, false, DA>, LDS_Z);
     }
     constexpr int ZT = ZI_TX[V], ZN = ZI_NT[V];
     constexpr size_t ZL = ((size_t)N * ZT + N) * sizeof(cx<T>);
-    set_lds((const void *)k_zi<T, N, ZT, ZN, true>, ZL);
-    set_lds((const void *)k_zi<T, N, ZT, ZN, false>, ZL);
+    set_lds((const void *)k_zi<T, N, ZT, ZN, true, DA>, ZL);
+    set_lds((const void *)k_zi<T, N, ZT, ZN, false, DA>, ZL);
+    if constexpr (DA == DEALIAS_PHASE_SHIFT) {
+      set_lds((const void *)k_zi<T, N, ZT, ZN, true, DA, true>, ZL);
+      set_lds((const void *)k_zi<T, N, ZT, ZN, false, DA, true>, ZL);
+    }
     if constexpr (XWAVE) {
       set_lds((const void *)k_xpass_wave<T, N, 7, 256, V == 0>, lds_xw<256>());
       set_lds((const void *)k_xpass_wave<T, N, 6, 256, V == 0>, lds_xw<256>());
     } else {
-      set_lds((const void *)k_xpass<T, N, 7, kXThreads[V]>, lds_x<kXThreads[V]>());
-      set_lds((const void *)k_xpass<T, N, 6, kXThreads[V]>, lds_x<kXThreads[V]>());
+      set_lds((const void *)k_xpass<T, N, 7, kXThreads[V], DA>, lds_x<kXThreads[V]>());
+      set_lds((const void *)k_xpass<T, N, 6, kXThreads[V], DA>, lds_x<kXThreads[V]>());
     }
   }
 
@@ -114,16 +123,49 @@ struct PipelineImpl final : Pipeline {
   PipelineImpl() {
     set_all<0>();
     set_all<1>();
-    set_lds((const void *)k_zinv_plain<T, N, TX, ZNT_PLAIN>, LDS_COL);
     set_lds((const void *)k_zfwd_plain<T, N, TX, ZNT_PLAIN>, LDS_COL);
-    set_lds((const void *)k_xadj<T, N, XT_ADJ>, lds_x<XT_ADJ>());
-    set_lds((const void *)k_zstrain<T, N, TX, ZNT_PLAIN>, LDS_COL);
-    set_lds((const void *)k_xdet<T, N, XT_ADJ>, lds_x<XT_ADJ>());
+    if constexpr (TWO_THIRDS) {
+      set_lds((const void *)k_zinv_plain<T, N, TX, ZNT_PLAIN>, LDS_COL);
+      set_lds((const void *)k_xadj<T, N, XT_ADJ>, lds_x<XT_ADJ>());
+      set_lds((const void *)k_zstrain<T, N, TX, ZNT_PLAIN>, LDS_COL);
+      set_lds((const void *)k_xdet<T, N, XT_ADJ>, lds_x<XT_ADJ>());
+    }
+  }
+
+  // --- PhaseShift's displaced evaluation (src/Generator.jl:502-519, :675-704) ------------------
+  // The state translated by half a cell on every axis, recovered on the displaced grid, the
+  // products formed there and transformed forward into c.shifted ([box][ncf][comp], raw and
+  // unshifted: the z forward pass shifts them back and averages them with the primary products).
+  // It overwrites the pencil buffers, so it runs before the primary inverse pass.
+  void shifted(Ctx &c) override {
+    if constexpr (DA == DEALIAS_PHASE_SHIFT) {
+      constexpr int ZT = ZI_TX[0], ZN = ZI_NT[0];
+      constexpr size_t ZL = ((size_t)N * ZT + N) * sizeof(cx<T>);
+      const dim3 grid(c.geo.kxp / ZT, K, c.nbox);
+      if (c.cfg.has_scalar)
+        hipLaunchKernelGGL((k_zi<T, N, ZT, ZN, true, DA, true>), grid, dim3(ZN), ZL, c.stream, (const cx<T> *)c.uh,
+                           (const cx<T> *)c.ph, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, c.band, c.partials);
+      else
+        hipLaunchKernelGGL((k_zi<T, N, ZT, ZN, false, DA, true>), grid, dim3(ZN), ZL, c.stream, (const cx<T> *)c.uh,
+                           (const cx<T> *)c.ph, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, c.band, c.partials);
+      yinv(c);
+      XArgs none{};   // no physical field is kept and no max taken on the displaced grid
+      xpass(c, none);
+      yfwd(c);
+      const size_t comp = c.comp;
+      for (int b = 0; b < c.nbox; ++b)
+        hipLaunchKernelGGL((k_zfwd_plain<T, N, TX, ZNT_PLAIN>), dim3(tiles(c), K, c.geo.ncf), dim3(ZNT_PLAIN), LDS_COL,
+                           c.stream, (const cx<T> *)c.bufb + b * c.geo.box_b, (cx<T> *)c.shifted + (size_t)b * c.geo.ncf * comp,
+                           c.geo, (const cx<double> *)c.tw, T(1));
+    }
   }
 
   // --- the derivative skewness's strain determinant -------------------------------------------
   int strain_det_blocks(const Ctx &) const override { return N * N / (2 * xr<XT_ADJ>()); }
   int strain_det(Ctx &c, double *partials) override {
+    if constexpr (!TWO_THIRDS) {
+      return 0;
+    } else {
     const cx<double> *tw = (const cx<double> *)c.tw;
     cx<T> *pa = (cx<T> *)c.bufa + c.box * c.geo.box_a, *pb = (cx<T> *)c.bufb + c.box * c.geo.box_b;
     hipLaunchKernelGGL((k_zstrain<T, N, TX, ZNT_PLAIN>), dim3(tiles(c), K, 6), dim3(ZNT_PLAIN), LDS_COL, c.stream,
@@ -136,6 +178,7 @@ struct PipelineImpl final : Pipeline {
     hipLaunchKernelGGL((k_xdet<T, N, XT_ADJ>), dim3(blocks), dim3(XT_ADJ), lds_x<XT_ADJ>(), c.stream, (const cx<T> *)pb,
                        c.geo, tw, partials);
     return blocks;
+    }
   }
 
   // --- the adjoint step -------------------------------------------------------------------
@@ -146,16 +189,21 @@ struct PipelineImpl final : Pipeline {
     return std::max((size_t)6 * N * N, (size_t)10 * N * K) * c.geo.kxp;   // R (6) or PF (10)
   }
   void adj_objective(Ctx &c) override {
+    if constexpr (TWO_THIRDS) {
     hipLaunchKernelGGL((k_adj_objective<T>), dim3(ADJ_BLOCKS), dim3(256), 0, c.stream, (cx<T> *)c.adj.lq,
                        (cx<T> *)c.adj.lh, (const cx<T> *)c.uh, (const cx<T> *)c.ph, c.geo, c.cfg.has_scalar);
+    }
   }
   void adj_begin(Ctx &c) override {   // the closing reality and Leray projections, transposed
+    if constexpr (TWO_THIRDS) {
     cx<T> *lq = (cx<T> *)c.adj.lq;
     hipLaunchKernelGGL((k_reality<T>), dim3((K * K + 255) / 256, 1), dim3(256), 0, c.stream, lq, lq + 3 * c.comp, c.geo,
                        c.cfg.has_scalar ? 4 : 3);
     hipLaunchKernelGGL((k_adj_leray<T>), dim3(ADJ_BLOCKS), dim3(256), 0, c.stream, lq, c.geo);
+    }
   }
   void adj_stage(Ctx &c, const AdjStage &as, int i) override {
+    if constexpr (TWO_THIRDS) {
     AdjBufs &a = c.adj;
     const cx<double> *tw = (const cx<double> *)c.tw;
     hipLaunchKernelGGL((k_adj_pre<T>), dim3(ADJ_BLOCKS), dim3(256), 0, c.stream, (const cx<T> *)a.lq, (cx<T> *)a.lh,
@@ -178,6 +226,7 @@ struct PipelineImpl final : Pipeline {
                        (const cx<T> *)a.pb, (cx<T> *)a.t10, c.geo, tw, T(1.0 / ((double)N * N * N)));
     hipLaunchKernelGGL((k_adj_post<T>), dim3(ADJ_BLOCKS), dim3(256), 0, c.stream, (cx<T> *)a.lq, (cx<T> *)a.lh,
                        (const cx<T> *)a.rq[i], (const cx<T> *)a.t10, c.geo, as);
+    }
   }
 
   int tiles(const Ctx &c) const { return c.geo.kxp / TX; }
@@ -205,17 +254,17 @@ struct PipelineImpl final : Pipeline {
     const cx<double> *tw = (const cx<double> *)c.tw;
     if constexpr (V == 0) {
       if (c.cfg.has_scalar)   // the scalar half reads the stage u_z before the velocity half updates it
-        hipLaunchKernelGGL((k_zs<T, N, ZTX, ZN, SSP, SF>), grid, dim3(ZN), LDS_Z, c.stream, pf, (const cx<T> *)uh, ph,
-                           hp, c.geo, tw, sa);
-      hipLaunchKernelGGL((k_zf<T, N, ZTX, ZN, false, ZFL, SSP>), grid, dim3(ZN), (zf_lds_bytes<N, ZTX, T>(ZFL)),
-                         c.stream, pf, uh, hu, ph, hp, c.geo, tw, sa);
+        hipLaunchKernelGGL((k_zs<T, N, ZTX, ZN, SSP, SF, DA>), grid, dim3(ZN), LDS_Z, c.stream, pf, (const cx<T> *)uh,
+                           ph, hp, c.geo, tw, sa);
+      hipLaunchKernelGGL((k_zf<T, N, ZTX, ZN, false, ZFL, SSP, DA>), grid, dim3(ZN), ZFL_BYTES, c.stream, pf, uh, hu, ph,
+                         hp, c.geo, tw, sa);
     } else {
       if (c.cfg.has_scalar)
-        hipLaunchKernelGGL((k_zf<T, N, ZTX, ZN, true>), grid, dim3(ZN), LDS_Z, c.stream, pf, uh, hu, ph, hp, c.geo,
-                           tw, sa);
+        hipLaunchKernelGGL((k_zf<T, N, ZTX, ZN, true, false, false, DA>), grid, dim3(ZN), LDS_Z, c.stream, pf, uh, hu,
+                           ph, hp, c.geo, tw, sa);
       else
-        hipLaunchKernelGGL((k_zf<T, N, ZTX, ZN, false>), grid, dim3(ZN), LDS_Z, c.stream, pf, uh, hu, ph, hp, c.geo,
-                           tw, sa);
+        hipLaunchKernelGGL((k_zf<T, N, ZTX, ZN, false, false, false, DA>), grid, dim3(ZN), LDS_Z, c.stream, pf, uh, hu,
+                           ph, hp, c.geo, tw, sa);
     }
   }
   void zf(Ctx &c, const StageArgs &sa) override {
@@ -230,8 +279,9 @@ struct PipelineImpl final : Pipeline {
 
   void reality(Ctx &c) override {
     const int n = K * K;
-    hipLaunchKernelGGL((k_reality<T>), dim3((n + 255) / 256, c.nbox), dim3(256), 0, c.stream, (cx<T> *)c.uh, (cx<T> *)c.ph,
-                       c.geo, c.cfg.has_scalar ? 4 : 3);
+    const int planes = 2 * M == N ? 2 : 1;   // kx = 0, and kx = N/2 when the strategy keeps it
+    hipLaunchKernelGGL((k_reality<T>), dim3((n + 255) / 256, c.nbox, planes), dim3(256), 0, c.stream, (cx<T> *)c.uh,
+                       (cx<T> *)c.ph, c.geo, c.cfg.has_scalar ? 4 : 3);
   }
 
   template <int V>
@@ -240,10 +290,10 @@ struct PipelineImpl final : Pipeline {
     constexpr size_t ZL = ((size_t)N * ZT + N) * sizeof(cx<T>);
     const dim3 grid(c.geo.kxp / ZT, K, c.nbox);
     if (c.cfg.has_scalar)
-      hipLaunchKernelGGL((k_zi<T, N, ZT, ZN, true>), grid, dim3(ZN), ZL, c.stream, (const cx<T> *)c.uh,
+      hipLaunchKernelGGL((k_zi<T, N, ZT, ZN, true, DA>), grid, dim3(ZN), ZL, c.stream, (const cx<T> *)c.uh,
                          (const cx<T> *)c.ph, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, c.band, c.partials);
     else
-      hipLaunchKernelGGL((k_zi<T, N, ZT, ZN, false>), grid, dim3(ZN), ZL, c.stream, (const cx<T> *)c.uh,
+      hipLaunchKernelGGL((k_zi<T, N, ZT, ZN, false, DA>), grid, dim3(ZN), ZL, c.stream, (const cx<T> *)c.uh,
                          (const cx<T> *)c.ph, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, c.band, c.partials);
   }
   void zi(Ctx &c) override { c.sched[2] ? zi_v<1>(c) : zi_v<0>(c); }
@@ -275,10 +325,10 @@ struct PipelineImpl final : Pipeline {
   void xpass_v(Ctx &c, const XArgs &xa) {
     const int blocks = N * N / (2 * xr<XT>());
     if (c.cfg.has_scalar)
-      hipLaunchKernelGGL((k_xpass<T, N, 7, XT>), dim3(blocks, c.nbox), dim3(XT), lds_x<XT>(), c.stream,
+      hipLaunchKernelGGL((k_xpass<T, N, 7, XT, DA>), dim3(blocks, c.nbox), dim3(XT), lds_x<XT>(), c.stream,
                          (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
     else
-      hipLaunchKernelGGL((k_xpass<T, N, 6, XT>), dim3(blocks, c.nbox), dim3(XT), lds_x<XT>(), c.stream,
+      hipLaunchKernelGGL((k_xpass<T, N, 6, XT, DA>), dim3(blocks, c.nbox), dim3(XT), lds_x<XT>(), c.stream,
                          (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
   }
   void xpass(Ctx &c, const XArgs &xa) override {
@@ -357,6 +407,20 @@ std::unique_ptr<Pipeline> make_pipeline_t(int n) {
     case 256: return std::make_unique<PipelineImpl<T, 256>>();
     case 512: return std::make_unique<PipelineImpl<T, 512>>();
     case 1024: return std::make_unique<PipelineImpl<T, 1024>>();
+    default: return nullptr;
+  }
+}
+
+// HouLi and PhaseShift pipelines: the grids they are built for (WIDE_GRIDS in the context code).
+template <typename T, int DA>
+std::unique_ptr<Pipeline> make_pipeline_wide_t(int n) {
+  switch (n) {
+    case 16: return std::make_unique<PipelineImpl<T, 16, DA>>();
+    case 32: return std::make_unique<PipelineImpl<T, 32, DA>>();
+    case 48: return std::make_unique<PipelineImpl<T, 48, DA>>();
+    case 64: return std::make_unique<PipelineImpl<T, 64, DA>>();
+    case 128: return std::make_unique<PipelineImpl<T, 128, DA>>();
+    case 256: return std::make_unique<PipelineImpl<T, 256, DA>>();
     default: return nullptr;
   }
 }

@@ -9,9 +9,6 @@
 
 namespace hdns {
 
-// The canonical 2/3 mask: the closed ball 9|m|^2 <= N^2 in exact integers
-// (src/SimulationDomain.jl:342-376 builds the ball of radius (2/3)(N/2)(2 pi/L)).
-__host__ __device__ __forceinline__ bool in_two_thirds(int m2, int n) { return 9 * m2 <= n * n; }
 
 struct BandSpec {
   int on;      // a band forcing present (every forcing but NoForcing)
@@ -47,7 +44,28 @@ struct StageArgs {
   int ssp;
   int pa, pb;
   double ca, cb;          // the Shu-Osher weights, rounded to the element type
+  // HouLi (src/SimulationDomain.jl:365-376): the filter exp(-s (|k|/k_Ny)^p) of each |m|^2,
+  // 0 .. 3 (N/2)^2, applied to the nonlinear terms (src/Generator.jl:502-519, :668-704)
+  const double *filter;
+  // PhaseShift (src/Generator.jl:502-519, :675-704): the raw transforms of the products formed on
+  // the grid displaced by half a cell, per box [ncf][comp] compact (the Lamb vector, then the flux)
+  const void *shifted;
 };
+
+// PhaseShift's translation of a mode by half a cell along every axis: e^{i s pi (mx + my + mz)/N},
+// s = +1 samples the field on the displaced grid, s = -1 shifts a displaced term back
+// (src/Generator.jl:166-175).  FP64, rounded to T.
+template <typename T>
+__device__ __forceinline__ cx<T> half_cell_phase(int msum, int n, int sign) {
+  double sn, cs;
+  sincospi((double)(sign * msum) / (double)n, &sn, &cs);
+  return {T(cs), T(sn)};
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(const cx<T> &a, const cx<T> &b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
 
 // Radix of the first and the last pass of a length-N transform (fft_io's plans).
 template <int N>
@@ -58,12 +76,12 @@ constexpr int last_radix() { return N % 3 == 0 ? 3 : ((N == 64 || N == 512) ? 8 
 // The scalar half of a stage (src/Generator.jl:648-730, src/Propagator.jl:366-391): the three
 // fluxes' forward z transforms, N_phi = -mask i k . F - G u_z at the stage velocity, and the
 // phi / h_phi update.  It runs before the velocity update, which overwrites u_z.
-template <typename T, int N, int TX, int NT, bool SSP, bool SF>
+template <typename T, int N, int TX, int NT, bool SSP, bool SF, int DA = DEALIAS_TWO_THIRDS>
 __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, const cx<T> *__restrict__ pf,
                                                const cx<T> *__restrict__ uh, cx<T> *__restrict__ ph,
                                                cx<T> *__restrict__ hp, const Geo &g, const StageArgs &sa, int kx0,
                                                int ky_idx, int my, int box) {
-  constexpr int M = N / 3, K = 2 * M + 1;
+  constexpr int K = band_k(N, DA);
   constexpr int RL = last_radix<N>();
   using SL = PassShape<N, RL, TX, NT>;
   constexpr int PER = SL::PER;
@@ -112,7 +130,7 @@ __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, con
       for (int i = 0; i < QS; ++i) {
         const int kz_idx = idx_of_bin(j + (q0 + i) * SL::NB, g);
         const int mz = kz_idx < 0 ? 0 : signed_of(kz_idx, g);
-        at[i] = kz_idx >= 0 && in_two_thirds(mx * mx + my * my + mz * mz, N)
+        at[i] = kz_idx >= 0 && keeps(mx * mx + my * my + mz * mz, g)
                     ? kz_idx * (int)zstride + ky_idx * g.kxp + mx
                     : -1;
         const int ld = at[i] < 0 ? 0 : at[i];
@@ -133,7 +151,17 @@ __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, con
         const int mz = signed_of(idx_of_bin(j + q * SL::NB, g), g);
         const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
         const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
-        cx<T> nl = cx<T>{G[p][q].im, -G[p][q].re} - scale(uz[i], Gm);   // -i k.F - G u_z
+        cx<T> kF = G[p][q];   // k . F
+        if constexpr (DA == DEALIAS_PHASE_SHIFT) {   // the two-grid average, the displaced term shifted back
+          const cx<T> *sh = reinterpret_cast<const cx<T> *>(sa.shifted) + (size_t)box * g.ncf * comp + at[i];
+          const cx<T> back = half_cell_phase<T>(mx + my + mz, N, -1);
+          const cx<T> ks = cmul(scale(sh[3 * comp], T(g.k0 * mx)) + scale(sh[4 * comp], T(g.k0 * my)) +
+                                    scale(sh[5 * comp], T(g.k0 * mz)),
+                                back);
+          kF = scale(kF + ks, T(0.5));
+        }
+        if constexpr (DA == DEALIAS_HOU_LI) kF = scale(kF, T(sa.filter[mx * mx + my * my + mz * mz]));
+        cx<T> nl = cx<T>{kF.im, -kF.re} - scale(uz[i], Gm);   // -i k.F - G u_z
         double xe = coef_p * k2d;   // the scalar's integrating-factor exponent over the substep
         if constexpr (SF) {
           if (in_band(mx * mx + my * my + mz * mz, sa.sband)) {   // src/Forcing.jl:1104-1117, :600-642
@@ -176,17 +204,17 @@ __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, con
 // transformed velocity products wait in LDS (retained bins only, [kz_idx][col]) instead of
 // registers, so the stage update holds only the third and more state loads can be in flight;
 // each thread reads back only what it wrote itself.
-template <int N, int TX, typename T>
+template <int N, int TX, typename T, int DA = DEALIAS_TWO_THIRDS>
 constexpr size_t zf_lds_bytes(bool fl) {
-  return ((size_t)N * TX + N + (fl ? 2 * (size_t)(2 * (N / 3) + 1) * TX : 0)) * sizeof(cx<T>);
+  return ((size_t)N * TX + N + (fl ? 2 * (size_t)band_k(N, DA) * TX : 0)) * sizeof(cx<T>);
 }
 
-template <typename T, int N, int TX, int NT, bool SCALAR, bool FL = false, bool SSP = false>
+template <typename T, int N, int TX, int NT, bool SCALAR, bool FL = false, bool SSP = false, int DA = DEALIAS_TWO_THIRDS>
 __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *__restrict__ uh,
                                            cx<T> *__restrict__ hu, cx<T> *__restrict__ ph,
                                            cx<T> *__restrict__ hp, Geo g, const cx<double> *__restrict__ tw,
                                            StageArgs sa) {
-  constexpr int M = N / 3, K = 2 * M + 1;
+  constexpr int K = band_k(N, DA);
   constexpr int RL = last_radix<N>();
   using SL = PassShape<N, RL, TX, NT>;
   constexpr int PER = SL::PER;
@@ -198,7 +226,7 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
   const int tid = threadIdx.x;
   const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y, box = blockIdx.z;
   const int my = signed_of(ky_idx, g);
-  if (!in_disk(kx0, my, N)) return;   // no column of this tile meets the ball (its PF lines are unwritten)
+  if (!in_disk(kx0, my, g)) return;   // no column of this tile meets the ball (its PF lines are unwritten)
   stage_twiddles<T, N, NT>(twl, tw);
   const T ky = T(g.k0 * my);
   const size_t comp = (size_t)K * K * g.kxp;           // one compact component
@@ -212,7 +240,8 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
     hp += box * comp;
   }
 
-  if constexpr (SCALAR) zf_scalar_part<T, N, TX, NT, SSP, false>(buf, twl, pf, uh, ph, hp, g, sa, kx0, ky_idx, my, box);
+  if constexpr (SCALAR)
+    zf_scalar_part<T, N, TX, NT, SSP, false, DA>(buf, twl, pf, uh, ph, hp, g, sa, kx0, ky_idx, my, box);
 
   // velocity products, forward z transform, straight into the owned modes.  The compiler
   // memory barrier keeps the scalar part's u_z loads from living on into the velocity part.
@@ -265,7 +294,7 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
       for (int i = 0; i < QG; ++i) {
         const int kz_idx = idx_of_bin(j + (q0 + i) * SL::NB, g);
         const int mz = kz_idx < 0 ? 0 : signed_of(kz_idx, g);
-        at[i] = kz_idx >= 0 && in_two_thirds(mx * mx + my * my + mz * mz, N)
+        at[i] = kz_idx >= 0 && keeps(mx * mx + my * my + mz * mz, g)
                     ? kz_idx * (int)zstride + ky_idx * g.kxp + mx
                     : -1;
         const int ld = at[i] < 0 ? 0 : at[i];
@@ -296,12 +325,24 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
         cx<T> nl[3];
         if (m2 > 0) {
           const T k2 = T(k2d);
-          const cx<T> f0 = Fv(0, p, q, kz_idx, col), f1 = Fv(1, p, q, kz_idx, col), f2 = Fv(2, p, q, kz_idx, col);
+          cx<T> f0 = Fv(0, p, q, kz_idx, col), f1 = Fv(1, p, q, kz_idx, col), f2 = Fv(2, p, q, kz_idx, col);
+          if constexpr (DA == DEALIAS_PHASE_SHIFT) {   // the two-grid average, the displaced term shifted back
+            const cx<T> *sh = reinterpret_cast<const cx<T> *>(sa.shifted) + (size_t)box * g.ncf * comp + at[i];
+            const cx<T> back = half_cell_phase<T>(mx + my + mz, N, -1);
+            f0 = scale(f0 + cmul(sh[0], back), T(0.5));
+            f1 = scale(f1 + cmul(sh[comp], back), T(0.5));
+            f2 = scale(f2 + cmul(sh[2 * comp], back), T(0.5));
+          }
           const cx<T> kf = {kx * f0.re + ky * f1.re + kz * f2.re, kx * f0.im + ky * f1.im + kz * f2.im};
           const cx<T> s = {kf.re / k2, kf.im / k2};
           nl[0] = f0 - scale(s, kx);
           nl[1] = f1 - scale(s, ky);
           nl[2] = f2 - scale(s, kz);
+          if constexpr (DA == DEALIAS_HOU_LI) {
+            const T F = T(sa.filter[m2]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) nl[c] = scale(nl[c], F);
+          }
         } else {
           nl[0] = nl[1] = nl[2] = czero<T>();
         }
@@ -356,7 +397,7 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
 
 // The scalar half on its own (the velocity half then runs as k_zf without SCALAR): two smaller
 // kernels hold fewer registers than the fused one.
-template <typename T, int N, int TX, int NT, bool SSP = false, bool SF = false>
+template <typename T, int N, int TX, int NT, bool SSP = false, bool SF = false, int DA = DEALIAS_TWO_THIRDS>
 __global__ __launch_bounds__(NT) void k_zs(const cx<T> *__restrict__ pf, const cx<T> *__restrict__ uh,
                                            cx<T> *__restrict__ ph, cx<T> *__restrict__ hp, Geo g,
                                            const cx<double> *__restrict__ tw, StageArgs sa) {
@@ -365,10 +406,10 @@ __global__ __launch_bounds__(NT) void k_zs(const cx<T> *__restrict__ pf, const c
   cx<T> *twl = buf + N * TX;
   const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y, box = blockIdx.z;
   const int my = signed_of(ky_idx, g);
-  if (!in_disk(kx0, my, N)) return;   // no column of this tile meets the ball (its PF lines are unwritten)
+  if (!in_disk(kx0, my, g)) return;   // no column of this tile meets the ball (its PF lines are unwritten)
   stage_twiddles<T, N, NT>(twl, tw);
   const size_t comp = comp_of(g);
-  zf_scalar_part<T, N, TX, NT, SSP, SF>(buf, twl, pf + box * g.box_b, uh + box * 3 * comp, ph + box * comp,
+  zf_scalar_part<T, N, TX, NT, SSP, SF, DA>(buf, twl, pf + box * g.box_b, uh + box * 3 * comp, ph + box * comp,
                                hp + box * comp, g, sa, kx0, ky_idx, my, box);
 }
 
@@ -382,11 +423,13 @@ __global__ __launch_bounds__(NT) void k_zs(const cx<T> *__restrict__ pf, const c
 // columns from the last pass straight to HBM.
 constexpr int NRED = 5;
 
-template <typename T, int N, int TX, int NT, bool SCALAR>
+// SHIFT (PhaseShift's displaced evaluation, src/Generator.jl:505-513): the state is first
+// translated by half a cell along every axis, so Q holds u, w and phi sampled on the displaced grid.
+template <typename T, int N, int TX, int NT, bool SCALAR, int DA = DEALIAS_TWO_THIRDS, bool SHIFT = false>
 __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const cx<T> *__restrict__ ph,
                                            cx<T> *__restrict__ q, Geo g, const cx<double> *__restrict__ tw,
                                            BandSpec band, double *__restrict__ partials) {
-  constexpr int M = N / 3, K = 2 * M + 1;
+  constexpr int M = band_m(N, DA), K = band_k(N, DA);
   constexpr int R1 = first_radix<N>();
   using S1 = PassShape<N, R1, TX, NT>;
   constexpr int PER = S1::PER;
@@ -400,7 +443,7 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
   const int my = signed_of(ky_idx, g);
   // the box's partials follow those of the boxes before it
   const size_t part = ((size_t)box * gridDim.y * gridDim.x + (size_t)blockIdx.y * gridDim.x + blockIdx.x) * NRED;
-  if (!in_disk(line_start<T>(kx0), my, N)) {   // the tile's line misses the ball: never read
+  if (!in_disk(line_start<T>(kx0), my, g)) {   // the tile's line misses the ball: never read
     if (tid < NRED) partials[part + tid] = 0.0;
     return;
   }
@@ -426,7 +469,7 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
     for (int qq = 0; qq < R1; ++qq) {
       const int kz_idx = live ? idx_of_bin(j + qq * S1::NB, g) : -1;
       const int mz = kz_idx < 0 ? 0 : signed_of(kz_idx, g);
-      if (kz_idx < 0 || !in_two_thirds(mx * mx + my * my + mz * mz, N)) {   // zero outside the ball
+      if (kz_idx < 0 || !keeps(mx * mx + my * my + mz * mz, g)) {   // zero outside the ball
 #pragma unroll
         for (int c = 0; c < 3; ++c) u[c][pp][qq] = czero<T>();
         p[pp][qq] = czero<T>();
@@ -435,18 +478,24 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
       const size_t at = (size_t)kz_idx * zstride + (size_t)ky_idx * g.kxp + mx;
       const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
       const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
-      const double w = mx == 0 ? 1.0 : 2.0;
+      const double w = parseval_w(mx, N);
       double e2 = 0.0;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         u[c][pp][qq] = uh[c * comp + at];
         e2 += (double)u[c][pp][qq].re * (double)u[c][pp][qq].re + (double)u[c][pp][qq].im * (double)u[c][pp][qq].im;
       }
+      if constexpr (SHIFT) {
+        const cx<T> sh = half_cell_phase<T>(mx + my + mz, N, +1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[c][pp][qq] = cmul(u[c][pp][qq], sh);
+      }
       acc[0] += w * e2;
       if (in_band(mx * mx + my * my + mz * mz, band)) acc[1] += w * e2;
       acc[2] += w * k2d * e2;
       if constexpr (SCALAR) {
         p[pp][qq] = ph[at];
+        if constexpr (SHIFT) p[pp][qq] = cmul(p[pp][qq], half_cell_phase<T>(mx + my + mz, N, +1));
         const double v2 = (double)p[pp][qq].re * (double)p[pp][qq].re + (double)p[pp][qq].im * (double)p[pp][qq].im;
         acc[3] += w * v2;
         acc[4] += w * k2d * v2;
@@ -503,13 +552,15 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
   }
 }
 
-// reality_projection! on the kx = 0 plane of u_hat and phi_hat (src/Propagator.jl:411-412): every
-// conjugate pair becomes (f(k) + conj f(-k))/2; the self-paired mean mode keeps its real part.
+// reality_projection! on the kx = 0 plane of u_hat and phi_hat, and on the kx = N/2 plane when the
+// strategy keeps it (src/Propagator.jl:411-412): every conjugate pair becomes (f(k) + conj f(-k))/2;
+// a self-paired mode keeps its real part.  Grid: (K^2 / 256, boxes, planes).
 template <typename T>
 __global__ void k_reality(cx<T> *__restrict__ uh, cx<T> *__restrict__ ph, Geo g, int ncomp) {
   const int K = g.k;
   const int l = blockIdx.x * blockDim.x + threadIdx.x, box = blockIdx.y;
   if (l >= K * K) return;
+  const int plane = blockIdx.z == 0 ? 0 : g.n / 2;   // the kx column of the plane
   uh += box * 3 * comp_of(g);
   if (ncomp > 3) ph += box * comp_of(g);
   const int kz_idx = l / K, ky_idx = l % K;
@@ -517,8 +568,8 @@ __global__ void k_reality(cx<T> *__restrict__ uh, cx<T> *__restrict__ ph, Geo g,
   const int lp = pz * K + py;
   if (lp < l) return;
   const size_t comp = (size_t)K * K * g.kxp;
-  const size_t a0 = (size_t)kz_idx * K * g.kxp + (size_t)ky_idx * g.kxp;
-  const size_t a1 = (size_t)pz * K * g.kxp + (size_t)py * g.kxp;
+  const size_t a0 = (size_t)kz_idx * K * g.kxp + (size_t)ky_idx * g.kxp + plane;
+  const size_t a1 = (size_t)pz * K * g.kxp + (size_t)py * g.kxp + plane;
   for (int c = 0; c < ncomp; ++c) {
     cx<T> *f = c < 3 ? uh + c * comp : ph;
     const cx<T> x = f[a0], y = f[a1];
@@ -604,13 +655,14 @@ template <typename T>
 __global__ void __launch_bounds__(SPEC_THREADS) k_spectrum_shells(const cx<T> *__restrict__ f, int ncomp, Geo g,
                                                                    double *__restrict__ partials) {
   const int n = blockIdx.x, s = blockIdx.y, t = threadIdx.x;
-  const int L = min(n, g.m), side = 2 * L + 1;
+  // the stored signed indices run from -(K - 1 - M) to M (under HouLi the Nyquist index is +N/2 only)
+  const int hi_m = min(n, g.m), lo_m = -min(n, g.k - 1 - g.m), side = hi_m - lo_m + 1;
   const long rows = (long)side * side;
   const int hi2 = (2 * n + 1) * (2 * n + 1), lo2 = n > 0 ? (2 * n - 1) * (2 * n - 1) : 0;
   const size_t comp = comp_of(g);
   double acc = 0.0;
   for (long j = s + (long)SPEC_CHUNKS * t; j < rows; j += (long)SPEC_CHUNKS * SPEC_THREADS) {
-    const int my = (int)(j % side) - L, mz = (int)(j / side) - L;
+    const int my = (int)(j % side) + lo_m, mz = (int)(j / side) + lo_m;
     const int r4 = 4 * (my * my + mz * mz);
     if (r4 >= hi2) continue;
     int lo = 0;
@@ -655,7 +707,7 @@ static __global__ void k_spectrum_sum(const double *__restrict__ partials, int n
 
 // The frame tables as the device computes them: signed indices of the three axes and the mask
 // over the reference's full half-spectrum layout [N][N][N/2+1].
-static __global__ void k_frame_tables(int n, int *mx, int *my, int *mz, unsigned char *mask) {
+static __global__ void k_frame_tables(int n, int ball9, int *mx, int *my, int *mz, unsigned char *mask) {
   const size_t hx = n / 2 + 1;
   const size_t total = (size_t)n * n * hx;
   for (size_t l = blockIdx.x * (size_t)blockDim.x + threadIdx.x; l < total; l += (size_t)gridDim.x * blockDim.x) {
@@ -664,7 +716,7 @@ static __global__ void k_frame_tables(int n, int *mx, int *my, int *mz, unsigned
     if (iy == 0 && iz == 0) mx[ix] = sx;
     if (ix == 0 && iz == 0) my[iy] = sy;
     if (ix == 0 && iy == 0) mz[iz] = sz;
-    mask[l] = in_two_thirds(sx * sx + sy * sy + sz * sz, n) ? 1 : 0;
+    mask[l] = 9 * (sx * sx + sy * sy + sz * sz) <= ball9 ? 1 : 0;
   }
 }
 

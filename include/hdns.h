@@ -35,7 +35,8 @@ typedef enum {
 enum { HDNS_FP32_BULK = 0, HDNS_FP64 = 1 };
 /* IFLowStorageRK3, IFLowStorageRK4 (src/Propagator.jl:264-320), IFSSPRK3 (:596-691) and IFHeun (:420-544) */
 enum { HDNS_IF_LOW_STORAGE_RK3 = 3, HDNS_IF_LOW_STORAGE_RK4 = 4, HDNS_IF_SSP_RK3 = 5, HDNS_IF_HEUN = 6 };
-enum { HDNS_TWO_THIRDS = 0 };
+/* TwoThirds, HouLi and PhaseShift dealiasing (src/SimulationDomain.jl:44-195) */
+enum { HDNS_TWO_THIRDS = 0, HDNS_HOU_LI = 1, HDNS_PHASE_SHIFT = 2 };
 /* NoForcing, ConstantPower, Linear, VarianceController, VarianceRescaling and StochasticOU
  * (src/Forcing.jl:464, :714-719, :600-642, :817-871, :566-597, :1120-1413) */
 enum { HDNS_NO_FORCING = 0, HDNS_CONSTANT_POWER = 1, HDNS_LINEAR = 2, HDNS_VARIANCE_CONTROLLER = 3,
@@ -56,7 +57,7 @@ typedef struct {
   double side_length;        /* L */
   int32_t precision;         /* HDNS_FP32_BULK or HDNS_FP64 */
   int32_t scheme;            /* HDNS_IF_LOW_STORAGE_RK3 / _RK4, HDNS_IF_SSP_RK3, HDNS_IF_HEUN */
-  int32_t dealiasing;        /* HDNS_TWO_THIRDS */
+  int32_t dealiasing;        /* HDNS_TWO_THIRDS, HDNS_HOU_LI or HDNS_PHASE_SHIFT (N in 16..256 but 96 and 192) */
   int32_t has_scalar;        /* evolve the passive scalar */
   double viscosity;          /* nu > 0 */
   double schmidt;            /* Sc > 0 */
@@ -85,6 +86,11 @@ typedef struct {
   double scalar_target;          /* VarianceController's or VarianceRescaling's target */
   double scalar_gain;            /* VarianceController G >= 0 */
   double scalar_time;            /* VarianceController turnover time > 0 */
+  /* HouLi(order, strength) (src/SimulationDomain.jl:131-138, :365-376): the filter
+   * exp(-strength (|k|/k_Ny)^order) on the nonlinear terms; strength NaN is the reference's default
+   * -log(eps(T)) of the element type */
+  int32_t filter_order;          /* >= 1 */
+  double filter_strength;
 } hdns_config;
 
 typedef struct hdns_ctx hdns_ctx;

@@ -11,7 +11,8 @@ The forward transform is unnormalised and the inverse carries 1/N^3
 Canonical choices where the reference delegates to the field library and its source is not
 available (each one is made identically by the HIP library):
   * 2/3 mask: keep a mode iff 9*(mx^2+my^2+mz^2) <= N^2, the closed ball of radius
-    (2/3)(N/2)(2pi/L) in exact integers (src/SimulationDomain.jl:342-376).
+    (2/3)(N/2)(2pi/L) in exact integers (src/SimulationDomain.jl:342-376); PhaseShift's ball
+    9|m|^2 <= 2 N^2; HouLi's filter exp(-s (|m|/(N/2))^p) evaluated in FP64.
   * shell band of shells lo..hi: 4|m|^2 >= (2lo-1)^2 and 4|m|^2 < (2hi+1)^2 (src/Forcing.jl:435-439);
     no lattice mode lies on an edge, so the open/closed choice is immaterial.
   * spectrum shell of a mode: the integer n with (2n-1)^2 <= 4|m|^2 < (2n+1)^2.
@@ -75,7 +76,13 @@ class Domain:
     planes, 1 on kx = 0 and kx = N/2).
     """
 
-    def __init__(self, n: int, side_length: float = 2 * math.pi):
+    def __init__(self, n: int, side_length: float = 2 * math.pi, dealiasing: str = "two_thirds",
+                 filter_order: int = 36, filter_strength: float | None = None, element_type=np.float64):
+        """``dealiasing`` is "two_thirds", "hou_li" (with ``filter_order`` and ``filter_strength``,
+        None for -log(eps(element_type))) or "phase_shift" (src/SimulationDomain.jl:44-195)."""
+        if dealiasing not in ("two_thirds", "hou_li", "phase_shift"):
+            raise ValueError(f"unknown dealiasing {dealiasing!r}")
+        self.dealiasing = dealiasing
         self.n = int(n)
         self.side_length = float(side_length)
         self.k0 = 2 * math.pi / self.side_length
@@ -96,7 +103,17 @@ class Domain:
                      np.broadcast_to(ky, self.m2.shape),
                      np.broadcast_to(kz, self.m2.shape))
         self.k2 = (kx * kx + ky * ky) + kz * kz
-        self.mask = (9 * self.m2 <= self.n * self.n)
+        # the retained ball: the closed 2/3 ball, PhaseShift's ball of radius sqrt(2) N/3
+        # (src/SimulationDomain.jl:344-346), or every mode under HouLi
+        if dealiasing == "hou_li":
+            self.mask = np.ones(self.m2.shape, dtype=bool)
+            s = -math.log(float(np.finfo(element_type).eps)) if filter_strength is None else float(filter_strength)
+            self.filter_strength = s
+            # src/SimulationDomain.jl:365-376: exp(-strength (|k|/k_Ny)^order), |k|/k_Ny = |m|/(N/2)
+            self.dealias = np.exp(-s * (np.sqrt(self.m2.astype(np.float64)) / (self.n / 2)) ** int(filter_order))
+        else:
+            self.mask = (9 * self.m2 <= (2 if dealiasing == "phase_shift" else 1) * self.n * self.n)
+            self.dealias = self.mask.astype(np.float64)
         w = np.full(self.n // 2 + 1, 2.0)
         w[0] = 1.0
         if self.n % 2 == 0:
@@ -106,8 +123,17 @@ class Domain:
 
     @property
     def max_retained_wavenumber(self) -> float:
-        """(2/3)(N/2)(2pi/L) (src/SimulationDomain.jl:181-195)."""
-        return (2.0 / 3.0) * (self.n / 2) * self.k0
+        """f_ret (N/2)(2pi/L), f_ret = 2/3, 1 (HouLi) or 2 sqrt 2 / 3 (PhaseShift) (src/SimulationDomain.jl:181-195)."""
+        f = {"two_thirds": 2.0 / 3.0, "hou_li": 1.0, "phase_shift": 2.0 * math.sqrt(2.0) / 3.0}[self.dealiasing]
+        return f * (self.n / 2) * self.k0
+
+    def half_cell_shift(self, sign: int) -> np.ndarray:
+        """e^{i sign kappa . delta} with delta half a cell along every axis, kappa . delta =
+        pi (mx + my + mz) / N: sign +1 samples a field on the displaced grid, -1 shifts back
+        (PhaseShift's translation kernels, src/Generator.jl:166-175).  The dealiased average does not
+        depend on the sign convention: the aliases it cancels carry (-1)^(wraps) either way."""
+        msum = self.mx[None, None, :] + self.my[None, :, None] + self.mz[:, None, None]
+        return np.exp(1j * sign * np.pi * msum / self.n)
 
     def shell_band(self, lowest: int, highest) -> np.ndarray:
         """Indicator of shells lowest..highest, [h-1/2, h+1/2) in units of 2pi/L (src/Forcing.jl:428-445)."""
@@ -332,7 +358,11 @@ def velocity_rhs(d: Domain, cond: Conditions, uh: np.ndarray, u: np.ndarray, w: 
     """
     c = np.cross(u, w, axis=0)
     f = rfft3(c)
-    nl = leray(d, f * d.mask)
+    if d.dealiasing == "phase_shift":   # src/Generator.jl:502-519: average with the displaced evaluation
+        us_hat = uh * d.half_cell_shift(+1)
+        us, ws = irfft3(us_hat, d.n), irfft3(curl(d, us_hat), d.n)
+        f = 0.5 * (f + d.half_cell_shift(-1) * rfft3(np.cross(us, ws, axis=0)))
+    nl = leray(d, f * d.dealias)
     nl[:, 0, 0, 0] = 0.0
     if cond.forcing is not None:
         amp = constant_power_amplitude(d, cond, uh)
@@ -357,9 +387,13 @@ def scalar_rhs(d: Domain, cond: Conditions, uh: np.ndarray, u: np.ndarray, phi: 
     """
     flux = u * phi[None]
     f = rfft3(flux)
+    if d.dealiasing == "phase_shift":   # src/Generator.jl:675-704: average with the displaced flux
+        sh = d.half_cell_shift(+1)
+        us, phs = irfft3(uh * sh, d.n), irfft3(ph * sh, d.n)
+        f = 0.5 * (f + d.half_cell_shift(-1) * rfft3(us * phs[None]))
     kx, ky, kz = d.kvec
     div = 1j * (kx * f[0] + ky * f[1] + kz * f[2])
-    nl = -(d.mask * div)
+    nl = -(d.dealias * div)
     if cond.mean_gradient != 0.0:
         nl = nl - cond.mean_gradient * uh[2]
     sf = cond.scalar_forcing

@@ -502,3 +502,43 @@ def test_the_seeded_spectrum_follows_k4_exp_gaussian_with_half_unit_energy(g, n)
     fit = np.linalg.lstsq(np.stack([np.ones(len(kk)), np.log(kk), (kk / 4.0) ** 2], 1), np.log(ratio), rcond=None)[0]
     assert abs(fit[1]) < 0.3 and abs(fit[2]) < 0.05
     assert int(np.argmax(E)) == 4                                 # the peak at k_p = 4
+
+
+def test_the_dealiasing_strategies_fix_the_bands_the_cfl_and_the_viscosity(g):
+    """src/SimulationDomain.jl:181-195,306-457 and src/Propagator.jl:919: the retained and resolved
+    Nyquist fractions of TwoThirds (2/3, 2/3), HouLi (1, 4/5) and PhaseShift (2 sqrt2/3 both); the
+    CFL ceiling scaled by (2/3)/f_ret; the viscosity from the resolved band.  The published HouLi
+    comparison (docs/src/validation.md:29-46): 128^3, eps = 1, k_max eta = 1.5 on the resolved
+    edge 0.8 k_Ny = 51.2 gives nu = 9.032e-3."""
+    two, hl, ps = g.TwoThirds(), g.HouLi(), g.PhaseShift()
+    doms = {k: g.FixedDomain(grid_points_per_axis=128, side_length=L, dealiasing=v, element_type=np.float64)
+            for k, v in (("two", two), ("hl", hl), ("ps", ps))}
+    assert doms["two"].max_retained_wavenumber() == pytest.approx(128 / 3, rel=1e-15)
+    assert doms["hl"].max_retained_wavenumber() == 64.0 and doms["hl"].max_resolved_wavenumber() == pytest.approx(51.2)
+    assert doms["ps"].max_retained_wavenumber() == pytest.approx(2 * math.sqrt(2) / 3 * 64, rel=1e-15)
+    assert g.recommended_cfl(g.IFLowStorageRK4, doms["two"]) == 0.96
+    assert g.recommended_cfl(g.IFSSPRK3, doms["hl"]) == pytest.approx(0.5 * 2 / 3, rel=1e-15)
+    assert g.recommended_cfl(g.IFSSPRK3, doms["ps"]) == pytest.approx(0.5 / math.sqrt(2), rel=1e-15)
+    nu = g.predicted_viscosity(doms["hl"], 1.0, kmax_eta=1.5)
+    assert nu == pytest.approx(9.032e-3, rel=1e-3)
+    assert nu == pytest.approx((1.5 / 51.2) ** (4 / 3), rel=1e-12)
+    assert hl.strength_for(np.float64) == pytest.approx(36.04365338911715, rel=1e-15)
+    assert hl.strength_for(np.float32) == pytest.approx(15.942385152878742, rel=1e-15)
+    # s = 36 (1 - 2/N)^-36 at N = 128 matches TurTLE's (N/2 - 1) normalisation: 63.463
+    assert 36 * (1 - 2 / 128) ** -36 == pytest.approx(63.463, abs=5e-4)
+    for bad in (0, -3, 2.5):
+        with pytest.raises(ValueError, match="order"):
+            g.HouLi(order=bad)
+    with pytest.raises(NotImplementedError, match="HouLi and PhaseShift"):
+        g.FixedDomain(grid_points_per_axis=512, side_length=L, dealiasing=hl, element_type=np.float32)
+
+
+def test_truncate_to_band_under_each_strategy(g):
+    """truncate_to_band! applies the dealias kernel (src/SimulationDomain.jl:520-537): the closed
+    balls 9|m|^2 <= N^2 (TwoThirds) and <= 2 N^2 (PhaseShift), and the HouLi filter's multiply."""
+    for dl, kind in ((g.TwoThirds(), "two_thirds"), (g.PhaseShift(), "phase_shift"), (g.HouLi(order=8, strength=3.0), "hou_li")):
+        dom = g.FixedDomain(grid_points_per_axis=32, side_length=L, dealiasing=dl, element_type=np.float64)
+        f32 = np.ones((32, 32, 17), dtype=complex)
+        got = g.truncate_to_band(f32, dom)
+        od = dns.Domain(32, dealiasing=kind, filter_order=8, filter_strength=3.0)
+        assert np.allclose(got, od.dealias, rtol=1e-15, atol=0)

@@ -623,3 +623,124 @@ def test_scalar_rescaling_holds_its_target_after_every_step(over):
         u, p, dep = dns.step(d, cond, "rk3", u, p, 1e-2, dep)
         held = dns.weighted_inner(d, p, d.shell_band(1, 2)) if over == "band" else dns.weighted_inner(d, p)
         assert held == pytest.approx(0.45, rel=1e-13)
+
+
+# --- HouLi and PhaseShift (src/SimulationDomain.jl:76-195, src/Generator.jl:153-182,502-519,668-704) ---
+
+def _field_in(d, seed, keep):
+    """A random solenoidal, Hermitian velocity and a random scalar, nonzero only where `keep`."""
+    rng = np.random.default_rng(seed)
+    shape = (3,) + d.m2.shape
+    f = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) * keep
+    f[:, 0, 0, 0] = 0
+    f = dns.reality_projection(d, dns.leray(d, f))
+    p = (rng.standard_normal(d.m2.shape) + 1j * rng.standard_normal(d.m2.shape)) * keep
+    p[0, 0, 0] = 0
+    return f, dns.reality_projection(d, p[None])[0]
+
+
+def _exact_products(d, uh, ph):
+    """The unaliased spectra of u x w and of -i k . (u phi) on the N grid, from a 2N grid on which
+    no product of the N grid's modes aliases (every N-grid mode, Nyquist planes included, is placed
+    at its own signed wavenumber)."""
+    n, n2 = d.n, 2 * d.n
+    d2 = dns.Domain(n2)
+    idx = np.array([i if i < n // 2 else i + n for i in range(n)])   # signed m -> bin of the 2N grid
+    big = np.zeros((3, n2, n2, n + 1), dtype=complex)
+    big[:, idx[:, None], idx[None, :], : n // 2 + 1] = 8 * uh
+    pbig = np.zeros((n2, n2, n + 1), dtype=complex)
+    pbig[idx[:, None], idx[None, :], : n // 2 + 1] = 8 * ph
+    dep2 = dns.deposit(d2, big, pbig)
+    lamb = dns.rfft3(np.cross(dep2.u, dep2.w, axis=0))[:, idx[:, None], idx[None, :], : n // 2 + 1] / 8
+    flux = dns.rfft3(dep2.u * dep2.phi[None])[:, idx[:, None], idx[None, :], : n // 2 + 1] / 8
+    kx, ky, kz = d.kvec
+    return lamb, -1j * (kx * flux[0] + ky * flux[1] + kz * flux[2])
+
+
+@pytest.mark.parametrize("n", [12, 16])
+def test_phase_shift_dealiases_the_quadratic_exactly(n):
+    """PhaseShift keeps the ball |m| <= sqrt(2) N/3 and, for fields inside it, its nonlinear terms
+    equal the unaliased products truncated to the ball (src/SimulationDomain.jl:140-176): the
+    half-cell average cancels the odd aliases and the ball excludes the two-axis ones.  A
+    single-grid evaluation with the same ball (the displaced term dropped) is visibly aliased."""
+    d = dns.Domain(n, dealiasing="phase_shift")
+    assert np.array_equal(d.mask, 9 * d.m2 <= 2 * n * n)
+    uh, ph = _field_in(d, 11, d.mask)
+    dep = dns.deposit(d, uh, ph)
+    cond = dns.Conditions(viscosity=1.0)
+    lamb, div = _exact_products(d, uh, ph)
+    want_u = dns.leray(d, lamb * d.mask)
+    want_u[:, 0, 0, 0] = 0
+    got_u = dns.velocity_rhs(d, cond, uh, dep.u, dep.w)
+    assert np.max(np.abs(got_u - want_u)) / np.max(np.abs(want_u)) < 1e-13
+    got_p = dns.scalar_rhs(d, cond, uh, dep.u, dep.phi, ph)
+    assert np.max(np.abs(got_p - div * d.mask)) / np.max(np.abs(div)) < 1e-13
+    plain = dns.leray(d, dns.rfft3(np.cross(dep.u, dep.w, axis=0)) * d.mask)
+    plain[:, 0, 0, 0] = 0
+    assert np.max(np.abs(plain - want_u)) / np.max(np.abs(want_u)) > 1e-2
+
+
+@pytest.mark.parametrize("n", [16, 24])
+def test_hou_li_filters_the_unaliased_products(n):
+    """HouLi keeps every mode and multiplies the nonlinear terms by exp(-s (|k|/k_Ny)^p)
+    (src/SimulationDomain.jl:365-376).  For fields with |m_a| < N/4 nothing aliases, so the terms
+    are the filtered exact products; the filter is 1 at the mean, e^{-s} at the axis Nyquist mode
+    and eps(T) there by default."""
+    order, s = 36, 2.5
+    d = dns.Domain(n, dealiasing="hou_li", filter_order=order, filter_strength=s)
+    assert d.mask.all()
+    assert d.dealias[0, 0, 0] == 1.0 and d.dealias[0, 0, n // 2] == pytest.approx(math.exp(-s), rel=1e-15)
+    q = n // 4 - 1
+    r = math.sqrt(3) * q / (n / 2)              # the mode (q, q, q): |k|/k_Ny = sqrt(3) q / (N/2)
+    assert d.dealias[q, q, q] == pytest.approx(math.exp(-s * r ** order), rel=1e-14)
+    assert dns.Domain(n, dealiasing="hou_li").dealias[0, 0, n // 2] == pytest.approx(np.finfo(np.float64).eps, rel=1e-12)
+    assert dns.Domain(n, dealiasing="hou_li", element_type=np.float32).dealias[0, 0, n // 2] == \
+        pytest.approx(np.finfo(np.float32).eps, rel=1e-6)
+    quarter = (np.abs(d.mx)[None, None, :] < n // 4) & (np.abs(d.my)[None, :, None] < n // 4) & \
+        (np.abs(d.mz)[:, None, None] < n // 4)
+    uh, ph = _field_in(d, 12, quarter)
+    dep = dns.deposit(d, uh, ph)
+    cond = dns.Conditions(viscosity=1.0)
+    lamb, div = _exact_products(d, uh, ph)
+    filt = np.exp(-s * (np.sqrt(d.m2.astype(float)) / (n / 2)) ** order)
+    want_u = dns.leray(d, lamb) * filt
+    want_u[:, 0, 0, 0] = 0
+    got_u = dns.velocity_rhs(d, cond, uh, dep.u, dep.w)
+    assert np.max(np.abs(got_u - want_u)) / np.max(np.abs(want_u)) < 1e-13
+    got_p = dns.scalar_rhs(d, cond, uh, dep.u, dep.phi, ph)
+    assert np.max(np.abs(got_p - div * filt)) / np.max(np.abs(div)) < 1e-13
+    # the filter is felt: at |k| > 1.2 k_Ny the products the terms carry are damped away
+    edge = (d.m2 > (0.6 * n) ** 2) & (np.abs(lamb).sum(axis=0) > 1e-8 * np.abs(lamb).max())
+    assert edge.any() and np.all(filt[edge] < 1e-3) and np.all(np.abs(got_u[:, edge]) < 1e-3 * np.abs(lamb[:, edge]))
+
+
+def test_the_wide_strategies_keep_the_step_hermitian_solenoidal_and_at_its_order():
+    """A step under each strategy keeps the state real (Hermitian kx = 0 and N/2 planes) and
+    solenoidal, and IF-RK4 under HouLi (default strength, Nyquist damped to eps) and PhaseShift
+    keeps its fourth order (Cauchy ratio 16)."""
+    n = 16
+    for kind in ("hou_li", "phase_shift"):
+        d = dns.Domain(n, dealiasing=kind)
+        uh, ph = _field_in(d, 21, d.mask & (9 * d.m2 <= n * n))
+        uh, ph = uh * 40.0, ph * 40.0
+        cond = dns.Conditions(viscosity=0.02, mean_gradient=1.0)
+        u1, p1, _ = dns.step(d, cond, "rk4", uh, ph, 2e-3)
+        # solenoidal away from the Nyquist indices, where +N/2 and -N/2 are one mode of a real field
+        # and the reality projection pairs them (the filter holds them at e^{-s} there)
+        inner = (np.abs(d.mx)[None, None, :] < n // 2) & (np.abs(d.my)[None, :, None] < n // 2) & \
+            (np.abs(d.mz)[:, None, None] < n // 2)
+        kx, ky, kz = d.kvec
+        div = np.abs(kx * u1[0] + ky * u1[1] + kz * u1[2])
+        assert np.max(div[inner]) < 1e-13 * np.max(np.sqrt(d.k2) * np.abs(u1).sum(axis=0))
+        for f in (u1, p1[None]):
+            assert np.allclose(f, dns.reality_projection(d, f), rtol=0, atol=1e-15 * np.abs(f).max())
+
+        def run(steps, t=0.08):
+            a, b = uh, ph
+            for _ in range(steps):
+                a, b, _ = dns.step(d, cond, "rk4", a, b, t / steps)
+            return a
+
+        ref = run(64)
+        e1, e2 = np.linalg.norm(run(4) - ref), np.linalg.norm(run(8) - ref)
+        assert 13 < e1 / e2 < 19, (kind, e1 / e2)
