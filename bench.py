@@ -264,6 +264,46 @@ def time_iterations(g, n: int, m: int, dtype, steps: int, dt: float) -> dict:
             "memory_GiB": round(mem / 2 ** 30, 2), "setup_s": round(setup, 1)}
 
 
+PUBLISHED_MS_PER_STEP = 14.4   # docs/src/validation.md:211-215 (AMD RX 7900 XTX, 265 s / 18,400 steps)
+
+
+def time_published(g, steps: int, dealiasing: str = "hou_li") -> dict:
+    """The reference's only published GPU step time, like for like: 128^3 FP64, IFSSPRK3,
+    HouLi(order 36, strength 63.463), ConstantPower eps = 1 on shells 1-2, nu = 9.032e-3 (k_max eta
+    = 1.5 on the resolved edge), fixed dt = 2e-3, no scalar, 10^4 AB(4) tracers with Catmull-Rom
+    (docs/src/validation.md:29-39).  Whole iterations (advance: step + deposit, traversal rebuild,
+    tracer push and sample, blow-up guard readback), wall clock after 20 untimed ones.  The initial
+    field is the seeded random-phase one, not the reference's TurTLE hand-off state."""
+    n, dt, m = 128, 2e-3, 10_000
+    dl = g.HouLi(order=36, strength=63.463) if dealiasing == "hou_li" else g.PhaseShift()
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=2 * math.pi, dealiasing=dl, element_type=np.float64)
+    nu = g.predicted_viscosity(g.FixedDomain(grid_points_per_axis=n, side_length=2 * math.pi,
+                                             dealiasing=g.HouLi(order=36, strength=63.463), element_type=np.float64),
+                               1.0, kmax_eta=1.5)
+    cond = g.FlowConditions(viscosity=nu, schmidt=1.0,
+                            velocity_forcing=g.ConstantPower(dom, injection_rate=1.0, lowest_shell=1, highest_shell=2))
+    with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(g.random_phase_velocity(n, seed=2026), None),
+                      field_integration_scheme=g.IFSSPRK3, tracers=g.Tracers(dom, number_of_particles=m, seed=2028),
+                      particle_integration_scheme=g.AB(4)) as sim:
+        for _ in range(20):
+            sim.advance(dt)
+        sim.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.advance(dt)
+        sim.synchronize()
+        per = (time.perf_counter() - t0) / steps
+    ms = per * 1e3
+    out = {"grid": n, "dtype": "f64", "dealiasing": repr(dl), "scheme": "IFSSPRK3", "forcing": "ConstantPower eps 1, shells 1-2",
+           "nu": round(nu, 6), "dt": dt, "tracers": m, "particle_scheme": "AB(4)", "iterations": steps,
+           "ms_per_step": round(ms, 3), "gridpoint_steps_per_s": round(n ** 3 / per, 1)}
+    if dealiasing == "hou_li":
+        out.update({"published_ms_per_step": PUBLISHED_MS_PER_STEP,
+                    "published_hardware": "AMD Radeon RX 7900 XTX (docs/src/validation.md:211-215)",
+                    "speedup_vs_published": round(PUBLISHED_MS_PER_STEP / ms, 2)})
+    return out
+
+
 def accuracy_report(g, n: int, steps: int, dt: float) -> dict:
     """FP32-bulk against a plain FP64 run of the same flow on the same GPU, per step: relative RMS
     distance of u and phi, relative error of k, eps, <phi^2>, eps_phi, and the largest relative
@@ -395,29 +435,60 @@ def kernel_table(res: dict, n: int, real_bytes: int, stages: int, ssp: bool = Fa
     return out
 
 
-def cpu_baseline(n_sample: int) -> dict:
-    """The FP64 oracle, one IF-RK4 step of the same physics at n_sample^3, on the host cores."""
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(steps: int = 100) -> dict:
+    """The FP64 oracle on the host cores, on the CPU-runnable configuration: 64^3, nu = 0.01,
+    unforced, a mean-gradient scalar from zero, IF-RK4 at dt = 2e-3 for `steps` steps with 4096
+    Heun tracers; and one IF-RK4 step at 128^3 (the grid of the reference's published CPU figures,
+    0.202-0.248 s/step on a 16-thread i7-12700K, validation/RUNBOOK.md:257-260,282-284)."""
     from oracle import dns
+    from oracle import tracers as trc
 
     import gbg_amd as g
     cores = min(os.cpu_count() or 1, 16)
     dns.set_workers(cores)
-    d = dns.Domain(n_sample)
-    u = g.random_phase_velocity(n_sample, seed=2026)
-    p = np.zeros((n_sample, n_sample, n_sample // 2 + 1), dtype=np.complex128)
-    dom = g.FixedDomain(grid_points_per_axis=n_sample, side_length=2 * math.pi, dealiasing=g.TwoThirds(),
-                        element_type=np.float64)
-    cond = dns.Conditions(viscosity=g.predicted_viscosity(dom, 0.1, kmax_eta=1.5), schmidt=1.0, forcing=(0.1, 1, 2),
-                          mean_gradient=1.0)
-    dep = dns.deposit(d, u, p)
+    L = 2 * math.pi
+    n, dt = 64, 2e-3
+    d = dns.Domain(n)
+    oc = dns.Conditions(viscosity=0.01, schmidt=1.0, mean_gradient=1.0)
+    u = g.random_phase_velocity(n, seed=2026)
+    p = np.zeros((n, n, n // 2 + 1), dtype=np.complex128)
+    X = g.uniform_positions(4096, L, 2028).T.copy()
     t0 = time.perf_counter()
-    dns.step(d, cond, "rk4", u, p, 1e-3, dep)
+    dep = dns.deposit(d, u, p)
+    v = trc.interpolate(dep.u, X, L)
+    for _ in range(steps):
+        u, p, dep = dns.step(d, oc, "rk4", u, p, dt, dep)
+        X, v = trc.heun_advect(X, v, dep.u, dt, L)
     sec = time.perf_counter() - t0
-    return {"value": round(n_sample ** 3 / sec / 1e6, 3), "unit": "Mpoint*steps/s", "cores": cores,
-            "kind": "port",
-            "sample": f"one IF-RK4 step (+ closing deposit) of the same physics at {n_sample}^3 in the FP64 numpy "
-                      f"oracle: scipy.fft (pocketfft) on {cores} threads, numpy pointwise on one; "
-                      f"{sec:.2f} s/step; rate per grid point, so comparable with the 512^3 metric"}
+    n2 = 128
+    d2 = dns.Domain(n2)
+    u2 = g.random_phase_velocity(n2, seed=2026)
+    p2 = np.zeros((n2, n2, n2 // 2 + 1), dtype=np.complex128)
+    dep2 = dns.deposit(d2, u2, p2)
+    t1 = time.perf_counter()
+    dns.step(d2, dns.Conditions(viscosity=0.01, schmidt=1.0, forcing=(0.1, 1, 2), mean_gradient=1.0), "rk4", u2, p2,
+             dt, dep2)
+    sec128 = time.perf_counter() - t1
+    return {"value": round(n ** 3 * steps / sec / 1e6, 3), "unit": "Mpoint*steps/s", "cores": cores, "kind": "port",
+            "cpu_model": _cpu_model(),
+            "sample": f"the 64^3 configuration: {steps} IF-RK4 steps of dt 2e-3 (nu 0.01, unforced, mean-gradient "
+                      f"scalar) with 4096 Heun tracers in the FP64 numpy oracle, {sec:.1f} s ({sec / steps * 1e3:.0f} "
+                      f"ms/step); transforms in scipy.fft (pocketfft) on {cores} threads, pointwise work and the "
+                      f"tracer gathers in numpy on one thread",
+            "s_per_step_128": round(sec128, 3),
+            "sample_128": f"one IF-RK4 step (+ deposit) at 128^3 with forcing: {sec128:.2f} s/step "
+                          f"({128 ** 3 / sec128 / 1e6:.2f} Mpoint*steps/s); the reference's own CPU path ran 0.202-0.248 "
+                          f"s/step at 128^3 on 16 threads (validation/RUNBOOK.md:257-260,282-284)"}
 
 
 def main() -> int:
@@ -428,7 +499,9 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain-FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-n", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=100, help="oracle steps of the 64^3 CPU baseline")
+    ap.add_argument("--no-published", action="store_true",
+                    help="skip the reference's published configuration (128^3 FP64 HouLi IFSSPRK3 + 10^4 AB(4) tracers)")
     ap.add_argument("--no-tracers", action="store_true", help="skip the tracer throughput figure")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 96-box ensemble figure")
     ap.add_argument("--ensemble-steps", type=int, default=200)
@@ -520,10 +593,16 @@ def main() -> int:
         adjoint = time_adjoint(g, directions=args.adjoint_directions)
     if args.copy_calibration:
         copy_calibration(H)
+    published = None
+    if not args.no_published:
+        print("bench: the published configuration, 128^3 FP64 HouLi IFSSPRK3 + 10^4 AB(4) tracers (and PhaseShift)",
+              file=sys.stderr, flush=True)
+        published = [time_published(g, 200), time_published(g, 100, "phase_shift")]
     cpu = None
     if not args.no_cpu_baseline:
-        print(f"bench: cpu baseline at {args.cpu_sample_n}^3", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(args.cpu_sample_n)
+        print(f"bench: cpu baseline, 64^3 x {args.cpu_steps} steps + 4096 tracers, and one 128^3 step",
+              file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_steps)
     line = {
         "metric": "Mpoint*steps/s at 512^3 IF-RK4, FP32-bulk vs FP64 (+% HBM roofline)",
         "value": round(value, 2), "unit": "Mpoint*steps/s", "n_gpus": 1, "steps": args.steps,
@@ -543,6 +622,18 @@ def main() -> int:
         "adjoint": adjoint,
         "iterations": iterations,
         "accuracy": accuracy,
+        "published_configuration": published,
+    }
+    # a compact summary last, so that a reader of the line's tail sees the figures that matter
+    line["summary"] = {
+        "fp32_bulk_ms_per_step": round(ms32, 3),
+        "fp64_ms_per_step": fp64["ms_per_step"] if fp64 else None,
+        "fp32_bulk_speedup_over_fp64": fp64["fp32_bulk_speedup"] if fp64 else None,
+        "fp64_x_pass_ms": fp64["kernels"]["x_pass_products"]["avg_ms"] if fp64 else None,
+        "roofline_frac": roofline["frac"], "survey_model_frac": roofline["survey_model_frac"],
+        "published_hou_li_128_ms_per_step": published[0]["ms_per_step"] if published else None,
+        "published_reference_ms_per_step": PUBLISHED_MS_PER_STEP,
+        "cpu_baseline_Mpoint_steps_per_s": cpu["value"] if cpu else None,
     }
     print(json.dumps(line), flush=True)
     return 0
