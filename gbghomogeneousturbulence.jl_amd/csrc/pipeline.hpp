@@ -364,12 +364,10 @@ struct PipelineImpl final : Pipeline {
   }
   void sample(Ctx &c) override {
     if (c.m == 0) return;
-    const size_t vol = (size_t)N * N * N;
-    const T *phi = c.cfg.has_scalar ? (const T *)c.phys + 3 * vol : nullptr;
     T *slot = (T *)c.vel + (size_t)c.trail_newest * 3 * c.cfg.tracer_capacity;
     hipLaunchKernelGGL((k_sample<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const T *)c.pos, slot,
-                       c.cfg.has_scalar ? (T *)c.phis : nullptr, (const T *)c.phys, phi,
-                       c.order_valid ? c.order : nullptr, targs(c));
+                       c.cfg.has_scalar ? (T *)c.phis : nullptr, (const T *)c.phys, c.order_valid ? c.order : nullptr,
+                       targs(c));
   }
   void cell_keys(Ctx &c, int64_t *keys) override {   // the selected box
     if (c.m == 0) return;

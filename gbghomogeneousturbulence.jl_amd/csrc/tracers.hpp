@@ -51,33 +51,87 @@ __device__ __forceinline__ Stencil stencil_of(T x, T y, T z, T inv_dx, int n, T 
   return {wrap_index((int)fx, n), wrap_index((int)fy, n), wrap_index((int)fz, n)};
 }
 
-// Separable 4^3 gather of NF components (stride vol apart) at one point.
-template <typename T, int NF>
-__device__ __forceinline__ void gather(const T *__restrict__ f, size_t vol, int n, const Stencil &s, const T *wx,
-                                       const T *wy, const T *wz, T *out) {
-  int ii[4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) ii[a] = wrap_index(s.i + a - 1, n);
+// Separable 4^3 gather of NF components at one point: at(q, c, b, a) is component q at the stencil
+// point (k + c - 1, j + b - 1, i + a - 1).  The sums are explicit fused multiply-adds in a fixed
+// order, so a gather from HBM and one from a staged copy of the same values agree to the bit.
+template <typename T, int NF, typename At>
+__device__ __forceinline__ void gather_at(At at, const T *wx, const T *wy, const T *wz, T *out) {
 #pragma unroll
   for (int q = 0; q < NF; ++q) out[q] = T(0);
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const int kk = wrap_index(s.k + c - 1, n);
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const int jj = wrap_index(s.j + b - 1, n);
-      const size_t row = ((size_t)kk * n + jj) * n;
       const T wzy = wz[c] * wy[b];
 #pragma unroll
       for (int q = 0; q < NF; ++q) {
-        const T *fr = f + q * vol + row;
         T r = T(0);
 #pragma unroll
-        for (int a = 0; a < 4; ++a) r += wx[a] * fr[ii[a]];
-        out[q] += wzy * r;
+        for (int a = 0; a < 4; ++a) r = fma(wx[a], at(q, c, b, a), r);
+        out[q] = fma(wzy, r, out[q]);
       }
     }
   }
+}
+
+// Four consecutive values of a row from a 4-byte-aligned (FP32) or 8-byte-aligned (FP64) address:
+// one 16-byte load (FP32) or two (FP64) instead of four.
+typedef float row4f __attribute__((ext_vector_type(4), aligned(4)));
+typedef double row2d __attribute__((ext_vector_type(2), aligned(8)));
+__device__ __forceinline__ void load_run4(const float *p, float *v) {
+  const row4f r = *reinterpret_cast<const row4f *>(p);
+  v[0] = r.x;
+  v[1] = r.y;
+  v[2] = r.z;
+  v[3] = r.w;
+}
+__device__ __forceinline__ void load_run4(const double *p, double *v) {
+  const row2d a = *reinterpret_cast<const row2d *>(p), b = *reinterpret_cast<const row2d *>(p + 2);
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = b.x;
+  v[3] = b.y;
+}
+
+// The gather from the field in HBM: NF components stride vol apart, periodic indices.  When the
+// stencil's four x points do not wrap (all but 3 of N cells), each row of each component is one
+// wide load; the values, and so the sums, are the same either way.
+template <typename T, int NF>
+__device__ __forceinline__ void gather(const T *__restrict__ f, size_t vol, int n, const Stencil &s, const T *wx,
+                                       const T *wy, const T *wz, T *out) {
+  size_t row[4][4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const int kk = wrap_index(s.k + c - 1, n);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) row[c][b] = ((size_t)kk * n + wrap_index(s.j + b - 1, n)) * n;
+  }
+  if (s.i >= 1 && s.i + 2 < n) {
+    const int i0 = s.i - 1;
+#pragma unroll
+    for (int q = 0; q < NF; ++q) out[q] = T(0);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const T wzy = wz[c] * wy[b];
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+          T v[4];
+          load_run4(f + q * vol + row[c][b] + i0, v);
+          T r = T(0);
+#pragma unroll
+          for (int a = 0; a < 4; ++a) r = fma(wx[a], v[a], r);
+          out[q] = fma(wzy, r, out[q]);
+        }
+      }
+    }
+    return;
+  }
+  int ii[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) ii[a] = wrap_index(s.i + a - 1, n);
+  gather_at<T, NF>([&](int q, int c, int b, int a) { return f[q * vol + row[c][b] + ii[a]]; }, wx, wy, wz, out);
 }
 
 struct TracerArgs {
@@ -107,20 +161,12 @@ struct AdamsArgs {
   double c[MAX_TRAIL + 1];
 };
 
-template <typename T>
-__global__ void k_advect(T *__restrict__ pos, T *__restrict__ upos, const T *__restrict__ trail, const T *__restrict__ u,
-                         const int *__restrict__ order, TracerArgs ta, AdamsArgs aa) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= ta.m) return;
-  const int box = blockIdx.y;
-  pos += box * 3 * ta.cap;
-  if (upos) upos += box * 3 * ta.cap;
-  trail += box * ta.vstride;
-  u += box * ta.box_phys;
-  if (order) order += box * ta.cap;
-  const int64_t p = order ? order[t] : t;
+// One tracer's Adams step; g3(stencil, wx, wy, wz, out) gathers u at a point.
+template <typename T, typename G3>
+__device__ __forceinline__ void advect_one(int64_t p, T *__restrict__ pos, T *__restrict__ upos,
+                                           const T *__restrict__ trail, const TracerArgs &ta, const AdamsArgs &aa,
+                                           G3 g3) {
   const T L = T(ta.side_length), inv_dx = T(ta.n / ta.side_length);
-  const size_t vol = (size_t)ta.n * ta.n * ta.n;
   T F[MAX_TRAIL][3], acc[3], Xs[3], ua[3] = {T(0), T(0), T(0)};
 #pragma unroll
   for (int d = 0; d < 3; ++d) acc[d] = pos[d * ta.m + p];
@@ -152,7 +198,7 @@ __global__ void k_advect(T *__restrict__ pos, T *__restrict__ upos, const T *__r
   }
   T wx[4], wy[4], wz[4], fs[3];
   const Stencil s = stencil_of(Xs[0], Xs[1], Xs[2], inv_dx, ta.n, wx, wy, wz);
-  gather<T, 3>(u, vol, ta.n, s, wx, wy, wz, fs);
+  g3(s, wx, wy, wz, fs);
   const T c0 = T(aa.c[0]);
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -177,6 +223,25 @@ __global__ void k_advect(T *__restrict__ pos, T *__restrict__ upos, const T *__r
   }
 }
 
+// The push in tracer (or traversal) order, one thread per tracer.
+template <typename T>
+__global__ void k_advect(T *__restrict__ pos, T *__restrict__ upos, const T *__restrict__ trail, const T *__restrict__ u,
+                         const int *__restrict__ order, TracerArgs ta, AdamsArgs aa) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= ta.m) return;
+  const int box = blockIdx.y;
+  pos += box * 3 * ta.cap;
+  if (upos) upos += box * 3 * ta.cap;
+  trail += box * ta.vstride;
+  u += box * ta.box_phys;
+  if (order) order += box * ta.cap;
+  const int64_t p = order ? order[t] : t;
+  const size_t vol = (size_t)ta.n * ta.n * ta.n;
+  advect_one(p, pos, upos, trail, ta, aa, [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out) {
+    gather<T, 3>(u, vol, ta.n, s, wx, wy, wz, out);
+  });
+}
+
 // sum_j w_j f_j over n trail samples of the selected box, added left to right (time_derivative!,
 // src/Particles.jl:597-631); out [3][m].
 struct TrailCombine {
@@ -198,11 +263,27 @@ __global__ void k_trail_combine(const T *__restrict__ trail, TracerArgs ta, Trai
 }
 
 // The particle deposit! (src/ParticlePropagator.jl:378-386): u (and phi) at the positions, into
-// the trail slot `vel` points at (inside box 0's trail; box b's is b * vstride further).
+// the trail slot `vel` points at (inside box 0's trail; box b's is b * vstride further).  The
+// physical fields are [4][N^3] per box (u, then phi), so phi is u's fourth component.
+template <typename T, typename G>
+__device__ __forceinline__ void sample_one(int64_t p, const T *__restrict__ pos, T *__restrict__ vel,
+                                           T *__restrict__ phis, const TracerArgs &ta, G g) {
+  const T inv_dx = T(ta.n / ta.side_length);
+  T wx[4], wy[4], wz[4], v[3];
+  const Stencil s = stencil_of(pos[p], pos[ta.m + p], pos[2 * ta.m + p], inv_dx, ta.n, wx, wy, wz);
+  g(s, wx, wy, wz, v, 0);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) vel[d * ta.m + p] = v[d];
+  if (phis) {
+    T q[1];
+    g(s, wx, wy, wz, q, 3);
+    phis[p] = q[0];
+  }
+}
+
 template <typename T>
 __global__ void k_sample(const T *__restrict__ pos, T *__restrict__ vel, T *__restrict__ phis,
-                         const T *__restrict__ u, const T *__restrict__ phi, const int *__restrict__ order,
-                         TracerArgs ta) {
+                         const T *__restrict__ u, const int *__restrict__ order, TracerArgs ta) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= ta.m) return;
   const int box = blockIdx.y;
@@ -210,21 +291,13 @@ __global__ void k_sample(const T *__restrict__ pos, T *__restrict__ vel, T *__re
   vel += box * ta.vstride;
   if (phis) phis += box * ta.cap;
   u += box * ta.box_phys;
-  if (phi) phi += box * ta.box_phys;
   if (order) order += box * ta.cap;
   const int64_t p = order ? order[t] : t;
-  const T inv_dx = T(ta.n / ta.side_length);
   const size_t vol = (size_t)ta.n * ta.n * ta.n;
-  T wx[4], wy[4], wz[4], v[3];
-  const Stencil s = stencil_of(pos[p], pos[ta.m + p], pos[2 * ta.m + p], inv_dx, ta.n, wx, wy, wz);
-  gather<T, 3>(u, vol, ta.n, s, wx, wy, wz, v);
-#pragma unroll
-  for (int d = 0; d < 3; ++d) vel[d * ta.m + p] = v[d];
-  if (phis) {
-    T q[1];
-    gather<T, 1>(phi, vol, ta.n, s, wx, wy, wz, q);
-    phis[p] = q[0];
-  }
+  sample_one(p, pos, vel, phis, ta, [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out, int c0) {
+    if (c0 == 0) gather<T, 3>(u, vol, ta.n, s, wx, wy, wz, out);
+    else gather<T, 1>(u + c0 * vol, vol, ta.n, s, wx, wy, wz, out);
+  });
 }
 
 template <typename T>
