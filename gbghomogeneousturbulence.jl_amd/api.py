@@ -678,7 +678,7 @@ class Simulation:
                  particle_integration_scheme=None, mean_gradient: float = 0.0, cfl: float | None = None,
                  max_dt: float = math.inf, keep_vorticity: bool = False, keep_physical: bool = True,
                  tracer_capacity: int | None = None, shortened_first_steps: bool = False,
-                 traversal_mode: str = "automatic"):
+                 traversal_mode: str = "automatic", use_graph: bool = True):
         carries_scalar = state.scalar_spectral is not None
         u, p = _state_arrays(state, domain)
         self.mean_gradient = float(mean_gradient)
@@ -698,6 +698,9 @@ class Simulation:
         if traversal_mode not in ("automatic", "manual", "none"):
             raise ValueError(f"`traversal_mode` must be `:automatic`, `:manual` or `:none`; got {traversal_mode!r}")
         self.traversal_mode = traversal_mode
+        # advance() replays the step as one captured HIP graph (bitwise the eager step; one capture
+        # serves every dt), or launches it kernel by kernel
+        self.use_graph = bool(use_graph)
         self.tracers = tracers
         self.cfl = recommended_cfl(field_integration_scheme, domain) if cfl is None else float(cfl)
         self.max_dt = float(max_dt)
@@ -1105,7 +1108,7 @@ class Simulation:
         t_new = self.time + dt_
         if not t_new > self.time:
             raise ValueError(f"the timestep {dt_} is too small to advance the clock, which reads {self.time}")
-        self.step_fields(dt_)
+        self.step_fields(dt_, graph=self.use_graph)
         if self.tracers is not None:
             if self.traversal_mode == "automatic":   # from the start-of-step positions (src/SimulationRun.jl:1197-1200)
                 self._call("hdns_rebuild_traversal")

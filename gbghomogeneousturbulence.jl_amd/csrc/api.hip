@@ -190,7 +190,7 @@ void release(Ctx &c) {
   void *ptrs[] = {c.uh, c.hu, c.ph, c.hp, c.bufa, c.bufb, c.phys, c.physw, c.tw, c.partials, c.scalars, c.rpart,
                   c.usrc, c.psrc,
                   c.maxbits, c.spec_bins, c.pos, c.upos, c.vel, c.phis, c.order, c.skeys, c.svals, c.sort_temp,
-                  c.visc_d, c.kappa_d, c.filter, c.shifted};
+                  c.visc_d, c.kappa_d, c.filter, c.shifted, c.step_values};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   AdjBufs &a = c.adj;
@@ -676,6 +676,7 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
   }
   if (s == HDNS_OK && k.dealiasing == HDNS_PHASE_SHIFT)   // the displaced products' transforms
     s = alloc(ctx, &c.shifted, B * ncf * c.comp * c.elem);
+  if (s == HDNS_OK) s = alloc(ctx, (void **)&c.step_values, 16 * sizeof(double));
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.visc_d, B * sizeof(double));
   if (s == HDNS_OK) s = alloc(ctx, (void **)&c.kappa_d, B * sizeof(double));
   if (s == HDNS_OK && k.tracer_capacity > 0) {
@@ -950,7 +951,33 @@ void shu_osher_stage(StageArgs &sa, int scheme, int i, double dt, bool fp32) {
   sa.cb = round_t(cb[i][0], cb[i][1], fp32);
 }
 
-void launch_step(Ctx &c, double dt, const StageRecord *rec = nullptr) {
+// The dt-dependent values of every stage, [2 stage] = dt in T, [2 stage + 1] = the exponent's
+// substep, exactly as launch_step puts them into the stage arguments.
+struct StepValues {
+  double v[10];
+};
+
+static __global__ void k_set_step_values(double *dst, StepValues sv) {
+  if (threadIdx.x < 10) dst[threadIdx.x] = sv.v[threadIdx.x];
+}
+
+StepValues step_values(const Ctx &c, double dt) {
+  const bool fp32 = !c.fp64;
+  const Coeffs k = scheme_coefficients(c.cfg.scheme, fp32);
+  StepValues sv{};
+  for (int i = 0; i < k.stages; ++i) {
+    StageArgs sa{};
+    sa.sub = k.d[i] * dt;
+    if (c.cfg.scheme == HDNS_IF_SSP_RK3 || c.cfg.scheme == HDNS_IF_HEUN) shu_osher_stage(sa, c.cfg.scheme, i, dt, fp32);
+    sv.v[2 * i] = fp32 ? (double)(float)dt : dt;
+    sv.v[2 * i + 1] = sa.sub;
+  }
+  return sv;
+}
+
+// dyn: the kernels read dt and the substeps from c.step_values (a captured step that serves every
+// dt); the caller writes them with k_set_step_values before each launch.
+void launch_step(Ctx &c, double dt, const StageRecord *rec = nullptr, bool dyn = false) {
   const bool fp32 = !c.fp64;
   const Coeffs k = scheme_coefficients(c.cfg.scheme, fp32);
   const double dt_t = fp32 ? (double)(float)dt : dt;
@@ -985,6 +1012,7 @@ void launch_step(Ctx &c, double dt, const StageRecord *rec = nullptr) {
     sa.slin_rate = c.cfg.scalar_forcing == HDNS_LINEAR ? c.cfg.scalar_rate : 0.0;
     sa.filter = c.filter;
     sa.shifted = c.shifted;
+    sa.dyn = dyn ? c.step_values : nullptr;
     if (c.cfg.scheme == HDNS_IF_SSP_RK3 || c.cfg.scheme == HDNS_IF_HEUN) shu_osher_stage(sa, c.cfg.scheme, i, dt, fp32);
     XArgs xa = last ? fin : mid;
     if (rec) {
@@ -1053,12 +1081,11 @@ hdns_status hdns_step_graph(hdns_ctx *ctx, double dt) {
   if (s != HDNS_OK) return s;
   Ctx &c = ctx->c;
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
-  if (!c.graph_exec || c.graph_dt != dt) {
-    drop_graph(c);
+  if (!c.graph_exec) {
     const bool prof = c.prof;
     c.prof = false;  // no event records inside the captured step
     HIP_TRY(ctx, hipStreamBeginCapture(c.stream, hipStreamCaptureModeRelaxed));
-    launch_step(c, dt);
+    launch_step(c, dt, nullptr, true);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(c.stream, &graph);
     c.prof = prof;
@@ -1071,8 +1098,10 @@ hdns_status hdns_step_graph(hdns_ctx *ctx, double dt) {
     }
     c.graph = graph;
     c.graph_exec = exec;
-    c.graph_dt = dt;
   }
+  // this step's dt into the values the captured kernels read
+  hipLaunchKernelGGL(k_set_step_values, dim3(1), dim3(64), 0, c.stream, c.step_values, step_values(c, dt));
+  HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipGraphLaunch((hipGraphExec_t)c.graph_exec, c.stream));
   c.deposited = true;
   return HDNS_OK;
@@ -2101,8 +2130,7 @@ struct hdns_ensemble {
   int device = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
-  double graph_dt = 0.0;
-  std::vector<int> graph_sched;             // the schedules the captured step holds
+  std::vector<int> graph_sched;             // the schedules the captured step holds (dt is written per step)
   std::string err;
 };
 
@@ -2241,7 +2269,7 @@ hdns_status hdns_ensemble_step_graph(hdns_ensemble *ens, double dt) {
   if (s != HDNS_OK) return s;
   if (hipSetDevice(ens->device) != hipSuccess) return efail(ens, HDNS_ERR_DEVICE, "hipSetDevice failed");
   const std::vector<int> sched = ensemble_schedules(*ens);
-  if (!ens->graph_exec || ens->graph_dt != dt || ens->graph_sched != sched) {
+  if (!ens->graph_exec || ens->graph_sched != sched) {
     drop_ensemble_graph(*ens);
     std::vector<bool> prof;
     for (hdns_ctx *c : ens->classes) {
@@ -2250,7 +2278,7 @@ hdns_status hdns_ensemble_step_graph(hdns_ensemble *ens, double dt) {
     }
     hipError_t e = hipStreamBeginCapture(ens->stream, hipStreamCaptureModeRelaxed);
     if (e != hipSuccess) return efail(ens, HDNS_ERR_DEVICE, std::string("capture: ") + hipGetErrorString(e));
-    for (hdns_ctx *c : ens->classes) launch_step(c->c, dt);
+    for (hdns_ctx *c : ens->classes) launch_step(c->c, dt, nullptr, true);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(ens->stream, &graph);
     for (size_t i = 0; i < ens->classes.size(); ++i) ens->classes[i]->c.prof = prof[i];
@@ -2263,9 +2291,10 @@ hdns_status hdns_ensemble_step_graph(hdns_ensemble *ens, double dt) {
     }
     ens->graph = graph;
     ens->graph_exec = exec;
-    ens->graph_dt = dt;
     ens->graph_sched = sched;
   }
+  for (hdns_ctx *c : ens->classes)   // this step's dt into the values the captured kernels read
+    hipLaunchKernelGGL(k_set_step_values, dim3(1), dim3(64), 0, ens->stream, c->c.step_values, step_values(c->c, dt));
   const hipError_t e = hipGraphLaunch(ens->graph_exec, ens->stream);
   if (e != hipSuccess) return efail(ens, HDNS_ERR_DEVICE, std::string("graph launch: ") + hipGetErrorString(e));
   return HDNS_OK;

@@ -138,9 +138,10 @@ struct Ctx {
   int prof_used = 0;
   double prof_ms[NKIND] = {0};
   long long prof_cnt[NKIND] = {0};
-  // the captured step (hdns_step_graph), valid for one dt
+  // the captured step (hdns_step_graph): its kernels read dt and the substeps from step_values,
+  // so one capture serves every dt; it is captured again when the schedule changes
   void *graph = nullptr, *graph_exec = nullptr;
-  double graph_dt = 0.0;
+  double *step_values = nullptr;   // [2 stage]: dt in T, [2 stage + 1]: the substep
   AdjBufs adj;
   std::unique_ptr<Pipeline> pipe;
   std::string err;

@@ -178,8 +178,8 @@ __global__ __launch_bounds__(256) void k_band_source(cx<T> *__restrict__ q, cx<T
   if (!in_band(mx * mx + my * my + mz * mz, band)) return;
   const double kxd = g.k0 * mx, kyd = g.k0 * my, kzd = g.k0 * mz;
   const double k2d = (kxd * kxd + kyd * kyd) + kzd * kzd;
-  const double xe = -(scalar ? sa.kappa[box] : sa.visc[box]) * sa.sub * k2d;
-  const T dt = T(sa.dt);
+  const double xe = -(scalar ? sa.kappa[box] : sa.visc[box]) * stage_sub(sa) * k2d;
+  const T dt = T(stage_dt(sa));
   const int ky_idx = my < 0 ? my + g.k : my, kz_idx = mz < 0 ? mz + g.k : mz;
   const size_t comp = comp_of(g), at = ((size_t)kz_idx * g.k + ky_idx) * g.kxp + mx;
   q += (size_t)box * ncomp * comp + at;

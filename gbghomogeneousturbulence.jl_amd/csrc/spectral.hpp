@@ -50,7 +50,13 @@ struct StageArgs {
   // PhaseShift (src/Generator.jl:502-519, :675-704): the raw transforms of the products formed on
   // the grid displaced by half a cell, per box [ncf][comp] compact (the Lamb vector, then the flux)
   const void *shifted;
+  // when set, dt and sub are read on the device from dyn[2 stage] and dyn[2 stage + 1] instead:
+  // a captured step then serves every dt (the host writes them before each replay)
+  const double *dyn;
 };
+
+__device__ __forceinline__ double stage_dt(const StageArgs &sa) { return sa.dyn ? sa.dyn[2 * sa.stage] : sa.dt; }
+__device__ __forceinline__ double stage_sub(const StageArgs &sa) { return sa.dyn ? sa.dyn[2 * sa.stage + 1] : sa.sub; }
 
 // PhaseShift's translation of a mode by half a cell along every axis: e^{i s pi (mx + my + mz)/N},
 // s = +1 samples the field on the displaced grid, s = -1 shifts a displaced term back
@@ -110,8 +116,9 @@ __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, con
         });
   }
   const T Gm = T(sa.mean_gradient);
-  const T dt = T(sa.dt), a = T(sa.a), b = T(sa.b);
-  const double coef_p = -sa.kappa[box] * sa.sub;
+  const double sub = stage_sub(sa);
+  const T dt = T(stage_dt(sa)), a = T(sa.a), b = T(sa.b);
+  const double coef_p = -sa.kappa[box] * sub;
   const T Aphi = SF ? T(sa.scalars[8 * box + 7]) : T(0);
   __builtin_amdgcn_sched_barrier(0);   // the state loads stay behind the transforms
 #pragma unroll
@@ -166,7 +173,7 @@ __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, con
         if constexpr (SF) {
           if (in_band(mx * mx + my * my + mz * mz, sa.sband)) {   // src/Forcing.jl:1104-1117, :600-642
             nl = nl + scale(pv[i], Aphi);
-            xe += sa.slin_rate * sa.sub;
+            xe += sa.slin_rate * sub;
           }
         }
         if constexpr (SSP) {
@@ -275,8 +282,9 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
   };
 
   const T A = sa.band.on ? T(sa.scalars[8 * box]) : T(0);
-  const T a = T(sa.a), b = T(sa.b), dt = T(sa.dt);
-  const double coef_u = -sa.visc[box] * sa.sub;
+  const double sub = stage_sub(sa);
+  const T a = T(sa.a), b = T(sa.b), dt = T(stage_dt(sa));
+  const double coef_u = -sa.visc[box] * sub;
   __builtin_amdgcn_sched_barrier(0);   // the state loads stay behind the transforms
 #pragma unroll
   for (int p = 0; p < PER; ++p) {
@@ -353,7 +361,7 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
         }
         // Linear forcing rides the integrating factor: exp((A - nu |k|^2) d_i dt) in the band
         // (src/Generator.jl:565-581, src/Forcing.jl:600-640)
-        const double xe = coef_u * k2d + (banded ? sa.lin_rate * sa.sub : 0.0);
+        const double xe = coef_u * k2d + (banded ? sa.lin_rate * sub : 0.0);
         cx<T> un[3];
         if constexpr (SSP) {
           const T Eb = T(exp(sa.pb * xe)), Ea = sa.stage > 0 ? T(exp(sa.pa * xe)) : T(0);

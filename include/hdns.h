@@ -131,7 +131,9 @@ hdns_status hdns_state_download(hdns_ctx *ctx, double *u_hat, double *phi_hat);
  * projections, and the recovery of u, w, phi and max|u| at the new state. */
 hdns_status hdns_step(hdns_ctx *ctx, double dt);
 /* The same step as hdns_step, launched as one HIP graph captured from its kernel sequence on the
- * first call (and again whenever dt changes), so a step costs one launch.  The step holds no host
+ * first call (and again after a schedule change), so a step costs two launches: a one-thread kernel
+ * that writes this step's dt and substeps into device memory, which the captured kernels read,
+ * and the graph.  One capture therefore serves every dt of an adaptive run.  The step holds no host
  * readback and no memset node, so the replay is bitwise the eager step. */
 hdns_status hdns_step_graph(hdns_ctx *ctx, double dt);
 /* The field deposit! of the current state on its own (src/Propagator.jl:808-842). */
@@ -281,7 +283,7 @@ hdns_status hdns_adjoint_gradient(hdns_ctx *ctx, double dt, int32_t steps, int32
 /* Ensembles of boxes of uneven sizes and viscosities (no counterpart in the reference).  The
  * configs may differ in grid_points and viscosity only.  Boxes of one size form a batch context;
  * all batches share one stream, and hdns_ensemble_step_graph advances every box with one graph
- * launch (captured once per dt).  hdns_ensemble_member returns box `box`'s batch context with that
+ * launch (captured once; each step's dt is written before the launch).  hdns_ensemble_member returns box `box`'s batch context with that
  * box selected, for the per-box entry points (the ensemble owns it: do not destroy it). */
 typedef struct hdns_ensemble hdns_ensemble;
 hdns_status hdns_ensemble_create(const hdns_config *configs, int32_t nbox, hdns_ensemble **out);
