@@ -26,7 +26,8 @@ constexpr int tile_width_of() { return sizeof(T) == 4 ? 16 : 8; }
 //   z inverse: a TX-wide tile on 64*TX threads (one first-pass butterfly per thread at N = 512)
 //              or on 32*TX threads (two);
 //   x pass: 256 or 512 threads per block (FP32 at N = 512 and 1024: the register-resident pass
-//           with or without the prefetch of the next component);
+//           with the next component staged in LDS by direct-to-LDS loads, or loaded into
+//           registers just before its transform);
 //   y radix plan: fft_io's plan or AltPlan's, led by a radix-16 pass.
 // At N >= 1024 the z schedules use narrower tiles (below).
 constexpr int kYThreads[2] = {256, 512};
@@ -55,8 +56,10 @@ struct PipelineImpl final : Pipeline {
   // per row (1024, as a 512-point complex sequence)
   static constexpr bool XWAVE = TWO_THIRDS && std::is_same<T, float>::value && (N == 512 || N == 1024);
   static constexpr int XW_ROWS = N == 1024 ? 1 : 2;   // rows per wavefront
+  // the register-resident x pass's stages (XFETCH_LDS): one component's rows per wavefront
+  static constexpr int KXP = (M + 1 + TX - 1) / TX * TX;   // the padded row of every compact array
   template <int XT>
-  static constexpr size_t lds_xw() { return (size_t)(XT / 64) * (512 + 512 / 8) * sizeof(cx<T>); }
+  static constexpr size_t lds_xw() { return (size_t)(XT / 64) * XW_ROWS * KXP * sizeof(cx<T>); }
 
   static void set_lds(const void *fn, size_t bytes) {
     (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
@@ -109,8 +112,9 @@ struct PipelineImpl final : Pipeline {
       set_lds((const void *)k_zi<T, N, ZT, ZN, false, DA, true>, ZL);
     }
     if constexpr (XWAVE) {
-      set_lds((const void *)k_xpass_wave<T, N, 7, 256, V == 0>, lds_xw<256>());
-      set_lds((const void *)k_xpass_wave<T, N, 6, 256, V == 0>, lds_xw<256>());
+      constexpr int F = V == 0 ? XFETCH_LDS : XFETCH_JIT;
+      set_lds((const void *)k_xpass_wave<T, N, 7, 256, F>, F == XFETCH_LDS ? lds_xw<256>() : 0);
+      set_lds((const void *)k_xpass_wave<T, N, 6, 256, F>, F == XFETCH_LDS ? lds_xw<256>() : 0);
     } else {
       set_lds((const void *)k_xpass<T, N, 7, kXThreads[V], DA>, lds_x<kXThreads[V]>());
       set_lds((const void *)k_xpass<T, N, 6, kXThreads[V], DA>, lds_x<kXThreads[V]>());
@@ -308,16 +312,20 @@ struct PipelineImpl final : Pipeline {
     else c.sched[0] ? yinv_v<YT[1], 0>(c) : yinv_v<YT[0], 0>(c);
   }
 
-  // FP32 at N = 512: schedule 0 prefetches the next component, schedule 1 does not (256 threads)
-  template <bool PF>
+  // FP32 at N = 512 and 1024: schedule 0 stages the next component in LDS, schedule 1 loads each
+  // component just before its transform (256 threads)
+  template <int F>
   void xpass_wave_v(Ctx &c, const XArgs &xa) {
     constexpr int XT = 256;
     const int blocks = N * N / (XW_ROWS * (XT / 64));
+    static_assert(!XWAVE || KXP == (M + 1 + 15) / 16 * 16, "the staged rows are the kernel's padded rows");
+    if (F == XFETCH_LDS && c.geo.kxp != KXP) return xpass_wave_v<XFETCH_JIT>(c, xa);   // rows of another padding
+    const size_t lds = F == XFETCH_LDS ? lds_xw<XT>() : 0;
     if (c.cfg.has_scalar)
-      hipLaunchKernelGGL((k_xpass_wave<T, N, 7, XT, PF>), dim3(blocks, c.nbox), dim3(XT), lds_xw<XT>(), c.stream,
+      hipLaunchKernelGGL((k_xpass_wave<T, N, 7, XT, F>), dim3(blocks, c.nbox), dim3(XT), lds, c.stream,
                          (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
     else
-      hipLaunchKernelGGL((k_xpass_wave<T, N, 6, XT, PF>), dim3(blocks, c.nbox), dim3(XT), lds_xw<XT>(), c.stream,
+      hipLaunchKernelGGL((k_xpass_wave<T, N, 6, XT, F>), dim3(blocks, c.nbox), dim3(XT), lds, c.stream,
                          (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
   }
 
@@ -332,7 +340,7 @@ struct PipelineImpl final : Pipeline {
                          (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
   }
   void xpass(Ctx &c, const XArgs &xa) override {
-    if constexpr (XWAVE) c.sched[3] ? xpass_wave_v<false>(c, xa) : xpass_wave_v<true>(c, xa);
+    if constexpr (XWAVE) c.sched[3] ? xpass_wave_v<XFETCH_JIT>(c, xa) : xpass_wave_v<XFETCH_LDS>(c, xa);
     else c.sched[3] ? xpass_v<kXThreads[1]>(c, xa) : xpass_v<kXThreads[0]>(c, xa);
   }
 

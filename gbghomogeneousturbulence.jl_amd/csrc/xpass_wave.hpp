@@ -78,26 +78,38 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// PF: fetch the next component while the current one transforms (more registers, two waves per
-// SIMD), or fetch each component just before its transform (three waves per SIMD).
-template <typename T, int N, int NCIN, int XT, bool PF>
-__global__ __launch_bounds__(XT, PF ? 2 : 3) void k_xpass_wave(const cx<T> *__restrict__ rin, cx<T> *__restrict__ pxo, Geo g,
-                                                   const cx<double> *__restrict__ tw, XArgs xa) {
+// How the coefficients of each component reach the wavefront:
+//   XFETCH_JIT: loaded into registers just before the component's transform (the wavefront waits
+//               for HBM, the other waves of the SIMD fill the gap);
+//   XFETCH_LDS: the wavefront's rows of the next component (RW * kxp coefficients, contiguous in
+//               R) are copied HBM -> LDS by direct-to-LDS loads while the current component
+//               transforms; the copy holds no registers, and the first pass reads the staged rows.
+enum { XFETCH_JIT = 0, XFETCH_LDS = 1 };
+
+template <typename T, int N, int NCIN, int XT, int FETCH>
+__global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ rin, cx<T> *__restrict__ pxo, Geo g,
+                                                      const cx<double> *__restrict__ tw, XArgs xa) {
   static_assert(N == 512 || N == 1024, "three radix-8 passes on 64 lanes");
   static_assert(NCIN == 7 || NCIN == 6, "u, w and optionally phi");
+  static_assert(sizeof(cx<T>) == 8, "the stage copy moves 16-byte pairs of FP32 coefficients");
   constexpr bool HALF = N == 1024;     // one real row per wavefront (else a row pair)
   constexpr int H = 512;               // the wavefront's transform length
   constexpr int RW = HALF ? 1 : 2;     // rows per wavefront
   constexpr int XR = XT / 64;          // wavefronts per block
   constexpr int M = N / 3;
   constexpr int NP = H + H / 8;        // padded exchange buffer
-  extern __shared__ __align__(16) unsigned char smem_raw[];
+  // The exchange buffers are a static array and the stages the dynamic one, so the compiler
+  // tells their accesses apart: waiting for a stage copy never holds up an exchange.
+  __shared__ __align__(16) cx<T> xbuf[XR * NP];
+  extern __shared__ __align__(16) unsigned char smem_raw[];   // XFETCH_LDS: XR stages of RW * kxp
   __shared__ double red[XR];
   __shared__ cx<T> t64[64];
   const int w = threadIdx.x >> 6, j = threadIdx.x & 63;
-  cx<T> *wb = reinterpret_cast<cx<T> *>(smem_raw) + w * NP;
-  if (threadIdx.x < 64) t64[j] = {T(tw[(N / 64) * j].re), T(tw[(N / 64) * j].im)};   // W_64^j
-  __syncthreads();
+  cx<T> *wb = xbuf + w * NP;
+  // the padded row of R (kx 0..M and zeros to the tile width of 16): the launch checks g.kxp == KXP
+  constexpr int KXP = (M + 1 + 15) / 16 * 16;
+  constexpr int SROW = RW * KXP;                    // coefficients of one component's rows
+  cx<T> *stage = reinterpret_cast<cx<T> *>(smem_raw) + w * SROW;
 
   const int rows_per_plane = N / (RW * XR);
   const int z = blockIdx.x / rows_per_plane, box = blockIdx.y;
@@ -106,26 +118,41 @@ __global__ __launch_bounds__(XT, PF ? 2 : 3) void k_xpass_wave(const cx<T> *__re
   pxo += box * g.box_a;
   const size_t plane = (size_t)N * g.kxp;           // one z plane of one component in R / PX
   const size_t vol = (size_t)N * N * N;
+  const cx<T> *rbase = rin + (size_t)z * plane + (size_t)ya * g.kxp;
+
+  // the rows of component c -> this wavefront's stage, 64 16-byte pieces per load instruction
+  // (lane l's piece lands at 16 l bytes past the instruction's stage address)
+  auto stage_in = [&](int c) {
+    const char *src = reinterpret_cast<const char *>(rbase + (size_t)c * N * plane);
+    char *dst = reinterpret_cast<char *>(stage);
+    constexpr int PIECES = SROW * (int)sizeof(cx<T>) / 16;
+#pragma unroll
+    for (int i = 0; i < PIECES; i += 64)
+      if (PIECES % 64 == 0 || i + j < PIECES)
+        __builtin_amdgcn_global_load_lds(src + 16 * (i + j), dst + 16 * i, 16, 0, 0);
+  };
+  if constexpr (FETCH == XFETCH_LDS) stage_in(0);
+
+  if (threadIdx.x < 64) t64[j] = {T(tw[(N / 64) * j].re), T(tw[(N / 64) * j].im)};   // W_64^j
+  __syncthreads();
 
   cx<T> t3[7];                                      // W_512^{q j}
 #pragma unroll
   for (int q = 1; q < 8; ++q) t3[q - 1] = {T(tw[(N / H) * q * j].re), T(tw[(N / H) * q * j].im)};
+
   cx<T> wq[HALF ? 8 : 1];                           // W_1024^{j + 64 q} (HALF)
   if constexpr (HALF) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) wq[q] = {T(tw[j + 64 * q].re), T(tw[j + 64 * q].im)};
   }
 
-  // The coefficients of component c of R that the first pass of its inverse transform reads:
-  // slot q holds (a, b) at kx = j + 64 q (when <= M) or at kx = N - j - 64 q (when >= N - M),
-  // and zeros in between.  The next component's coefficients are fetched while the current
-  // one transforms, so its HBM latency hides behind the transform.
-  const cx<T> *rbase = rin + (size_t)z * plane + (size_t)ya * g.kxp;
-  cx<T> ra[8], rb[8];
+  // The coefficients of a component that the first pass of its inverse transform reads, from
+  // `src` (its rows in R, or their staged copy; row b follows row a at kxp): slot q holds (a, b)
+  // at kx = j + 64 q (when <= M) or at kx = N - j - 64 q (when >= N - M), and zeros in between.
   // (HALF: slot q holds X[m] and X[H - m] of the row at m = j + 64 q, zero beyond M; X[H] is
   // outside the ball)
-  auto fetch = [&](int c) {
-    const cx<T> *src = rbase + (size_t)c * N * plane;
+  cx<T> ra[8], rb[8];
+  auto coefficients = [&](const cx<T> *src) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int n = j + 64 * q;
@@ -143,11 +170,12 @@ __global__ __launch_bounds__(XT, PF ? 2 : 3) void k_xpass_wave(const cx<T> *__re
       }
     }
   };
-  // pack the fetched coefficients of component c as the spectrum of (row a) + i (row b), fetch
-  // component `next` (if any), and transform to physical space: x[q] = (row a, row b) at
-  // x = j + 64 q
+  // pack the coefficients of component c as the spectrum of (row a) + i (row b), start the copy of
+  // component `next` (if any) into the stage, and transform to physical space: x[q] = (row a,
+  // row b) at x = j + 64 q
   auto inverse = [&](cx<T>(&x)[8], int c, int next) {
-    if (!PF) fetch(c);
+    if constexpr (FETCH == XFETCH_LDS) coefficients(stage);
+    else coefficients(rbase + (size_t)c * N * plane);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       const int n = j + 64 * q;
@@ -170,11 +198,17 @@ __global__ __launch_bounds__(XT, PF ? 2 : 3) void k_xpass_wave(const cx<T> *__re
         x[q] = {a.re + b.im, b.re - a.im};
       }
     }
-    if (PF && next >= 0) fetch(next);
+    if constexpr (FETCH == XFETCH_LDS) {
+      if (next >= 0) {
+        // the stage's reads have returned (x holds them) before the copy overwrites it
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        stage_in(next);
+      }
+    }
     wave_fft512<+1>(x, wb, t64, t3, j);
   };
 
-  // product d to the spectrum, kx <= M kept, zeros up to KXP
+  // product d to the spectrum, kx <= M kept, zeros up to kxp
   auto forward = [&](int d, cx<T>(&x)[8]) {
     wave_fft512<-1>(x, wb, t64, t3, j);
     cx<T> *dst = pxo + (size_t)d * N * plane + (size_t)z * plane + (size_t)ya * g.kxp;
@@ -233,7 +267,6 @@ __global__ __launch_bounds__(XT, PF ? 2 : 3) void k_xpass_wave(const cx<T> *__re
 
   constexpr int W0 = 3, PHI = 6;   // components of R: u (0..2), w (3..5), phi (6)
   cx<T> u[3][8];
-  if (PF) fetch(0);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     inverse(u[c], c, c < 2 ? c + 1 : (NCIN == 7 ? PHI : W0));
