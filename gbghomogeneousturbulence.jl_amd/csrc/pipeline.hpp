@@ -25,9 +25,9 @@ constexpr int tile_width_of() { return sizeof(T) == 4 ? 16 : 8; }
 //              products staged in LDS where they fit (N <= 512); or (1) one fused kernel on 512;
 //   z inverse: a TX-wide tile on 64*TX threads (one first-pass butterfly per thread at N = 512)
 //              or on 32*TX threads (two);
-//   x pass: 256 or 512 threads per block (FP32 at N = 512 and 1024: the register-resident pass
-//           with the next component staged in LDS by direct-to-LDS loads, or loaded into
-//           registers just before its transform);
+//   x pass: 256 or 512 threads per block, or for the register-resident pass (FP32 at N = 512
+//           and 1024, FP64 at N = 256 and 512) the next component staged in LDS by direct-to-LDS
+//           loads, or each component loaded into registers just before its transform;
 //   y radix plan: fft_io's plan or AltPlan's, led by a radix-16 pass.
 // At N >= 1024 the z schedules use narrower tiles (below).
 constexpr int kYThreads[2] = {256, 512};
@@ -52,10 +52,10 @@ struct PipelineImpl final : Pipeline {
   static constexpr int xr() { return XT / xpass_lanes<T, N, XT>(); }
   template <int XT>
   static constexpr size_t lds_x() { return ((size_t)xr<XT>() * N + N) * sizeof(cx<T>); }
-  // FP32 at N = 512 and 1024: the register-resident x pass, one wavefront per row pair (512) or
-  // per row (1024, as a 512-point complex sequence)
-  static constexpr bool XWAVE = TWO_THIRDS && std::is_same<T, float>::value && (N == 512 || N == 1024);
-  static constexpr int XW_ROWS = N == 1024 ? 1 : 2;   // rows per wavefront
+  // the register-resident x pass, one wavefront per row pair or per row: FP32 at N = 512 and 1024
+  // (a 512-point complex transform per wavefront), FP64 at N = 256 and 512 (256 points)
+  static constexpr bool XWAVE = TWO_THIRDS && XWave<T, N>::SUPPORTED;
+  static constexpr int XW_ROWS = XWave<T, N>::RW;   // rows per wavefront
   // the register-resident x pass's stages (XFETCH_LDS): one component's rows per wavefront
   static constexpr int KXP = (M + 1 + TX - 1) / TX * TX;   // the padded row of every compact array
   template <int XT>
@@ -312,13 +312,13 @@ struct PipelineImpl final : Pipeline {
     else c.sched[0] ? yinv_v<YT[1], 0>(c) : yinv_v<YT[0], 0>(c);
   }
 
-  // FP32 at N = 512 and 1024: schedule 0 stages the next component in LDS, schedule 1 loads each
-  // component just before its transform (256 threads)
+  // schedule 0 stages the next component in LDS, schedule 1 loads each component just before its
+  // transform (256 threads)
   template <int F>
   void xpass_wave_v(Ctx &c, const XArgs &xa) {
     constexpr int XT = 256;
     const int blocks = N * N / (XW_ROWS * (XT / 64));
-    static_assert(!XWAVE || KXP == (M + 1 + 15) / 16 * 16, "the staged rows are the kernel's padded rows");
+    static_assert(!XWAVE || KXP == XWave<T, N>::KXP, "the staged rows are the kernel's padded rows");
     if (F == XFETCH_LDS && c.geo.kxp != KXP) return xpass_wave_v<XFETCH_JIT>(c, xa);   // rows of another padding
     const size_t lds = F == XFETCH_LDS ? lds_xw<XT>() : 0;
     if (c.cfg.has_scalar)

@@ -1,39 +1,58 @@
-// The x pass at N = 512 with one wavefront per row pair and the data in registers.
+// The x pass with one wavefront per row pair (or per row) and the data in registers: FP32 at
+// N = 512 (pairs) and 1024 (rows), FP64 at N = 256 (pairs) and 512 (rows).
 //
-// A 512-point transform on the 64 lanes of a wavefront is three radix-8 Stockham passes in which
-// lane j is butterfly j of every pass: it reads elements j + 64 q (q = 0..7) and, after the last
-// pass, holds the outputs j + 64 q.  So the inverse transforms leave lane j with the physical
-// values at x = j + 64 q of every component, the products are formed lane-locally, and they feed
-// the forward transforms' first pass directly.  Only the two exchanges between passes go through
-// LDS (a wave-private, padded buffer).  The per-lane twiddles of the third pass are held in
-// registers for the whole kernel; the second pass's, W_64^{q (j mod 8)}, come from a 64-entry
-// table in LDS.
+// A wavefront transforms H complex points, H = 512 (FP32) or 256 (FP64), S = H / 64 per lane: three
+// radix-8 or four radix-4 Stockham passes in which lane j is butterfly j of every pass: it reads
+// elements j + 64 q (q = 0..S-1) and, after the last pass, holds the outputs j + 64 q.  So the
+// inverse transforms leave lane j with the physical values at x = j + 64 q of every component,
+// the products are formed lane-locally, and they feed the forward transforms' first pass
+// directly.  Only the exchanges between passes go through LDS (a wave-private, padded buffer).
+// The per-lane twiddles of the last pass are held in registers for the whole kernel; the earlier
+// passes' come from a 64-entry table of W_64 in LDS.  Either way a component costs S complex
+// values (16 registers) per lane.
 //
-// The Hermitian pack of two real rows (row a in the real part, row b in the imaginary part) is
-// read straight from HBM into the first pass: element n of the packed spectrum is built from
-// (a, b) at kx = n (n <= M) or kx = N - n (n >= N - M), and is zero in between.  The forward
-// transforms' outputs are unpacked with the conjugate partner C[N - kx], which lives in lane
-// (64 - j) mod 64 and is fetched with a cross-lane permute.
+// With H = N a wavefront takes a row pair: the Hermitian pack of two real rows (row a in the real
+// part, row b in the imaginary part) is read into the first pass: element n of the packed
+// spectrum is built from (a, b) at kx = n (n <= M) or kx = N - n (n >= N - M), and is zero in
+// between.  The forward transforms' outputs are unpacked with the conjugate partner C[N - kx],
+// which lives in lane (64 - j) mod 64 and is fetched with a cross-lane permute.
 //
-// The components are streamed so that at most u, two partial Lamb components and the component
-// in flight are live: u, then phi and the three fluxes u phi, then w_x, w_y, w_z, each adding
-// its terms of u x w, each Lamb component transformed as soon as it is complete.
-//
-// At N = 1024 a wavefront takes one real row instead of a pair, as the 512-point complex sequence
-// z[n] = x[2n] + i x[2n+1] (the same three-pass transform, the same registers).  With H = 512 and
-// W = exp(-2 pi i / 1024):
+// With N = 2 H a wavefront takes one real row, as the H-point complex sequence
+// z[n] = x[2n] + i x[2n+1].  With W = exp(-2 pi i / N):
 //   inverse:  Z[m] = (X[m] + conj X[H-m]) + i W^-m (X[m] - conj X[H-m]),  z = IFFT_H(Z);
 //   forward:  Z = FFT_H(z),  X[k] = (Z[k] + conj Z[H-k]) / 2 + W^k (Z[k] - conj Z[H-k]) / (2 i).
 // Lane j then holds the physical pairs x[2n], x[2n+1] at n = j + 64 q, and the products stay
 // lane-local exactly as for a row pair.
+//
+// The components are streamed so that at most u, two partial Lamb components and the component
+// in flight are live: u, then phi and the three fluxes u phi, then w_x, w_y, w_z, each adding
+// its terms of u x w, each Lamb component transformed as soon as it is complete.
 #pragma once
 #include "pencil.hpp"
 
 namespace hdns {
 
-// wave-private exchange buffer: one pad slot per 8 values spreads a pass's strided accesses
-// over the LDS banks
-__device__ __forceinline__ int xpad(int i) { return i + (i >> 3); }
+// The wave-private exchange buffer's padding: one pad slot per 8 (FP32) or 4 (FP64) values
+// spreads a pass's strided accesses over the LDS banks.
+template <typename T>
+constexpr int xpad_shift() { return sizeof(T) == 4 ? 3 : 2; }
+template <typename T>
+__device__ __forceinline__ int xpad(int i) { return i + (i >> xpad_shift<T>()); }
+
+// The shape of the register-resident x pass for (T, N).
+template <typename T, int N>
+struct XWave {
+  static constexpr int H = sizeof(T) == 4 ? 512 : 256;   // the wavefront's complex transform length
+  static constexpr int S = H / 64;                       // values per lane per component
+  static constexpr bool SUPPORTED = N == H || N == 2 * H;
+  static constexpr bool HALF = N == 2 * H;               // one real row per wavefront (else a pair)
+  static constexpr int RW = HALF ? 1 : 2;                // rows per wavefront
+  static constexpr int M = N / 3;
+  static constexpr int TXW = sizeof(T) == 4 ? 16 : 8;    // the tile width that pads every row
+  static constexpr int KXP = (M + 1 + TXW - 1) / TXW * TXW;
+  static constexpr int QF = (KXP + 63) / 64;             // slots holding stored outputs kx < KXP
+  static constexpr int NP = H + (H >> xpad_shift<T>());  // the padded exchange buffer
+};
 
 template <int SIGN, typename T>
 __device__ __forceinline__ cx<T> twiddled(cx<T> v, cx<T> w) {
@@ -47,24 +66,59 @@ __device__ __forceinline__ void wave_fft512(cx<T> (&x)[8], cx<T> *wb, const cx<T
                                             int j) {
   dft8<SIGN>(x);                                     // NS = 1: outputs 8 j + q
 #pragma unroll
-  for (int q = 0; q < 8; ++q) wb[xpad(8 * j + q)] = x[q];
+  for (int q = 0; q < 8; ++q) wb[xpad<T>(8 * j + q)] = x[q];
   pass_sync<true>();
 #pragma unroll
-  for (int q = 0; q < 8; ++q) x[q] = wb[xpad(j + 64 * q)];
+  for (int q = 0; q < 8; ++q) x[q] = wb[xpad<T>(j + 64 * q)];
   pass_sync<true>();
 #pragma unroll
   for (int q = 1; q < 8; ++q) x[q] = twiddled<SIGN>(x[q], t64[q * (j & 7)]);
   dft8<SIGN>(x);                                     // NS = 8: outputs 64 (j/8) + j mod 8 + 8 q
   const int base = (j >> 3) * 64 + (j & 7);
 #pragma unroll
-  for (int q = 0; q < 8; ++q) wb[xpad(base + 8 * q)] = x[q];
+  for (int q = 0; q < 8; ++q) wb[xpad<T>(base + 8 * q)] = x[q];
   pass_sync<true>();
 #pragma unroll
-  for (int q = 0; q < 8; ++q) x[q] = wb[xpad(j + 64 * q)];
+  for (int q = 0; q < 8; ++q) x[q] = wb[xpad<T>(j + 64 * q)];
   pass_sync<true>();                                 // the buffer may be rewritten at once
 #pragma unroll
   for (int q = 1; q < 8; ++q) x[q] = twiddled<SIGN>(x[q], t3[q - 1]);
   dft8<SIGN>(x);                                     // NS = 64: outputs j + 64 q
+}
+
+// The 256-point transform in four radix-4 passes: t64[m] = W_64^m (W_16^m = t64[4 m]),
+// t3[q - 1] = W_256^{q j}.
+template <int SIGN, typename T>
+__device__ __forceinline__ void wave_fft256(cx<T> (&x)[4], cx<T> *wb, const cx<T> *t64, const cx<T> (&t3)[3],
+                                            int j) {
+  auto exchange = [&](int base, int stride) {        // outputs base + stride q -> inputs j + 64 q
+#pragma unroll
+    for (int q = 0; q < 4; ++q) wb[xpad<T>(base + stride * q)] = x[q];
+    pass_sync<true>();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) x[q] = wb[xpad<T>(j + 64 * q)];
+    pass_sync<true>();
+  };
+  dft4<SIGN>(x);                                     // NS = 1: outputs 4 j + q
+  exchange(4 * j, 1);
+#pragma unroll
+  for (int q = 1; q < 4; ++q) x[q] = twiddled<SIGN>(x[q], t64[4 * q * (j & 3)]);
+  dft4<SIGN>(x);                                     // NS = 4: outputs 16 (j/4) + j mod 4 + 4 q
+  exchange((j >> 2) * 16 + (j & 3), 4);
+#pragma unroll
+  for (int q = 1; q < 4; ++q) x[q] = twiddled<SIGN>(x[q], t64[q * (j & 15)]);
+  dft4<SIGN>(x);                                     // NS = 16: outputs 64 (j/16) + j mod 16 + 16 q
+  exchange((j >> 4) * 64 + (j & 15), 16);
+#pragma unroll
+  for (int q = 1; q < 4; ++q) x[q] = twiddled<SIGN>(x[q], t3[q - 1]);
+  dft4<SIGN>(x);                                     // NS = 64: outputs j + 64 q
+}
+
+template <int SIGN, typename T, int S>
+__device__ __forceinline__ void wave_fft(cx<T> (&x)[S], cx<T> *wb, const cx<T> *t64, const cx<T> (&t3)[S - 1],
+                                         int j) {
+  if constexpr (S == 8) wave_fft512<SIGN>(x, wb, t64, t3, j);
+  else wave_fft256<SIGN>(x, wb, t64, t3, j);
 }
 
 // the value v of lane `lane` (all lanes active)
@@ -89,15 +143,12 @@ enum { XFETCH_JIT = 0, XFETCH_LDS = 1 };
 template <typename T, int N, int NCIN, int XT, int FETCH>
 __global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ rin, cx<T> *__restrict__ pxo, Geo g,
                                                       const cx<double> *__restrict__ tw, XArgs xa) {
-  static_assert(N == 512 || N == 1024, "three radix-8 passes on 64 lanes");
+  using X = XWave<T, N>;
+  static_assert(X::SUPPORTED, "a row pair of H points or a row of 2 H points per wavefront");
   static_assert(NCIN == 7 || NCIN == 6, "u, w and optionally phi");
-  static_assert(sizeof(cx<T>) == 8, "the stage copy moves 16-byte pairs of FP32 coefficients");
-  constexpr bool HALF = N == 1024;     // one real row per wavefront (else a row pair)
-  constexpr int H = 512;               // the wavefront's transform length
-  constexpr int RW = HALF ? 1 : 2;     // rows per wavefront
+  constexpr bool HALF = X::HALF;
+  constexpr int H = X::H, S = X::S, RW = X::RW, M = X::M, NP = X::NP;
   constexpr int XR = XT / 64;          // wavefronts per block
-  constexpr int M = N / 3;
-  constexpr int NP = H + H / 8;        // padded exchange buffer
   // The exchange buffers are a static array and the stages the dynamic one, so the compiler
   // tells their accesses apart: waiting for a stage copy never holds up an exchange.
   __shared__ __align__(16) cx<T> xbuf[XR * NP];
@@ -106,9 +157,8 @@ __global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ 
   __shared__ cx<T> t64[64];
   const int w = threadIdx.x >> 6, j = threadIdx.x & 63;
   cx<T> *wb = xbuf + w * NP;
-  // the padded row of R (kx 0..M and zeros to the tile width of 16): the launch checks g.kxp == KXP
-  constexpr int KXP = (M + 1 + 15) / 16 * 16;
-  constexpr int SROW = RW * KXP;                    // coefficients of one component's rows
+  // the padded row of R (kx 0..M and zeros to the tile width): the launch checks g.kxp == KXP
+  constexpr int SROW = RW * X::KXP;                 // coefficients of one component's rows
   cx<T> *stage = reinterpret_cast<cx<T> *>(smem_raw) + w * SROW;
 
   const int rows_per_plane = N / (RW * XR);
@@ -136,14 +186,13 @@ __global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ 
   if (threadIdx.x < 64) t64[j] = {T(tw[(N / 64) * j].re), T(tw[(N / 64) * j].im)};   // W_64^j
   __syncthreads();
 
-  cx<T> t3[7];                                      // W_512^{q j}
+  cx<T> t3[S - 1];                                  // W_H^{q j}
 #pragma unroll
-  for (int q = 1; q < 8; ++q) t3[q - 1] = {T(tw[(N / H) * q * j].re), T(tw[(N / H) * q * j].im)};
-
-  cx<T> wq[HALF ? 8 : 1];                           // W_1024^{j + 64 q} (HALF)
+  for (int q = 1; q < S; ++q) t3[q - 1] = {T(tw[(N / H) * q * j].re), T(tw[(N / H) * q * j].im)};
+  cx<T> wq[HALF ? S : 1];                           // W_N^{j + 64 q} (HALF)
   if constexpr (HALF) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) wq[q] = {T(tw[j + 64 * q].re), T(tw[j + 64 * q].im)};
+    for (int q = 0; q < S; ++q) wq[q] = {T(tw[j + 64 * q].re), T(tw[j + 64 * q].im)};
   }
 
   // The coefficients of a component that the first pass of its inverse transform reads, from
@@ -151,10 +200,10 @@ __global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ 
   // at kx = j + 64 q (when <= M) or at kx = N - j - 64 q (when >= N - M), and zeros in between.
   // (HALF: slot q holds X[m] and X[H - m] of the row at m = j + 64 q, zero beyond M; X[H] is
   // outside the ball)
-  cx<T> ra[8], rb[8];
+  cx<T> ra[S], rb[S];
   auto coefficients = [&](const cx<T> *src) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < S; ++q) {
       const int n = j + 64 * q;
       ra[q] = czero<T>();
       rb[q] = czero<T>();
@@ -173,11 +222,17 @@ __global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ 
   // pack the coefficients of component c as the spectrum of (row a) + i (row b), start the copy of
   // component `next` (if any) into the stage, and transform to physical space: x[q] = (row a,
   // row b) at x = j + 64 q
-  auto inverse = [&](cx<T>(&x)[8], int c, int next) {
-    if constexpr (FETCH == XFETCH_LDS) coefficients(stage);
-    else coefficients(rbase + (size_t)c * N * plane);
+  auto inverse = [&](cx<T>(&x)[S], int c, int next) {
+    if constexpr (FETCH == XFETCH_LDS) {
+      // the copy has landed: direct-to-LDS loads count in vmcnt until their data are in LDS, and
+      // the compiler does not always order LDS reads after them by itself
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      coefficients(stage);
+    } else {
+      coefficients(rbase + (size_t)c * N * plane);
+    }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < S; ++q) {
       const int n = j + 64 * q;
       cx<T> a = ra[q], b = rb[q];
       if constexpr (HALF) {
@@ -205,21 +260,21 @@ __global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ 
         stage_in(next);
       }
     }
-    wave_fft512<+1>(x, wb, t64, t3, j);
+    wave_fft<+1>(x, wb, t64, t3, j);
   };
 
   // product d to the spectrum, kx <= M kept, zeros up to kxp
-  auto forward = [&](int d, cx<T>(&x)[8]) {
-    wave_fft512<-1>(x, wb, t64, t3, j);
+  auto forward = [&](int d, cx<T>(&x)[S]) {
+    wave_fft<-1>(x, wb, t64, t3, j);
     cx<T> *dst = pxo + (size_t)d * N * plane + (size_t)z * plane + (size_t)ya * g.kxp;
     const int partner = (64 - j) & 63;
     if constexpr (HALF) {
-      // X[k] for k = j + 64 q <= M (q <= 5): Z[H - k] is slot 7 - q of lane 64 - j, or slot
-      // (8 - q) mod 8 of lane 0
+      // X[k] for k = j + 64 q (q < QF): Z[H - k] is slot S - 1 - q of lane 64 - j, or slot
+      // (S - q) mod S of lane 0
 #pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        cx<T> zp = {lane_value(x[7 - q].re, partner), lane_value(x[7 - q].im, partner)};
-        if (j == 0) zp = x[(8 - q) & 7];
+      for (int q = 0; q < X::QF; ++q) {
+        cx<T> zp = {lane_value(x[S - 1 - q].re, partner), lane_value(x[S - 1 - q].im, partner)};
+        if (j == 0) zp = x[(S - q) & (S - 1)];
         const int kx = j + 64 * q;
         if (kx <= M) {
           const cx<T> Z = x[q];
@@ -234,10 +289,10 @@ __global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ 
       return;
     }
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      // C[N - kx] for kx = j + 64 q: slot 7 - q of lane 64 - j, or slot (8 - q) mod 8 of lane 0
-      cx<T> cm = {lane_value(x[7 - q].re, partner), lane_value(x[7 - q].im, partner)};
-      if (j == 0) cm = x[(8 - q) & 7];
+    for (int q = 0; q < X::QF; ++q) {
+      // C[N - kx] for kx = j + 64 q: slot S - 1 - q of lane 64 - j, or slot (S - q) mod S of lane 0
+      cx<T> cm = {lane_value(x[S - 1 - q].re, partner), lane_value(x[S - 1 - q].im, partner)};
+      if (j == 0) cm = x[(S - q) & (S - 1)];
       const int kx = j + 64 * q;
       if (kx <= M) {
         const cx<T> C = x[q];
@@ -253,9 +308,9 @@ __global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ 
   const size_t rowa = ((size_t)z * N + ya) * N, rowb = rowa + N;
   T *phys = reinterpret_cast<T *>(xa.phys) + box * xa.box_phys;
   T *physw = reinterpret_cast<T *>(xa.physw) + box * xa.box_physw;
-  auto store_phys = [&](T *base, const cx<T>(&x)[8]) {
+  auto store_phys = [&](T *base, const cx<T>(&x)[S]) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < S; ++q) {
       if constexpr (HALF) {   // (x[2n], x[2n+1]) at n = j + 64 q
         *reinterpret_cast<cx<T> *>(base + rowa + 2 * (j + 64 * q)) = x[q];
       } else {
@@ -266,7 +321,7 @@ __global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ 
   };
 
   constexpr int W0 = 3, PHI = 6;   // components of R: u (0..2), w (3..5), phi (6)
-  cx<T> u[3][8];
+  cx<T> u[3][S];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     inverse(u[c], c, c < 2 ? c + 1 : (NCIN == 7 ? PHI : W0));
@@ -276,7 +331,7 @@ __global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ 
   double mx = 0.0;
   if (xa.want_max) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < S; ++q) {
       const T sa = u[0][q].re * u[0][q].re + u[1][q].re * u[1][q].re + u[2][q].re * u[2][q].re;
       const T sb = u[0][q].im * u[0][q].im + u[1][q].im * u[1][q].im + u[2][q].im * u[2][q].im;
       const double da = (double)sa, db = (double)sb;
@@ -286,47 +341,47 @@ __global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ 
   }
 
   if constexpr (NCIN == 7) {  // the fluxes u phi (multiply!(u, phi, u), src/Generator.jl:727)
-    cx<T> ph[8];
+    cx<T> ph[S];
     inverse(ph, PHI, W0);
     if (xa.store_phi) store_phys(phys + 3 * vol, ph);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      cx<T> p[8];
+      cx<T> p[S];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) p[q] = {u[d][q].re * ph[q].re, u[d][q].im * ph[q].im};
+      for (int q = 0; q < S; ++q) p[q] = {u[d][q].re * ph[q].re, u[d][q].im * ph[q].im};
       forward(3 + d, p);
     }
   }
 
   // the negated Lamb vector L = u x w (cross!(w, u, w), src/Generator.jl:529):
   //   L_0 = u_1 w_2 - u_2 w_1,  L_1 = u_2 w_0 - u_0 w_2,  L_2 = u_0 w_1 - u_1 w_0
-  cx<T> l0[8], l1[8];
+  cx<T> l0[S], l1[S];
   {
-    cx<T> w0[8];
+    cx<T> w0[S];
     inverse(w0, W0, W0 + 1);
     if (xa.store_w) store_phys(physw, w0);
-    cx<T> l2[8];
+    cx<T> l2[S];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < S; ++q) {
       l1[q] = {u[2][q].re * w0[q].re, u[2][q].im * w0[q].im};
       l2[q] = {-(u[1][q].re * w0[q].re), -(u[1][q].im * w0[q].im)};
     }
-    cx<T> w1[8];
+    cx<T> w1[S];
     inverse(w1, W0 + 1, W0 + 2);
     if (xa.store_w) store_phys(physw + vol, w1);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < S; ++q) {
       l0[q] = {-(u[2][q].re * w1[q].re), -(u[2][q].im * w1[q].im)};
       l2[q] = {u[0][q].re * w1[q].re + l2[q].re, u[0][q].im * w1[q].im + l2[q].im};
     }
     forward(2, l2);
   }
   {
-    cx<T> w2[8];
+    cx<T> w2[S];
     inverse(w2, W0 + 2, -1);
     if (xa.store_w) store_phys(physw + 2 * vol, w2);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < S; ++q) {
       l0[q] = {u[1][q].re * w2[q].re + l0[q].re, u[1][q].im * w2[q].im + l0[q].im};
       l1[q] = {l1[q].re - u[0][q].re * w2[q].re, l1[q].im - u[0][q].im * w2[q].im};
     }

@@ -225,15 +225,17 @@ def test_fp32_bulk_at_512_agrees_with_the_fp64_step(g, scalar):
         assert rel(a.physical(0), b.physical(0)) < 3e-6
 
 
+@pytest.mark.parametrize("n,dtype,tol", [(512, np.float32, 1e-6), (256, np.float64, 1e-14)])
 @pytest.mark.parametrize("scalar", [True, False])
-def test_x_pass_fetch_schedules_agree_at_512(g, scalar):
-    """The register-resident x pass reads each component's rows either from their copy staged in
+def test_x_pass_fetch_schedules_agree(g, n, dtype, tol, scalar):
+    """The register-resident x pass (FP32 at 512: row pairs of 512 points; FP64 at 256: row pairs
+    of 256 points in radix-4 passes) reads each component's rows either from their copy staged in
     LDS by direct-to-LDS loads (x schedule 0) or straight from HBM (x schedule 1).  The values are
     the same; the two builds may contract different multiply-adds, so two steps, the physical
-    fields and max|u| agree to FP32 rounding (a misplaced row would be an O(1) error)."""
-    n, dt = 512, 1e-3
-    a, *_ = make_sim(g, n, np.float32, forcing=0.1, nu=8.42e-4, scalar=scalar)
-    b, *_ = make_sim(g, n, np.float32, forcing=0.1, nu=8.42e-4, scalar=scalar)
+    fields and max|u| agree to rounding (a misplaced row would be an O(1) error)."""
+    dt = 1e-3
+    a, *_ = make_sim(g, n, dtype, forcing=0.1, nu=8.42e-4, scalar=scalar)
+    b, *_ = make_sim(g, n, dtype, forcing=0.1, nu=8.42e-4, scalar=scalar)
     with a, b:
         a.schedule((0, 0, 0, 0, 0))
         b.schedule((0, 0, 0, 1, 0))
@@ -244,9 +246,10 @@ def test_x_pass_fetch_schedules_agree_at_512(g, scalar):
         eu = rel(sa.velocity_spectral, sb.velocity_spectral)
         ep = rel(sa.scalar_spectral, sb.scalar_spectral) if scalar else 0.0
         ex = rel(a.physical(0), b.physical(0))
-        print(f"x fetch schedules at 512, scalar={scalar}: u {eu:.2e} phi {ep:.2e} physical {ex:.2e}", flush=True)
-        assert eu < 1e-6 and ep < 1e-6 and ex < 1e-6
-        assert a.max_speed() == pytest.approx(b.max_speed(), rel=1e-6)
+        print(f"x fetch schedules at {n} {np.dtype(dtype).name}, scalar={scalar}: u {eu:.2e} phi {ep:.2e} "
+              f"physical {ex:.2e}", flush=True)
+        assert eu < tol and ep < tol and ex < tol
+        assert a.max_speed() == pytest.approx(b.max_speed(), rel=tol)
 
 
 def test_graph_captured_step_replays_the_eager_step_bitwise(g):
