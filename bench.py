@@ -427,8 +427,8 @@ def kernel_table(res: dict, n: int, real_bytes: int, stages: int, ssp: bool = Fa
         avg_ms = float(res["kernel_ms"][i]) / cnt
         b = ab.get(name, 0)
         sym = KERNEL_SYMBOLS[i]
-        if sym == "k_xpass" and (real_bytes, n) in ((4, 512), (4, 1024), (8, 256), (8, 512)):
-            sym = "k_xpass_wave"   # the register-resident x pass: FP32 at N = 512, 1024; FP64 at 256, 512
+        if sym == "k_xpass" and (real_bytes, n) in ((4, 256), (4, 512), (4, 1024), (8, 256), (8, 512)):
+            sym = "k_xpass_wave"   # the register-resident x pass: FP32 at N = 256, 512, 1024; FP64 at 256, 512
         out[name] = {"symbol": sym, "launches": cnt, "avg_ms": round(avg_ms, 5),
                      "share": round(float(res["kernel_ms"][i]) / max(res["ms_total"], 1e-9), 4),
                      "algorithmic_bytes": b, "achieved_GBs": round(b / (avg_ms * 1e-3) / 1e9, 1) if b else None}

@@ -154,22 +154,46 @@ struct XArgs {
   size_t box_phys, box_physw;    // reals per box in phys / physw
 };
 
-// lanes per row pair: a lane holds N/LPR values of each component
-template <typename T, int N, int XT>
+// lanes per row pair: a lane holds N/LPR values of each component.  BATCH: below 64 lanes a row
+// pair takes the power of two at or above N/e lanes, and a wavefront holds 64/LPR row pairs and
+// transforms them as one batch (else a row pair takes a whole wavefront).
+template <typename T, int N, int XT, bool BATCH = false>
 constexpr int xpass_lanes() {
   constexpr int e = sizeof(T) == 4 ? 8 : 4;
   constexpr int l = N / e;
-  return l < 64 ? 64 : (l > XT ? XT : l);
+  if constexpr (BATCH && l < 64) {
+    int p = 1;
+    while (p < l) p *= 2;
+    return p;
+  } else {
+    return l < 64 ? 64 : (l > XT ? XT : l);
+  }
+}
+
+// The transform of this thread's row pair, whose buffer wb is N values.  LPR < 64: the wavefront's
+// 64/LPR row pairs have consecutive buffers and go through the passes together, one butterfly per
+// lane where the old layout left all but N/R lanes idle; LPR = 64: one row pair per wavefront;
+// LPR > 64: the row pair spans wavefronts and the passes synchronise the block.
+template <typename T, int N, int LPR, int SIGN>
+__device__ __forceinline__ void xrow_fft(cx<T> *wb, const cx<T> *twl) {
+  if constexpr (LPR < 64) {
+    constexpr int G = 64 / LPR;
+    cx<T> *first = wb - (int)((threadIdx.x % 64) / LPR) * N;
+    fft_columns<T, N, G, 64, SIGN, false, true>(first, twl);
+  } else {
+    fft_columns<T, N, 1, LPR, SIGN, false, LPR == 64>(wb, twl);
+  }
 }
 
 template <typename T, int N, int NCIN, int XT, int DA = DEALIAS_TWO_THIRDS>
 __global__ __launch_bounds__(XT) void k_xpass(const cx<T> *__restrict__ rin,
                                                               cx<T> *__restrict__ pxo, Geo g,
                                                               const cx<double> *__restrict__ tw, XArgs xa) {
-  constexpr int LPR = xpass_lanes<T, N, XT>();
-  constexpr bool WAVE = LPR == 64;
+  constexpr int LPR = xpass_lanes<T, N, XT, true>();
+  constexpr bool WAVE = LPR <= 64;                 // a wavefront holds whole row pairs
   constexpr int NT = XT;
   constexpr int XR = NT / LPR;                     // row pairs per block
+  static_assert((N * N / 2) % XR == 0, "the blocks tile the N^2/2 row pairs of a box");
   constexpr int E = (N + LPR - 1) / LPR;           // physical elements per lane per row pair
   constexpr int M = band_m(N, DA);
   constexpr int PF = (M + 1 + LPR - 1) / LPR;      // retained coefficients per lane per row
@@ -182,10 +206,9 @@ __global__ __launch_bounds__(XT) void k_xpass(const cx<T> *__restrict__ rin,
   stage_twiddles<T, N, NT>(twl, tw);
   __syncthreads();
 
-  const int rows_per_plane = N / (2 * XR);
-  const int z = blockIdx.x / rows_per_plane, box = blockIdx.y;
-  const int w = threadIdx.x / LPR, lane = threadIdx.x % LPR;
-  const int ya = (blockIdx.x % rows_per_plane) * (2 * XR) + 2 * w;
+  const int w = threadIdx.x / LPR, lane = threadIdx.x % LPR, box = blockIdx.y;
+  const int rp = blockIdx.x * XR + w;              // row pair (z, ya, ya + 1), y fastest
+  const int z = rp / (N / 2), ya = 2 * (rp % (N / 2));
   rin += box * g.box_b;
   pxo += box * g.box_a;
   const size_t plane = (size_t)N * g.kxp;           // one z plane of one component in R / PX
@@ -225,7 +248,7 @@ __global__ __launch_bounds__(XT) void k_xpass(const cx<T> *__restrict__ rin,
     }
     if (c + 1 < NCIN) fetch(c + 1);
     pass_sync<WAVE>();
-    fft_columns<T, N, 1, LPR, +1, false, WAVE>(wb, twl);
+    xrow_fft<T, N, LPR, +1>(wb, twl);
 #pragma unroll
     for (int q = 0; q < E; ++q)
       if (lane + LPR * q < N) f[c][q] = wb[lane + LPR * q];
@@ -293,7 +316,7 @@ __global__ __launch_bounds__(XT) void k_xpass(const cx<T> *__restrict__ rin,
       wb[lane + LPR * q] = v;
     }
     pass_sync<WAVE>();
-    fft_columns<T, N, 1, LPR, -1, false, WAVE>(wb, twl);
+    xrow_fft<T, N, LPR, -1>(wb, twl);
     cx<T> *dst = pxo + (size_t)d * N * plane + (size_t)z * plane + (size_t)ya * g.kxp;
     for (int kx = lane; kx < g.kxp; kx += LPR) {   // zeros past M keep every line whole
       cx<T> A = czero<T>(), B = czero<T>();

@@ -25,9 +25,10 @@ constexpr int tile_width_of() { return sizeof(T) == 4 ? 16 : 8; }
 //              products staged in LDS where they fit (N <= 512); or (1) one fused kernel on 512;
 //   z inverse: a TX-wide tile on 64*TX threads (one first-pass butterfly per thread at N = 512)
 //              or on 32*TX threads (two);
-//   x pass: 256 or 512 threads per block, or for the register-resident pass (FP32 at N = 512
-//           and 1024, FP64 at N = 256 and 512) the next component staged in LDS by direct-to-LDS
-//           loads, or each component loaded into registers just before its transform;
+//   x pass: 256 or 512 threads per block, or for the register-resident pass (FP32 at N = 256,
+//           512 and 1024, FP64 at N = 256 and 512) the next component staged in LDS by
+//           direct-to-LDS loads, or each component loaded into registers just before its
+//           transform;
 //   y radix plan: fft_io's plan or AltPlan's, led by a radix-16 pass.
 // At N >= 1024 the z schedules use narrower tiles (below).
 constexpr int kYThreads[2] = {256, 512};
@@ -48,12 +49,20 @@ struct PipelineImpl final : Pipeline {
   static constexpr int M = band_m(N, DA), K = band_k(N, DA);
   static constexpr bool TWO_THIRDS = DA == DEALIAS_TWO_THIRDS;
   static constexpr size_t LDS_COL = ((size_t)N * TX + N) * sizeof(cx<T>);   // tile + twiddles
-  template <int XT>
-  static constexpr int xr() { return XT / xpass_lanes<T, N, XT>(); }
-  template <int XT>
-  static constexpr size_t lds_x() { return ((size_t)xr<XT>() * N + N) * sizeof(cx<T>); }
-  // the register-resident x pass, one wavefront per row pair or per row: FP32 at N = 512 and 1024
-  // (a 512-point complex transform per wavefront), FP64 at N = 256 and 512 (256 points)
+  // row pairs per block and LDS of the x kernels on XT threads (BATCH: the forward x pass, whose
+  // wavefronts batch several row pairs at small N)
+  template <int XT, bool BATCH = false>
+  static constexpr int xr() { return XT / xpass_lanes<T, N, XT, BATCH>(); }
+  template <int XT, bool BATCH = false>
+  static constexpr size_t lds_x() { return ((size_t)xr<XT, BATCH>() * N + N) * sizeof(cx<T>); }
+  // the forward x pass's threads per block under schedule V: no more row pairs than a box has
+  template <int V>
+  static constexpr int xthreads() {
+    constexpr int cap = xpass_lanes<T, N, kXThreads[V], true>() * (N * N / 2);
+    return kXThreads[V] < cap ? kXThreads[V] : cap;
+  }
+  // the register-resident x pass, one wavefront per row pair or per row: FP32 at N = 256, 512 and
+  // 1024 (a 256- or 512-point complex transform per wavefront), FP64 at N = 256 and 512 (256)
   static constexpr bool XWAVE = TWO_THIRDS && XWave<T, N>::SUPPORTED;
   static constexpr int XW_ROWS = XWave<T, N>::RW;   // rows per wavefront
   // the register-resident x pass's stages (XFETCH_LDS): one component's rows per wavefront
@@ -116,8 +125,9 @@ struct PipelineImpl final : Pipeline {
       set_lds((const void *)k_xpass_wave<T, N, 7, 256, F>, F == XFETCH_LDS ? lds_xw<256>() : 0);
       set_lds((const void *)k_xpass_wave<T, N, 6, 256, F>, F == XFETCH_LDS ? lds_xw<256>() : 0);
     } else {
-      set_lds((const void *)k_xpass<T, N, 7, kXThreads[V], DA>, lds_x<kXThreads[V]>());
-      set_lds((const void *)k_xpass<T, N, 6, kXThreads[V], DA>, lds_x<kXThreads[V]>());
+      constexpr int XT = xthreads<V>();
+      set_lds((const void *)k_xpass<T, N, 7, XT, DA>, lds_x<XT, true>());
+      set_lds((const void *)k_xpass<T, N, 6, XT, DA>, lds_x<XT, true>());
     }
   }
 
@@ -329,19 +339,21 @@ struct PipelineImpl final : Pipeline {
                          (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
   }
 
-  template <int XT>
+  template <int V>
   void xpass_v(Ctx &c, const XArgs &xa) {
-    const int blocks = N * N / (2 * xr<XT>());
+    constexpr int XT = xthreads<V>();
+    const int blocks = N * N / (2 * xr<XT, true>());
+    constexpr size_t lds = lds_x<XT, true>();
     if (c.cfg.has_scalar)
-      hipLaunchKernelGGL((k_xpass<T, N, 7, XT, DA>), dim3(blocks, c.nbox), dim3(XT), lds_x<XT>(), c.stream,
+      hipLaunchKernelGGL((k_xpass<T, N, 7, XT, DA>), dim3(blocks, c.nbox), dim3(XT), lds, c.stream,
                          (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
     else
-      hipLaunchKernelGGL((k_xpass<T, N, 6, XT, DA>), dim3(blocks, c.nbox), dim3(XT), lds_x<XT>(), c.stream,
+      hipLaunchKernelGGL((k_xpass<T, N, 6, XT, DA>), dim3(blocks, c.nbox), dim3(XT), lds, c.stream,
                          (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
   }
   void xpass(Ctx &c, const XArgs &xa) override {
     if constexpr (XWAVE) c.sched[3] ? xpass_wave_v<XFETCH_JIT>(c, xa) : xpass_wave_v<XFETCH_LDS>(c, xa);
-    else c.sched[3] ? xpass_v<kXThreads[1]>(c, xa) : xpass_v<kXThreads[0]>(c, xa);
+    else c.sched[3] ? xpass_v<1>(c, xa) : xpass_v<0>(c, xa);
   }
 
   void spectrum(Ctx &c, int which, double *partials, double *bins) override {   // the selected box

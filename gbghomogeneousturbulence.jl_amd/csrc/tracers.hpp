@@ -19,6 +19,16 @@ __device__ __forceinline__ int wrap_index(int i, int n) {
   return r < 0 ? r + n : r;
 }
 
+// The chunk of tracers block b takes when the blocks of a launch go to the 8 XCDs in turn (block
+// b on XCD b mod 8): the blocks of one XCD take consecutive chunks of the traversal order, so the
+// field lines that neighbouring tiles share stay in that XCD's L2.  A bijection of [0, nb).
+__device__ __forceinline__ int64_t xcd_chunk(int64_t b, int64_t nb) {
+  constexpr int64_t X = 8;
+  const int64_t per = (nb + X - 1) / X, rem = nb % X, r = b % X;
+  const int64_t start = rem == 0 || r <= rem ? r * per : r * per - (r - rem);
+  return start + b / X;
+}
+
 template <typename T>
 __device__ __forceinline__ T wrap_box(T x, T L) {
   // test/regressions/02-api-contracts-02.jl:1-6: fmod, lift negatives by L, and an equality
@@ -227,7 +237,7 @@ __device__ __forceinline__ void advect_one(int64_t p, T *__restrict__ pos, T *__
 template <typename T>
 __global__ void k_advect(T *__restrict__ pos, T *__restrict__ upos, const T *__restrict__ trail, const T *__restrict__ u,
                          const int *__restrict__ order, TracerArgs ta, AdamsArgs aa) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t t = xcd_chunk(blockIdx.x, gridDim.x) * (int64_t)blockDim.x + threadIdx.x;
   if (t >= ta.m) return;
   const int box = blockIdx.y;
   pos += box * 3 * ta.cap;
@@ -284,7 +294,7 @@ __device__ __forceinline__ void sample_one(int64_t p, const T *__restrict__ pos,
 template <typename T>
 __global__ void k_sample(const T *__restrict__ pos, T *__restrict__ vel, T *__restrict__ phis,
                          const T *__restrict__ u, const int *__restrict__ order, TracerArgs ta) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t t = xcd_chunk(blockIdx.x, gridDim.x) * (int64_t)blockDim.x + threadIdx.x;
   if (t >= ta.m) return;
   const int box = blockIdx.y;
   pos += box * 3 * ta.cap;

@@ -1,15 +1,15 @@
 // The x pass with one wavefront per row pair (or per row) and the data in registers: FP32 at
-// N = 512 (pairs) and 1024 (rows), FP64 at N = 256 (pairs) and 512 (rows).
+// N = 256 and 512 (pairs) and 1024 (rows), FP64 at N = 256 (pairs) and 512 (rows).
 //
-// A wavefront transforms H complex points, H = 512 (FP32) or 256 (FP64), S = H / 64 per lane: three
-// radix-8 or four radix-4 Stockham passes in which lane j is butterfly j of every pass: it reads
-// elements j + 64 q (q = 0..S-1) and, after the last pass, holds the outputs j + 64 q.  So the
-// inverse transforms leave lane j with the physical values at x = j + 64 q of every component,
-// the products are formed lane-locally, and they feed the forward transforms' first pass
-// directly.  Only the exchanges between passes go through LDS (a wave-private, padded buffer).
-// The per-lane twiddles of the last pass are held in registers for the whole kernel; the earlier
-// passes' come from a 64-entry table of W_64 in LDS.  Either way a component costs S complex
-// values (16 registers) per lane.
+// A wavefront transforms H complex points, H = 512 (FP32 from N = 512 on) or 256, S = H / 64 per
+// lane: three radix-8 or four radix-4 Stockham passes in which lane j is butterfly j of every
+// pass: it reads elements j + 64 q (q = 0..S-1) and, after the last pass, holds the outputs
+// j + 64 q.  So the inverse transforms leave lane j with the physical values at x = j + 64 q of
+// every component, the products are formed lane-locally, and they feed the forward transforms'
+// first pass directly.  Only the exchanges between passes go through LDS (a wave-private, padded
+// buffer).  The per-lane twiddles of the last pass are held in registers for the whole kernel;
+// the earlier passes' come from a 64-entry table of W_64 in LDS.  A component costs S complex
+// values per lane: 16 registers (8 for FP32 at N = 256).
 //
 // With H = N a wavefront takes a row pair: the Hermitian pack of two real rows (row a in the real
 // part, row b in the imaginary part) is read into the first pass: element n of the packed
@@ -42,7 +42,8 @@ __device__ __forceinline__ int xpad(int i) { return i + (i >> xpad_shift<T>()); 
 // The shape of the register-resident x pass for (T, N).
 template <typename T, int N>
 struct XWave {
-  static constexpr int H = sizeof(T) == 4 ? 512 : 256;   // the wavefront's complex transform length
+  // the wavefront's complex transform length: 512 FP32 points from N = 512 on, else 256
+  static constexpr int H = sizeof(T) == 4 && N >= 512 ? 512 : 256;
   static constexpr int S = H / 64;                       // values per lane per component
   static constexpr bool SUPPORTED = N == H || N == 2 * H;
   static constexpr bool HALF = N == 2 * H;               // one real row per wavefront (else a pair)

@@ -225,11 +225,11 @@ def test_fp32_bulk_at_512_agrees_with_the_fp64_step(g, scalar):
         assert rel(a.physical(0), b.physical(0)) < 3e-6
 
 
-@pytest.mark.parametrize("n,dtype,tol", [(512, np.float32, 1e-6), (256, np.float64, 1e-14)])
+@pytest.mark.parametrize("n,dtype,tol", [(512, np.float32, 1e-6), (256, np.float32, 1e-6), (256, np.float64, 1e-14)])
 @pytest.mark.parametrize("scalar", [True, False])
 def test_x_pass_fetch_schedules_agree(g, n, dtype, tol, scalar):
-    """The register-resident x pass (FP32 at 512: row pairs of 512 points; FP64 at 256: row pairs
-    of 256 points in radix-4 passes) reads each component's rows either from their copy staged in
+    """The register-resident x pass (FP32 at 512: row pairs of 512 points in radix-8 passes; FP32
+    and FP64 at 256: row pairs of 256 points in radix-4 passes) reads each component's rows either from their copy staged in
     LDS by direct-to-LDS loads (x schedule 0) or straight from HBM (x schedule 1).  The values are
     the same; the two builds may contract different multiply-adds, so two steps, the physical
     fields and max|u| agree to rounding (a misplaced row would be an O(1) error)."""
