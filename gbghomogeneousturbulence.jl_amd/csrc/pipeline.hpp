@@ -81,12 +81,21 @@ struct PipelineImpl final : Pipeline {
   static constexpr bool ZWIDE = N >= 1024;
   // y passes: from N = 512 on the two schedules are 512 and 1024 threads (a column tile of 64 or
   // 128 KiB leaves room for two blocks per CU, or one)
-  static constexpr int YT[2] = {N >= 512 ? 512 : kYThreads[0], N >= 512 ? 1024 : kYThreads[1]};
+  // Up to N = 128, schedule 0 of the y and z passes gives a block no more threads than the first
+  // pass of its tile has butterflies (whole wavefronts), where the wider blocks would leave most
+  // lanes idle through it; schedule 1 keeps the wider block.
+  static constexpr int fit0(int nt, int tx) {
+    const int b = (N / first_radix<N>() * tx + 63) / 64 * 64;
+    return N <= 128 && b < nt ? b : nt;
+  }
+  static constexpr int YT[2] = {fit0(N >= 512 ? 512 : kYThreads[0], TX), N >= 512 ? 1024 : kYThreads[1]};
   static constexpr int ZNT_WIDE = sizeof(T) == 8 ? 512 : 1024;
   static constexpr int ZTX = ZWIDE ? std::min(TX, ZNT_WIDE / (N / first_radix<N>())) : TX;
-  static constexpr int ZF_NT[2] = {ZWIDE ? ZNT_WIDE : ZSched<T>::zf_nt[0], ZWIDE ? ZNT_WIDE : ZSched<T>::zf_nt[1]};
+  static constexpr int ZF_NT[2] = {ZWIDE ? ZNT_WIDE : fit0(ZSched<T>::zf_nt[0], ZTX),
+                                   ZWIDE ? ZNT_WIDE : ZSched<T>::zf_nt[1]};
   static constexpr int ZI_TX[2] = {ZWIDE ? ZTX : ZSched<T>::zi_tx[0], ZWIDE ? ZTX : ZSched<T>::zi_tx[1]};
-  static constexpr int ZI_NT[2] = {ZWIDE ? ZNT_WIDE : ZSched<T>::zi_nt[0], ZWIDE ? ZNT_WIDE : ZSched<T>::zi_nt[1]};
+  static constexpr int ZI_NT[2] = {ZWIDE ? ZNT_WIDE : fit0(ZSched<T>::zi_nt[0], ZSched<T>::zi_tx[0]),
+                                   ZWIDE ? ZNT_WIDE : ZSched<T>::zi_nt[1]};
   static constexpr size_t LDS_Z = ((size_t)N * ZTX + N) * sizeof(cx<T>);   // z tile + twiddles
 
   // the z forward schedule 0 stages two velocity products in LDS when they fit beside the tile
