@@ -403,7 +403,9 @@ def time_adjoint(g, n: int = 128, steps: int = 200, every: int = 20, directions:
 
 
 def copy_calibration(H, nbytes: int = 1 << 30) -> None:
-    """One device-to-device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    """Kernels of known traffic for scaling the FETCH_SIZE / WRITE_SIZE counters: two
+    device-to-device copies (1 GiB read + 1 GiB written each), then, where torch is importable,
+    a 1 GiB fill (written only) and a sum over it (read only) as an independent cross-check."""
     H.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
     H.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
     H.hipFree.argtypes = [ctypes.c_void_p]
@@ -415,6 +417,17 @@ def copy_calibration(H, nbytes: int = 1 << 30) -> None:
     H.hipDeviceSynchronize()
     H.hipFree(a)
     H.hipFree(b)
+    try:
+        import torch
+    except ImportError:
+        return
+    x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    x.fill_(1.0)
+    total = float(x.sum())
+    torch.cuda.synchronize()
+    if abs(total - nbytes // 4) > 1e-3 * (nbytes // 4):
+        raise RuntimeError("the calibration sum is off")
+    del x
 
 
 def kernel_table(res: dict, n: int, real_bytes: int, stages: int, ssp: bool = False) -> dict:

@@ -11,22 +11,14 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "xcd.hpp"
+
 namespace hdns {
 
 __device__ __forceinline__ int wrap_index(int i, int n) {
   if ((n & (n - 1)) == 0) return i & (n - 1);
   const int r = i % n;
   return r < 0 ? r + n : r;
-}
-
-// The chunk of tracers block b takes when the blocks of a launch go to the 8 XCDs in turn (block
-// b on XCD b mod 8): the blocks of one XCD take consecutive chunks of the traversal order, so the
-// field lines that neighbouring tiles share stay in that XCD's L2.  A bijection of [0, nb).
-__device__ __forceinline__ int64_t xcd_chunk(int64_t b, int64_t nb) {
-  constexpr int64_t X = 8;
-  const int64_t per = (nb + X - 1) / X, rem = nb % X, r = b % X;
-  const int64_t start = rem == 0 || r <= rem ? r * per : r * per - (r - rem);
-  return start + b / X;
 }
 
 template <typename T>
@@ -237,6 +229,8 @@ __device__ __forceinline__ void advect_one(int64_t p, T *__restrict__ pos, T *__
 template <typename T>
 __global__ void k_advect(T *__restrict__ pos, T *__restrict__ upos, const T *__restrict__ trail, const T *__restrict__ u,
                          const int *__restrict__ order, TracerArgs ta, AdamsArgs aa) {
+  // blocks of one XCD take consecutive chunks of the traversal order: the field lines that
+  // neighbouring tiles share stay in that XCD's L2
   const int64_t t = xcd_chunk(blockIdx.x, gridDim.x) * (int64_t)blockDim.x + threadIdx.x;
   if (t >= ta.m) return;
   const int box = blockIdx.y;
@@ -294,6 +288,8 @@ __device__ __forceinline__ void sample_one(int64_t p, const T *__restrict__ pos,
 template <typename T>
 __global__ void k_sample(const T *__restrict__ pos, T *__restrict__ vel, T *__restrict__ phis,
                          const T *__restrict__ u, const int *__restrict__ order, TracerArgs ta) {
+  // blocks of one XCD take consecutive chunks of the traversal order: the field lines that
+  // neighbouring tiles share stay in that XCD's L2
   const int64_t t = xcd_chunk(blockIdx.x, gridDim.x) * (int64_t)blockDim.x + threadIdx.x;
   if (t >= ta.m) return;
   const int box = blockIdx.y;
