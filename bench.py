@@ -139,40 +139,85 @@ def time_gpu(g, n: int, dtype, steps: int, warmup: int, H, seed: int = 2026, sch
             "schedule": tuned}
 
 
+def tracer_bytes(n: int, m: int, real_bytes: int) -> dict:
+    """Compulsory HBM bytes of the tracer kernels of one Heun step, M tracers on an n^3 grid: each
+    gather kernel reads its fields once (every grid point lies in some tracer's 4^3 stencil at
+    these densities) plus each tracer's own records (order 4 B, positions, trail or samples)."""
+    w = real_bytes
+    return {"advect": 3 * n ** 3 * w + m * (4 + 3 * w + 3 * w + 3 * w),    # u; order, X, f, X written
+            "sample": 4 * n ** 3 * w + m * (4 + 3 * w + 3 * w + w),        # u, phi; order, X, u and phi written
+            # one pass: u and phi once; order, X, f read; X, the u and phi samples written
+            "advect_sample": 4 * n ** 3 * w + m * (4 + 3 * w + 3 * w + 3 * w + 3 * w + w),
+            "rebuild": m * (3 * w + 4 + 4)}                                 # X read, keys and order (sort aside)
+
+
 def time_tracers(g, n: int, m: int, dtype, steps: int, H) -> dict:
     """Tracer throughput: traversal rebuild + Heun push + closing sample per step, M tracers on an
-    n^3 field (the field step itself is excluded from this figure)."""
+    n^3 field (the field step itself is excluded from this figure), as the run loop makes them (the
+    push and the sample in one launch, hdns_advect_deposit); each phase between its own HIP events
+    on the context's stream, and the push-and-sample's rate against the HBM roofline.  A second run
+    times the push and the sample as two launches (hdns_advect, hdns_deposit_tracers)."""
     dom = g.FixedDomain(grid_points_per_axis=n, side_length=2 * math.pi, dealiasing=g.TwoThirds(), element_type=dtype)
     u = g.random_phase_velocity(n, seed=2026)
     p = np.zeros((n, n, n // 2 + 1), dtype=np.complex128)
     cond = g.FlowConditions(viscosity=g.predicted_viscosity(dom, 0.1, kmax_eta=1.5), schmidt=1.0)
-    tr = g.Tracers(dom, number_of_particles=m, seed=2028)
     dt = 1e-3
-    total = 0.0
-    with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=1.0, tracers=tr) as sim:
-        stream = ctypes.c_void_p(sim.stream())
-        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
-        H.hipEventCreate(ctypes.byref(e0))
-        H.hipEventCreate(ctypes.byref(e1))
-        t = 0.0
-        for s in range(steps + 1):
-            sim.step_fields(dt)
-            t += dt
-            H.hipEventRecord(e0, stream)
-            sim._call("hdns_rebuild_traversal")
-            sim.advect(dt)
-            sim.deposit_tracers(t)
-            H.hipEventRecord(e1, stream)
-            H.hipEventSynchronize(e1)
-            ms = ctypes.c_float()
-            H.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
-            if s > 0:
-                total += ms.value
-        H.hipEventDestroy(e0)
-        H.hipEventDestroy(e1)
-    per = total / steps
-    return {"grid": n, "tracers": m, "dtype": np.dtype(dtype).name, "ms_per_step": round(per, 4),
-            "tracer_steps_per_s": round(m / (per * 1e-3), 1)}
+    w = np.dtype(dtype).itemsize
+    ab = tracer_bytes(n, m, w)
+
+    def run(fused: bool) -> dict:
+        phases = ("rebuild", "advect_sample") if fused else ("rebuild", "advect", "sample")
+        total = dict.fromkeys(phases, 0.0)
+        tr = g.Tracers(dom, number_of_particles=m, seed=2028)
+        with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=1.0, tracers=tr) as sim:
+            stream = ctypes.c_void_p(sim.stream())
+            ev = [ctypes.c_void_p() for _ in range(len(phases) + 1)]
+            for e in ev:
+                H.hipEventCreate(ctypes.byref(e))
+            t = 0.0
+            for s in range(steps + 1):
+                sim.step_fields(dt)
+                t += dt
+                H.hipEventRecord(ev[0], stream)
+                sim._call("hdns_rebuild_traversal")
+                H.hipEventRecord(ev[1], stream)
+                if fused:
+                    sim.advect_and_deposit(dt, t)
+                else:
+                    sim.advect(dt)
+                    H.hipEventRecord(ev[2], stream)
+                    sim.deposit_tracers(t)
+                H.hipEventRecord(ev[-1], stream)
+                H.hipEventSynchronize(ev[-1])
+                if s > 0:
+                    for i, ph in enumerate(phases):
+                        ms = ctypes.c_float()
+                        H.hipEventElapsedTime(ctypes.byref(ms), ev[i], ev[i + 1])
+                        total[ph] += ms.value
+            for e in ev:
+                H.hipEventDestroy(e)
+        return {ph: total[ph] / steps for ph in phases}
+
+    per = run(True)
+    sep = run(False)
+    ms_step = sum(per.values())
+    gms = per["advect_sample"]
+    gb = ab["advect_sample"]
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic.json"
+    if tfile.exists():
+        try:
+            t = json.loads(tfile.read_text()).get(f"k_advect_sample<{'float' if w == 4 else 'double'}>@{n}x{m}")
+            traffic = t["bytes_per_launch"] if t else None
+        except (ValueError, KeyError):
+            traffic = None
+    return {"grid": n, "tracers": m, "dtype": np.dtype(dtype).name, "ms_per_step": round(ms_step, 4),
+            "ms": {ph: round(v, 4) for ph, v in per.items()},
+            "tracer_steps_per_s": round(m / (ms_step * 1e-3), 1),
+            "two_launches_ms": {ph: round(v, 4) for ph, v in sep.items()},
+            "roofline": {"bound": "hbm", "kernel": "k_advect_sample", "algorithmic_bytes": gb,
+                         "achieved": round(gb / (gms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gb / (gms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": traffic}}
 
 
 def time_ensemble(g, steps: int, H, boxes_per_size: int = 32, tracers: int = 4096) -> dict:
@@ -220,12 +265,24 @@ def time_ensemble(g, steps: int, H, boxes_per_size: int = 32, tracers: int = 409
             H.hipEventDestroy(e)
     points = sum(n ** 3 for n in sizes)
     per = total.value / steps
+    fms = field_ms / steps
+    # the field step's compulsory bytes: every box's stage kernels under the same model as the
+    # headline's kernel table, three stages of IF-RK3
+    step_bytes = 0.0
+    for n in set(sizes):
+        ab = algorithmic_bytes(n, 4, 3)
+        per_stage = sum(v for k, v in ab.items() if k != "reality_projection")
+        step_bytes += sizes.count(n) * (3 * per_stage + ab["reality_projection"])
     return {"boxes": len(sizes), "sizes": {"32": boxes_per_size, "48": boxes_per_size, "64": boxes_per_size},
             "scheme": "IFLowStorageRK3", "dtype": "f32", "graph": True, "steps": steps,
-            "ms_per_iteration": round(per, 4), "field_ms_per_step": round(field_ms / steps, 4),
-            "gridpoint_steps_per_s": round(points / (field_ms / steps * 1e-3), 1),
+            "ms_per_iteration": round(per, 4), "field_ms_per_step": round(fms, 4),
+            "gridpoint_steps_per_s": round(points / (fms * 1e-3), 1),
             "tracer_steps_per_s": round(len(sizes) * tracers / (per * 1e-3), 1),
-            "tracers_per_box": tracers}
+            "tracers_per_box": tracers,
+            "roofline": {"bound": "hbm", "what": "the field step (every box, every stage kernel)",
+                         "algorithmic_bytes": int(step_bytes), "achieved": round(step_bytes / (fms * 1e-3) / 1e9, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(step_bytes / (fms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
 def _forced_flow(g, n: int, dtype, tracers: int = 0):

@@ -77,6 +77,7 @@ _SIGS = {
     "hdns_tracers_unwrapped_upload": [c_void_p, c_int64, _D],
     "hdns_rebuild_traversal": [c_void_p],
     "hdns_advect": [c_void_p, c_double],
+    "hdns_advect_deposit": [c_void_p, c_double],
     "hdns_deposit_tracers": [c_void_p],
     "hdns_tracer_samples": [c_void_p, _D, _D],
     "hdns_tracer_trail": [c_void_p, POINTER(c_int32), POINTER(c_int32), _D],

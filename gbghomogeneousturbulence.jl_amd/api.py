@@ -662,6 +662,7 @@ def _config(domain: FixedDomain, conditions: FlowConditions, scheme, carries_sca
 
 # the kernel families the autotuner schedules, in the order of hdns_schedule's entries
 SCHEDULE_FAMILIES = ("y_passes", "z_forward", "z_inverse", "x_pass", "y_radix_plan")
+SCHEDULE_VARIANTS = 4   # the autotune report's row per family (HDNS_SCHED_VARIANTS)
 
 
 class Simulation:
@@ -911,17 +912,20 @@ class Simulation:
         return s.value or 0
 
     def autotune(self, reps: int = 3) -> dict:
-        """Time both launch schedules of every stage kernel family and keep the fastest (the state is
-        left bitwise unchanged); returns {family: (ms_variant0, ms_variant1, chosen)}."""
-        ms = np.zeros(2 * len(SCHEDULE_FAMILIES))
+        """Time every launch schedule of every stage kernel family and keep the fastest (the state is
+        left bitwise unchanged); returns {family: {"ms": [ms of each variant], "chosen": index}}."""
+        ms = np.zeros(SCHEDULE_VARIANTS * len(SCHEDULE_FAMILIES))
         chosen = np.zeros(len(SCHEDULE_FAMILIES), dtype=np.int32)
         self._call("hdns_autotune", int(reps), L.dptr(ms), chosen.ctypes.data_as(ctypes.POINTER(c_int32)))
-        return {name: (float(ms[2 * i]), float(ms[2 * i + 1]), int(chosen[i]))
-                for i, name in enumerate(SCHEDULE_FAMILIES)}
+        out = {}
+        for i, name in enumerate(SCHEDULE_FAMILIES):
+            row = ms[SCHEDULE_VARIANTS * i: SCHEDULE_VARIANTS * (i + 1)]
+            out[name] = {"ms": [round(float(t), 5) for t in row if t >= 0], "chosen": int(chosen[i])}
+        return out
 
     def schedule(self, set_to=None):
-        """Read (and optionally set) the launch schedule: one entry of 0 or 1 per kernel family
-        (y passes, z forward, z inverse, x pass, y radix plan)."""
+        """Read (and optionally set) the launch schedule: one variant index per kernel family
+        (y passes 0-1, z forward 0-1, z inverse 0-2, x pass 0-1, y radix plan 0-1)."""
         get = np.zeros(len(SCHEDULE_FAMILIES), dtype=np.int32)
         P = ctypes.POINTER(c_int32)
         s = None if set_to is None else np.ascontiguousarray(set_to, dtype=np.int32)
@@ -1076,6 +1080,22 @@ class Simulation:
         self.trail_epoch = (self.fields_epoch, self.particles_epoch)
         self.trail_time = t
 
+    def advect_and_deposit(self, dt: float, t: float) -> None:
+        """advect(dt) then deposit_tracers(t) as one launch (hdns_advect_deposit): the iteration's
+        push and closing sample, bit for bit the two calls, with both calls' checks made before
+        anything moves."""
+        if self.tracers is None:
+            raise ValueError("this run carries no tracers")
+        if self.trail_epoch != (self.fields_epoch - 1, self.particles_epoch):
+            raise RuntimeError("the fields must be exactly one step ahead of the tracer positions, and the "
+                               "trail's newest sample taken at those positions")
+        if not math.isfinite(t) or (self.trail_time is not None and t <= self.trail_time):
+            raise ValueError(f"the sample time must be finite and later than the previous sample's, got {t}")
+        self._call("hdns_advect_deposit", float(dt))
+        self.particles_epoch += 1
+        self.trail_epoch = (self.fields_epoch, self.particles_epoch)
+        self.trail_time = t
+
     def _check_trail_between_iterations(self) -> None:
         """The checked read at an iteration's start (src/Cache.jl:342-349)."""
         fe, pe = self.trail_epoch if self.trail_epoch is not None else (None, None)
@@ -1112,8 +1132,7 @@ class Simulation:
         if self.tracers is not None:
             if self.traversal_mode == "automatic":   # from the start-of-step positions (src/SimulationRun.jl:1197-1200)
                 self._call("hdns_rebuild_traversal")
-            self.advect(dt_)
-            self.deposit_tracers(t_new)
+            self.advect_and_deposit(dt_, t_new)   # advect! then deposit! (src/SimulationRun.jl:1203-1207)
         if check:
             red = None
             if not math.isfinite(self.max_speed()):

@@ -1489,8 +1489,12 @@ hdns_status hdns_rebuild_traversal(hdns_ctx *ctx) {
   return launched(ctx);
 }
 
-hdns_status hdns_advect(hdns_ctx *ctx, double dt) {
-  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+}  // extern "C"
+
+namespace {
+
+// The checks and the Adams weights of a push of dt (hdns_advect, hdns_advect_deposit).
+hdns_status push_args(hdns_ctx *ctx, double dt, AdamsArgs &aa) {
   Ctx &c = ctx->c;
   if (!(isfinite(dt) && dt > 0))
     return fail(ctx, HDNS_ERR_INVALID_ARGUMENT,
@@ -1503,12 +1507,11 @@ hdns_status hdns_advect(hdns_ctx *ctx, double dt) {
   if (c.advected)
     return fail(ctx, HDNS_ERR_STATE,
                 "the positions moved since the trail's newest sample: deposit the tracers before the next push");
-  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
   // src/ParticlePropagator.jl:349-372: the scheme's own step once the trail holds its k samples,
   // else the predictor-corrector the samples present allow
   const int k = c.cfg.particle_scheme == HDNS_HEUN ? 1 : c.cfg.particle_steps;
   const bool full = c.trail_filled >= k;
-  AdamsArgs aa{};
+  aa = AdamsArgs{};
   aa.J = full ? k : c.trail_filled;
   aa.corrected = !full || c.cfg.particle_scheme != HDNS_AB;
   double nodes[MAX_TRAIL + 1];
@@ -1526,20 +1529,13 @@ hdns_status hdns_advect(hdns_ctx *ctx, double dt) {
     aa.c[0] = beta[0];
     for (int j = 0; j < aa.J; ++j) aa.c[j + 1] = beta[j + 1] - aa.b[j];
   }
-  c.pipe->advect(c, aa);
-  c.advected = true;
-  c.pending_t = c.trail_t[c.trail_newest] + dt;
-  return launched(ctx);
+  return HDNS_OK;
 }
 
-hdns_status hdns_deposit_tracers(hdns_ctx *ctx) {
-  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
-  Ctx &c = ctx->c;
-  if (c.cfg.tracer_capacity == 0) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "this context carries no tracers");
-  if (!c.deposited) return fail(ctx, HDNS_ERR_STATE, "no deposited velocity to sample at the tracers");
-  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
-  // the ring (src/Particles.jl:454-490): a pushed set of positions takes the next slot, retiring the
-  // oldest sample of a full trail; unmoved positions refresh the newest sample in place
+// The ring's bookkeeping of a deposit (src/Particles.jl:454-490): a pushed set of positions takes
+// the next slot, retiring the oldest sample of a full trail; unmoved positions refresh the newest
+// sample in place.
+void trail_push(Ctx &c) {
   if (c.trail_filled == 0) {   // the first sample: t = 0 after an upload, the stored clock after a restore
     c.trail_newest = 0;
     c.trail_t[0] = c.pending_t;
@@ -1550,6 +1546,47 @@ hdns_status hdns_deposit_tracers(hdns_ctx *ctx) {
     c.trail_filled = std::min(c.trail_filled + 1, c.trail_cap);
   }
   c.advected = false;
+}
+
+}  // namespace
+
+extern "C" {
+
+hdns_status hdns_advect(hdns_ctx *ctx, double dt) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  Ctx &c = ctx->c;
+  AdamsArgs aa;
+  const hdns_status s = push_args(ctx, dt, aa);
+  if (s != HDNS_OK) return s;
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  c.pipe->advect(c, aa);
+  c.advected = true;
+  c.pending_t = c.trail_t[c.trail_newest] + dt;
+  return launched(ctx);
+}
+
+hdns_status hdns_advect_deposit(hdns_ctx *ctx, double dt) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  Ctx &c = ctx->c;
+  AdamsArgs aa;
+  const hdns_status s = push_args(ctx, dt, aa);
+  if (s != HDNS_OK) return s;
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  c.advected = true;
+  c.pending_t = c.trail_t[c.trail_newest] + dt;
+  // the slots the push reads were fixed in aa above; the ring then takes the new sample's slot
+  trail_push(c);
+  c.pipe->advect_sample(c, aa);
+  return launched(ctx);
+}
+
+hdns_status hdns_deposit_tracers(hdns_ctx *ctx) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  Ctx &c = ctx->c;
+  if (c.cfg.tracer_capacity == 0) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "this context carries no tracers");
+  if (!c.deposited) return fail(ctx, HDNS_ERR_STATE, "no deposited velocity to sample at the tracers");
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  trail_push(c);
   c.pipe->sample(c);
   return launched(ctx);
 }
@@ -1729,6 +1766,8 @@ hdns_status hdns_frame_tables(hdns_ctx *ctx, int32_t *mx, int32_t *my, int32_t *
   return HDNS_OK;
 }
 
+static_assert(Ctx::NFAM == HDNS_SCHED_FAMILIES && Ctx::MAXVAR == HDNS_SCHED_VARIANTS, "the header's schedule shape");
+
 hdns_status hdns_autotune(hdns_ctx *ctx, int32_t reps, double *ms, int32_t *chosen) {
   if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
   Ctx &c = ctx->c;
@@ -1757,12 +1796,14 @@ hdns_status hdns_autotune(hdns_ctx *ctx, int32_t reps, double *ms, int32_t *chos
   xa.max_bits = c.maxbits;
   xa.box_phys = c.box_phys;
   xa.box_physw = c.box_physw;
-  constexpr int NF = Ctx::NFAM;
+  constexpr int NF = Ctx::NFAM, MV = Ctx::MAXVAR;
   int saved[NF];
   for (int f = 0; f < NF; ++f) saved[f] = c.sched[f];
-  double best[NF][2] = {};
+  double best[NF][MV];
+  for (int f = 0; f < NF; ++f)
+    for (int v = 0; v < MV; ++v) best[f][v] = -1.0;   // no such variant
   for (int fam = 0; fam < NF; ++fam) {
-    for (int v = 0; v < 2; ++v) {
+    for (int v = 0; v < Ctx::NVAR[fam]; ++v) {
       for (int f = 0; f < NF; ++f) c.sched[f] = saved[f];
       c.sched[fam] = v;
       double total = 0.0;
@@ -1789,10 +1830,15 @@ hdns_status hdns_autotune(hdns_ctx *ctx, int32_t reps, double *ms, int32_t *chos
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  for (int fam = 0; fam < NF; ++fam) c.sched[fam] = best[fam][1] < best[fam][0] ? 1 : 0;
+  for (int fam = 0; fam < NF; ++fam) {
+    int pick = 0;
+    for (int v = 1; v < Ctx::NVAR[fam]; ++v)
+      if (best[fam][v] < best[fam][pick]) pick = v;
+    c.sched[fam] = pick;
+  }
   c.prof = prof;
   if (ms)
-    for (int i = 0; i < 2 * NF; ++i) ms[i] = best[i / 2][i % 2];
+    for (int i = 0; i < MV * NF; ++i) ms[i] = best[i / MV][i % MV];
   if (chosen)
     for (int i = 0; i < NF; ++i) chosen[i] = c.sched[i];
   hdns_status s = launched(ctx);
@@ -1806,7 +1852,10 @@ hdns_status hdns_schedule(hdns_ctx *ctx, const int32_t *set, int32_t *get) {
   Ctx &c = ctx->c;
   if (set) {
     for (int i = 0; i < Ctx::NFAM; ++i)
-      if (set[i] != 0 && set[i] != 1) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "schedule entries are 0 or 1");
+      if (set[i] < 0 || set[i] >= Ctx::NVAR[i])
+        return fail(ctx, HDNS_ERR_INVALID_ARGUMENT,
+                    "schedule entry " + std::to_string(i) + " is " + std::to_string(set[i]) + ": the family has " +
+                        std::to_string(Ctx::NVAR[i]) + " variants");
     for (int i = 0; i < Ctx::NFAM; ++i) c.sched[i] = set[i];
     drop_graph(c);
   }
@@ -2304,8 +2353,7 @@ hdns_status hdns_ensemble_advance_tracers(hdns_ensemble *ens, double dt) {
   if (!ens) return HDNS_ERR_INVALID_ARGUMENT;
   for (hdns_ctx *c : ens->classes) {
     hdns_status s = hdns_rebuild_traversal(c);
-    if (s == HDNS_OK) s = hdns_advect(c, dt);
-    if (s == HDNS_OK) s = hdns_deposit_tracers(c);
+    if (s == HDNS_OK) s = hdns_advect_deposit(c, dt);
     if (s != HDNS_OK) return efail(ens, s, c->c.err);
   }
   return HDNS_OK;

@@ -44,6 +44,8 @@ struct Pipeline {
   // out [3][m] of the selected box <- sum_j w_j (trail sample j), in the element type
   virtual void trail_combine(Ctx &c, const TrailCombine &tc, void *out) = 0;
   virtual void sample(Ctx &c) = 0;
+  // advect then sample in one pass, the sample into the trail's newest slot (already advanced)
+  virtual void advect_sample(Ctx &c, const AdamsArgs &aa) = 0;
   virtual void cell_keys(Ctx &c, int64_t *keys) = 0;
   virtual hipError_t tile_sort(Ctx &c) = 0;   // the traversal order of every box
   // the adjoint step (one box): J's adjoint from the current state, the transposed closing
@@ -86,10 +88,12 @@ struct Ctx {
   size_t comp = 0;          // complex elements of one compact component
   size_t buf_a = 0, buf_b = 0;
   int stages = 0;
-  // launch schedule of the y passes, the z forward pass, the z inverse pass and the x pass
-  // (index into the two variants of each), set by hdns_autotune; the defaults are the fastest
-  // measured at 512^3 on MI355X
+  // launch schedule of the y passes, the z forward pass, the z inverse pass, the x pass and the y
+  // radix plan (index into the NVAR[f] variants of each), set by hdns_autotune; the defaults are
+  // the fastest measured at 512^3 on MI355X
   static constexpr int NFAM = 5;   // y threads, z forward, z inverse, x pass, y radix plan
+  static constexpr int MAXVAR = 4;   // the widest family's variants (the autotune report's row)
+  static constexpr int NVAR[NFAM] = {2, 2, 3, 2, 2};
   int sched[NFAM] = {1, 0, 0, 0, 0};
   BandSpec band{};
   BandSpec sband{};   // the scalar forcing's band

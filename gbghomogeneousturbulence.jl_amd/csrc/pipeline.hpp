@@ -18,13 +18,15 @@ namespace hdns {
 template <typename T>
 constexpr int tile_width_of() { return sizeof(T) == 4 ? 16 : 8; }
 
-// Two launch schedules per stage kernel family, chosen per context by hdns_autotune:
+// Launch schedules per stage kernel family (two, or three for the z inverse), chosen per context by
+// hdns_autotune:
 //   y passes: 256 or 512 threads per block (512 or 1024 from N = 512 on);
 //   z forward: (0) the scalar half and the velocity half as two kernels on 64*TX threads, one
 //              last-pass butterfly per thread at N = 512, the first two transformed velocity
 //              products staged in LDS where they fit (N <= 512); or (1) one fused kernel on 512;
 //   z inverse: a TX-wide tile on 64*TX threads (one first-pass butterfly per thread at N = 512)
-//              or on 32*TX threads (two);
+//              or on 32*TX threads (two), or (2) schedule 0's tile storing through LDS 16 bytes
+//              per lane;
 //   x pass: 256 or 512 threads per block, or for the register-resident pass (FP32 at N = 256,
 //           512 and 1024, FP64 at N = 256 and 512) the next component staged in LDS by
 //           direct-to-LDS loads, or each component loaded into registers just before its
@@ -125,6 +127,10 @@ struct PipelineImpl final : Pipeline {
     constexpr size_t ZL = ((size_t)N * ZT + N) * sizeof(cx<T>);
     set_lds((const void *)k_zi<T, N, ZT, ZN, true, DA>, ZL);
     set_lds((const void *)k_zi<T, N, ZT, ZN, false, DA>, ZL);
+    if constexpr (V == 0) {   // the z inverse's third schedule: schedule 0's tile with 16-byte stores
+      set_lds((const void *)k_zi<T, N, ZT, ZN, true, DA, false, true>, zi_lds_bytes<N, ZT, T>(true));
+      set_lds((const void *)k_zi<T, N, ZT, ZN, false, DA, false, true>, zi_lds_bytes<N, ZT, T>(true));
+    }
     if constexpr (DA == DEALIAS_PHASE_SHIFT) {
       set_lds((const void *)k_zi<T, N, ZT, ZN, true, DA, true>, ZL);
       set_lds((const void *)k_zi<T, N, ZT, ZN, false, DA, true>, ZL);
@@ -254,7 +260,7 @@ struct PipelineImpl final : Pipeline {
 
   int tiles(const Ctx &c) const { return c.geo.kxp / TX; }
   int tile_width() const override { return TX; }
-  int z_blocks(const Ctx &c) const override { return c.geo.kxp / ZI_TX[c.sched[2]] * K; }
+  int z_blocks(const Ctx &c) const override { return c.geo.kxp / ZI_TX[c.sched[2] == 1 ? 1 : 0] * K; }
 
   template <int NTY, int PLAN>
   void yfwd_v(Ctx &c) {
@@ -307,19 +313,24 @@ struct PipelineImpl final : Pipeline {
                        (cx<T> *)c.ph, c.geo, c.cfg.has_scalar ? 4 : 3);
   }
 
-  template <int V>
+  // z inverse schedule V (0, 1) on its tile; WIDE: schedule 0's tile with 16-byte stores (schedule 2)
+  template <int V, bool WIDE = false>
   void zi_v(Ctx &c) {
     constexpr int ZT = ZI_TX[V], ZN = ZI_NT[V];
-    constexpr size_t ZL = ((size_t)N * ZT + N) * sizeof(cx<T>);
+    constexpr size_t ZL = zi_lds_bytes<N, ZT, T>(WIDE);
     const dim3 grid(c.geo.kxp / ZT, K, c.nbox);
     if (c.cfg.has_scalar)
-      hipLaunchKernelGGL((k_zi<T, N, ZT, ZN, true, DA>), grid, dim3(ZN), ZL, c.stream, (const cx<T> *)c.uh,
+      hipLaunchKernelGGL((k_zi<T, N, ZT, ZN, true, DA, false, WIDE>), grid, dim3(ZN), ZL, c.stream, (const cx<T> *)c.uh,
                          (const cx<T> *)c.ph, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, c.band, c.partials);
     else
-      hipLaunchKernelGGL((k_zi<T, N, ZT, ZN, false, DA>), grid, dim3(ZN), ZL, c.stream, (const cx<T> *)c.uh,
-                         (const cx<T> *)c.ph, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, c.band, c.partials);
+      hipLaunchKernelGGL((k_zi<T, N, ZT, ZN, false, DA, false, WIDE>), grid, dim3(ZN), ZL, c.stream,
+                         (const cx<T> *)c.uh, (const cx<T> *)c.ph, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw,
+                         c.band, c.partials);
   }
-  void zi(Ctx &c) override { c.sched[2] ? zi_v<1>(c) : zi_v<0>(c); }
+  void zi(Ctx &c) override {
+    if (c.sched[2] == 2) zi_v<0, true>(c);
+    else c.sched[2] ? zi_v<1>(c) : zi_v<0>(c);
+  }
 
   template <int NTY, int PLAN>
   void yinv_v(Ctx &c) {
@@ -397,6 +408,13 @@ struct PipelineImpl final : Pipeline {
     hipLaunchKernelGGL((k_sample<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const T *)c.pos, slot,
                        c.cfg.has_scalar ? (T *)c.phis : nullptr, (const T *)c.phys, c.order_valid ? c.order : nullptr,
                        targs(c));
+  }
+  void advect_sample(Ctx &c, const AdamsArgs &aa) override {
+    if (c.m == 0) return;
+    T *slot = (T *)c.vel + (size_t)c.trail_newest * 3 * c.cfg.tracer_capacity;
+    hipLaunchKernelGGL((k_advect_sample<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (T *)c.pos, (T *)c.upos,
+                       (const T *)c.vel, slot, c.cfg.has_scalar ? (T *)c.phis : nullptr, (const T *)c.phys,
+                       c.order_valid ? c.order : nullptr, targs(c), aa);
   }
   void cell_keys(Ctx &c, int64_t *keys) override {   // the selected box
     if (c.m == 0) return;

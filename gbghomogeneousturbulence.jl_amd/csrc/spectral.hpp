@@ -433,7 +433,16 @@ constexpr int NRED = 5;
 
 // SHIFT (PhaseShift's displaced evaluation, src/Generator.jl:505-513): the state is first
 // translated by half a cell along every axis, so Q holds u, w and phi sampled on the displaced grid.
-template <typename T, int N, int TX, int NT, bool SCALAR, int DA = DEALIAS_TWO_THIRDS, bool SHIFT = false>
+// WIDE: the last pass's outputs go to a second LDS tile, and the finished tile is written to Q 16
+// bytes per lane (two FP32 values of a row), half the store instructions of the 8-byte stores
+// straight from the last pass.
+template <int N, int TX, typename T>
+constexpr size_t zi_lds_bytes(bool wide) {
+  return ((size_t)N * TX + N + (wide ? (size_t)N * TX : 0)) * sizeof(cx<T>);
+}
+
+template <typename T, int N, int TX, int NT, bool SCALAR, int DA = DEALIAS_TWO_THIRDS, bool SHIFT = false,
+          bool WIDE = false>
 __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const cx<T> *__restrict__ ph,
                                            cx<T> *__restrict__ q, Geo g, const cx<double> *__restrict__ tw,
                                            BandSpec band, double *__restrict__ partials) {
@@ -516,6 +525,7 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
   // buffer stores need every offset of a Q component below 2^31 bytes (kxp is at most M+1
   // rounded up to 16); beyond that (FP64 at 1024^3) the stores take 64-bit addresses
   constexpr bool BUFQ = (size_t)N * K * ((M + 16) / 16 * 16) * sizeof(cx<T>) < 0x7fff0000u;
+  cx<T> *obuf = twl + N;   // WIDE: the transformed tile [n][TX]
 #pragma unroll
   for (int c = 0; c < NCIN; ++c) {
     cx<T> *qc = q + (size_t)c * pencil + (size_t)ky_idx * g.kxp + kx0;
@@ -540,9 +550,28 @@ __global__ __launch_bounds__(NT) void k_zi(const cx<T> *__restrict__ uh, const c
           return scale(v, inv);
         },
         [&](int, int, int n, int col, cx<T> v) {
-          if constexpr (BUFQ) dst.store(v, (uint32_t)(n * (int)zstride + col), 0u);
+          if constexpr (WIDE) obuf[n * TX + col] = v;
+          else if constexpr (BUFQ) dst.store(v, (uint32_t)(n * (int)zstride + col), 0u);
           else qc[(size_t)n * zstride + col] = v;
         });
+    if constexpr (WIDE) {
+      // the last pass ended with a barrier: the tile is complete.  Piece i of a row is its 16
+      // bytes at column V i; the next component's last pass writes obuf only after the barriers
+      // of its first and middle passes, by which time these reads have returned.
+      constexpr int V = 16 / (int)sizeof(cx<T>), PR = TX / V, PIECES = N * PR;
+      static_assert(TX % V == 0, "a tile row holds whole 16-byte pieces");
+      using P16 = decltype(__builtin_amdgcn_raw_buffer_load_b128(dst.r, 0u, 0u, 0));
+#pragma unroll
+      for (int i = threadIdx.x; i < PIECES; i += NT) {
+        const int n = i / PR, col = (i % PR) * V;
+        const P16 v = *reinterpret_cast<const P16 *>(obuf + n * TX + col);
+        if constexpr (BUFQ)
+          __builtin_amdgcn_raw_buffer_store_b128(v, dst.r, (uint32_t)(n * (int)zstride + col) * (uint32_t)sizeof(cx<T>),
+                                                 0u, 0);
+        else
+          *reinterpret_cast<P16 *>(qc + (size_t)n * zstride + col) = v;
+      }
+    }
   }
 
   // deterministic block reduction: wave shuffles, then the waves in order

@@ -163,11 +163,12 @@ struct AdamsArgs {
   double c[MAX_TRAIL + 1];
 };
 
-// One tracer's Adams step; g3(stencil, wx, wy, wz, out) gathers u at a point.
+// One tracer's Adams step; g3(stencil, wx, wy, wz, out) gathers u at a point.  The new (wrapped)
+// position, as stored, is also left in xn.
 template <typename T, typename G3>
 __device__ __forceinline__ void advect_one(int64_t p, T *__restrict__ pos, T *__restrict__ upos,
                                            const T *__restrict__ trail, const TracerArgs &ta, const AdamsArgs &aa,
-                                           G3 g3) {
+                                           G3 g3, T (&xn)[3]) {
   const T L = T(ta.side_length), inv_dx = T(ta.n / ta.side_length);
   T F[MAX_TRAIL][3], acc[3], Xs[3], ua[3] = {T(0), T(0), T(0)};
 #pragma unroll
@@ -191,7 +192,7 @@ __device__ __forceinline__ void advect_one(int64_t p, T *__restrict__ pos, T *__
   for (int d = 0; d < 3; ++d) Xs[d] = wrap_box(acc[d], L);
   if (!aa.corrected) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) pos[d * ta.m + p] = Xs[d];
+    for (int d = 0; d < 3; ++d) pos[d * ta.m + p] = xn[d] = Xs[d];
     if (upos) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) upos[d * ta.m + p] = ua[d];
@@ -218,7 +219,7 @@ __device__ __forceinline__ void advect_one(int64_t p, T *__restrict__ pos, T *__
     }
   }
 #pragma unroll
-  for (int d = 0; d < 3; ++d) pos[d * ta.m + p] = wrap_box(acc[d], L);
+  for (int d = 0; d < 3; ++d) pos[d * ta.m + p] = xn[d] = wrap_box(acc[d], L);
   if (upos) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) upos[d * ta.m + p] = ua[d];
@@ -241,9 +242,11 @@ __global__ void k_advect(T *__restrict__ pos, T *__restrict__ upos, const T *__r
   if (order) order += box * ta.cap;
   const int64_t p = order ? order[t] : t;
   const size_t vol = (size_t)ta.n * ta.n * ta.n;
-  advect_one(p, pos, upos, trail, ta, aa, [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out) {
-    gather<T, 3>(u, vol, ta.n, s, wx, wy, wz, out);
-  });
+  T xn[3];
+  advect_one(
+      p, pos, upos, trail, ta, aa,
+      [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out) { gather<T, 3>(u, vol, ta.n, s, wx, wy, wz, out); },
+      xn);
 }
 
 // sum_j w_j f_j over n trail samples of the selected box, added left to right (time_derivative!,
@@ -270,11 +273,11 @@ __global__ void k_trail_combine(const T *__restrict__ trail, TracerArgs ta, Trai
 // the trail slot `vel` points at (inside box 0's trail; box b's is b * vstride further).  The
 // physical fields are [4][N^3] per box (u, then phi), so phi is u's fourth component.
 template <typename T, typename G>
-__device__ __forceinline__ void sample_one(int64_t p, const T *__restrict__ pos, T *__restrict__ vel,
-                                           T *__restrict__ phis, const TracerArgs &ta, G g) {
+__device__ __forceinline__ void sample_at(int64_t p, T x, T y, T z, T *__restrict__ vel, T *__restrict__ phis,
+                                          const TracerArgs &ta, G g) {
   const T inv_dx = T(ta.n / ta.side_length);
   T wx[4], wy[4], wz[4], v[3];
-  const Stencil s = stencil_of(pos[p], pos[ta.m + p], pos[2 * ta.m + p], inv_dx, ta.n, wx, wy, wz);
+  const Stencil s = stencil_of(x, y, z, inv_dx, ta.n, wx, wy, wz);
   g(s, wx, wy, wz, v, 0);
 #pragma unroll
   for (int d = 0; d < 3; ++d) vel[d * ta.m + p] = v[d];
@@ -283,6 +286,12 @@ __device__ __forceinline__ void sample_one(int64_t p, const T *__restrict__ pos,
     g(s, wx, wy, wz, q, 3);
     phis[p] = q[0];
   }
+}
+
+template <typename T, typename G>
+__device__ __forceinline__ void sample_one(int64_t p, const T *__restrict__ pos, T *__restrict__ vel,
+                                           T *__restrict__ phis, const TracerArgs &ta, G g) {
+  sample_at(p, pos[p], pos[ta.m + p], pos[2 * ta.m + p], vel, phis, ta, g);
 }
 
 template <typename T>
@@ -304,6 +313,41 @@ __global__ void k_sample(const T *__restrict__ pos, T *__restrict__ vel, T *__re
     if (c0 == 0) gather<T, 3>(u, vol, ta.n, s, wx, wy, wz, out);
     else gather<T, 1>(u + c0 * vol, vol, ta.n, s, wx, wy, wz, out);
   });
+}
+
+// The push and the closing sample in one pass (an iteration's advect! then deposit!, both on the
+// new field): the sample is taken at the new position the push leaves in registers, whose 4^3
+// stencil mostly overlaps the one the corrector just gathered, so its loads find the lines in
+// cache instead of sweeping the field a second time.  Each value is the one k_advect then k_sample
+// would produce, bit for bit.  The sample goes into the trail slot `vel` points at; when that slot
+// is also one the push reads (a full AB(k) ring), each thread reads its tracer's samples there
+// before it writes the new one, and no other thread touches that tracer's entries.
+template <typename T>
+__global__ void k_advect_sample(T *__restrict__ pos, T *__restrict__ upos, const T *trail, T *vel,
+                                T *__restrict__ phis, const T *__restrict__ u, const int *__restrict__ order,
+                                TracerArgs ta, AdamsArgs aa) {
+  const int64_t t = xcd_chunk(blockIdx.x, gridDim.x) * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= ta.m) return;
+  const int box = blockIdx.y;
+  pos += box * 3 * ta.cap;
+  if (upos) upos += box * 3 * ta.cap;
+  trail += box * ta.vstride;
+  vel += box * ta.vstride;
+  if (phis) phis += box * ta.cap;
+  u += box * ta.box_phys;
+  if (order) order += box * ta.cap;
+  const int64_t p = order ? order[t] : t;
+  const size_t vol = (size_t)ta.n * ta.n * ta.n;
+  T xn[3];
+  advect_one(
+      p, pos, upos, trail, ta, aa,
+      [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out) { gather<T, 3>(u, vol, ta.n, s, wx, wy, wz, out); },
+      xn);
+  sample_at(p, xn[0], xn[1], xn[2], vel, phis, ta,
+            [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out, int c0) {
+              if (c0 == 0) gather<T, 3>(u, vol, ta.n, s, wx, wy, wz, out);
+              else gather<T, 1>(u + c0 * vol, vol, ta.n, s, wx, wy, wz, out);
+            });
 }
 
 template <typename T>

@@ -194,6 +194,11 @@ hdns_status hdns_advect(hdns_ctx *ctx, double dt);
  * After an advect the sample is new, at the newest sample's time plus that step's dt (the first
  * sample after an upload is at t = 0); otherwise it replaces the newest sample. */
 hdns_status hdns_deposit_tracers(hdns_ctx *ctx);
+/* hdns_advect(dt) then hdns_deposit_tracers() -- an iteration's advect! and particle deposit!
+ * (src/SimulationRun.jl:1203-1207) -- as one launch: the sample is taken at the new positions while
+ * the push holds them, so its gathers find the field lines the corrector just read.  Results, trail
+ * and times are bit for bit those of the two calls. */
+hdns_status hdns_advect_deposit(hdns_ctx *ctx, double dt);
 /* The run's velocity trail (src/Particles.jl:404-526): *filled samples, their times newest first
  * (times[capacity], capacity = k + 1 for ABM(k), k for AB(k), 2 for Heun). */
 hdns_status hdns_tracer_trail(hdns_ctx *ctx, int32_t *filled, int32_t *capacity, double *times);
@@ -243,19 +248,22 @@ hdns_status hdns_scheme_coefficients(int32_t scheme, int32_t precision, double *
  * 4-point Gauss-Legendre (src/ParticlePropagator.jl:261-315).  Needs no device. */
 hdns_status hdns_heun_weights(double h, double *out3);
 
-/* Autotuner: time both launch schedules of each kernel family -- the y passes (256/512 threads,
+/* Autotuner: time every launch schedule of each kernel family -- the y passes (256/512 threads,
  * 512/1024 from N = 512 on), the z forward pass (split scalar/velocity kernels with LDS-staged
- * products, or one fused kernel), the z inverse pass (full or half-width tiles), the x pass
+ * products, or one fused kernel), the z inverse pass (full or half-width tiles, or full tiles
+ * written back 16 bytes per lane through LDS), the x pass
  * (256/512 threads, or with/without prefetch) and the y passes' radix plan (the default plan, or
- * one led by a radix-16 pass) -- on this
- * context's own buffers, keep the fastest of each, and redo the deposit.  The timed sequence
- * leaves the state bitwise unchanged.  No counterpart in the reference (whose field library has
- * no launch shapes); it tunes the kernels behind hdns_step.  ms[10] (optional) receives the average
- * ms of (y: v0, v1, z forward: v0, v1, z inverse: v0, v1, x: v0, v1, y radix plan: v0, v1),
- * chosen[5] (optional) the picks. */
+ * one led by a radix-16 pass) -- on this context's own buffers, keep the fastest of each, and redo
+ * the deposit.  The timed sequence leaves the state bitwise
+ * unchanged.  No counterpart in the reference (whose field library has no launch shapes); it tunes
+ * the kernels behind hdns_step.  ms[HDNS_SCHED_FAMILIES * HDNS_SCHED_VARIANTS] (optional) receives
+ * the average ms of each family's variants, row by row in the family order below (-1 where a
+ * family has fewer variants); chosen[HDNS_SCHED_FAMILIES] (optional) the picks. */
+#define HDNS_SCHED_FAMILIES 5
+#define HDNS_SCHED_VARIANTS 4
 hdns_status hdns_autotune(hdns_ctx *ctx, int32_t reps, double *ms, int32_t *chosen);
-/* Set (set != NULL) and/or read (get != NULL) the schedule: five entries of 0 or 1, in the order
- * y passes, z forward, z inverse, x pass, y radix plan. */
+/* Set (set != NULL) and/or read (get != NULL) the schedule: five entries, in the order y passes
+ * (0-1), z forward (0-1), z inverse (0-2), x pass (0-1), y radix plan (0-1). */
 hdns_status hdns_schedule(hdns_ctx *ctx, const int32_t *set, int32_t *get);
 
 /* Per-kernel timing with HIP events on the context's stream, for the roofline report.  Kinds:

@@ -298,6 +298,30 @@ def test_autotune_leaves_the_state_bitwise_unchanged_and_schedules_agree(g):
         assert rel(a.scalar_spectral, b.scalar_spectral) < 1e-6
 
 
+@pytest.mark.parametrize("n,dtype", [(16, np.float64), (48, np.float64), (64, np.float32), (96, np.float64),
+                                     (128, np.float32), (256, np.float32)])
+def test_z_inverse_wide_stores_are_bitwise_the_plain_stores(g, n, dtype):
+    """The z inverse's third schedule writes each transformed tile back through LDS, 16 bytes per
+    lane: the same values as schedule 0, so one forced step with the scalar (and the deposit it
+    ends with) is bitwise schedule 0's; and it matches the oracle."""
+    sim, u, p, oc = make_sim(g, n, dtype, forcing=0.1)
+    ref, *_ = make_sim(g, n, dtype, forcing=0.1)
+    with sim, ref:
+        sim.schedule((1, 0, 2, 0, 0))
+        ref.schedule((1, 0, 0, 0, 0))
+        for s in (sim, ref):
+            s._call("hdns_deposit")
+            s.advance(2e-3)
+        a, b = sim.state(), ref.state()
+        assert np.array_equal(a.velocity_spectral, b.velocity_spectral)
+        assert np.array_equal(a.scalar_spectral, b.scalar_spectral)
+        assert sim.reductions() == ref.reductions()
+        if dtype == np.float64:
+            uo, po, _ = dns.step(dns.Domain(n), oc, "rk4", u, p, 2e-3)
+            assert rel(a.velocity_spectral, uo) < 1e-12
+            assert rel(a.scalar_spectral, po) < 1e-12
+
+
 @pytest.mark.parametrize("n", [16, 32, 48, 64, 96, 128])
 def test_y_radix_plan_led_by_radix_16_matches_the_oracle(g, n):
     """The y passes' second radix plan (16.2, 16.4, 16.3, 16.8, 16.2.3 ..., 2.8 at 16) in one FP64
