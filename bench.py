@@ -165,11 +165,12 @@ def time_tracers(g, n: int, m: int, dtype, steps: int, H) -> dict:
     w = np.dtype(dtype).itemsize
     ab = tracer_bytes(n, m, w)
 
-    def run(fused: bool) -> dict:
+    def run(fused: bool, sched: int = -1) -> dict:
         phases = ("rebuild", "advect_sample") if fused else ("rebuild", "advect", "sample")
         total = dict.fromkeys(phases, 0.0)
         tr = g.Tracers(dom, number_of_particles=m, seed=2028)
         with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(u, p), mean_gradient=1.0, tracers=tr) as sim:
+            total["gather_launch"] = sim.tracer_schedule(sched)
             stream = ctypes.c_void_p(sim.stream())
             ev = [ctypes.c_void_p() for _ in range(len(phases) + 1)]
             for e in ev:
@@ -196,10 +197,16 @@ def time_tracers(g, n: int, m: int, dtype, steps: int, H) -> dict:
                         total[ph] += ms.value
             for e in ev:
                 H.hipEventDestroy(e)
-        return {ph: total[ph] / steps for ph in phases}
+        out = {ph: total[ph] / steps for ph in phases}
+        out["gather_launch"] = total["gather_launch"]
+        return out
 
     per = run(True)
+    launch = per.pop("gather_launch")
+    other = run(True, 1 - launch)
+    other.pop("gather_launch")
     sep = run(False)
+    sep.pop("gather_launch")
     ms_step = sum(per.values())
     gms = per["advect_sample"]
     gb = ab["advect_sample"]
@@ -215,6 +222,8 @@ def time_tracers(g, n: int, m: int, dtype, steps: int, H) -> dict:
             "ms": {ph: round(v, 4) for ph, v in per.items()},
             "tracer_steps_per_s": round(m / (ms_step * 1e-3), 1),
             "two_launches_ms": {ph: round(v, 4) for ph, v in sep.items()},
+            "gather_launch": {"in_force": "capped" if launch else "uncapped",
+                              "other_launch_ms": round(other["advect_sample"], 4)},
             "roofline": {"bound": "hbm", "kernel": "k_advect_sample", "algorithmic_bytes": gb,
                          "achieved": round(gb / (gms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(gb / (gms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": traffic}}

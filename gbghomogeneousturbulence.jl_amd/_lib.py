@@ -78,6 +78,7 @@ _SIGS = {
     "hdns_rebuild_traversal": [c_void_p],
     "hdns_advect": [c_void_p, c_double],
     "hdns_advect_deposit": [c_void_p, c_double],
+    "hdns_tracer_schedule": [c_void_p, c_int32, POINTER(c_int32)],
     "hdns_deposit_tracers": [c_void_p],
     "hdns_tracer_samples": [c_void_p, _D, _D],
     "hdns_tracer_trail": [c_void_p, POINTER(c_int32), POINTER(c_int32), _D],

@@ -925,7 +925,7 @@ class Simulation:
 
     def schedule(self, set_to=None):
         """Read (and optionally set) the launch schedule: one variant index per kernel family
-        (y passes 0-1, z forward 0-1, z inverse 0-2, x pass 0-1, y radix plan 0-1)."""
+        (y passes 0-1, z forward 0-2, z inverse 0-2, x pass 0-1, y radix plan 0-1)."""
         get = np.zeros(len(SCHEDULE_FAMILIES), dtype=np.int32)
         P = ctypes.POINTER(c_int32)
         s = None if set_to is None else np.ascontiguousarray(set_to, dtype=np.int32)
@@ -933,6 +933,13 @@ class Simulation:
             raise ValueError(f"a schedule has {len(SCHEDULE_FAMILIES)} entries, one per kernel family")
         self._call("hdns_schedule", s.ctypes.data_as(P) if s is not None else None, get.ctypes.data_as(P))
         return tuple(int(v) for v in get)
+
+    def tracer_schedule(self, set_to: int | None = None) -> int:
+        """Read (and optionally set) the tracer gathers' launch: 1 capped (128-thread blocks, four
+        per CU), 0 uncapped, -1 by the tracer count; returns the launch in force."""
+        get = c_int32()
+        self._call("hdns_tracer_schedule", -2 if set_to is None else int(set_to), ctypes.byref(get))
+        return get.value
 
     def profile(self, on: bool) -> None:
         """Time every kernel launch with HIP events on the context's stream."""

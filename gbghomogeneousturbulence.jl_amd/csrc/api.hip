@@ -1864,6 +1864,15 @@ hdns_status hdns_schedule(hdns_ctx *ctx, const int32_t *set, int32_t *get) {
   return HDNS_OK;
 }
 
+hdns_status hdns_tracer_schedule(hdns_ctx *ctx, int32_t set, int32_t *get) {
+  if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
+  Ctx &c = ctx->c;
+  if (set < -2 || set > 1) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "the tracer schedule is -1 (by count), 0 or 1");
+  if (set != -2) c.gather_sched = set;
+  if (get) *get = tracer_gather_capped(c) ? 1 : 0;
+  return HDNS_OK;
+}
+
 hdns_status hdns_profile_enable(hdns_ctx *ctx, int32_t on) {
   if (!ctx) return HDNS_ERR_INVALID_ARGUMENT;
   Ctx &c = ctx->c;

@@ -93,7 +93,7 @@ struct Ctx {
   // the fastest measured at 512^3 on MI355X
   static constexpr int NFAM = 5;   // y threads, z forward, z inverse, x pass, y radix plan
   static constexpr int MAXVAR = 4;   // the widest family's variants (the autotune report's row)
-  static constexpr int NVAR[NFAM] = {2, 2, 3, 2, 2};
+  static constexpr int NVAR[NFAM] = {2, 3, 3, 2, 2};
   int sched[NFAM] = {1, 0, 0, 0, 0};
   BandSpec band{};
   BandSpec sband{};   // the scalar forcing's band
@@ -124,6 +124,10 @@ struct Ctx {
   int sort_key_bits = 1;       // bits of the largest key, nbox * ntiles - 1
   int ntiles = 0;
   bool order_valid = false;
+  // the gathering tracer kernels' launch schedule: 1 capped (128-thread blocks, four per CU), 0
+  // uncapped, -1 by the tracer count (capped from GATHER_CAP_MIN tracers per launch on)
+  int gather_sched = -1;
+  static constexpr int64_t GATHER_CAP_MIN = 1 << 16;
   // the velocity trail at the tracers (src/Particles.jl:404-526): a ring of trail_cap slots on the
   // device, [box][slot][3][cap]; the times and the ring's bookkeeping on the host
   int trail_cap = 2;           // k + 1 for ABM(k) (Heun: 2), k for AB(k)
@@ -150,5 +154,9 @@ struct Ctx {
   std::unique_ptr<Pipeline> pipe;
   std::string err;
 };
+
+inline bool tracer_gather_capped(const Ctx &c) {
+  return c.gather_sched == 1 || (c.gather_sched < 0 && c.m * (int64_t)c.nbox >= Ctx::GATHER_CAP_MIN);
+}
 
 }  // namespace hdns

@@ -328,25 +328,37 @@ __device__ __forceinline__ void fft_io_plan(Radices<R1, REST...>, cx<T> *buf, co
 
 // The radix plan of each length in the register-in/register-out form (first_radix / last_radix
 // in spectral.hpp name its first and last pass).
+template <int N> struct IoPlan;
+template <> struct IoPlan<16> { using type = Radices<4, 4>; };
+template <> struct IoPlan<32> { using type = Radices<8, 4>; };
+template <> struct IoPlan<64> { using type = Radices<8, 8>; };
+template <> struct IoPlan<128> { using type = Radices<8, 4, 4>; };
+template <> struct IoPlan<256> { using type = Radices<8, 8, 4>; };
+template <> struct IoPlan<512> { using type = Radices<8, 8, 8>; };
+template <> struct IoPlan<1024> { using type = Radices<8, 8, 4, 4>; };
+template <> struct IoPlan<48> { using type = Radices<4, 4, 3>; };
+template <> struct IoPlan<96> { using type = Radices<8, 4, 3>; };
+template <> struct IoPlan<192> { using type = Radices<8, 8, 3>; };
+
 template <typename T, int N, int TX, int NT, int SIGN, bool IL, bool WAVE, bool SYNC_READ = false, class Load,
           class Store>
 __device__ __forceinline__ void fft_io(cx<T> *buf, const cx<T> *tw, Load &&load, Store &&store) {
-  if constexpr (N == 16) fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<4, 4>{}, buf, tw, load, store);
-  else if constexpr (N == 32) fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 4>{}, buf, tw, load, store);
-  else if constexpr (N == 64) fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 8>{}, buf, tw, load, store);
-  else if constexpr (N == 128)
-    fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 4, 4>{}, buf, tw, load, store);
-  else if constexpr (N == 256)
-    fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 8, 4>{}, buf, tw, load, store);
-  else if constexpr (N == 512)
-    fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 8, 8>{}, buf, tw, load, store);
-  else if constexpr (N == 1024)
-    fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 8, 4, 4>{}, buf, tw, load, store);
-  else if constexpr (N == 48) fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<4, 4, 3>{}, buf, tw, load, store);
-  else if constexpr (N == 96) fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 4, 3>{}, buf, tw, load, store);
-  else if constexpr (N == 192)
-    fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(Radices<8, 8, 3>{}, buf, tw, load, store);
-  else static_assert(N == 16, "unsupported transform length");
+  fft_io_plan<T, N, TX, NT, SIGN, IL, WAVE, SYNC_READ>(typename IoPlan<N>::type{}, buf, tw, load, store);
+}
+
+// fft_io with hook() run between the first pass and the rest: by then every thread has passed the
+// first pass's closing barrier, so whatever the loader read may be overwritten.
+template <typename T, int N, int TX, int NT, int SIGN, bool IL, int R1, int... REST, class Load, class Store,
+          class Hook>
+__device__ __forceinline__ void fft_io_hook_plan(Radices<R1, REST...>, cx<T> *buf, const cx<T> *tw, Load &&load,
+                                                 Store &&store, Hook &&hook) {
+  first_pass<T, N, R1, TX, NT, SIGN, IL, false, false>(buf, load);
+  hook();
+  middle_and_last<T, N, TX, NT, SIGN, IL, false, R1>(Radices<REST...>{}, buf, tw, store);
+}
+template <typename T, int N, int TX, int NT, int SIGN, bool IL, class Load, class Store, class Hook>
+__device__ __forceinline__ void fft_io_hook(cx<T> *buf, const cx<T> *tw, Load &&load, Store &&store, Hook &&hook) {
+  fft_io_hook_plan<T, N, TX, NT, SIGN, IL>(typename IoPlan<N>::type{}, buf, tw, load, store, hook);
 }
 
 // The second radix plan of each length, led by a radix-16 pass (one pass fewer at 256 and 1024);

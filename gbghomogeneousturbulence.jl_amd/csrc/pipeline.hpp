@@ -18,12 +18,13 @@ namespace hdns {
 template <typename T>
 constexpr int tile_width_of() { return sizeof(T) == 4 ? 16 : 8; }
 
-// Launch schedules per stage kernel family (two, or three for the z inverse), chosen per context by
+// Launch schedules per stage kernel family (two, or three for the z passes), chosen per context by
 // hdns_autotune:
 //   y passes: 256 or 512 threads per block (512 or 1024 from N = 512 on);
 //   z forward: (0) the scalar half and the velocity half as two kernels on 64*TX threads, one
 //              last-pass butterfly per thread at N = 512, the first two transformed velocity
 //              products staged in LDS where they fit (N <= 512); or (1) one fused kernel on 512;
+//              or (2) schedule 0 with the scalar half's flux tiles streamed through LDS;
 //   z inverse: a TX-wide tile on 64*TX threads (one first-pass butterfly per thread at N = 512)
 //              or on 32*TX threads (two), or (2) schedule 0's tile storing through LDS 16 bytes
 //              per lane;
@@ -118,6 +119,8 @@ struct PipelineImpl final : Pipeline {
       set_lds((const void *)k_zs<T, N, ZTX, ZN, false, true, DA>, LDS_Z);
       set_lds((const void *)k_zs<T, N, ZTX, ZN, true, true, DA>, LDS_Z);
       set_lds((const void *)k_zf<T, N, ZTX, ZN, false, ZFL, true, DA>, ZFL_BYTES);
+      // the z forward's third schedule: schedule 0 with the flux tiles streamed through LDS
+      set_lds((const void *)k_zs<T, N, ZTX, ZN, false, false, DA, true>, zs_lds_bytes<N, ZTX, T>(true));
     } else {
       constexpr int ZN = ZF_NT[1];
       set_lds((const void *)k_zf<T, N, ZTX, ZN, true, false, false, DA>, LDS_Z);
@@ -274,7 +277,7 @@ struct PipelineImpl final : Pipeline {
 
   // SSP: the Shu-Osher stage update (IFSSPRK3, IFHeun); SF: a scalar forcing.  Both are built for
   // schedule 0 only, which those runs always take.
-  template <int V, bool SSP = false, bool SF = false>
+  template <int V, bool SSP = false, bool SF = false, bool STAGE = false>
   void zf_v(Ctx &c, const StageArgs &sa) {
     constexpr int ZN = ZF_NT[V];
     const dim3 grid(c.geo.kxp / ZTX, K, c.nbox);
@@ -283,8 +286,8 @@ struct PipelineImpl final : Pipeline {
     const cx<double> *tw = (const cx<double> *)c.tw;
     if constexpr (V == 0) {
       if (c.cfg.has_scalar)   // the scalar half reads the stage u_z before the velocity half updates it
-        hipLaunchKernelGGL((k_zs<T, N, ZTX, ZN, SSP, SF, DA>), grid, dim3(ZN), LDS_Z, c.stream, pf, (const cx<T> *)uh,
-                           ph, hp, c.geo, tw, sa);
+        hipLaunchKernelGGL((k_zs<T, N, ZTX, ZN, SSP, SF, DA, STAGE>), grid, dim3(ZN), (zs_lds_bytes<N, ZTX, T>(STAGE)),
+                           c.stream, pf, (const cx<T> *)uh, ph, hp, c.geo, tw, sa);
       hipLaunchKernelGGL((k_zf<T, N, ZTX, ZN, false, ZFL, SSP, DA>), grid, dim3(ZN), ZFL_BYTES, c.stream, pf, uh, hu, ph,
                          hp, c.geo, tw, sa);
     } else {
@@ -303,6 +306,7 @@ struct PipelineImpl final : Pipeline {
                     c.cfg.scalar_forcing != HDNS_VARIANCE_RESCALING && c.cfg.scalar_forcing != HDNS_STOCHASTIC_OU;
     if (sa.ssp) sf ? zf_v<0, true, true>(c, sa) : zf_v<0, true>(c, sa);
     else if (sf) zf_v<0, false, true>(c, sa);
+    else if (c.sched[1] == 2) zf_v<0, false, false, true>(c, sa);
     else c.sched[1] ? zf_v<1>(c, sa) : zf_v<0>(c, sa);
   }
 
@@ -391,9 +395,21 @@ struct PipelineImpl final : Pipeline {
                       (int64_t)c.trail_cap * 3 * c.cfg.tracer_capacity};
   }
   int tblocks(const Ctx &c) const { return (int)((c.m + 255) / 256); }
+  // The gathering kernels' launch (k_advect, k_sample, k_advect_sample).  Capped: 128-thread
+  // blocks, at most four per CU (eight waves) -- a 40 KiB LDS reservation the kernels leave
+  // unused -- so fewer 4^3 stencils are in flight per XCD and more of their field lines stay in L2
+  // between the tracers that share them.  Uncapped: 256-thread blocks at the occupancy the
+  // registers allow.  Each tracer's arithmetic is the same under both.  Measured push + sample
+  // (2^20 tracers at 256^3; 2^22 at 512^3), capped against uncapped: FP64 0.96 against 1.22 ms,
+  // FP32 equal at 256^3 and 3.35 against 3.59 ms at 512^3; two blocks per CU were slower everywhere.
+  static constexpr size_t GATHER_CAP_LDS = 40 * 1024;
+  int gthreads(const Ctx &c) const { return tracer_gather_capped(c) ? 128 : 256; }
+  int gblocks(const Ctx &c) const { return (int)((c.m + gthreads(c) - 1) / gthreads(c)); }
+  size_t glds(const Ctx &c) const { return tracer_gather_capped(c) ? GATHER_CAP_LDS : 0; }
   void advect(Ctx &c, const AdamsArgs &aa) override {
     if (c.m == 0) return;
-    hipLaunchKernelGGL((k_advect<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (T *)c.pos, (T *)c.upos,
+    hipLaunchKernelGGL((k_advect<T>), dim3(gblocks(c), c.nbox), dim3(gthreads(c)), glds(c), c.stream, (T *)c.pos,
+                       (T *)c.upos,
                        (const T *)c.vel, (const T *)c.phys, c.order_valid ? c.order : nullptr, targs(c), aa);
   }
   void trail_combine(Ctx &c, const TrailCombine &tc, void *out) override {   // the selected box
@@ -405,14 +421,16 @@ struct PipelineImpl final : Pipeline {
   void sample(Ctx &c) override {
     if (c.m == 0) return;
     T *slot = (T *)c.vel + (size_t)c.trail_newest * 3 * c.cfg.tracer_capacity;
-    hipLaunchKernelGGL((k_sample<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const T *)c.pos, slot,
+    hipLaunchKernelGGL((k_sample<T>), dim3(gblocks(c), c.nbox), dim3(gthreads(c)), glds(c), c.stream,
+                       (const T *)c.pos, slot,
                        c.cfg.has_scalar ? (T *)c.phis : nullptr, (const T *)c.phys, c.order_valid ? c.order : nullptr,
                        targs(c));
   }
   void advect_sample(Ctx &c, const AdamsArgs &aa) override {
     if (c.m == 0) return;
     T *slot = (T *)c.vel + (size_t)c.trail_newest * 3 * c.cfg.tracer_capacity;
-    hipLaunchKernelGGL((k_advect_sample<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (T *)c.pos, (T *)c.upos,
+    hipLaunchKernelGGL((k_advect_sample<T>), dim3(gblocks(c), c.nbox), dim3(gthreads(c)), glds(c), c.stream,
+                       (T *)c.pos, (T *)c.upos,
                        (const T *)c.vel, slot, c.cfg.has_scalar ? (T *)c.phis : nullptr, (const T *)c.phys,
                        c.order_valid ? c.order : nullptr, targs(c), aa);
   }

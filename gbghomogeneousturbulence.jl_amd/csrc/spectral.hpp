@@ -79,14 +79,35 @@ constexpr int first_radix() { return (N == 16 || N == 48) ? 4 : 8; }
 template <int N>
 constexpr int last_radix() { return N % 3 == 0 ? 3 : ((N == 64 || N == 512) ? 8 : 4); }
 
+// The flux tile of PF component c for the staged scalar half: N rows of TX values, copied HBM ->
+// LDS 16 bytes per lane by direct-to-LDS loads (no registers held), into `stage` as [n][TX].  Lane
+// l of a wavefront's instruction lands at 16 l bytes past the instruction's (uniform) LDS address.
+template <typename T, int N, int TX, int NT>
+__device__ __forceinline__ void stage_tile(cx<T> *stage, const cx<T> *src, size_t zstride) {
+  constexpr int PR = TX * (int)sizeof(cx<T>) / 16;   // 16-byte pieces per tile row
+  constexpr int PIECES = N * PR;
+  static_assert(TX * sizeof(cx<T>) % 16 == 0, "a tile row holds whole 16-byte pieces");
+  const int lane = threadIdx.x & 63, w0 = threadIdx.x & ~63;
+  const char *s = reinterpret_cast<const char *>(src);
+  char *d = reinterpret_cast<char *>(stage);
+#pragma unroll
+  for (int i0 = w0; i0 < PIECES; i0 += NT) {
+    const int i = i0 + lane, n = i / PR, r = i % PR;
+    if (PIECES % NT == 0 || i < PIECES)
+      __builtin_amdgcn_global_load_lds(s + (size_t)n * zstride * sizeof(cx<T>) + 16 * r, d + 16 * i0, 16, 0, 0);
+  }
+}
+
 // The scalar half of a stage (src/Generator.jl:648-730, src/Propagator.jl:366-391): the three
 // fluxes' forward z transforms, N_phi = -mask i k . F - G u_z at the stage velocity, and the
 // phi / h_phi update.  It runs before the velocity update, which overwrites u_z.
-template <typename T, int N, int TX, int NT, bool SSP, bool SF, int DA = DEALIAS_TWO_THIRDS>
+// STAGE: each flux tile reaches the first pass through `stage` in LDS, the next one streaming in
+// while the current one transforms (the caller has started component 0's copy).
+template <typename T, int N, int TX, int NT, bool SSP, bool SF, int DA = DEALIAS_TWO_THIRDS, bool STAGE = false>
 __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, const cx<T> *__restrict__ pf,
                                                const cx<T> *__restrict__ uh, cx<T> *__restrict__ ph,
                                                cx<T> *__restrict__ hp, const Geo &g, const StageArgs &sa, int kx0,
-                                               int ky_idx, int my, int box) {
+                                               int ky_idx, int my, int box, cx<T> *stage = nullptr) {
   constexpr int K = band_k(N, DA);
   constexpr int RL = last_radix<N>();
   using SL = PassShape<N, RL, TX, NT>;
@@ -106,14 +127,26 @@ __device__ __forceinline__ void zf_scalar_part(cx<T> *buf, const cx<T> *twl, con
   for (int c = 0; c < 3; ++c) {
     __builtin_amdgcn_sched_barrier(0);   // one component's loads in flight at a time
     const cx<T> *src = pf + (size_t)(3 + c) * pencil + (size_t)ky_idx * g.kxp + kx0;
-    fft_io<T, N, TX, NT, -1, true, false>(
-        buf, twl,
-        [&](int, int, int n, int col) { return src[(size_t)n * zstride + col]; },
-        [&](int p, int q, int n, int col, cx<T> v) {
-          const int mc = c == 0 ? kx0 + col : (c == 1 ? my : signed_of(idx_of_bin(n, g), g));
-          const T kc = T(g.k0 * mc);
-          G[p][q] = {G[p][q].re + kc * v.re, G[p][q].im + kc * v.im};
-        });
+    auto accumulate = [&](int p, int q, int n, int col, cx<T> v) {
+      const int mc = c == 0 ? kx0 + col : (c == 1 ? my : signed_of(idx_of_bin(n, g), g));
+      const T kc = T(g.k0 * mc);
+      G[p][q] = {G[p][q].re + kc * v.re, G[p][q].im + kc * v.im};
+    };
+    if constexpr (STAGE) {
+      // the copy has landed: direct-to-LDS loads count in vmcnt until their data are in LDS, and
+      // the other threads' pieces are seen after the barrier
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();
+      fft_io_hook<T, N, TX, NT, -1, true>(
+          buf, twl, [&](int, int, int n, int col) { return stage[n * TX + col]; }, accumulate,
+          [&]() {   // every thread has read the stage: the next flux streams in during this transform
+            if (c < 2)
+              stage_tile<T, N, TX, NT>(stage, pf + (size_t)(4 + c) * pencil + (size_t)ky_idx * g.kxp + kx0, zstride);
+          });
+    } else {
+      fft_io<T, N, TX, NT, -1, true, false>(
+          buf, twl, [&](int, int, int n, int col) { return src[(size_t)n * zstride + col]; }, accumulate);
+    }
   }
   const T Gm = T(sa.mean_gradient);
   const double sub = stage_sub(sa);
@@ -405,20 +438,33 @@ __global__ __launch_bounds__(NT) void k_zf(const cx<T> *__restrict__ pf, cx<T> *
 
 // The scalar half on its own (the velocity half then runs as k_zf without SCALAR): two smaller
 // kernels hold fewer registers than the fused one.
-template <typename T, int N, int TX, int NT, bool SSP = false, bool SF = false, int DA = DEALIAS_TWO_THIRDS>
+// STAGE: the flux tiles stream through a third LDS tile (zs_lds_bytes), the first one's copy
+// started before the twiddles are staged.
+template <int N, int TX, typename T>
+constexpr size_t zs_lds_bytes(bool stage) {
+  return ((size_t)N * TX + N + (stage ? (size_t)N * TX : 0)) * sizeof(cx<T>);
+}
+
+template <typename T, int N, int TX, int NT, bool SSP = false, bool SF = false, int DA = DEALIAS_TWO_THIRDS,
+          bool STAGE = false>
 __global__ __launch_bounds__(NT) void k_zs(const cx<T> *__restrict__ pf, const cx<T> *__restrict__ uh,
                                            cx<T> *__restrict__ ph, cx<T> *__restrict__ hp, Geo g,
                                            const cx<double> *__restrict__ tw, StageArgs sa) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
   cx<T> *twl = buf + N * TX;
+  cx<T> *stage = twl + N;
   const int kx0 = blockIdx.x * TX, ky_idx = blockIdx.y, box = blockIdx.z;
   const int my = signed_of(ky_idx, g);
   if (!in_disk(kx0, my, g)) return;   // no column of this tile meets the ball (its PF lines are unwritten)
+  const cx<T> *pfb = pf + box * g.box_b;
+  if constexpr (STAGE)
+    stage_tile<T, N, TX, NT>(stage, pfb + (size_t)3 * N * g.k * g.kxp + (size_t)ky_idx * g.kxp + kx0,
+                             (size_t)g.k * g.kxp);
   stage_twiddles<T, N, NT>(twl, tw);
   const size_t comp = comp_of(g);
-  zf_scalar_part<T, N, TX, NT, SSP, SF, DA>(buf, twl, pf + box * g.box_b, uh + box * 3 * comp, ph + box * comp,
-                               hp + box * comp, g, sa, kx0, ky_idx, my, box);
+  zf_scalar_part<T, N, TX, NT, SSP, SF, DA, STAGE>(buf, twl, pfb, uh + box * 3 * comp, ph + box * comp,
+                                                   hp + box * comp, g, sa, kx0, ky_idx, my, box, stage);
 }
 
 // The inverse z pass of the state: u, w = i k x u and phi (with the 1/N^3 of the inverse

@@ -250,7 +250,8 @@ hdns_status hdns_heun_weights(double h, double *out3);
 
 /* Autotuner: time every launch schedule of each kernel family -- the y passes (256/512 threads,
  * 512/1024 from N = 512 on), the z forward pass (split scalar/velocity kernels with LDS-staged
- * products, or one fused kernel), the z inverse pass (full or half-width tiles, or full tiles
+ * products, the same with the scalar half's flux tiles streamed through LDS, or one fused
+ * kernel), the z inverse pass (full or half-width tiles, or full tiles
  * written back 16 bytes per lane through LDS), the x pass
  * (256/512 threads, or with/without prefetch) and the y passes' radix plan (the default plan, or
  * one led by a radix-16 pass) -- on this context's own buffers, keep the fastest of each, and redo
@@ -263,8 +264,15 @@ hdns_status hdns_heun_weights(double h, double *out3);
 #define HDNS_SCHED_VARIANTS 4
 hdns_status hdns_autotune(hdns_ctx *ctx, int32_t reps, double *ms, int32_t *chosen);
 /* Set (set != NULL) and/or read (get != NULL) the schedule: five entries, in the order y passes
- * (0-1), z forward (0-1), z inverse (0-2), x pass (0-1), y radix plan (0-1). */
+ * (0-1), z forward (0-2), z inverse (0-2), x pass (0-1), y radix plan (0-1). */
 hdns_status hdns_schedule(hdns_ctx *ctx, const int32_t *set, int32_t *get);
+
+/* The launch of the tracer gathers (hdns_advect, hdns_deposit_tracers, hdns_advect_deposit): 1 puts
+ * them on 128-thread blocks at most four to a CU, so fewer 4^3 stencils are in flight per XCD and
+ * more of their field lines stay in L2; 0 on 256-thread blocks at full occupancy; -1 (the default)
+ * chooses 1 from 2^16 tracers per launch on; -2 leaves the setting.  *get (optional) receives the
+ * launch in force.  The results do not depend on it.  No counterpart in the reference. */
+hdns_status hdns_tracer_schedule(hdns_ctx *ctx, int32_t set, int32_t *get);
 
 /* Per-kernel timing with HIP events on the context's stream, for the roofline report.  Kinds:
  * 0 y forward, 1 z forward + stage update, 2 reality projection, 3 z inverse + reductions,

@@ -300,14 +300,16 @@ def test_autotune_leaves_the_state_bitwise_unchanged_and_schedules_agree(g):
 
 @pytest.mark.parametrize("n,dtype", [(16, np.float64), (48, np.float64), (64, np.float32), (96, np.float64),
                                      (128, np.float32), (256, np.float32)])
-def test_z_inverse_wide_stores_are_bitwise_the_plain_stores(g, n, dtype):
-    """The z inverse's third schedule writes each transformed tile back through LDS, 16 bytes per
-    lane: the same values as schedule 0, so one forced step with the scalar (and the deposit it
-    ends with) is bitwise schedule 0's; and it matches the oracle."""
+@pytest.mark.parametrize("sched", [(1, 0, 2, 0, 0), (1, 2, 0, 0, 0)], ids=["z_inverse_wide", "z_forward_streamed"])
+def test_z_pass_staging_schedules_are_bitwise_schedule_0(g, n, dtype, sched):
+    """The z passes' third schedules move the same values through LDS: the z inverse writes each
+    transformed tile back 16 bytes per lane, the z forward's scalar half streams each flux tile
+    into LDS by direct-to-LDS loads while the previous one transforms.  So one forced step with the
+    scalar (and the deposit it ends with) is bitwise schedule 0's; and it matches the oracle."""
     sim, u, p, oc = make_sim(g, n, dtype, forcing=0.1)
     ref, *_ = make_sim(g, n, dtype, forcing=0.1)
     with sim, ref:
-        sim.schedule((1, 0, 2, 0, 0))
+        sim.schedule(sched)
         ref.schedule((1, 0, 0, 0, 0))
         for s in (sim, ref):
             s._call("hdns_deposit")

@@ -215,19 +215,21 @@ def _tracked(g, mode, scheme, dtype, steps=6, m=300, n=16):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_the_fused_push_and_sample_is_bitwise_the_two_calls(g, dtype):
     """advance() pushes and samples in one launch (hdns_advect_deposit); a run that makes the two
-    calls itself (step_fields, rebuild, advect, deposit_tracers) is bitwise the same in positions,
-    unwrapped positions, every trail sample and time, and the scalar samples.  AB(3) fills its ring
-    of three, so the sample's slot is one the push reads; 1000 tracers make several blocks."""
+    calls itself (step_fields, rebuild, advect, deposit_tracers), and one whose gathers run on the
+    capped launch (128-thread blocks, two per CU), are bitwise the same in positions, unwrapped
+    positions, every trail sample and time, and the scalar samples.  AB(3) fills its ring of three,
+    so the sample's slot is one the push reads; 1000 tracers make several blocks."""
     n, m, dt = 32, 1000, 0.01
     for scheme in (g.Heun(), g.AB(3), g.ABM(2)):
         runs = []
-        for fused in (True, False):
+        for fused, capped in ((True, 0), (False, 0), (True, 1)):
             dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
             u, p = g.random_phase_velocity(n, seed=3), g.random_phase_scalar(n, seed=4)
             sim = g.Simulation(dom, conditions=g.FlowConditions(viscosity=0.05, schmidt=1.0),
                                state=g.ResolvedFields(u, p), mean_gradient=1.0,
                                tracers=g.Tracers(dom, number_of_particles=m, seed=5, unwrapped=True),
                                particle_integration_scheme=scheme)
+            assert sim.tracer_schedule(capped) == capped
             for i in range(7):
                 if fused:
                     sim.advance(dt)
@@ -237,8 +239,8 @@ def test_the_fused_push_and_sample_is_bitwise_the_two_calls(g, dtype):
                     sim.advect(dt)
                     sim.deposit_tracers((i + 1) * dt)
             runs.append(sim)
-        a, b = runs
-        with a, b:
+        a = runs[0]
+        for b in runs[1:]:
             assert np.array_equal(a.tracer_positions(), b.tracer_positions())
             assert np.array_equal(a.unwrapped_positions(), b.unwrapped_positions())
             assert a.trail_times() == b.trail_times()
@@ -246,6 +248,8 @@ def test_the_fused_push_and_sample_is_bitwise_the_two_calls(g, dtype):
                 assert np.array_equal(a.trail_sample(-j), b.trail_sample(-j))
             (va, sa), (vb, sb) = a.tracer_samples(), b.tracer_samples()
             assert np.array_equal(va, vb) and np.array_equal(sa, sb)
+        for r in runs:
+            r.close()
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
