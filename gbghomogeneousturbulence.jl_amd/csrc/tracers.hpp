@@ -53,29 +53,6 @@ __device__ __forceinline__ Stencil stencil_of(T x, T y, T z, T inv_dx, int n, T 
   return {wrap_index((int)fx, n), wrap_index((int)fy, n), wrap_index((int)fz, n)};
 }
 
-// Separable 4^3 gather of NF components at one point: at(q, c, b, a) is component q at the stencil
-// point (k + c - 1, j + b - 1, i + a - 1).  The sums are explicit fused multiply-adds in a fixed
-// order, so a gather from HBM and one from a staged copy of the same values agree to the bit.
-template <typename T, int NF, typename At>
-__device__ __forceinline__ void gather_at(At at, const T *wx, const T *wy, const T *wz, T *out) {
-#pragma unroll
-  for (int q = 0; q < NF; ++q) out[q] = T(0);
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const T wzy = wz[c] * wy[b];
-#pragma unroll
-      for (int q = 0; q < NF; ++q) {
-        T r = T(0);
-#pragma unroll
-        for (int a = 0; a < 4; ++a) r = fma(wx[a], at(q, c, b, a), r);
-        out[q] = fma(wzy, r, out[q]);
-      }
-    }
-  }
-}
-
 // Four consecutive values of a row from a 4-byte-aligned (FP32) or 8-byte-aligned (FP64) address:
 // one 16-byte load (FP32) or two (FP64) instead of four.
 typedef float row4f __attribute__((ext_vector_type(4), aligned(4)));
@@ -95,37 +72,32 @@ __device__ __forceinline__ void load_run4(const double *p, double *v) {
   v[3] = b.y;
 }
 
-// The gather from the field in HBM: NF components stride vol apart, periodic indices.  When the
-// stencil's four x points do not wrap (all but 3 of N cells), each row of each component is one
-// wide load; the values, and so the sums, are the same either way.
+// The gather of NF components of one stencil plane at one point: plane c (z index k + c - 1)
+// holds rows b = 0..3 (y index j + b - 1), each weighted wz[c] wy[b] and summed over its four x
+// points with explicit fused multiply-adds in a fixed order.  Components stride vol apart; indices
+// are periodic.  When the stencil's four x points do not wrap (all but 3 of N cells) a row of a
+// component is one wide load; the values, and so the sums, are the same either way.
 template <typename T, int NF>
-__device__ __forceinline__ void gather(const T *__restrict__ f, size_t vol, int n, const Stencil &s, const T *wx,
-                                       const T *wy, const T *wz, T *out) {
-  size_t row[4][4];
+__device__ __forceinline__ void gather_plane(const T *__restrict__ f, size_t vol, int n, const Stencil &s, int c,
+                                             const T *wx, const T *wy, const T *wz, T *out) {
+  const int kk = wrap_index(s.k + c - 1, n);
+  const T wc = wz[c];
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int kk = wrap_index(s.k + c - 1, n);
-#pragma unroll
-    for (int b = 0; b < 4; ++b) row[c][b] = ((size_t)kk * n + wrap_index(s.j + b - 1, n)) * n;
-  }
+  for (int q = 0; q < NF; ++q) out[q] = T(0);
   if (s.i >= 1 && s.i + 2 < n) {
     const int i0 = s.i - 1;
 #pragma unroll
-    for (int q = 0; q < NF; ++q) out[q] = T(0);
+    for (int b = 0; b < 4; ++b) {
+      const size_t row = ((size_t)kk * n + wrap_index(s.j + b - 1, n)) * n + i0;
+      const T wzy = wc * wy[b];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+      for (int q = 0; q < NF; ++q) {
+        T v[4];
+        load_run4(f + q * vol + row, v);
+        T r = T(0);
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const T wzy = wz[c] * wy[b];
-#pragma unroll
-        for (int q = 0; q < NF; ++q) {
-          T v[4];
-          load_run4(f + q * vol + row[c][b] + i0, v);
-          T r = T(0);
-#pragma unroll
-          for (int a = 0; a < 4; ++a) r = fma(wx[a], v[a], r);
-          out[q] = fma(wzy, r, out[q]);
-        }
+        for (int a = 0; a < 4; ++a) r = fma(wx[a], v[a], r);
+        out[q] = fma(wzy, r, out[q]);
       }
     }
     return;
@@ -133,7 +105,34 @@ __device__ __forceinline__ void gather(const T *__restrict__ f, size_t vol, int 
   int ii[4];
 #pragma unroll
   for (int a = 0; a < 4; ++a) ii[a] = wrap_index(s.i + a - 1, n);
-  gather_at<T, NF>([&](int q, int c, int b, int a) { return f[q * vol + row[c][b] + ii[a]]; }, wx, wy, wz, out);
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    const size_t row = ((size_t)kk * n + wrap_index(s.j + b - 1, n)) * n;
+    const T wzy = wc * wy[b];
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+      T r = T(0);
+#pragma unroll
+      for (int a = 0; a < 4; ++a) r = fma(wx[a], f[q * vol + row + ii[a]], r);
+      out[q] = fma(wzy, r, out[q]);
+    }
+  }
+}
+
+// The 4^3 gather of one tracer by the four lanes of a quad: lane c of the quad gathers plane c, and
+// the quad adds its four plane sums as (p0 + p1) + (p2 + p3), which every lane forms alike (the
+// additions commute exactly), so all four hold the same values.  A quad covers a quarter of the
+// stencils a lane-per-tracer wavefront would: four times fewer tracers in flight for the same
+// waves, and their field lines stay in L2 while the tracers that share them are gathered.
+template <typename T, int NF>
+__device__ __forceinline__ void gather_quad(const T *__restrict__ f, size_t vol, int n, const Stencil &s, int c,
+                                            const T *wx, const T *wy, const T *wz, T *out) {
+  gather_plane<T, NF>(f, vol, n, s, c, wx, wy, wz, out);
+#pragma unroll
+  for (int q = 0; q < NF; ++q) {
+    out[q] = out[q] + __shfl_xor(out[q], 1, 4);
+    out[q] = out[q] + __shfl_xor(out[q], 2, 4);
+  }
 }
 
 struct TracerArgs {
@@ -164,11 +163,12 @@ struct AdamsArgs {
 };
 
 // One tracer's Adams step; g3(stencil, wx, wy, wz, out) gathers u at a point.  The new (wrapped)
-// position, as stored, is also left in xn.
+// position, as stored, is also left in xn.  The four lanes of the tracer's quad all run it (the
+// gather needs every one of them); only the writer stores.
 template <typename T, typename G3>
 __device__ __forceinline__ void advect_one(int64_t p, T *__restrict__ pos, T *__restrict__ upos,
                                            const T *__restrict__ trail, const TracerArgs &ta, const AdamsArgs &aa,
-                                           G3 g3, T (&xn)[3]) {
+                                           G3 g3, T (&xn)[3], bool writer) {
   const T L = T(ta.side_length), inv_dx = T(ta.n / ta.side_length);
   T F[MAX_TRAIL][3], acc[3], Xs[3], ua[3] = {T(0), T(0), T(0)};
 #pragma unroll
@@ -192,7 +192,10 @@ __device__ __forceinline__ void advect_one(int64_t p, T *__restrict__ pos, T *__
   for (int d = 0; d < 3; ++d) Xs[d] = wrap_box(acc[d], L);
   if (!aa.corrected) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) pos[d * ta.m + p] = xn[d] = Xs[d];
+    for (int d = 0; d < 3; ++d) xn[d] = Xs[d];
+    if (!writer) return;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pos[d * ta.m + p] = Xs[d];
     if (upos) {
 #pragma unroll
       for (int d = 0; d < 3; ++d) upos[d * ta.m + p] = ua[d];
@@ -219,21 +222,35 @@ __device__ __forceinline__ void advect_one(int64_t p, T *__restrict__ pos, T *__
     }
   }
 #pragma unroll
-  for (int d = 0; d < 3; ++d) pos[d * ta.m + p] = xn[d] = wrap_box(acc[d], L);
+  for (int d = 0; d < 3; ++d) xn[d] = wrap_box(acc[d], L);
+  if (!writer) return;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) pos[d * ta.m + p] = xn[d];
   if (upos) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) upos[d * ta.m + p] = ua[d];
   }
 }
 
-// The push in tracer (or traversal) order, one thread per tracer.
+// The thread of a gathering kernel: tracer t (in traversal order) of its box, lane c of the
+// tracer's quad.  Blocks of one XCD take consecutive chunks of the order, so the field lines that
+// neighbouring tiles share stay in that XCD's L2.
+struct QuadThread {
+  int64_t t;
+  int c;
+};
+__device__ __forceinline__ QuadThread quad_thread() {
+  const int64_t g = xcd_chunk(blockIdx.x, gridDim.x) * (int64_t)blockDim.x + threadIdx.x;
+  return {g >> 2, (int)(g & 3)};
+}
+
+// The push in tracer (or traversal) order, one quad of lanes per tracer.
 template <typename T>
 __global__ void k_advect(T *__restrict__ pos, T *__restrict__ upos, const T *__restrict__ trail, const T *__restrict__ u,
                          const int *__restrict__ order, TracerArgs ta, AdamsArgs aa) {
-  // blocks of one XCD take consecutive chunks of the traversal order: the field lines that
-  // neighbouring tiles share stay in that XCD's L2
-  const int64_t t = xcd_chunk(blockIdx.x, gridDim.x) * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= ta.m) return;
+  const QuadThread qt = quad_thread();
+  const int64_t t = qt.t;
+  if (t >= ta.m) return;   // the whole quad leaves together
   const int box = blockIdx.y;
   pos += box * 3 * ta.cap;
   if (upos) upos += box * 3 * ta.cap;
@@ -245,8 +262,10 @@ __global__ void k_advect(T *__restrict__ pos, T *__restrict__ upos, const T *__r
   T xn[3];
   advect_one(
       p, pos, upos, trail, ta, aa,
-      [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out) { gather<T, 3>(u, vol, ta.n, s, wx, wy, wz, out); },
-      xn);
+      [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out) {
+        gather_quad<T, 3>(u, vol, ta.n, s, qt.c, wx, wy, wz, out);
+      },
+      xn, qt.c == 0);
 }
 
 // sum_j w_j f_j over n trail samples of the selected box, added left to right (time_derivative!,
@@ -274,33 +293,25 @@ __global__ void k_trail_combine(const T *__restrict__ trail, TracerArgs ta, Trai
 // physical fields are [4][N^3] per box (u, then phi), so phi is u's fourth component.
 template <typename T, typename G>
 __device__ __forceinline__ void sample_at(int64_t p, T x, T y, T z, T *__restrict__ vel, T *__restrict__ phis,
-                                          const TracerArgs &ta, G g) {
+                                          const TracerArgs &ta, G g, bool writer) {
   const T inv_dx = T(ta.n / ta.side_length);
   T wx[4], wy[4], wz[4], v[3];
   const Stencil s = stencil_of(x, y, z, inv_dx, ta.n, wx, wy, wz);
   g(s, wx, wy, wz, v, 0);
+  T q[1] = {T(0)};
+  if (phis) g(s, wx, wy, wz, q, 3);
+  if (!writer) return;
 #pragma unroll
   for (int d = 0; d < 3; ++d) vel[d * ta.m + p] = v[d];
-  if (phis) {
-    T q[1];
-    g(s, wx, wy, wz, q, 3);
-    phis[p] = q[0];
-  }
-}
-
-template <typename T, typename G>
-__device__ __forceinline__ void sample_one(int64_t p, const T *__restrict__ pos, T *__restrict__ vel,
-                                           T *__restrict__ phis, const TracerArgs &ta, G g) {
-  sample_at(p, pos[p], pos[ta.m + p], pos[2 * ta.m + p], vel, phis, ta, g);
+  if (phis) phis[p] = q[0];
 }
 
 template <typename T>
 __global__ void k_sample(const T *__restrict__ pos, T *__restrict__ vel, T *__restrict__ phis,
                          const T *__restrict__ u, const int *__restrict__ order, TracerArgs ta) {
-  // blocks of one XCD take consecutive chunks of the traversal order: the field lines that
-  // neighbouring tiles share stay in that XCD's L2
-  const int64_t t = xcd_chunk(blockIdx.x, gridDim.x) * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= ta.m) return;
+  const QuadThread qt = quad_thread();
+  const int64_t t = qt.t;
+  if (t >= ta.m) return;   // the whole quad leaves together
   const int box = blockIdx.y;
   pos += box * 3 * ta.cap;
   vel += box * ta.vstride;
@@ -309,25 +320,28 @@ __global__ void k_sample(const T *__restrict__ pos, T *__restrict__ vel, T *__re
   if (order) order += box * ta.cap;
   const int64_t p = order ? order[t] : t;
   const size_t vol = (size_t)ta.n * ta.n * ta.n;
-  sample_one(p, pos, vel, phis, ta, [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out, int c0) {
-    if (c0 == 0) gather<T, 3>(u, vol, ta.n, s, wx, wy, wz, out);
-    else gather<T, 1>(u + c0 * vol, vol, ta.n, s, wx, wy, wz, out);
-  });
+  sample_at(
+      p, pos[p], pos[ta.m + p], pos[2 * ta.m + p], vel, phis, ta,
+      [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out, int c0) {
+        if (c0 == 0) gather_quad<T, 3>(u, vol, ta.n, s, qt.c, wx, wy, wz, out);
+        else gather_quad<T, 1>(u + c0 * vol, vol, ta.n, s, qt.c, wx, wy, wz, out);
+      },
+      qt.c == 0);
 }
 
 // The push and the closing sample in one pass (an iteration's advect! then deposit!, both on the
 // new field): the sample is taken at the new position the push leaves in registers, whose 4^3
-// stencil mostly overlaps the one the corrector just gathered, so its loads find the lines in
-// cache instead of sweeping the field a second time.  Each value is the one k_advect then k_sample
-// would produce, bit for bit.  The sample goes into the trail slot `vel` points at; when that slot
-// is also one the push reads (a full AB(k) ring), each thread reads its tracer's samples there
-// before it writes the new one, and no other thread touches that tracer's entries.
+// stencil mostly overlaps the one the corrector just gathered.  Each value is the one k_advect then
+// k_sample would produce, bit for bit.  The sample goes into the trail slot `vel` points at; when
+// that slot is also one the push reads (a full AB(k) ring), the quad reads its tracer's samples
+// there before its writer stores the new one, and no other quad touches that tracer's entries.
 template <typename T>
 __global__ void k_advect_sample(T *__restrict__ pos, T *__restrict__ upos, const T *trail, T *vel,
                                 T *__restrict__ phis, const T *__restrict__ u, const int *__restrict__ order,
                                 TracerArgs ta, AdamsArgs aa) {
-  const int64_t t = xcd_chunk(blockIdx.x, gridDim.x) * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= ta.m) return;
+  const QuadThread qt = quad_thread();
+  const int64_t t = qt.t;
+  if (t >= ta.m) return;   // the whole quad leaves together
   const int box = blockIdx.y;
   pos += box * 3 * ta.cap;
   if (upos) upos += box * 3 * ta.cap;
@@ -341,13 +355,17 @@ __global__ void k_advect_sample(T *__restrict__ pos, T *__restrict__ upos, const
   T xn[3];
   advect_one(
       p, pos, upos, trail, ta, aa,
-      [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out) { gather<T, 3>(u, vol, ta.n, s, wx, wy, wz, out); },
-      xn);
-  sample_at(p, xn[0], xn[1], xn[2], vel, phis, ta,
-            [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out, int c0) {
-              if (c0 == 0) gather<T, 3>(u, vol, ta.n, s, wx, wy, wz, out);
-              else gather<T, 1>(u + c0 * vol, vol, ta.n, s, wx, wy, wz, out);
-            });
+      [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out) {
+        gather_quad<T, 3>(u, vol, ta.n, s, qt.c, wx, wy, wz, out);
+      },
+      xn, qt.c == 0);
+  sample_at(
+      p, xn[0], xn[1], xn[2], vel, phis, ta,
+      [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out, int c0) {
+        if (c0 == 0) gather_quad<T, 3>(u, vol, ta.n, s, qt.c, wx, wy, wz, out);
+        else gather_quad<T, 1>(u + c0 * vol, vol, ta.n, s, qt.c, wx, wy, wz, out);
+      },
+      qt.c == 0);
 }
 
 template <typename T>
