@@ -404,7 +404,9 @@ struct PipelineImpl final : Pipeline {
   // FP32 equal at 256^3 and 3.35 against 3.59 ms at 512^3; two blocks per CU were slower everywhere.
   static constexpr size_t GATHER_CAP_LDS = 40 * 1024;
   int gthreads(const Ctx &c) const { return tracer_gather_capped(c) ? 128 : 256; }
-  int gblocks(const Ctx &c) const { return (int)((4 * c.m + gthreads(c) - 1) / gthreads(c)); }   // a quad per tracer
+  int gblocks(const Ctx &c) const {   // TRACER_LANES lanes per tracer
+    return (int)((TRACER_LANES * c.m + gthreads(c) - 1) / gthreads(c));
+  }
   size_t glds(const Ctx &c) const { return tracer_gather_capped(c) ? GATHER_CAP_LDS : 0; }
   void advect(Ctx &c, const AdamsArgs &aa) override {
     if (c.m == 0) return;

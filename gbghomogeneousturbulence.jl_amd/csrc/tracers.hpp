@@ -72,14 +72,14 @@ __device__ __forceinline__ void load_run4(const double *p, double *v) {
   v[3] = b.y;
 }
 
-// The gather of NF components of one stencil plane at one point: plane c (z index k + c - 1)
-// holds rows b = 0..3 (y index j + b - 1), each weighted wz[c] wy[b] and summed over its four x
-// points with explicit fused multiply-adds in a fixed order.  Components stride vol apart; indices
-// are periodic.  When the stencil's four x points do not wrap (all but 3 of N cells) a row of a
-// component is one wide load; the values, and so the sums, are the same either way.
-template <typename T, int NF>
-__device__ __forceinline__ void gather_plane(const T *__restrict__ f, size_t vol, int n, const Stencil &s, int c,
-                                             const T *wx, const T *wy, const T *wz, T *out) {
+// The gather of NF components of part of a stencil at one point: the rows b0 .. b0 + NB - 1 (y
+// index j + b - 1) of plane c (z index k + c - 1), each weighted wz[c] wy[b] and summed over its
+// four x points with explicit fused multiply-adds in a fixed order.  Components stride vol apart;
+// indices are periodic.  When the stencil's four x points do not wrap (all but 3 of N cells) a row
+// of a component is one wide load; the values, and so the sums, are the same either way.
+template <typename T, int NF, int NB>
+__device__ __forceinline__ void gather_rows(const T *__restrict__ f, size_t vol, int n, const Stencil &s, int c,
+                                            int b0, const T *wx, const T *wy, const T *wz, T *out) {
   const int kk = wrap_index(s.k + c - 1, n);
   const T wc = wz[c];
 #pragma unroll
@@ -87,7 +87,8 @@ __device__ __forceinline__ void gather_plane(const T *__restrict__ f, size_t vol
   if (s.i >= 1 && s.i + 2 < n) {
     const int i0 = s.i - 1;
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
+    for (int bb = 0; bb < NB; ++bb) {
+      const int b = b0 + bb;
       const size_t row = ((size_t)kk * n + wrap_index(s.j + b - 1, n)) * n + i0;
       const T wzy = wc * wy[b];
 #pragma unroll
@@ -106,7 +107,8 @@ __device__ __forceinline__ void gather_plane(const T *__restrict__ f, size_t vol
 #pragma unroll
   for (int a = 0; a < 4; ++a) ii[a] = wrap_index(s.i + a - 1, n);
 #pragma unroll
-  for (int b = 0; b < 4; ++b) {
+  for (int bb = 0; bb < NB; ++bb) {
+    const int b = b0 + bb;
     const size_t row = ((size_t)kk * n + wrap_index(s.j + b - 1, n)) * n;
     const T wzy = wc * wy[b];
 #pragma unroll
@@ -119,20 +121,22 @@ __device__ __forceinline__ void gather_plane(const T *__restrict__ f, size_t vol
   }
 }
 
-// The 4^3 gather of one tracer by the four lanes of a quad: lane c of the quad gathers plane c, and
-// the quad adds its four plane sums as (p0 + p1) + (p2 + p3), which every lane forms alike (the
-// additions commute exactly), so all four hold the same values.  A quad covers a quarter of the
-// stencils a lane-per-tracer wavefront would: four times fewer tracers in flight for the same
-// waves, and their field lines stay in L2 while the tracers that share them are gathered.
+// The 4^3 gather of one tracer by a group of LPT lanes (4, 8 or 16): lane l gathers plane l mod 4,
+// rows (l / 4) 16/LPT .. of it, and the group adds its partial sums by butterflies (lanes l and
+// l xor 1, then l xor 2, ...), which every lane forms alike (each addition commutes exactly), so all
+// of them hold the same values.  A group covers 1/LPT of the stencils a lane-per-tracer wavefront
+// would: fewer tracers in flight for the same waves, and their field lines stay in L2 while the
+// tracers that share them are gathered.
+constexpr int TRACER_LANES = 4;
 template <typename T, int NF>
-__device__ __forceinline__ void gather_quad(const T *__restrict__ f, size_t vol, int n, const Stencil &s, int c,
-                                            const T *wx, const T *wy, const T *wz, T *out) {
-  gather_plane<T, NF>(f, vol, n, s, c, wx, wy, wz, out);
+__device__ __forceinline__ void gather_group(const T *__restrict__ f, size_t vol, int n, const Stencil &s, int l,
+                                             const T *wx, const T *wy, const T *wz, T *out) {
+  constexpr int NB = 16 / TRACER_LANES;   // rows per lane
+  gather_rows<T, NF, NB>(f, vol, n, s, l & 3, (l >> 2) * NB, wx, wy, wz, out);
 #pragma unroll
-  for (int q = 0; q < NF; ++q) {
-    out[q] = out[q] + __shfl_xor(out[q], 1, 4);
-    out[q] = out[q] + __shfl_xor(out[q], 2, 4);
-  }
+  for (int q = 0; q < NF; ++q)
+#pragma unroll
+    for (int m = 1; m < TRACER_LANES; m <<= 1) out[q] = out[q] + __shfl_xor(out[q], m, TRACER_LANES);
 }
 
 struct TracerArgs {
@@ -163,8 +167,8 @@ struct AdamsArgs {
 };
 
 // One tracer's Adams step; g3(stencil, wx, wy, wz, out) gathers u at a point.  The new (wrapped)
-// position, as stored, is also left in xn.  The four lanes of the tracer's quad all run it (the
-// gather needs every one of them); only the writer stores.
+// position, as stored, is also left in xn.  Every lane of the tracer's group runs it (the gather
+// needs all of them); only the writer stores.
 template <typename T, typename G3>
 __device__ __forceinline__ void advect_one(int64_t p, T *__restrict__ pos, T *__restrict__ upos,
                                            const T *__restrict__ trail, const TracerArgs &ta, const AdamsArgs &aa,
@@ -233,24 +237,24 @@ __device__ __forceinline__ void advect_one(int64_t p, T *__restrict__ pos, T *__
 }
 
 // The thread of a gathering kernel: tracer t (in traversal order) of its box, lane c of the
-// tracer's quad.  Blocks of one XCD take consecutive chunks of the order, so the field lines that
-// neighbouring tiles share stay in that XCD's L2.
-struct QuadThread {
+// tracer's group of TRACER_LANES.  Blocks of one XCD take consecutive chunks of the order, so the
+// field lines that neighbouring tiles share stay in that XCD's L2.
+struct GroupThread {
   int64_t t;
   int c;
 };
-__device__ __forceinline__ QuadThread quad_thread() {
+__device__ __forceinline__ GroupThread group_thread() {
   const int64_t g = xcd_chunk(blockIdx.x, gridDim.x) * (int64_t)blockDim.x + threadIdx.x;
-  return {g >> 2, (int)(g & 3)};
+  return {g / TRACER_LANES, (int)(g % TRACER_LANES)};
 }
 
-// The push in tracer (or traversal) order, one quad of lanes per tracer.
+// The push in tracer (or traversal) order, a group of TRACER_LANES lanes per tracer.
 template <typename T>
 __global__ void k_advect(T *__restrict__ pos, T *__restrict__ upos, const T *__restrict__ trail, const T *__restrict__ u,
                          const int *__restrict__ order, TracerArgs ta, AdamsArgs aa) {
-  const QuadThread qt = quad_thread();
+  const GroupThread qt = group_thread();
   const int64_t t = qt.t;
-  if (t >= ta.m) return;   // the whole quad leaves together
+  if (t >= ta.m) return;   // the whole group leaves together
   const int box = blockIdx.y;
   pos += box * 3 * ta.cap;
   if (upos) upos += box * 3 * ta.cap;
@@ -263,7 +267,7 @@ __global__ void k_advect(T *__restrict__ pos, T *__restrict__ upos, const T *__r
   advect_one(
       p, pos, upos, trail, ta, aa,
       [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out) {
-        gather_quad<T, 3>(u, vol, ta.n, s, qt.c, wx, wy, wz, out);
+        gather_group<T, 3>(u, vol, ta.n, s, qt.c, wx, wy, wz, out);
       },
       xn, qt.c == 0);
 }
@@ -309,9 +313,9 @@ __device__ __forceinline__ void sample_at(int64_t p, T x, T y, T z, T *__restric
 template <typename T>
 __global__ void k_sample(const T *__restrict__ pos, T *__restrict__ vel, T *__restrict__ phis,
                          const T *__restrict__ u, const int *__restrict__ order, TracerArgs ta) {
-  const QuadThread qt = quad_thread();
+  const GroupThread qt = group_thread();
   const int64_t t = qt.t;
-  if (t >= ta.m) return;   // the whole quad leaves together
+  if (t >= ta.m) return;   // the whole group leaves together
   const int box = blockIdx.y;
   pos += box * 3 * ta.cap;
   vel += box * ta.vstride;
@@ -323,8 +327,8 @@ __global__ void k_sample(const T *__restrict__ pos, T *__restrict__ vel, T *__re
   sample_at(
       p, pos[p], pos[ta.m + p], pos[2 * ta.m + p], vel, phis, ta,
       [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out, int c0) {
-        if (c0 == 0) gather_quad<T, 3>(u, vol, ta.n, s, qt.c, wx, wy, wz, out);
-        else gather_quad<T, 1>(u + c0 * vol, vol, ta.n, s, qt.c, wx, wy, wz, out);
+        if (c0 == 0) gather_group<T, 3>(u, vol, ta.n, s, qt.c, wx, wy, wz, out);
+        else gather_group<T, 1>(u + c0 * vol, vol, ta.n, s, qt.c, wx, wy, wz, out);
       },
       qt.c == 0);
 }
@@ -333,15 +337,15 @@ __global__ void k_sample(const T *__restrict__ pos, T *__restrict__ vel, T *__re
 // new field): the sample is taken at the new position the push leaves in registers, whose 4^3
 // stencil mostly overlaps the one the corrector just gathered.  Each value is the one k_advect then
 // k_sample would produce, bit for bit.  The sample goes into the trail slot `vel` points at; when
-// that slot is also one the push reads (a full AB(k) ring), the quad reads its tracer's samples
-// there before its writer stores the new one, and no other quad touches that tracer's entries.
+// that slot is also one the push reads (a full AB(k) ring), the group reads its tracer's samples
+// there before its writer stores the new one, and no other group touches that tracer's entries.
 template <typename T>
 __global__ void k_advect_sample(T *__restrict__ pos, T *__restrict__ upos, const T *trail, T *vel,
                                 T *__restrict__ phis, const T *__restrict__ u, const int *__restrict__ order,
                                 TracerArgs ta, AdamsArgs aa) {
-  const QuadThread qt = quad_thread();
+  const GroupThread qt = group_thread();
   const int64_t t = qt.t;
-  if (t >= ta.m) return;   // the whole quad leaves together
+  if (t >= ta.m) return;   // the whole group leaves together
   const int box = blockIdx.y;
   pos += box * 3 * ta.cap;
   if (upos) upos += box * 3 * ta.cap;
@@ -356,14 +360,14 @@ __global__ void k_advect_sample(T *__restrict__ pos, T *__restrict__ upos, const
   advect_one(
       p, pos, upos, trail, ta, aa,
       [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out) {
-        gather_quad<T, 3>(u, vol, ta.n, s, qt.c, wx, wy, wz, out);
+        gather_group<T, 3>(u, vol, ta.n, s, qt.c, wx, wy, wz, out);
       },
       xn, qt.c == 0);
   sample_at(
       p, xn[0], xn[1], xn[2], vel, phis, ta,
       [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out, int c0) {
-        if (c0 == 0) gather_quad<T, 3>(u, vol, ta.n, s, qt.c, wx, wy, wz, out);
-        else gather_quad<T, 1>(u + c0 * vol, vol, ta.n, s, qt.c, wx, wy, wz, out);
+        if (c0 == 0) gather_group<T, 3>(u, vol, ta.n, s, qt.c, wx, wy, wz, out);
+        else gather_group<T, 1>(u + c0 * vol, vol, ta.n, s, qt.c, wx, wy, wz, out);
       },
       qt.c == 0);
 }
