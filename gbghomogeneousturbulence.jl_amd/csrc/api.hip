@@ -2185,6 +2185,12 @@ struct hdns_ensemble {
   std::vector<hdns_ctx *> classes;          // one batch context per grid size
   std::vector<std::pair<int, int>> where;   // box -> (class, slot)
   hipStream_t stream = nullptr;
+  // the captured step runs every batch after the first on a side stream of its own, forked from and
+  // joined back to `stream` by events, so the graph holds one branch per grid size and the small
+  // boxes' kernels overlap (default-priority streams; a batch's own kernels stay in order)
+  std::vector<hipStream_t> side;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
   int device = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -2215,6 +2221,14 @@ void release_ensemble(hdns_ensemble *e) {
     hdns_ctx_destroy(c);
   }
   e->classes.clear();
+  for (hipStream_t st : e->side)
+    if (st) (void)hipStreamDestroy(st);
+  e->side.clear();
+  for (hipEvent_t ev : e->join)
+    if (ev) (void)hipEventDestroy(ev);
+  e->join.clear();
+  if (e->fork) (void)hipEventDestroy(e->fork);
+  e->fork = nullptr;
   if (e->stream) (void)hipStreamDestroy(e->stream);
   e->stream = nullptr;
 }
@@ -2288,6 +2302,19 @@ hdns_status hdns_ensemble_create(const hdns_config *configs, int32_t nbox, hdns_
     c->c.in_ensemble = true;
     e->classes.push_back(c);
   }
+  // the side streams and events of the captured step's branches
+  he = hipEventCreateWithFlags(&e->fork, hipEventDisableTiming);
+  e->side.assign(e->classes.size(), nullptr);
+  e->join.assign(e->classes.size(), nullptr);
+  for (size_t ci = 1; ci < e->classes.size() && he == hipSuccess; ++ci) {
+    he = hipStreamCreateWithFlags(&e->side[ci], hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&e->join[ci], hipEventDisableTiming);
+  }
+  if (he != hipSuccess) {
+    release_ensemble(e);
+    delete e;
+    return efail(nullptr, HDNS_ERR_DEVICE, std::string("ensemble streams: ") + hipGetErrorString(he));
+  }
   *out = e;
   return HDNS_OK;
 }
@@ -2336,9 +2363,25 @@ hdns_status hdns_ensemble_step_graph(hdns_ensemble *ens, double dt) {
     }
     hipError_t e = hipStreamBeginCapture(ens->stream, hipStreamCaptureModeRelaxed);
     if (e != hipSuccess) return efail(ens, HDNS_ERR_DEVICE, std::string("capture: ") + hipGetErrorString(e));
-    for (hdns_ctx *c : ens->classes) launch_step(c->c, dt, nullptr, true);
+    // fork: batch i >= 1 captures on its side stream, which waits for the origin stream
+    const size_t nc = ens->classes.size();
+    if (nc > 1) e = hipEventRecord(ens->fork, ens->stream);
+    for (size_t i = 1; i < nc && e == hipSuccess; ++i) {
+      e = hipStreamWaitEvent(ens->side[i], ens->fork, 0);
+      ens->classes[i]->c.stream = ens->side[i];
+    }
+    if (e == hipSuccess)
+      for (hdns_ctx *c : ens->classes) launch_step(c->c, dt, nullptr, true);
+    // join: the origin stream waits for every branch
+    for (size_t i = 1; i < nc; ++i) {
+      if (e == hipSuccess) e = hipEventRecord(ens->join[i], ens->side[i]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(ens->stream, ens->join[i], 0);
+      ens->classes[i]->c.stream = ens->stream;
+    }
     hipGraph_t graph = nullptr;
-    e = hipStreamEndCapture(ens->stream, &graph);
+    const hipError_t ec = hipStreamEndCapture(ens->stream, &graph);
+    if (e == hipSuccess) e = ec;
+    else if (graph) (void)hipGraphDestroy(graph);
     for (size_t i = 0; i < ens->classes.size(); ++i) ens->classes[i]->c.prof = prof[i];
     if (e != hipSuccess) return efail(ens, HDNS_ERR_DEVICE, std::string("step capture failed: ") + hipGetErrorString(e));
     hipGraphExec_t exec = nullptr;
@@ -2360,11 +2403,37 @@ hdns_status hdns_ensemble_step_graph(hdns_ensemble *ens, double dt) {
 
 hdns_status hdns_ensemble_advance_tracers(hdns_ensemble *ens, double dt) {
   if (!ens) return HDNS_ERR_INVALID_ARGUMENT;
-  for (hdns_ctx *c : ens->classes) {
-    hdns_status s = hdns_rebuild_traversal(c);
-    if (s == HDNS_OK) s = hdns_advect_deposit(c, dt);
-    if (s != HDNS_OK) return efail(ens, s, c->c.err);
+  if (hipSetDevice(ens->device) != hipSuccess) return efail(ens, HDNS_ERR_DEVICE, "hipSetDevice failed");
+  // the batches' rebuilds and pushes are independent: batch i >= 1 runs on its side stream, forked
+  // from the ensemble's stream and joined back to it, as in the captured field step
+  const size_t nc = ens->classes.size();
+  hipError_t he = nc > 1 ? hipEventRecord(ens->fork, ens->stream) : hipSuccess;
+  for (size_t i = 1; i < nc && he == hipSuccess; ++i) {
+    he = hipStreamWaitEvent(ens->side[i], ens->fork, 0);
+    if (he == hipSuccess) ens->classes[i]->c.stream = ens->side[i];
   }
+  hdns_status s = he == hipSuccess ? HDNS_OK : HDNS_ERR_DEVICE;
+  std::string msg = he == hipSuccess ? std::string() : std::string("tracer streams: ") + hipGetErrorString(he);
+  for (size_t i = 0; i < nc && s == HDNS_OK; ++i) {
+    hdns_ctx *c = ens->classes[i];
+    s = hdns_rebuild_traversal(c);
+    if (s == HDNS_OK) s = hdns_advect_deposit(c, dt);
+    if (s != HDNS_OK) msg = c->c.err;
+  }
+  for (size_t i = 1; i < nc; ++i) {
+    if (ens->classes[i]->c.stream == ens->side[i]) {
+      if (hipEventRecord(ens->join[i], ens->side[i]) != hipSuccess ||
+          hipStreamWaitEvent(ens->stream, ens->join[i], 0) != hipSuccess) {
+        (void)hipStreamSynchronize(ens->side[i]);   // the join must hold even if its event failed
+        if (s == HDNS_OK) {
+          s = HDNS_ERR_DEVICE;
+          msg = "tracer streams: the join failed";
+        }
+      }
+      ens->classes[i]->c.stream = ens->stream;
+    }
+  }
+  if (s != HDNS_OK) return efail(ens, s, msg);
   return HDNS_OK;
 }
 
