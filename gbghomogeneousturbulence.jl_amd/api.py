@@ -925,7 +925,7 @@ class Simulation:
 
     def schedule(self, set_to=None):
         """Read (and optionally set) the launch schedule: one variant index per kernel family
-        (y passes 0-1, z forward 0-2, z inverse 0-2, x pass 0-1, y radix plan 0-1)."""
+        (y passes 0-1, z forward 0-2, z inverse 0-3, x pass 0-1, y radix plan 0-1)."""
         get = np.zeros(len(SCHEDULE_FAMILIES), dtype=np.int32)
         P = ctypes.POINTER(c_int32)
         s = None if set_to is None else np.ascontiguousarray(set_to, dtype=np.int32)

@@ -93,7 +93,7 @@ struct Ctx {
   // the fastest measured at 512^3 on MI355X
   static constexpr int NFAM = 5;   // y threads, z forward, z inverse, x pass, y radix plan
   static constexpr int MAXVAR = 4;   // the widest family's variants (the autotune report's row)
-  static constexpr int NVAR[NFAM] = {2, 3, 3, 2, 2};
+  static constexpr int NVAR[NFAM] = {2, 3, 4, 2, 2};
   int sched[NFAM] = {1, 0, 0, 0, 0};
   BandSpec band{};
   BandSpec sband{};   // the scalar forcing's band

@@ -9,6 +9,7 @@
 #include "adjoint.hpp"
 #include "context.hpp"
 #include "observe.hpp"
+#include "pairs.hpp"
 #include "sort.hpp"
 #include "tracers.hpp"
 #include "xpass_wave.hpp"
@@ -18,8 +19,8 @@ namespace hdns {
 template <typename T>
 constexpr int tile_width_of() { return sizeof(T) == 4 ? 16 : 8; }
 
-// Launch schedules per stage kernel family (two, or three for the z passes), chosen per context by
-// hdns_autotune:
+// Launch schedules per stage kernel family (two; three for the z forward, four for the z inverse),
+// chosen per context by hdns_autotune:
 //   y passes: 256 or 512 threads per block (512 or 1024 from N = 512 on);
 //   z forward: (0) the scalar half and the velocity half as two kernels on 64*TX threads, one
 //              last-pass butterfly per thread at N = 512, the first two transformed velocity
@@ -27,7 +28,7 @@ constexpr int tile_width_of() { return sizeof(T) == 4 ? 16 : 8; }
 //              or (2) schedule 0 with the scalar half's flux tiles streamed through LDS;
 //   z inverse: a TX-wide tile on 64*TX threads (one first-pass butterfly per thread at N = 512)
 //              or on 32*TX threads (two), or (2) schedule 0's tile storing through LDS 16 bytes
-//              per lane;
+//              per lane, or (3) FP32 on column pairs (pairs.hpp);
 //   x pass: 256 or 512 threads per block, or for the register-resident pass (FP32 at N = 256,
 //           512 and 1024, FP64 at N = 256 and 512) the next component staged in LDS by
 //           direct-to-LDS loads, or each component loaded into registers just before its
@@ -51,6 +52,8 @@ struct PipelineImpl final : Pipeline {
   static constexpr int TX = tile_width_of<T>();
   static constexpr int M = band_m(N, DA), K = band_k(N, DA);
   static constexpr bool TWO_THIRDS = DA == DEALIAS_TWO_THIRDS;
+  // the FP32 z passes on column pairs (pairs.hpp): 2/3 rule, where a tile row holds whole pairs
+  static constexpr bool PAIRS = std::is_same<T, float>::value && TWO_THIRDS && N >= 16;
   static constexpr size_t LDS_COL = ((size_t)N * TX + N) * sizeof(cx<T>);   // tile + twiddles
   // row pairs per block and LDS of the x kernels on XT threads (BATCH: the forward x pass, whose
   // wavefronts batch several row pairs at small N)
@@ -133,6 +136,10 @@ struct PipelineImpl final : Pipeline {
     if constexpr (V == 0) {   // the z inverse's third schedule: schedule 0's tile with 16-byte stores
       set_lds((const void *)k_zi<T, N, ZT, ZN, true, DA, false, true>, zi_lds_bytes<N, ZT, T>(true));
       set_lds((const void *)k_zi<T, N, ZT, ZN, false, DA, false, true>, zi_lds_bytes<N, ZT, T>(true));
+      if constexpr (PAIRS) {   // the fourth: FP32 on column pairs
+        set_lds((const void *)k_zi_pairs<N, true>, PairShape<N>::LDS);
+        set_lds((const void *)k_zi_pairs<N, false>, PairShape<N>::LDS);
+      }
     }
     if constexpr (DA == DEALIAS_PHASE_SHIFT) {
       set_lds((const void *)k_zi<T, N, ZT, ZN, true, DA, true>, ZL);
@@ -263,7 +270,11 @@ struct PipelineImpl final : Pipeline {
 
   int tiles(const Ctx &c) const { return c.geo.kxp / TX; }
   int tile_width() const override { return TX; }
-  int z_blocks(const Ctx &c) const override { return c.geo.kxp / ZI_TX[c.sched[2] == 1 ? 1 : 0] * K; }
+  int z_blocks(const Ctx &c) const override {
+    if constexpr (PAIRS)
+      if (c.sched[2] == 3) return c.geo.kxp / (2 * PairShape<N>::TXP) * K;
+    return c.geo.kxp / ZI_TX[c.sched[2] == 1 ? 1 : 0] * K;
+  }
 
   template <int NTY, int PLAN>
   void yfwd_v(Ctx &c) {
@@ -331,8 +342,25 @@ struct PipelineImpl final : Pipeline {
                          (const cx<T> *)c.uh, (const cx<T> *)c.ph, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw,
                          c.band, c.partials);
   }
+  void zi_pairs(Ctx &c) {
+    if constexpr (PAIRS) {
+      using S = PairShape<N>;
+      const dim3 grid(c.geo.kxp / (2 * S::TXP), K, c.nbox);
+      if (c.cfg.has_scalar)
+        hipLaunchKernelGGL((k_zi_pairs<N, true>), grid, dim3(S::NT), S::LDS, c.stream, (const cx<float> *)c.uh,
+                           (const cx<float> *)c.ph, (cx<float> *)c.bufa, c.geo, (const cx<double> *)c.tw, c.band,
+                           c.partials);
+      else
+        hipLaunchKernelGGL((k_zi_pairs<N, false>), grid, dim3(S::NT), S::LDS, c.stream, (const cx<float> *)c.uh,
+                           (const cx<float> *)c.ph, (cx<float> *)c.bufa, c.geo, (const cx<double> *)c.tw, c.band,
+                           c.partials);
+    } else {
+      zi_v<0>(c);   // no pair kernels for this type or strategy: schedule 0
+    }
+  }
   void zi(Ctx &c) override {
-    if (c.sched[2] == 2) zi_v<0, true>(c);
+    if (c.sched[2] == 3) zi_pairs(c);
+    else if (c.sched[2] == 2) zi_v<0, true>(c);
     else c.sched[2] ? zi_v<1>(c) : zi_v<0>(c);
   }
 

@@ -248,23 +248,25 @@ hdns_status hdns_scheme_coefficients(int32_t scheme, int32_t precision, double *
  * 4-point Gauss-Legendre (src/ParticlePropagator.jl:261-315).  Needs no device. */
 hdns_status hdns_heun_weights(double h, double *out3);
 
-/* Autotuner: time every launch schedule of each kernel family -- the y passes (256/512 threads,
- * 512/1024 from N = 512 on), the z forward pass (split scalar/velocity kernels with LDS-staged
- * products, the same with the scalar half's flux tiles streamed through LDS, or one fused
- * kernel), the z inverse pass (full or half-width tiles, or full tiles
- * written back 16 bytes per lane through LDS), the x pass
- * (256/512 threads, or with/without prefetch) and the y passes' radix plan (the default plan, or
- * one led by a radix-16 pass) -- on this context's own buffers, keep the fastest of each, and redo
- * the deposit.  The timed sequence leaves the state bitwise
- * unchanged.  No counterpart in the reference (whose field library has no launch shapes); it tunes
- * the kernels behind hdns_step.  ms[HDNS_SCHED_FAMILIES * HDNS_SCHED_VARIANTS] (optional) receives
- * the average ms of each family's variants, row by row in the family order below (-1 where a
- * family has fewer variants); chosen[HDNS_SCHED_FAMILIES] (optional) the picks. */
+/* Autotuner: time every launch schedule of each kernel family on this context's own buffers,
+ * keep the fastest of each, and redo the deposit.  The families and their schedules:
+ *   y passes: 256/512 threads (512/1024 from N = 512 on);
+ *   z forward: (0) split scalar and velocity kernels with LDS-staged products, (1) one fused
+ *              kernel, (2) schedule 0 with the scalar half's flux tiles streamed through LDS;
+ *   z inverse: (0) full or (1) half-width tiles, (2) full tiles written back 16 bytes per lane
+ *              through LDS, (3) FP32 transforms on column pairs;
+ *   x pass: 256/512 threads, or with/without prefetch;
+ *   y radix plan: the default plan, or one led by a radix-16 pass.
+ * The timed sequence leaves the state bitwise unchanged.  No counterpart in the reference (whose
+ * field library has no launch shapes); it tunes the kernels behind hdns_step.
+ * ms[HDNS_SCHED_FAMILIES * HDNS_SCHED_VARIANTS] (optional) receives the average ms of each
+ * family's schedules, row by row in the family order (-1 where a family has fewer);
+ * chosen[HDNS_SCHED_FAMILIES] (optional) the picks. */
 #define HDNS_SCHED_FAMILIES 5
 #define HDNS_SCHED_VARIANTS 4
 hdns_status hdns_autotune(hdns_ctx *ctx, int32_t reps, double *ms, int32_t *chosen);
 /* Set (set != NULL) and/or read (get != NULL) the schedule: five entries, in the order y passes
- * (0-1), z forward (0-2), z inverse (0-2), x pass (0-1), y radix plan (0-1). */
+ * (0-1), z forward (0-2), z inverse (0-3), x pass (0-1), y radix plan (0-1). */
 hdns_status hdns_schedule(hdns_ctx *ctx, const int32_t *set, int32_t *get);
 
 /* The launch of the tracer gathers (hdns_advect, hdns_deposit_tracers, hdns_advect_deposit): 1 puts

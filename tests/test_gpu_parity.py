@@ -324,6 +324,32 @@ def test_z_pass_staging_schedules_are_bitwise_schedule_0(g, n, dtype, sched):
             assert rel(a.scalar_spectral, po) < 1e-12
 
 
+@pytest.mark.parametrize("n", [16, 32, 48, 64, 96, 128, 256])
+def test_z_inverse_on_column_pairs_matches_schedule_0_and_the_oracle(g, n):
+    """The z inverse's fourth schedule (FP32: two adjacent columns per transform element, the
+    FP64 transforms' shape): one forced FP32 step with the scalar agrees with schedule 0's to FP32
+    rounding (the FP64 partial sums add the modes in another order) and with the FP64 oracle to
+    the FP32 bar; its deposit's reductions agree with schedule 0's to 1e-12."""
+    sim, u, p, oc = make_sim(g, n, np.float32, forcing=0.1)
+    ref, *_ = make_sim(g, n, np.float32, forcing=0.1)
+    with sim, ref:
+        sim.schedule((1, 0, 3, 0, 0))
+        ref.schedule((1, 0, 0, 0, 0))
+        for s in (sim, ref):
+            s._call("hdns_deposit")
+        ra, rb = sim.reductions(), ref.reductions()
+        for key in ra:
+            assert abs(ra[key] - rb[key]) <= 1e-12 * abs(rb[key]) + 1e-300, key
+        for s in (sim, ref):
+            s.advance(2e-3)
+        a, b = sim.state(), ref.state()
+        assert rel(a.velocity_spectral, b.velocity_spectral) < 1e-6
+        assert rel(a.scalar_spectral, b.scalar_spectral) < 1e-6
+        uo, po, _ = dns.step(dns.Domain(n), oc, "rk4", u, p, 2e-3)
+        assert rel(a.velocity_spectral, uo) < 3e-6
+        assert rel(a.scalar_spectral, po) < 3e-6
+
+
 @pytest.mark.parametrize("n", [16, 32, 48, 64, 96, 128])
 def test_y_radix_plan_led_by_radix_16_matches_the_oracle(g, n):
     """The y passes' second radix plan (16.2, 16.4, 16.3, 16.8, 16.2.3 ..., 2.8 at 16) in one FP64
