@@ -1,29 +1,33 @@
 """Checkpoint and restart: write_simulation / read_simulation with the reference's schema
 (ext/GBGHomogeneousTurbulenceHDF5Ext.jl:119-545, schema version 3).
 
-The reference writes an HDF5 file.  There is no HDF5 library here.  So the same tree of groups,
-attributes and datasets is kept in a ``Store`` and written as one ``.npz`` file. Each dataset is an
-array entry named by its group path. The attributes of every group are one UTF-8 JSON entry,
-``__attributes__``. Loading runs nothing from the file: the arrays are read with
-``allow_pickle=False`` and the attributes with ``json``.
+The file is HDF5, as the reference's is: the same tree of groups, each tagged by its ``type`` and
+``version`` attributes, with the parameters as attributes and the arrays as datasets, written by
+the package's own HDF5 writer (``hdf5.py``: superblock 0, symbol-table groups, contiguous
+datasets, the layout libhdf5 writes by default), so h5py, HDF5.jl and h5dump open it.  The
+reference writes its leaves (spectral fields, point sets, the interpolation scheme) through its
+field library's persistence, whose on-disk layout is not part of the reference; here a spectral
+field is one complex dataset in the device's reference layout, a point set one (M, 3) array and
+a trail sample one (3, M) array.  Loading runs nothing from the file.  A ``.npz`` file written
+by an earlier version of this module (the same tree, attributes as JSON) is still read.
 
 What a restored run continues from: the spectral state in the reference layout (an FP32-bulk run's
-values survive the FP64 round trip exactly), the clock, the forcing and scheme settings, the
-tracer positions and the earlier samples of the velocity trail with their times. The newest
-sample is measured again at the stored positions, as the reference does.  Two extensions ride
-along as attributes: the mean scalar gradient and the launch schedule. The schedule fixes the
-rounding of the device kernels, so with it the restored run continues bit for bit.
+values survive the FP64 round trip exactly), the clock, the domain with its dealiasing, the
+forcing and scheme settings, the tracer positions and the earlier samples of the velocity trail
+with their times. The newest sample is measured again at the stored positions, as the reference
+does.  Two extensions ride along as attributes: the mean scalar gradient and the launch schedule.
+The schedule fixes the rounding of the device kernels, so with it the restored run continues bit
+for bit.
 """
 from __future__ import annotations
 
-import io
 import json
 import math
 import os
 
 import numpy as np
 
-from . import api
+from . import api, hdf5
 
 SCHEMA_VERSION = 3
 _TAG = "GBGHomogeneousTurbulence."
@@ -66,7 +70,7 @@ class Group:
 
 
 class Store:
-    """A tree of groups, attributes and datasets, saved to and loaded from one .npz file."""
+    """A tree of groups, attributes and datasets, saved to and loaded from one HDF5 file."""
 
     def __init__(self):
         self.attrs: dict = {}
@@ -77,19 +81,36 @@ class Store:
         return Group(self, "")
 
     def save(self, path) -> None:
-        buf = io.BytesIO()
-        text = json.dumps({k: v for k, v in self.attrs.items() if k}, sort_keys=True)
-        arrays = dict(self.data)
-        arrays["__attributes__"] = np.frombuffer(text.encode("utf-8"), dtype=np.uint8)
-        np.savez(buf, **arrays)
-        tmp = f"{os.fspath(path)}.tmp"
-        with open(tmp, "wb") as f:
-            f.write(buf.getvalue())
-        os.replace(tmp, path)
+        """Write the tree as an HDF5 file (through a temporary file, then renamed)."""
+        groups = {"": hdf5.Group()}
+
+        def group(where: str) -> hdf5.Group:
+            if where not in groups:
+                parent, _, name = where.rpartition("/")
+                groups[where] = hdf5.Group()
+                group(parent).members[name] = groups[where]
+            return groups[where]
+
+        for where, attrs in self.attrs.items():
+            group(where).attrs.update(attrs)
+        for key, value in self.data.items():
+            parent, _, name = key.rpartition("/")
+            group(parent).members[name] = hdf5.Dataset(np.asarray(value))
+        hdf5.write_file(path, groups[""])
 
     @classmethod
     def load(cls, path) -> "Store":
         s = cls()
+        if hdf5.is_hdf5(path):
+            def walk(g: hdf5.Group, where: str) -> None:
+                s.attrs[where] = {k: (v.tolist() if isinstance(v, np.ndarray) else v) for k, v in g.attrs.items()}
+                for name, m in g.members.items():
+                    if isinstance(m, hdf5.Group):
+                        walk(m, f"{where}/{name}" if where else name)
+                    else:
+                        s.data[f"{where}/{name}"] = m.data
+            walk(hdf5.read_file(path), "")
+            return s
         with np.load(path, allow_pickle=False) as z:
             for key in z.files:
                 if key == "__attributes__":
@@ -126,23 +147,42 @@ def _shell(h):
 # --- the parts ---------------------------------------------------------------------------------
 
 def write_domain(parent: Group, name: str, domain) -> Group:
-    """ext/GBGHomogeneousTurbulenceHDF5Ext.jl:119-137."""
+    """ext/GBGHomogeneousTurbulenceHDF5Ext.jl:119-137: the grid, the box, the element type and the
+    dealiasing strategy with its parameters (HouLi's order, its strength when set, and its
+    materialized flag only when true)."""
     g = _tagged(parent, name, "FixedDomain")
+    d = domain.dealiasing
     g.attrs.update(grid_points_per_axis=domain.grid_points_per_axis, side_length=domain.side_length,
-                   eltype="Float32" if domain.element_type is np.float32 else "Float64", dealiasing="TwoThirds")
+                   eltype="Float32" if domain.element_type is np.float32 else "Float64",
+                   dealiasing=type(d).__name__)
+    if isinstance(d, api.HouLi):
+        g.attrs["dealiasing_order"] = d.order
+        if d.strength is not None:
+            g.attrs["dealiasing_strength"] = d.strength
+        if d.materialized:
+            g.attrs["dealiasing_materialized"] = True
     return g
 
 
 def read_domain(parent: Group, name: str):
+    """ext/GBGHomogeneousTurbulenceHDF5Ext.jl:139-154."""
     g = parent.group(name, "read_domain")
     a = _require_tag(g, "read_domain", "FixedDomain")
-    if a["dealiasing"] != "TwoThirds":
-        raise ValueError(f"read_domain: dealiasing {a['dealiasing']!r} is not built here (TwoThirds is)")
+    tag = a["dealiasing"]
+    if tag == "TwoThirds":
+        dealiasing = api.TwoThirds()
+    elif tag == "PhaseShift":
+        dealiasing = api.PhaseShift()
+    elif tag == "HouLi":
+        dealiasing = api.HouLi(order=int(a["dealiasing_order"]), strength=a.get("dealiasing_strength"),
+                               materialized=bool(a.get("dealiasing_materialized", False)))
+    else:
+        raise ValueError(f"read_domain: unrecognized dealiasing tag {tag!r} in group {name!r}")
     et = {"Float32": np.float32, "Float64": np.float64}.get(a["eltype"])
     if et is None:
         raise ValueError(f"read_domain: unrecognized element type {a['eltype']!r}")
     return api.FixedDomain(grid_points_per_axis=int(a["grid_points_per_axis"]), side_length=float(a["side_length"]),
-                           dealiasing=api.TwoThirds(), element_type=et)
+                           dealiasing=dealiasing, element_type=et)
 
 
 def write_forcing(parent: Group, name: str, f) -> Group:

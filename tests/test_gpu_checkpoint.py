@@ -1,7 +1,8 @@
 """Checkpoint and restart on the device: a run written after 3 steps and read back continues bit
 for bit like the run that never stopped (the reference's test/hdf5_checkpoint.jl:202-230 asks the
 same of an IFSSPRK3 run), through the state, the clock, the forcing, the tracer positions and the
-velocity trail with its times, and the unwrapped positions (test/hdf5_checkpoint.jl:114-135)."""
+velocity trail with its times, and the unwrapped positions (test/hdf5_checkpoint.jl:114-135).
+The file is HDF5; a HouLi or PhaseShift run comes back with its dealiasing."""
 import math
 
 import numpy as np
@@ -11,9 +12,10 @@ pytestmark = pytest.mark.gpu
 L = 2 * math.pi
 
 
-def _run(g, dtype, scheme, forcing, particles, unwrapped=True):
+def _run(g, dtype, scheme, forcing, particles, unwrapped=True, dealiasing=None):
     n = 32
-    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=g.TwoThirds(), element_type=dtype)
+    dom = g.FixedDomain(grid_points_per_axis=n, side_length=L, dealiasing=dealiasing or g.TwoThirds(),
+                        element_type=dtype)
     vf = {"constant_power": lambda: g.ConstantPower(dom, injection_rate=0.1),
           "stochastic_ou": lambda: g.StochasticOU(dom, forcing_variance=0.3, correlation_time=0.05, seed=17),
           "controller": lambda: g.VarianceController(dom, target=0.6, turnover_time=0.7),
@@ -49,7 +51,7 @@ def test_a_restored_run_continues_bit_for_bit(g, tmp_path, dtype, scheme, forcin
         for dt in dts:
             a.advance(dt)
         want = _snapshot(a)
-    path = tmp_path / "run.npz"
+    path = tmp_path / "run.h5"
     with _run(g, dtype, sch, forcing, ps()) as b:
         for dt in dts[:3]:
             b.advance(dt)
@@ -75,8 +77,8 @@ def test_the_checkpoint_keeps_the_settings(g, tmp_path):
     with _run(g, np.float64, g.IFHeun, "controller", g.AB(4)) as a:
         a.advance(1e-3)
         a.schedule(np.array([0, 1, 1, 0, 1], dtype=np.int32))
-        ck.write_simulation(tmp_path / "s.npz", a)
-    with ck.read_simulation(tmp_path / "s.npz") as b:
+        ck.write_simulation(tmp_path / "s.h5", a)
+    with ck.read_simulation(tmp_path / "s.h5") as b:
         assert b.field_integration_scheme is g.IFHeun
         assert repr(b.particle_integration_scheme) == "AB(4)"
         f = b.conditions.velocity_forcing
@@ -93,8 +95,35 @@ def test_an_untracked_set_stays_untracked(g, tmp_path):
         a.advance(1e-3)
         with pytest.raises(ValueError, match="no unwrapped positions"):
             a.unwrapped_positions()
-        ck.write_simulation(tmp_path / "p.npz", a)
-    with ck.read_simulation(tmp_path / "p.npz") as b:
+        ck.write_simulation(tmp_path / "p.h5", a)
+    with ck.read_simulation(tmp_path / "p.h5") as b:
         assert not b.tracers.unwrapped
         with pytest.raises(ValueError, match="no unwrapped positions"):
             g.unwrapped_positions(b)
+
+
+@pytest.mark.parametrize("dealiasing", ["houli", "phase_shift"])
+def test_a_restored_run_keeps_its_dealiasing(g, tmp_path, dealiasing):
+    """write_domain / read_domain (ext/GBGHomogeneousTurbulenceHDF5Ext.jl:119-154): the strategy and
+    HouLi's order and strength come back, and the run continues bit for bit."""
+    from gbg_amd import checkpoint as ck
+    from gbg_amd import hdf5
+    make = {"houli": lambda: g.HouLi(order=36, strength=63.463), "phase_shift": g.PhaseShift}[dealiasing]
+    dts = [2e-3, 1.8e-3, 2.1e-3, 1.9e-3]
+    with _run(g, np.float64, g.IFSSPRK3, "constant_power", g.AB(4), dealiasing=make()) as a:
+        for dt in dts:
+            a.advance(dt)
+        want = _snapshot(a)
+    with _run(g, np.float64, g.IFSSPRK3, "constant_power", g.AB(4), dealiasing=make()) as b:
+        for dt in dts[:2]:
+            b.advance(dt)
+        ck.write_simulation(tmp_path / "d.h5", b)
+    assert hdf5.is_hdf5(tmp_path / "d.h5")
+    with ck.read_simulation(tmp_path / "d.h5") as c:
+        assert repr(c.domain.dealiasing) == repr(make())
+        for dt in dts[2:]:
+            c.advance(dt)
+        got = _snapshot(c)
+    assert got["clock"] == want["clock"]
+    assert np.array_equal(got["u"], want["u"]) and np.array_equal(got["phi"], want["phi"])
+    assert np.array_equal(got["x"], want["x"])
