@@ -310,9 +310,11 @@ def _forced_flow(g, n: int, dtype, tracers: int = 0):
 
 def time_iterations(g, n: int, m: int, dtype, steps: int, dt: float) -> dict:
     """Whole iterations of the run loop (advance!: step + deposit, traversal rebuild, Heun push and
-    sample of M tracers, the blow-up guards' readbacks), wall clock, after 2 untimed ones."""
+    sample of M tracers, the blow-up guards' readbacks), wall clock, after the autotuner (as for
+    the headline figure) and 2 untimed iterations."""
     t0 = time.perf_counter()
     with _forced_flow(g, n, dtype, tracers=m) as sim:
+        sim.autotune(reps=2)
         setup = time.perf_counter() - t0
         for _ in range(2):
             sim.advance(dt)
@@ -338,7 +340,8 @@ def time_published(g, steps: int, dealiasing: str = "hou_li") -> dict:
     HouLi(order 36, strength 63.463), ConstantPower eps = 1 on shells 1-2, nu = 9.032e-3 (k_max eta
     = 1.5 on the resolved edge), fixed dt = 2e-3, no scalar, 10^4 AB(4) tracers with Catmull-Rom
     (docs/src/validation.md:29-39).  Whole iterations (advance: step + deposit, traversal rebuild,
-    tracer push and sample, blow-up guard readback), wall clock after 20 untimed ones.  The initial
+    tracer push and sample, blow-up guard readback), wall clock after the autotuner and 20 untimed
+    ones.  The initial
     field is the seeded random-phase one, not the reference's TurTLE hand-off state."""
     n, dt, m = 128, 2e-3, 10_000
     dl = g.HouLi(order=36, strength=63.463) if dealiasing == "hou_li" else g.PhaseShift()
@@ -351,6 +354,7 @@ def time_published(g, steps: int, dealiasing: str = "hou_li") -> dict:
     with g.Simulation(dom, conditions=cond, state=g.ResolvedFields(g.random_phase_velocity(n, seed=2026), None),
                       field_integration_scheme=g.IFSSPRK3, tracers=g.Tracers(dom, number_of_particles=m, seed=2028),
                       particle_integration_scheme=g.AB(4)) as sim:
+        sim.autotune(reps=2)
         for _ in range(20):
             sim.advance(dt)
         sim.synchronize()
@@ -442,7 +446,19 @@ def time_adjoint(g, n: int = 128, steps: int = 200, every: int = 20, directions:
             sim.synchronize()
             tf = time.perf_counter() - t0
             r = {"J": J, "gradient_s": round(tg, 4), "forward_ms_per_step": round(tf * 1e3 / steps, 4),
-                 "gradient_over_forward": round(tg / tf, 3)}
+                 "gradient_over_forward": round(tg / tf, 3),
+                 "state_slots": steps // every + (steps % every > 0) + every}
+            # binomial (revolve) checkpointing with a third of those state slots: the same bits
+            slots = max(2, r["state_slots"] // 3)
+            t0 = time.perf_counter()
+            Jr, gur, _ = sim.adjoint_gradient(dt, steps, snapshots=slots)
+            tr = time.perf_counter() - t0
+            rep = 1
+            while math.comb(slots + rep, rep) < steps:
+                rep += 1
+            r["revolve"] = {"state_slots": slots, "gradient_s": round(tr, 4), "repetitions": rep,
+                            "forward_steps": rep * steps - math.comb(slots + rep, rep - 1),
+                            "bitwise_equal": bool(Jr == J and np.array_equal(gur, gu))}
             if key == "fp64" and directions:
                 def J_of(u0):
                     sim.set_fields(g.ResolvedFields(u0, p))

@@ -100,6 +100,8 @@ _SIGS = {
     "hdns_stream": [c_void_p, POINTER(c_void_p)],
     "hdns_step_vjp": [c_void_p, c_double, _D, _D, _D, _D],
     "hdns_adjoint_gradient": [c_void_p, c_double, c_int32, c_int32, _D, _D, _D],
+    "hdns_adjoint_gradient_revolve": [c_void_p, c_double, c_int32, c_int32, _D, _D, _D],
+    "hdns_revolve_schedule": [c_int32, c_int32, POINTER(c_int32), c_int64, POINTER(c_int64)],
     "hdns_ctx_create_batch": [POINTER(HdnsConfig), c_int32, _D, POINTER(c_void_p)],
     "hdns_select_box": [c_void_p, c_int32],
     "hdns_ensemble_create": [POINTER(HdnsConfig), c_int32, POINTER(c_void_p)],

@@ -665,6 +665,21 @@ SCHEDULE_FAMILIES = ("y_passes", "z_forward", "z_inverse", "x_pass", "y_radix_pl
 SCHEDULE_VARIANTS = 4   # the autotune report's row per family (HDNS_SCHED_VARIANTS)
 
 
+
+REVOLVE_OPS = ("advance", "snapshot", "restore", "first_turn", "adjoint")
+
+
+def revolve_schedule(steps: int, snapshots: int) -> list:
+    """The binomial checkpointing schedule hdns_adjoint_gradient_revolve runs: a list of
+    (action, step, slot) with action in REVOLVE_OPS (host arithmetic, no device)."""
+    count = c_int64()
+    L.check(L.lib().hdns_revolve_schedule(int(steps), int(snapshots), None, 0, ctypes.byref(count)))
+    buf = np.zeros(3 * count.value, dtype=np.int32)
+    L.check(L.lib().hdns_revolve_schedule(int(steps), int(snapshots), buf.ctypes.data_as(ctypes.POINTER(c_int32)),
+                                          count.value, ctypes.byref(count)))
+    return [(REVOLVE_OPS[op], int(step), int(slot)) for op, step, slot in buf.reshape(-1, 3)]
+
+
 class Simulation:
     """A run: the device context (generator, registers, cache), the tracers and the clock.
 
@@ -994,16 +1009,22 @@ class Simulation:
                    L.dptr(out_p) if out_p is not None else None)
         return out_u, out_p
 
-    def adjoint_gradient(self, dt: float, steps: int, checkpoint_every: int = 20):
+    def adjoint_gradient(self, dt: float, steps: int, checkpoint_every: int = 20, snapshots: int | None = None):
         """J = k + <phi^2> after `steps` steps from the current state, and its gradient with respect
         to the current u_hat (and phi_hat), by the hand-written adjoint with checkpoints every
-        `checkpoint_every` steps.  The run's state is left unchanged.  Returns (J, grad_u, grad_phi)."""
+        `checkpoint_every` steps, or, given `snapshots`, under binomial (revolve) checkpointing with
+        that many state slots (the initial state's included); both give the same bits.  The run's state is
+        left unchanged.  Returns (J, grad_u, grad_phi)."""
         n = self.domain.grid_points_per_axis
         J = c_double()
         gu = np.zeros((3, n, n, n // 2 + 1), dtype=np.complex128)
         gp = np.zeros((n, n, n // 2 + 1), dtype=np.complex128) if self.carries_scalar else None
-        self._call("hdns_adjoint_gradient", float(dt), int(steps), int(checkpoint_every), ctypes.byref(J), L.dptr(gu),
-                   L.dptr(gp) if gp is not None else None)
+        if snapshots is None:
+            self._call("hdns_adjoint_gradient", float(dt), int(steps), int(checkpoint_every), ctypes.byref(J),
+                       L.dptr(gu), L.dptr(gp) if gp is not None else None)
+        else:
+            self._call("hdns_adjoint_gradient_revolve", float(dt), int(steps), int(snapshots), ctypes.byref(J),
+                       L.dptr(gu), L.dptr(gp) if gp is not None else None)
         return J.value, gu, gp
 
     def cfl_timestep(self) -> float:

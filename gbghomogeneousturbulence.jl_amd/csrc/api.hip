@@ -1982,7 +1982,9 @@ void state_copy_out(Ctx &c, void *dst) {
 
 // the transpose of one step at the current state: lam (in c.adj.lq, the output's adjoint) becomes
 // the input's.  The state is recorded (advanced by one step) and restored.
-hdns_status adj_step(hdns_ctx *ctx, double dt) {
+// With `objective` non-null the step is the last one: its recorded forward step reaches the final
+// state, whose J is written there and whose gradient seeds lam before the transpose runs.
+hdns_status adj_step(hdns_ctx *ctx, double dt, double *objective = nullptr) {
   Ctx &c = ctx->c;
   AdjBufs &a = c.adj;
   StageRecord rec{};
@@ -1995,6 +1997,14 @@ hdns_status adj_step(hdns_ctx *ctx, double dt) {
   hdns_status s = run_deposit(ctx, a.rphys[0], a.rphysw[0]);
   if (s != HDNS_OK) return s;
   launch_step(c, dt, &rec);
+  if (objective) {
+    double sc[8];
+    hipError_t e = hipMemcpyAsync(sc, c.scalars, sizeof sc, hipMemcpyDeviceToHost, c.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) return fail(ctx, HDNS_ERR_DEVICE, std::string("adjoint forward: ") + hipGetErrorString(e));
+    *objective = 0.5 * sc[1] + (c.cfg.has_scalar ? sc[4] : 0.0);
+    c.pipe->adj_objective(c);
+  }
   state_copy_in(c, a.rq[0]);   // back to the step's input
   s = run_deposit(ctx);
   if (s != HDNS_OK) return s;
@@ -2171,6 +2181,155 @@ hdns_status hdns_adjoint_gradient(hdns_ctx *ctx, double dt, int32_t steps, int32
   }
   if (s == HDNS_OK) {
     e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, std::string("adjoint: ") + hipGetErrorString(e));
+  }
+  cleanup();
+  return s;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Binomial checkpointing (A. Griewank, "Achieving logarithmic growth of temporal and spatial
+// complexity in reverse automatic differentiation", Optim. Methods Softw. 1, 1992; the "revolve"
+// schedule of Griewank and Walther, ACM TOMS 26, 2000).  With s state slots, one of them holding
+// state l, the steps [l, l+n) are reversed by advancing from l to a split l+m, storing the state
+// there, reversing [l+m, l+n) with s-1 slots, restoring state l and reversing [l, l+m) with s
+// slots; with one slot every step is recomputed from state l.  beta(s, r) = C(s+r, s) steps can be
+// reversed with s slots when each step runs forward at most r times.  For the smallest such r the
+// split is m = max(n - beta(s-1, r), beta(s, r-2)): the right part then runs its steps r times and
+// the left part r-1 times besides the advance to l+m, which balances the two (a step moved from
+// one part to the other does not lower the count), and the total number of forward steps is
+// Griewank's minimum r n - beta(s+1, r-1).  The adjoint of a step records its own forward step, so
+// the last step's, the first turn, also yields the final state, whose objective seeds the adjoint.
+enum { RV_ADVANCE = 0, RV_SNAPSHOT = 1, RV_RESTORE = 2, RV_FIRST_TURN = 3, RV_ADJOINT = 4 };
+struct RvAction {
+  int32_t op, step, slot;   // advance to `step`; store / restore state `step` in / from `slot`; the
+                            // adjoint of step `step` (first turn: with the final state's seed)
+};
+
+int64_t binomial_reach(int s, int r) {   // C(s + r, s), saturated at 2^62
+  int64_t b = 1;
+  for (int i = 1; i <= r; ++i) {
+    const __int128 p = (__int128)b * (s + i) / i;
+    if (p > ((__int128)1 << 62)) return (int64_t)1 << 62;
+    b = (int64_t)p;
+  }
+  return b;
+}
+
+void revolve_gen(int l, int n, int s, int slot, bool rightmost, std::vector<RvAction> &out) {
+  // on entry the context holds state l, which slot `slot` also holds; s slots from `slot` on are ours
+  if (n == 1) {
+    out.push_back({rightmost ? RV_FIRST_TURN : RV_ADJOINT, l, slot});
+    return;
+  }
+  if (s == 1) {   // no free slot: each step is recomputed from state l
+    for (int t = l + n - 1; t >= l; --t) {
+      if (t < l + n - 1) out.push_back({RV_RESTORE, l, slot});
+      if (t > l) out.push_back({RV_ADVANCE, t, -1});
+      out.push_back({rightmost && t == l + n - 1 ? RV_FIRST_TURN : RV_ADJOINT, t, slot});
+    }
+    return;
+  }
+  int r = 1;
+  while (binomial_reach(s, r) < n) ++r;
+  int64_t m = std::max<int64_t>(n - binomial_reach(s - 1, r), r >= 2 ? binomial_reach(s, r - 2) : 0);
+  m = std::min<int64_t>(std::max<int64_t>(m, 1), n - 1);
+  out.push_back({RV_ADVANCE, l + (int)m, -1});
+  out.push_back({RV_SNAPSHOT, l + (int)m, slot + 1});
+  revolve_gen(l + (int)m, n - (int)m, s - 1, slot + 1, rightmost, out);
+  out.push_back({RV_RESTORE, l, slot});
+  revolve_gen(l, (int)m, s, slot, false, out);
+}
+
+std::vector<RvAction> revolve_schedule(int steps, int snapshots) {
+  std::vector<RvAction> out;
+  out.push_back({RV_SNAPSHOT, 0, 0});
+  revolve_gen(0, steps, std::min(snapshots, steps), 0, true, out);
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+hdns_status hdns_revolve_schedule(int32_t steps, int32_t snapshots, int32_t *actions, int64_t capacity,
+                                  int64_t *count) {
+  if (steps < 1 || snapshots < 1 || !count) return HDNS_ERR_INVALID_ARGUMENT;
+  const std::vector<RvAction> sched = revolve_schedule(steps, snapshots);
+  *count = (int64_t)sched.size();
+  if (!actions) return HDNS_OK;
+  if (capacity < (int64_t)sched.size()) return HDNS_ERR_INVALID_ARGUMENT;
+  for (size_t i = 0; i < sched.size(); ++i) {
+    actions[3 * i] = sched[i].op;
+    actions[3 * i + 1] = sched[i].step;
+    actions[3 * i + 2] = sched[i].slot;
+  }
+  return HDNS_OK;
+}
+
+hdns_status hdns_adjoint_gradient_revolve(hdns_ctx *ctx, double dt, int32_t steps, int32_t snapshots,
+                                          double *objective, double *grad_u, double *grad_phi) {
+  hdns_status s = step_checks(ctx, dt);
+  if (s != HDNS_OK) return s;
+  if (steps < 1 || snapshots < 1)
+    return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "steps and snapshots must be at least 1");
+  if (!objective || !grad_u) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "objective and grad_u must not be NULL");
+  Ctx &c = ctx->c;
+  HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  s = adj_alloc(ctx);
+  if (s != HDNS_OK) return s;
+  const std::vector<RvAction> sched = revolve_schedule(steps, snapshots);
+  int nslot = 0;
+  for (const RvAction &a : sched) nslot = std::max(nslot, a.slot + 1);
+  const size_t sb = 4 * c.comp * c.elem;   // one saved state
+  std::vector<void *> slot(nslot, nullptr);
+  auto cleanup = [&]() {
+    for (void *p : slot)
+      if (p) (void)hipFree(p);
+  };
+  for (int i = 0; i < nslot && s == HDNS_OK; ++i) {
+    const hipError_t e = hipMalloc(&slot[i], sb);
+    if (e != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, "snapshot allocation failed");
+  }
+  int cur = 0;                 // the step whose state the context holds
+  bool deposited = true;       // the context's products belong to its state
+  for (size_t i = 0; i < sched.size() && s == HDNS_OK; ++i) {
+    const RvAction &a = sched[i];
+    switch (a.op) {
+      case RV_ADVANCE:
+        if (!deposited) s = run_deposit(ctx);
+        for (; cur < a.step && s == HDNS_OK; ++cur) launch_step(c, dt);
+        deposited = true;
+        break;
+      case RV_SNAPSHOT:
+        state_copy_out(c, slot[a.slot]);
+        break;
+      case RV_RESTORE:
+        state_copy_in(c, slot[a.slot]);
+        cur = a.step;
+        deposited = false;
+        break;
+      case RV_FIRST_TURN:   // the last step: its recorded forward step reaches the final state
+        s = adj_step(ctx, dt, objective);
+        deposited = true;
+        break;
+      case RV_ADJOINT:
+        s = adj_step(ctx, dt);   // the context holds state a.step, as the schedule guarantees
+        deposited = true;
+        break;
+    }
+  }
+  if (s == HDNS_OK) s = compact_out(ctx, c.adj.lq, grad_u, c.cfg.has_scalar ? grad_phi : nullptr);
+  // the context is left at the initial state
+  if (s == HDNS_OK) {
+    state_copy_in(c, slot[0]);
+    s = run_deposit(ctx);
+  }
+  if (s == HDNS_OK) {
+    const hipError_t e = hipStreamSynchronize(c.stream);
     if (e != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, std::string("adjoint: ") + hipGetErrorString(e));
   }
   cleanup();

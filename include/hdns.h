@@ -297,6 +297,17 @@ hdns_status hdns_step_vjp(hdns_ctx *ctx, double dt, const double *lam_u_out, con
                           double *lam_u_in, double *lam_phi_in);
 hdns_status hdns_adjoint_gradient(hdns_ctx *ctx, double dt, int32_t steps, int32_t checkpoint_every,
                                   double *objective, double *grad_u, double *grad_phi);
+/* The same gradient under binomial checkpointing (Griewank's revolve): `snapshots` state slots, the
+ * initial state's included, each step run forward at most r times, r the smallest with
+ * C(snapshots + r, r) >= steps, and r steps - C(snapshots + r, r - 1) forward steps besides the
+ * adjoint steps' own (the minimum).  The result equals hdns_adjoint_gradient's bit for bit.  hdns_revolve_schedule writes the
+ * schedule's actions as (op, step, slot) triples (op: 0 advance to step, 1 store state `step` in
+ * `slot`, 2 restore it, 3 first turn, 4 adjoint of step `step`) for inspection; with actions NULL it
+ * only sets *count.  It is host arithmetic and needs no device. */
+hdns_status hdns_adjoint_gradient_revolve(hdns_ctx *ctx, double dt, int32_t steps, int32_t snapshots,
+                                          double *objective, double *grad_u, double *grad_phi);
+hdns_status hdns_revolve_schedule(int32_t steps, int32_t snapshots, int32_t *actions, int64_t capacity,
+                                  int64_t *count);
 
 /* Ensembles of boxes of uneven sizes and viscosities (no counterpart in the reference).  The
  * configs may differ in grid_points and viscosity only.  Boxes of one size form a batch context;
