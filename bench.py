@@ -142,13 +142,16 @@ def time_gpu(g, n: int, dtype, steps: int, warmup: int, H, seed: int = 2026, sch
 def tracer_bytes(n: int, m: int, real_bytes: int) -> dict:
     """Compulsory HBM bytes of the tracer kernels of one Heun step, M tracers on an n^3 grid: each
     gather kernel reads its fields once (every grid point lies in some tracer's 4^3 stencil at
-    these densities) plus each tracer's own records (order 4 B, positions, trail or samples)."""
+    these densities) plus each tracer's own records (positions, trail or samples), which the
+    rebuild keeps in traversal order so the gathers visit them in slot order."""
     w = real_bytes
-    return {"advect": 3 * n ** 3 * w + m * (4 + 3 * w + 3 * w + 3 * w),    # u; order, X, f, X written
-            "sample": 4 * n ** 3 * w + m * (4 + 3 * w + 3 * w + w),        # u, phi; order, X, u and phi written
-            # one pass: u and phi once; order, X, f read; X, the u and phi samples written
-            "advect_sample": 4 * n ** 3 * w + m * (4 + 3 * w + 3 * w + 3 * w + 3 * w + w),
-            "rebuild": m * (3 * w + 4 + 4)}                                 # X read, keys and order (sort aside)
+    return {"advect": 3 * n ** 3 * w + m * (3 * w + 3 * w + 3 * w),        # u; X, f, X written
+            "sample": 4 * n ** 3 * w + m * (3 * w + 3 * w + w),            # u, phi; X, u and phi written
+            # one pass: u and phi once; X, f read; X, the u and phi samples written
+            "advect_sample": 4 * n ** 3 * w + m * (3 * w + 3 * w + 3 * w + 3 * w + w),
+            # X and the slot map read, keys and indices written (sort aside); the records moved
+            # into the new order (X, two trail slots, phi; read and written) with their slot maps
+            "rebuild": m * (3 * w + 4 + 8) + m * (2 * 10 * w + 16)}
 
 
 def time_tracers(g, n: int, m: int, dtype, steps: int, H) -> dict:

@@ -190,6 +190,7 @@ void release(Ctx &c) {
   void *ptrs[] = {c.uh, c.hu, c.ph, c.hp, c.bufa, c.bufb, c.phys, c.physw, c.tw, c.partials, c.scalars, c.rpart,
                   c.usrc, c.psrc,
                   c.maxbits, c.spec_bins, c.pos, c.upos, c.vel, c.phis, c.order, c.skeys, c.svals, c.sort_temp,
+                  c.ids, c.slot_of, c.pos2, c.upos2, c.vel2, c.phis2, c.ids2,
                   c.visc_d, c.kappa_d, c.filter, c.shifted, c.step_values};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -686,6 +687,21 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
     if (s == HDNS_OK) s = alloc(ctx, &c.vel, B * (size_t)c.trail_cap * 3 * cap * realb);
     if (s == HDNS_OK && k.has_scalar) s = alloc(ctx, &c.phis, B * cap * realb);
     if (s == HDNS_OK) s = alloc(ctx, (void **)&c.order, B * cap * sizeof(int));
+    // the records' second buffers and the slot maps (every slot i holding tracer i to begin with)
+    if (s == HDNS_OK) s = alloc(ctx, &c.pos2, B * 3 * cap * realb);
+    if (s == HDNS_OK && k.unwrapped) s = alloc(ctx, &c.upos2, B * 3 * cap * realb);
+    if (s == HDNS_OK) s = alloc(ctx, &c.vel2, B * (size_t)c.trail_cap * 3 * cap * realb);
+    if (s == HDNS_OK && k.has_scalar) s = alloc(ctx, &c.phis2, B * cap * realb);
+    if (s == HDNS_OK) s = alloc(ctx, (void **)&c.ids, B * cap * sizeof(int32_t));
+    if (s == HDNS_OK) s = alloc(ctx, (void **)&c.ids2, B * cap * sizeof(int32_t));
+    if (s == HDNS_OK) s = alloc(ctx, (void **)&c.slot_of, B * cap * sizeof(int32_t));
+    if (s == HDNS_OK) {
+      std::vector<int32_t> iota(B * cap);
+      for (size_t i = 0; i < iota.size(); ++i) iota[i] = (int32_t)(i % cap);
+      e = hipMemcpy(c.ids, iota.data(), iota.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+      if (e == hipSuccess) e = hipMemcpy(c.slot_of, iota.data(), iota.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+      if (e != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, std::string("tracer slot maps: ") + hipGetErrorString(e));
+    }
     if (s == HDNS_OK) s = alloc(ctx, (void **)&c.skeys, 2 * B * cap * sizeof(uint32_t));
     if (s == HDNS_OK) s = alloc(ctx, (void **)&c.svals, 2 * B * cap * sizeof(int32_t));
     if (s == HDNS_OK) {   // the sort's scratch for every tracer slot of every box
@@ -1421,6 +1437,13 @@ hdns_status hdns_tracers_upload(hdns_ctx *ctx, int64_t m, const double *position
   c.advected = false;
   c.pending_t = 0.0;
   if (m == 0) return HDNS_OK;
+  // tracer i in slot i
+  std::vector<int32_t> iota((size_t)m);
+  for (int64_t i = 0; i < m; ++i) iota[i] = (int32_t)i;
+  const size_t at = (size_t)c.box * c.cfg.tracer_capacity;
+  HIP_TRY(ctx, hipMemcpyAsync(c.ids + at, iota.data(), m * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+  HIP_TRY(ctx, hipMemcpyAsync(c.slot_of + at, iota.data(), m * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
   const hdns_status s = copy_real_in(ctx, c.pos, 3 * (size_t)m, positions);
   if (s != HDNS_OK || !c.upos) return s;
   // the unwrapped record starts equal to the positions (src/Particles.jl:171-175)
@@ -1442,13 +1465,65 @@ static hdns_status copy_real_out(hdns_ctx *ctx, const void *src, size_t count, d
   return HDNS_OK;
 }
 
+// The tracer each slot of the selected box holds (the records move with every traversal rebuild).
+static hdns_status slot_ids(hdns_ctx *ctx, std::vector<int32_t> &ids) {
+  Ctx &c = ctx->c;
+  ids.resize((size_t)c.m);
+  if (c.m == 0) return HDNS_OK;
+  HIP_TRY(ctx, hipMemcpyAsync(ids.data(), c.ids + (size_t)c.box * c.cfg.tracer_capacity, c.m * sizeof(int32_t),
+                              hipMemcpyDeviceToHost, c.stream));
+  HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  for (const int32_t id : ids)
+    if (id < 0 || id >= c.m) return fail(ctx, HDNS_ERR_DEVICE, "the tracer slot map is corrupt");
+  return HDNS_OK;
+}
+
+// comps [comps][m] blocks of slot-ordered reals from the device, into tracer order on the host.
+static hdns_status copy_tracers_out(hdns_ctx *ctx, const void *src, int comps, double *out) {
+  Ctx &c = ctx->c;
+  const size_t m = (size_t)c.m;
+  if (m == 0) return HDNS_OK;
+  std::vector<int32_t> ids;
+  hdns_status s = slot_ids(ctx, ids);
+  if (s != HDNS_OK) return s;
+  std::vector<double> tmp(comps * m);
+  s = copy_real_out(ctx, src, comps * m, tmp.data());
+  if (s != HDNS_OK) return s;
+  for (int d = 0; d < comps; ++d)
+    for (size_t k = 0; k < m; ++k) out[d * m + ids[k]] = tmp[d * m + k];
+  return HDNS_OK;
+}
+
+// comps [comps][m] blocks in tracer order from the host, into the slots at dst (device).
+static hdns_status copy_tracers_in(hdns_ctx *ctx, void *dst, int comps, const double *in) {
+  Ctx &c = ctx->c;
+  const size_t m = (size_t)c.m;
+  if (m == 0) return HDNS_OK;
+  std::vector<int32_t> ids;
+  hdns_status s = slot_ids(ctx, ids);
+  if (s != HDNS_OK) return s;
+  if (c.fp64) {
+    std::vector<double> tmp(comps * m);
+    for (int d = 0; d < comps; ++d)
+      for (size_t k = 0; k < m; ++k) tmp[d * m + k] = in[d * m + ids[k]];
+    HIP_TRY(ctx, hipMemcpyAsync(dst, tmp.data(), tmp.size() * 8, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  } else {
+    std::vector<float> tmp(comps * m);
+    for (int d = 0; d < comps; ++d)
+      for (size_t k = 0; k < m; ++k) tmp[d * m + k] = (float)in[d * m + ids[k]];
+    HIP_TRY(ctx, hipMemcpyAsync(dst, tmp.data(), tmp.size() * 4, hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+  }
+  return HDNS_OK;
+}
+
 hdns_status hdns_tracers_download(hdns_ctx *ctx, double *positions) {
   if (!ctx || !positions) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "positions must not be NULL");
   HIP_TRY(ctx, hipSetDevice(ctx->c.cfg.device));
   const Ctx &c = ctx->c;
   const size_t realb = c.fp64 ? 8 : 4;
-  return copy_real_out(ctx, (const char *)c.pos + (size_t)c.box * 3 * c.cfg.tracer_capacity * realb, 3 * (size_t)c.m,
-                       positions);
+  return copy_tracers_out(ctx, (const char *)c.pos + (size_t)c.box * 3 * c.cfg.tracer_capacity * realb, 3, positions);
 }
 
 static const char *NO_UNWRAPPED =
@@ -1461,8 +1536,7 @@ hdns_status hdns_tracers_unwrapped_download(hdns_ctx *ctx, double *positions) {
   if (!c.upos) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, NO_UNWRAPPED);
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
   const size_t realb = c.fp64 ? 8 : 4;
-  return copy_real_out(ctx, (const char *)c.upos + (size_t)c.box * 3 * c.cfg.tracer_capacity * realb,
-                       3 * (size_t)c.m, positions);
+  return copy_tracers_out(ctx, (const char *)c.upos + (size_t)c.box * 3 * c.cfg.tracer_capacity * realb, 3, positions);
 }
 
 hdns_status hdns_tracers_unwrapped_upload(hdns_ctx *ctx, int64_t m, const double *positions) {
@@ -1475,7 +1549,8 @@ hdns_status hdns_tracers_unwrapped_upload(hdns_ctx *ctx, int64_t m, const double
                     std::to_string(m));
   if (m > 0 && !positions) return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "positions must not be NULL");
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
-  return copy_real_in(ctx, c.upos, 3 * (size_t)m, positions);
+  const size_t realb = c.fp64 ? 8 : 4;
+  return copy_tracers_in(ctx, (char *)c.upos + (size_t)c.box * 3 * c.cfg.tracer_capacity * realb, 3, positions);
 }
 
 hdns_status hdns_rebuild_traversal(hdns_ctx *ctx) {
@@ -1618,18 +1693,11 @@ hdns_status hdns_tracer_trail_restore(hdns_ctx *ctx, int32_t n, const double *ti
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
   const size_t realb = c.fp64 ? 8 : 4, cap = (size_t)c.cfg.tracer_capacity, count = 3 * (size_t)c.m;
   // oldest in slot 0: the newest restored sample in slot n - 1, and the next deposit takes slot n
-  std::vector<float> tmp(c.fp64 ? 0 : count);
   for (int j = 0; j < n; ++j) {
     const int slot = n - 1 - j;
     char *dst = (char *)c.vel + ((size_t)c.box * c.trail_cap + slot) * 3 * cap * realb;
-    const double *src = samples + (size_t)j * count;
-    if (c.fp64) {
-      HIP_TRY(ctx, hipMemcpyAsync(dst, src, count * 8, hipMemcpyHostToDevice, c.stream));
-    } else {
-      for (size_t i = 0; i < count; ++i) tmp[i] = (float)src[i];
-      HIP_TRY(ctx, hipMemcpyAsync(dst, tmp.data(), count * 4, hipMemcpyHostToDevice, c.stream));
-    }
-    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+    const hdns_status st = copy_tracers_in(ctx, dst, 3, samples + (size_t)j * count);
+    if (st != HDNS_OK) return st;
     c.trail_t[slot] = times[j];
   }
   c.trail_filled = n;
@@ -1667,7 +1735,7 @@ hdns_status hdns_tracer_trail_sample(hdns_ctx *ctx, int32_t offset, double *velo
   const size_t realb = c.fp64 ? 8 : 4, cap = (size_t)c.cfg.tracer_capacity;
   const int slot = (c.trail_newest + offset + c.trail_cap) % c.trail_cap;
   const char *src = (const char *)c.vel + ((size_t)c.box * c.trail_cap + slot) * 3 * cap * realb;
-  return copy_real_out(ctx, src, 3 * (size_t)c.m, velocity);
+  return copy_tracers_out(ctx, src, 3, velocity);
 }
 
 hdns_status hdns_tracer_time_derivative(hdns_ctx *ctx, int32_t order, double *out) {
@@ -1699,7 +1767,7 @@ hdns_status hdns_tracer_time_derivative(hdns_ctx *ctx, int32_t order, double *ou
   void *dev = nullptr;
   HIP_TRY(ctx, hipMalloc(&dev, 3 * (size_t)c.m * realb));
   c.pipe->trail_combine(c, tc, dev);
-  const hdns_status s = copy_real_out(ctx, dev, 3 * (size_t)c.m, out);
+  const hdns_status s = copy_tracers_out(ctx, dev, 3, out);
   (void)hipFree(dev);
   return s;
 }
@@ -1710,9 +1778,9 @@ hdns_status hdns_tracer_samples(hdns_ctx *ctx, double *velocity, double *scalar)
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
   const size_t realb = c.fp64 ? 8 : 4, cap = (size_t)c.cfg.tracer_capacity;
   const char *newest = (const char *)c.vel + ((size_t)c.box * c.trail_cap + c.trail_newest) * 3 * cap * realb;
-  hdns_status s = copy_real_out(ctx, newest, 3 * (size_t)c.m, velocity);
+  hdns_status s = copy_tracers_out(ctx, newest, 3, velocity);
   if (s != HDNS_OK || !scalar || !c.cfg.has_scalar) return s;
-  return copy_real_out(ctx, (const char *)c.phis + c.box * cap * realb, (size_t)c.m, scalar);
+  return copy_tracers_out(ctx, (const char *)c.phis + c.box * cap * realb, 1, scalar);
 }
 
 hdns_status hdns_tracer_cells(hdns_ctx *ctx, int64_t *keys) {
@@ -1723,10 +1791,15 @@ hdns_status hdns_tracer_cells(hdns_ctx *ctx, int64_t *keys) {
   int64_t *dev = nullptr;
   HIP_TRY(ctx, hipMalloc((void **)&dev, c.m * sizeof(int64_t)));
   c.pipe->cell_keys(c, dev);
-  hipError_t e = hipMemcpyAsync(keys, dev, c.m * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream);
+  std::vector<int64_t> by_slot((size_t)c.m);
+  hipError_t e = hipMemcpyAsync(by_slot.data(), dev, c.m * sizeof(int64_t), hipMemcpyDeviceToHost, c.stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
   (void)hipFree(dev);
   if (e != hipSuccess) return fail(ctx, HDNS_ERR_DEVICE, std::string("cell keys: ") + hipGetErrorString(e));
+  std::vector<int32_t> ids;
+  const hdns_status s = slot_ids(ctx, ids);
+  if (s != HDNS_OK) return s;
+  for (size_t k = 0; k < by_slot.size(); ++k) keys[ids[k]] = by_slot[k];
   return HDNS_OK;
 }
 

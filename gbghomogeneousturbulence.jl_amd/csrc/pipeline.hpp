@@ -440,7 +440,7 @@ struct PipelineImpl final : Pipeline {
     if (c.m == 0) return;
     hipLaunchKernelGGL((k_advect<T>), dim3(gblocks(c), c.nbox), dim3(gthreads(c)), glds(c), c.stream, (T *)c.pos,
                        (T *)c.upos,
-                       (const T *)c.vel, (const T *)c.phys, c.order_valid ? c.order : nullptr, targs(c), aa);
+                       (const T *)c.vel, (const T *)c.phys, nullptr, targs(c), aa);
   }
   void trail_combine(Ctx &c, const TrailCombine &tc, void *out) override {   // the selected box
     if (c.m == 0) return;
@@ -453,8 +453,7 @@ struct PipelineImpl final : Pipeline {
     T *slot = (T *)c.vel + (size_t)c.trail_newest * 3 * c.cfg.tracer_capacity;
     hipLaunchKernelGGL((k_sample<T>), dim3(gblocks(c), c.nbox), dim3(gthreads(c)), glds(c), c.stream,
                        (const T *)c.pos, slot,
-                       c.cfg.has_scalar ? (T *)c.phis : nullptr, (const T *)c.phys, c.order_valid ? c.order : nullptr,
-                       targs(c));
+                       c.cfg.has_scalar ? (T *)c.phis : nullptr, (const T *)c.phys, nullptr, targs(c));
   }
   void advect_sample(Ctx &c, const AdamsArgs &aa) override {
     if (c.m == 0) return;
@@ -462,7 +461,7 @@ struct PipelineImpl final : Pipeline {
     hipLaunchKernelGGL((k_advect_sample<T>), dim3(gblocks(c), c.nbox), dim3(gthreads(c)), glds(c), c.stream,
                        (T *)c.pos, (T *)c.upos,
                        (const T *)c.vel, slot, c.cfg.has_scalar ? (T *)c.phis : nullptr, (const T *)c.phys,
-                       c.order_valid ? c.order : nullptr, targs(c), aa);
+                       nullptr, targs(c), aa);
   }
   void cell_keys(Ctx &c, int64_t *keys) override {   // the selected box
     if (c.m == 0) return;
@@ -474,8 +473,8 @@ struct PipelineImpl final : Pipeline {
     const size_t per = (size_t)c.nbox * c.cfg.tracer_capacity;
     uint32_t *ka = c.skeys, *kb = c.skeys + per;
     int32_t *va = c.svals, *vb = c.svals + per;
-    hipLaunchKernelGGL((k_tile_keys<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const T *)c.pos, ka, va,
-                       targs(c));
+    hipLaunchKernelGGL((k_tile_keys<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const T *)c.pos,
+                       (const int32_t *)c.ids, ka, va, targs(c));
     bool in_b = false;
     size_t bytes = c.sort_temp_bytes;
     const hipError_t e = tile_sort_pairs(c.sort_temp, &bytes, ka, kb, va, vb, (int64_t)c.nbox * c.m, c.sort_key_bits,
@@ -483,6 +482,15 @@ struct PipelineImpl final : Pipeline {
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_order_place, dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const int32_t *)(in_b ? vb : va),
                        c.order, c.m, (int64_t)c.cfg.tracer_capacity);
+    // every record into the new order; the gathering kernels then visit the slots in order
+    hipLaunchKernelGGL((k_slot_permute<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const int *)c.order,
+                       c.slot_of, c.ids2, (const T *)c.pos, (T *)c.pos2, (const T *)c.upos, (T *)c.upos2,
+                       (const T *)c.vel, (T *)c.vel2, c.trail_cap, (const T *)c.phis, (T *)c.phis2, targs(c));
+    std::swap(c.pos, c.pos2);
+    std::swap(c.upos, c.upos2);
+    std::swap(c.vel, c.vel2);
+    std::swap(c.phis, c.phis2);
+    std::swap(c.ids, c.ids2);
     return hipSuccess;
   }
 };
