@@ -190,7 +190,7 @@ void release(Ctx &c) {
   void *ptrs[] = {c.uh, c.hu, c.ph, c.hp, c.bufa, c.bufb, c.phys, c.physw, c.tw, c.partials, c.scalars, c.rpart,
                   c.usrc, c.psrc,
                   c.maxbits, c.spec_bins, c.pos, c.upos, c.vel, c.phis, c.order, c.skeys, c.svals, c.sort_temp,
-                  c.ids, c.slot_of, c.pos2, c.upos2, c.vel2, c.phis2, c.ids2,
+                  c.ids, c.pos2, c.upos2, c.vel2, c.phis2, c.ids2,
                   c.visc_d, c.kappa_d, c.filter, c.shifted, c.step_values};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -694,13 +694,11 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
     if (s == HDNS_OK && k.has_scalar) s = alloc(ctx, &c.phis2, B * cap * realb);
     if (s == HDNS_OK) s = alloc(ctx, (void **)&c.ids, B * cap * sizeof(int32_t));
     if (s == HDNS_OK) s = alloc(ctx, (void **)&c.ids2, B * cap * sizeof(int32_t));
-    if (s == HDNS_OK) s = alloc(ctx, (void **)&c.slot_of, B * cap * sizeof(int32_t));
     if (s == HDNS_OK) {
       std::vector<int32_t> iota(B * cap);
       for (size_t i = 0; i < iota.size(); ++i) iota[i] = (int32_t)(i % cap);
       e = hipMemcpy(c.ids, iota.data(), iota.size() * sizeof(int32_t), hipMemcpyHostToDevice);
-      if (e == hipSuccess) e = hipMemcpy(c.slot_of, iota.data(), iota.size() * sizeof(int32_t), hipMemcpyHostToDevice);
-      if (e != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, std::string("tracer slot maps: ") + hipGetErrorString(e));
+      if (e != hipSuccess) s = fail(ctx, HDNS_ERR_DEVICE, std::string("tracer slot map: ") + hipGetErrorString(e));
     }
     if (s == HDNS_OK) s = alloc(ctx, (void **)&c.skeys, 2 * B * cap * sizeof(uint32_t));
     if (s == HDNS_OK) s = alloc(ctx, (void **)&c.svals, 2 * B * cap * sizeof(int32_t));
@@ -1442,7 +1440,6 @@ hdns_status hdns_tracers_upload(hdns_ctx *ctx, int64_t m, const double *position
   for (int64_t i = 0; i < m; ++i) iota[i] = (int32_t)i;
   const size_t at = (size_t)c.box * c.cfg.tracer_capacity;
   HIP_TRY(ctx, hipMemcpyAsync(c.ids + at, iota.data(), m * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
-  HIP_TRY(ctx, hipMemcpyAsync(c.slot_of + at, iota.data(), m * sizeof(int32_t), hipMemcpyHostToDevice, c.stream));
   HIP_TRY(ctx, hipStreamSynchronize(c.stream));
   const hdns_status s = copy_real_in(ctx, c.pos, 3 * (size_t)m, positions);
   if (s != HDNS_OK || !c.upos) return s;

@@ -116,12 +116,12 @@ struct Ctx {
   void *pos = nullptr, *vel = nullptr, *phis = nullptr;
   void *upos = nullptr;        // the unwrapped record [box][3][cap], when tracked; else null
   int *order = nullptr;
-  // The tracers' records live in slots: slot s of a box holds tracer ids[s] (slot_of is the
-  // inverse).  An upload puts tracer i in slot i; each traversal rebuild moves every record into
-  // the new traversal order (k_slot_permute, into the second buffers, which then swap in), so the
-  // gathering kernels visit the slots in order and read and write each record contiguously.  The
-  // host entry points map slots to tracer indices through ids.
-  int32_t *ids = nullptr, *slot_of = nullptr;
+  // The tracers' records live in slots: slot s of a box holds tracer ids[s].  An upload puts
+  // tracer i in slot i; each traversal rebuild moves every record into the new traversal order
+  // (k_slot_permute, into the second buffers, which then swap in), so the gathering kernels visit
+  // the slots in order and read and write each record contiguously.  The host entry points map
+  // slots to tracer indices through ids.
+  int32_t *ids = nullptr;
   void *pos2 = nullptr, *upos2 = nullptr, *vel2 = nullptr, *phis2 = nullptr;
   int32_t *ids2 = nullptr;
   // the traversal sort's double buffers [2][box][cap] of keys and tracer indices, and its scratch

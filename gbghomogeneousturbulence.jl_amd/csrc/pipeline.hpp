@@ -480,12 +480,11 @@ struct PipelineImpl final : Pipeline {
     const hipError_t e = tile_sort_pairs(c.sort_temp, &bytes, ka, kb, va, vb, (int64_t)c.nbox * c.m, c.sort_key_bits,
                                          c.stream, &in_b);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_order_place, dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const int32_t *)(in_b ? vb : va),
-                       c.order, c.m, (int64_t)c.cfg.tracer_capacity);
-    // every record into the new order; the gathering kernels then visit the slots in order
-    hipLaunchKernelGGL((k_slot_permute<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream, (const int *)c.order,
-                       c.slot_of, c.ids2, (const T *)c.pos, (T *)c.pos2, (const T *)c.upos, (T *)c.upos2,
-                       (const T *)c.vel, (T *)c.vel2, c.trail_cap, (const T *)c.phis, (T *)c.phis2, targs(c));
+    // every record into the new order (and the order itself); the gathers then visit the slots in order
+    hipLaunchKernelGGL((k_slot_permute<T>), dim3(tblocks(c), c.nbox), dim3(256), 0, c.stream,
+                       (const int32_t *)(in_b ? vb : va), (const int32_t *)c.ids, c.ids2, c.order, (const T *)c.pos,
+                       (T *)c.pos2, (const T *)c.upos, (T *)c.upos2, (const T *)c.vel, (T *)c.vel2, c.trail_cap,
+                       (const T *)c.phis, (T *)c.phis2, targs(c));
     std::swap(c.pos, c.pos2);
     std::swap(c.upos, c.upos2);
     std::swap(c.vel, c.vel2);

@@ -396,9 +396,9 @@ __device__ __forceinline__ int tile_of(const T *pos, int64_t m, int64_t p, int n
   return ((k >> 3) * nt + (j >> 3)) * nt + (i >> 3);
 }
 
-// Slot s holds tracer ids[s]: its (key, index) pair goes to entry box * m + index, so the sort's
-// input is in tracer-index order whatever the slots' arrangement, and a rebuilt order equals a
-// fresh one.
+// Slot s holds tracer ids[s]: its (key, slot) pair goes to entry box * m + ids[s], so the sort's
+// input is in tracer-index order whatever the slots' arrangement (a rebuilt order equals a fresh
+// one), and the sorted values name the slot each tracer of the new order comes from.
 template <typename T>
 __global__ void k_tile_keys(const T *__restrict__ pos, const int32_t *__restrict__ ids, uint32_t *__restrict__ keys,
                             int32_t *__restrict__ vals, TracerArgs ta) {
@@ -407,33 +407,31 @@ __global__ void k_tile_keys(const T *__restrict__ pos, const int32_t *__restrict
   const int box = blockIdx.y;
   pos += box * 3 * ta.cap;
   const int32_t id = ids[box * ta.cap + s];
-  if (id < 0 || id >= ta.m) return;   // the slot maps are permutations of 0 .. m-1
+  if (id < 0 || id >= ta.m) return;   // ids is a permutation of 0 .. m-1
   const size_t i = (size_t)box * ta.m + id;
   keys[i] = (uint32_t)box * (uint32_t)ta.ntiles + (uint32_t)tile_of(pos, ta.m, s, ta.n, T(ta.n / ta.side_length));
-  vals[i] = id;
+  vals[i] = (int32_t)s;
 }
 
-// The records into the traversal order: slot t of box b receives tracer order[t] from the slot it
-// held (slot_of), with everything it owns -- its position, its unwrapped position, its samples in
-// every slot of the trail's ring, its scalar sample -- and slot_of and ids follow.  The order is a
-// permutation of the box's tracers, so each index is read and written by one thread only.  The
-// outputs are the second buffers, which the caller swaps in.
+// The records into the traversal order: slot t of box b receives the tracer of slot from[t] (box
+// b's run of the sorted values), with everything it owns -- its position, its unwrapped position,
+// its samples in every slot of the trail's ring, its scalar sample -- and its index, which is also
+// entry t of the traversal order.  The outputs are the second buffers, which the caller swaps in.
 template <typename T>
-__global__ void k_slot_permute(const int *__restrict__ order, int32_t *__restrict__ slot_of,
-                               int32_t *__restrict__ ids_out, const T *__restrict__ pos, T *__restrict__ pos_out,
-                               const T *__restrict__ upos, T *__restrict__ upos_out, const T *__restrict__ vel,
-                               T *__restrict__ vel_out, int ring, const T *__restrict__ phis, T *__restrict__ phis_out,
-                               TracerArgs ta) {
+__global__ void k_slot_permute(const int32_t *__restrict__ from, const int32_t *__restrict__ ids,
+                               int32_t *__restrict__ ids_out, int *__restrict__ order, const T *__restrict__ pos,
+                               T *__restrict__ pos_out, const T *__restrict__ upos, T *__restrict__ upos_out,
+                               const T *__restrict__ vel, T *__restrict__ vel_out, int ring, const T *__restrict__ phis,
+                               T *__restrict__ phis_out, TracerArgs ta) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= ta.m) return;
   const int box = blockIdx.y;
   const int64_t cap = ta.cap, m = ta.m;
-  const int32_t id = order[box * cap + t];
-  if (id < 0 || id >= m) return;
-  const int64_t s = slot_of[box * cap + id];
+  const int64_t s = from[box * m + t];
   if (s < 0 || s >= m) return;
-  slot_of[box * cap + id] = (int32_t)t;
+  const int32_t id = ids[box * cap + s];
   ids_out[box * cap + t] = id;
+  order[box * cap + t] = id;
   const size_t pb = (size_t)box * 3 * cap;
 #pragma unroll
   for (int d = 0; d < 3; ++d) pos_out[pb + d * m + t] = pos[pb + d * m + s];
@@ -446,14 +444,6 @@ __global__ void k_slot_permute(const int *__restrict__ order, int32_t *__restric
 #pragma unroll
     for (int d = 0; d < 3; ++d) vel_out[vb + (size_t)r * 3 * cap + d * m + t] = vel[vb + (size_t)r * 3 * cap + d * m + s];
   if (phis) phis_out[box * cap + t] = phis[box * cap + s];
-}
-
-// Box b's sorted run [b m, (b + 1) m) into its order slots [b cap, b cap + m).
-static __global__ void k_order_place(const int32_t *__restrict__ sorted, int *__restrict__ order, int64_t m, int64_t cap) {
-  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (p >= m) return;
-  const int box = blockIdx.y;
-  order[box * cap + p] = sorted[box * m + p];
 }
 
 }  // namespace hdns
