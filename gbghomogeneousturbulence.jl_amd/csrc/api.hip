@@ -372,6 +372,7 @@ hdns_status run_deposit(hdns_ctx *ctx, void *phys_to = nullptr, void *physw_to =
   xa.want_max = 1;
   xa.phys = c.phys;
   xa.physw = c.physw;
+  xa.interleave = c.phys_il;
   xa.max_bits = c.maxbits;
   if (phys_to) {   // the adjoint's stage record: u, phi and w into the given arrays
     xa.store_u = 1;
@@ -379,6 +380,7 @@ hdns_status run_deposit(hdns_ctx *ctx, void *phys_to = nullptr, void *physw_to =
     xa.store_w = 1;
     xa.phys = phys_to;
     xa.physw = physw_to;
+    xa.interleave = 0;
   }
   c.pipe->xpass(c, xa);
   c.deposited = true;
@@ -621,7 +623,8 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
   c.geo.ncf = (int)ncf;
   c.geo.nci = (int)nci;
   const size_t vol = nn * n;
-  c.box_phys = (k.has_scalar ? 4 : 3) * vol;
+  c.phys_il = k.tracer_capacity > 0;
+  c.box_phys = (k.has_scalar || c.phys_il ? 4 : 3) * vol;
   c.box_physw = 3 * vol;
   const size_t B = (size_t)nbox;
 
@@ -739,6 +742,7 @@ hdns_status create_ctx(const hdns_config *config, int nbox, const double *visc, 
     zero_f64(c, c.scalars, B * 8 * sizeof(double));
     if (c.phys) zero_f64(c, c.phys, B * c.box_phys * realb);
     if (c.physw) zero_f64(c, c.physw, B * c.box_physw * realb);
+    if (c.shifted) zero_f64(c, c.shifted, B * ncf * c.comp * c.elem);
     s = launched(ctx);
     if (s == HDNS_OK) {
       e = hipStreamSynchronize(c.stream);
@@ -1006,6 +1010,7 @@ void launch_step(Ctx &c, double dt, const StageRecord *rec = nullptr, bool dyn =
   fin.want_max = 1;
   fin.phys = c.phys;
   fin.physw = c.physw;
+  fin.interleave = c.phys_il;
   fin.max_bits = c.maxbits;
   for (int i = 0; i < k.stages; ++i) {
     const bool last = i == k.stages - 1;
@@ -1040,6 +1045,7 @@ void launch_step(Ctx &c, double dt, const StageRecord *rec = nullptr, bool dyn =
         xa.store_w = 1;
         xa.phys = rec->phys[i + 1];
         xa.physw = rec->physw[i + 1];
+        xa.interleave = 0;
       }
     }
     TIMED(c, K_YFWD, c.pipe->yfwd(c));
@@ -1375,12 +1381,14 @@ hdns_status hdns_field_download(hdns_ctx *ctx, int32_t which, double *out) {
   const char *phys = (const char *)c.phys + c.box * c.box_phys * realb;
   const void *src = nullptr;
   int ncomp = 0;
+  int first = 0;   // the interleaved layout's first component
   if (which == HDNS_FIELD_VELOCITY && (c.cfg.store_physical & 1)) {
     src = phys;
     ncomp = 3;
   } else if (which == HDNS_FIELD_SCALAR && (c.cfg.store_physical & 2) && c.cfg.has_scalar) {
-    src = phys + 3 * vol * realb;
+    src = c.phys_il ? phys : phys + 3 * vol * realb;
     ncomp = 1;
+    first = 3;
   } else if (which == HDNS_FIELD_VORTICITY && (c.cfg.store_physical & 4)) {
     src = (const char *)c.physw + c.box * c.box_physw * realb;
     ncomp = 3;
@@ -1388,6 +1396,16 @@ hdns_status hdns_field_download(hdns_ctx *ctx, int32_t which, double *out) {
     return fail(ctx, HDNS_ERR_INVALID_ARGUMENT, "this context does not keep that physical field (store_physical)");
   }
   HIP_TRY(ctx, hipSetDevice(c.cfg.device));
+  if (c.phys_il && which != HDNS_FIELD_VORTICITY) {   // (u_x, u_y, u_z, phi) per point -> [ncomp][N^3]
+    std::vector<unsigned char> tmp(4 * vol * realb);
+    HIP_TRY(ctx, hipMemcpyAsync(tmp.data(), src, tmp.size(), hipMemcpyDeviceToHost, c.stream));
+    HIP_TRY(ctx, hipStreamSynchronize(c.stream));
+    for (int q = 0; q < ncomp; ++q)
+      for (size_t i = 0; i < vol; ++i)
+        out[q * vol + i] = c.fp64 ? ((const double *)tmp.data())[4 * i + first + q]
+                                  : (double)((const float *)tmp.data())[4 * i + first + q];
+    return HDNS_OK;
+  }
   if (c.fp64) {
     HIP_TRY(ctx, hipMemcpyAsync(out, src, ncomp * vol * 8, hipMemcpyDeviceToHost, c.stream));
     HIP_TRY(ctx, hipStreamSynchronize(c.stream));
@@ -1860,6 +1878,8 @@ hdns_status hdns_autotune(hdns_ctx *ctx, int32_t reps, double *ms, int32_t *chos
   idle.visc = c.visc_d;
   idle.kappa = c.kappa_d;
   idle.scalars = c.scalars;
+  idle.filter = c.filter;     // HouLi: the z forward kernels read the filter table
+  idle.shifted = c.shifted;   // PhaseShift: and the displaced products (finite: the last deposit's)
   XArgs xa{};
   xa.phys = c.phys;
   xa.physw = c.physw;
