@@ -951,7 +951,8 @@ class Simulation:
 
     def tracer_schedule(self, set_to: int | None = None) -> int:
         """Read (and optionally set) the tracer gathers' launch: 1 capped (128-thread blocks, four
-        per CU), 0 uncapped, -1 by the tracer count; returns the launch in force."""
+        per CU), 0 uncapped, -1 by the precision and the tracer count (FP64 capped from 2^16
+        tracers on, FP32 uncapped); returns the launch in force."""
         get = c_int32()
         self._call("hdns_tracer_schedule", -2 if set_to is None else int(set_to), ctypes.byref(get))
         return get.value

@@ -81,6 +81,9 @@ struct Ctx {
   std::vector<double> visc;   // per box: nu
   double *visc_d = nullptr, *kappa_d = nullptr;   // per box on the device: nu, nu/Sc
   size_t box_phys = 0, box_physw = 0;   // reals of phys / physw per box
+  // With tracers, phys holds u and phi interleaved, (u_x, u_y, u_z, phi) at every point ([N^3][4],
+  // phi 0 without a scalar), so a tracer's stencil row is one contiguous run; else [4][N^3].
+  bool phys_il = false;
   bool own_stream = true;     // false inside an ensemble, which owns the shared stream
   bool in_ensemble = false;
   int fp64 = 0;
@@ -133,7 +136,7 @@ struct Ctx {
   int ntiles = 0;
   bool order_valid = false;
   // the gathering tracer kernels' launch schedule: 1 capped (128-thread blocks, four per CU), 0
-  // uncapped, -1 by the tracer count (capped from GATHER_CAP_MIN tracers per launch on)
+  // uncapped, -1 by the precision and the tracer count (tracer_gather_capped)
   int gather_sched = -1;
   static constexpr int64_t GATHER_CAP_MIN = 1 << 16;
   // the velocity trail at the tracers (src/Particles.jl:404-526): a ring of trail_cap slots on the
@@ -163,8 +166,13 @@ struct Ctx {
   std::string err;
 };
 
+// By default FP64 gathers are capped from GATHER_CAP_MIN tracers on and FP32 gathers never: over the
+// interleaved fields a FP32 stencil row is one 64-byte run, and the uncapped launch's extra waves
+// win (2^20 tracers at 256^3: push + sample 0.257 against 0.303 ms; FP64 0.443 capped against
+// 0.476 uncapped).
 inline bool tracer_gather_capped(const Ctx &c) {
-  return c.gather_sched == 1 || (c.gather_sched < 0 && c.m * (int64_t)c.nbox >= Ctx::GATHER_CAP_MIN);
+  return c.gather_sched == 1 ||
+         (c.gather_sched < 0 && c.fp64 && c.m * (int64_t)c.nbox >= Ctx::GATHER_CAP_MIN);
 }
 
 }  // namespace hdns

@@ -148,11 +148,23 @@ __global__ __launch_bounds__(NT) void k_yinv(const cx<T> *__restrict__ q, cx<T> 
 struct XArgs {
   int store_u, store_phi, store_w;
   int want_max;
-  void *phys;             // per box [4][N][N][N] real: u_x, u_y, u_z, phi
+  void *phys;             // per box [4][N][N][N] real: u_x, u_y, u_z, phi; interleaved: [N][N][N][4]
   void *physw;            // per box [3][N][N][N] real: w
   unsigned long long *max_bits;  // per box: max |u|^2 as the bits of a non-negative double
   size_t box_phys, box_physw;    // reals per box in phys / physw
+  int interleave;         // store u and phi as (u_x, u_y, u_z, phi) per point (phi 0 without a scalar)
 };
+
+// One point of the interleaved physical fields: four reals in 16-byte stores.
+typedef float point4f __attribute__((ext_vector_type(4)));
+typedef double point2d __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_point4(float *p, float a, float b, float c, float d) {
+  *reinterpret_cast<point4f *>(p) = point4f{a, b, c, d};
+}
+__device__ __forceinline__ void store_point4(double *p, double a, double b, double c, double d) {
+  reinterpret_cast<point2d *>(p)[0] = point2d{a, b};
+  reinterpret_cast<point2d *>(p)[1] = point2d{c, d};
+}
 
 // lanes per row pair: a lane holds N/LPR values of each component.  BATCH: below 64 lanes a row
 // pair takes the power of two at or above N/e lanes, and a wavefront holds 64/LPR row pairs and
@@ -185,7 +197,8 @@ __device__ __forceinline__ void xrow_fft(cx<T> *wb, const cx<T> *twl) {
   }
 }
 
-template <typename T, int N, int NCIN, int XT, int DA = DEALIAS_TWO_THIRDS>
+// IL: the deposit's variant for runs with tracers, which stores u and phi interleaved.
+template <typename T, int N, int NCIN, int XT, int DA = DEALIAS_TWO_THIRDS, bool IL = false>
 __global__ __launch_bounds__(XT) void k_xpass(const cx<T> *__restrict__ rin,
                                                               cx<T> *__restrict__ pxo, Geo g,
                                                               const cx<double> *__restrict__ tw, XArgs xa) {
@@ -276,7 +289,13 @@ __global__ __launch_bounds__(XT) void k_xpass(const cx<T> *__restrict__ rin,
     for (int q = 0; q < E; ++q) {
       const int x = lane + LPR * q;
       if (x >= N) continue;
-      if (xa.store_u) {
+      if constexpr (IL) {
+        if (xa.store_u || xa.store_phi) {
+          const T pa = NCIN == 7 ? f[NCIN - 1][q].re : T(0), pb = NCIN == 7 ? f[NCIN - 1][q].im : T(0);
+          store_point4(ph + 4 * (rowa + x), f[0][q].re, f[1][q].re, f[2][q].re, pa);
+          store_point4(ph + 4 * (rowb + x), f[0][q].im, f[1][q].im, f[2][q].im, pb);
+        }
+      } else if (xa.store_u) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           ph[c * vol + rowa + x] = f[c][q].re;
@@ -284,7 +303,7 @@ __global__ __launch_bounds__(XT) void k_xpass(const cx<T> *__restrict__ rin,
         }
       }
       if constexpr (NCIN == 7) {
-        if (xa.store_phi) {
+        if (xa.store_phi && !IL) {
           ph[3 * vol + rowa + x] = f[NCIN - 1][q].re;
           ph[3 * vol + rowb + x] = f[NCIN - 1][q].im;
         }

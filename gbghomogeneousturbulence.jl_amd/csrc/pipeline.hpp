@@ -149,10 +149,14 @@ struct PipelineImpl final : Pipeline {
       constexpr int F = V == 0 ? XFETCH_LDS : XFETCH_JIT;
       set_lds((const void *)k_xpass_wave<T, N, 7, 256, F>, F == XFETCH_LDS ? lds_xw<256>() : 0);
       set_lds((const void *)k_xpass_wave<T, N, 6, 256, F>, F == XFETCH_LDS ? lds_xw<256>() : 0);
+      set_lds((const void *)k_xpass_wave<T, N, 7, 256, F, true>, F == XFETCH_LDS ? lds_xw<256>() : 0);
+      set_lds((const void *)k_xpass_wave<T, N, 6, 256, F, true>, F == XFETCH_LDS ? lds_xw<256>() : 0);
     } else {
       constexpr int XT = xthreads<V>();
       set_lds((const void *)k_xpass<T, N, 7, XT, DA>, lds_x<XT, true>());
       set_lds((const void *)k_xpass<T, N, 6, XT, DA>, lds_x<XT, true>());
+      set_lds((const void *)k_xpass<T, N, 7, XT, DA, true>, lds_x<XT, true>());
+      set_lds((const void *)k_xpass<T, N, 6, XT, DA, true>, lds_x<XT, true>());
     }
   }
 
@@ -383,8 +387,15 @@ struct PipelineImpl final : Pipeline {
     static_assert(!XWAVE || KXP == XWave<T, N>::KXP, "the staged rows are the kernel's padded rows");
     if (F == XFETCH_LDS && c.geo.kxp != KXP) return xpass_wave_v<XFETCH_JIT>(c, xa);   // rows of another padding
     const size_t lds = F == XFETCH_LDS ? lds_xw<XT>() : 0;
-    if (c.cfg.has_scalar)
+    const bool il = xa.interleave != 0;
+    if (c.cfg.has_scalar && il)
+      hipLaunchKernelGGL((k_xpass_wave<T, N, 7, XT, F, true>), dim3(blocks, c.nbox), dim3(XT), lds, c.stream,
+                         (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
+    else if (c.cfg.has_scalar)
       hipLaunchKernelGGL((k_xpass_wave<T, N, 7, XT, F>), dim3(blocks, c.nbox), dim3(XT), lds, c.stream,
+                         (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
+    else if (il)
+      hipLaunchKernelGGL((k_xpass_wave<T, N, 6, XT, F, true>), dim3(blocks, c.nbox), dim3(XT), lds, c.stream,
                          (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
     else
       hipLaunchKernelGGL((k_xpass_wave<T, N, 6, XT, F>), dim3(blocks, c.nbox), dim3(XT), lds, c.stream,
@@ -396,8 +407,15 @@ struct PipelineImpl final : Pipeline {
     constexpr int XT = xthreads<V>();
     const int blocks = N * N / (2 * xr<XT, true>());
     constexpr size_t lds = lds_x<XT, true>();
-    if (c.cfg.has_scalar)
+    const bool il = xa.interleave != 0;
+    if (c.cfg.has_scalar && il)
+      hipLaunchKernelGGL((k_xpass<T, N, 7, XT, DA, true>), dim3(blocks, c.nbox), dim3(XT), lds, c.stream,
+                         (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
+    else if (c.cfg.has_scalar)
       hipLaunchKernelGGL((k_xpass<T, N, 7, XT, DA>), dim3(blocks, c.nbox), dim3(XT), lds, c.stream,
+                         (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
+    else if (il)
+      hipLaunchKernelGGL((k_xpass<T, N, 6, XT, DA, true>), dim3(blocks, c.nbox), dim3(XT), lds, c.stream,
                          (const cx<T> *)c.bufb, (cx<T> *)c.bufa, c.geo, (const cx<double> *)c.tw, xa);
     else
       hipLaunchKernelGGL((k_xpass<T, N, 6, XT, DA>), dim3(blocks, c.nbox), dim3(XT), lds, c.stream,

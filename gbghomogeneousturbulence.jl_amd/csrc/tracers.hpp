@@ -1,9 +1,11 @@
 // Tracer kernels: the Catmull-Rom gather, the Adams pushes (Heun = ABM(1), ABM(k), AB(k)) with the
 // periodic wrap, the closing sample into the velocity trail, the trail's time derivative, the
-// canonical cell keys, and the keys and placement of the traversal order's stable sort.
+// canonical cell keys, the keys of the traversal order's stable sort, and the move of every
+// tracer's records into that order.
 //
 // Positions and samples are stored [3][M] (the reference's (M, 3) column-major PointSet matrix).
-// Grid point (i, j, k) sits at (i, j, k) * L/N; the field is [c][z][y][x].  The cell of x is
+// Grid point (i, j, k) sits at (i, j, k) * L/N; the fields the gathers read are interleaved,
+// [z][y][x][4] with (u_x, u_y, u_z, phi) at every point.  The cell of x is
 // floor(x * (N/L)) with N/L rounded once to the element type and the product formed in it, the
 // contract the tests hold bit-exact.  Periodic indices wrap modulo N (a mask when N is a power of
 // two); any integer, including the cast of a non-finite coordinate, lands inside the grid.
@@ -53,90 +55,73 @@ __device__ __forceinline__ Stencil stencil_of(T x, T y, T z, T inv_dx, int n, T 
   return {wrap_index((int)fx, n), wrap_index((int)fy, n), wrap_index((int)fz, n)};
 }
 
-// Four consecutive values of a row from a 4-byte-aligned (FP32) or 8-byte-aligned (FP64) address:
-// one 16-byte load (FP32) or two (FP64) instead of four.
-typedef float row4f __attribute__((ext_vector_type(4), aligned(4)));
-typedef double row2d __attribute__((ext_vector_type(2), aligned(8)));
-__device__ __forceinline__ void load_run4(const float *p, float *v) {
-  const row4f r = *reinterpret_cast<const row4f *>(p);
+// One point of the interleaved physical fields, (u_x, u_y, u_z, phi), from a 16-byte-aligned
+// address: one 16-byte load (FP32) or two (FP64).
+typedef float point4_f __attribute__((ext_vector_type(4)));
+typedef double point2_d __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void load_point4(const float *p, float *v) {
+  const point4_f r = *reinterpret_cast<const point4_f *>(p);
   v[0] = r.x;
   v[1] = r.y;
   v[2] = r.z;
   v[3] = r.w;
 }
-__device__ __forceinline__ void load_run4(const double *p, double *v) {
-  const row2d a = *reinterpret_cast<const row2d *>(p), b = *reinterpret_cast<const row2d *>(p + 2);
+__device__ __forceinline__ void load_point4(const double *p, double *v) {
+  const point2_d a = reinterpret_cast<const point2_d *>(p)[0], b = reinterpret_cast<const point2_d *>(p)[1];
   v[0] = a.x;
   v[1] = a.y;
   v[2] = b.x;
   v[3] = b.y;
 }
 
-// The gather of NF components of part of a stencil at one point: the rows b0 .. b0 + NB - 1 (y
-// index j + b - 1) of plane c (z index k + c - 1), each weighted wz[c] wy[b] and summed over its
-// four x points with explicit fused multiply-adds in a fixed order.  Components stride vol apart;
-// indices are periodic.  When the stencil's four x points do not wrap (all but 3 of N cells) a row
-// of a component is one wide load; the values, and so the sums, are the same either way.
-template <typename T, int NF, int NB>
-__device__ __forceinline__ void gather_rows(const T *__restrict__ f, size_t vol, int n, const Stencil &s, int c,
-                                            int b0, const T *wx, const T *wy, const T *wz, T *out) {
-  const int kk = wrap_index(s.k + c - 1, n);
-  const T wc = wz[c];
-#pragma unroll
-  for (int q = 0; q < NF; ++q) out[q] = T(0);
-  if (s.i >= 1 && s.i + 2 < n) {
-    const int i0 = s.i - 1;
-#pragma unroll
-    for (int bb = 0; bb < NB; ++bb) {
-      const int b = b0 + bb;
-      const size_t row = ((size_t)kk * n + wrap_index(s.j + b - 1, n)) * n + i0;
-      const T wzy = wc * wy[b];
-#pragma unroll
-      for (int q = 0; q < NF; ++q) {
-        T v[4];
-        load_run4(f + q * vol + row, v);
-        T r = T(0);
-#pragma unroll
-        for (int a = 0; a < 4; ++a) r = fma(wx[a], v[a], r);
-        out[q] = fma(wzy, r, out[q]);
-      }
-    }
-    return;
-  }
-  int ii[4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) ii[a] = wrap_index(s.i + a - 1, n);
-#pragma unroll
-  for (int bb = 0; bb < NB; ++bb) {
-    const int b = b0 + bb;
-    const size_t row = ((size_t)kk * n + wrap_index(s.j + b - 1, n)) * n;
-    const T wzy = wc * wy[b];
-#pragma unroll
-    for (int q = 0; q < NF; ++q) {
-      T r = T(0);
-#pragma unroll
-      for (int a = 0; a < 4; ++a) r = fma(wx[a], f[q * vol + row + ii[a]], r);
-      out[q] = fma(wzy, r, out[q]);
-    }
-  }
+// a * b rounded on its own: the product carries no contraction flag, so it is never fused into a
+// following addition
+template <typename T>
+__device__ __forceinline__ T rounded_product(T a, T b) {
+#pragma clang fp contract(off)
+  return a * b;
 }
 
-// The 4^3 gather of one tracer by a group of LPT lanes (4, 8 or 16): lane l gathers plane l mod 4,
-// rows (l / 4) 16/LPT .. of it, and the group adds its partial sums by butterflies (lanes l and
-// l xor 1, then l xor 2, ...), which every lane forms alike (each addition commutes exactly), so all
-// of them hold the same values.  A group covers 1/LPT of the stencils a lane-per-tracer wavefront
-// would: fewer tracers in flight for the same waves, and their field lines stay in L2 while the
-// tracers that share them are gathered.
+// The 4^3 gather of one tracer by a group of TRACER_LANES = 4 lanes from the interleaved fields f4
+// ([N][N][N][4] per box: u_x, u_y, u_z, phi at every point): lane a takes the stencil's x point a
+// (periodic) on all 16 (y, z) rows, so in every load the group reads the 64 contiguous bytes (FP32)
+// of one stencil row, and sums wz[c] wy[b] f over the rows with fused multiply-adds in a fixed
+// order; the group then adds wx[a] times its sums by butterflies (lanes l and l xor 1, then l xor
+// 2), which every lane forms alike, so all of them hold the same values.  The first NF components
+// are gathered: 3 (u) or 4 (u and phi).
 constexpr int TRACER_LANES = 4;
 template <typename T, int NF>
-__device__ __forceinline__ void gather_group(const T *__restrict__ f, size_t vol, int n, const Stencil &s, int l,
-                                             const T *wx, const T *wy, const T *wz, T *out) {
-  constexpr int NB = 16 / TRACER_LANES;   // rows per lane
-  gather_rows<T, NF, NB>(f, vol, n, s, l & 3, (l >> 2) * NB, wx, wy, wz, out);
+__device__ __forceinline__ void gather_points(const T *__restrict__ f4, int n, const Stencil &s, int l, const T *wx,
+                                              const T *wy, const T *wz, T *out) {
+  static_assert(TRACER_LANES == 4, "one x point of the stencil per lane");
+  const int a = l & 3;
+  const size_t ii = (size_t)wrap_index(s.i + a - 1, n);
+  T acc[NF];
 #pragma unroll
-  for (int q = 0; q < NF; ++q)
+  for (int q = 0; q < NF; ++q) acc[q] = T(0);
+  // FP32: all 16 points in flight (64 registers); FP64 points are twice as wide, and one plane at a
+  // time keeps the gathering kernels inside their registers
+#pragma unroll(sizeof(T) == 4 ? 4 : 1)
+  for (int c = 0; c < 4; ++c) {
+    const size_t kk = (size_t)wrap_index(s.k + c - 1, n);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const size_t jj = (size_t)wrap_index(s.j + b - 1, n);
+      T v[4];
+      load_point4(f4 + 4 * ((kk * n + jj) * n + ii), v);
+      const T w = wz[c] * wy[b];
+#pragma unroll
+      for (int q = 0; q < NF; ++q) acc[q] = fma(w, v[q], acc[q]);
+    }
+  }
+  // the product rounded on its own (never fused into the first butterfly's addition, which would
+  // round the lanes' sums differently)
+#pragma unroll
+  for (int q = 0; q < NF; ++q) {
+    out[q] = rounded_product(wx[a], acc[q]);
 #pragma unroll
     for (int m = 1; m < TRACER_LANES; m <<= 1) out[q] = out[q] + __shfl_xor(out[q], m, TRACER_LANES);
+  }
 }
 
 struct TracerArgs {
@@ -262,12 +247,11 @@ __global__ void k_advect(T *__restrict__ pos, T *__restrict__ upos, const T *__r
   u += box * ta.box_phys;
   if (order) order += box * ta.cap;
   const int64_t p = order ? order[t] : t;
-  const size_t vol = (size_t)ta.n * ta.n * ta.n;
   T xn[3];
   advect_one(
       p, pos, upos, trail, ta, aa,
       [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out) {
-        gather_group<T, 3>(u, vol, ta.n, s, qt.c, wx, wy, wz, out);
+        gather_points<T, 3>(u, ta.n, s, qt.c, wx, wy, wz, out);
       },
       xn, qt.c == 0);
 }
@@ -299,15 +283,13 @@ template <typename T, typename G>
 __device__ __forceinline__ void sample_at(int64_t p, T x, T y, T z, T *__restrict__ vel, T *__restrict__ phis,
                                           const TracerArgs &ta, G g, bool writer) {
   const T inv_dx = T(ta.n / ta.side_length);
-  T wx[4], wy[4], wz[4], v[3];
+  T wx[4], wy[4], wz[4], v[4];
   const Stencil s = stencil_of(x, y, z, inv_dx, ta.n, wx, wy, wz);
-  g(s, wx, wy, wz, v, 0);
-  T q[1] = {T(0)};
-  if (phis) g(s, wx, wy, wz, q, 3);
+  g(s, wx, wy, wz, v, phis != nullptr);   // u, and phi when sampled
   if (!writer) return;
 #pragma unroll
   for (int d = 0; d < 3; ++d) vel[d * ta.m + p] = v[d];
-  if (phis) phis[p] = q[0];
+  if (phis) phis[p] = v[3];
 }
 
 template <typename T>
@@ -323,12 +305,11 @@ __global__ void k_sample(const T *__restrict__ pos, T *__restrict__ vel, T *__re
   u += box * ta.box_phys;
   if (order) order += box * ta.cap;
   const int64_t p = order ? order[t] : t;
-  const size_t vol = (size_t)ta.n * ta.n * ta.n;
   sample_at(
       p, pos[p], pos[ta.m + p], pos[2 * ta.m + p], vel, phis, ta,
-      [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out, int c0) {
-        if (c0 == 0) gather_group<T, 3>(u, vol, ta.n, s, qt.c, wx, wy, wz, out);
-        else gather_group<T, 1>(u + c0 * vol, vol, ta.n, s, qt.c, wx, wy, wz, out);
+      [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out, bool with_phi) {
+        if (with_phi) gather_points<T, 4>(u, ta.n, s, qt.c, wx, wy, wz, out);
+        else gather_points<T, 3>(u, ta.n, s, qt.c, wx, wy, wz, out);
       },
       qt.c == 0);
 }
@@ -355,19 +336,18 @@ __global__ void k_advect_sample(T *__restrict__ pos, T *__restrict__ upos, const
   u += box * ta.box_phys;
   if (order) order += box * ta.cap;
   const int64_t p = order ? order[t] : t;
-  const size_t vol = (size_t)ta.n * ta.n * ta.n;
   T xn[3];
   advect_one(
       p, pos, upos, trail, ta, aa,
       [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out) {
-        gather_group<T, 3>(u, vol, ta.n, s, qt.c, wx, wy, wz, out);
+        gather_points<T, 3>(u, ta.n, s, qt.c, wx, wy, wz, out);
       },
       xn, qt.c == 0);
   sample_at(
       p, xn[0], xn[1], xn[2], vel, phis, ta,
-      [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out, int c0) {
-        if (c0 == 0) gather_group<T, 3>(u, vol, ta.n, s, qt.c, wx, wy, wz, out);
-        else gather_group<T, 1>(u + c0 * vol, vol, ta.n, s, qt.c, wx, wy, wz, out);
+      [&](const Stencil &s, const T *wx, const T *wy, const T *wz, T *out, bool with_phi) {
+        if (with_phi) gather_points<T, 4>(u, ta.n, s, qt.c, wx, wy, wz, out);
+        else gather_points<T, 3>(u, ta.n, s, qt.c, wx, wy, wz, out);
       },
       qt.c == 0);
 }

@@ -141,7 +141,9 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
 //               transforms; the copy holds no registers, and the first pass reads the staged rows.
 enum { XFETCH_JIT = 0, XFETCH_LDS = 1 };
 
-template <typename T, int N, int NCIN, int XT, int FETCH>
+// IL: the deposit's variant for runs with tracers, which stores u and phi interleaved
+// ((u_x, u_y, u_z, phi) per point); the stage passes and runs without tracers use IL = false.
+template <typename T, int N, int NCIN, int XT, int FETCH, bool IL = false>
 __global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ rin, cx<T> *__restrict__ pxo, Geo g,
                                                       const cx<double> *__restrict__ tw, XArgs xa) {
   using X = XWave<T, N>;
@@ -321,12 +323,37 @@ __global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ 
     }
   };
 
+  // the interleaved copy for the tracers: (u_x, u_y, u_z, phi) at every point of the row(s)
+  auto store_points = [&](const cx<T>(&u)[3][S], const cx<T>(&ph)[S]) {
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      if constexpr (HALF) {   // (x[2n], x[2n+1]) at n = j + 64 q
+        const size_t at = rowa + 2 * (j + 64 * q);
+        store_point4(phys + 4 * at, u[0][q].re, u[1][q].re, u[2][q].re, ph[q].re);
+        store_point4(phys + 4 * (at + 1), u[0][q].im, u[1][q].im, u[2][q].im, ph[q].im);
+      } else {
+        store_point4(phys + 4 * (rowa + j + 64 * q), u[0][q].re, u[1][q].re, u[2][q].re, ph[q].re);
+        store_point4(phys + 4 * (rowb + j + 64 * q), u[0][q].im, u[1][q].im, u[2][q].im, ph[q].im);
+      }
+    }
+  };
+
   constexpr int W0 = 3, PHI = 6;   // components of R: u (0..2), w (3..5), phi (6)
   cx<T> u[3][S];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     inverse(u[c], c, c < 2 ? c + 1 : (NCIN == 7 ? PHI : W0));
-    if (xa.store_u) store_phys(phys + c * vol, u[c]);
+    if constexpr (!IL) {
+      if (xa.store_u) store_phys(phys + c * vol, u[c]);
+    }
+  }
+  if constexpr (IL && NCIN == 6) {
+    if (xa.store_u) {
+      cx<T> none[S];
+#pragma unroll
+      for (int q = 0; q < S; ++q) none[q] = czero<T>();
+      store_points(u, none);
+    }
   }
 
   double mx = 0.0;
@@ -344,7 +371,11 @@ __global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ 
   if constexpr (NCIN == 7) {  // the fluxes u phi (multiply!(u, phi, u), src/Generator.jl:727)
     cx<T> ph[S];
     inverse(ph, PHI, W0);
-    if (xa.store_phi) store_phys(phys + 3 * vol, ph);
+    if constexpr (IL) {
+      if (xa.store_u || xa.store_phi) store_points(u, ph);
+    } else {
+      if (xa.store_phi) store_phys(phys + 3 * vol, ph);
+    }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       cx<T> p[S];

@@ -272,7 +272,7 @@ hdns_status hdns_schedule(hdns_ctx *ctx, const int32_t *set, int32_t *get);
 /* The launch of the tracer gathers (hdns_advect, hdns_deposit_tracers, hdns_advect_deposit): 1 puts
  * them on 128-thread blocks at most four to a CU, so fewer 4^3 stencils are in flight per XCD and
  * more of their field lines stay in L2; 0 on 256-thread blocks at full occupancy; -1 (the default)
- * chooses 1 from 2^16 tracers per launch on; -2 leaves the setting.  *get (optional) receives the
+ * chooses 1 for FP64 from 2^16 tracers per launch on, else 0; -2 leaves the setting.  *get (optional) receives the
  * launch in force.  The results do not depend on it.  No counterpart in the reference. */
 hdns_status hdns_tracer_schedule(hdns_ctx *ctx, int32_t set, int32_t *get);
 
