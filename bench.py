@@ -442,6 +442,13 @@ def time_adjoint(g, n: int = 128, steps: int = 200, every: int = 20, directions:
             t0 = time.perf_counter()
             J, gu, _ = sim.adjoint_gradient(dt, steps, every)
             tg = time.perf_counter() - t0
+            slots_uniform = steps // every + (steps % every > 0) + every
+            # binomial (revolve) checkpointing from the same state, with a third of those state
+            # slots: the same bits (the gradient leaves the state where it was)
+            slots = max(2, slots_uniform // 3)
+            t0 = time.perf_counter()
+            Jr, gur, _ = sim.adjoint_gradient(dt, steps, snapshots=slots)
+            tr = time.perf_counter() - t0
             sim.synchronize()
             t0 = time.perf_counter()
             for _ in range(steps):
@@ -449,13 +456,7 @@ def time_adjoint(g, n: int = 128, steps: int = 200, every: int = 20, directions:
             sim.synchronize()
             tf = time.perf_counter() - t0
             r = {"J": J, "gradient_s": round(tg, 4), "forward_ms_per_step": round(tf * 1e3 / steps, 4),
-                 "gradient_over_forward": round(tg / tf, 3),
-                 "state_slots": steps // every + (steps % every > 0) + every}
-            # binomial (revolve) checkpointing with a third of those state slots: the same bits
-            slots = max(2, r["state_slots"] // 3)
-            t0 = time.perf_counter()
-            Jr, gur, _ = sim.adjoint_gradient(dt, steps, snapshots=slots)
-            tr = time.perf_counter() - t0
+                 "gradient_over_forward": round(tg / tf, 3), "state_slots": slots_uniform}
             rep = 1
             while math.comb(slots + rep, rep) < steps:
                 rep += 1
