@@ -323,18 +323,16 @@ __global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ 
     }
   };
 
-  // the interleaved copy for the tracers: (u_x, u_y, u_z, phi) at every point of the row(s)
-  auto store_points = [&](const cx<T>(&u)[3][S], const cx<T>(&ph)[S]) {
-#pragma unroll
-    for (int q = 0; q < S; ++q) {
-      if constexpr (HALF) {   // (x[2n], x[2n+1]) at n = j + 64 q
-        const size_t at = rowa + 2 * (j + 64 * q);
-        store_point4(phys + 4 * at, u[0][q].re, u[1][q].re, u[2][q].re, ph[q].re);
-        store_point4(phys + 4 * (at + 1), u[0][q].im, u[1][q].im, u[2][q].im, ph[q].im);
-      } else {
-        store_point4(phys + 4 * (rowa + j + 64 * q), u[0][q].re, u[1][q].re, u[2][q].re, ph[q].re);
-        store_point4(phys + 4 * (rowb + j + 64 * q), u[0][q].im, u[1][q].im, u[2][q].im, ph[q].im);
-      }
+  // the interleaved copy for the tracers: (u_x, u_y, u_z, phi) at the points of slot q of the
+  // row(s), the values passed one by one so that they stay in registers
+  auto store_points = [&](int q, cx<T> u0, cx<T> u1, cx<T> u2, cx<T> ph) {
+    if constexpr (HALF) {   // (x[2n], x[2n+1]) at n = j + 64 q
+      const size_t at = rowa + 2 * (j + 64 * q);
+      store_point4(phys + 4 * at, u0.re, u1.re, u2.re, ph.re);
+      store_point4(phys + 4 * (at + 1), u0.im, u1.im, u2.im, ph.im);
+    } else {
+      store_point4(phys + 4 * (rowa + j + 64 * q), u0.re, u1.re, u2.re, ph.re);
+      store_point4(phys + 4 * (rowb + j + 64 * q), u0.im, u1.im, u2.im, ph.im);
     }
   };
 
@@ -349,10 +347,8 @@ __global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ 
   }
   if constexpr (IL && NCIN == 6) {
     if (xa.store_u) {
-      cx<T> none[S];
 #pragma unroll
-      for (int q = 0; q < S; ++q) none[q] = czero<T>();
-      store_points(u, none);
+      for (int q = 0; q < S; ++q) store_points(q, u[0][q], u[1][q], u[2][q], czero<T>());
     }
   }
 
@@ -372,7 +368,10 @@ __global__ __launch_bounds__(XT, 3) void k_xpass_wave(const cx<T> *__restrict__ 
     cx<T> ph[S];
     inverse(ph, PHI, W0);
     if constexpr (IL) {
-      if (xa.store_u || xa.store_phi) store_points(u, ph);
+      if (xa.store_u || xa.store_phi) {
+#pragma unroll
+        for (int q = 0; q < S; ++q) store_points(q, u[0][q], u[1][q], u[2][q], ph[q]);
+      }
     } else {
       if (xa.store_phi) store_phys(phys + 3 * vol, ph);
     }
