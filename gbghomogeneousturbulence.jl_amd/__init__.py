@@ -10,7 +10,8 @@ from .api import (AB, ABM, ConstantPower, Ensemble, EnsembleBox, EnsembleMember,
                   VarianceRescaling, ResolvedFields, Simulation, Tracers, TwoThirds, HouLi, PhaseShift,
                   StationarityCriterion, cfl_bound, integral_length, predicted_viscosity,
                   recommended_cfl, truncate_to_band, warmup, reynolds_box, reynolds_taylor, taylor_length,
-                  kolmogorov_length, kolmogorov_time, kolmogorov_velocity, unwrapped_positions, revolve_schedule)
+                  kolmogorov_length, kolmogorov_time, kolmogorov_velocity, unwrapped_positions, revolve_schedule,
+                  SCHEDULE_FAMILIES)
 from .seeding import random_phase_scalar, random_phase_velocity, uniform_positions
 from .stochastic import StochasticOU
 
@@ -21,5 +22,5 @@ __all__ = [
     "predicted_viscosity",
     "recommended_cfl", "cfl_bound", "integral_length", "StationarityCriterion", "warmup", "truncate_to_band", "random_phase_scalar", "random_phase_velocity",
     "uniform_positions", "StochasticOU", "reynolds_box", "reynolds_taylor", "taylor_length", "kolmogorov_length",
-    "kolmogorov_time", "kolmogorov_velocity", "unwrapped_positions", "revolve_schedule",
+    "kolmogorov_time", "kolmogorov_velocity", "unwrapped_positions", "revolve_schedule", "SCHEDULE_FAMILIES",
 ]

@@ -1330,6 +1330,34 @@ class Ensemble:
     def synchronize(self) -> None:
         self._ecall("hdns_ensemble_synchronize")
 
+    def _batches(self) -> dict:
+        """{grid size: the first box of that size}: each size is one batch (one device context)."""
+        heads = {}
+        for box in self.boxes:
+            heads.setdefault(box.domain.grid_points_per_axis, box)
+        return heads
+
+    def autotune(self, reps: int = 3) -> dict:
+        """Time every launch schedule of every stage kernel family on each batch of equal-size boxes
+        (every box of the batch in one launch, as the step runs them) and keep the fastest; the
+        states are left bitwise unchanged and the next graph step captures the new schedules.
+        Returns {grid size: {family: {"ms": [ms of each variant], "chosen": index}}}."""
+        if self._h is None:
+            raise RuntimeError("this Ensemble has been closed")
+        return {n: Simulation.autotune(head, reps) for n, head in sorted(self._batches().items())}
+
+    def schedules(self, set_to: dict | None = None) -> dict:
+        """Read (and optionally set, {grid size: one variant per family}) each batch's launch
+        schedule, as Simulation.schedule does for one run."""
+        if self._h is None:
+            raise RuntimeError("this Ensemble has been closed")
+        heads = self._batches()
+        for n in set_to or {}:
+            if n not in heads:
+                raise ValueError(f"the ensemble has no box of size {n}: its sizes are {sorted(heads)}")
+        return {n: Simulation.schedule(head, None if set_to is None else set_to.get(n))
+                for n, head in sorted(heads.items())}
+
     def close(self) -> None:
         if getattr(self, "_h", None) is not None:
             L.lib().hdns_ensemble_destroy(self._h)

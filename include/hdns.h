@@ -258,7 +258,9 @@ hdns_status hdns_heun_weights(double h, double *out3);
  *   x pass: 256/512 threads, or with/without prefetch;
  *   y radix plan: the default plan, or one led by a radix-16 pass.
  * The timed sequence leaves the state bitwise unchanged.  No counterpart in the reference (whose
- * field library has no launch shapes); it tunes the kernels behind hdns_step.
+ * field library has no launch shapes); it tunes the kernels behind hdns_step.  On an ensemble's
+ * member context it tunes that box's batch (every box of its size, as the ensemble step launches
+ * them); the ensemble's next graph step captures the new schedule.
  * ms[HDNS_SCHED_FAMILIES * HDNS_SCHED_VARIANTS] (optional) receives the average ms of each
  * family's schedules, row by row in the family order (-1 where a family has fewer);
  * chosen[HDNS_SCHED_FAMILIES] (optional) the picks. */

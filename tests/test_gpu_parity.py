@@ -680,3 +680,46 @@ def test_the_adjoint_refuses_ssprk3(g):
     with sim:
         with pytest.raises(Exception, match="low-storage"):
             sim.adjoint_gradient(1e-3, 2)
+
+
+def test_an_autotuned_ensemble_equals_standalone_runs_on_its_schedules(g):
+    """Each batch of equal-size boxes tunes its own launch schedules, every box of the batch in one
+    launch: the tuning leaves every state bitwise unchanged, the batches keep what they chose, and
+    each box then equals a standalone run on its batch's schedule bit for bit (tracers included)."""
+    sizes = [32, 48, 32, 64]
+    nus = [0.005, 0.012, 0.02, 0.008]
+    m, dt, steps = 64, 2e-3, 3
+    members = [_member(g, n, np.float32, nu, 500 + 7 * b, m) for b, (n, nu) in enumerate(zip(sizes, nus))]
+    with g.Ensemble(members, field_integration_scheme=g.IFLowStorageRK3, mean_gradient=1.0) as ens:
+        before = [ens.box(b).state() for b in range(len(sizes))]
+        report = ens.autotune(reps=1)
+        assert sorted(report) == [32, 48, 64]
+        chosen = ens.schedules()
+        for n in chosen:
+            assert chosen[n] == tuple(report[n][f]["chosen"] for f in g.SCHEDULE_FAMILIES), n
+        for b in range(len(sizes)):
+            after = ens.box(b).state()
+            assert np.array_equal(before[b].velocity_spectral, after.velocity_spectral), b
+            assert np.array_equal(before[b].scalar_spectral, after.scalar_spectral), b
+        for s in range(steps):
+            ens.step(dt, graph=s != 1)
+            ens.advance_tracers(dt)
+        print("schedules", chosen, flush=True)
+        for b, mb in enumerate(members):
+            with g.Simulation(mb.domain, conditions=mb.conditions, state=mb.state, tracers=mb.tracers,
+                              field_integration_scheme=g.IFLowStorageRK3, mean_gradient=1.0) as alone:
+                alone.schedule(chosen[sizes[b]])
+                alone._call("hdns_deposit")
+                for _ in range(steps):
+                    alone.advance(dt, check=False)
+                box = ens.box(b)
+                a, e = alone.state(), box.state()
+                assert np.array_equal(a.velocity_spectral, e.velocity_spectral), b
+                assert np.array_equal(a.scalar_spectral, e.scalar_spectral), b
+                assert alone.reductions() == box.reductions(), b
+                assert np.array_equal(alone.tracer_positions(), box.tracer_positions()), b
+        # a schedule set by hand holds; a size the ensemble lacks is refused
+        want = {32: (0, 1, 1, 0, 0)}
+        assert ens.schedules(want)[32] == want[32]
+        with pytest.raises(ValueError, match="no box of size 128"):
+            ens.schedules({128: (0, 0, 0, 0, 0)})
